@@ -1,0 +1,232 @@
+"""Deck writer and runner for the compiled reference solver (test infra only).
+
+Writes a reference input deck (``set/*.txt``, format read by
+Schemi/main/Schemi.cpp:84-163 and Schemi/src/thermodynamics/phaseInitialization.cpp)
+for a :class:`schemi_amd.case.Case`, runs ``oracle/_ref/schemi`` on it, and
+parses the full-precision output tables (Schemi/src/service/output.cpp:14-183).
+
+Arbitrary initial fields are injected through the reference's restart path:
+``fieldsOutput/output_000001/*.dat`` plus a two-line ``result/Time.tsv`` and
+``#Start_From_Time-Step 1`` (Schemi/main/Schemi.cpp:342-573,
+phaseInitialization.cpp:1382-1744).  Output number N+1 then holds the state
+after step N; a zone-initialised run writes output 0 (initial state) and
+output N after step N.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_BIN = os.path.join(HERE, "_ref", "schemi")
+
+_BCNAME = {0: "blank", 1: "freeBoundary", 2: "slip", 3: "fixedValueCell",
+           4: "fixedValueSurface"}
+_PATCH_LABEL = ("#Tail_boundary_condition", "#Point_boundary_condition",
+                "#Bottom_boundary_condition", "#Right_boundary_condition",
+                "#Left_boundary_condition", "#Top_boundary_condition")
+_FLUX = {0: "HLL", 2: "HLLC"}
+_LIM = {0: "zero", 1: "minmod", 2: "vanLeer", 3: "linear", 4: "vanAlbada"}
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_BIN)
+
+
+def _r(x) -> str:
+    return repr(float(x))
+
+
+def _bc_lines(case, bcs, vec=False):
+    out = []
+    for p in range(6):
+        out.append(_PATCH_LABEL[p])
+        if not case.patch_used(p):
+            out.append("blank")
+            continue
+        bc = bcs[p]
+        name = _BCNAME[int(bc.type)]
+        if int(bc.type) == 3:
+            vals = bc.value if vec else bc.value[:1]
+            name = name + " " + " ".join(_r(x) for x in vals)
+        out.append(name)
+    return out
+
+
+def _free_lines(case):
+    return [x for p in range(6) for x in (
+        _PATCH_LABEL[p], "freeBoundary" if case.patch_used(p) else "blank")]
+
+
+def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
+               const_dt=-1.0, min_time=None, courant=None):
+    """Writes ``root/set`` for ``case``.
+
+    zones:   ((c_left[Ns], v_left[3], p_left), (c_right, v_right, p_right), x0)
+             two half-spaces split by the plane x = x0 (zonesDescription.txt).
+    restart: (c[Ns, n], v[3, n], p[n]) initial primitive fields in reference
+             cell order, injected through the restart reader.
+    """
+    set_dir = os.path.join(root, "set")
+    os.makedirs(set_dir, exist_ok=True)
+    Ns = case.Ns
+    dims = {1: "1D", 2: "2D", 3: "3D"}[case.dim]
+    mt = case.min_time if min_time is None else min_time
+    cfl = case.courant if courant is None else courant
+    main = [
+        "#Radius-vector_of_system's_parallelepiped",
+        " ".join(_r(x) for x in case.L),
+        "#Number_of_cells_in_direction",
+        " ".join(str(int(x)) for x in case.N),
+        "#Time_of_calculation", _r(t_end),
+        "#Courant_number", _r(cfl),
+        "#Frequency_of_output", str(int(out_freq)),
+        "#Minimal_timestep", _r(mt),
+        "#Number_of_components", str(Ns),
+        "#Defines_universal_gas_constant", "1" if case.R == 1.0 else "SI",
+        "#Equation_of_state", "ideal",
+        "#TVD_limiter_type", _LIM[int(case.limiter)],
+        "#Number_of_iterations_of_SLE_matrix_solver", "2000",
+        "#Diffusion_flag", "off",
+        "#Turbulence_flag", "off",
+        "#Type_of_SLE_solver_flag", "GaussSeidel",
+        "#Type_of_source_flag", "BHR",
+        "#Type_of_flow_solver", _FLUX[int(case.flux)],
+        "#Dimensions_of_the_task", dims,
+        "#Constant_timestep", _r(const_dt),
+        "#Source_timestep_correction", "Courant=0.25,0.5",
+        "#Reconstruction", "star",
+        "#Approximation_order", "SecondOrderRK",
+        "#Start_From_Time-Step", "1" if restart is not None else "no",
+        "#Calculation_of_mixing_zone_width", "off",
+        "#Nonlinearity_iterations", "off",
+    ]
+    if any(abs(o) != 0.0 for o in case.origin):
+        raise ValueError("the reference deck has no origin entry; use origin 0")
+    files = {"main.txt": main}
+    files["g.txt"] = ["#Free_fall_acceleration_flag",
+                      "on" if case.gravity else "off",
+                      "#Free_fall_acceleration_vector_components",
+                      " ".join(_r(x) for x in case.g),
+                      "#Acceleration_growth", "0 0 0"]
+    if zones is None:
+        zl = ([1.0] * Ns, (0.0, 0.0, 0.0), 1.0)
+        zones = (zl, zl, 0.5 * case.L[0])
+    (cL, vL, pL), (cR, vR, pR), x0 = zones
+    plane = (f"plane\n1 0 0; {_r(x0)} {_r(0.5 * case.L[1])} {_r(0.5 * case.L[2])}\n"
+             f"plane\n-1 0 0; {_r(x0)} {_r(0.5 * case.L[1])} {_r(0.5 * case.L[2])}")
+    files["zonesDescription.txt"] = ["#Number_of_zones", "2", "#Outer_zone: no",
+                                     "#Zones_description", plane]
+    for k in range(Ns):
+        files[f"sub_{k + 1}.txt"] = [
+            "#Molar_mass_(kg/mol)", _r(case.M[k]),
+            "#Molar_isohoric_heat_capacity_(J/T/mol)", _r(case.Cv[k]),
+            "#Critical_temperature_(K)", "1", "#Critical_pressure_(Pa)", "1",
+            "#Kinetic_diameter_(A)", "1", "#Lennard_Jones_eps/kB_(K)", "1",
+            "#Lennard_Jones_sigma_(A)", "1",
+        ] + _bc_lines(case, case.bc_c[k]) + [
+            "#Values_in_zones", _r(cL[k]), _r(cR[k])]
+    files["velocity.txt"] = _bc_lines(case, case.bc_v, vec=True) + [
+        "#Values_in_zones", " ".join(_r(x) for x in vL),
+        " ".join(_r(x) for x in vR)]
+    files["pressure.txt"] = _bc_lines(case, case.bc_p) + [
+        "#Values_in_zones", _r(pL), _r(pR)]
+    for name, val in (("k.txt", "0.0"), ("epsilon.txt", "0.0"), ("b.txt", "0")):
+        files[name] = _free_lines(case) + ["#Values_in_zones", val, val]
+    files["a.txt"] = _free_lines(case) + ["#Values_in_zones", "0 0 0", "0 0 0"]
+    files["chemicalKinetics.txt"] = ["#Type_of_chemical_reaction", "no"]
+    files["MPIRanks.txt"] = ["#MPI_nodes_in_directions", "1 1 1"]
+    files["GoncharovModel.txt"] = ["off", "#Number_of_particles", "1",
+                                   "#Initialisation_check_type", "viaRe",
+                                   "#Initial_data", "0.51 0.2 0.2 1 2 2 7E-4 1E-2 1E15"]
+    files["turbPeak.txt"] = ["#Type_of_peak", "no", "#Maximum_k", "1",
+                             "#Maximum_epsilon", "1000", "#Center_coordinate", "1.0",
+                             "#Left_width", "0.005", "#Right_width", "0.005"]
+    files["transportCoefficients.txt"] = [
+        "#Transport_coefficient_model", "constant", "#Viscosity_coefficient", "2E-5",
+        "#Diffusion_coefficient", "2E-5", "#Heat_conduction_coefficient", "2E-2",
+        "#Flag_for_enthalpy_flow_solution", "explicit",
+        "#Flag_for_molar_mass_diffusion_correction", "on"]
+    turb = ["Cmu 0.09", "Turbulent_Schmidt 0.7", "Turbulent_Prandtl_T 0.7",
+            "Turbulent_Prandtl_e 1E6", "sigma_k 0.7", "sigma_epsilon 0.6",
+            "sigma_a 1.4", "sigma_b 0.7", "CMS_R 1E-1", "CMS_D 1E0",
+            "Minimal_k 1E-15", "Minimal_epsilon 1E-15", "Minimal_b 1E-15",
+            "C0 0.92", "C1 1.44", "C2 1.92", "C3 2.0", "C4 1.1", "Ca 2.5",
+            "Cb1 2.0", "Cb2 5.0", "alpha_2 0.15", "alpha_3 0.2", "alpha_4 0.1",
+            "CMS_M 1.0"]
+    files["turbulentParameters.txt"] = [y for x in turb for y in
+                                        ("#" + x.split()[0], x.split()[1])]
+    for name, lines in files.items():
+        with open(os.path.join(set_dir, name), "w") as f:
+            f.write("\n".join(lines) + "\n")
+
+    if restart is not None:
+        c, v, p = restart
+        n = case.ncells
+        d = os.path.join(root, "fieldsOutput", "output_000001")
+        os.makedirs(d, exist_ok=True)
+        os.makedirs(os.path.join(root, "result"), exist_ok=True)
+        for k in range(Ns):
+            np.savetxt(os.path.join(d, f"sub_{k + 1}.dat"), np.asarray(c[k]),
+                       fmt="%.20g")
+        np.savetxt(os.path.join(d, "pressure.dat"), np.asarray(p), fmt="%.20g")
+        np.savetxt(os.path.join(d, "velocity.dat"), np.asarray(v).T, fmt="%.20g",
+                   delimiter="\t")
+        z = np.zeros(n)
+        for name in ("k.dat", "epsilon.dat", "b.dat"):
+            np.savetxt(os.path.join(d, name), z, fmt="%.20g")
+        np.savetxt(os.path.join(d, "a.dat"), np.zeros((n, 3)), fmt="%.20g",
+                   delimiter="\t")
+        with open(os.path.join(root, "result", "Time.tsv"), "w") as f:
+            f.write("0\t0\n1\t0\n")
+
+
+def run(root, timeout=600):
+    """Runs the reference binary in ``root``; returns the completed process."""
+    if not ref_available():
+        raise RuntimeError("oracle/_ref/schemi is not built (make -C oracle ref)")
+    return subprocess.run([REF_BIN], cwd=root, stdin=subprocess.DEVNULL,
+                          stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                          timeout=timeout, text=True)
+
+
+def read_output(root, number, Ns):
+    """Parses result/output_NNNNNN.tsv into a state dict (reference order)."""
+    path = os.path.join(root, "result", f"output_{number:06d}.tsv")
+    with open(path) as f:
+        header = f.readline().rstrip("\n").split("\t")
+    rows = []
+    with open(path) as f:
+        f.readline()
+        for line in f:
+            if line.strip():
+                rows.append([float(x) for x in line.rstrip("\n").split("\t") if x])
+    data = np.array(rows, dtype=np.float64)
+    col = {name: data[:, i] for i, name in enumerate(header) if name}
+    st = {
+        "c0": col["concentration"],
+        "c": np.stack([col[f"concentration_{k}"] for k in range(1, Ns + 1)]),
+        "rho0": col["density"],
+        "rho": np.stack([col[f"density_{k}"] for k in range(1, Ns + 1)]),
+        "v": np.stack([col["velocity_x"], col["velocity_y"], col["velocity_z"]]),
+        "p": col["pressure"],
+        "U": col["internal_energy"],
+        "T": col["temperature"],
+        "m": np.stack([col["momentum_x"], col["momentum_y"], col["momentum_z"]]),
+        "E": col["total_energy"],
+        "xyz": np.stack([col["coordinate_x"], col["coordinate_y"],
+                         col["coordinate_z"]]),
+    }
+    return st
+
+
+def read_times(root):
+    path = os.path.join(root, "result", "Time.tsv")
+    return np.loadtxt(path, ndmin=2)
+
+
+def clean(root):
+    shutil.rmtree(root, ignore_errors=True)

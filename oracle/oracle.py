@@ -1,0 +1,244 @@
+"""ctypes wrapper of the C oracle (test infrastructure only).
+
+Only the tests, the smoke check and the benchmark's CPU-baseline leg use this
+module, as the checker. It loads ``oracle/liboracle.so`` built by
+``oracle/Makefile`` (``make -C oracle``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MAXS = 9
+
+
+class _BC(C.Structure):
+    _fields_ = [("type", C.c_int), ("val", C.c_double * 3)]
+
+
+class _Config(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int),
+        ("N", C.c_long * 3),
+        ("L", C.c_double * 3),
+        ("origin", C.c_double * 3),
+        ("Ns", C.c_int),
+        ("M", C.c_double * MAXS),
+        ("Cv", C.c_double * MAXS),
+        ("R", C.c_double),
+        ("flux", C.c_int),
+        ("limiter", C.c_int),
+        ("gravity", C.c_int),
+        ("g", C.c_double * 3),
+        ("bc_c", (_BC * 6) * MAXS),
+        ("bc_v", _BC * 6),
+        ("bc_p", _BC * 6),
+    ]
+
+
+class _State(C.Structure):
+    _fields_ = [
+        ("n", C.c_long),
+        ("Ns", C.c_int),
+        ("c", C.POINTER(C.c_double) * (MAXS + 1)),
+        ("rho", C.POINTER(C.c_double) * (MAXS + 1)),
+        ("v", C.POINTER(C.c_double) * 3),
+        ("p", C.POINTER(C.c_double)),
+        ("U", C.POINTER(C.c_double)),
+        ("T", C.POINTER(C.c_double)),
+        ("m", C.POINTER(C.c_double) * 3),
+        ("E", C.POINTER(C.c_double)),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        L.or_create.restype = C.c_void_p
+        L.or_create.argtypes = [C.POINTER(_Config)]
+        L.or_destroy.argtypes = [C.c_void_p]
+        L.or_ncells.restype = C.c_long
+        L.or_ncells.argtypes = [C.c_void_p]
+        L.or_nfaces.restype = C.c_long
+        L.or_nfaces.argtypes = [C.c_void_p]
+        L.or_get_state.restype = C.POINTER(_State)
+        L.or_get_state.argtypes = [C.c_void_p]
+        L.or_init_primitive.argtypes = [C.c_void_p, dp, dp, dp]
+        L.or_set_full.argtypes = [C.c_void_p] + [dp] * 10
+        L.or_set_conserved.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_step.restype = C.c_int
+        L.or_step.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.or_stage.restype = C.c_int
+        L.or_stage.argtypes = [C.c_void_p, C.c_double]
+        L.or_courant_dt.restype = C.c_double
+        L.or_courant_dt.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.or_cell_geometry.argtypes = [C.c_void_p, dp, dp]
+        lp = C.POINTER(C.c_long)
+        L.or_face_geometry.argtypes = [C.c_void_p, dp, dp, dp, lp, lp, dp, dp]
+        L.or_cell_faces.argtypes = [C.c_void_p, lp]
+        L.or_hypot3.restype = C.c_double
+        L.or_hypot3.argtypes = [C.c_double] * 3
+        L.or_face_flux.restype = C.c_int
+        L.or_face_flux.argtypes = [C.c_void_p, dp, dp, dp, C.c_double, dp, dp,
+                                   C.c_double, dp, dp, dp]
+        L.or_limiter_phi.restype = C.c_double
+        L.or_limiter_phi.argtypes = [C.c_int, C.c_double]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_config(case) -> _Config:
+    cfg = _Config()
+    cfg.dim = case.dim
+    for a in range(3):
+        cfg.N[a] = int(case.N[a])
+        cfg.L[a] = float(case.L[a])
+        cfg.origin[a] = float(case.origin[a])
+        cfg.g[a] = float(case.g[a])
+    cfg.Ns = case.Ns
+    for k in range(case.Ns):
+        cfg.M[k] = float(case.M[k])
+        cfg.Cv[k] = float(case.Cv[k])
+    cfg.R = float(case.R)
+    cfg.flux = int(case.flux)
+    cfg.limiter = int(case.limiter)
+    cfg.gravity = 1 if case.gravity else 0
+    for p in range(6):
+        for k in range(case.Ns):
+            cfg.bc_c[k][p].type = int(case.bc_c[k][p].type)
+            for a in range(3):
+                cfg.bc_c[k][p].val[a] = float(case.bc_c[k][p].value[a])
+        cfg.bc_v[p].type = int(case.bc_v[p].type)
+        cfg.bc_p[p].type = int(case.bc_p[p].type)
+        for a in range(3):
+            cfg.bc_v[p].val[a] = float(case.bc_v[p].value[a])
+            cfg.bc_p[p].val[a] = float(case.bc_p[p].value[a])
+    return cfg
+
+
+class Oracle:
+    """The reference's SSP-RK2 step restated on the CPU (checker only)."""
+
+    def __init__(self, case):
+        self.case = case
+        self._cfg = make_config(case)
+        self._s = lib().or_create(C.byref(self._cfg))
+        if not self._s:
+            raise ValueError("oracle: bad configuration")
+        self.n = lib().or_ncells(self._s)
+        self.nf = lib().or_nfaces(self._s)
+
+    def __del__(self):
+        if getattr(self, "_s", None):
+            lib().or_destroy(self._s)
+            self._s = None
+
+    # -- state ---------------------------------------------------------------
+    def init_primitive(self, c, v, p):
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        lib().or_init_primitive(self._s, _dp(c), _dp(v), _dp(p))
+
+    def set_full(self, st: dict):
+        a = {k: np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+             for k, v in st.items()}
+        lib().or_set_full(self._s, _dp(a["c0"]), _dp(a["c"]), _dp(a["rho0"]),
+                          _dp(a["rho"]), _dp(a["v"]), _dp(a["p"]), _dp(a["U"]),
+                          _dp(a["T"]), _dp(a["m"]), _dp(a["E"]))
+
+    def set_conserved(self, rho_k, rho, m, E):
+        a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+             for x in (rho_k, rho, m, E)]
+        lib().or_set_conserved(self._s, *[_dp(x) for x in a])
+
+    def state(self) -> dict:
+        st = lib().or_get_state(self._s).contents
+        n, Ns = self.n, self.case.Ns
+
+        def arr(ptr):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+
+        return {
+            "c0": arr(st.c[0]),
+            "c": np.stack([arr(st.c[k]) for k in range(1, Ns + 1)]),
+            "rho0": arr(st.rho[0]),
+            "rho": np.stack([arr(st.rho[k]) for k in range(1, Ns + 1)]),
+            "v": np.stack([arr(st.v[a]) for a in range(3)]),
+            "p": arr(st.p),
+            "U": arr(st.U),
+            "T": arr(st.T),
+            "m": np.stack([arr(st.m[a]) for a in range(3)]),
+            "E": arr(st.E),
+        }
+
+    def conserved(self) -> np.ndarray:
+        """(Ns + 5, n) array: rho_1..rho_Ns, rho, m_x, m_y, m_z, E."""
+        s = self.state()
+        return np.concatenate([s["rho"], s["rho0"][None], s["m"], s["E"][None]])
+
+    # -- stepping ------------------------------------------------------------
+    def step(self, dt: float, mode: int = 0) -> int:
+        return lib().or_step(self._s, float(dt), int(mode))
+
+    def stage(self, dt: float) -> int:
+        return lib().or_stage(self._s, float(dt))
+
+    def courant_dt(self, courant: float, min_time: float) -> float:
+        return lib().or_courant_dt(self._s, float(courant), float(min_time))
+
+    # -- geometry --------------------------------------------------------------
+    def cell_geometry(self):
+        rc = np.zeros(3 * self.n)
+        V = np.zeros(self.n)
+        lib().or_cell_geometry(self._s, _dp(rc), _dp(V))
+        return rc.reshape(self.n, 3), V
+
+    def face_geometry(self):
+        nf = self.nf
+        rc, N, S = np.zeros(3 * nf), np.zeros(3 * nf), np.zeros(nf)
+        own, nei = np.zeros(nf, dtype=np.int64), np.zeros(nf, dtype=np.int64)
+        wo, wn = np.zeros(nf), np.zeros(nf)
+        lp = C.POINTER(C.c_long)
+        lib().or_face_geometry(self._s, _dp(rc), _dp(N), _dp(S),
+                               own.ctypes.data_as(lp), nei.ctypes.data_as(lp),
+                               _dp(wo), _dp(wn))
+        return dict(rc=rc.reshape(nf, 3), N=N.reshape(nf, 3), S=S, owner=own,
+                    neighbour=nei, wo=wo, wn=wn)
+
+    def cell_faces(self):
+        f = np.zeros(6 * self.n, dtype=np.int64)
+        lib().or_cell_faces(self._s, f.ctypes.data_as(C.POINTER(C.c_long)))
+        return f.reshape(self.n, 6)
+
+    def face_flux(self, N, cL, vL, pL, cR, vR, pR):
+        Ns = self.case.Ns
+        Fr, Fm, FE = np.zeros(Ns + 1), np.zeros(3), np.zeros(1)
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (N, cL, vL, cR, vR)]
+        err = lib().or_face_flux(self._s, _dp(a[0]), _dp(a[1]), _dp(a[2]),
+                                 float(pL), _dp(a[3]), _dp(a[4]), float(pR),
+                                 _dp(Fr), _dp(Fm), _dp(FE))
+        return err, Fr, Fm, FE[0]
+
+
+def hypot3(x, y, z):
+    return lib().or_hypot3(x, y, z)
+
+
+def limiter_phi(lim, r):
+    return lib().or_limiter_phi(int(lim), float(r))

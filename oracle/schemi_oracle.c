@@ -1,0 +1,1560 @@
+/*
+ * schemi_oracle.c -- CPU oracle (test infrastructure only).
+ *
+ * Plain-C restatement of the reference's explicit SSP-RK2 finite-volume step.
+ * Every function cites the reference file:line it restates.  Operation order
+ * follows the reference exactly; build with -ffp-contract=off.  The oracle is
+ * the checker for the GPU library; no product code path may use it.
+ *
+ * Reference types are restated as:
+ *   vector  -> double[3]   (Schemi/headers/vector.hpp)
+ *   tensor  -> double[9]   row-major, t[3a+b] (Schemi/headers/tensor.hpp)
+ */
+#include "schemi_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define STAB (5.0 * DBL_EPSILON) /* globalConstants.hpp:33 stabilizator */
+
+enum { ERR_NONE = 0, ERR_RIEMANN = 1, ERR_BC = 5, ERR_INIT = 11 };
+
+struct or_solver {
+    or_config cfg;
+    long nc, nf;
+    double *crc, *cV, *cminlen;      /* cells: centre (3), volume, 1/min edge */
+    double *frc, *fN, *fS;           /* faces: centre (3), unit normal (3), area */
+    long *own, *nei;
+    int *fpatch;                     /* -1 inner, else patch 0..5 */
+    double *wo, *wn;
+    int *cnf;                        /* faces per cell */
+    long *cf;                        /* 6 per cell, surfacesOfCells order */
+    int *cnn;                        /* neighbours per cell */
+    long *cn;                        /* 6 per cell, neighboursOfCells order */
+    or_state st;
+    double *pool;
+};
+
+/* ------------------------------------------------------------------------ */
+/* primitives                                                                */
+
+/* libstdc++ 11 std::hypot(x, y, z) (bits/... cmath:1854-1865), used by
+ * vector::mag (Schemi/src/primitives/vector.cpp:37-43). */
+double or_hypot3(double x, double y, double z)
+{
+    x = fabs(x);
+    y = fabs(y);
+    z = fabs(z);
+    double a = x < y ? (y < z ? z : y) : (x < z ? z : x);
+    if (a != 0.0)
+        return a * sqrt((x / a) * (x / a) + (y / a) * (y / a) + (z / a) * (z / a));
+    return 0.0;
+}
+
+/* vector & vector (vector.cpp:109-119): result starts at 0. */
+static double dot3(const double *a, const double *b)
+{
+    double r = 0;
+    for (int i = 0; i < 3; ++i)
+        r += a[i] * b[i];
+    return r;
+}
+
+/* std::min / std::max semantics */
+static double smin(double a, double b) { return (b < a) ? b : a; }
+static double smax(double a, double b) { return (a < b) ? b : a; }
+
+/* ------------------------------------------------------------------------ */
+/* mesh (Schemi/src/dataStructures/mesh.cpp)                                */
+
+static void set_cell(or_solver *s, long id, double X0, double X1, double Y0,
+                     double Y1, double Z0, double Z1)
+{
+    /* vertices r000, rX00, r0Y0, rXY0, r00Z, rX0Z, r0YZ, rXYZ
+     * (mesh.cpp:1229-1285); centre = sum / 8 (mesh.cpp:1287-1289). */
+    const double vx[8] = {X0, X1, X0, X1, X0, X1, X0, X1};
+    const double vy[8] = {Y0, Y0, Y1, Y1, Y0, Y0, Y1, Y1};
+    const double vz[8] = {Z0, Z0, Z0, Z0, Z1, Z1, Z1, Z1};
+    double sx = vx[0], sy = vy[0], sz = vz[0];
+    for (int q = 1; q < 8; ++q) {
+        sx = sx + vx[q];
+        sy = sy + vy[q];
+        sz = sz + vz[q];
+    }
+    s->crc[3 * id + 0] = sx / 8.;
+    s->crc[3 * id + 1] = sy / 8.;
+    s->crc[3 * id + 2] = sz / 8.;
+    /* V = |rX00-r000| |r0Y0-r000| |r00Z-r000| (mesh.cpp:1291-1293) */
+    const double e1 = or_hypot3(X1 - X0, Y0 - Y0, Z0 - Z0);
+    const double e2 = or_hypot3(X0 - X0, Y1 - Y0, Z0 - Z0);
+    const double e3 = or_hypot3(X0 - X0, Y0 - Y0, Z1 - Z0);
+    s->cV[id] = e1 * e2 * e3;
+    /* minimalLengthScale = 1/min edge (Schemi.cpp:649-662) */
+    const double me = smin(smin(e1, e2), e3);
+    s->cminlen[id] = 1. / me;
+}
+
+/* Face from its four vertices r00, rX0, r0Y, rXY (mesh.cpp:1337-1340). */
+static void set_face(or_solver *s, long id, const double *r00,
+                     const double *rX0, const double *r0Y, const double *rXY,
+                     long owner, long neigh, int patch)
+{
+    for (int a = 0; a < 3; ++a)
+        s->frc[3 * id + a] = (((r00[a] + rX0[a]) + r0Y[a]) + rXY[a]) / 4.;
+    double V1[3], V2[3];
+    for (int a = 0; a < 3; ++a) {
+        V1[a] = rX0[a] - r00[a];
+        V2[a] = r0Y[a] - r00[a];
+    }
+    s->fS[id] = or_hypot3(V1[0], V1[1], V1[2]) * or_hypot3(V2[0], V2[1], V2[2]);
+    /* normal = V1 ^ V2 normalised (mesh.cpp:34-53; vector.cpp:121-130) */
+    double N[3];
+    N[0] = V1[1] * V2[2] - V1[2] * V2[1];
+    N[1] = -(V1[0] * V2[2] - V1[2] * V2[0]);
+    N[2] = V1[0] * V2[1] - V1[1] * V2[0];
+    const double mag = or_hypot3(N[0], N[1], N[2]);
+    for (int a = 0; a < 3; ++a)
+        N[a] = N[a] / mag;
+    s->own[id] = owner;
+    s->nei[id] = neigh;
+    s->fpatch[id] = patch;
+    memcpy(&s->fN[3 * id], N, sizeof N);
+}
+
+static void orient_and_weights(or_solver *s)
+{
+    for (long f = 0; f < s->nf; ++f) {
+        double *N = &s->fN[3 * f];
+        const long o = s->own[f];
+        double d[3];
+        for (int a = 0; a < 3; ++a)
+            d[a] = s->frc[3 * f + a] - s->crc[3 * o + a];
+        if (dot3(d, N) < 0.)
+            for (int a = 0; a < 3; ++a)
+                N[a] = N[a] * (-1.);
+        /* calculateWeights (mesh.cpp:55-86) */
+        if (s->fpatch[f] < 0) {
+            const long n = s->nei[f];
+            double dn[3];
+            for (int a = 0; a < 3; ++a)
+                dn[a] = s->frc[3 * f + a] - s->crc[3 * n + a];
+            const double ro = or_hypot3(d[0], d[1], d[2]);
+            const double rn = or_hypot3(dn[0], dn[1], dn[2]);
+            const double sum = ro + rn;
+            s->wo[f] = rn / sum;
+            s->wn[f] = ro / sum;
+        } else {
+            s->wo[f] = 1.0;
+            s->wn[f] = 0.0;
+        }
+    }
+}
+
+static void vtx(double *r, double x, double y, double z)
+{
+    r[0] = x;
+    r[1] = y;
+    r[2] = z;
+}
+
+static int build_mesh(or_solver *s)
+{
+    const or_config *c = &s->cfg;
+    const long Nx = c->N[0], Ny = c->dim >= 2 ? c->N[1] : 1,
+               Nz = c->dim >= 3 ? c->N[2] : 1;
+    const double Lx = c->L[0], Ly = c->L[1], Lz = c->L[2];
+    const double ox = c->origin[0], oy = c->origin[1], oz = c->origin[2];
+    double r00[3], rX0[3], r0Y[3], rXY[3];
+    const long NONE = -1;
+
+    s->nc = Nx * Ny * Nz;
+    if (c->dim == 1)
+        s->nf = Nx + 1;
+    else if (c->dim == 2)
+        s->nf = Ny * (Nx + 1) + Nx * (Ny + 1);
+    else
+        s->nf = Ny * Nz * (Nx + 1) + Nx * Nz * (Ny + 1) + Nx * Ny * (Nz + 1);
+
+    const long nc = s->nc, nf = s->nf;
+    s->crc = calloc(3 * nc, sizeof(double));
+    s->cV = calloc(nc, sizeof(double));
+    s->cminlen = calloc(nc, sizeof(double));
+    s->frc = calloc(3 * nf, sizeof(double));
+    s->fN = calloc(3 * nf, sizeof(double));
+    s->fS = calloc(nf, sizeof(double));
+    s->own = calloc(nf, sizeof(long));
+    s->nei = calloc(nf, sizeof(long));
+    s->fpatch = calloc(nf, sizeof(int));
+    s->wo = calloc(nf, sizeof(double));
+    s->wn = calloc(nf, sizeof(double));
+    s->cnf = calloc(nc, sizeof(int));
+    s->cf = calloc(6 * nc, sizeof(long));
+    s->cnn = calloc(nc, sizeof(int));
+    s->cn = calloc(6 * nc, sizeof(long));
+    if (!s->crc || !s->cn)
+        return ERR_INIT;
+
+#define CID(i, j, k) ((i) + Nx * ((j) + Ny * (k)))
+    if (c->dim == 1) {
+        /* oneDParallelepiped (mesh.cpp:296-520) */
+        const double dx = Lx / Nx;
+        for (long i = 0; i < Nx; ++i)
+            set_cell(s, i, ox + dx * i, ox + dx * (i + 1), 0., Ly, 0., Lz);
+        vtx(r00, ox + 0., 0., 0.);
+        vtx(rX0, ox + 0., 0., Lz);
+        vtx(r0Y, ox + 0., Ly, 0.);
+        vtx(rXY, ox + 0., Ly, Lz);
+        set_face(s, 0, r00, rX0, r0Y, rXY, 0, NONE, 0);
+        for (long i = 1; i < Nx; ++i) {
+            const double x = ox + dx * i;
+            vtx(r00, x, 0., 0.);
+            vtx(rX0, x, 0., Lz);
+            vtx(r0Y, x, Ly, 0.);
+            vtx(rXY, x, Ly, Lz);
+            set_face(s, i, r00, rX0, r0Y, rXY, i - 1, i, -1);
+        }
+        vtx(r00, ox + Lx, 0., 0.);
+        vtx(rX0, ox + Lx, 0., Lz);
+        vtx(r0Y, ox + Lx, Ly, 0.);
+        vtx(rXY, ox + Lx, Ly, Lz);
+        set_face(s, Nx, r00, rX0, r0Y, rXY, Nx - 1, NONE, 1);
+        for (long i = 0; i < Nx; ++i) {
+            s->cnf[i] = 2;
+            s->cf[6 * i + 0] = i;
+            s->cf[6 * i + 1] = i + 1;
+            int n = 0;
+            if (i > 0)
+                s->cn[6 * i + n++] = i - 1;
+            if (i < Nx - 1)
+                s->cn[6 * i + n++] = i + 1;
+            s->cnn[i] = n;
+        }
+    } else if (c->dim == 2) {
+        /* twoDParallelepiped (mesh.cpp:522-1203) */
+        const double dx = Lx / Nx, dy = Ly / Ny;
+        for (long j = 0; j < Ny; ++j)
+            for (long i = 0; i < Nx; ++i)
+                set_cell(s, CID(i, j, 0), ox + dx * i, ox + dx * (i + 1),
+                         oy + dy * j, oy + dy * (j + 1), 0., Lz);
+        const long T0 = 0, X0 = Ny, Y0 = X0 + Ny * (Nx - 1), P0 = Y0 + Nx * (Ny - 1),
+                   R0 = P0 + Ny, Le0 = R0 + Nx;
+        for (long j = 0; j < Ny; ++j) {
+            const double x = ox + 0.;
+            vtx(r00, x, oy + dy * j, 0.);
+            vtx(rX0, x, oy + dy * j, Lz);
+            vtx(r0Y, x, oy + dy * (j + 1), 0.);
+            vtx(rXY, x, oy + dy * (j + 1), Lz);
+            set_face(s, T0 + j, r00, rX0, r0Y, rXY, CID(0, j, 0), NONE, 0);
+        }
+        for (long j = 0; j < Ny; ++j)
+            for (long i = 0; i < Nx - 1; ++i) {
+                const double x = ox + dx * (i + 1);
+                vtx(r00, x, oy + dy * j, 0.);
+                vtx(rX0, x, oy + dy * j, Lz);
+                vtx(r0Y, x, oy + dy * (j + 1), 0.);
+                vtx(rXY, x, oy + dy * (j + 1), Lz);
+                set_face(s, X0 + i + (Nx - 1) * j, r00, rX0, r0Y, rXY,
+                         CID(i, j, 0), CID(i + 1, j, 0), -1);
+            }
+        for (long j = 0; j < Ny - 1; ++j)
+            for (long i = 0; i < Nx; ++i) {
+                const double y = oy + dy * (j + 1);
+                vtx(r00, ox + dx * i, y, 0.);
+                vtx(rX0, ox + dx * (i + 1), y, 0.);
+                vtx(r0Y, ox + dx * i, y, Lz);
+                vtx(rXY, ox + dx * (i + 1), y, Lz);
+                set_face(s, Y0 + i + Nx * j, r00, rX0, r0Y, rXY, CID(i, j, 0),
+                         CID(i, j + 1, 0), -1);
+            }
+        for (long j = 0; j < Ny; ++j) {
+            const double x = ox + Lx;
+            vtx(r00, x, oy + dy * j, 0.);
+            vtx(rX0, x, oy + dy * j, Lz);
+            vtx(r0Y, x, oy + dy * (j + 1), 0.);
+            vtx(rXY, x, oy + dy * (j + 1), Lz);
+            set_face(s, P0 + j, r00, rX0, r0Y, rXY, CID(Nx - 1, j, 0), NONE, 1);
+        }
+        for (long i = 0; i < Nx; ++i) {
+            const double y = oy + 0.;
+            vtx(r00, ox + dx * i, y, 0.);
+            vtx(rX0, ox + dx * i, y, Lz);
+            vtx(r0Y, ox + dx * (i + 1), y, 0.);
+            vtx(rXY, ox + dx * (i + 1), y, Lz);
+            set_face(s, R0 + i, r00, rX0, r0Y, rXY, CID(i, 0, 0), NONE, 3);
+        }
+        for (long i = 0; i < Nx; ++i) {
+            const double y = oy + Ly;
+            vtx(r00, ox + dx * i, y, 0.);
+            vtx(rX0, ox + dx * i, y, Lz);
+            vtx(r0Y, ox + dx * (i + 1), y, 0.);
+            vtx(rXY, ox + dx * (i + 1), y, Lz);
+            set_face(s, Le0 + i, r00, rX0, r0Y, rXY, CID(i, Ny - 1, 0), NONE, 4);
+        }
+        for (long j = 0; j < Ny; ++j)
+            for (long i = 0; i < Nx; ++i) {
+                const long id = CID(i, j, 0);
+                long *f = &s->cf[6 * id];
+                f[0] = (i == 0) ? T0 + j : X0 + (i - 1) + (Nx - 1) * j;
+                f[1] = (i == Nx - 1) ? P0 + j : X0 + i + (Nx - 1) * j;
+                f[2] = (j == Ny - 1) ? Le0 + i : Y0 + i + Nx * j;
+                f[3] = (j == 0) ? R0 + i : Y0 + i + Nx * (j - 1);
+                s->cnf[id] = 4;
+                int n = 0;
+                long *nb = &s->cn[6 * id];
+                if (i > 0)
+                    nb[n++] = CID(i - 1, j, 0);
+                if (i < Nx - 1)
+                    nb[n++] = CID(i + 1, j, 0);
+                if (j < Ny - 1)
+                    nb[n++] = CID(i, j + 1, 0);
+                if (j > 0)
+                    nb[n++] = CID(i, j - 1, 0);
+                s->cnn[id] = n;
+            }
+    } else {
+        /* threeDParallelepiped (mesh.cpp:1205-3018); note dz uses the y
+         * extent (mesh.cpp:1219). */
+        const double dx = Lx / Nx, dy = Ly / Ny, dz = Ly / Nz;
+        for (long k = 0; k < Nz; ++k)
+            for (long j = 0; j < Ny; ++j)
+                for (long i = 0; i < Nx; ++i)
+                    set_cell(s, CID(i, j, k), ox + dx * i, ox + dx * (i + 1),
+                             oy + dy * j, oy + dy * (j + 1), oz + dz * k,
+                             oz + dz * (k + 1));
+        const long T0 = 0, X0 = Ny * Nz, Y0 = X0 + Nz * Ny * (Nx - 1),
+                   Z0 = Y0 + Nz * (Ny - 1) * Nx, P0 = Z0 + (Nz - 1) * Ny * Nx,
+                   B0 = P0 + Ny * Nz, R0 = B0 + Nx * Ny, Le0 = R0 + Nx * Nz,
+                   To0 = Le0 + Nx * Nz;
+        for (long k = 0; k < Nz; ++k)
+            for (long j = 0; j < Ny; ++j) {
+                const double x = ox + 0.;
+                vtx(r00, x, oy + dy * j, oz + dz * k);
+                vtx(rX0, x, oy + dy * j, oz + dz * (k + 1));
+                vtx(r0Y, x, oy + dy * (j + 1), oz + dz * k);
+                vtx(rXY, x, oy + dy * (j + 1), oz + dz * (k + 1));
+                set_face(s, T0 + j + Ny * k, r00, rX0, r0Y, rXY, CID(0, j, k),
+                         NONE, 0);
+            }
+        for (long k = 0; k < Nz; ++k)
+            for (long j = 0; j < Ny; ++j)
+                for (long i = 0; i < Nx - 1; ++i) {
+                    const double x = ox + dx * (i + 1);
+                    vtx(r00, x, oy + dy * j, oz + dz * k);
+                    vtx(rX0, x, oy + dy * j, oz + dz * (k + 1));
+                    vtx(r0Y, x, oy + dy * (j + 1), oz + dz * k);
+                    vtx(rXY, x, oy + dy * (j + 1), oz + dz * (k + 1));
+                    set_face(s, X0 + i + (Nx - 1) * (j + Ny * k), r00, rX0, r0Y,
+                             rXY, CID(i, j, k), CID(i + 1, j, k), -1);
+                }
+        for (long k = 0; k < Nz; ++k)
+            for (long j = 0; j < Ny - 1; ++j)
+                for (long i = 0; i < Nx; ++i) {
+                    const double y = oy + dy * (j + 1);
+                    vtx(r00, ox + dx * i, y, oz + dz * k);
+                    vtx(rX0, ox + dx * (i + 1), y, oz + dz * k);
+                    vtx(r0Y, ox + dx * i, y, oz + dz * (k + 1));
+                    vtx(rXY, ox + dx * (i + 1), y, oz + dz * (k + 1));
+                    set_face(s, Y0 + i + Nx * (j + (Ny - 1) * k), r00, rX0, r0Y,
+                             rXY, CID(i, j, k), CID(i, j + 1, k), -1);
+                }
+        for (long k = 0; k < Nz - 1; ++k)
+            for (long j = 0; j < Ny; ++j)
+                for (long i = 0; i < Nx; ++i) {
+                    const double z = oz + dz * (k + 1);
+                    vtx(r00, ox + dx * i, oy + dy * j, z);
+                    vtx(rX0, ox + dx * (i + 1), oy + dy * j, z);
+                    vtx(r0Y, ox + dx * i, oy + dy * (j + 1), z);
+                    vtx(rXY, ox + dx * (i + 1), oy + dy * (j + 1), z);
+                    set_face(s, Z0 + i + Nx * (j + Ny * k), r00, rX0, r0Y, rXY,
+                             CID(i, j, k), CID(i, j, k + 1), -1);
+                }
+        for (long k = 0; k < Nz; ++k)
+            for (long j = 0; j < Ny; ++j) {
+                const double x = ox + Lx;
+                vtx(r00, x, oy + dy * j, oz + dz * k);
+                vtx(rX0, x, oy + dy * j, oz + dz * (k + 1));
+                vtx(r0Y, x, oy + dy * (j + 1), oz + dz * k);
+                vtx(rXY, x, oy + dy * (j + 1), oz + dz * (k + 1));
+                set_face(s, P0 + j + Ny * k, r00, rX0, r0Y, rXY, CID(Nx - 1, j, k),
+                         NONE, 1);
+            }
+        for (long j = 0; j < Ny; ++j)
+            for (long i = 0; i < Nx; ++i) {
+                const double z = oz + 0.;
+                vtx(r00, ox + dx * i, oy + dy * j, z);
+                vtx(rX0, ox + dx * (i + 1), oy + dy * j, z);
+                vtx(r0Y, ox + dx * i, oy + dy * (j + 1), z);
+                vtx(rXY, ox + dx * (i + 1), oy + dy * (j + 1), z);
+                set_face(s, B0 + i + Nx * j, r00, rX0, r0Y, rXY, CID(i, j, 0),
+                         NONE, 2);
+            }
+        for (long k = 0; k < Nz; ++k)
+            for (long i = 0; i < Nx; ++i) {
+                const double y = oy + 0.;
+                vtx(r00, ox + dx * i, y, oz + dz * k);
+                vtx(rX0, ox + dx * i, y, oz + dz * (k + 1));
+                vtx(r0Y, ox + dx * (i + 1), y, oz + dz * k);
+                vtx(rXY, ox + dx * (i + 1), y, oz + dz * (k + 1));
+                set_face(s, R0 + i + Nx * k, r00, rX0, r0Y, rXY, CID(i, 0, k),
+                         NONE, 3);
+            }
+        for (long k = 0; k < Nz; ++k)
+            for (long i = 0; i < Nx; ++i) {
+                const double y = oy + Ly;
+                vtx(r00, ox + dx * i, y, oz + dz * k);
+                vtx(rX0, ox + dx * i, y, oz + dz * (k + 1));
+                vtx(r0Y, ox + dx * (i + 1), y, oz + dz * k);
+                vtx(rXY, ox + dx * (i + 1), y, oz + dz * (k + 1));
+                set_face(s, Le0 + i + Nx * k, r00, rX0, r0Y, rXY,
+                         CID(i, Ny - 1, k), NONE, 4);
+            }
+        for (long j = 0; j < Ny; ++j)
+            for (long i = 0; i < Nx; ++i) {
+                const double z = oz + Lz;
+                vtx(r00, ox + dx * i, oy + dy * j, z);
+                vtx(rX0, ox + dx * (i + 1), oy + dy * j, z);
+                vtx(r0Y, ox + dx * i, oy + dy * (j + 1), z);
+                vtx(rXY, ox + dx * (i + 1), oy + dy * (j + 1), z);
+                set_face(s, To0 + i + Nx * j, r00, rX0, r0Y, rXY,
+                         CID(i, j, Nz - 1), NONE, 5);
+            }
+        for (long k = 0; k < Nz; ++k)
+            for (long j = 0; j < Ny; ++j)
+                for (long i = 0; i < Nx; ++i) {
+                    const long id = CID(i, j, k);
+                    long *f = &s->cf[6 * id];
+                    f[0] = (i == 0) ? T0 + j + Ny * k
+                                    : X0 + (i - 1) + (Nx - 1) * (j + Ny * k);
+                    f[1] = (i == Nx - 1) ? P0 + j + Ny * k
+                                         : X0 + i + (Nx - 1) * (j + Ny * k);
+                    f[2] = (j == Ny - 1) ? Le0 + i + Nx * k
+                                         : Y0 + i + Nx * (j + (Ny - 1) * k);
+                    f[3] = (j == 0) ? R0 + i + Nx * k
+                                    : Y0 + i + Nx * ((j - 1) + (Ny - 1) * k);
+                    f[4] = (k == 0) ? B0 + i + Nx * j
+                                    : Z0 + i + Nx * (j + Ny * (k - 1));
+                    f[5] = (k == Nz - 1) ? To0 + i + Nx * j
+                                         : Z0 + i + Nx * (j + Ny * k);
+                    s->cnf[id] = 6;
+                    int n = 0;
+                    long *nb = &s->cn[6 * id];
+                    if (i > 0)
+                        nb[n++] = CID(i - 1, j, k);
+                    if (i < Nx - 1)
+                        nb[n++] = CID(i + 1, j, k);
+                    if (j < Ny - 1)
+                        nb[n++] = CID(i, j + 1, k);
+                    if (j > 0)
+                        nb[n++] = CID(i, j - 1, k);
+                    if (k > 0)
+                        nb[n++] = CID(i, j, k - 1);
+                    if (k < Nz - 1)
+                        nb[n++] = CID(i, j, k + 1);
+                    s->cnn[id] = n;
+                }
+    }
+#undef CID
+    orient_and_weights(s);
+    return ERR_NONE;
+}
+
+/* ------------------------------------------------------------------------ */
+/* EOS: ideal mixture (mixtureIdeal.cpp, idealFluid.cpp,                     */
+/* fractionCalculation.cpp)                                                  */
+
+/* CvMixture = sum_k (c_k / c_0) Cv_k (mixtureIdeal.cpp:387-392) */
+static double cv_mix(const or_config *cfg, const double *c /* c0..cNs */)
+{
+    double cv = 0;
+    for (int k = 0; k < cfg->Ns; ++k)
+        cv += (c[k + 1] / c[0]) * cfg->Cv[k];
+    return cv;
+}
+/* pFromUv = (R / Cv) * U (mixtureIdeal.cpp:383-395, idealFluid.cpp:13-17) */
+static double eos_p(const or_config *cfg, const double *c, double U)
+{
+    return (cfg->R / cv_mix(cfg, c)) * U;
+}
+/* UvFromp = p / (R / Cv) (mixtureIdeal.cpp:397-409, idealFluid.cpp:19-23) */
+static double eos_U(const or_config *cfg, const double *c, double p)
+{
+    return p / (cfg->R / cv_mix(cfg, c));
+}
+/* TFromUv = U / ((c0 + stab) Cv) (mixtureIdeal.cpp:443-455, idealFluid.cpp:37-41) */
+static double eos_T(const or_config *cfg, const double *c, double U)
+{
+    return U / ((c[0] + STAB) * cv_mix(cfg, c));
+}
+/* sqSonicSpeed = dpdrho + dpdUv (U + p) / rho (mixtureIdeal.cpp:495-501) */
+static double eos_c2(const or_config *cfg, const double *c, double rho, double U,
+                     double p)
+{
+    const double dpdrho = 0;
+    const double dpdUv = cfg->R / cv_mix(cfg, c);
+    return dpdrho + dpdUv * (U + p) / rho;
+}
+
+/* ------------------------------------------------------------------------ */
+/* state                                                                     */
+
+static void alloc_state(or_solver *s)
+{
+    const long n = s->nc;
+    const int Ns = s->cfg.Ns;
+    const int nfield = 2 * (Ns + 1) + 3 + 3 + 3 + 1;
+    s->pool = calloc((size_t)nfield * n, sizeof(double));
+    double *q = s->pool;
+    or_state *st = &s->st;
+    st->n = n;
+    st->Ns = Ns;
+    for (int k = 0; k <= Ns; ++k) {
+        st->c[k] = q;
+        q += n;
+    }
+    for (int k = 0; k <= Ns; ++k) {
+        st->rho[k] = q;
+        q += n;
+    }
+    for (int a = 0; a < 3; ++a) {
+        st->v[a] = q;
+        q += n;
+    }
+    st->p = q;
+    q += n;
+    st->U = q;
+    q += n;
+    st->T = q;
+    q += n;
+    for (int a = 0; a < 3; ++a) {
+        st->m[a] = q;
+        q += n;
+    }
+    st->E = q;
+}
+
+static void copy_state(const or_state *src, or_state *dst, long n, int Ns)
+{
+    for (int k = 0; k <= Ns; ++k) {
+        memcpy(dst->c[k], src->c[k], n * sizeof(double));
+        memcpy(dst->rho[k], src->rho[k], n * sizeof(double));
+    }
+    for (int a = 0; a < 3; ++a) {
+        memcpy(dst->v[a], src->v[a], n * sizeof(double));
+        memcpy(dst->m[a], src->m[a], n * sizeof(double));
+    }
+    memcpy(dst->p, src->p, n * sizeof(double));
+    memcpy(dst->U, src->U, n * sizeof(double));
+    memcpy(dst->T, src->T, n * sizeof(double));
+    memcpy(dst->E, src->E, n * sizeof(double));
+}
+
+static double *state_pool_new(long n, int Ns, or_state *st)
+{
+    const int nfield = 2 * (Ns + 1) + 3 + 3 + 3 + 1;
+    double *pool = calloc((size_t)nfield * n, sizeof(double));
+    double *q = pool;
+    st->n = n;
+    st->Ns = Ns;
+    for (int k = 0; k <= Ns; ++k) {
+        st->c[k] = q;
+        q += n;
+    }
+    for (int k = 0; k <= Ns; ++k) {
+        st->rho[k] = q;
+        q += n;
+    }
+    for (int a = 0; a < 3; ++a) {
+        st->v[a] = q;
+        q += n;
+    }
+    st->p = q;
+    q += n;
+    st->U = q;
+    q += n;
+    st->T = q;
+    q += n;
+    for (int a = 0; a < 3; ++a) {
+        st->m[a] = q;
+        q += n;
+    }
+    st->E = q;
+    return pool;
+}
+
+or_solver *or_create(const or_config *cfg)
+{
+    if (cfg->dim < 1 || cfg->dim > 3 || cfg->Ns < 1 || cfg->Ns > OR_MAXS)
+        return NULL;
+    or_solver *s = calloc(1, sizeof(or_solver));
+    s->cfg = *cfg;
+    if (build_mesh(s) != ERR_NONE) {
+        or_destroy(s);
+        return NULL;
+    }
+    alloc_state(s);
+    return s;
+}
+
+void or_destroy(or_solver *s)
+{
+    if (!s)
+        return;
+    free(s->crc);
+    free(s->cV);
+    free(s->cminlen);
+    free(s->frc);
+    free(s->fN);
+    free(s->fS);
+    free(s->own);
+    free(s->nei);
+    free(s->fpatch);
+    free(s->wo);
+    free(s->wn);
+    free(s->cnf);
+    free(s->cf);
+    free(s->cnn);
+    free(s->cn);
+    free(s->pool);
+    free(s);
+}
+
+long or_ncells(const or_solver *s) { return s->nc; }
+long or_nfaces(const or_solver *s) { return s->nf; }
+or_state *or_get_state(or_solver *s) { return &s->st; }
+
+/* primitive -> conserved initialisation (phaseInitialization.cpp:1827-1883) */
+void or_init_primitive(or_solver *s, const double *c, const double *v,
+                       const double *p)
+{
+    or_state *st = &s->st;
+    const int Ns = s->cfg.Ns;
+    const long n = s->nc;
+    double cc[OR_MAXS + 1];
+    for (long i = 0; i < n; ++i) {
+        double c0 = 0;
+        for (int k = 1; k <= Ns; ++k) {
+            st->c[k][i] = c[(k - 1) * n + i];
+            c0 += st->c[k][i];
+        }
+        st->c[0][i] = c0;
+        double r0 = 0;
+        for (int k = 1; k <= Ns; ++k) {
+            st->rho[k][i] = st->c[k][i] * s->cfg.M[k - 1];
+        }
+        for (int k = 1; k <= Ns; ++k)
+            r0 += st->rho[k][i];
+        st->rho[0][i] = r0;
+        for (int a = 0; a < 3; ++a) {
+            st->v[a][i] = v[a * n + i];
+            st->m[a][i] = st->v[a][i] * r0;
+        }
+        st->p[i] = p[i];
+        for (int k = 0; k <= Ns; ++k)
+            cc[k] = st->c[k][i];
+        st->U[i] = eos_U(&s->cfg, cc, st->p[i]);
+        st->T[i] = eos_T(&s->cfg, cc, st->U[i]);
+        double vel[3] = {st->v[0][i], st->v[1][i], st->v[2][i]};
+        const double v2 = dot3(vel, vel);
+        st->E[i] = (st->U[i] + r0 * v2 * 0.5) + 0.0;
+    }
+}
+
+void or_set_full(or_solver *s, const double *c0, const double *c,
+                 const double *rho0, const double *rho, const double *v,
+                 const double *p, const double *U, const double *T,
+                 const double *m, const double *E)
+{
+    or_state *st = &s->st;
+    const int Ns = s->cfg.Ns;
+    const long n = s->nc;
+    memcpy(st->c[0], c0, n * sizeof(double));
+    memcpy(st->rho[0], rho0, n * sizeof(double));
+    for (int k = 1; k <= Ns; ++k) {
+        memcpy(st->c[k], c + (k - 1) * n, n * sizeof(double));
+        memcpy(st->rho[k], rho + (k - 1) * n, n * sizeof(double));
+    }
+    for (int a = 0; a < 3; ++a) {
+        memcpy(st->v[a], v + a * n, n * sizeof(double));
+        memcpy(st->m[a], m + a * n, n * sizeof(double));
+    }
+    memcpy(st->p, p, n * sizeof(double));
+    memcpy(st->U, U, n * sizeof(double));
+    memcpy(st->T, T, n * sizeof(double));
+    memcpy(st->E, E, n * sizeof(double));
+}
+
+/* Primitive recovery after the stage update (Advection2dOrder.cpp:578-602). */
+static void recover_primitives(or_solver *s, or_state *st)
+{
+    const int Ns = s->cfg.Ns;
+    double cc[OR_MAXS + 1];
+    for (long i = 0; i < s->nc; ++i) {
+        double c0 = 0;
+        for (int k = 1; k <= Ns; ++k) {
+            st->c[k][i] = st->rho[k][i] / s->cfg.M[k - 1];
+            c0 += st->c[k][i];
+        }
+        st->c[0][i] = c0;
+        for (int a = 0; a < 3; ++a)
+            st->v[a][i] = st->m[a][i] / st->rho[0][i];
+        double vel[3] = {st->v[0][i], st->v[1][i], st->v[2][i]};
+        const double v2 = dot3(vel, vel);
+        st->U[i] = (st->E[i] - st->rho[0][i] * v2 * 0.5) - 0.0;
+        for (int k = 0; k <= Ns; ++k)
+            cc[k] = st->c[k][i];
+        st->p[i] = eos_p(&s->cfg, cc, st->U[i]);
+        st->T[i] = eos_T(&s->cfg, cc, st->U[i]);
+    }
+}
+
+void or_set_conserved(or_solver *s, const double *rho_k, const double *rho,
+                      const double *m, const double *E)
+{
+    or_state *st = &s->st;
+    const long n = s->nc;
+    for (int k = 1; k <= s->cfg.Ns; ++k)
+        memcpy(st->rho[k], rho_k + (k - 1) * n, n * sizeof(double));
+    memcpy(st->rho[0], rho, n * sizeof(double));
+    for (int a = 0; a < 3; ++a)
+        memcpy(st->m[a], m + a * n, n * sizeof(double));
+    memcpy(st->E, E, n * sizeof(double));
+    recover_primitives(s, st);
+}
+
+/* ------------------------------------------------------------------------ */
+/* boundary conditions (boundaryConditionValue.hpp:33-276, 278-511;         */
+/* slipFunction.cpp:15-35)                                                   */
+
+static int bc_surf_scalar(const or_bc *bc, double q, double *out)
+{
+    switch (bc->type) {
+    case OR_BC_FREE:
+        *out = q;
+        return 0;
+    case OR_BC_SLIP:
+        *out = q; /* slipFunction(scalar) is the identity */
+        return 0;
+    case OR_BC_FIXED_CELL:
+        *out = (q + bc->val[0]) * 0.5;
+        return 0;
+    case OR_BC_FIXED_SURF:
+        *out = bc->val[0];
+        return 0;
+    default:
+        return ERR_BC;
+    }
+}
+
+static int bc_cell_scalar(const or_bc *bc, double q, double *out)
+{
+    switch (bc->type) {
+    case OR_BC_FREE:
+        *out = q;
+        return 0;
+    case OR_BC_SLIP:
+        *out = q * 2 - q;
+        return 0;
+    case OR_BC_FIXED_CELL:
+        *out = bc->val[0];
+        return 0;
+    default:
+        return ERR_BC; /* fixedValueSurface throws for cell values */
+    }
+}
+
+static void slip_vec(const double *v, const double *N, double *out)
+{
+    const double s = dot3(v, N);
+    for (int a = 0; a < 3; ++a)
+        out[a] = v[a] - N[a] * s;
+}
+
+static int bc_surf_vec(const or_bc *bc, const double *v, const double *N,
+                       double *out)
+{
+    switch (bc->type) {
+    case OR_BC_FREE:
+        for (int a = 0; a < 3; ++a)
+            out[a] = v[a];
+        return 0;
+    case OR_BC_SLIP:
+        slip_vec(v, N, out);
+        return 0;
+    case OR_BC_FIXED_CELL:
+        for (int a = 0; a < 3; ++a)
+            out[a] = (v[a] + bc->val[a]) * 0.5;
+        return 0;
+    case OR_BC_FIXED_SURF:
+        for (int a = 0; a < 3; ++a)
+            out[a] = bc->val[a];
+        return 0;
+    default:
+        return ERR_BC;
+    }
+}
+
+static int bc_cell_vec(const or_bc *bc, const double *v, const double *N,
+                       double *out)
+{
+    switch (bc->type) {
+    case OR_BC_FREE:
+        for (int a = 0; a < 3; ++a)
+            out[a] = v[a];
+        return 0;
+    case OR_BC_SLIP: {
+        double sl[3];
+        slip_vec(v, N, sl);
+        for (int a = 0; a < 3; ++a)
+            out[a] = sl[a] * 2 - v[a];
+        return 0;
+    }
+    case OR_BC_FIXED_CELL:
+        for (int a = 0; a < 3; ++a)
+            out[a] = bc->val[a];
+        return 0;
+    default:
+        return ERR_BC;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* gradient (gradient.hpp:19-58) with linearInterpolate                      */
+/* (linearInterpolate.hpp:214-251)                                           */
+
+static int grad_scalar(or_solver *s, const double *q, const or_bc *bcs,
+                       double *G /* 3 per cell */)
+{
+    double *qf = malloc(s->nf * sizeof(double));
+    for (long f = 0; f < s->nf; ++f) {
+        if (s->fpatch[f] < 0)
+            qf[f] = q[s->own[f]] * s->wo[f] + q[s->nei[f]] * s->wn[f];
+        else if (bc_surf_scalar(&bcs[s->fpatch[f]], q[s->own[f]], &qf[f])) {
+            free(qf);
+            return ERR_BC;
+        }
+    }
+    for (long i = 0; i < s->nc; ++i) {
+        double acc[3] = {0, 0, 0};
+        for (int j = 0; j < s->cnf[i]; ++j) {
+            const long f = s->cf[6 * i + j];
+            double n[3];
+            if (s->own[f] == i)
+                memcpy(n, &s->fN[3 * f], sizeof n);
+            else
+                for (int a = 0; a < 3; ++a)
+                    n[a] = s->fN[3 * f + a] * -1;
+            for (int a = 0; a < 3; ++a)
+                acc[a] += (n[a] * qf[f]) * s->fS[f];
+        }
+        for (int a = 0; a < 3; ++a)
+            G[3 * i + a] = acc[a] / s->cV[i];
+    }
+    free(qf);
+    return 0;
+}
+
+static int grad_vector(or_solver *s, double *const *v, const or_bc *bcs,
+                       double *G /* 9 per cell */)
+{
+    double *vf = malloc(3 * s->nf * sizeof(double));
+    for (long f = 0; f < s->nf; ++f) {
+        const long o = s->own[f];
+        if (s->fpatch[f] < 0) {
+            const long n = s->nei[f];
+            for (int a = 0; a < 3; ++a)
+                vf[3 * f + a] = v[a][o] * s->wo[f] + v[a][n] * s->wn[f];
+        } else {
+            double vo[3] = {v[0][o], v[1][o], v[2][o]};
+            if (bc_surf_vec(&bcs[s->fpatch[f]], vo, &s->fN[3 * f], &vf[3 * f])) {
+                free(vf);
+                return ERR_BC;
+            }
+        }
+    }
+    for (long i = 0; i < s->nc; ++i) {
+        double acc[9] = {0};
+        for (int j = 0; j < s->cnf[i]; ++j) {
+            const long f = s->cf[6 * i + j];
+            double n[3];
+            if (s->own[f] == i)
+                memcpy(n, &s->fN[3 * f], sizeof n);
+            else
+                for (int a = 0; a < 3; ++a)
+                    n[a] = s->fN[3 * f + a] * -1;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    acc[3 * a + b] += (vf[3 * f + a] * n[b]) * s->fS[f];
+        }
+        for (int q = 0; q < 9; ++q)
+            G[9 * i + q] = acc[q] / s->cV[i];
+    }
+    free(vf);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* limiters (minmodLimiter.cpp:14-57, vanLeerLimiter.cpp:14-53,              */
+/* vanAlbadaLimiter.cpp:15-55, linearLimiter.cpp, zeroLimiter.cpp)           */
+
+static double phi_of(int lim, double r, double xi)
+{
+    switch (lim) {
+    case OR_LIM_MINMOD:
+        if (r <= 0.)
+            return 0;
+        else if (r <= 1.)
+            return r;
+        return smin(1.0, xi);
+    case OR_LIM_VANLEER:
+        if (r < 0.)
+            return 0;
+        return smin(2 * r / (1 + r), xi);
+    case OR_LIM_VANALBADA:
+        if (r <= 0.)
+            return 0;
+        return smin((r * r + r) / (1 + r * r), xi);
+    default:
+        return 0;
+    }
+}
+
+double or_limiter_phi(int limiter, double r)
+{
+    return phi_of(limiter, r, 2 / (1 + r));
+}
+
+/* limited value = limiter.calculate(r, G) for n components */
+static void limit_apply(int lim, const double *r, const double *G, double *L,
+                        int n)
+{
+    if (lim == OR_LIM_LINEAR) {
+        for (int q = 0; q < n; ++q)
+            L[q] = G[q];
+        return;
+    }
+    if (lim == OR_LIM_ZERO) {
+        for (int q = 0; q < n; ++q)
+            L[q] = 0;
+        return;
+    }
+    double xi[9];
+    for (int q = 0; q < n; ++q)
+        xi[q] = 2 / (1 + r[q]);
+    for (int q = 0; q < n; ++q)
+        L[q] = phi_of(lim, r[q], xi[q]) * G[q];
+}
+
+/* TVDLimiter scalar overload (TVDLimiter.cpp:12-125) */
+static int tvd_scalar(or_solver *s, const double *G, const double *q,
+                      const or_bc *bcs, double *L)
+{
+    double ent[12][3];
+    for (long i = 0; i < s->nc; ++i) {
+        const double *ci = &s->crc[3 * i];
+        int ne = 0;
+        for (int cx = 0; cx < s->cnn[i]; ++cx) {
+            const long n = s->cn[6 * i + cx];
+            double d[3];
+            for (int a = 0; a < 3; ++a)
+                d[a] = s->crc[3 * n + a] - ci[a];
+            const double dm = or_hypot3(d[0], d[1], d[2]);
+            double dn[3];
+            for (int a = 0; a < 3; ++a)
+                dn[a] = d[a] / dm;
+            for (int a = 0; a < 3; ++a)
+                ent[ne][a] = (q[n] - q[i]) / dm * dn[a];
+            ++ne;
+        }
+        if (s->cnn[i] < s->cnf[i]) {
+            for (int j = 0; j < s->cnf[i]; ++j) {
+                const long f = s->cf[6 * i + j];
+                if (s->fpatch[f] < 0)
+                    continue;
+                double d[3];
+                for (int a = 0; a < 3; ++a)
+                    d[a] = (s->frc[3 * f + a] - ci[a]) * 2;
+                const double dm = or_hypot3(d[0], d[1], d[2]);
+                double dn[3];
+                for (int a = 0; a < 3; ++a)
+                    dn[a] = d[a] / dm;
+                double qb;
+                if (bc_cell_scalar(&bcs[s->fpatch[f]], q[i], &qb))
+                    return ERR_BC;
+                for (int a = 0; a < 3; ++a)
+                    ent[ne][a] = (qb - q[i]) / dm * dn[a];
+                ++ne;
+            }
+        }
+        double mn[3], mx[3], r[3];
+        for (int a = 0; a < 3; ++a)
+            mn[a] = mx[a] = ent[0][a];
+        for (int e = 1; e < ne; ++e) {
+            for (int a = 0; a < 3; ++a)
+                mn[a] = smin(mn[a], ent[e][a]);
+            for (int a = 0; a < 3; ++a)
+                mx[a] = smax(mx[a], ent[e][a]);
+        }
+        for (int a = 0; a < 3; ++a)
+            r[a] = mn[a] / (mx[a] + STAB);
+        limit_apply(s->cfg.limiter, r, &G[3 * i], &L[3 * i], 3);
+    }
+    return 0;
+}
+
+/* TVDLimiter vector->tensor overload (TVDLimiter.cpp:127-273); entries are
+ * stored at [3 j + c] (direction j, component c) while the gradient is
+ * [3 a + b] (component a, direction b). */
+static int tvd_vector(or_solver *s, const double *G, double *const *v,
+                      const or_bc *bcs, double *L)
+{
+    double ent[12][9];
+    for (long i = 0; i < s->nc; ++i) {
+        const double *ci = &s->crc[3 * i];
+        const double vi[3] = {v[0][i], v[1][i], v[2][i]};
+        int ne = 0;
+        for (int cx = 0; cx < s->cnn[i]; ++cx) {
+            const long n = s->cn[6 * i + cx];
+            double d[3];
+            for (int a = 0; a < 3; ++a)
+                d[a] = s->crc[3 * n + a] - ci[a];
+            const double dm = or_hypot3(d[0], d[1], d[2]);
+            double dn[3];
+            for (int a = 0; a < 3; ++a)
+                dn[a] = d[a] / dm;
+            double dv[3];
+            for (int c = 0; c < 3; ++c)
+                dv[c] = v[c][n] - vi[c];
+            for (int j = 0; j < 3; ++j)
+                for (int c = 0; c < 3; ++c)
+                    ent[ne][3 * j + c] = dv[c] / dm * dn[j];
+            ++ne;
+        }
+        if (s->cnn[i] < s->cnf[i]) {
+            for (int jj = 0; jj < s->cnf[i]; ++jj) {
+                const long f = s->cf[6 * i + jj];
+                if (s->fpatch[f] < 0)
+                    continue;
+                double d[3];
+                for (int a = 0; a < 3; ++a)
+                    d[a] = (s->frc[3 * f + a] - ci[a]) * 2;
+                const double dm = or_hypot3(d[0], d[1], d[2]);
+                double dn[3];
+                for (int a = 0; a < 3; ++a)
+                    dn[a] = d[a] / dm;
+                double vb[3];
+                if (bc_cell_vec(&bcs[s->fpatch[f]], vi, &s->fN[3 * f], vb))
+                    return ERR_BC;
+                double dv[3];
+                for (int c = 0; c < 3; ++c)
+                    dv[c] = vb[c] - vi[c];
+                for (int j = 0; j < 3; ++j)
+                    for (int c = 0; c < 3; ++c)
+                        ent[ne][3 * j + c] = dv[c] / dm * dn[j];
+                ++ne;
+            }
+        }
+        double mn[9], mx[9], r[9];
+        for (int q = 0; q < 9; ++q)
+            mn[q] = mx[q] = ent[0][q];
+        for (int e = 1; e < ne; ++e) {
+            for (int q = 0; q < 9; ++q)
+                mn[q] = smin(mn[q], ent[e][q]);
+            for (int q = 0; q < 9; ++q)
+                mx[q] = smax(mx[q], ent[e][q]);
+        }
+        for (int q = 0; q < 9; ++q)
+            r[q] = mn[q] / (mx[q] + STAB);
+        limit_apply(s->cfg.limiter, r, &G[9 * i], &L[9 * i], 9);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* face states and numerical flux                                            */
+
+typedef struct {
+    double c[OR_MAXS + 1];
+    double rho[OR_MAXS + 1];
+    double v[3];
+    double m[3];
+    double p, U, T, E;
+} face_state;
+
+/* Face-side closure: Advection2dOrder.cpp:233-289 (owner side) and 301-372
+ * (neighbour side); both give the same per-face arithmetic. */
+static void close_face(const or_config *cfg, face_state *fs)
+{
+    double c0 = 0, r0 = 0;
+    for (int k = 1; k <= cfg->Ns; ++k) {
+        c0 += fs->c[k];
+        fs->rho[k] = fs->c[k] * cfg->M[k - 1];
+        r0 += fs->rho[k];
+    }
+    fs->c[0] = c0;
+    fs->rho[0] = r0;
+    fs->U = eos_U(cfg, fs->c, fs->p);
+    fs->T = eos_T(cfg, fs->c, fs->U);
+    for (int a = 0; a < 3; ++a)
+        fs->m[a] = fs->v[a] * r0;
+    const double v2 = dot3(fs->v, fs->v);
+    fs->E = fs->U + r0 * v2 * 0.5 + 0.0;
+}
+
+/* Flux vectors stored like conservativeFlows (conservativeFlows.hpp:22-34):
+ * scalars as f N (3 comps), momentum as tensor (9 comps). */
+typedef struct {
+    double rho[OR_MAXS + 1][3];
+    double m[9];
+    double E[3];
+} face_flux;
+
+/* HLLCSolver::calculateFlows per-face body (HLLCSolver.cpp:52-388) */
+static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
+                     const face_state *Nb, face_flux *F)
+{
+    const int nd = cfg->Ns + 1;
+    const double cO = sqrt(eos_c2(cfg, O->c, O->rho[0], O->U, O->p) +
+                           (2. / 3.) * 0.0 * (1. + cfg->R / cv_mix(cfg, O->c)));
+    const double cN = sqrt(eos_c2(cfg, Nb->c, Nb->rho[0], Nb->U, Nb->p) +
+                           (2. / 3.) * 0.0 * (1. + cfg->R / cv_mix(cfg, Nb->c)));
+    const double uO = dot3(O->v, N);
+    const double uN = dot3(Nb->v, N);
+    const double ptO = (2. / 3.) * 0.0, ptN = (2. / 3.) * 0.0;
+    const double SO = smin(uO - cO, uN - cN);
+    const double SN = smax(uO + cO, uN + cN);
+    const double rO = O->rho[0], rN = Nb->rho[0];
+    const double Sc = ((Nb->p + ptN) - (O->p + ptO) + rO * uO * (SO - uO) -
+                       rN * uN * (SN - uN)) /
+                      (rO * (SO - uO) - rN * (SN - uN));
+    const double PLR = 0.5 * ((O->p + ptO) + (Nb->p + ptN) +
+                              rO * (SO - uO) * (Sc - uO) + rN * (SN - uN) * (Sc - uN));
+    double IN[3]; /* tensor(1,0,0,0,1,0,0,0,1) & N (tensor.cpp:163-176) */
+    IN[0] = 1 * N[0] + 0 * N[1] + 0 * N[2];
+    IN[1] = 0 * N[0] + 1 * N[1] + 0 * N[2];
+    IN[2] = 0 * N[0] + 0 * N[1] + 1 * N[2];
+
+    if (SO >= 0) {
+        for (int k = 0; k < nd; ++k)
+            for (int a = 0; a < 3; ++a)
+                F->rho[k][a] = (N[a] * uO) * O->rho[k];
+        double w[3];
+        for (int a = 0; a < 3; ++a)
+            w[a] = O->m[a] * uO + IN[a] * (O->p + ptO);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                F->m[3 * a + b] = w[a] * N[b];
+        const double e = uO * (O->E + O->p + ptO);
+        for (int a = 0; a < 3; ++a)
+            F->E[a] = N[a] * e;
+    } else if (Sc >= 0) {
+        const double factor = SO - uO;
+        const double den = 1 / (SO - Sc);
+        double rs[OR_MAXS + 1];
+        for (int k = 0; k < nd; ++k)
+            rs[k] = O->rho[k] * factor * den;
+        double ms[3];
+        for (int a = 0; a < 3; ++a)
+            ms[a] = (O->m[a] * factor - N[a] * (O->p + ptO - PLR)) * den;
+        const double Es = (O->E * factor - (O->p + ptO) * uO + PLR * Sc) * den;
+        for (int k = 0; k < nd; ++k) {
+            const double f = O->rho[k] * uO + SO * (rs[k] - O->rho[k]);
+            for (int a = 0; a < 3; ++a)
+                F->rho[k][a] = N[a] * f;
+        }
+        double w[3];
+        for (int a = 0; a < 3; ++a)
+            w[a] = (O->m[a] * uO + IN[a] * (O->p + ptO)) + (ms[a] - O->m[a]) * SO;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                F->m[3 * a + b] = w[a] * N[b];
+        const double e = (O->E + O->p + ptO) * uO + SO * (Es - O->E);
+        for (int a = 0; a < 3; ++a)
+            F->E[a] = N[a] * e;
+    } else if (SN > 0) {
+        const double factor = SN - uN;
+        const double den = 1 / (SN - Sc);
+        double rs[OR_MAXS + 1];
+        for (int k = 0; k < nd; ++k)
+            rs[k] = Nb->rho[k] * factor * den;
+        double ms[3];
+        for (int a = 0; a < 3; ++a)
+            ms[a] = (Nb->m[a] * factor - N[a] * (Nb->p + ptN - PLR)) * den;
+        const double Es = (Nb->E * factor - (Nb->p + ptN) * uN + PLR * Sc) * den;
+        for (int k = 0; k < nd; ++k) {
+            const double f = Nb->rho[k] * uN + SN * (rs[k] - Nb->rho[k]);
+            for (int a = 0; a < 3; ++a)
+                F->rho[k][a] = N[a] * f;
+        }
+        double w[3];
+        for (int a = 0; a < 3; ++a)
+            w[a] = (Nb->m[a] * uN + IN[a] * (Nb->p + ptN)) + (ms[a] - Nb->m[a]) * SN;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                F->m[3 * a + b] = w[a] * N[b];
+        const double e = (Nb->E + Nb->p + ptN) * uN + SN * (Es - Nb->E);
+        for (int a = 0; a < 3; ++a)
+            F->E[a] = N[a] * e;
+    } else if (SN <= 0) {
+        for (int k = 0; k < nd; ++k)
+            for (int a = 0; a < 3; ++a)
+                F->rho[k][a] = (N[a] * uN) * Nb->rho[k];
+        double w[3];
+        for (int a = 0; a < 3; ++a)
+            w[a] = Nb->m[a] * uN + IN[a] * (Nb->p + ptN);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                F->m[3 * a + b] = w[a] * N[b];
+        const double e = uN * (Nb->E + Nb->p + ptN);
+        for (int a = 0; a < 3; ++a)
+            F->E[a] = N[a] * e;
+    } else {
+        return ERR_RIEMANN;
+    }
+    return 0;
+}
+
+/* HLLSolver::calculateFlows per-face body (HLLSolver.cpp:52-300) */
+static int hll_face(const or_config *cfg, const double *N, const face_state *O,
+                    const face_state *Nb, face_flux *F)
+{
+    const int nd = cfg->Ns + 1;
+    const double cO = sqrt(eos_c2(cfg, O->c, O->rho[0], O->U, O->p) +
+                           (2. / 3.) * 0.0 * (1. + cfg->R / cv_mix(cfg, O->c)));
+    const double cN = sqrt(eos_c2(cfg, Nb->c, Nb->rho[0], Nb->U, Nb->p) +
+                           (2. / 3.) * 0.0 * (1. + cfg->R / cv_mix(cfg, Nb->c)));
+    const double uO = dot3(O->v, N);
+    const double uN = dot3(Nb->v, N);
+    const double ptO = (2. / 3.) * 0.0, ptN = (2. / 3.) * 0.0;
+    const double SO = smin(uO - cO, uN - cN);
+    const double SN = smax(uO + cO, uN + cN);
+    double IN[3];
+    IN[0] = 1 * N[0] + 0 * N[1] + 0 * N[2];
+    IN[1] = 0 * N[0] + 1 * N[1] + 0 * N[2];
+    IN[2] = 0 * N[0] + 0 * N[1] + 1 * N[2];
+
+    if (SO >= 0) {
+        for (int k = 0; k < nd; ++k)
+            for (int a = 0; a < 3; ++a)
+                F->rho[k][a] = N[a] * (uO * O->rho[k]);
+        double w[3];
+        for (int a = 0; a < 3; ++a)
+            w[a] = O->m[a] * uO + IN[a] * (O->p + ptO);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                F->m[3 * a + b] = w[a] * N[b];
+        const double e = uO * (O->E + O->p + ptO);
+        for (int a = 0; a < 3; ++a)
+            F->E[a] = N[a] * e;
+    } else if (SN > 0) {
+        const double den = 1 / (SN - SO);
+        for (int k = 0; k < nd; ++k) {
+            const double f = (SN * uO * O->rho[k] - SO * uN * Nb->rho[k] +
+                              SN * SO * (Nb->rho[k] - O->rho[k])) *
+                             den;
+            for (int a = 0; a < 3; ++a)
+                F->rho[k][a] = N[a] * f;
+        }
+        double w[3];
+        for (int a = 0; a < 3; ++a)
+            w[a] = (((O->m[a] * uO + N[a] * (O->p + ptO)) * SN -
+                     (Nb->m[a] * uN + N[a] * (Nb->p + ptN)) * SO) +
+                    (Nb->m[a] - O->m[a]) * (SN * SO)) *
+                   den;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                F->m[3 * a + b] = w[a] * N[b];
+        const double e = (SN * (uO * (O->E + O->p + ptO)) -
+                          SO * (uN * (Nb->E + Nb->p + ptN)) +
+                          SN * SO * (Nb->E - O->E)) *
+                         den;
+        for (int a = 0; a < 3; ++a)
+            F->E[a] = N[a] * e;
+    } else if (SN <= 0) {
+        for (int k = 0; k < nd; ++k)
+            for (int a = 0; a < 3; ++a)
+                F->rho[k][a] = N[a] * (uN * Nb->rho[k]);
+        double w[3];
+        for (int a = 0; a < 3; ++a)
+            w[a] = Nb->m[a] * uN + IN[a] * (Nb->p + ptN);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                F->m[3 * a + b] = w[a] * N[b];
+        const double e = uN * (Nb->E + Nb->p + ptN);
+        for (int a = 0; a < 3; ++a)
+            F->E[a] = N[a] * e;
+    } else {
+        return ERR_RIEMANN;
+    }
+    return 0;
+}
+
+static int face_flux_dispatch(const or_config *cfg, const double *N,
+                              const face_state *O, const face_state *Nb,
+                              face_flux *F)
+{
+    if (cfg->flux == OR_FLUX_HLLC)
+        return hllc_face(cfg, N, O, Nb, F);
+    if (cfg->flux == OR_FLUX_HLL)
+        return hll_face(cfg, N, O, Nb, F);
+    return ERR_INIT;
+}
+
+int or_face_flux(const or_solver *s, const double *N, const double *cL,
+                 const double *vL, double pL, const double *cR,
+                 const double *vR, double pR, double *Frho, double *Fm,
+                 double *FE)
+{
+    face_state O, Nb;
+    memset(&O, 0, sizeof O);
+    memset(&Nb, 0, sizeof Nb);
+    for (int k = 1; k <= s->cfg.Ns; ++k) {
+        O.c[k] = cL[k - 1];
+        Nb.c[k] = cR[k - 1];
+    }
+    for (int a = 0; a < 3; ++a) {
+        O.v[a] = vL[a];
+        Nb.v[a] = vR[a];
+    }
+    O.p = pL;
+    Nb.p = pR;
+    close_face(&s->cfg, &O);
+    close_face(&s->cfg, &Nb);
+    face_flux F;
+    const int err = face_flux_dispatch(&s->cfg, N, &O, &Nb, &F);
+    if (err)
+        return err;
+    for (int k = 0; k <= s->cfg.Ns; ++k)
+        Frho[k] = dot3(F.rho[k], N);
+    for (int a = 0; a < 3; ++a)
+        Fm[a] = (F.m[3 * a] * N[0] + F.m[3 * a + 1] * N[1]) + F.m[3 * a + 2] * N[2];
+    *FE = dot3(F.E, N);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* one stage: Advection2dOrder (Advection2dOrder.cpp:16-636)                 */
+
+int or_stage(or_solver *s, double dt)
+{
+    const or_config *cfg = &s->cfg;
+    or_state *st = &s->st;
+    const int Ns = cfg->Ns;
+    const long nc = s->nc, nf = s->nf;
+    int err = 0;
+
+    double *Gc = malloc(3 * nc * Ns * sizeof(double));
+    double *Lc = malloc(3 * nc * Ns * sizeof(double));
+    double *Gv = malloc(9 * nc * sizeof(double));
+    double *Lv = malloc(9 * nc * sizeof(double));
+    double *Gp = malloc(3 * nc * sizeof(double));
+    double *Lp = malloc(3 * nc * sizeof(double));
+    face_state *FO = calloc(nf, sizeof(face_state));
+    face_state *FN = calloc(nf, sizeof(face_state));
+    face_flux *FF = calloc(nf, sizeof(face_flux));
+
+    /* limited gradients (Advection2dOrder.cpp:54-75) */
+    for (int k = 0; k < Ns && !err; ++k) {
+        err = grad_scalar(s, st->c[k + 1], cfg->bc_c[k], &Gc[3 * nc * k]);
+        if (!err)
+            err = tvd_scalar(s, &Gc[3 * nc * k], st->c[k + 1], cfg->bc_c[k],
+                             &Lc[3 * nc * k]);
+    }
+    if (!err)
+        err = grad_vector(s, st->v, cfg->bc_v, Gv);
+    if (!err)
+        err = tvd_vector(s, Gv, st->v, cfg->bc_v, Lv);
+    if (!err)
+        err = grad_scalar(s, st->p, cfg->bc_p, Gp);
+    if (!err)
+        err = tvd_scalar(s, Gp, st->p, cfg->bc_p, Lp);
+    if (err)
+        goto done;
+
+    /* reconstruction to faces (Advection2dOrder.cpp:110-231) */
+    for (long i = 0; i < nc; ++i) {
+        for (int j = 0; j < s->cnf[i]; ++j) {
+            const long f = s->cf[6 * i + j];
+            double d[3];
+            for (int a = 0; a < 3; ++a)
+                d[a] = s->frc[3 * f + a] - s->crc[3 * i + a];
+            face_state *fs = (s->own[f] == i) ? &FO[f] : &FN[f];
+            for (int k = 0; k < Ns; ++k)
+                fs->c[k + 1] = dot3(&Lc[3 * nc * k + 3 * i], d) + st->c[k + 1][i];
+            for (int a = 0; a < 3; ++a) {
+                const double *T = &Lv[9 * i + 3 * a];
+                fs->v[a] = (T[0] * d[0] + T[1] * d[1] + T[2] * d[2]) + st->v[a][i];
+            }
+            fs->p = dot3(&Lp[3 * i], d) + st->p[i];
+        }
+    }
+    /* owner side everywhere, neighbour side on inner faces */
+    for (long f = 0; f < nf; ++f) {
+        close_face(cfg, &FO[f]);
+        if (s->fpatch[f] < 0)
+            close_face(cfg, &FN[f]);
+    }
+    /* boundary ghosts from the owner face value (Advection2dOrder.cpp:398-521) */
+    for (long f = 0; f < nf && !err; ++f) {
+        if (s->fpatch[f] < 0)
+            continue;
+        const int pt = s->fpatch[f];
+        face_state *g = &FN[f];
+        for (int k = 1; k <= Ns && !err; ++k)
+            err = bc_cell_scalar(&cfg->bc_c[k - 1][pt], FO[f].c[k], &g->c[k]);
+        if (!err)
+            err = bc_cell_vec(&cfg->bc_v[pt], FO[f].v, &s->fN[3 * f], g->v);
+        if (!err)
+            err = bc_cell_scalar(&cfg->bc_p[pt], FO[f].p, &g->p);
+        if (!err)
+            close_face(cfg, g);
+    }
+    if (err)
+        goto done;
+
+    /* numerical fluxes (Advection2dOrder.cpp:528) */
+    for (long f = 0; f < nf && !err; ++f)
+        err = face_flux_dispatch(cfg, &s->fN[3 * f], &FO[f], &FN[f], &FF[f]);
+    if (err)
+        goto done;
+
+    /* time integration (Advection2dOrder.cpp:541-602) with divergence
+     * (divergence.hpp:315-350) */
+    for (long i = 0; i < nc; ++i) {
+        double accE = 0, accm[3] = {0, 0, 0}, accr[OR_MAXS + 1] = {0};
+        for (int j = 0; j < s->cnf[i]; ++j) {
+            const long f = s->cf[6 * i + j];
+            double n[3];
+            if (s->own[f] == i)
+                memcpy(n, &s->fN[3 * f], sizeof n);
+            else
+                for (int a = 0; a < 3; ++a)
+                    n[a] = s->fN[3 * f + a] * -1;
+            const double S = s->fS[f];
+            accE += dot3(FF[f].E, n) * S;
+            for (int a = 0; a < 3; ++a) {
+                const double *T = &FF[f].m[3 * a];
+                accm[a] += (T[0] * n[0] + T[1] * n[1] + T[2] * n[2]) * S;
+            }
+            for (int k = 0; k <= Ns; ++k)
+                accr[k] += dot3(FF[f].rho[k], n) * S;
+        }
+        const double V = s->cV[i];
+        const double r0 = st->rho[0][i];
+        const double vold[3] = {st->v[0][i], st->v[1][i], st->v[2][i]};
+        st->E[i] = st->E[i] - (accE / V) * dt;
+        for (int a = 0; a < 3; ++a)
+            st->m[a][i] = st->m[a][i] - (accm[a] / V) * dt;
+        if (cfg->gravity) {
+            st->E[i] = st->E[i] + r0 * dot3(vold, cfg->g) * dt;
+            for (int a = 0; a < 3; ++a)
+                st->m[a][i] = st->m[a][i] + (cfg->g[a] * r0) * dt;
+        }
+        for (int k = 0; k <= Ns; ++k)
+            st->rho[k][i] = st->rho[k][i] - (accr[k] / V) * dt;
+    }
+    recover_primitives(s, st);
+
+done:
+    free(Gc);
+    free(Lc);
+    free(Gv);
+    free(Lv);
+    free(Gp);
+    free(Lp);
+    free(FO);
+    free(FN);
+    free(FF);
+    return err;
+}
+
+/* bunchOfFields::average + copyFrom (bunchOfFields.hpp:166-235) */
+static void rk_average(or_solver *s, const or_state *Un)
+{
+    or_state *st = &s->st;
+    const int Ns = s->cfg.Ns;
+    const double w = 0.5, iw = 1 - w;
+    double cc[OR_MAXS + 1];
+    for (long i = 0; i < s->nc; ++i) {
+        for (int k = 0; k <= Ns; ++k) {
+            st->c[k][i] = w * st->c[k][i] + iw * Un->c[k][i];
+            st->rho[k][i] = w * st->rho[k][i] + iw * Un->rho[k][i];
+        }
+        for (int a = 0; a < 3; ++a)
+            st->m[a][i] = st->m[a][i] * w + Un->m[a][i] * iw;
+        st->E[i] = w * st->E[i] + iw * Un->E[i];
+        for (int a = 0; a < 3; ++a)
+            st->v[a][i] = st->m[a][i] / st->rho[0][i];
+        double vel[3] = {st->v[0][i], st->v[1][i], st->v[2][i]};
+        st->U[i] = st->E[i] - 0.0 - 0.5 * st->rho[0][i] * dot3(vel, vel);
+        for (int k = 0; k <= Ns; ++k)
+            cc[k] = st->c[k][i];
+        st->p[i] = eos_p(&s->cfg, cc, st->U[i]);
+        st->T[i] = eos_T(&s->cfg, cc, st->U[i]);
+    }
+}
+
+/* secondOrderStepSolverRK::calculateStep (secondOrderStepSolverRK.cpp:48-71) */
+int or_step(or_solver *s, double dt, int mode)
+{
+    if (mode == 1)
+        recover_primitives(s, &s->st);
+    or_state Un;
+    double *pool = state_pool_new(s->nc, s->cfg.Ns, &Un);
+    copy_state(&s->st, &Un, s->nc, s->cfg.Ns);
+    int err = or_stage(s, dt);
+    if (!err)
+        err = or_stage(s, dt);
+    if (!err)
+        rk_average(s, &Un);
+    if (!err && mode == 1)
+        recover_primitives(s, &s->st);
+    free(pool);
+    return err;
+}
+
+/* Courant time step (Schemi.cpp:807-833). */
+double or_courant_dt(or_solver *s, double courant, double min_time)
+{
+    or_state *st = &s->st;
+    double cc[OR_MAXS + 1];
+    double mx = 0;
+    for (long i = 0; i < s->nc; ++i) {
+        for (int k = 0; k <= s->cfg.Ns; ++k)
+            cc[k] = st->c[k][i];
+        const double c = sqrt(eos_c2(&s->cfg, cc, st->rho[0][i], st->U[i], st->p[i]));
+        const double sig =
+            (or_hypot3(st->v[0][i], st->v[1][i], st->v[2][i]) + c) * s->cminlen[i];
+        if (i == 0 || mx < sig)
+            mx = sig;
+    }
+    double dt = courant / mx;
+    dt = smax(dt, min_time);
+    return dt;
+}
+
+void or_cell_geometry(const or_solver *s, double *rc, double *V)
+{
+    memcpy(rc, s->crc, 3 * s->nc * sizeof(double));
+    memcpy(V, s->cV, s->nc * sizeof(double));
+}
+
+void or_face_geometry(const or_solver *s, double *rc, double *N, double *S,
+                      long *owner, long *neigh, double *wo, double *wn)
+{
+    memcpy(rc, s->frc, 3 * s->nf * sizeof(double));
+    memcpy(N, s->fN, 3 * s->nf * sizeof(double));
+    memcpy(S, s->fS, s->nf * sizeof(double));
+    memcpy(owner, s->own, s->nf * sizeof(long));
+    memcpy(neigh, s->nei, s->nf * sizeof(long));
+    memcpy(wo, s->wo, s->nf * sizeof(double));
+    memcpy(wn, s->wn, s->nf * sizeof(double));
+}
+
+void or_cell_faces(const or_solver *s, long *faces)
+{
+    for (long i = 0; i < s->nc; ++i)
+        for (int j = 0; j < 6; ++j)
+            faces[6 * i + j] = (j < s->cnf[i]) ? s->cf[6 * i + j] : -1;
+}

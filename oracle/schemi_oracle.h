@@ -1,0 +1,118 @@
+/*
+ * schemi_oracle.h -- CPU oracle for the explicit finite-volume step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference
+ * solver's SSP-RK2 step (Schemi/src/solverStages/secondOrderStepSolverRK.cpp),
+ * used by the tests, by the smoke check and by the CPU-baseline leg of the
+ * benchmark as the checker. No product code path may call it.
+ *
+ * The mesh is built exactly as the reference builds it (reference cell and face
+ * numbering, per-cell face order, owner/neighbour lists, normals, weights),
+ * and every arithmetic operation follows the reference's operation order, so
+ * that with -ffp-contract=off the oracle reproduces the reference bit for bit.
+ */
+#ifndef SCHEMI_ORACLE_H
+#define SCHEMI_ORACLE_H
+
+#include <stddef.h>
+
+#define OR_MAXS 9
+
+/* Reference enum values (Schemi/enumerations/ headers). */
+enum { OR_BC_BLANK = 0, OR_BC_FREE = 1, OR_BC_SLIP = 2, OR_BC_FIXED_CELL = 3,
+       OR_BC_FIXED_SURF = 4, OR_BC_INNER = 5 };
+enum { OR_FLUX_HLL = 0, OR_FLUX_HLLC = 2 };
+enum { OR_LIM_ZERO = 0, OR_LIM_MINMOD = 1, OR_LIM_VANLEER = 2, OR_LIM_LINEAR = 3,
+       OR_LIM_VANALBADA = 4 };
+/* errors enum: noErrors=0, RiemannSolverError=1, boundaryConditionError=5,
+ * initialisationError=11 */
+
+typedef struct {
+    int type;
+    double val[3];
+} or_bc;
+
+typedef struct {
+    int dim;            /* 1, 2 or 3 */
+    long N[3];
+    double L[3];        /* radius-vector of the parallelepiped */
+    double origin[3];   /* zero point */
+    int Ns;
+    double M[OR_MAXS];
+    double Cv[OR_MAXS];
+    double R;
+    int flux;
+    int limiter;
+    int gravity;
+    double g[3];
+    /* boundary conditions per patch: 0 tail, 1 point, 2 bottom, 3 right,
+     * 4 left, 5 top */
+    or_bc bc_c[OR_MAXS][6];
+    or_bc bc_v[6];
+    or_bc bc_p[6];
+} or_config;
+
+typedef struct or_solver or_solver;
+
+/* Cell state in the reference layout.  Arrays are owned by the solver. */
+typedef struct {
+    long n;
+    int Ns;
+    double *c[OR_MAXS + 1];    /* molar concentrations, c[0] = sum */
+    double *rho[OR_MAXS + 1];  /* partial densities, rho[0] = total */
+    double *v[3];
+    double *p;
+    double *U;
+    double *T;
+    double *m[3];
+    double *E;
+} or_state;
+
+or_solver *or_create(const or_config *cfg);
+void or_destroy(or_solver *s);
+long or_ncells(const or_solver *s);
+long or_nfaces(const or_solver *s);
+or_state *or_get_state(or_solver *s);
+
+/* Fills the state from primitives (c_1..c_Ns, v, p) the way the reference's
+ * initialisation does (phaseInitialization.cpp:1827-1883). */
+void or_init_primitive(or_solver *s, const double *c, const double *v,
+                       const double *p);
+/* Sets the full reference state (as written by the reference's output). */
+void or_set_full(or_solver *s, const double *c0, const double *c,
+                 const double *rho0, const double *rho, const double *v,
+                 const double *p, const double *U, const double *T,
+                 const double *m, const double *E);
+/* Sets the state from the conserved vector only (rho_1..rho_Ns, rho, m, E),
+ * deriving primitives as the stage update does. */
+void or_set_conserved(or_solver *s, const double *rho_k, const double *rho,
+                      const double *m, const double *E);
+
+/* One SSP-RK2 step with time step dt.  mode 0: literal reference state
+ * (concentrations averaged independently); mode 1: the state is re-derived
+ * from the conserved variables at the start of the step.  Returns an errors
+ * code (0 = noErrors). */
+int or_step(or_solver *s, double dt, int mode);
+/* One forward-Euler stage (Advection2dOrder). */
+int or_stage(or_solver *s, double dt);
+/* Courant time step of the reference main loop (Schemi.cpp:807-833). */
+double or_courant_dt(or_solver *s, double courant, double min_time);
+
+/* Geometry accessors (reference numbering). */
+void or_cell_geometry(const or_solver *s, double *rc, double *V);
+void or_face_geometry(const or_solver *s, double *rc, double *N, double *S,
+                      long *owner, long *neigh, double *wo, double *wn);
+void or_cell_faces(const or_solver *s, long *faces /* 6 per cell, -1 pad */);
+
+/* Point kernels exposed for unit tests. */
+double or_hypot3(double x, double y, double z);
+/* Face flux between two face states (HLL or HLLC).  state layout:
+ * [rho_0..rho_Ns, v0,v1,v2, p, U(ignored, recomputed), ...]; see the C file.
+ * Outputs the scalar fluxes (F & N) for rho_k, m (3) and E. */
+int or_face_flux(const or_solver *s, const double *N, const double *cL,
+                 const double *vL, double pL, const double *cR,
+                 const double *vR, double pR, double *Frho, double *Fm,
+                 double *FE);
+double or_limiter_phi(int limiter, double r);
+
+#endif

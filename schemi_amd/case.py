@@ -1,0 +1,87 @@
+"""Case description: mesh, mixture, scheme and boundary conditions.
+
+A :class:`Case` carries what the reference reads from its ``set/*.txt`` deck
+for the explicit step (Schemi/main/Schemi.cpp:84-163 and
+Schemi/src/thermodynamics/phaseInitialization.cpp:17-660): the parallelepiped
+(``#Radius-vector_of_system's_parallelepiped``, ``#Number_of_cells_in_direction``),
+the species (molar mass and molar Cv per ``sub_k.txt``), the universal gas
+constant, the flux scheme and limiter, gravity, and one boundary condition per
+field and patch.
+
+Cells are numbered as in the reference: ``id = i + Nx (j + Ny k)``, x fastest
+(Schemi/src/dataStructures/mesh.cpp:1223-1230).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import List, Sequence, Tuple
+
+from .enums import boundaryConditionType as BCT
+from .enums import flowSolver, typeOfTVDLimiter
+
+Vec3 = Tuple[float, float, float]
+
+
+@dataclass
+class BC:
+    """One boundary condition: type plus fixed value (scalar in ``value[0]``)."""
+
+    type: BCT = BCT.freeBoundary
+    value: Vec3 = (0.0, 0.0, 0.0)
+
+
+def _free6() -> List[BC]:
+    return [BC() for _ in range(6)]
+
+
+@dataclass
+class Case:
+    dim: int
+    N: Tuple[int, int, int]
+    L: Vec3
+    M: Sequence[float]
+    Cv: Sequence[float]
+    origin: Vec3 = (0.0, 0.0, 0.0)
+    R: float = 1.0
+    flux: flowSolver = flowSolver.HLLC
+    limiter: typeOfTVDLimiter = typeOfTVDLimiter.minmod
+    gravity: bool = False
+    g: Vec3 = (0.0, 0.0, 0.0)
+    courant: float = 0.4
+    min_time: float = 1e-15
+    bc_c: List[List[BC]] = field(default_factory=list)
+    bc_v: List[BC] = field(default_factory=_free6)
+    bc_p: List[BC] = field(default_factory=_free6)
+
+    def __post_init__(self):
+        if not self.bc_c:
+            self.bc_c = [_free6() for _ in self.M]
+        n = list(self.N)
+        if self.dim < 3:
+            n[2] = 1
+        if self.dim < 2:
+            n[1] = 1
+        self.N = tuple(int(x) for x in n)
+
+    @property
+    def Ns(self) -> int:
+        return len(self.M)
+
+    @property
+    def ncells(self) -> int:
+        return self.N[0] * self.N[1] * self.N[2]
+
+    def set_bc(self, patch: int, type_: BCT, field_: str = "all", value=None):
+        """Sets one patch's boundary condition on ``c``, ``v``, ``p`` or all."""
+        if field_ in ("all", "c"):
+            for k in range(self.Ns):
+                self.bc_c[k][patch] = BC(type_, tuple(value or (0.0, 0.0, 0.0)))
+        if field_ in ("all", "v"):
+            self.bc_v[patch] = BC(type_, tuple(value or (0.0, 0.0, 0.0)))
+        if field_ in ("all", "p"):
+            self.bc_p[patch] = BC(type_, tuple(value or (0.0, 0.0, 0.0)))
+
+    def patch_used(self, patch: int) -> bool:
+        # tail/point always; right/left from 2D; bottom/top in 3D
+        return patch in (0, 1) or (patch in (3, 4) and self.dim >= 2) or (
+            patch in (2, 5) and self.dim >= 3)

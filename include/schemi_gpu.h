@@ -1,0 +1,200 @@
+/*
+ * schemi_gpu.h -- C ABI of the MI355X explicit finite-volume step.
+ *
+ * The library replaces the reference solver's explicit SSP-RK2 step
+ * (secondOrderStepSolverRK::calculateStep,
+ *  Schemi/src/solverStages/secondOrderStepSolverRK.cpp:48-120, created by
+ *  abstractStepSolver::createStepSolver, Schemi/src/interfaces/abstractStepSolver.cpp:56-146)
+ * together with the pieces it drives: Advection2dOrder
+ * (Schemi/src/Advections/Advection2dOrder.cpp:16-636), the HLL/HLLC flux
+ * (Schemi/src/RiemannSolvers/HLLSolver.cpp, HLLCSolver.cpp), the TVD limiter
+ * (Schemi/src/limiters/TVDLimiter.cpp), the ideal-mixture closure
+ * (Schemi/src/thermodynamics/mixtureIdeal.cpp) and the Courant time step of the
+ * main loop (Schemi/main/Schemi.cpp:807-868).
+ *
+ * Conventions
+ *  - Plain C types only.  Host arrays are borrowed for the duration of a call.
+ *  - Conserved state is structure-of-arrays, Ns + 5 fields of ncells doubles:
+ *      rho_1 .. rho_Ns, rho, m_x, m_y, m_z, E
+ *    (partial densities, total density evolved separately, momentum, total
+ *    energy; Schemi/templates/bunchOfFields.hpp:28-45).
+ *  - Primitive output is Ns + 1 + 3 + 4 fields:
+ *      c_1 .. c_Ns, c, v_x, v_y, v_z, p, U, T, sound speed
+ *  - Cell order: SCHEMI_NUMBERING_REFERENCE is the reference's
+ *    id = i + Nx (j + Ny k) (Schemi/src/dataStructures/mesh.cpp:1223-1230), which
+ *    is also the device layout (x fastest: coalesced owner/neighbour gathers
+ *    along x, constant-stride gathers along y and z).
+ *  - Every function returns an errors code with the reference's values
+ *    (Schemi/enumerations/errorsEnum.hpp:15-30): 0 = noErrors.
+ *  - One context owns its device buffers and one HIP stream; calls on one
+ *    context are not re-entrant.
+ */
+#ifndef SCHEMI_GPU_H
+#define SCHEMI_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCHEMI_MAX_SPECIES 9
+
+/* errors (Schemi/enumerations/errorsEnum.hpp:15-30) */
+#define SCHEMI_OK 0
+#define SCHEMI_ERR_RIEMANN 1
+#define SCHEMI_ERR_SYSTEM 4
+#define SCHEMI_ERR_BOUNDARY 5
+#define SCHEMI_ERR_MESH 10
+#define SCHEMI_ERR_INIT 11
+
+/* boundaryConditionType (boundaryConditionTypesEnum.hpp:15-34); PERIODIC has
+ * no counterpart in the reference */
+#define SCHEMI_BC_FREE 1
+#define SCHEMI_BC_SLIP 2
+#define SCHEMI_BC_FIXED_CELL 3
+#define SCHEMI_BC_FIXED_SURFACE 4
+#define SCHEMI_BC_PERIODIC 32
+
+/* flowSolver (flowSolverEnum.hpp:15-18) */
+#define SCHEMI_FLUX_HLL 0
+#define SCHEMI_FLUX_HLLC 2
+
+/* typeOfTVDLimiter (typeOfTVDLimiterEnum.hpp:15-32) */
+#define SCHEMI_LIM_ZERO 0
+#define SCHEMI_LIM_MINMOD 1
+#define SCHEMI_LIM_VANLEER 2
+#define SCHEMI_LIM_LINEAR 3
+#define SCHEMI_LIM_VANALBADA 4
+
+/* precision of the reconstruction / limiter / Riemann solve */
+#define SCHEMI_PREC_FP64 0  /* every operation in FP64, reference order */
+#define SCHEMI_PREC_MIXED 1 /* FP32 flux path, FP64 accumulation and update */
+
+#define SCHEMI_NUMBERING_REFERENCE 0
+
+/* Patch order: 0 tail (x=0), 1 point (x=L), 2 bottom (z=0), 3 right (y=0),
+ * 4 left (y=L), 5 top (z=L) (Schemi/main/Schemi.cpp:575-592). */
+typedef struct {
+    int32_t type;
+    double value[3]; /* fixed value; scalars use value[0] */
+} schemi_bc;
+
+/* Mesh: the reference's parallelepiped generators
+ * (mesh::oneD/twoD/threeDParallelepiped, mesh.cpp:296-3018). */
+typedef struct {
+    int32_t dim;       /* 1, 2, 3 */
+    int64_t n[3];      /* cells per direction (unused ones = 1) */
+    double extent[3];  /* radius-vector of the parallelepiped */
+    double origin[3];  /* zero point */
+} schemi_mesh_desc;
+
+/* Mixture: ideal gas mixture (mixtureIdeal.cpp), molar masses and molar Cv. */
+typedef struct {
+    int32_t ns;
+    double M[SCHEMI_MAX_SPECIES];
+    double Cv[SCHEMI_MAX_SPECIES];
+    double R;
+    int32_t gas_model; /* gasModel::ideal = 0 */
+} schemi_mix_desc;
+
+/* Scheme: flux, limiter, precision, gravity, time-step control, and one
+ * boundary condition per field (c_k, v, p) and patch. */
+typedef struct {
+    int32_t flux;
+    int32_t limiter;
+    int32_t precision;
+    int32_t gravity;
+    double g[3];
+    double courant;
+    double min_time;   /* #Minimal_timestep */
+    double const_dt;   /* #Constant_timestep; <= 0 selects the Courant step */
+    schemi_bc bc_c[SCHEMI_MAX_SPECIES][6];
+    schemi_bc bc_v[6];
+    schemi_bc bc_p[6];
+} schemi_scheme_desc;
+
+typedef struct schemi_gpu_ctx schemi_gpu_ctx;
+
+/* Creates a context on the current HIP device: validates the description
+ * (mesh::instance / phaseInitialization checks), builds the per-axis
+ * geometry tables and allocates the device state. */
+int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
+                      const schemi_scheme_desc *scheme, schemi_gpu_ctx **out);
+
+void schemi_gpu_destroy(schemi_gpu_ctx *ctx);
+
+/* Uploads the conserved state (Ns + 5 fields x ncells doubles). */
+int schemi_gpu_set_state(schemi_gpu_ctx *ctx, const double *cons,
+                         int64_t ncells, int32_t numbering);
+
+/* Downloads the conserved state and, when prim is not NULL, the primitives
+ * recovered as in Advection2dOrder.cpp:578-602 (Ns + 8 fields). */
+int schemi_gpu_get_state(schemi_gpu_ctx *ctx, double *cons, double *prim,
+                         int32_t numbering);
+
+/* One SSP-RK2 step with time step dt (secondOrderStepSolverRK::calculateStep).
+ * On return *dt_next (if not NULL) holds the Courant step computed from the
+ * new state (Schemi.cpp:807-833). */
+int schemi_gpu_step(schemi_gpu_ctx *ctx, double dt, double *dt_next);
+
+/* nsteps SSP-RK2 steps with the time step kept on the device: the first
+ * step uses dt0, every later one the Courant step of the previous state,
+ * clamped so that the time does not pass t_end (Schemi.cpp:933-935).  Stops
+ * early when t_end is reached.  Outputs may be NULL. */
+int schemi_gpu_run(schemi_gpu_ctx *ctx, int64_t nsteps, double dt0,
+                   double t_end, double *t_out, int64_t *steps_out,
+                   double *dt_out);
+
+/* One forward-Euler stage (Advection2dOrder) with time step dt. */
+int schemi_gpu_stage(schemi_gpu_ctx *ctx, double dt);
+
+/* Numerical flux of nfaces independent faces (unit-test entry of
+ * abstractFlowSolver::calculateFlows, abstractFlowSolver.hpp:31-33).
+ * Face f has unit normal along axis[f] with sign sgn[f] (+1/-1); left and
+ * right states are primitives [c_1..c_Ns, v_x, v_y, v_z, p] (Ns + 4 doubles
+ * per face, face-major).  Output: Ns + 5 scalar fluxes per face
+ * (rho_1..rho_Ns, rho, m_x, m_y, m_z, E), i.e. F & N. */
+int schemi_gpu_face_flux(schemi_gpu_ctx *ctx, int64_t nfaces,
+                         const int32_t *axis, const double *sgn,
+                         const double *left, const double *right,
+                         double *flux);
+
+/* Device pointer of the conserved state (Ns + 5 fields x ncells), for
+ * callers that keep the state resident and pass it to other device code. */
+int schemi_gpu_device_state(schemi_gpu_ctx *ctx, double **dev_ptr,
+                            int64_t *ncells);
+
+/* Blocks until the context's stream is idle; returns any pending error. */
+int schemi_gpu_sync(schemi_gpu_ctx *ctx);
+
+/* Average duration (ms) of the last stage-kernel launches timed with HIP
+ * events on the context's stream (0 when none were timed); enables timing
+ * when enable != 0. */
+int schemi_gpu_timing(schemi_gpu_ctx *ctx, int32_t enable, double *avg_ms,
+                      int64_t *launches);
+
+const char *schemi_gpu_strerror(int code);
+
+/* Reference face id -> structured face (axis, i, j, k) and back; cells are
+ * not permuted.  The face permutation is a bijection between the
+ * reference's face blocks (mesh.cpp:1301-1790) and axis-major face arrays. */
+int schemi_gpu_face_permutation(const schemi_mesh_desc *mesh, int64_t *ref_to_gpu,
+                                int64_t nfaces);
+
+/* Per-axis geometry table of a mesh (host only, no device needed), as used
+ * by the kernels: which = 0 cell centres, 1 edge lengths, 2 face coordinates,
+ * 3 owner weights, 4 neighbour weights, 5 centre distances across faces,
+ * 6 x_f(i) - x_c(i), 7 x_f(i+1) - x_c(i), 8 limiter ghost distances (low,
+ * high), 9 separability flag (1 when every face centre shares the cells'
+ * transverse coordinates).  periodic_mask bit a marks periodic axes.  With
+ * out == NULL only *len is set.  Replaces the reference's mesh accessors
+ * (mesh.hpp:39-176: cells(), surfaces(), surfOwnW(), surfNeiW()). */
+int schemi_gpu_axis_table(const schemi_mesh_desc *mesh, int32_t periodic_mask,
+                          int32_t axis, int32_t which, double *out, int64_t *len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

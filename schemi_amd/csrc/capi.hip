@@ -1,0 +1,683 @@
+// capi.hip -- C ABI of the MI355X explicit finite-volume step (schemi_gpu.h).
+//
+// A context owns the device state (two conserved buffers: U^n, updated in
+// place by stage 2, and U^1), the per-axis geometry tables, the parameter
+// block and a small block of device scalars (time step, time, Courant
+// reduction), so that a run of many steps needs no host round trip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/schemi_gpu.h"
+#include "fv_launch.hpp"
+#include "geometry.hpp"
+
+using namespace sg;
+
+namespace {
+
+// device scalar block
+enum { SC_DT = 0, SC_TIME, SC_TEND, SC_SIGMAX, SC_DTNEXT, SC_STEPS, SC_DONE, SC_N };
+
+__global__ void finalize_step_kernel(double *sc, const DevParams *P, double const_dt)
+{
+    // Schemi.cpp:802-868, 933-935 for one step
+    if (sc[SC_DONE] != 0.0)
+        return;
+    const double dt = sc[SC_DT];
+    const double t = sc[SC_TIME] + dt;
+    unsigned long long bits = reinterpret_cast<unsigned long long *>(sc)[SC_SIGMAX];
+    double dtn;
+    if (const_dt > 0.0) {
+        dtn = const_dt;
+    } else {
+        const double smax = __longlong_as_double(static_cast<long long>(bits));
+        dtn = P->courant / smax;
+        dtn = (dtn < P->min_time) ? P->min_time : dtn;  // std::max(dt, minTime)
+    }
+    sc[SC_DTNEXT] = dtn;
+    if (t + dtn > sc[SC_TEND])
+        dtn = sc[SC_TEND] - t;
+    sc[SC_TIME] = t;
+    sc[SC_DT] = dtn;
+    sc[SC_STEPS] = sc[SC_STEPS] + 1.0;
+    reinterpret_cast<unsigned long long *>(sc)[SC_SIGMAX] = 0ull;
+    if (!(t < sc[SC_TEND]))
+        sc[SC_DONE] = 1.0;
+}
+
+__global__ void set_scalars_kernel(double *sc, double dt, double t, double tend)
+{
+    sc[SC_DT] = dt;
+    sc[SC_TIME] = t;
+    sc[SC_TEND] = tend;
+    reinterpret_cast<unsigned long long *>(sc)[SC_SIGMAX] = 0ull;
+    sc[SC_DTNEXT] = 0.0;
+    sc[SC_STEPS] = 0.0;
+    sc[SC_DONE] = 0.0;
+}
+
+template <int NS>
+__global__ void prims_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ U,
+                             double *__restrict__ W, long long n)
+{
+    const DevParams &P = *Pp;
+    const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (id >= n)
+        return;
+    double u[NS + 5], w[NS + 4];
+    for (int q = 0; q < NS + 5; ++q)
+        u[q] = U[q * n + id];
+    prims_from_cons<double, NS>(P, u, w);
+    double c0 = 0;
+    for (int k = 0; k < NS; ++k) {
+        W[k * n + id] = w[k];
+        c0 += w[k];
+    }
+    W[NS * n + id] = c0;
+    for (int b = 0; b < 3; ++b)
+        W[(NS + 1 + b) * n + id] = w[NS + b];
+    const double vx = w[NS], vy = w[NS + 1], vz = w[NS + 2], p = w[NS + 3];
+    const double v2 = ((0. + vx * vx) + vy * vy) + vz * vz;
+    const double Uint = (u[NS + 4] - u[NS] * v2 * 0.5) - 0.;
+    double cv = 0;
+    for (int k = 0; k < NS; ++k)
+        cv += (w[k] / c0) * P.Cv[k];
+    W[(NS + 4) * n + id] = p;
+    W[(NS + 5) * n + id] = Uint;
+    W[(NS + 6) * n + id] = Uint / ((c0 + STAB) * cv);  // TFromUv (idealFluid.cpp:37-41)
+    const double c2 = 0. + gamma1<double, NS>(P, w, c0) * (Uint + p) / u[NS];
+    W[(NS + 7) * n + id] = __builtin_sqrt(c2);
+}
+
+template <class T, int NS>
+__global__ void face_flux_kernel(const DevParams *__restrict__ Pp, long long nf,
+                                 const int *__restrict__ axis, const double *__restrict__ sgn,
+                                 const double *__restrict__ Lp, const double *__restrict__ Rp,
+                                 double *__restrict__ F, int *err)
+{
+    const DevParams &P = *Pp;
+    const long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (f >= nf)
+        return;
+    constexpr int NQ = NS + 4, NF = NS + 5;
+    T cL[NS], vL[3], cR[NS], vR[3];
+    for (int k = 0; k < NS; ++k) {
+        cL[k] = T(Lp[f * NQ + k]);
+        cR[k] = T(Rp[f * NQ + k]);
+    }
+    for (int b = 0; b < 3; ++b) {
+        vL[b] = T(Lp[f * NQ + NS + b]);
+        vR[b] = T(Rp[f * NQ + NS + b]);
+    }
+    Side<T, NS> O, B;
+    close_side<T, NS>(P, cL, vL, T(Lp[f * NQ + NS + 3]), O);
+    close_side<T, NS>(P, cR, vR, T(Rp[f * NQ + NS + 3]), B);
+    T g[NF];
+    const bool ok = (P.flux == FLUX_HLL) ? hll_flux<T, NS>(O, B, axis[f], T(sgn[f]), g)
+                                         : hllc_flux<T, NS>(O, B, axis[f], T(sgn[f]), g);
+    if (!ok)
+        atomicOr(err, ERRF_RIEMANN);
+    for (int q = 0; q < NF; ++q)
+        F[f * NF + q] = double(g[q]);
+}
+
+}  // namespace
+
+struct schemi_gpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevParams hp{};
+    DevParams *dP = nullptr;
+    double *dgeo = nullptr;
+    double *U[2] = {nullptr, nullptr};  // U^n (in place), U^1
+    double *sc = nullptr;
+    int *derr = nullptr;
+    int dim = 1, ns = 1, lin = 0, prec = 0;
+    long long n = 0;
+    int kchunk = 16;
+    double const_dt = -1.0;
+    double time = 0.0;
+    int pending_error = 0;  // reported by the next step (reference throws lazily)
+    MeshTables tables;
+    bool timing = false;
+    std::vector<hipEvent_t> ev;  // start/stop pairs of the timed stage launches
+};
+
+namespace {
+
+int hip_fail(hipError_t e)
+{
+    return e == hipSuccess ? SCHEMI_OK : SCHEMI_ERR_SYSTEM;
+}
+
+#define HIPCHK(x)                                \
+    do {                                         \
+        hipError_t e_ = (x);                     \
+        if (e_ != hipSuccess)                    \
+            return SCHEMI_ERR_SYSTEM;            \
+    } while (0)
+
+int launch_stage(schemi_gpu_ctx *c, const double *Uin, double *Uout, const double *Un,
+                 bool stage2)
+{
+    StageLaunch L;
+    L.P = c->dP;
+    L.geo = c->dgeo;
+    L.Uin = Uin;
+    L.Uout = Uout;
+    L.Un = Un;
+    L.dt = c->sc;
+    L.sigmax = reinterpret_cast<unsigned long long *>(c->sc + SC_SIGMAX);
+    L.err = c->derr;
+    L.dim = c->dim;
+    L.ns = c->ns;
+    L.lin = c->lin;
+    L.stage2 = stage2 ? 1 : 0;
+    L.nx = c->hp.N[0];
+    L.ny = c->hp.N[1];
+    L.nz = c->hp.N[2];
+    L.kchunk = c->kchunk;
+    L.stream = c->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, c->stream));
+    }
+    const int r = c->prec == SCHEMI_PREC_MIXED ? launch_stage_mixed(L) : launch_stage_f64(L);
+    if (c->timing) {
+        HIPCHK(hipEventRecord(e1, c->stream));
+        c->ev.push_back(e0);
+        c->ev.push_back(e1);
+    }
+    if (r == -1)
+        return SCHEMI_ERR_INIT;
+    return r == 0 ? SCHEMI_OK : SCHEMI_ERR_SYSTEM;
+}
+
+int one_step(schemi_gpu_ctx *c)
+{
+    // secondOrderStepSolverRK::calculateStep (secondOrderStepSolverRK.cpp:48-71):
+    // U1 = L(Un); U2 = L(U1); U^{n+1} = 0.5 U2 + 0.5 Un (fused in stage 2)
+    int r = launch_stage(c, c->U[0], c->U[1], nullptr, false);
+    if (r)
+        return r;
+    r = launch_stage(c, c->U[1], c->U[0], c->U[0], true);
+    if (r)
+        return r;
+    hipLaunchKernelGGL(finalize_step_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, c->dP,
+                       c->const_dt);
+    return hip_fail(hipGetLastError());
+}
+
+int read_error(schemi_gpu_ctx *c)
+{
+    int e = 0;
+    HIPCHK(hipMemcpyAsync(&e, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (e & ERRF_RIEMANN)
+        return SCHEMI_ERR_RIEMANN;
+    return SCHEMI_OK;
+}
+
+bool bc_ok(int t)
+{
+    return t == SCHEMI_BC_FREE || t == SCHEMI_BC_SLIP || t == SCHEMI_BC_FIXED_CELL ||
+           t == SCHEMI_BC_FIXED_SURFACE || t == SCHEMI_BC_PERIODIC;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *schemi_gpu_strerror(int code)
+{
+    switch (code) {
+    case SCHEMI_OK:
+        return "noErrors";
+    case SCHEMI_ERR_RIEMANN:
+        return "RiemannSolverError: Fatal error in Riemann solver.";
+    case SCHEMI_ERR_SYSTEM:
+        return "systemError: HIP runtime failure";
+    case SCHEMI_ERR_BOUNDARY:
+        return "boundaryConditionError";
+    case SCHEMI_ERR_MESH:
+        return "meshGenerationError";
+    case SCHEMI_ERR_INIT:
+        return "initialisationError: configuration not supported by the GPU step";
+    default:
+        return "unknown error";
+    }
+}
+
+int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
+                      const schemi_scheme_desc *scheme, schemi_gpu_ctx **out)
+{
+    if (!mesh || !mix || !scheme || !out)
+        return SCHEMI_ERR_INIT;
+    *out = nullptr;
+    const int dim = mesh->dim;
+    if (dim < 1 || dim > 3)
+        return SCHEMI_ERR_MESH;
+    for (int a = 0; a < dim; ++a)
+        if (mesh->n[a] < 1 || (dim >= 2 && mesh->n[a] < 2) || mesh->n[a] > (1 << 30))
+            return SCHEMI_ERR_MESH;
+    if (mix->ns < 1 || mix->ns > 3 || mix->gas_model != 0)
+        return SCHEMI_ERR_INIT;
+    if (scheme->flux != SCHEMI_FLUX_HLL && scheme->flux != SCHEMI_FLUX_HLLC)
+        return SCHEMI_ERR_INIT;
+    const int lim = scheme->limiter;
+    if (lim != SCHEMI_LIM_ZERO && lim != SCHEMI_LIM_MINMOD && lim != SCHEMI_LIM_VANLEER &&
+        lim != SCHEMI_LIM_LINEAR && lim != SCHEMI_LIM_VANALBADA)
+        return SCHEMI_ERR_INIT;
+    if (scheme->precision != SCHEMI_PREC_FP64 && scheme->precision != SCHEMI_PREC_MIXED)
+        return SCHEMI_ERR_INIT;
+
+    schemi_gpu_ctx *c = new (std::nothrow) schemi_gpu_ctx();
+    if (!c)
+        return SCHEMI_ERR_SYSTEM;
+    c->dim = dim;
+    c->ns = mix->ns;
+    c->lin = (lim == SCHEMI_LIM_LINEAR) ? 1 : 0;
+    c->prec = scheme->precision;
+    c->const_dt = scheme->const_dt;
+    DevParams &P = c->hp;
+    std::memset(&P, 0, sizeof P);
+    P.Ns = mix->ns;
+    P.dim = dim;
+    for (int a = 0; a < 3; ++a)
+        P.N[a] = (a < dim) ? static_cast<int>(mesh->n[a]) : 1;
+    P.n = (long long)P.N[0] * P.N[1] * P.N[2];
+    c->n = P.n;
+    P.stride[0] = 1;
+    P.stride[1] = P.N[0];
+    P.stride[2] = (long long)P.N[0] * P.N[1];
+    P.limiter = lim;
+    P.flux = scheme->flux;
+    for (int k = 0; k < mix->ns; ++k) {
+        P.M[k] = mix->M[k];
+        P.Cv[k] = mix->Cv[k];
+        P.Mf[k] = static_cast<float>(mix->M[k]);
+        P.Cvf[k] = static_cast<float>(mix->Cv[k]);
+    }
+    P.R = mix->R;
+    P.Rf = static_cast<float>(mix->R);
+    P.gravity = scheme->gravity ? 1 : 0;
+    for (int b = 0; b < 3; ++b)
+        P.g[b] = scheme->g[b];
+    P.courant = scheme->courant;
+    P.min_time = scheme->min_time;
+
+    // boundary conditions; periodic must pair both patches of an axis for
+    // every field
+    bool periodic[3] = {false, false, false};
+    for (int p = 0; p < 6; ++p) {
+        for (int k = 0; k < mix->ns; ++k) {
+            P.bct[p][k] = scheme->bc_c[k][p].type;
+            P.bcv[p][k][0] = scheme->bc_c[k][p].value[0];
+        }
+        P.bct[p][mix->ns] = scheme->bc_v[p].type;
+        for (int b = 0; b < 3; ++b)
+            P.bcv[p][mix->ns][b] = scheme->bc_v[p].value[b];
+        P.bct[p][mix->ns + 1] = scheme->bc_p[p].type;
+        P.bcv[p][mix->ns + 1][0] = scheme->bc_p[p].value[0];
+    }
+    for (int a = 0; a < dim; ++a) {
+        const int plo = patch_of(a, 0), phi = patch_of(a, 1);
+        int nper = 0, nfield = 0;
+        for (int q = 0; q < mix->ns + 2; ++q) {
+            for (int p : {plo, phi}) {
+                const int t = P.bct[p][q];
+                if (!bc_ok(t)) {
+                    delete c;
+                    return SCHEMI_ERR_BOUNDARY;
+                }
+                if (t == SCHEMI_BC_FIXED_SURFACE)
+                    c->pending_error = SCHEMI_ERR_BOUNDARY;
+                nper += (t == SCHEMI_BC_PERIODIC);
+                ++nfield;
+            }
+        }
+        if (nper != 0 && nper != nfield) {
+            delete c;
+            return SCHEMI_ERR_BOUNDARY;
+        }
+        periodic[a] = nper != 0;
+        P.periodic[a] = periodic[a] ? 1 : 0;
+    }
+    // periodic axes need whole tiles so the wrapped neighbours are staged
+    const int tile[3] = {dim == 1 ? 256 : 32, 8, 1};
+    for (int a = 0; a < 2; ++a)
+        if (periodic[a] && (P.N[a] % tile[a]) != 0) {
+            delete c;
+            return SCHEMI_ERR_INIT;
+        }
+    for (int a = 0; a < 3; ++a)
+        P.zero_entry[a] = (dim >= 2) ? 1 : 0;
+
+    c->tables = build_tables(dim, mesh->n, mesh->extent, mesh->origin, periodic);
+    if (!c->tables.separable && dim >= 2 && lim == SCHEMI_LIM_LINEAR) {
+        delete c;
+        return SCHEMI_ERR_MESH;
+    }
+    // flatten the per-axis tables
+    std::vector<double> geo;
+    auto put = [&](const std::vector<double> &v) {
+        const int off = static_cast<int>(geo.size());
+        geo.insert(geo.end(), v.begin(), v.end());
+        return off;
+    };
+    for (int a = 0; a < 3; ++a) {
+        const AxisTables &T = c->tables.ax[a];
+        P.off_xc[a] = put(T.xc);
+        P.off_e[a] = put(T.e);
+        P.off_xf[a] = put(T.xf);
+        P.off_wo[a] = put(T.wo);
+        P.off_wn[a] = put(T.wn);
+        P.off_dn[a] = put(T.dn);
+        P.off_dlo[a] = put(T.dlo);
+        P.off_dhi[a] = put(T.dhi);
+        P.ghost_lo[a] = T.ghost_lo;
+        P.ghost_hi[a] = T.ghost_hi;
+    }
+    c->kchunk = 16;
+
+    if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return SCHEMI_ERR_SYSTEM;
+    }
+    const size_t state_bytes = sizeof(double) * (size_t)(mix->ns + 5) * (size_t)c->n;
+    bool ok = hipMalloc(&c->dP, sizeof(DevParams)) == hipSuccess &&
+              hipMalloc(&c->dgeo, geo.size() * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->U[0], state_bytes) == hipSuccess &&
+              hipMalloc(&c->U[1], state_bytes) == hipSuccess &&
+              hipMalloc(&c->sc, SC_N * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->derr, sizeof(int)) == hipSuccess;
+    ok = ok &&
+         hipMemcpyAsync(c->dP, &P, sizeof P, hipMemcpyHostToDevice, c->stream) == hipSuccess &&
+         hipMemcpyAsync(c->dgeo, geo.data(), geo.size() * sizeof(double),
+                        hipMemcpyHostToDevice, c->stream) == hipSuccess &&
+         hipMemsetAsync(c->derr, 0, sizeof(int), c->stream) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, 0.0, 0.0,
+                           0.0);
+        ok = hipStreamSynchronize(c->stream) == hipSuccess;
+    }
+    if (!ok) {
+        schemi_gpu_destroy(c);
+        return SCHEMI_ERR_SYSTEM;
+    }
+    *out = c;
+    return SCHEMI_OK;
+}
+
+void schemi_gpu_destroy(schemi_gpu_ctx *c)
+{
+    if (!c)
+        return;
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    for (hipEvent_t e : c->ev)
+        (void)hipEventDestroy(e);
+    (void)hipFree(c->dP);
+    (void)hipFree(c->dgeo);
+    (void)hipFree(c->U[0]);
+    (void)hipFree(c->U[1]);
+    (void)hipFree(c->sc);
+    (void)hipFree(c->derr);
+    if (c->stream)
+        (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int schemi_gpu_set_state(schemi_gpu_ctx *c, const double *cons, int64_t ncells,
+                         int32_t numbering)
+{
+    if (!c || !cons || ncells != c->n || numbering != SCHEMI_NUMBERING_REFERENCE)
+        return SCHEMI_ERR_INIT;
+    const size_t bytes = sizeof(double) * (size_t)(c->ns + 5) * (size_t)c->n;
+    HIPCHK(hipMemcpyAsync(c->U[0], cons, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->derr, 0, sizeof(int), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->time = 0.0;
+    return SCHEMI_OK;
+}
+
+int schemi_gpu_get_state(schemi_gpu_ctx *c, double *cons, double *prim, int32_t numbering)
+{
+    if (!c || numbering != SCHEMI_NUMBERING_REFERENCE)
+        return SCHEMI_ERR_INIT;
+    const size_t bytes = sizeof(double) * (size_t)(c->ns + 5) * (size_t)c->n;
+    if (cons)
+        HIPCHK(hipMemcpyAsync(cons, c->U[0], bytes, hipMemcpyDeviceToHost, c->stream));
+    if (prim) {
+        double *dW = nullptr;
+        const size_t pbytes = sizeof(double) * (size_t)(c->ns + 8) * (size_t)c->n;
+        HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dW), pbytes, c->stream));
+        const int bs = 256;
+        const dim3 g((unsigned)((c->n + bs - 1) / bs));
+        switch (c->ns) {
+        case 1:
+            hipLaunchKernelGGL(prims_kernel<1>, g, dim3(bs), 0, c->stream, c->dP, c->U[0], dW, c->n);
+            break;
+        case 2:
+            hipLaunchKernelGGL(prims_kernel<2>, g, dim3(bs), 0, c->stream, c->dP, c->U[0], dW, c->n);
+            break;
+        default:
+            hipLaunchKernelGGL(prims_kernel<3>, g, dim3(bs), 0, c->stream, c->dP, c->U[0], dW, c->n);
+            break;
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(prim, dW, pbytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipFreeAsync(dW, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SCHEMI_OK;
+}
+
+int schemi_gpu_step(schemi_gpu_ctx *c, double dt, double *dt_next)
+{
+    if (!c)
+        return SCHEMI_ERR_INIT;
+    if (c->pending_error)
+        return c->pending_error;
+    hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, dt, c->time,
+                       HUGE_VAL);
+    HIPCHK(hipGetLastError());
+    int r = one_step(c);
+    if (r)
+        return r;
+    double sc[SC_N];
+    HIPCHK(hipMemcpyAsync(sc, c->sc, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+    r = read_error(c);
+    c->time = sc[SC_TIME];
+    if (dt_next)
+        *dt_next = sc[SC_DTNEXT];
+    return r;
+}
+
+int schemi_gpu_run(schemi_gpu_ctx *c, int64_t nsteps, double dt0, double t_end, double *t_out,
+                   int64_t *steps_out, double *dt_out)
+{
+    if (!c || nsteps < 0)
+        return SCHEMI_ERR_INIT;
+    if (c->pending_error)
+        return c->pending_error;
+    // first step: min(dt0, t_end - t) (Schemi.cpp:665-670)
+    double dt = dt0;
+    if (c->time + dt > t_end)
+        dt = t_end - c->time;
+    hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, dt, c->time,
+                       t_end);
+    HIPCHK(hipGetLastError());
+    for (int64_t s = 0; s < nsteps; ++s) {
+        const int r = one_step(c);
+        if (r)
+            return r;
+    }
+    double sc[SC_N];
+    HIPCHK(hipMemcpyAsync(sc, c->sc, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+    const int r = read_error(c);
+    c->time = sc[SC_TIME];
+    if (t_out)
+        *t_out = sc[SC_TIME];
+    if (steps_out)
+        *steps_out = static_cast<int64_t>(sc[SC_STEPS]);
+    if (dt_out)
+        *dt_out = sc[SC_DT];
+    return r;
+}
+
+int schemi_gpu_stage(schemi_gpu_ctx *c, double dt)
+{
+    if (!c)
+        return SCHEMI_ERR_INIT;
+    if (c->pending_error)
+        return c->pending_error;
+    hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, dt, c->time,
+                       HUGE_VAL);
+    HIPCHK(hipGetLastError());
+    int r = launch_stage(c, c->U[0], c->U[1], nullptr, false);
+    if (r)
+        return r;
+    const size_t bytes = sizeof(double) * (size_t)(c->ns + 5) * (size_t)c->n;
+    HIPCHK(hipMemcpyAsync(c->U[0], c->U[1], bytes, hipMemcpyDeviceToDevice, c->stream));
+    return read_error(c);
+}
+
+int schemi_gpu_face_flux(schemi_gpu_ctx *c, int64_t nfaces, const int32_t *axis,
+                         const double *sgn, const double *left, const double *right,
+                         double *flux)
+{
+    if (!c || nfaces < 0 || (nfaces > 0 && (!axis || !sgn || !left || !right || !flux)))
+        return SCHEMI_ERR_INIT;
+    if (nfaces == 0)
+        return SCHEMI_OK;
+    for (int64_t f = 0; f < nfaces; ++f)
+        if (axis[f] < 0 || axis[f] > 2)
+            return SCHEMI_ERR_INIT;
+    const int NQ = c->ns + 4, NF = c->ns + 5;
+    int *dax = nullptr;
+    double *dsg = nullptr, *dL = nullptr, *dR = nullptr, *dF = nullptr;
+    HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dax), nfaces * sizeof(int), c->stream));
+    HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dsg), nfaces * sizeof(double), c->stream));
+    HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dL), nfaces * NQ * sizeof(double), c->stream));
+    HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dR), nfaces * NQ * sizeof(double), c->stream));
+    HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dF), nfaces * NF * sizeof(double), c->stream));
+    HIPCHK(hipMemcpyAsync(dax, axis, nfaces * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dsg, sgn, nfaces * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dL, left, nfaces * NQ * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dR, right, nfaces * NQ * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->derr, 0, sizeof(int), c->stream));
+    const int bs = 256;
+    const dim3 g((unsigned)((nfaces + bs - 1) / bs));
+#define SG_FF(T, NS)                                                                    \
+    hipLaunchKernelGGL((face_flux_kernel<T, NS>), g, dim3(bs), 0, c->stream, c->dP,    \
+                       (long long)nfaces, dax, dsg, dL, dR, dF, c->derr)
+    if (c->prec == SCHEMI_PREC_MIXED) {
+        if (c->ns == 1) SG_FF(float, 1); else if (c->ns == 2) SG_FF(float, 2); else SG_FF(float, 3);
+    } else {
+        if (c->ns == 1) SG_FF(double, 1); else if (c->ns == 2) SG_FF(double, 2); else SG_FF(double, 3);
+    }
+#undef SG_FF
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(flux, dF, nfaces * NF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipFreeAsync(dax, c->stream));
+    HIPCHK(hipFreeAsync(dsg, c->stream));
+    HIPCHK(hipFreeAsync(dL, c->stream));
+    HIPCHK(hipFreeAsync(dR, c->stream));
+    HIPCHK(hipFreeAsync(dF, c->stream));
+    return read_error(c);
+}
+
+int schemi_gpu_device_state(schemi_gpu_ctx *c, double **dev_ptr, int64_t *ncells)
+{
+    if (!c)
+        return SCHEMI_ERR_INIT;
+    if (dev_ptr)
+        *dev_ptr = c->U[0];
+    if (ncells)
+        *ncells = c->n;
+    return SCHEMI_OK;
+}
+
+int schemi_gpu_sync(schemi_gpu_ctx *c)
+{
+    if (!c)
+        return SCHEMI_ERR_INIT;
+    return read_error(c);
+}
+
+int schemi_gpu_timing(schemi_gpu_ctx *c, int32_t enable, double *avg_ms, int64_t *launches)
+{
+    if (!c)
+        return SCHEMI_ERR_INIT;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double tot = 0;
+    int64_t cnt = 0;
+    for (size_t q = 0; q + 1 < c->ev.size(); q += 2) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->ev[q], c->ev[q + 1]) == hipSuccess) {
+            tot += ms;
+            ++cnt;
+        }
+    }
+    if (avg_ms)
+        *avg_ms = cnt ? tot / cnt : 0.0;
+    if (launches)
+        *launches = cnt;
+    for (hipEvent_t e : c->ev)
+        (void)hipEventDestroy(e);
+    c->ev.clear();
+    c->timing = enable != 0;
+    return SCHEMI_OK;
+}
+
+int schemi_gpu_face_permutation(const schemi_mesh_desc *mesh, int64_t *ref_to_gpu,
+                                int64_t nfaces)
+{
+    if (!mesh || !ref_to_gpu || mesh->dim < 1 || mesh->dim > 3)
+        return SCHEMI_ERR_INIT;
+    const std::vector<int64_t> p = face_permutation(mesh->dim, mesh->n);
+    if ((int64_t)p.size() != nfaces)
+        return SCHEMI_ERR_INIT;
+    std::memcpy(ref_to_gpu, p.data(), p.size() * sizeof(int64_t));
+    return SCHEMI_OK;
+}
+
+int schemi_gpu_axis_table(const schemi_mesh_desc *mesh, int32_t periodic_mask, int32_t axis,
+                          int32_t which, double *out, int64_t *len)
+{
+    if (!mesh || !len || mesh->dim < 1 || mesh->dim > 3 || axis < 0 || axis > 2 ||
+        which < 0 || which > 9)
+        return SCHEMI_ERR_INIT;
+    const bool per[3] = {(periodic_mask & 1) != 0, (periodic_mask & 2) != 0,
+                         (periodic_mask & 4) != 0};
+    const MeshTables M = build_tables(mesh->dim, mesh->n, mesh->extent, mesh->origin, per);
+    const AxisTables &T = M.ax[axis];
+    std::vector<double> v;
+    switch (which) {
+    case 0: v = T.xc; break;
+    case 1: v = T.e; break;
+    case 2: v = T.xf; break;
+    case 3: v = T.wo; break;
+    case 4: v = T.wn; break;
+    case 5: v = T.dn; break;
+    case 6: v = T.dlo; break;
+    case 7: v = T.dhi; break;
+    case 8: v = {T.ghost_lo, T.ghost_hi}; break;
+    default: v = {M.separable ? 1.0 : 0.0}; break;
+    }
+    *len = static_cast<int64_t>(v.size());
+    if (out)
+        std::memcpy(out, v.data(), v.size() * sizeof(double));
+    return SCHEMI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,367 @@
+// fv_device.hpp -- device-side building blocks of the explicit FV step.
+//
+// Everything here is templated on the flux precision T (double for the FP64
+// path, float for the mixed path).  In the FP64 path every expression keeps
+// the reference's operation order, and the translation unit is compiled with
+// -ffp-contract=off, so results are bit-identical to the reference's CPU step.
+//
+// Faces of the structured mesh are axis-aligned: the reference's unit normals
+// are exactly +-1 along the axis and +-0 across it (mesh.cpp:34-53), so a dot
+// product v & N reduces to +-v_axis, and a flux vector f N contracted with
+// the cell-side normal s N reduces to s f.  The terms dropped by these
+// reductions are exact signed zeros.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sg {
+
+constexpr int MAXS = 9;
+constexpr double STAB = 5.0 * 2.220446049250313080847e-16;  // globalConstants.hpp:33
+
+enum : int { BC_FREE = 1, BC_SLIP = 2, BC_FIXED_CELL = 3, BC_FIXED_SURF = 4,
+             BC_PERIODIC = 32 };
+enum : int { LIM_ZERO = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_LINEAR = 3,
+             LIM_VANALBADA = 4 };
+enum : int { FLUX_HLL = 0, FLUX_HLLC = 2 };
+enum : int { ERRF_RIEMANN = 1, ERRF_BOUNDARY = 2 };
+
+// Parameters shared by all kernels of a context, kept in device memory and
+// read with uniform (scalar) loads.
+struct DevParams {
+    int Ns, dim;
+    int N[3];
+    long long n;
+    long long stride[3];  // 1, Nx, Nx*Ny
+    int periodic[3];
+    int zero_entry[3];    // the limiter ratio along the axis has an exact 0 entry
+    int limiter, flux;
+    double M[MAXS], Cv[MAXS], R;
+    float Mf[MAXS], Cvf[MAXS], Rf;
+    int gravity;
+    double g[3];
+    // boundary conditions per patch (0 tail .. 5 top) and field
+    // (0..Ns-1: c_k, Ns: v, Ns+1: p)
+    int bct[6][MAXS + 2];
+    double bcv[6][MAXS + 2][3];
+    double ghost_lo[3], ghost_hi[3];  // |2 (x_f - x_c)| of the boundary ghosts
+    // offsets (in doubles) of the per-axis tables inside geo[]
+    int off_xc[3], off_e[3], off_xf[3], off_wo[3], off_wn[3], off_dn[3],
+        off_dlo[3], off_dhi[3];
+    double courant, min_time;
+};
+
+// patch of axis a, side (0 = low, 1 = high): tail/point, right/left, bottom/top
+__host__ __device__ inline int patch_of(int a, int side)
+{
+    return a == 0 ? side : (a == 1 ? (side ? 4 : 3) : (side ? 5 : 2));
+}
+
+template <class T> __device__ __forceinline__ T tmin(T a, T b) { return (b < a) ? b : a; }
+template <class T> __device__ __forceinline__ T tmax(T a, T b) { return (a < b) ? b : a; }
+
+template <class T> struct Real;
+template <> struct Real<double> {
+    __device__ static double sqrt_(double x) { return __builtin_sqrt(x); }
+};
+template <> struct Real<float> {
+    __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
+};
+
+// libstdc++ 11 std::hypot(x, y, z), used by vector::mag (vector.cpp:37-43).
+__device__ inline double hypot3(double x, double y, double z)
+{
+    x = __builtin_fabs(x);
+    y = __builtin_fabs(y);
+    z = __builtin_fabs(z);
+    const double a = x < y ? (y < z ? z : y) : (x < z ? z : x);
+    if (a != 0.0)
+        return a * __builtin_sqrt((x / a) * (x / a) + (y / a) * (y / a) +
+                                  (z / a) * (z / a));
+    return 0.0;
+}
+
+template <class T> __device__ __forceinline__ T mixM(const DevParams &P, int k);
+template <> __device__ __forceinline__ double mixM<double>(const DevParams &P, int k) { return P.M[k]; }
+template <> __device__ __forceinline__ float mixM<float>(const DevParams &P, int k) { return P.Mf[k]; }
+template <class T> __device__ __forceinline__ T mixCv(const DevParams &P, int k);
+template <> __device__ __forceinline__ double mixCv<double>(const DevParams &P, int k) { return P.Cv[k]; }
+template <> __device__ __forceinline__ float mixCv<float>(const DevParams &P, int k) { return P.Cvf[k]; }
+template <class T> __device__ __forceinline__ T mixR(const DevParams &P);
+template <> __device__ __forceinline__ double mixR<double>(const DevParams &P) { return P.R; }
+template <> __device__ __forceinline__ float mixR<float>(const DevParams &P) { return P.Rf; }
+
+// R / Cv_mix with Cv_mix = sum_k (c_k / c_0) Cv_k (mixtureIdeal.cpp:383-395,
+// fractionCalculation.cpp:35-44).
+template <class T, int NS>
+__device__ __forceinline__ T gamma1(const DevParams &P, const T *c, T c0)
+{
+    T cv = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        cv += (c[k] / c0) * mixCv<T>(P, k);
+    return mixR<T>(P) / cv;
+}
+
+// Primitive recovery from the conserved vector, as after the stage update
+// (Advection2dOrder.cpp:578-602): c_k = rho_k / M_k, c_0 = sum c_k,
+// v = m / rho, U = E - rho v^2 / 2 - 0, p = (R / Cv) U.
+// u: [rho_1..rho_Ns, rho, m_x, m_y, m_z, E]; w: [c_1..c_Ns, v_x, v_y, v_z, p]
+template <class T, int NS>
+__device__ __forceinline__ void prims_from_cons(const DevParams &P, const T *u, T *w)
+{
+    T c0 = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        w[k] = u[k] / mixM<T>(P, k);
+        c0 += w[k];
+    }
+    const T rho = u[NS];
+    const T vx = u[NS + 1] / rho, vy = u[NS + 2] / rho, vz = u[NS + 3] / rho;
+    w[NS] = vx;
+    w[NS + 1] = vy;
+    w[NS + 2] = vz;
+    const T v2 = ((T(0) + vx * vx) + vy * vy) + vz * vz;
+    const T U = (u[NS + 4] - rho * v2 * T(0.5)) - T(0);
+    w[NS + 3] = gamma1<T, NS>(P, w, c0) * U;
+}
+
+// Reconstructed face side closed with the EOS (Advection2dOrder.cpp:233-289
+// owner side, 301-372 neighbour side).
+template <class T, int NS> struct Side {
+    T rho[NS];  // partial densities
+    T r0;       // total density
+    T v[3];
+    T m[3];
+    T p, U, E, cs;
+};
+
+template <class T, int NS>
+__device__ __forceinline__ void close_side(const DevParams &P, const T *c, const T *v,
+                                           T p, Side<T, NS> &s)
+{
+    T c0 = 0, r0 = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        c0 += c[k];
+        s.rho[k] = c[k] * mixM<T>(P, k);
+        r0 += s.rho[k];
+    }
+    s.r0 = r0;
+    const T g1 = gamma1<T, NS>(P, c, c0);
+    s.p = p;
+    s.U = p / g1;  // UvFromp (idealFluid.cpp:19-23)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s.v[a] = v[a];
+        s.m[a] = v[a] * r0;
+    }
+    const T v2 = ((T(0) + v[0] * v[0]) + v[1] * v[1]) + v[2] * v[2];
+    s.E = (s.U + r0 * v2 * T(0.5)) + T(0);
+    // sqSonicSpeed (mixtureIdeal.cpp:495-501); the turbulent term is 0
+    const T c2 = T(0) + g1 * (s.U + p) / r0;
+    s.cs = Real<T>::sqrt_(c2);
+}
+
+// Flux through one axis-aligned face: g = F & N for every conserved field
+// (rho_1..rho_Ns, rho, m_x, m_y, m_z, E).  ax: axis, sg: sign of the normal
+// (+1 for inner and high-side faces, -1 for the low-side boundary faces).
+// O is the owner side, B the neighbour side.  Returns false when no wave
+// pattern applies (NaN speeds), as HLLCSolver.cpp:385-388 / HLLSolver.cpp:297-300.
+template <class T, int NS>
+__device__ __forceinline__ bool hllc_flux(const Side<T, NS> &O, const Side<T, NS> &B,
+                                          int ax, T sg, T *g)
+{
+    const T uO = sg * O.v[ax], uN = sg * B.v[ax];  // v & N
+    const T SO = tmin(uO - O.cs, uN - B.cs);
+    const T SN = tmax(uO + O.cs, uN + B.cs);
+    const T rO = O.r0, rN = B.r0;
+    const T Sc = (B.p - O.p + rO * uO * (SO - uO) - rN * uN * (SN - uN)) /
+                 (rO * (SO - uO) - rN * (SN - uN));
+    const T PLR = T(0.5) * (O.p + B.p + rO * (SO - uO) * (Sc - uO) +
+                            rN * (SN - uN) * (Sc - uN));
+    if (SO >= T(0)) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            g[k] = uO * O.rho[k];
+        g[NS] = uO * O.r0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            g[NS + 1 + b] = (b == ax) ? O.m[b] * uO + sg * O.p : O.m[b] * uO;
+        g[NS + 4] = uO * (O.E + O.p);
+    } else if (Sc >= T(0)) {
+        const T factor = SO - uO;
+        const T den = T(1) / (SO - Sc);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const T rs = O.rho[k] * factor * den;
+            g[k] = O.rho[k] * uO + SO * (rs - O.rho[k]);
+        }
+        {
+            const T rs = O.r0 * factor * den;
+            g[NS] = O.r0 * uO + SO * (rs - O.r0);
+        }
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            if (b == ax) {
+                const T ms = (O.m[b] * factor - sg * (O.p - PLR)) * den;
+                g[NS + 1 + b] = (O.m[b] * uO + sg * O.p) + (ms - O.m[b]) * SO;
+            } else {
+                const T ms = (O.m[b] * factor) * den;
+                g[NS + 1 + b] = (O.m[b] * uO) + (ms - O.m[b]) * SO;
+            }
+        }
+        const T Es = (O.E * factor - O.p * uO + PLR * Sc) * den;
+        g[NS + 4] = (O.E + O.p) * uO + SO * (Es - O.E);
+    } else if (SN > T(0)) {
+        const T factor = SN - uN;
+        const T den = T(1) / (SN - Sc);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const T rs = B.rho[k] * factor * den;
+            g[k] = B.rho[k] * uN + SN * (rs - B.rho[k]);
+        }
+        {
+            const T rs = B.r0 * factor * den;
+            g[NS] = B.r0 * uN + SN * (rs - B.r0);
+        }
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            if (b == ax) {
+                const T ms = (B.m[b] * factor - sg * (B.p - PLR)) * den;
+                g[NS + 1 + b] = (B.m[b] * uN + sg * B.p) + (ms - B.m[b]) * SN;
+            } else {
+                const T ms = (B.m[b] * factor) * den;
+                g[NS + 1 + b] = (B.m[b] * uN) + (ms - B.m[b]) * SN;
+            }
+        }
+        const T Es = (B.E * factor - B.p * uN + PLR * Sc) * den;
+        g[NS + 4] = (B.E + B.p) * uN + SN * (Es - B.E);
+    } else if (SN <= T(0)) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            g[k] = uN * B.rho[k];
+        g[NS] = uN * B.r0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            g[NS + 1 + b] = (b == ax) ? B.m[b] * uN + sg * B.p : B.m[b] * uN;
+        g[NS + 4] = uN * (B.E + B.p);
+    } else {
+#pragma unroll
+        for (int q = 0; q < NS + 5; ++q)
+            g[q] = SO;  // NaN
+        return false;
+    }
+    return true;
+}
+
+template <class T, int NS>
+__device__ __forceinline__ bool hll_flux(const Side<T, NS> &O, const Side<T, NS> &B,
+                                         int ax, T sg, T *g)
+{
+    const T uO = sg * O.v[ax], uN = sg * B.v[ax];
+    const T SO = tmin(uO - O.cs, uN - B.cs);
+    const T SN = tmax(uO + O.cs, uN + B.cs);
+    if (SO >= T(0)) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            g[k] = uO * O.rho[k];
+        g[NS] = uO * O.r0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            g[NS + 1 + b] = (b == ax) ? O.m[b] * uO + sg * O.p : O.m[b] * uO;
+        g[NS + 4] = uO * (O.E + O.p);
+    } else if (SN > T(0)) {
+        const T den = T(1) / (SN - SO);
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            g[k] = (SN * uO * O.rho[k] - SO * uN * B.rho[k] +
+                    SN * SO * (B.rho[k] - O.rho[k])) * den;
+        g[NS] = (SN * uO * O.r0 - SO * uN * B.r0 + SN * SO * (B.r0 - O.r0)) * den;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            if (b == ax)
+                g[NS + 1 + b] = (((O.m[b] * uO + sg * O.p) * SN -
+                                  (B.m[b] * uN + sg * B.p) * SO) +
+                                 (B.m[b] - O.m[b]) * (SN * SO)) * den;
+            else
+                g[NS + 1 + b] = (((O.m[b] * uO) * SN - (B.m[b] * uN) * SO) +
+                                 (B.m[b] - O.m[b]) * (SN * SO)) * den;
+        }
+        g[NS + 4] = (SN * (uO * (O.E + O.p)) - SO * (uN * (B.E + B.p)) +
+                     SN * SO * (B.E - O.E)) * den;
+    } else if (SN <= T(0)) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            g[k] = uN * B.rho[k];
+        g[NS] = uN * B.r0;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            g[NS + 1 + b] = (b == ax) ? B.m[b] * uN + sg * B.p : B.m[b] * uN;
+        g[NS + 4] = uN * (B.E + B.p);
+    } else {
+#pragma unroll
+        for (int q = 0; q < NS + 5; ++q)
+            g[q] = SO;
+        return false;
+    }
+    return true;
+}
+
+// Limiter function phi(r) with xi = 2 / (1 + r) evaluated where used
+// (minmodLimiter.cpp:14-27, vanLeerLimiter.cpp:14-24, vanAlbadaLimiter.cpp:15-26;
+// zeroLimiter.cpp returns 0).  lim is uniform across the launch.
+template <class T> __device__ __forceinline__ T phi(int lim, T r)
+{
+    if (lim == LIM_MINMOD) {
+        if (r <= T(0))
+            return T(0);
+        else if (r <= T(1))
+            return r;
+        return tmin(T(1), T(2) / (T(1) + r));
+    } else if (lim == LIM_VANLEER) {
+        if (r < T(0))
+            return T(0);
+        return tmin(T(2) * r / (T(1) + r), T(2) / (T(1) + r));
+    } else if (lim == LIM_VANALBADA) {
+        if (r <= T(0))
+            return T(0);
+        return tmin((r * r + r) / (T(1) + r * r), T(2) / (T(1) + r));
+    }
+    return T(0);
+}
+
+// Boundary values (boundaryConditionValue.hpp:33-276 surface, 278-511 cell;
+// slipFunction.cpp:15-35).  comp: -1 for a scalar, else the velocity
+// component, ax the axis of the face.  slip reflects only the normal
+// velocity component; for scalars slip is the identity.
+template <class T>
+__device__ __forceinline__ T bc_surface(int type, T val, T q, int comp, int ax)
+{
+    switch (type) {
+    case BC_SLIP:
+        return (comp == ax) ? q - q : q;  // v - (v & N) N
+    case BC_FIXED_CELL:
+        return (q + val) * T(0.5);
+    case BC_FIXED_SURF:
+        return val;
+    default:
+        return q;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T bc_cell(int type, T val, T q, int comp, int ax)
+{
+    switch (type) {
+    case BC_SLIP:
+        return (comp == ax) ? (q - q) * T(2) - q : q * T(2) - q;
+    case BC_FIXED_CELL:
+        return val;
+    default:
+        return q;
+    }
+}
+
+}  // namespace sg

@@ -1,0 +1,676 @@
+// fv_stage.hpp -- the fused forward-Euler stage kernel (one Advection2dOrder
+// call, Schemi/src/Advections/Advection2dOrder.cpp:16-636) and its helpers.
+//
+// One workgroup owns a TX x TY tile of columns and marches along z over a
+// chunk of planes (2.5-D blocking).  For every plane it
+//   1. stages the primitives of the plane (tile + 2-cell halo in x and y) in
+//      LDS; the own column's primitives at k-2..k+2 stay in registers,
+//   2. computes the limited slopes along x and y of the tile cells and their
+//      face neighbours (Green-Gauss gradient, TVDLimiter ratio, phi) into LDS,
+//      and the z slope of the own column in registers,
+//   3. reconstructs both sides of every x, y and z face once, closes them with
+//      the EOS and evaluates the HLLC/HLL flux (LDS for x/y faces, registers
+//      for the z faces),
+//   4. sums the six face contributions of every cell in the reference's order
+//      [x-, x+, y+, y-, z-, z+] (divergence.hpp:315-350 over surfacesOfCells,
+//      mesh.cpp:1809-1822) and updates the conserved state.  Each face is
+//      written by one thread and read by its two cells: no atomics, the result
+//      is deterministic.
+// In stage 2 the epilogue also forms the SSP-RK2 average with U^n
+// (bunchOfFields.hpp:166-203) and reduces the Courant signal speed of the new
+// state (Schemi.cpp:807-833).
+//
+// Slopes along one axis only use the 1-D stencil along that axis: the
+// reference's gradient components along an axis come only from that axis's
+// faces (the normals are exactly axis-aligned), its limiter ratio along an
+// axis only from the neighbours along it plus exact zero entries from the
+// other neighbours (their centres share the coordinate), and the transverse
+// velocity slopes have a limiter ratio with an exact zero entry or a zero
+// gradient, so phi (r) G = 0 for them (TVD limiters).  The face-to-centre
+// offsets across an axis are exactly zero on the meshes the host accepts.
+#pragma once
+
+#include "fv_device.hpp"
+
+namespace sg {
+
+template <class T> __device__ __forceinline__ T to_T(double x) { return T(x); }
+
+// Axis geometry read from the per-axis tables.
+struct AxisGeo {
+    const double *xc, *e, *xf, *wo, *wn, *dn, *dlo, *dhi;
+};
+
+__device__ __forceinline__ AxisGeo axis_geo(const DevParams &P, const double *geo, int a)
+{
+    AxisGeo g;
+    g.xc = geo + P.off_xc[a];
+    g.e = geo + P.off_e[a];
+    g.xf = geo + P.off_xf[a];
+    g.wo = geo + P.off_wo[a];
+    g.wn = geo + P.off_wn[a];
+    g.dn = geo + P.off_dn[a];
+    g.dlo = geo + P.off_dlo[a];
+    g.dhi = geo + P.off_dhi[a];
+    return g;
+}
+
+// Number of slope components stored per cell and axis.  TVD limiters: the
+// species, the pressure and the velocity component along the axis; the linear
+// limiter also keeps the transverse velocity components.
+template <int NS, bool LIN> struct SlopeLayout {
+    static constexpr int NQ = NS + 4;
+    static constexpr int NL = LIN ? NQ : NS + 2;
+    // primitive index of slope entry s along axis a
+    __device__ static int prim(int s, int a)
+    {
+        if constexpr (LIN) {
+            (void)a;
+            return s;
+        } else {
+            return s < NS ? s : (s == NS ? NS + 3 : NS + a);
+        }
+    }
+    // velocity component of slope entry s (-1 for scalars)
+    __device__ static int comp(int s, int a)
+    {
+        const int q = prim(s, a);
+        return (q >= NS && q < NS + 3) ? q - NS : -1;
+    }
+};
+
+// Boundary condition lookup for primitive q on patch p.
+template <int NS>
+__device__ __forceinline__ void bc_of(const DevParams &P, int p, int q, int &type, double &val)
+{
+    if (q < NS) {
+        type = P.bct[p][q];
+        val = P.bcv[p][q][0];
+    } else if (q < NS + 3) {
+        type = P.bct[p][NS];
+        val = P.bcv[p][NS][q - NS];
+    } else {
+        type = P.bct[p][NS + 1];
+        val = P.bcv[p][NS + 1][0];
+    }
+}
+
+// Limited slopes along axis a of the cell with index i along that axis
+// (Green-Gauss gradient, gradient.hpp:19-58 + linearInterpolate.hpp:214-251;
+// TVDLimiter.cpp:12-125 and 127-273; limiter phi).  wm/w0/wp: primitives at
+// i-1, i, i+1 (wm/wp unused at a physical boundary).  S: area of the cell's
+// a-faces, V: volume.
+template <class T, int NS, bool LIN>
+__device__ __forceinline__ void slopes_axis(const DevParams &P, const AxisGeo &G, int a,
+                                            int i, const T *wm, const T *w0,
+                                            const T *wp, T S, T V, T *L)
+{
+    using SL = SlopeLayout<NS, LIN>;
+    const int Na = P.N[a];
+    const bool per = P.periodic[a] != 0;
+    const bool has_lo = (i > 0) || per;
+    const bool has_hi = (i < Na - 1) || per;
+    const int plo = patch_of(a, 0), phi_ = patch_of(a, 1);
+    const T wo_lo = to_T<T>(G.wo[i]), wn_lo = to_T<T>(G.wn[i]);
+    const T wo_hi = to_T<T>(G.wo[i + 1]), wn_hi = to_T<T>(G.wn[i + 1]);
+    const T dn_lo = to_T<T>(G.dn[i]), dn_hi = to_T<T>(G.dn[i + 1]);
+    const T gh_lo = to_T<T>(P.ghost_lo[a]), gh_hi = to_T<T>(P.ghost_hi[a]);
+    const bool zero = P.zero_entry[a] != 0;
+#pragma unroll
+    for (int s = 0; s < SL::NL; ++s) {
+        const int q = SL::prim(s, a);
+        const int comp = SL::comp(s, a);
+        const T q0 = w0[q];
+        int tlo = BC_FREE, thi = BC_FREE;
+        double vlo = 0, vhi = 0;
+        if (!has_lo)
+            bc_of<NS>(P, plo, q, tlo, vlo);
+        if (!has_hi)
+            bc_of<NS>(P, phi_, q, thi, vhi);
+        // face values for the Green-Gauss gradient
+        const T flo = has_lo ? wm[q] * wo_lo + q0 * wn_lo
+                             : bc_surface<T>(tlo, T(vlo), q0, comp, a);
+        const T fhi = has_hi ? q0 * wo_hi + wp[q] * wn_hi
+                             : bc_surface<T>(thi, T(vhi), q0, comp, a);
+        const T Gq = (fhi * S - flo * S) / V;
+        if constexpr (LIN) {
+            L[s] = Gq;
+        } else {
+            // limiter entries (d-hat = -1 below, +1 above the cell)
+            const T elo = has_lo ? -((wm[q] - q0) / dn_lo)
+                                 : -((bc_cell<T>(tlo, T(vlo), q0, comp, a) - q0) / gh_lo);
+            const T ehi = has_hi ? (wp[q] - q0) / dn_hi
+                                 : (bc_cell<T>(thi, T(vhi), q0, comp, a) - q0) / gh_hi;
+            T mn = tmin(elo, ehi), mx = tmax(elo, ehi);
+            if (zero) {
+                mn = tmin(mn, T(0));
+                mx = tmax(mx, T(0));
+            }
+            const T r = mn / (mx + T(STAB));
+            L[s] = phi<T>(P.limiter, r) * Gq;
+        }
+    }
+}
+
+// Reconstructed face value of primitive q from a cell with slopes L along
+// axis a and face offset d (Advection2dOrder.cpp:110-137).
+template <class T, int NS, bool LIN>
+__device__ __forceinline__ void reconstruct(const T *w, const T *L, int a, T d, T *c, T *v, T &p)
+{
+    using SL = SlopeLayout<NS, LIN>;
+    T f[NS + 4];
+#pragma unroll
+    for (int q = 0; q < NS + 4; ++q)
+        f[q] = w[q];
+#pragma unroll
+    for (int s = 0; s < SL::NL; ++s) {
+        const int q = SL::prim(s, a);
+        f[q] = L[s] * d + w[q];
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        c[k] = f[k];
+    v[0] = f[NS];
+    v[1] = f[NS + 1];
+    v[2] = f[NS + 2];
+    p = f[NS + 3];
+}
+
+// Flux of face f (0..Na) along axis a.  Owner/neighbour data: primitives and
+// slopes of cells f-1 (owner, lower) and f (neighbour); at the low boundary
+// the owner is cell 0 (slopes Ln, primitives wn) and the normal points to -a;
+// at the high boundary the owner is cell Na-1 (wo, Lo).
+template <class T, int NS, bool LIN>
+__device__ __forceinline__ bool face_flux(const DevParams &P, const AxisGeo &G, int a, int f,
+                                          const T *wo, const T *Lo, const T *wn,
+                                          const T *Ln, T *g)
+{
+    const int Na = P.N[a];
+    const bool per = P.periodic[a] != 0;
+    T cO[NS], vO[3], pO, cB[NS], vB[3], pB;
+    T sgn = T(1);
+    if (!per && f == 0) {
+        // low boundary face: owner is cell 0, ghost from the owner face value
+        reconstruct<T, NS, LIN>(wn, Ln, a, to_T<T>(G.dlo[0]), cO, vO, pO);
+        const int p = patch_of(a, 0);
+        int t;
+        double val;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            bc_of<NS>(P, p, k, t, val);
+            cB[k] = bc_cell<T>(t, T(val), cO[k], -1, a);
+        }
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            bc_of<NS>(P, p, NS + b, t, val);
+            vB[b] = bc_cell<T>(t, T(val), vO[b], b, a);
+        }
+        bc_of<NS>(P, p, NS + 3, t, val);
+        pB = bc_cell<T>(t, T(val), pO, -1, a);
+        sgn = T(-1);
+    } else if (!per && f == Na) {
+        reconstruct<T, NS, LIN>(wo, Lo, a, to_T<T>(G.dhi[Na - 1]), cO, vO, pO);
+        const int p = patch_of(a, 1);
+        int t;
+        double val;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            bc_of<NS>(P, p, k, t, val);
+            cB[k] = bc_cell<T>(t, T(val), cO[k], -1, a);
+        }
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            bc_of<NS>(P, p, NS + b, t, val);
+            vB[b] = bc_cell<T>(t, T(val), vO[b], b, a);
+        }
+        bc_of<NS>(P, p, NS + 3, t, val);
+        pB = bc_cell<T>(t, T(val), pO, -1, a);
+    } else {
+        const int io = (f == 0) ? Na - 1 : f - 1;  // periodic wrap
+        const int in = (f == Na) ? 0 : f;
+        reconstruct<T, NS, LIN>(wo, Lo, a, to_T<T>(G.dhi[io]), cO, vO, pO);
+        reconstruct<T, NS, LIN>(wn, Ln, a, to_T<T>(G.dlo[in]), cB, vB, pB);
+    }
+    Side<T, NS> O, B;
+    close_side<T, NS>(P, cO, vO, pO, O);
+    close_side<T, NS>(P, cB, vB, pB, B);
+    if (P.flux == FLUX_HLL)
+        return hll_flux<T, NS>(O, B, a, sgn, g);
+    return hllc_flux<T, NS>(O, B, a, sgn, g);
+}
+
+// Geometry of one cell: volume (ex ey) ez (mesh.cpp:1291-1293) and face areas.
+struct CellGeo {
+    double V, S[3], minlen;
+};
+
+__device__ __forceinline__ CellGeo cell_geo(const DevParams &P, const double *geo, int i, int j, int k)
+{
+    const double ex = geo[P.off_e[0] + i];
+    const double ey = geo[P.off_e[1] + j];
+    const double ez = geo[P.off_e[2] + k];
+    CellGeo c;
+    c.V = ex * ey * ez;
+    c.S[0] = ey * ez;  // |rX0 - r00| |r0Y - r00| of an x-face
+    c.S[1] = ex * ez;
+    c.S[2] = ex * ey;
+    c.minlen = 1. / tmin(tmin(ex, ey), ez);  // Schemi.cpp:649-662
+    return c;
+}
+
+template <int NS> __device__ __forceinline__ void load_cons(const double *U, long long n, long long id, double *u)
+{
+#pragma unroll
+    for (int q = 0; q < NS + 5; ++q)
+        u[q] = U[q * n + id];
+}
+
+// Courant signal speed (|v| + c) / min edge of one cell (Schemi.cpp:810-828).
+template <int NS>
+__device__ __forceinline__ double signal_speed(const DevParams &P, const double *u, double minlen)
+{
+    double w[NS + 4];
+    prims_from_cons<double, NS>(P, u, w);
+    double c0 = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        c0 += w[k];
+    const double vx = w[NS], vy = w[NS + 1], vz = w[NS + 2], p = w[NS + 3];
+    const double v2 = ((0. + vx * vx) + vy * vy) + vz * vz;
+    const double U = (u[NS + 4] - u[NS] * v2 * 0.5) - 0.;
+    const double c2 = 0. + gamma1<double, NS>(P, w, c0) * (U + p) / u[NS];
+    const double cs = __builtin_sqrt(c2);
+    return (hypot3(vx, vy, vz) + cs) * minlen;
+}
+
+// Index helper for the x-y plane tile in LDS with a 2-cell halo.
+template <int TX, int TY, int DIM> struct Tile {
+    static constexpr int BX = TX + 4;
+    static constexpr int BY = (DIM >= 2) ? TY + 4 : 1;
+    static constexpr int OY = (DIM >= 2) ? 2 : 0;
+};
+
+template <class T, int NS, int DIM, bool LIN, bool STAGE2, bool MIXED, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY)
+stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
+             const double *__restrict__ Uin, double *Uout, const double *Un,
+             const double *__restrict__ dt_ptr, unsigned long long *sigmax, int *errflag,
+             int kchunk)
+{
+    constexpr int NQ = NS + 4;
+    constexpr int NF = NS + 5;
+    using SL = SlopeLayout<NS, LIN>;
+    constexpr int NL = SL::NL;
+    constexpr int NT = TX * TY;
+    using TL = Tile<TX, TY, DIM>;
+    constexpr int BX = TL::BX, BY = TL::BY, OY = TL::OY;
+    constexpr int SXW = TX + 2;
+    constexpr int SYH = (DIM >= 2) ? TY + 2 : 1;
+    constexpr int FYH = (DIM >= 2) ? TY + 1 : 1;
+
+    __shared__ T sW[NQ][BY][BX];
+    __shared__ T sSx[NL][TY][SXW];
+    __shared__ T sSy[(DIM >= 2) ? NL : 1][SYH][TX];
+    __shared__ T sFx[NF][TY][TX + 1];
+    __shared__ T sFy[(DIM >= 2) ? NF : 1][FYH][TX];
+    __shared__ double sRed[NT / 64];
+
+    const DevParams &P = *Pp;
+    // dt_ptr points at the context's scalar block; entry 6 flags that the
+    // run has reached its end time, so queued steps do nothing
+    if (dt_ptr[6] != 0.0)
+        return;
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = tid / TX;
+    const int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
+    const long long n = P.n;
+    const long long sy = P.stride[1], sz = P.stride[2];
+    const int x0 = blockIdx.x * TX;
+    const int y0 = (DIM >= 2) ? blockIdx.y * TY : 0;
+    const int kb = (DIM == 3) ? blockIdx.z * kchunk : 0;
+    const int ke = (DIM == 3) ? min(kb + kchunk, Nz) : 1;
+    const int i = x0 + tx, j = y0 + ty;
+    const bool act = (i < Nx) && (j < Ny);
+    const double dt = *dt_ptr;
+    const AxisGeo GX = axis_geo(P, geo, 0);
+    const AxisGeo GY = axis_geo(P, geo, 1);
+    const AxisGeo GZ = axis_geo(P, geo, 2);
+    int err = 0;
+    double sig = 0.0;
+
+    auto wrapx = [&](int x) { return (x < 0) ? x + Nx : (x >= Nx ? x - Nx : x); };
+    auto wrapy = [&](int y) { return (y < 0) ? y + Ny : (y >= Ny ? y - Ny : y); };
+    auto wrapz = [&](int z) { return (z < 0) ? z + Nz : (z >= Nz ? z - Nz : z); };
+    auto prims_at = [&](long long id, T *w) {
+        double u[NF];
+        load_cons<NS>(Uin, n, id, u);
+        if constexpr (MIXED) {
+            T ut[NF];
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                ut[q] = T(u[q]);
+            prims_from_cons<T, NS>(P, ut, w);
+        } else {
+            prims_from_cons<T, NS>(P, u, w);
+        }
+    };
+
+    // ---- z-direction register window of the own column (DIM == 3) --------
+    T wz[5][NQ];   // primitives at k-2 .. k+2
+    T lz0[NL], lzp[NL];
+    T fzm[NF], fzp[NF];
+    const bool zper = (DIM == 3) && P.periodic[2] != 0;
+    auto zvalid = [&](int z) { return (z >= 0 && z < Nz) || zper; };
+
+    if constexpr (DIM == 3) {
+        if (act) {
+            for (int d = 0; d < 4; ++d) {
+                const int z = kb - 2 + d;
+                if (zvalid(z))
+                    prims_at(i + sy * j + sz * wrapz(z), wz[d]);
+            }
+            const CellGeo cg0 = cell_geo(P, geo, i, j, kb);
+            T lzm[NL];
+            const int km = kb - 1;
+            if (zvalid(km)) {
+                const int kmw = wrapz(km);
+                const CellGeo cgm = cell_geo(P, geo, i, j, kmw);
+                slopes_axis<T, NS, LIN>(P, GZ, 2, kmw, wz[0], wz[1], wz[2], T(cgm.S[2]),
+                                        T(cgm.V), lzm);
+            }
+            slopes_axis<T, NS, LIN>(P, GZ, 2, kb, wz[1], wz[2], wz[3], T(cg0.S[2]), T(cg0.V), lz0);
+            // z- face of plane kb
+            if (!face_flux<T, NS, LIN>(P, GZ, 2, kb, wz[1], lzm, wz[2], lz0, fzm))
+                err |= ERRF_RIEMANN;
+        }
+    }
+
+    for (int k = kb; k < ke; ++k) {
+        // ---- 1. primitives of plane k (and k+2 for the own column) ---------
+        if constexpr (DIM == 3) {
+            if (act) {
+                if (zvalid(k + 2))
+                    prims_at(i + sy * j + sz * wrapz(k + 2), wz[4]);
+                // z slope at k+1 and z+ face of plane k
+                if (zvalid(k + 1)) {
+                    const int kp = wrapz(k + 1);
+                    const CellGeo cgp = cell_geo(P, geo, i, j, kp);
+                    slopes_axis<T, NS, LIN>(P, GZ, 2, kp, wz[2], wz[3], wz[4], T(cgp.S[2]),
+                                            T(cgp.V), lzp);
+                }
+                if (!face_flux<T, NS, LIN>(P, GZ, 2, k + 1, wz[2], lz0, wz[3], lzp, fzp))
+                    err |= ERRF_RIEMANN;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+                    sW[q][ty + OY][tx + 2] = wz[2][q];
+            }
+        } else {
+            if (act) {
+                T w[NQ];
+                prims_at(i + sy * j, w);
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+                    sW[q][ty + OY][tx + 2] = w[q];
+            }
+        }
+        // halo cells of the plane
+        {
+            constexpr int NHX = 4 * TY;
+            constexpr int NH = NHX + ((DIM >= 2) ? 4 * TX : 0);
+            for (int h = tid; h < NH; h += NT) {
+                int bxc, byc;
+                if (h < NHX) {
+                    byc = h / 4 + OY;
+                    const int c = h % 4;
+                    bxc = (c < 2) ? c : TX + c;
+                } else {
+                    const int hh = h - NHX;
+                    bxc = hh % TX + 2;
+                    const int r = hh / TX;
+                    byc = (r < 2) ? r : TY + r;
+                }
+                int gx = x0 - 2 + bxc;
+                int gy = y0 - OY + byc;
+                bool ok = true;
+                if (gx < 0 || gx >= Nx) {
+                    if (P.periodic[0])
+                        gx = wrapx(gx);
+                    else
+                        ok = false;
+                }
+                if (DIM >= 2 && (gy < 0 || gy >= Ny)) {
+                    if (P.periodic[1])
+                        gy = wrapy(gy);
+                    else
+                        ok = false;
+                }
+                // halo columns of a partial tile that fall inside the domain
+                // are regular cells; rows/columns past the tile edge inside
+                // the domain are covered here too
+                if (ok && (gx >= Nx || (DIM >= 2 && gy >= Ny)))
+                    ok = false;
+                if (ok) {
+                    T w[NQ];
+                    prims_at(gx + sy * gy + (DIM == 3 ? sz * (long long)k : 0), w);
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q)
+                        sW[q][byc][bxc] = w[q];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- 2. slopes along x and y ------------------------------------
+        {
+            constexpr int NTX = SXW * TY;
+            constexpr int NTY = (DIM >= 2) ? TX * SYH : 0;
+            for (int t = tid; t < NTX + NTY; t += NT) {
+                if (t < NTX) {
+                    const int u = t % SXW, r = t / SXW;
+                    int cx = x0 - 1 + u;
+                    const int cy = y0 + r;
+                    if (cy >= Ny)
+                        continue;
+                    if (cx < 0 || cx >= Nx) {
+                        if (!P.periodic[0])
+                            continue;
+                        cx = wrapx(cx);
+                    }
+                    T wm[NQ], w0[NQ], wp[NQ], L[NL];
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        wm[q] = sW[q][r + OY][u];
+                        w0[q] = sW[q][r + OY][u + 1];
+                        wp[q] = sW[q][r + OY][u + 2];
+                    }
+                    const CellGeo cg = cell_geo(P, geo, cx, (DIM >= 2) ? cy : 0, (DIM == 3) ? k : 0);
+                    slopes_axis<T, NS, LIN>(P, GX, 0, cx, wm, w0, wp, T(cg.S[0]), T(cg.V), L);
+#pragma unroll
+                    for (int s = 0; s < NL; ++s)
+                        sSx[s][r][u] = L[s];
+                } else if constexpr (DIM >= 2) {
+                    const int tt = t - NTX;
+                    const int c = tt % TX, v = tt / TX;
+                    const int cx = x0 + c;
+                    int cy = y0 - 1 + v;
+                    if (cx >= Nx)
+                        continue;
+                    if (cy < 0 || cy >= Ny) {
+                        if (!P.periodic[1])
+                            continue;
+                        cy = wrapy(cy);
+                    }
+                    T wm[NQ], w0[NQ], wp[NQ], L[NL];
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        wm[q] = sW[q][v][c + 2];
+                        w0[q] = sW[q][v + 1][c + 2];
+                        wp[q] = sW[q][v + 2][c + 2];
+                    }
+                    const CellGeo cg = cell_geo(P, geo, cx, cy, (DIM == 3) ? k : 0);
+                    slopes_axis<T, NS, LIN>(P, GY, 1, cy, wm, w0, wp, T(cg.S[1]), T(cg.V), L);
+#pragma unroll
+                    for (int s = 0; s < NL; ++s)
+                        sSy[s][v][c] = L[s];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- 3. x and y face fluxes ------------------------------------
+        {
+            constexpr int NTX = (TX + 1) * TY;
+            constexpr int NTY = (DIM >= 2) ? TX * FYH : 0;
+            for (int t = tid; t < NTX + NTY; t += NT) {
+                T gq[NF];
+                if (t < NTX) {
+                    const int u = t % (TX + 1), r = t / (TX + 1);
+                    const int fx = x0 + u;
+                    if (fx > Nx || y0 + r >= Ny)
+                        continue;
+                    T wo[NQ], wn[NQ], Lo[NL], Ln[NL];
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        wo[q] = sW[q][r + OY][u + 1];
+                        wn[q] = sW[q][r + OY][u + 2];
+                    }
+#pragma unroll
+                    for (int s = 0; s < NL; ++s) {
+                        Lo[s] = sSx[s][r][u];
+                        Ln[s] = sSx[s][r][u + 1];
+                    }
+                    if (!face_flux<T, NS, LIN>(P, GX, 0, fx, wo, Lo, wn, Ln, gq))
+                        err |= ERRF_RIEMANN;
+#pragma unroll
+                    for (int q = 0; q < NF; ++q)
+                        sFx[q][r][u] = gq[q];
+                } else if constexpr (DIM >= 2) {
+                    const int tt = t - NTX;
+                    const int c = tt % TX, v = tt / TX;
+                    const int fy = y0 + v;
+                    if (fy > Ny || x0 + c >= Nx)
+                        continue;
+                    T wo[NQ], wn[NQ], Lo[NL], Ln[NL];
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        wo[q] = sW[q][v + 1][c + 2];
+                        wn[q] = sW[q][v + 2][c + 2];
+                    }
+#pragma unroll
+                    for (int s = 0; s < NL; ++s) {
+                        Lo[s] = sSy[s][v][c];
+                        Ln[s] = sSy[s][v + 1][c];
+                    }
+                    if (!face_flux<T, NS, LIN>(P, GY, 1, fy, wo, Lo, wn, Ln, gq))
+                        err |= ERRF_RIEMANN;
+#pragma unroll
+                    for (int q = 0; q < NF; ++q)
+                        sFy[q][v][c] = gq[q];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- 4. residual and update of the own cell ---------------------
+        if (act) {
+            const int kk = (DIM == 3) ? k : 0;
+            const long long id = i + sy * j + sz * kk;
+            const CellGeo cg = cell_geo(P, geo, i, (DIM >= 2) ? j : 0, kk);
+            const bool xlo = (i == 0) && !P.periodic[0];
+            const bool ylo = (DIM >= 2) && (j == 0) && !P.periodic[1];
+            const bool zlo = (DIM == 3) && (k == 0) && !zper;
+            double u[NF];
+            load_cons<NS>(Uin, n, id, u);
+            double unew[NF];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) {
+                // contributions (s g) S in the order x-, x+, y+, y-, z-, z+
+                double acc = 0;
+                acc += (xlo ? double(sFx[q][ty][tx]) : -double(sFx[q][ty][tx])) * cg.S[0];
+                acc += double(sFx[q][ty][tx + 1]) * cg.S[0];
+                if constexpr (DIM >= 2) {
+                    acc += double(sFy[q][ty + 1][tx]) * cg.S[1];
+                    acc += (ylo ? double(sFy[q][ty][tx]) : -double(sFy[q][ty][tx])) * cg.S[1];
+                }
+                if constexpr (DIM == 3) {
+                    acc += (zlo ? double(fzm[q]) : -double(fzm[q])) * cg.S[2];
+                    acc += double(fzp[q]) * cg.S[2];
+                }
+                if constexpr (MIXED)
+                    unew[q] = u[q] - (acc / cg.V) * dt;
+                else
+                    unew[q] = u[q] - (acc / cg.V) * dt;
+            }
+            if (P.gravity) {
+                // pre-update rho and v (Advection2dOrder.cpp:547-554)
+                const double rho = u[NS];
+                const double vx = u[NS + 1] / rho, vy = u[NS + 2] / rho, vz = u[NS + 3] / rho;
+                const double vg = ((0. + vx * P.g[0]) + vy * P.g[1]) + vz * P.g[2];
+                unew[NS + 4] = unew[NS + 4] + rho * vg * dt;
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+                    unew[NS + 1 + b] = unew[NS + 1 + b] + (P.g[b] * rho) * dt;
+            }
+            if constexpr (MIXED) {
+                // species renormalisation: sum_k rho_k = rho
+                double sr = 0;
+#pragma unroll
+                for (int q = 0; q < NS; ++q)
+                    sr += unew[q];
+                const double f = unew[NS] / sr;
+#pragma unroll
+                for (int q = 0; q < NS; ++q)
+                    unew[q] *= f;
+            }
+            if constexpr (STAGE2) {
+                double un[NF];
+                load_cons<NS>(Un, n, id, un);
+#pragma unroll
+                for (int q = 0; q < NF; ++q) {
+                    if (q >= NS + 1 && q <= NS + 3)
+                        unew[q] = unew[q] * 0.5 + un[q] * 0.5;
+                    else
+                        unew[q] = 0.5 * unew[q] + 0.5 * un[q];
+                }
+                sig = tmax(sig, signal_speed<NS>(P, unew, cg.minlen));
+            }
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                Uout[q * n + id] = unew[q];
+        }
+
+        if constexpr (DIM == 3) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d)
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+                    wz[d][q] = wz[d + 1][q];
+#pragma unroll
+            for (int s = 0; s < NL; ++s)
+                lz0[s] = lzp[s];
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                fzm[q] = fzp[q];
+        }
+    }
+
+    if (err)
+        atomicOr(errflag, err);
+    if constexpr (STAGE2) {
+        // wavefront max of the Courant signal speed, then one atomic per block
+        for (int off = 32; off > 0; off >>= 1)
+            sig = tmax(sig, __shfl_xor(sig, off, 64));
+        if ((tid & 63) == 0)
+            sRed[tid >> 6] = sig;
+        __syncthreads();
+        if (tid == 0) {
+            double m = sRed[0];
+            for (int w = 1; w < NT / 64; ++w)
+                m = tmax(m, sRed[w]);
+            // positive doubles order like their bit patterns
+            atomicMax(sigmax, (unsigned long long)__double_as_longlong(m));
+        }
+    }
+}
+
+}  // namespace sg

@@ -1,0 +1,332 @@
+"""ctypes binding of the C ABI in ``include/schemi_gpu.h``.
+
+This is the Python side of the drop-in boundary: every call goes through
+``schemi_amd/libschemi_gpu.so``.  There is no CPU fallback: when the library
+is missing or no HIP device is present, constructing a solver raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .case import Case
+from .enums import boundaryConditionType as BCT
+from .enums import errors
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libschemi_gpu.so")
+MAXS = 9
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+
+class SchemiGpuError(RuntimeError):
+    """Raised with the reference's error code (errorsEnum.hpp:15-30)."""
+
+    def __init__(self, code: int, where: str = ""):
+        self.code = int(code)
+        msg = _lib().schemi_gpu_strerror(self.code).decode()
+        super().__init__(f"{where}: {msg} (errors::{_name(self.code)})")
+
+
+def _name(code):
+    try:
+        return errors(code).name
+    except ValueError:
+        return str(code)
+
+
+class _BC(C.Structure):
+    _fields_ = [("type", C.c_int32), ("value", C.c_double * 3)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("n", C.c_int64 * 3), ("extent", C.c_double * 3),
+                ("origin", C.c_double * 3)]
+
+
+class MixDesc(C.Structure):
+    _fields_ = [("ns", C.c_int32), ("M", C.c_double * MAXS), ("Cv", C.c_double * MAXS),
+                ("R", C.c_double), ("gas_model", C.c_int32)]
+
+
+class SchemeDesc(C.Structure):
+    _fields_ = [("flux", C.c_int32), ("limiter", C.c_int32), ("precision", C.c_int32),
+                ("gravity", C.c_int32), ("g", C.c_double * 3), ("courant", C.c_double),
+                ("min_time", C.c_double), ("const_dt", C.c_double),
+                ("bc_c", (_BC * 6) * MAXS), ("bc_v", _BC * 6), ("bc_p", _BC * 6)]
+
+
+_LIB = None
+_SYMBOLS = ("schemi_gpu_create", "schemi_gpu_destroy", "schemi_gpu_set_state",
+            "schemi_gpu_get_state", "schemi_gpu_step", "schemi_gpu_run",
+            "schemi_gpu_stage", "schemi_gpu_face_flux", "schemi_gpu_device_state",
+            "schemi_gpu_sync", "schemi_gpu_timing", "schemi_gpu_strerror",
+            "schemi_gpu_face_permutation", "schemi_gpu_axis_table")
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "schemi_amd/libschemi_gpu.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, lp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)
+        L.schemi_gpu_create.restype = C.c_int
+        L.schemi_gpu_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(MixDesc),
+                                        C.POINTER(SchemeDesc), C.POINTER(vp)]
+        L.schemi_gpu_destroy.restype = None
+        L.schemi_gpu_destroy.argtypes = [vp]
+        L.schemi_gpu_set_state.restype = C.c_int
+        L.schemi_gpu_set_state.argtypes = [vp, dp, C.c_int64, C.c_int32]
+        L.schemi_gpu_get_state.restype = C.c_int
+        L.schemi_gpu_get_state.argtypes = [vp, dp, dp, C.c_int32]
+        L.schemi_gpu_step.restype = C.c_int
+        L.schemi_gpu_step.argtypes = [vp, C.c_double, dp]
+        L.schemi_gpu_run.restype = C.c_int
+        L.schemi_gpu_run.argtypes = [vp, C.c_int64, C.c_double, C.c_double, dp, lp, dp]
+        L.schemi_gpu_stage.restype = C.c_int
+        L.schemi_gpu_stage.argtypes = [vp, C.c_double]
+        L.schemi_gpu_face_flux.restype = C.c_int
+        L.schemi_gpu_face_flux.argtypes = [vp, C.c_int64, C.POINTER(C.c_int32), dp, dp, dp,
+                                           dp]
+        L.schemi_gpu_device_state.restype = C.c_int
+        L.schemi_gpu_device_state.argtypes = [vp, C.POINTER(dp), lp]
+        L.schemi_gpu_sync.restype = C.c_int
+        L.schemi_gpu_sync.argtypes = [vp]
+        L.schemi_gpu_timing.restype = C.c_int
+        L.schemi_gpu_timing.argtypes = [vp, C.c_int32, dp, lp]
+        L.schemi_gpu_strerror.restype = C.c_char_p
+        L.schemi_gpu_strerror.argtypes = [C.c_int]
+        L.schemi_gpu_face_permutation.restype = C.c_int
+        L.schemi_gpu_face_permutation.argtypes = [C.POINTER(MeshDesc), lp, C.c_int64]
+        L.schemi_gpu_axis_table.restype = C.c_int
+        L.schemi_gpu_axis_table.argtypes = [C.POINTER(MeshDesc), C.c_int32, C.c_int32,
+                                            C.c_int32, dp, lp]
+        _LIB = L
+    return _LIB
+
+
+def library():
+    """The loaded C-ABI library (raises when it is not built)."""
+    return _lib()
+
+
+def exported_symbols():
+    return _SYMBOLS
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def mesh_desc(case: Case) -> MeshDesc:
+    m = MeshDesc()
+    m.dim = case.dim
+    for a in range(3):
+        m.n[a] = int(case.N[a])
+        m.extent[a] = float(case.L[a])
+        m.origin[a] = float(case.origin[a])
+    return m
+
+
+def mix_desc(case: Case) -> MixDesc:
+    x = MixDesc()
+    x.ns = case.Ns
+    for k in range(case.Ns):
+        x.M[k] = float(case.M[k])
+        x.Cv[k] = float(case.Cv[k])
+    x.R = float(case.R)
+    x.gas_model = 0
+    return x
+
+
+def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) -> SchemeDesc:
+    s = SchemeDesc()
+    s.flux = int(case.flux)
+    s.limiter = int(case.limiter)
+    s.precision = int(precision)
+    s.gravity = 1 if case.gravity else 0
+    for a in range(3):
+        s.g[a] = float(case.g[a])
+    s.courant = float(case.courant)
+    s.min_time = float(case.min_time)
+    s.const_dt = float(const_dt)
+    for p in range(6):
+        for k in range(case.Ns):
+            s.bc_c[k][p].type = int(case.bc_c[k][p].type)
+            for a in range(3):
+                s.bc_c[k][p].value[a] = float(case.bc_c[k][p].value[a])
+        s.bc_v[p].type = int(case.bc_v[p].type)
+        s.bc_p[p].type = int(case.bc_p[p].type)
+        for a in range(3):
+            s.bc_v[p].value[a] = float(case.bc_v[p].value[a])
+            s.bc_p[p].value[a] = float(case.bc_p[p].value[a])
+    return s
+
+
+def face_permutation(case: Case) -> np.ndarray:
+    """Reference face id -> structured (axis-major) face id."""
+    nf = _nfaces(case)
+    out = np.zeros(nf, dtype=np.int64)
+    m = mesh_desc(case)
+    r = _lib().schemi_gpu_face_permutation(C.byref(m),
+                                           out.ctypes.data_as(C.POINTER(C.c_int64)), nf)
+    if r:
+        raise SchemiGpuError(r, "face_permutation")
+    return out
+
+
+TABLES = ("xc", "e", "xf", "wo", "wn", "dn", "dlo", "dhi", "ghost", "separable")
+
+
+def axis_table(case: Case, axis: int, which: str, periodic_mask: int = 0) -> np.ndarray:
+    """One per-axis geometry table of ``case``'s mesh (host computation)."""
+    m = mesh_desc(case)
+    w = TABLES.index(which)
+    n = C.c_int64()
+    r = _lib().schemi_gpu_axis_table(C.byref(m), periodic_mask, axis, w, None, C.byref(n))
+    if r:
+        raise SchemiGpuError(r, "axis_table")
+    out = np.zeros(n.value)
+    r = _lib().schemi_gpu_axis_table(C.byref(m), periodic_mask, axis, w, _dp(out),
+                                     C.byref(n))
+    if r:
+        raise SchemiGpuError(r, "axis_table")
+    return out
+
+
+def _nfaces(case: Case) -> int:
+    Nx, Ny, Nz = case.N
+    if case.dim == 1:
+        return Nx + 1
+    if case.dim == 2:
+        return Ny * (Nx + 1) + Nx * (Ny + 1)
+    return Ny * Nz * (Nx + 1) + Nx * Nz * (Ny + 1) + Nx * Ny * (Nz + 1)
+
+
+class GpuSolver:
+    """One device context: the SSP-RK2 step of ``case`` on the current GPU."""
+
+    def __init__(self, case: Case, precision: int = PREC_FP64, const_dt: float = -1.0):
+        self.case = case
+        self.precision = precision
+        self._m = mesh_desc(case)
+        self._x = mix_desc(case)
+        self._s = scheme_desc(case, precision, const_dt)
+        self._ctx = C.c_void_p()
+        r = _lib().schemi_gpu_create(C.byref(self._m), C.byref(self._x), C.byref(self._s),
+                                     C.byref(self._ctx))
+        if r:
+            self._ctx = None
+            raise SchemiGpuError(r, "schemi_gpu_create")
+        self.n = case.ncells
+        self.nf = case.Ns + 5
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            _lib().schemi_gpu_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, r, where):
+        if r:
+            raise SchemiGpuError(r, where)
+
+    def set_conserved(self, U: np.ndarray):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nf, self.n)
+        self._check(_lib().schemi_gpu_set_state(self._ctx, _dp(U), self.n, 0), "set_state")
+
+    def conserved(self) -> np.ndarray:
+        U = np.zeros((self.nf, self.n))
+        self._check(_lib().schemi_gpu_get_state(self._ctx, _dp(U), None, 0), "get_state")
+        return U
+
+    def primitives(self) -> dict:
+        Ns, n = self.case.Ns, self.n
+        W = np.zeros((Ns + 8, n))
+        self._check(_lib().schemi_gpu_get_state(self._ctx, None, _dp(W), 0), "get_state")
+        return {"c": W[:Ns], "c0": W[Ns], "v": W[Ns + 1:Ns + 4], "p": W[Ns + 4],
+                "U": W[Ns + 5], "T": W[Ns + 6], "cs": W[Ns + 7]}
+
+    def step(self, dt: float) -> float:
+        nxt = C.c_double(0.0)
+        self._check(_lib().schemi_gpu_step(self._ctx, float(dt), C.byref(nxt)), "step")
+        return nxt.value
+
+    def stage(self, dt: float):
+        self._check(_lib().schemi_gpu_stage(self._ctx, float(dt)), "stage")
+
+    def run(self, nsteps: int, dt0: float, t_end: float = float("inf")):
+        t, s, d = C.c_double(), C.c_int64(), C.c_double()
+        self._check(_lib().schemi_gpu_run(self._ctx, int(nsteps), float(dt0), float(t_end),
+                                          C.byref(t), C.byref(s), C.byref(d)), "run")
+        return t.value, s.value, d.value
+
+    def face_flux(self, axis, sgn, left, right) -> np.ndarray:
+        axis = np.ascontiguousarray(axis, dtype=np.int32)
+        sgn = np.ascontiguousarray(sgn, dtype=np.float64)
+        left = np.ascontiguousarray(left, dtype=np.float64)
+        right = np.ascontiguousarray(right, dtype=np.float64)
+        nf = axis.shape[0]
+        F = np.zeros((nf, self.case.Ns + 5))
+        self._check(_lib().schemi_gpu_face_flux(
+            self._ctx, nf, axis.ctypes.data_as(C.POINTER(C.c_int32)), _dp(sgn), _dp(left),
+            _dp(right), _dp(F)), "face_flux")
+        return F
+
+    def device_state(self):
+        p = C.POINTER(C.c_double)()
+        n = C.c_int64()
+        self._check(_lib().schemi_gpu_device_state(self._ctx, C.byref(p), C.byref(n)),
+                    "device_state")
+        return C.cast(p, C.c_void_p).value, n.value
+
+    def sync(self):
+        self._check(_lib().schemi_gpu_sync(self._ctx), "sync")
+
+    def timing(self, enable: bool):
+        avg, cnt = C.c_double(), C.c_int64()
+        self._check(_lib().schemi_gpu_timing(self._ctx, 1 if enable else 0, C.byref(avg),
+                                             C.byref(cnt)), "timing")
+        return avg.value, cnt.value
+
+
+def conserved_from_primitive(case: Case, c, v, p) -> np.ndarray:
+    """Conserved vector from primitives (c_k, v, p), with the reference's
+    initialisation arithmetic (phaseInitialization.cpp:1827-1883).  Pure host
+    arithmetic on numpy arrays; used to prepare initial states."""
+    c = np.asarray(c, dtype=np.float64)
+    v = np.asarray(v, dtype=np.float64)
+    p = np.asarray(p, dtype=np.float64)
+    Ns = case.Ns
+    n = c.shape[1]
+    c0 = np.zeros(n)
+    for k in range(Ns):
+        c0 = c0 + c[k]
+    rho_k = np.stack([c[k] * case.M[k] for k in range(Ns)])
+    rho = np.zeros(n)
+    for k in range(Ns):
+        rho = rho + rho_k[k]
+    m = v * rho
+    cv = np.zeros(n)
+    for k in range(Ns):
+        cv = cv + (c[k] / c0) * case.Cv[k]
+    U = p / (case.R / cv)
+    v2 = ((0.0 + v[0] * v[0]) + v[1] * v[1]) + v[2] * v[2]
+    E = (U + rho * v2 * 0.5) + 0.0
+    return np.concatenate([rho_k, rho[None], m, E[None]])
+
+
+def periodic(case: Case, axis: int):
+    """Marks both patches of ``axis`` periodic for every field."""
+    lo, hi = [(0, 1), (3, 4), (2, 5)][axis]
+    for p in (lo, hi):
+        case.set_bc(p, BCT.periodic)
+    return case

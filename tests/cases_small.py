@@ -1,0 +1,51 @@
+"""Small parity cases shared by the CPU and GPU tests (seeded, deterministic)."""
+import numpy as np
+
+from schemi_amd import Case, typeOfTVDLimiter as TL, flowSolver as FS
+from schemi_amd import boundaryConditionType as BCT
+
+BASE = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
+
+
+def random_primitives(case, seed, amp_v=0.2):
+    r = np.random.default_rng(seed)
+    n = case.ncells
+    c = np.stack([0.02 + 0.03 * r.random(n) for _ in range(case.Ns)])
+    v = amp_v * (r.random((3, n)) - 0.5)
+    if case.dim < 3:
+        v[2] = 0
+    if case.dim < 2:
+        v[1] = 0
+    p = 0.5 + r.random(n)
+    return c, v, p
+
+
+def cases():
+    out = {}
+    for lim in (TL.minmod, TL.vanLeer, TL.vanAlbada, TL.linear, TL.zero):
+        out[f"1d_{lim.name}"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), limiter=lim, **BASE)
+    out["1d_hll"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), flux=FS.HLL, **BASE)
+    c = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), **BASE)
+    c.set_bc(0, BCT.slip)
+    c.set_bc(1, BCT.slip)
+    out["1d_slip"] = c
+    c = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), **BASE)
+    c.set_bc(0, BCT.fixedValueCell, "p", (1.2, 0, 0))
+    c.set_bc(0, BCT.fixedValueCell, "v", (0.1, 0.0, 0.0))
+    out["1d_fixed"] = c
+    for lim in (TL.vanLeer, TL.linear):
+        c = Case(dim=2, N=(40, 24, 1), L=(0.25, 1.0, 0.1), limiter=lim, **BASE)
+        c.set_bc(3, BCT.slip)
+        c.set_bc(4, BCT.slip)
+        out[f"2d_{lim.name}"] = c
+    c = Case(dim=2, N=(36, 20, 1), L=(0.3, 0.7, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             gravity=True, g=(0.0, -1.0, 0.0), **BASE)
+    out["2d_hll_grav"] = c
+    for lim in (TL.vanAlbada, TL.linear):
+        c = Case(dim=3, N=(40, 12, 20), L=(2.0, 1.0, 1.0), limiter=lim, **BASE)
+        for pt in (2, 3, 4, 5):
+            c.set_bc(pt, BCT.slip)
+        out[f"3d_{lim.name}"] = c
+    c = Case(dim=3, N=(33, 9, 18), L=(1.0, 0.5, 0.5), limiter=TL.minmod, flux=FS.HLL, **BASE)
+    out["3d_hll_partial_tiles"] = c
+    return out

@@ -1,0 +1,37 @@
+"""Loading of the golden fixtures written by tests/golden/make_golden.py."""
+import glob
+import json
+import os
+
+import numpy as np
+
+from schemi_amd import BC, Case
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KEYS = ("c0", "c", "rho0", "rho", "v", "p", "U", "T", "m", "E")
+
+
+def names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    d = json.loads(str(z["case"]))
+    case = Case(dim=d["dim"], N=tuple(d["N"]), L=tuple(d["L"]), M=d["M"], Cv=d["Cv"],
+                R=d["R"], flux=d["flux"], limiter=d["limiter"], gravity=d["gravity"],
+                g=tuple(d["g"]), courant=d["courant"], min_time=d["min_time"],
+                bc_c=[[BC(t, tuple(v)) for t, v in row] for row in d["bc_c"]],
+                bc_v=[BC(t, tuple(v)) for t, v in d["bc_v"]],
+                bc_p=[BC(t, tuple(v)) for t, v in d["bc_p"]])
+    snaps = {int(s): {k: z[f"s{s}_{k}"] for k in KEYS} for s in z["steps"]}
+    out = {"case": case, "dts": z["dts"], "snaps": snaps}
+    if "times" in z:
+        out["times"] = z["times"]
+    if "init_c" in z:
+        out["restart"] = (z["init_c"], z["init_v"], z["init_p"])
+    return out
+
+
+def conserved(st):
+    return np.concatenate([st["rho"], st["rho0"][None], st["m"], st["E"][None]])

@@ -1,0 +1,242 @@
+"""Benchmark of the explicit SSP-RK2 step on the 3-D Richtmyer-Meshkov case.
+
+Workload: 256 x 128 x 128 = 4,194,304 hexahedral cells on [0,2]x[0,1]x[0,1],
+two species, HLLC flux with the van Albada TVD limiter, Courant time step
+kept on the device.  One step = one SSP-RK2 step (two fused stage kernels +
+the time-step reduction) over all cells; inputs are resident in HBM.
+
+Reported: Mcell-updates/s of the mixed-precision path (FP32 reconstruction,
+limiter and Riemann solve; FP64 state, accumulation and update) next to the
+FP64 path, the density/pressure L1 error of the mixed path against FP64 after
+1000 steps, the HBM roofline fraction of the stage kernel (algorithmic bytes
+per launch over its average HIP-event duration), and the reference solver's
+CPU step timed on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+NS, DIM = 2, 3
+# algorithmic bytes per cell and step: conserved FP64 vector of Ns + 2 + D
+# values, read and written once per stage, plus the U^n read of stage 2
+BYTES_PER_CELL_STEP = 5 * 8 * (NS + 2 + DIM)  # 280
+BYTES_PER_CELL_LAUNCH = BYTES_PER_CELL_STEP / 2  # average of the two stage kernels
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ny", type=int, default=128)
+    ap.add_argument("--nz", type=int, default=128)
+    ap.add_argument("--l1-steps", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def timed_run(solver, U0, steps, warmup):
+    import torch
+
+    solver.set_conserved(U0)
+    t, s, dt = solver.run(warmup, 1e-15)  # first step: #Minimal_timestep
+    solver.timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t1_, s1, dt1 = solver.run(steps, dt)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    avg_ms, launches = solver.timing(False)
+    if s1 != steps:
+        raise RuntimeError(f"ran {s1} steps, expected {steps}")
+    return wall, avg_ms, launches
+
+
+def mixing_width(U, V):
+    rho_h = U[1]
+    y = rho_h / U[2]
+    return float(np.sum(y * (1.0 - y)) * V)
+
+
+def l1_gate(case, U0, nsteps):
+    """Density and pressure L1 of the mixed path against FP64 after nsteps
+    steps with the same constant time step (0.8 x the initial Courant step)."""
+    from schemi_amd.gpu import PREC_FP64, PREC_MIXED, GpuSolver
+
+    probe = GpuSolver(case, PREC_FP64)
+    probe.set_conserved(U0)
+    dtc = probe.step(1e-15)
+    probe.close()
+    dt = 0.8 * dtc
+    out = {}
+    for prec in (PREC_FP64, PREC_MIXED):
+        s = GpuSolver(case, prec, const_dt=dt)
+        s.set_conserved(U0)
+        s.run(nsteps, dt)
+        out[prec] = (s.conserved(), s.primitives()["p"])
+        s.close()
+    (U64, p64), (Umx, pmx) = out[PREC_FP64], out[PREC_MIXED]
+    V = (2.0 / case.N[0]) * (1.0 / case.N[1]) * (1.0 / case.N[2])
+    w64, wmx = mixing_width(U64, V), mixing_width(Umx, V)
+    return {
+        "steps": nsteps, "dt": dt,
+        "rho_l1_rel": float(np.abs(Umx[NS] - U64[NS]).sum() / np.abs(U64[NS]).sum()),
+        "p_l1_rel": float(np.abs(pmx - p64).sum() / np.abs(p64).sum()),
+        "mixing_width_rel": float(abs(wmx - w64) / abs(w64)),
+    }
+
+
+def _ref_stage_ms(root):
+    """Sum of the reference's own stage timers (TVD, Hancock, Riemann, time
+    integration; Schemi/main/Schemi.cpp:946-963), i.e. the step without I/O."""
+    txt = open(os.path.join(root, "timeOfCalculation.tsv")).read().splitlines()
+    vals = {}
+    for i, line in enumerate(txt[:-1]):
+        if line.startswith("Time of") and "stage" in line or "solver work" in line \
+                or "integration work" in line:
+            vals[line] = float(txt[i + 1])
+    return sum(v for k, v in vals.items() if "Diffusion" not in k), vals
+
+
+def cpu_baseline(nx=96, ny=48, nz=48, steps=2):
+    """The reference solver (oracle/_ref/schemi, serial) on a 96x48x48 sample
+    of the same workload; falls back to the C oracle port when the reference
+    binary is absent.  Returns Mcell-updates/s per the same definition, from
+    the reference's own stage timers (no I/O)."""
+    try:
+        from oracle import decks
+        from schemi_amd import cases
+        from schemi_amd.gpu import conserved_from_primitive  # noqa: F401
+    except Exception as e:  # pragma: no cover
+        return {"error": str(e)}
+    case, U0 = cases.richtmyer_meshkov_3d(nx=nx, ny=ny, nz=nz, seed=2024)
+    n = case.ncells
+    sample = f"3-D RM {nx}x{ny}x{nz} ({n} cells), {steps} SSP-RK2 steps"
+    if decks.ref_available():
+        # primitives for the restart reader
+        rho_k, rho, m, E = U0[:NS], U0[NS], U0[NS + 1:NS + 4], U0[NS + 4]
+        c = np.stack([rho_k[k] / case.M[k] for k in range(NS)])
+        v = m / rho
+        c0 = c[0] + c[1]
+        cv = (c[0] / c0) * case.Cv[0] + (c[1] / c0) * case.Cv[1]
+        U = E - rho * (v * v).sum(0) * 0.5
+        p = (case.R / cv) * U
+        dt = 1e-4
+        t_end = 0.0
+        for _ in range(steps):
+            t_end = t_end + dt
+        with tempfile.TemporaryDirectory() as root:
+            decks.write_deck(root, case, t_end=t_end, restart=(c, v, p), const_dt=dt,
+                             out_freq=1000)
+            t0 = time.perf_counter()
+            r = decks.run(root, timeout=600)
+            wall = time.perf_counter() - t0
+            if r.returncode != 0:
+                return {"error": "reference run failed"}
+            ms, parts = _ref_stage_ms(root)
+        return {"value": n * steps / (ms * 1e-3) / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+                "kind": "reference", "wall_s_incl_io": wall, "stage_ms": ms,
+                "sample": sample + ", serial reference build (-O2, one process, MPI "
+                "decomposition collapsed); time = sum of its stage timers"}
+    from oracle.oracle import Oracle
+
+    o = Oracle(case)
+    o.set_conserved(U0[:NS], U0[NS], U0[NS + 1:NS + 4], U0[NS + 4])
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step(1e-4, 1)
+    wall = time.perf_counter() - t0
+    return {"value": n * steps / wall / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+            "kind": "port", "sample": sample + ", C restatement (oracle), one core"}
+
+
+def traffic_from_profiles():
+    """HBM bytes per stage launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", "stage_kernel_rm3d_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {a.gpus}); the "
+              "multi-GPU decomposition is a design, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    from schemi_amd import cases
+    from schemi_amd.gpu import PREC_FP64, PREC_MIXED, GpuSolver
+
+    case, U0 = cases.richtmyer_meshkov_3d(nx=a.nx, ny=a.ny, nz=a.nz, seed=2024)
+    n = case.ncells
+    res = {}
+    for prec, name in ((PREC_FP64, "fp64"), (PREC_MIXED, "mixed")):
+        s = GpuSolver(case, prec)
+        wall, avg_ms, launches = timed_run(s, U0, a.steps, a.warmup)
+        s.close()
+        res[name] = {"ms_per_step": 1e3 * wall / a.steps,
+                     "value": n * a.steps / wall / 1e6,
+                     "stage_kernel_ms": avg_ms, "launches": launches}
+        print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
+    gate = l1_gate(case, U0, a.l1_steps)
+    print(f"# l1 gate: {gate}", file=sys.stderr, flush=True)
+    cpu = None if a.no_cpu else cpu_baseline()
+    mx = res["mixed"]
+    achieved = BYTES_PER_CELL_LAUNCH * n / (mx["stage_kernel_ms"] * 1e-3) / 1e9
+    traffic = traffic_from_profiles()
+    line = {
+        "metric": "Mcell-updates/s, 3D RM 4.2M cells: mixed vs FP64 at L1<=1e-5; % HBM peak",
+        "value": mx["value"],
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": mx["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 flux + f64 state",
+        "data": "synthetic (seeded 3-D Richtmyer-Meshkov initial state)",
+        "config": {"workload": f"3-D Richtmyer-Meshkov {a.nx}x{a.ny}x{a.nz} "
+                               f"({n} cells), 2 species, HLLC + van Albada, SSP-RK2, "
+                               "Courant step on device",
+                   "cells": n, "precision": "mixed"},
+        "fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
+                 "stage_kernel_ms": res["fp64"]["stage_kernel_ms"],
+                 "hbm_frac": BYTES_PER_CELL_LAUNCH * n / (res["fp64"]["stage_kernel_ms"] *
+                                                          1e-3) / 1e9 / HBM_PEAK_GBS},
+        "mixed_over_fp64": mx["value"] / res["fp64"]["value"],
+        "l1_gate": gate,
+        "l1_ok": gate["rho_l1_rel"] <= 1e-5,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

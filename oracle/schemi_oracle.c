@@ -33,6 +33,9 @@ struct or_solver {
     long *cf;                        /* 6 per cell, surfacesOfCells order */
     int *cnn;                        /* neighbours per cell */
     long *cn;                        /* 6 per cell, neighboursOfCells order */
+    double *cfd;                     /* 6 x 3 per cell: face centre - cell centre */
+    double *cnd;                     /* 6 x 3 per cell: neighbour centre - cell centre */
+    double *fper;                    /* periodic faces: centre distance across the face */
     or_state st;
     double *pool;
 };
@@ -159,6 +162,106 @@ static void vtx(double *r, double x, double y, double z)
     r[2] = z;
 }
 
+/* Periodic boundaries (no counterpart in the reference): when both patches
+ * of an axis are periodic for every field, the low-side boundary face of each
+ * row is replaced in its cell's face list by the high-side face of the same
+ * row, which becomes an inner face with owner = last cell and neighbour =
+ * first cell.  Its weights use the two half-distances, the wrapped neighbours
+ * are at distance own + nei along the axis, and the low-side cell keeps its own
+ * face offset for the reconstruction.  The low-side faces are left unused. */
+static int axis_periodic(const or_solver *s, int a)
+{
+    static const int lo[3] = {0, 3, 2}, hi[3] = {1, 4, 5};
+    const or_config *c = &s->cfg;
+    if (a >= c->dim)
+        return 0;
+    int n = 0, tot = 0;
+    for (int side = 0; side < 2; ++side) {
+        const int p = side ? hi[a] : lo[a];
+        for (int k = 0; k < c->Ns; ++k) {
+            n += c->bc_c[k][p].type == OR_BC_PERIODIC;
+            ++tot;
+        }
+        n += c->bc_v[p].type == OR_BC_PERIODIC;
+        n += c->bc_p[p].type == OR_BC_PERIODIC;
+        tot += 2;
+    }
+    return n == tot ? 1 : (n == 0 ? 0 : -1);
+}
+
+static void pair_periodic(or_solver *s)
+{
+    const or_config *c = &s->cfg;
+    const long Nx = c->N[0], Ny = c->dim >= 2 ? c->N[1] : 1, Nz = c->dim >= 3 ? c->N[2] : 1;
+    const long NN[3] = {Nx, Ny, Nz};
+    /* slot of the low/high face of axis a in surfacesOfCells: x-, x+, y+, y-, z-, z+ */
+    static const int slot_lo[3] = {0, 3, 4}, slot_hi[3] = {1, 2, 5};
+    int any = 0;
+    for (int a = 0; a < c->dim; ++a) {
+        if (axis_periodic(s, a) != 1)
+            continue;
+        any = 1;
+        for (long id = 0; id < s->nc; ++id) {
+            const long idx[3] = {id % Nx, (id / Nx) % Ny, id / (Nx * Ny)};
+            if (idx[a] != 0)
+                continue;
+            long idh[3] = {idx[0], idx[1], idx[2]};
+            idh[a] = NN[a] - 1;
+            const long ch = idh[0] + Nx * (idh[1] + Ny * idh[2]);
+            const long flo = s->cf[6 * id + slot_lo[a]];
+            const long fhi = s->cf[6 * ch + slot_hi[a]];
+            const double own = or_hypot3(s->frc[3 * fhi] - s->crc[3 * ch],
+                                         s->frc[3 * fhi + 1] - s->crc[3 * ch + 1],
+                                         s->frc[3 * fhi + 2] - s->crc[3 * ch + 2]);
+            const double nei = or_hypot3(s->frc[3 * flo] - s->crc[3 * id],
+                                         s->frc[3 * flo + 1] - s->crc[3 * id + 1],
+                                         s->frc[3 * flo + 2] - s->crc[3 * id + 2]);
+            const double sum = own + nei;
+            s->wo[fhi] = nei / sum;
+            s->wn[fhi] = own / sum;
+            s->fpatch[fhi] = -1;
+            s->nei[fhi] = id;
+            s->fpatch[flo] = -3;
+            s->cf[6 * id + slot_lo[a]] = fhi;
+            s->fper[fhi] = sum;
+        }
+    }
+    if (!any)
+        return;
+    /* neighbour lists in the order x-, x+, y+, y-, z-, z+ with wrapping */
+    for (long id = 0; id < s->nc; ++id) {
+        const long idx[3] = {id % Nx, (id / Nx) % Ny, id / (Nx * Ny)};
+        static const int ax[6] = {0, 0, 1, 1, 2, 2}, dir[6] = {-1, 1, 1, -1, -1, 1};
+        int n = 0;
+        for (int q = 0; q < 2 * c->dim; ++q) {
+            const int a = ax[q];
+            long j[3] = {idx[0], idx[1], idx[2]};
+            j[a] += dir[q];
+            int wrapped = 0;
+            if (j[a] < 0 || j[a] >= NN[a]) {
+                if (axis_periodic(s, a) != 1)
+                    continue;
+                j[a] = (j[a] + NN[a]) % NN[a];
+                wrapped = 1;
+            }
+            const long nb = j[0] + Nx * (j[1] + Ny * j[2]);
+            s->cn[6 * id + n] = nb;
+            for (int b = 0; b < 3; ++b)
+                s->cnd[18 * id + 3 * n + b] = s->crc[3 * nb + b] - s->crc[3 * id + b];
+            if (wrapped) {
+                /* face slot of this direction holds the periodic face */
+                const long f = s->cf[6 * id + (dir[q] < 0 ? (a == 0 ? 0 : a == 1 ? 3 : 4)
+                                                         : (a == 0 ? 1 : a == 1 ? 2 : 5))];
+                for (int b = 0; b < 3; ++b)
+                    s->cnd[18 * id + 3 * n + b] = 0.0;
+                s->cnd[18 * id + 3 * n + a] = dir[q] * s->fper[f];
+            }
+            ++n;
+        }
+        s->cnn[id] = n;
+    }
+}
+
 static int build_mesh(or_solver *s)
 {
     const or_config *c = &s->cfg;
@@ -193,6 +296,7 @@ static int build_mesh(or_solver *s)
     s->cf = calloc(6 * nc, sizeof(long));
     s->cnn = calloc(nc, sizeof(int));
     s->cn = calloc(6 * nc, sizeof(long));
+    s->fper = calloc(nf, sizeof(double));
     if (!s->crc || !s->cn)
         return ERR_INIT;
 
@@ -457,6 +561,17 @@ static int build_mesh(or_solver *s)
     }
 #undef CID
     orient_and_weights(s);
+    s->cfd = calloc(18 * nc, sizeof(double));
+    s->cnd = calloc(18 * nc, sizeof(double));
+    for (long i = 0; i < nc; ++i) {
+        for (int j = 0; j < s->cnf[i]; ++j)
+            for (int a = 0; a < 3; ++a)
+                s->cfd[18 * i + 3 * j + a] = s->frc[3 * s->cf[6 * i + j] + a] - s->crc[3 * i + a];
+        for (int cx = 0; cx < s->cnn[i]; ++cx)
+            for (int a = 0; a < 3; ++a)
+                s->cnd[18 * i + 3 * cx + a] = s->crc[3 * s->cn[6 * i + cx] + a] - s->crc[3 * i + a];
+    }
+    pair_periodic(s);
     return ERR_NONE;
 }
 
@@ -589,6 +704,11 @@ or_solver *or_create(const or_config *cfg)
         return NULL;
     or_solver *s = calloc(1, sizeof(or_solver));
     s->cfg = *cfg;
+    for (int a = 0; a < cfg->dim; ++a)
+        if (axis_periodic(s, a) < 0) {
+            free(s);
+            return NULL;
+        }
     if (build_mesh(s) != ERR_NONE) {
         or_destroy(s);
         return NULL;
@@ -616,6 +736,9 @@ void or_destroy(or_solver *s)
     free(s->cf);
     free(s->cnn);
     free(s->cn);
+    free(s->cfd);
+    free(s->cnd);
+    free(s->fper);
     free(s->pool);
     free(s);
 }
@@ -828,6 +951,8 @@ static int grad_scalar(or_solver *s, const double *q, const or_bc *bcs,
 {
     double *qf = malloc(s->nf * sizeof(double));
     for (long f = 0; f < s->nf; ++f) {
+        if (s->fpatch[f] == -3)
+            continue;
         if (s->fpatch[f] < 0)
             qf[f] = q[s->own[f]] * s->wo[f] + q[s->nei[f]] * s->wn[f];
         else if (bc_surf_scalar(&bcs[s->fpatch[f]], q[s->own[f]], &qf[f])) {
@@ -861,6 +986,8 @@ static int grad_vector(or_solver *s, double *const *v, const or_bc *bcs,
     double *vf = malloc(3 * s->nf * sizeof(double));
     for (long f = 0; f < s->nf; ++f) {
         const long o = s->own[f];
+        if (s->fpatch[f] == -3)
+            continue;
         if (s->fpatch[f] < 0) {
             const long n = s->nei[f];
             for (int a = 0; a < 3; ++a)
@@ -956,9 +1083,7 @@ static int tvd_scalar(or_solver *s, const double *G, const double *q,
         int ne = 0;
         for (int cx = 0; cx < s->cnn[i]; ++cx) {
             const long n = s->cn[6 * i + cx];
-            double d[3];
-            for (int a = 0; a < 3; ++a)
-                d[a] = s->crc[3 * n + a] - ci[a];
+            const double *d = &s->cnd[18 * i + 3 * cx];
             const double dm = or_hypot3(d[0], d[1], d[2]);
             double dn[3];
             for (int a = 0; a < 3; ++a)
@@ -1016,9 +1141,7 @@ static int tvd_vector(or_solver *s, const double *G, double *const *v,
         int ne = 0;
         for (int cx = 0; cx < s->cnn[i]; ++cx) {
             const long n = s->cn[6 * i + cx];
-            double d[3];
-            for (int a = 0; a < 3; ++a)
-                d[a] = s->crc[3 * n + a] - ci[a];
+            const double *d = &s->cnd[18 * i + 3 * cx];
             const double dm = or_hypot3(d[0], d[1], d[2]);
             double dn[3];
             for (int a = 0; a < 3; ++a)
@@ -1376,9 +1499,7 @@ int or_stage(or_solver *s, double dt)
     for (long i = 0; i < nc; ++i) {
         for (int j = 0; j < s->cnf[i]; ++j) {
             const long f = s->cf[6 * i + j];
-            double d[3];
-            for (int a = 0; a < 3; ++a)
-                d[a] = s->frc[3 * f + a] - s->crc[3 * i + a];
+            const double *d = &s->cfd[18 * i + 3 * j];
             face_state *fs = (s->own[f] == i) ? &FO[f] : &FN[f];
             for (int k = 0; k < Ns; ++k)
                 fs->c[k + 1] = dot3(&Lc[3 * nc * k + 3 * i], d) + st->c[k + 1][i];
@@ -1391,6 +1512,8 @@ int or_stage(or_solver *s, double dt)
     }
     /* owner side everywhere, neighbour side on inner faces */
     for (long f = 0; f < nf; ++f) {
+        if (s->fpatch[f] == -3)
+            continue; /* low-side face of a periodic pair, unused */
         close_face(cfg, &FO[f]);
         if (s->fpatch[f] < 0)
             close_face(cfg, &FN[f]);
@@ -1415,7 +1538,8 @@ int or_stage(or_solver *s, double dt)
 
     /* numerical fluxes (Advection2dOrder.cpp:528) */
     for (long f = 0; f < nf && !err; ++f)
-        err = face_flux_dispatch(cfg, &s->fN[3 * f], &FO[f], &FN[f], &FF[f]);
+        if (s->fpatch[f] != -3)
+            err = face_flux_dispatch(cfg, &s->fN[3 * f], &FO[f], &FN[f], &FF[f]);
     if (err)
         goto done;
 

@@ -20,7 +20,8 @@
 
 /* Reference enum values (Schemi/enumerations/ headers). */
 enum { OR_BC_BLANK = 0, OR_BC_FREE = 1, OR_BC_SLIP = 2, OR_BC_FIXED_CELL = 3,
-       OR_BC_FIXED_SURF = 4, OR_BC_INNER = 5 };
+       OR_BC_FIXED_SURF = 4, OR_BC_INNER = 5,
+       OR_BC_PERIODIC = 32 /* no reference counterpart */ };
 enum { OR_FLUX_HLL = 0, OR_FLUX_HLLC = 2 };
 enum { OR_LIM_ZERO = 0, OR_LIM_MINMOD = 1, OR_LIM_VANLEER = 2, OR_LIM_LINEAR = 3,
        OR_LIM_VANALBADA = 4 };

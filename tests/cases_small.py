@@ -3,6 +3,7 @@ import numpy as np
 
 from schemi_amd import Case, typeOfTVDLimiter as TL, flowSolver as FS
 from schemi_amd import boundaryConditionType as BCT
+from schemi_amd.gpu import periodic
 
 BASE = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
 
@@ -34,7 +35,7 @@ def cases():
     c.set_bc(0, BCT.fixedValueCell, "v", (0.1, 0.0, 0.0))
     out["1d_fixed"] = c
     for lim in (TL.vanLeer, TL.linear):
-        c = Case(dim=2, N=(40, 24, 1), L=(0.25, 1.0, 0.1), limiter=lim, **BASE)
+        c = Case(dim=2, N=(40, 32, 1), L=(0.25, 1.0, 0.1), limiter=lim, **BASE)
         c.set_bc(3, BCT.slip)
         c.set_bc(4, BCT.slip)
         out[f"2d_{lim.name}"] = c
@@ -42,10 +43,22 @@ def cases():
              gravity=True, g=(0.0, -1.0, 0.0), **BASE)
     out["2d_hll_grav"] = c
     for lim in (TL.vanAlbada, TL.linear):
-        c = Case(dim=3, N=(40, 12, 20), L=(2.0, 1.0, 1.0), limiter=lim, **BASE)
+        c = Case(dim=3, N=(40, 16, 16), L=(2.0, 1.0, 1.0), limiter=lim, **BASE)
         for pt in (2, 3, 4, 5):
             c.set_bc(pt, BCT.slip)
         out[f"3d_{lim.name}"] = c
     c = Case(dim=3, N=(33, 9, 18), L=(1.0, 0.5, 0.5), limiter=TL.minmod, flux=FS.HLL, **BASE)
     out["3d_hll_partial_tiles"] = c
+    # periodic boundaries (no reference counterpart; the oracle pairs the faces)
+    c = Case(dim=1, N=(256, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, **BASE)
+    periodic(c, 0)
+    out["1d_periodic"] = c
+    c = Case(dim=2, N=(64, 32, 1), L=(1.0, 1.0, 0.1), limiter=TL.minmod, flux=FS.HLL, **BASE)
+    periodic(c, 0)
+    periodic(c, 1)
+    out["2d_periodic_hll"] = c
+    c = Case(dim=3, N=(24, 16, 20), L=(1.5, 1.0, 1.0), limiter=TL.linear, **BASE)
+    periodic(c, 1)
+    periodic(c, 2)
+    out["3d_periodic_yz_linear"] = c
     return out

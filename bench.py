@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--l1-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one 1 GiB device copy (calibrates rocprofv3 byte counters)")
     return ap.parse_args()
 
 
@@ -189,6 +191,12 @@ def main():
     from schemi_amd import cases
     from schemi_amd.gpu import PREC_FP64, PREC_MIXED, GpuSolver
 
+    if a.copy_calib:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     case, U0 = cases.richtmyer_meshkov_3d(nx=a.nx, ny=a.ny, nz=a.nz, seed=2024)
     n = case.ncells
     res = {}

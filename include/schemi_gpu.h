@@ -187,7 +187,10 @@ int schemi_gpu_face_permutation(const schemi_mesh_desc *mesh, int64_t *ref_to_gp
  * 3 owner weights, 4 neighbour weights, 5 centre distances across faces,
  * 6 x_f(i) - x_c(i), 7 x_f(i+1) - x_c(i), 8 limiter ghost distances (low,
  * high), 9 separability flag (1 when every face centre shares the cells'
- * transverse coordinates).  periodic_mask bit a marks periodic axes.  With
+ * transverse coordinates), 10 + 2 b + cls the offsets of the centres of the
+ * faces normal to `axis` from their cells' centres along transverse axis b,
+ * per cell index along b (cls 0 inner faces, 1 boundary-plane faces; empty
+ * for b == axis).  periodic_mask bit a marks periodic axes.  With
  * out == NULL only *len is set.  Replaces the reference's mesh accessors
  * (mesh.hpp:39-176: cells(), surfaces(), surfOwnW(), surfNeiW()). */
 int schemi_gpu_axis_table(const schemi_mesh_desc *mesh, int32_t periodic_mask,

@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -139,6 +140,9 @@ struct schemi_gpu_ctx {
     double *sc = nullptr;
     int *derr = nullptr;
     int dim = 1, ns = 1, lin = 0, prec = 0;
+    int mode = MODE_SLOPES;
+    void *scratch = nullptr;  // MODE_GENERAL: primitives and gradients
+    void *W = nullptr, *Lg = nullptr;
     long long n = 0;
     int kchunk = 16;
     double const_dt = -1.0;
@@ -179,6 +183,9 @@ int launch_stage(schemi_gpu_ctx *c, const double *Uin, double *Uout, const doubl
     L.ns = c->ns;
     L.lin = c->lin;
     L.stage2 = stage2 ? 1 : 0;
+    L.mode = c->mode;
+    L.W = c->W;
+    L.Lg = c->Lg;
     L.nx = c->hp.N[0];
     L.ny = c->hp.N[1];
     L.nz = c->hp.N[2];
@@ -265,6 +272,8 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     const int dim = mesh->dim;
     if (dim < 1 || dim > 3)
         return SCHEMI_ERR_MESH;
+    // at least two cells along every axis of a 2-D/3-D mesh (the first-order
+    // kernel relies on a neighbour across each other axis)
     for (int a = 0; a < dim; ++a)
         if (mesh->n[a] < 1 || (dim >= 2 && mesh->n[a] < 2) || mesh->n[a] > (1 << 30))
             return SCHEMI_ERR_MESH;
@@ -362,10 +371,35 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         P.zero_entry[a] = (dim >= 2) ? 1 : 0;
 
     c->tables = build_tables(dim, mesh->n, mesh->extent, mesh->origin, periodic);
-    if (!c->tables.separable && dim >= 2 && lim == SCHEMI_LIM_LINEAR) {
-        delete c;
-        return SCHEMI_ERR_MESH;
+    const bool separable = c->tables.separable;
+    // orientation of the high boundary faces: the reference points a face
+    // normal away from its owner's centre (mesh.cpp:34-53); the high plane
+    // x_f(N) = o + L can lie below the last cell centre along z in 3-D, where
+    // the vertices are spaced by Ly / Nz.  A face through the centre (no
+    // orientation) and a periodic axis with a reversed face are declined.
+    for (int a = 0; a < dim; ++a) {
+        const AxisTables &T = c->tables.ax[a];
+        const double d = T.xf[T.N] - T.xc[T.N - 1];
+        if (d == 0.0 || (d < 0.0 && periodic[a])) {
+            delete c;
+            return SCHEMI_ERR_MESH;
+        }
+        P.hi_flip[a] = d < 0.0 ? 1 : 0;
     }
+    // Kernel family.  In 2-D and 3-D the TVD ratio along an axis has an exact
+    // zero entry from the neighbours across another axis (their centres share
+    // the coordinate; every axis has at least two cells, checked above), so
+    // phi = 0 and the slopes vanish: first-order kernel, unless the full
+    // limiter evaluation is requested.  The linear limiter keeps its slopes;
+    // on non-separable meshes its face offsets have transverse components.
+    const char *full = std::getenv("SCHEMI_GPU_FULL_LIMITER");
+    const bool force_slopes = full && full[0] == '1';
+    if (lim == SCHEMI_LIM_LINEAR)
+        c->mode = (dim >= 2 && !separable) ? MODE_GENERAL : MODE_SLOPES;
+    else if (lim == SCHEMI_LIM_ZERO || dim >= 2)
+        c->mode = force_slopes ? MODE_SLOPES : MODE_FIRST_ORDER;
+    else
+        c->mode = MODE_SLOPES;
     // flatten the per-axis tables
     std::vector<double> geo;
     auto put = [&](const std::vector<double> &v) {
@@ -386,6 +420,10 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         P.ghost_lo[a] = T.ghost_lo;
         P.ghost_hi[a] = T.ghost_hi;
     }
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int cls = 0; cls < 2; ++cls)
+                P.off_td[a][b][cls] = (a == b) ? 0 : put(c->tables.td[a][b][cls]);
     c->kchunk = 16;
 
     if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
@@ -399,6 +437,15 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
               hipMalloc(&c->U[1], state_bytes) == hipSuccess &&
               hipMalloc(&c->sc, SC_N * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->derr, sizeof(int)) == hipSuccess;
+    if (ok && c->mode == MODE_GENERAL) {
+        const size_t tsz = (c->prec == SCHEMI_PREC_MIXED) ? sizeof(float) : sizeof(double);
+        const size_t nq = (size_t)(mix->ns + 4) * (size_t)c->n;
+        ok = hipMalloc(&c->scratch, tsz * nq * (size_t)(1 + dim)) == hipSuccess;
+        if (ok) {
+            c->W = c->scratch;
+            c->Lg = static_cast<char *>(c->scratch) + tsz * nq;
+        }
+    }
     ok = ok &&
          hipMemcpyAsync(c->dP, &P, sizeof P, hipMemcpyHostToDevice, c->stream) == hipSuccess &&
          hipMemcpyAsync(c->dgeo, geo.data(), geo.size() * sizeof(double),
@@ -431,6 +478,7 @@ void schemi_gpu_destroy(schemi_gpu_ctx *c)
     (void)hipFree(c->U[1]);
     (void)hipFree(c->sc);
     (void)hipFree(c->derr);
+    (void)hipFree(c->scratch);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
@@ -655,7 +703,7 @@ int schemi_gpu_axis_table(const schemi_mesh_desc *mesh, int32_t periodic_mask, i
                           int32_t which, double *out, int64_t *len)
 {
     if (!mesh || !len || mesh->dim < 1 || mesh->dim > 3 || axis < 0 || axis > 2 ||
-        which < 0 || which > 9)
+        which < 0 || which > 15)
         return SCHEMI_ERR_INIT;
     const bool per[3] = {(periodic_mask & 1) != 0, (periodic_mask & 2) != 0,
                          (periodic_mask & 4) != 0};
@@ -672,7 +720,13 @@ int schemi_gpu_axis_table(const schemi_mesh_desc *mesh, int32_t periodic_mask, i
     case 6: v = T.dlo; break;
     case 7: v = T.dhi; break;
     case 8: v = {T.ghost_lo, T.ghost_hi}; break;
-    default: v = {M.separable ? 1.0 : 0.0}; break;
+    case 9: v = {M.separable ? 1.0 : 0.0}; break;
+    default: {
+        const int b = (which - 10) / 2, cls = (which - 10) % 2;
+        if (b != axis)
+            v = M.td[axis][b][cls];
+        break;
+    }
     }
     *len = static_cast<int64_t>(v.size());
     if (out)

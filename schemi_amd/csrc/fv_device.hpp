@@ -36,6 +36,10 @@ struct DevParams {
     long long stride[3];  // 1, Nx, Nx*Ny
     int periodic[3];
     int zero_entry[3];    // the limiter ratio along the axis has an exact 0 entry
+    // the high boundary face of the axis lies below the last cell centre, so
+    // the reference orients its normal to -axis (mesh.cpp:34-53); happens
+    // along z in 3-D when Lz < Ly (the z spacing is Ly / Nz, mesh.cpp:1219)
+    int hi_flip[3];
     int limiter, flux;
     double M[MAXS], Cv[MAXS], R;
     float Mf[MAXS], Cvf[MAXS], Rf;
@@ -49,6 +53,9 @@ struct DevParams {
     // offsets (in doubles) of the per-axis tables inside geo[]
     int off_xc[3], off_e[3], off_xf[3], off_wo[3], off_wn[3], off_dn[3],
         off_dlo[3], off_dhi[3];
+    // transverse face-centre offsets: faces normal to a, along b, class
+    // (0 inner, 1 boundary plane), indexed by the cell index along b
+    int off_td[3][3][2];
     double courant, min_time;
 };
 
@@ -181,77 +188,59 @@ __device__ __forceinline__ bool hllc_flux(const Side<T, NS> &O, const Side<T, NS
                  (rO * (SO - uO) - rN * (SN - uN));
     const T PLR = T(0.5) * (O.p + B.p + rO * (SO - uO) * (Sc - uO) +
                             rN * (SN - uN) * (Sc - uN));
-    if (SO >= T(0)) {
-#pragma unroll
-        for (int k = 0; k < NS; ++k)
-            g[k] = uO * O.rho[k];
-        g[NS] = uO * O.r0;
-#pragma unroll
-        for (int b = 0; b < 3; ++b)
-            g[NS + 1 + b] = (b == ax) ? O.m[b] * uO + sg * O.p : O.m[b] * uO;
-        g[NS + 4] = uO * (O.E + O.p);
-    } else if (Sc >= T(0)) {
-        const T factor = SO - uO;
-        const T den = T(1) / (SO - Sc);
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const T rs = O.rho[k] * factor * den;
-            g[k] = O.rho[k] * uO + SO * (rs - O.rho[k]);
-        }
-        {
-            const T rs = O.r0 * factor * den;
-            g[NS] = O.r0 * uO + SO * (rs - O.r0);
-        }
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            if (b == ax) {
-                const T ms = (O.m[b] * factor - sg * (O.p - PLR)) * den;
-                g[NS + 1 + b] = (O.m[b] * uO + sg * O.p) + (ms - O.m[b]) * SO;
-            } else {
-                const T ms = (O.m[b] * factor) * den;
-                g[NS + 1 + b] = (O.m[b] * uO) + (ms - O.m[b]) * SO;
-            }
-        }
-        const T Es = (O.E * factor - O.p * uO + PLR * Sc) * den;
-        g[NS + 4] = (O.E + O.p) * uO + SO * (Es - O.E);
-    } else if (SN > T(0)) {
-        const T factor = SN - uN;
-        const T den = T(1) / (SN - Sc);
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const T rs = B.rho[k] * factor * den;
-            g[k] = B.rho[k] * uN + SN * (rs - B.rho[k]);
-        }
-        {
-            const T rs = B.r0 * factor * den;
-            g[NS] = B.r0 * uN + SN * (rs - B.r0);
-        }
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            if (b == ax) {
-                const T ms = (B.m[b] * factor - sg * (B.p - PLR)) * den;
-                g[NS + 1 + b] = (B.m[b] * uN + sg * B.p) + (ms - B.m[b]) * SN;
-            } else {
-                const T ms = (B.m[b] * factor) * den;
-                g[NS + 1 + b] = (B.m[b] * uN) + (ms - B.m[b]) * SN;
-            }
-        }
-        const T Es = (B.E * factor - B.p * uN + PLR * Sc) * den;
-        g[NS + 4] = (B.E + B.p) * uN + SN * (Es - B.E);
-    } else if (SN <= T(0)) {
-#pragma unroll
-        for (int k = 0; k < NS; ++k)
-            g[k] = uN * B.rho[k];
-        g[NS] = uN * B.r0;
-#pragma unroll
-        for (int b = 0; b < 3; ++b)
-            g[NS + 1 + b] = (b == ax) ? B.m[b] * uN + sg * B.p : B.m[b] * uN;
-        g[NS + 4] = uN * (B.E + B.p);
-    } else {
+    // Wave pattern (HLLCSolver.cpp:140-388): SO >= 0 -> F(O); Sc >= 0 -> F*(O);
+    // SN > 0 -> F*(B); SN <= 0 -> F(B); otherwise NaN speeds.  The star flux
+    // of side X is F(X) + S (U*_X - U_X), evaluated as that sum, so one code
+    // path serves both sides: pick the side, form F(X), add the star term.
+    const bool own = (SO >= T(0)) || (Sc >= T(0));
+    const bool star = own ? !(SO >= T(0)) : (SN > T(0));
+    if (!own && !(SN > T(0)) && !(SN <= T(0))) {
 #pragma unroll
         for (int q = 0; q < NS + 5; ++q)
             g[q] = SO;  // NaN
         return false;
+    }
+    const T u = own ? uO : uN;
+    const T S = own ? SO : SN;
+    T rho[NS], m[3];
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        rho[k] = own ? O.rho[k] : B.rho[k];
+    const T r0 = own ? O.r0 : B.r0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+        m[b] = own ? O.m[b] : B.m[b];
+    const T p = own ? O.p : B.p;
+    const T E = own ? O.E : B.E;
+    // F(X) & N
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        g[k] = rho[k] * u;
+    g[NS] = r0 * u;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+        g[NS + 1 + b] = (b == ax) ? m[b] * u + sg * p : m[b] * u;
+    g[NS + 4] = (E + p) * u;
+    if (star) {
+        const T factor = S - u;
+        const T den = T(1) / (S - Sc);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const T rs = rho[k] * factor * den;
+            g[k] = g[k] + S * (rs - rho[k]);
+        }
+        {
+            const T rs = r0 * factor * den;
+            g[NS] = g[NS] + S * (rs - r0);
+        }
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const T ms = (b == ax) ? (m[b] * factor - sg * (p - PLR)) * den
+                                   : (m[b] * factor) * den;
+            g[NS + 1 + b] = g[NS + 1 + b] + (ms - m[b]) * S;
+        }
+        const T Es = (E * factor - p * u + PLR * Sc) * den;
+        g[NS + 4] = g[NS + 4] + S * (Es - E);
     }
     return true;
 }

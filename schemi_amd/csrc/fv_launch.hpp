@@ -7,6 +7,13 @@
 
 namespace sg {
 
+// Kernel family of a context.
+enum : int {
+    MODE_FIRST_ORDER = 0,  // slopes vanish identically: stage0_kernel
+    MODE_SLOPES = 1,       // fused slopes (TVD in 1-D, linear on separable meshes)
+    MODE_GENERAL = 2,      // linear limiter with transverse face offsets
+};
+
 struct StageLaunch {
     const DevParams *P;
     const double *geo;
@@ -16,13 +23,14 @@ struct StageLaunch {
     const double *dt;
     unsigned long long *sigmax;
     int *err;
-    int dim, ns, lin, stage2;
+    int dim, ns, lin, stage2, mode;
     int nx, ny, nz;
     int kchunk;
+    void *W, *Lg;  // MODE_GENERAL scratch: primitives and gradients
     hipStream_t stream;
 };
 
-// Returns 0 on success, -1 when the species count has no instantiation,
+// Returns 0 on success, -1 when the combination has no instantiation,
 // otherwise a hipError_t value.
 int launch_stage_f64(const StageLaunch &L);
 int launch_stage_mixed(const StageLaunch &L);

@@ -2,21 +2,62 @@
 // flux precision (included by fv_f64.hip and fv_mixed.hip).
 #pragma once
 
+#include "fv_general.hpp"
 #include "fv_launch.hpp"
 #include "fv_stage.hpp"
 
 namespace sg {
 
-template <class T, int NS, int DIM, bool LIN, bool S2, bool MIXED>
-static int launch_one(const StageLaunch &L)
+template <int DIM> struct TileShape {
+    static constexpr int TX = (DIM == 1) ? 256 : 32;
+    static constexpr int TY = (DIM == 1) ? 1 : 8;
+};
+
+template <int DIM> static dim3 tile_grid(const StageLaunch &L)
 {
-    constexpr int TX = (DIM == 1) ? 256 : 32;
-    constexpr int TY = (DIM == 1) ? 1 : 8;
-    dim3 grid((L.nx + TX - 1) / TX, (DIM >= 2) ? (L.ny + TY - 1) / TY : 1,
-              (DIM == 3) ? (L.nz + L.kchunk - 1) / L.kchunk : 1);
-    hipLaunchKernelGGL((stage_kernel<T, NS, DIM, LIN, S2, MIXED, TX, TY>), grid, dim3(TX * TY), 0,
-                       L.stream, L.P, L.geo, L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err,
-                       L.kchunk);
+    constexpr int TX = TileShape<DIM>::TX, TY = TileShape<DIM>::TY;
+    return dim3((L.nx + TX - 1) / TX, (DIM >= 2) ? (L.ny + TY - 1) / TY : 1,
+                (DIM == 3) ? (L.nz + L.kchunk - 1) / L.kchunk : 1);
+}
+
+template <class T, int NS, int DIM, bool S2, bool MIXED>
+static int launch_mode(const StageLaunch &L)
+{
+    constexpr int TX = TileShape<DIM>::TX, TY = TileShape<DIM>::TY;
+    switch (L.mode) {
+    case MODE_FIRST_ORDER:
+        hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, S2, MIXED, TX, TY>), tile_grid<DIM>(L),
+                           dim3(TX * TY), 0, L.stream, L.P, L.geo, L.Uin, L.Uout, L.Un, L.dt,
+                           L.sigmax, L.err, L.kchunk);
+        break;
+    case MODE_SLOPES:
+        if (L.lin)
+            hipLaunchKernelGGL((stage_kernel<T, NS, DIM, true, S2, MIXED, TX, TY>),
+                               tile_grid<DIM>(L), dim3(TX * TY), 0, L.stream, L.P, L.geo, L.Uin,
+                               L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
+        else
+            hipLaunchKernelGGL((stage_kernel<T, NS, DIM, false, S2, MIXED, TX, TY>),
+                               tile_grid<DIM>(L), dim3(TX * TY), 0, L.stream, L.P, L.geo, L.Uin,
+                               L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
+        break;
+    case MODE_GENERAL: {
+        if constexpr (DIM == 1) {
+            return -1;
+        } else {
+            const long long n = (long long)L.nx * L.ny * L.nz;
+            const dim3 g((unsigned)((n + 255) / 256));
+            T *W = static_cast<T *>(L.W), *Lg = static_cast<T *>(L.Lg);
+            hipLaunchKernelGGL((gen_grad_kernel<T, NS, DIM, MIXED>), g, dim3(256), 0, L.stream,
+                               L.P, L.geo, L.Uin, W, Lg, L.dt);
+            hipLaunchKernelGGL((gen_update_kernel<T, NS, DIM, S2, MIXED>), g, dim3(256), 0,
+                               L.stream, L.P, L.geo, L.Uin, L.Uout, L.Un, W, Lg, L.dt, L.sigmax,
+                               L.err);
+        }
+        break;
+    }
+    default:
+        return -1;
+    }
     return static_cast<int>(hipGetLastError());
 }
 
@@ -24,13 +65,9 @@ template <class T, int NS, bool MIXED>
 static int launch_ns(const StageLaunch &L)
 {
 #define SG_CASE(D)                                                               \
-    if (L.dim == D) {                                                            \
-        if (L.lin)                                                               \
-            return L.stage2 ? launch_one<T, NS, D, true, true, MIXED>(L)         \
-                            : launch_one<T, NS, D, true, false, MIXED>(L);       \
-        return L.stage2 ? launch_one<T, NS, D, false, true, MIXED>(L)            \
-                        : launch_one<T, NS, D, false, false, MIXED>(L);          \
-    }
+    if (L.dim == D)                                                              \
+        return L.stage2 ? launch_mode<T, NS, D, true, MIXED>(L)                  \
+                        : launch_mode<T, NS, D, false, MIXED>(L);
     SG_CASE(1)
     SG_CASE(2)
     SG_CASE(3)

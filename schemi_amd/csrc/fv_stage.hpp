@@ -26,8 +26,10 @@
 // axis only from the neighbours along it plus exact zero entries from the
 // other neighbours (their centres share the coordinate), and the transverse
 // velocity slopes have a limiter ratio with an exact zero entry or a zero
-// gradient, so phi (r) G = 0 for them (TVD limiters).  The face-to-centre
-// offsets across an axis are exactly zero on the meshes the host accepts.
+// gradient, so phi (r) G = 0 for them (TVD limiters).  This kernel serves the
+// TVD limiters in 1-D and the linear limiter on meshes whose face-to-centre
+// offsets across an axis are exactly zero; stage0_kernel (below) serves the
+// schemes whose slopes vanish, fv_general.hpp the linear limiter elsewhere.
 #pragma once
 
 #include "fv_device.hpp"
@@ -116,6 +118,8 @@ __device__ __forceinline__ void slopes_axis(const DevParams &P, const AxisGeo &G
     const T dn_lo = to_T<T>(G.dn[i]), dn_hi = to_T<T>(G.dn[i + 1]);
     const T gh_lo = to_T<T>(P.ghost_lo[a]), gh_hi = to_T<T>(P.ghost_hi[a]);
     const bool zero = P.zero_entry[a] != 0;
+    // normal of the high face: -1 on a flipped high boundary face
+    const bool flip = !has_hi && P.hi_flip[a] != 0;
 #pragma unroll
     for (int s = 0; s < SL::NL; ++s) {
         const int q = SL::prim(s, a);
@@ -128,26 +132,38 @@ __device__ __forceinline__ void slopes_axis(const DevParams &P, const AxisGeo &G
         if (!has_hi)
             bc_of<NS>(P, phi_, q, thi, vhi);
         // face values for the Green-Gauss gradient
-        const T flo = has_lo ? wm[q] * wo_lo + q0 * wn_lo
-                             : bc_surface<T>(tlo, T(vlo), q0, comp, a);
-        const T fhi = has_hi ? q0 * wo_hi + wp[q] * wn_hi
-                             : bc_surface<T>(thi, T(vhi), q0, comp, a);
-        const T Gq = (fhi * S - flo * S) / V;
+        auto gradient = [&]() {
+            const T flo = has_lo ? wm[q] * wo_lo + q0 * wn_lo
+                                 : bc_surface<T>(tlo, T(vlo), q0, comp, a);
+            const T fhi = has_hi ? q0 * wo_hi + wp[q] * wn_hi
+                                 : bc_surface<T>(thi, T(vhi), q0, comp, a);
+            return ((flip ? -fhi : fhi) * S - flo * S) / V;
+        };
         if constexpr (LIN) {
-            L[s] = Gq;
+            L[s] = gradient();
         } else {
             // limiter entries (d-hat = -1 below, +1 above the cell)
             const T elo = has_lo ? -((wm[q] - q0) / dn_lo)
                                  : -((bc_cell<T>(tlo, T(vlo), q0, comp, a) - q0) / gh_lo);
-            const T ehi = has_hi ? (wp[q] - q0) / dn_hi
-                                 : (bc_cell<T>(thi, T(vhi), q0, comp, a) - q0) / gh_hi;
+            T ehi = has_hi ? (wp[q] - q0) / dn_hi
+                           : (bc_cell<T>(thi, T(vhi), q0, comp, a) - q0) / gh_hi;
+            if (flip)
+                ehi = -ehi;  // d-hat of the ghost is -1
             T mn = tmin(elo, ehi), mx = tmax(elo, ehi);
             if (zero) {
                 mn = tmin(mn, T(0));
                 mx = tmax(mx, T(0));
             }
+            // r = mn / (mx + eps) <= 0 exactly when mn <= 0 and mx + eps > 0;
+            // every limiter here has phi(r <= 0) = +-0, so phi (r) G is +-0 and
+            // neither the ratio nor the gradient is needed (the reference
+            // evaluates both and multiplies by the zero)
+            if (mn <= T(0) && mx + T(STAB) > T(0)) {
+                L[s] = T(0);
+                continue;
+            }
             const T r = mn / (mx + T(STAB));
-            L[s] = phi<T>(P.limiter, r) * Gq;
+            L[s] = phi<T>(P.limiter, r) * gradient();
         }
     }
 }
@@ -176,6 +192,44 @@ __device__ __forceinline__ void reconstruct(const T *w, const T *L, int a, T d, 
     p = f[NS + 3];
 }
 
+// Ghost state of a boundary face of axis a (side 0 = low plane, 1 = high)
+// from the owner's face value (Advection2dOrder.cpp:398-521).
+template <class T, int NS>
+__device__ __forceinline__ void ghost_side(const DevParams &P, int a, int side, const T *cO,
+                                           const T *vO, T pO, T *cB, T *vB, T &pB)
+{
+    const int p = patch_of(a, side);
+    int t;
+    double val;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        bc_of<NS>(P, p, k, t, val);
+        cB[k] = bc_cell<T>(t, T(val), cO[k], -1, a);
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        bc_of<NS>(P, p, NS + b, t, val);
+        vB[b] = bc_cell<T>(t, T(val), vO[b], b, a);
+    }
+    bc_of<NS>(P, p, NS + 3, t, val);
+    pB = bc_cell<T>(t, T(val), pO, -1, a);
+}
+
+// EOS closure of both face sides and the HLLC / HLL flux F & N of a face
+// normal to axis a with sign sgn (Advection2dOrder.cpp:240-528); false when
+// the face has no valid wave pattern.
+template <class T, int NS>
+__device__ __forceinline__ bool riemann(const DevParams &P, int a, T sgn, const T *cO,
+                                        const T *vO, T pO, const T *cB, const T *vB, T pB, T *g)
+{
+    Side<T, NS> O, B;
+    close_side<T, NS>(P, cO, vO, pO, O);
+    close_side<T, NS>(P, cB, vB, pB, B);
+    if (P.flux == FLUX_HLL)
+        return hll_flux<T, NS>(O, B, a, sgn, g);
+    return hllc_flux<T, NS>(O, B, a, sgn, g);
+}
+
 // Flux of face f (0..Na) along axis a.  Owner/neighbour data: primitives and
 // slopes of cells f-1 (owner, lower) and f (neighbour); at the low boundary
 // the owner is cell 0 (slopes Ln, primitives wn) and the normal points to -a;
@@ -192,51 +246,20 @@ __device__ __forceinline__ bool face_flux(const DevParams &P, const AxisGeo &G, 
     if (!per && f == 0) {
         // low boundary face: owner is cell 0, ghost from the owner face value
         reconstruct<T, NS, LIN>(wn, Ln, a, to_T<T>(G.dlo[0]), cO, vO, pO);
-        const int p = patch_of(a, 0);
-        int t;
-        double val;
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            bc_of<NS>(P, p, k, t, val);
-            cB[k] = bc_cell<T>(t, T(val), cO[k], -1, a);
-        }
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            bc_of<NS>(P, p, NS + b, t, val);
-            vB[b] = bc_cell<T>(t, T(val), vO[b], b, a);
-        }
-        bc_of<NS>(P, p, NS + 3, t, val);
-        pB = bc_cell<T>(t, T(val), pO, -1, a);
+        ghost_side<T, NS>(P, a, 0, cO, vO, pO, cB, vB, pB);
         sgn = T(-1);
     } else if (!per && f == Na) {
         reconstruct<T, NS, LIN>(wo, Lo, a, to_T<T>(G.dhi[Na - 1]), cO, vO, pO);
-        const int p = patch_of(a, 1);
-        int t;
-        double val;
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            bc_of<NS>(P, p, k, t, val);
-            cB[k] = bc_cell<T>(t, T(val), cO[k], -1, a);
-        }
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            bc_of<NS>(P, p, NS + b, t, val);
-            vB[b] = bc_cell<T>(t, T(val), vO[b], b, a);
-        }
-        bc_of<NS>(P, p, NS + 3, t, val);
-        pB = bc_cell<T>(t, T(val), pO, -1, a);
+        ghost_side<T, NS>(P, a, 1, cO, vO, pO, cB, vB, pB);
+        if (P.hi_flip[a])
+            sgn = T(-1);
     } else {
         const int io = (f == 0) ? Na - 1 : f - 1;  // periodic wrap
         const int in = (f == Na) ? 0 : f;
         reconstruct<T, NS, LIN>(wo, Lo, a, to_T<T>(G.dhi[io]), cO, vO, pO);
         reconstruct<T, NS, LIN>(wn, Ln, a, to_T<T>(G.dlo[in]), cB, vB, pB);
     }
-    Side<T, NS> O, B;
-    close_side<T, NS>(P, cO, vO, pO, O);
-    close_side<T, NS>(P, cB, vB, pB, B);
-    if (P.flux == FLUX_HLL)
-        return hll_flux<T, NS>(O, B, a, sgn, g);
-    return hllc_flux<T, NS>(O, B, a, sgn, g);
+    return riemann<T, NS>(P, a, sgn, cO, vO, pO, cB, vB, pB, g);
 }
 
 // Geometry of one cell: volume (ex ey) ez (mesh.cpp:1291-1293) and face areas.
@@ -265,6 +288,25 @@ template <int NS> __device__ __forceinline__ void load_cons(const double *U, lon
         u[q] = U[q * n + id];
 }
 
+// Primitives of cell id in the flux precision T; the mixed path rounds the
+// conserved values to FP32 first.
+template <class T, int NS, bool MIXED>
+__device__ __forceinline__ void cell_prims(const DevParams &P, const double *U, long long n,
+                                           long long id, T *w)
+{
+    double u[NS + 5];
+    load_cons<NS>(U, n, id, u);
+    if constexpr (MIXED) {
+        T ut[NS + 5];
+#pragma unroll
+        for (int q = 0; q < NS + 5; ++q)
+            ut[q] = T(u[q]);
+        prims_from_cons<T, NS>(P, ut, w);
+    } else {
+        prims_from_cons<T, NS>(P, u, w);
+    }
+}
+
 // Courant signal speed (|v| + c) / min edge of one cell (Schemi.cpp:810-828).
 template <int NS>
 __device__ __forceinline__ double signal_speed(const DevParams &P, const double *u, double minlen)
@@ -281,6 +323,97 @@ __device__ __forceinline__ double signal_speed(const DevParams &P, const double 
     const double c2 = 0. + gamma1<double, NS>(P, w, c0) * (U + p) / u[NS];
     const double cs = __builtin_sqrt(c2);
     return (hypot3(vx, vy, vz) + cs) * minlen;
+}
+
+// Residual and update of one cell (Advection2dOrder.cpp:541-602 with the
+// divergence of divergence.hpp:315-350).  F(s, q) is flux component q of the
+// cell's face s in the order x-, x+, y+, y-, z-, z+ (F & N of the face's own
+// normal); the cell owns its high faces and the low faces on a non-periodic
+// boundary (xlo/ylo/zlo), and is the neighbour of its other low faces.  The
+// contributions (s g) S are summed in that order in FP64.  Stage 2 also forms
+// the SSP-RK2 average with U^n (bunchOfFields.hpp:166-203) and the Courant
+// signal speed of the new state.
+template <int NS, int DIM, bool STAGE2, bool MIXED, class FluxAt>
+__device__ __forceinline__ void update_cell(const DevParams &P, const CellGeo &cg, long long n,
+                                            long long id, const double *Uin, double *Uout,
+                                            const double *Un, double dt, bool xlo, bool ylo,
+                                            bool zlo, const FluxAt &F, double &sig)
+{
+    constexpr int NF = NS + 5;
+    double u[NF];
+    load_cons<NS>(Uin, n, id, u);
+    double unew[NF];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        double acc = 0;
+        acc += (xlo ? F(0, q) : -F(0, q)) * cg.S[0];
+        acc += F(1, q) * cg.S[0];
+        if constexpr (DIM >= 2) {
+            acc += F(2, q) * cg.S[1];
+            acc += (ylo ? F(3, q) : -F(3, q)) * cg.S[1];
+        }
+        if constexpr (DIM == 3) {
+            acc += (zlo ? F(4, q) : -F(4, q)) * cg.S[2];
+            acc += F(5, q) * cg.S[2];
+        }
+        unew[q] = u[q] - (acc / cg.V) * dt;
+    }
+    if (P.gravity) {
+        // pre-update rho and v (Advection2dOrder.cpp:547-554)
+        const double rho = u[NS];
+        const double vx = u[NS + 1] / rho, vy = u[NS + 2] / rho, vz = u[NS + 3] / rho;
+        const double vg = ((0. + vx * P.g[0]) + vy * P.g[1]) + vz * P.g[2];
+        unew[NS + 4] = unew[NS + 4] + rho * vg * dt;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            unew[NS + 1 + b] = unew[NS + 1 + b] + (P.g[b] * rho) * dt;
+    }
+    if constexpr (MIXED) {
+        // species renormalisation: sum_k rho_k = rho
+        double sr = 0;
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+            sr += unew[q];
+        const double f = unew[NS] / sr;
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+            unew[q] *= f;
+    }
+    if constexpr (STAGE2) {
+        double un[NF];
+        load_cons<NS>(Un, n, id, un);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            if (q >= NS + 1 && q <= NS + 3)
+                unew[q] = unew[q] * 0.5 + un[q] * 0.5;
+            else
+                unew[q] = 0.5 * unew[q] + 0.5 * un[q];
+        }
+        sig = tmax(sig, signal_speed<NS>(P, unew, cg.minlen));
+    }
+#pragma unroll
+    for (int q = 0; q < NF; ++q)
+        Uout[q * n + id] = unew[q];
+}
+
+// Wavefront and workgroup maximum of the Courant signal speed, then one
+// atomic per workgroup (positive doubles order like their bit patterns).
+template <int NT>
+__device__ __forceinline__ void reduce_signal(double sig, double *sRed,
+                                              unsigned long long *sigmax)
+{
+    const int tid = threadIdx.x;
+    for (int off = 32; off > 0; off >>= 1)
+        sig = tmax(sig, __shfl_xor(sig, off, 64));
+    if ((tid & 63) == 0)
+        sRed[tid >> 6] = sig;
+    __syncthreads();
+    if (tid == 0) {
+        double m = sRed[0];
+        for (int w = 1; w < NT / 64; ++w)
+            m = tmax(m, sRed[w]);
+        atomicMax(sigmax, (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 // Index helper for the x-y plane tile in LDS with a 2-cell halo.
@@ -341,19 +474,7 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
     auto wrapx = [&](int x) { return (x < 0) ? x + Nx : (x >= Nx ? x - Nx : x); };
     auto wrapy = [&](int y) { return (y < 0) ? y + Ny : (y >= Ny ? y - Ny : y); };
     auto wrapz = [&](int z) { return (z < 0) ? z + Nz : (z >= Nz ? z - Nz : z); };
-    auto prims_at = [&](long long id, T *w) {
-        double u[NF];
-        load_cons<NS>(Uin, n, id, u);
-        if constexpr (MIXED) {
-            T ut[NF];
-#pragma unroll
-            for (int q = 0; q < NF; ++q)
-                ut[q] = T(u[q]);
-            prims_from_cons<T, NS>(P, ut, w);
-        } else {
-            prims_from_cons<T, NS>(P, u, w);
-        }
-    };
+    auto prims_at = [&](long long id, T *w) { cell_prims<T, NS, MIXED>(P, Uin, n, id, w); };
 
     // ---- z-direction register window of the own column (DIM == 3) --------
     T wz[5][NQ];   // primitives at k-2 .. k+2
@@ -579,64 +700,18 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
             const bool xlo = (i == 0) && !P.periodic[0];
             const bool ylo = (DIM >= 2) && (j == 0) && !P.periodic[1];
             const bool zlo = (DIM == 3) && (k == 0) && !zper;
-            double u[NF];
-            load_cons<NS>(Uin, n, id, u);
-            double unew[NF];
-#pragma unroll
-            for (int q = 0; q < NF; ++q) {
-                // contributions (s g) S in the order x-, x+, y+, y-, z-, z+
-                double acc = 0;
-                acc += (xlo ? double(sFx[q][ty][tx]) : -double(sFx[q][ty][tx])) * cg.S[0];
-                acc += double(sFx[q][ty][tx + 1]) * cg.S[0];
-                if constexpr (DIM >= 2) {
-                    acc += double(sFy[q][ty + 1][tx]) * cg.S[1];
-                    acc += (ylo ? double(sFy[q][ty][tx]) : -double(sFy[q][ty][tx])) * cg.S[1];
+            auto F = [&](int s, int q) -> double {
+                switch (s) {
+                case 0: return double(sFx[q][ty][tx]);
+                case 1: return double(sFx[q][ty][tx + 1]);
+                case 2: return double(sFy[q][ty + 1][tx]);
+                case 3: return double(sFy[q][ty][tx]);
+                case 4: return double(fzm[q]);
+                default: return double(fzp[q]);
                 }
-                if constexpr (DIM == 3) {
-                    acc += (zlo ? double(fzm[q]) : -double(fzm[q])) * cg.S[2];
-                    acc += double(fzp[q]) * cg.S[2];
-                }
-                if constexpr (MIXED)
-                    unew[q] = u[q] - (acc / cg.V) * dt;
-                else
-                    unew[q] = u[q] - (acc / cg.V) * dt;
-            }
-            if (P.gravity) {
-                // pre-update rho and v (Advection2dOrder.cpp:547-554)
-                const double rho = u[NS];
-                const double vx = u[NS + 1] / rho, vy = u[NS + 2] / rho, vz = u[NS + 3] / rho;
-                const double vg = ((0. + vx * P.g[0]) + vy * P.g[1]) + vz * P.g[2];
-                unew[NS + 4] = unew[NS + 4] + rho * vg * dt;
-#pragma unroll
-                for (int b = 0; b < 3; ++b)
-                    unew[NS + 1 + b] = unew[NS + 1 + b] + (P.g[b] * rho) * dt;
-            }
-            if constexpr (MIXED) {
-                // species renormalisation: sum_k rho_k = rho
-                double sr = 0;
-#pragma unroll
-                for (int q = 0; q < NS; ++q)
-                    sr += unew[q];
-                const double f = unew[NS] / sr;
-#pragma unroll
-                for (int q = 0; q < NS; ++q)
-                    unew[q] *= f;
-            }
-            if constexpr (STAGE2) {
-                double un[NF];
-                load_cons<NS>(Un, n, id, un);
-#pragma unroll
-                for (int q = 0; q < NF; ++q) {
-                    if (q >= NS + 1 && q <= NS + 3)
-                        unew[q] = unew[q] * 0.5 + un[q] * 0.5;
-                    else
-                        unew[q] = 0.5 * unew[q] + 0.5 * un[q];
-                }
-                sig = tmax(sig, signal_speed<NS>(P, unew, cg.minlen));
-            }
-#pragma unroll
-            for (int q = 0; q < NF; ++q)
-                Uout[q * n + id] = unew[q];
+            };
+            update_cell<NS, DIM, STAGE2, MIXED>(P, cg, n, id, Uin, Uout, Un, dt, xlo, ylo, zlo,
+                                                F, sig);
         }
 
         if constexpr (DIM == 3) {
@@ -656,21 +731,296 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
 
     if (err)
         atomicOr(errflag, err);
-    if constexpr (STAGE2) {
-        // wavefront max of the Courant signal speed, then one atomic per block
-        for (int off = 32; off > 0; off >>= 1)
-            sig = tmax(sig, __shfl_xor(sig, off, 64));
-        if ((tid & 63) == 0)
-            sRed[tid >> 6] = sig;
-        __syncthreads();
-        if (tid == 0) {
-            double m = sRed[0];
-            for (int w = 1; w < NT / 64; ++w)
-                m = tmax(m, sRed[w]);
-            // positive doubles order like their bit patterns
-            atomicMax(sigmax, (unsigned long long)__double_as_longlong(m));
+    if constexpr (STAGE2)
+        reduce_signal<NT>(sig, sRed, sigmax);
+}
+
+// Cell state closed with the EOS once per cell, for schemes whose face
+// values are the cell values: primitives plus the closure terms that need a
+// division or a square root (close_side); the partial densities c_k M_k and
+// the momentum v r0 are re-formed per face with the same products.
+template <class T, int NS> struct Closed {
+    static constexpr int NV = NS + 8;  // c[NS], v[3], p, r0, U, E, cs
+};
+
+template <class T, int NS>
+__device__ __forceinline__ void close_cell(const DevParams &P, const T *w, T *z)
+{
+    Side<T, NS> s;
+    close_side<T, NS>(P, w, w + NS, w[NS + 3], s);
+#pragma unroll
+    for (int q = 0; q < NS + 4; ++q)
+        z[q] = w[q];
+    z[NS + 4] = s.r0;
+    z[NS + 5] = s.U;
+    z[NS + 6] = s.E;
+    z[NS + 7] = s.cs;
+}
+
+template <class T, int NS>
+__device__ __forceinline__ void side_of(const DevParams &P, const T *z, Side<T, NS> &s)
+{
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        s.rho[k] = z[k] * mixM<T>(P, k);
+    s.r0 = z[NS + 4];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s.v[a] = z[NS + a];
+        s.m[a] = z[NS + a] * s.r0;
+    }
+    s.p = z[NS + 3];
+    s.U = z[NS + 5];
+    s.E = z[NS + 6];
+    s.cs = z[NS + 7];
+}
+
+// Flux of face f along axis a from the closed states of cells f-1 (zo) and f
+// (zn); on a non-periodic boundary only the owner's state is read and the
+// ghost is formed from its primitives (Advection2dOrder.cpp:398-521).
+template <class T, int NS>
+__device__ __forceinline__ bool face_flux_closed(const DevParams &P, int a, int f, const T *zo,
+                                                 const T *zn, T *g)
+{
+    const int Na = P.N[a];
+    const bool per = P.periodic[a] != 0;
+    Side<T, NS> O, B;
+    if (!per && (f == 0 || f == Na)) {
+        const int side = (f == 0) ? 0 : 1;
+        T z[Closed<T, NS>::NV];  // element-wise select keeps both in registers
+#pragma unroll
+        for (int q = 0; q < Closed<T, NS>::NV; ++q)
+            z[q] = side ? zo[q] : zn[q];
+        T cB[NS], vB[3], pB;
+        ghost_side<T, NS>(P, a, side, z, z + NS, z[NS + 3], cB, vB, pB);
+        side_of<T, NS>(P, z, O);
+        close_side<T, NS>(P, cB, vB, pB, B);
+        const T sg = (side && !P.hi_flip[a]) ? T(1) : T(-1);
+        if (P.flux == FLUX_HLL)
+            return hll_flux<T, NS>(O, B, a, sg, g);
+        return hllc_flux<T, NS>(O, B, a, sg, g);
+    }
+    side_of<T, NS>(P, zo, O);
+    side_of<T, NS>(P, zn, B);
+    if (P.flux == FLUX_HLL)
+        return hll_flux<T, NS>(O, B, a, T(1), g);
+    return hllc_flux<T, NS>(O, B, a, T(1), g);
+}
+
+// Stage kernel for schemes whose limited slopes vanish identically (zero
+// limiter; TVD limiters on 2-D and 3-D meshes: the reference's ratio along an
+// axis takes its min/max over all neighbours of the cell, and the neighbours
+// across the other axes contribute exact zero entries, so r <= 0 and
+// phi (r) = 0 for minmod, van Leer and van Albada, TVDLimiter.cpp:12-125,
+// minmodLimiter.cpp:14-57, vanLeerLimiter.cpp:14-53,
+// vanAlbadaLimiter.cpp:15-55).  Every face value is then the cell value
+// ((phi G) . d + q = q), so each cell is closed with the EOS once, its closed
+// state staged in LDS with a one-cell halo, and no slopes are stored.  Same
+// z-marching tile and update as stage_kernel.
+template <class T, int NS, int DIM, bool STAGE2, bool MIXED, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY)
+stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
+              const double *__restrict__ Uin, double *Uout, const double *Un,
+              const double *__restrict__ dt_ptr, unsigned long long *sigmax, int *errflag,
+              int kchunk)
+{
+    constexpr int NQ = NS + 4;
+    constexpr int NV = Closed<T, NS>::NV;
+    constexpr int NF = NS + 5;
+    constexpr int NT = TX * TY;
+    constexpr int BX = TX + 2;
+    constexpr int BY = (DIM >= 2) ? TY + 2 : 1;
+    constexpr int OY = (DIM >= 2) ? 1 : 0;
+    constexpr int FYH = (DIM >= 2) ? TY + 1 : 1;
+
+    __shared__ T sZ[NV][BY][BX];
+    __shared__ T sFx[NF][TY][TX + 1];
+    __shared__ T sFy[(DIM >= 2) ? NF : 1][FYH][TX];
+    __shared__ double sRed[NT / 64];
+
+    const DevParams &P = *Pp;
+    if (dt_ptr[6] != 0.0)
+        return;
+    const int tid = threadIdx.x;
+    const int tx = tid % TX, ty = tid / TX;
+    const int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
+    const long long n = P.n;
+    const long long sy = P.stride[1], sz = P.stride[2];
+    const int x0 = blockIdx.x * TX;
+    const int y0 = (DIM >= 2) ? blockIdx.y * TY : 0;
+    const int kb = (DIM == 3) ? blockIdx.z * kchunk : 0;
+    const int ke = (DIM == 3) ? min(kb + kchunk, Nz) : 1;
+    const int i = x0 + tx, j = y0 + ty;
+    const bool act = (i < Nx) && (j < Ny);
+    const double dt = *dt_ptr;
+    int err = 0;
+    double sig = 0.0;
+    const bool zper = (DIM == 3) && P.periodic[2] != 0;
+
+    auto closed_at = [&](long long id, T *z) {
+        T w[NQ];
+        cell_prims<T, NS, MIXED>(P, Uin, n, id, w);
+        close_cell<T, NS>(P, w, z);
+    };
+
+    // own column: closed states of planes k and k+1, fluxes of its z faces
+    T zc[NV], znx[NV];
+    T fzm[NF], fzp[NF];
+    if constexpr (DIM == 3) {
+        if (act) {
+            closed_at(i + sy * j + sz * kb, zc);
+            if (kb > 0 || zper) {
+                T zm[NV];
+                const int km = (kb > 0) ? kb - 1 : Nz - 1;
+                closed_at(i + sy * j + sz * km, zm);
+                if (!face_flux_closed<T, NS>(P, 2, kb, zm, zc, fzm))
+                    err |= ERRF_RIEMANN;
+            } else if (!face_flux_closed<T, NS>(P, 2, 0, zc, zc, fzm)) {
+                err |= ERRF_RIEMANN;
+            }
         }
     }
+
+    for (int k = kb; k < ke; ++k) {
+        // ---- 1. closed states of plane k (tile + one-cell halo) ----------
+        if (act) {
+            if constexpr (DIM == 3) {
+                if ((k + 1 < Nz) || zper) {
+                    const int kp = (k + 1 < Nz) ? k + 1 : 0;
+                    closed_at(i + sy * j + sz * kp, znx);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NV; ++q)
+                        znx[q] = zc[q];  // not read: the top face is a boundary face
+                }
+                if (!face_flux_closed<T, NS>(P, 2, k + 1, zc, znx, fzp))
+                    err |= ERRF_RIEMANN;
+            } else {
+                closed_at(i + sy * j, zc);
+            }
+#pragma unroll
+            for (int q = 0; q < NV; ++q)
+                sZ[q][ty + OY][tx + 1] = zc[q];
+        }
+        {
+            constexpr int NHX = 2 * TY;
+            constexpr int NH = NHX + ((DIM >= 2) ? 2 * TX : 0);
+            for (int h = tid; h < NH; h += NT) {
+                int bxc, byc;
+                if (h < NHX) {
+                    byc = h / 2 + OY;
+                    bxc = (h % 2) ? TX + 1 : 0;
+                } else {
+                    const int hh = h - NHX;
+                    bxc = hh % TX + 1;
+                    byc = (hh / TX) ? TY + 1 : 0;
+                }
+                int gx = x0 - 1 + bxc;
+                int gy = y0 - OY + byc;
+                // x: the halo columns wrap on a periodic axis; y: the halo rows
+                // (h >= NHX) wrap, the tile rows of the x halo never do
+                bool okx = true, oky = true;
+                if (gx < 0 || gx >= Nx) {
+                    if (P.periodic[0] && (gx == -1 || gx == Nx))
+                        gx = (gx < 0) ? Nx - 1 : 0;
+                    else
+                        okx = false;
+                }
+                if (gy < 0 || gy >= Ny) {
+                    if (DIM >= 2 && h >= NHX && P.periodic[1] && (gy == -1 || gy == Ny))
+                        gy = (gy < 0) ? Ny - 1 : 0;
+                    else
+                        oky = false;
+                }
+                if (okx && oky) {
+                    T z[NV];
+                    closed_at(gx + sy * gy + ((DIM == 3) ? sz * (long long)k : 0), z);
+#pragma unroll
+                    for (int q = 0; q < NV; ++q)
+                        sZ[q][byc][bxc] = z[q];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- 2. x and y face fluxes ------------------------------------
+        {
+            constexpr int NTX = (TX + 1) * TY;
+            constexpr int NTY = (DIM >= 2) ? TX * FYH : 0;
+            for (int t = tid; t < NTX + NTY; t += NT) {
+                T g[NF];
+                T zo[NV], zn[NV];
+                if (t < NTX) {
+                    const int u = t % (TX + 1), r = t / (TX + 1);
+                    const int fx = x0 + u;
+                    if (fx > Nx || y0 + r >= Ny)
+                        continue;
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) {
+                        zo[q] = sZ[q][r + OY][u];
+                        zn[q] = sZ[q][r + OY][u + 1];
+                    }
+                    if (!face_flux_closed<T, NS>(P, 0, fx, zo, zn, g))
+                        err |= ERRF_RIEMANN;
+#pragma unroll
+                    for (int q = 0; q < NF; ++q)
+                        sFx[q][r][u] = g[q];
+                } else if constexpr (DIM >= 2) {
+                    const int tt = t - NTX;
+                    const int c = tt % TX, v = tt / TX;
+                    const int fy = y0 + v;
+                    if (fy > Ny || x0 + c >= Nx)
+                        continue;
+#pragma unroll
+                    for (int q = 0; q < NV; ++q) {
+                        zo[q] = sZ[q][v][c + 1];
+                        zn[q] = sZ[q][v + 1][c + 1];
+                    }
+                    if (!face_flux_closed<T, NS>(P, 1, fy, zo, zn, g))
+                        err |= ERRF_RIEMANN;
+#pragma unroll
+                    for (int q = 0; q < NF; ++q)
+                        sFy[q][v][c] = g[q];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- 3. residual and update of the own cell ---------------------
+        if (act) {
+            const int kk = (DIM == 3) ? k : 0;
+            const long long id = i + sy * j + sz * kk;
+            const CellGeo cg = cell_geo(P, geo, i, (DIM >= 2) ? j : 0, kk);
+            const bool xlo = (i == 0) && !P.periodic[0];
+            const bool ylo = (DIM >= 2) && (j == 0) && !P.periodic[1];
+            const bool zlo = (DIM == 3) && (k == 0) && !zper;
+            auto F = [&](int s, int q) -> double {
+                switch (s) {
+                case 0: return double(sFx[q][ty][tx]);
+                case 1: return double(sFx[q][ty][tx + 1]);
+                case 2: return double(sFy[q][ty + 1][tx]);
+                case 3: return double(sFy[q][ty][tx]);
+                case 4: return double(fzm[q]);
+                default: return double(fzp[q]);
+                }
+            };
+            update_cell<NS, DIM, STAGE2, MIXED>(P, cg, n, id, Uin, Uout, Un, dt, xlo, ylo, zlo,
+                                                F, sig);
+        }
+        if constexpr (DIM == 3) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q)
+                zc[q] = znx[q];
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                fzm[q] = fzp[q];
+        }
+    }
+
+    if (err)
+        atomicOr(errflag, err);
+    if constexpr (STAGE2)
+        reduce_signal<NT>(sig, sRed, sigmax);
 }
 
 }  // namespace sg

@@ -157,6 +157,22 @@ MeshTables build_tables(int dim, const int64_t n[3], const double extent[3],
         }
     };
     static const int LLHH[4] = {0, 0, 1, 1}, LHLH[4] = {0, 1, 0, 1};
+    // transverse offsets per face class, from the vertex patterns below
+    auto offsets = [&](int fa, int b, int cls, const int pat[4]) {
+        const AxisTables &Tb = M.ax[b];
+        std::vector<double> &v = M.td[fa][b][cls];
+        v.resize(Tb.N);
+        for (int t = 0; t < Tb.N; ++t)
+            v[t] = face_centre(pat, A[b].vert(t), A[b].vert(t + 1)) - Tb.xc[t];
+    };
+    for (int cls = 0; cls < 2; ++cls) {
+        offsets(0, 1, cls, LLHH);
+        offsets(0, 2, cls, LHLH);
+        offsets(1, 0, cls, cls ? LLHH : LHLH);
+        offsets(1, 2, cls, cls ? LHLH : LLHH);
+        offsets(2, 0, cls, LHLH);
+        offsets(2, 1, cls, LLHH);
+    }
     // x faces: y [lo,lo,hi,hi], z [lo,hi,lo,hi] in every dimension
     check(0, 1, LLHH, "x");
     check(0, 2, LHLH, "x");

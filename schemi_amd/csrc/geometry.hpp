@@ -35,6 +35,11 @@ struct MeshTables {
     AxisTables ax[3];
     bool separable = true;  // face centres share the cells' transverse coordinates
     std::string why;        // first transverse mismatch found
+    // Offset of a face centre from its cell's centre across the face normal:
+    // td[a][b][cls][t] for faces normal to axis a, along transverse axis b
+    // (b != a), cell index t along b; cls 0 = inner faces, 1 = faces on the
+    // two boundary planes of axis a.  Exactly zero on separable meshes.
+    std::vector<double> td[3][3][2];
 };
 
 double hypot3_host(double x, double y, double z);

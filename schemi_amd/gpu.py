@@ -181,7 +181,8 @@ def face_permutation(case: Case) -> np.ndarray:
     return out
 
 
-TABLES = ("xc", "e", "xf", "wo", "wn", "dn", "dlo", "dhi", "ghost", "separable")
+TABLES = ("xc", "e", "xf", "wo", "wn", "dn", "dlo", "dhi", "ghost", "separable",
+          "td_x_inner", "td_x_bnd", "td_y_inner", "td_y_bnd", "td_z_inner", "td_z_bnd")
 
 
 def axis_table(case: Case, axis: int, which: str, periodic_mask: int = 0) -> np.ndarray:

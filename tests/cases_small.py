@@ -61,4 +61,32 @@ def cases():
     periodic(c, 1)
     periodic(c, 2)
     out["3d_periodic_yz_linear"] = c
+    # the linear limiter on separable (dyadic) meshes uses the fused slope
+    # kernel; the non-dyadic ones above use the general-geometry kernels
+    c = Case(dim=2, N=(32, 32, 1), L=(0.25, 1.0, 0.125), limiter=TL.linear, flux=FS.HLL, **BASE)
+    c.set_bc(3, BCT.slip)
+    out["2d_linear_dyadic"] = c
+    c = Case(dim=3, N=(32, 16, 16), L=(2.0, 1.0, 1.0), limiter=TL.linear, **BASE)
+    for pt in (2, 3, 4, 5):
+        c.set_bc(pt, BCT.slip)
+    out["3d_linear_dyadic"] = c
+    c = Case(dim=2, N=(24, 16, 1), L=(0.7, 0.5, 0.1), limiter=TL.linear, **BASE)
+    periodic(c, 1)
+    out["2d_periodic_y_linear"] = c
+    c = Case(dim=2, N=(40, 24, 1), L=(0.3, 0.7, 0.1), limiter=TL.zero, **BASE)
+    c.set_bc(0, BCT.fixedValueCell, "p", (1.2, 0, 0))
+    out["2d_zero_fixed"] = c
+    c = Case(dim=3, N=(64, 16, 8), L=(1.0, 0.5, 0.25), limiter=TL.vanLeer, flux=FS.HLL, **BASE)
+    periodic(c, 0)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    out["3d_periodic_x_vanLeer"] = c
+    # Lz != Ly in 3-D (z spaced by Ly / Nz): reversed top face normal (Lz < Ly)
+    # for the fused linear kernel, and a top face above the last cell (Lz > Ly)
+    # for the general-geometry kernels
+    c = Case(dim=3, N=(32, 8, 8), L=(1.0, 0.25, 0.125), limiter=TL.linear, **BASE)
+    c.set_bc(5, BCT.slip)
+    out["3d_lz_short_linear_dyadic"] = c
+    c = Case(dim=3, N=(10, 6, 6), L=(1.0, 0.6, 0.65), limiter=TL.linear, flux=FS.HLL, **BASE)
+    out["3d_lz_long_linear"] = c
     return out

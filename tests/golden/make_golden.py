@@ -170,7 +170,24 @@ def main():
     run_const("seeded_3d_rm_vanalbada", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 15))
     c = Case(dim=3, N=(8, 6, 6), L=(1.0, 0.5, 0.5), limiter=TL.linear, **base)
     run_const("seeded_3d_linear", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 16))
+    lz_cases(base)
+
+
+def lz_cases(base):
+    """3-D meshes with Lz != Ly: the reference spaces z by Ly / Nz, so with
+    Lz < Ly the top face plane lies below the last cell centre (its normal
+    is reversed) and with Lz > Ly above the last cell."""
+    c = Case(dim=3, N=(8, 6, 6), L=(1.0, 0.5, 0.25), limiter=TL.vanLeer, **base)
+    c.set_bc(5, BCT.slip)
+    run_const("seeded_3d_lz_short_vanleer", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 17))
+    c = Case(dim=3, N=(8, 4, 4), L=(1.0, 0.5, 0.25), limiter=TL.linear, flux=FS.HLL, **base)
+    run_const("seeded_3d_lz_short_linear", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 18))
+    c = Case(dim=3, N=(8, 4, 4), L=(1.0, 0.5, 0.55), limiter=TL.linear, **base)
+    run_const("seeded_3d_lz_long_linear", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 19))
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "lz":
+        lz_cases(dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67]))
+    else:
+        main()

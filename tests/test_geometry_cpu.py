@@ -21,6 +21,11 @@ MESHES = {
     "2d_nondyadic": Case(dim=2, N=(12, 10, 1), L=(0.3, 0.7, 0.1), **BASE),
     "3d_rm": Case(dim=3, N=(32, 16, 16), L=(2.0, 1.0, 1.0), **BASE),
     "3d_odd": Case(dim=3, N=(7, 5, 6), L=(0.7, 0.3, 0.3), **BASE),
+    "3d_nondyadic": Case(dim=3, N=(9, 7, 5), L=(2.0, 1.0, 1.0), **BASE),
+    # the reference spaces z by Ly / Nz in 3-D; with Lz < Ly the top face
+    # plane (at Lz) lies below the last cell centre and its normal is -z
+    "3d_lz_short": Case(dim=3, N=(5, 4, 6), L=(0.7, 0.5, 0.25), **BASE),
+    "3d_lz_long": Case(dim=3, N=(5, 4, 3), L=(0.7, 0.4, 0.9), **BASE),
 }
 
 
@@ -57,6 +62,7 @@ def test_faces_match_oracle(name):
     inv[perm] = np.arange(nf)
     assert np.array_equal(perm[inv], np.arange(nf))
     T = tables(case)
+    dhi = [axis_table(case, a, "dhi") for a in range(3)]
     Nx, Ny, Nz = case.N
     nfx = (Nx + 1) * Ny * Nz
     nfy = Nx * (Ny + 1) * Nz if case.dim >= 2 else 0
@@ -88,13 +94,61 @@ def test_faces_match_oracle(name):
         assert fg["rc"][f, a] == T[a]["xf"][fi]
         assert fg["S"][f] == S
         n = fg["N"][f]
-        expect = -1.0 if fi == 0 else 1.0
+        # normals point away from the owner centre: the high boundary face
+        # is reversed when it lies below the last cell centre
+        flipped = fi == Na and dhi[a][Na - 1] < 0
+        expect = -1.0 if (fi == 0 or flipped) else 1.0
         assert n[a] == expect and np.all(np.abs(np.delete(n, a)) == 0.0)
         if 0 < fi < Na and (separable or case.dim == 1):
             assert fg["wo"][f] == T[a]["wo"][fi] and fg["wn"][f] == T[a]["wn"][fi]
+
+
+def test_reversed_top_face_only_when_lz_below_last_centre():
+    assert axis_table(MESHES["3d_lz_short"], 2, "dhi")[-1] < 0
+    assert axis_table(MESHES["3d_lz_long"], 2, "dhi")[-1] > 0
+    assert axis_table(MESHES["3d_rm"], 2, "dhi")[-1] > 0
 
 
 def test_separability_flags():
     assert axis_table(MESHES["3d_rm"], 0, "separable")[0] == 1.0
     assert axis_table(MESHES["2d_rt"], 0, "separable")[0] == 1.0
     assert axis_table(MESHES["1d_odd"], 0, "separable")[0] == 1.0
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+def test_face_offsets_match_oracle(name):
+    """Full face-centre minus cell-centre vectors (the reconstruction offsets
+    of Advection2dOrder.cpp:110-137) from the per-axis tables: the offset
+    along the face normal from x_f - x_c, the transverse ones from the
+    per-class offset tables.  Compared with the oracle's literal mesh on
+    every cell and face, including non-dyadic spacings where the transverse
+    offsets are rounding-sized but non-zero."""
+    case = MESHES[name]
+    o = Oracle(case)
+    rc, _ = o.cell_geometry()
+    fg = o.face_geometry()
+    cf = o.cell_faces()
+    Nx, Ny, Nz = case.N
+    dlo = [axis_table(case, a, "dlo") for a in range(3)]
+    dhi = [axis_table(case, a, "dhi") for a in range(3)]
+    names = ("x", "y", "z")
+    td = {(a, b, cls): axis_table(case, a, f"td_{names[b]}_{'bnd' if cls else 'inner'}")
+          for a in range(3) for b in range(3) if a != b for cls in (0, 1)}
+    slots = [(0, 0), (0, 1), (1, 1), (1, 0), (2, 0), (2, 1)]  # x-, x+, y+, y-, z-, z+
+    nonzero = 0
+    for cid in range(case.ncells):
+        idx = (cid % Nx, (cid // Nx) % Ny, cid // (Nx * Ny))
+        for s, (a, hi) in enumerate(slots[:2 * case.dim]):
+            f = cf[cid, s]
+            expect = fg["rc"][f] - rc[cid]
+            fi = idx[a] + hi
+            cls = 1 if fi in (0, case.N[a]) else 0
+            got = np.zeros(3)
+            got[a] = dhi[a][idx[a]] if hi else dlo[a][idx[a]]
+            for b in range(3):
+                if b != a:
+                    got[b] = td[(a, b, cls)][idx[b]]
+            assert np.array_equal(got, expect), (name, cid, s, got, expect)
+            nonzero += int(np.count_nonzero(np.delete(expect, a)))
+    if name in ("2d_nondyadic", "3d_nondyadic"):
+        assert nonzero > 0  # the case exercises non-zero transverse offsets

@@ -1,16 +1,27 @@
-"""FP64 GPU step against the oracle, bit for bit (conserved-state mode)."""
+"""FP64 GPU step against the oracle, bit for bit (conserved-state mode).
+
+The library picks one of three kernel families per context: the first-order
+kernel when the limited slopes vanish identically (zero limiter; TVD limiters
+in 2-D and 3-D), the fused slope kernel (TVD limiters in 1-D, the linear
+limiter on separable meshes) and the general-geometry kernels (the linear
+limiter with rounding-sized transverse face offsets).  The cases cover all
+three; the multi-dimensional TVD cases also run with the limiter evaluated in
+full (SCHEMI_GPU_FULL_LIMITER=1), which must give the same bits."""
 import numpy as np
 import pytest
 
+from schemi_amd import typeOfTVDLimiter as TL
 from schemi_amd.gpu import GpuSolver, conserved_from_primitive
 from tests.cases_small import cases, random_primitives
 
 CASES = cases()
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("name", sorted(CASES))
-def test_fp64_step_bitexact(name, hip_device):
+TVD_MULTI_D = sorted(n for n, c in CASES.items()
+                     if c.dim >= 2 and c.limiter in (TL.minmod, TL.vanLeer, TL.vanAlbada))
+
+
+def run_parity(name):
     from oracle.oracle import Oracle
 
     case = CASES[name]
@@ -30,3 +41,18 @@ def test_fp64_step_bitexact(name, hip_device):
         diff = np.abs(got - ref)
         print(name, s, "max abs diff", diff.max(), "n differing", int((got != ref).sum()))
         assert np.array_equal(got, ref), f"{name}: step {s + 1} differs"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_step_bitexact(name, hip_device):
+    run_parity(name)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", TVD_MULTI_D)
+def test_fp64_full_limiter_bitexact(name, hip_device, monkeypatch):
+    """The fused slope kernel evaluating the TVD limiter everywhere gives the
+    same bits as the first-order kernel (and the oracle)."""
+    monkeypatch.setenv("SCHEMI_GPU_FULL_LIMITER", "1")
+    run_parity(name)

@@ -201,11 +201,56 @@ def test_full_size_invariants(hip_device):
 
 
 @pytest.mark.gpu
-def test_linear_limiter_nonseparable_mesh_rejected(hip_device):
-    """The linear limiter on a multi-D mesh whose face centres do not share
-    the cells' transverse coordinates is declined (meshGenerationError)."""
-    case = Case(dim=2, N=(40, 24, 1), L=(0.25, 1.0, 0.1), limiter=TL.linear,
-                M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
+def test_multi_d_mesh_needs_two_cells_per_axis(hip_device):
+    """2-D and 3-D meshes need at least two cells along every axis
+    (meshGenerationError otherwise).  With that, every cell has a neighbour
+    across each other axis, so the TVD ratio always has its exact zero entry
+    and the multi-dimensional TVD slopes vanish identically."""
+    base = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
+    for lim in (TL.vanLeer, TL.linear):
+        case = Case(dim=2, N=(40, 1, 1), L=(0.3, 0.7, 0.1), limiter=lim, **base)
+        with pytest.raises(SchemiGpuError) as e:
+            GpuSolver(case)
+        assert e.value.code == errors.meshGenerationError
+    GpuSolver(Case(dim=2, N=(40, 2, 1), L=(0.3, 0.7, 0.1), limiter=TL.vanLeer, **base)).close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["1d_vanLeer", "2d_linear", "2d_linear_dyadic", "3d_vanAlbada",
+                                  "3d_periodic_yz_linear"])
+def test_mixed_precision_all_kernels(name, hip_device):
+    """The mixed path of every kernel family stays within 1e-5 (relative L1
+    of the density) of the FP64 path over 3 steps and keeps sum rho_k = rho."""
+    from tests.cases_small import cases
+
+    case = cases()[name]
+    c, v, p = random_primitives(case, 11)
+    U0 = conserved_from_primitive(case, c, v, p)
+    out = {}
+    for prec in (PREC_FP64, PREC_MIXED):
+        s = GpuSolver(case, prec)
+        s.set_conserved(U0)
+        for _ in range(3):
+            s.step(2e-4)
+        out[prec] = s.conserved()
+        s.close()
+    a, b = out[PREC_MIXED], out[PREC_FP64]
+    Ns = case.Ns
+    l1 = np.abs(a[Ns] - b[Ns]).sum() / np.abs(b[Ns]).sum()
+    assert l1 < 1e-5, l1
+    assert np.max(np.abs(a[:Ns].sum(0) - a[Ns]) / a[Ns]) < 1e-14
+
+
+@pytest.mark.gpu
+def test_periodic_axis_with_reversed_face_rejected(hip_device):
+    """Periodic z on a 3-D mesh whose top face plane lies below the last cell
+    centre (Lz < Ly; the reference spaces z by Ly / Nz) has no consistent
+    pairing: declined with meshGenerationError."""
+    from schemi_amd.gpu import periodic
+
+    base = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
+    case = Case(dim=3, N=(32, 8, 8), L=(1.0, 0.5, 0.25), **base)
+    periodic(case, 2)
     with pytest.raises(SchemiGpuError) as e:
         GpuSolver(case)
     assert e.value.code == errors.meshGenerationError

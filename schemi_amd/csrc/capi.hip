@@ -20,6 +20,11 @@
 
 using namespace sg;
 
+// z planes per workgroup of the 3-D stage kernels
+#ifndef SG_KCHUNK
+#define SG_KCHUNK 16
+#endif
+
 namespace {
 
 // device scalar block
@@ -360,13 +365,6 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         periodic[a] = nper != 0;
         P.periodic[a] = periodic[a] ? 1 : 0;
     }
-    // periodic axes need whole tiles so the wrapped neighbours are staged
-    const int tile[3] = {dim == 1 ? 256 : 32, 8, 1};
-    for (int a = 0; a < 2; ++a)
-        if (periodic[a] && (P.N[a] % tile[a]) != 0) {
-            delete c;
-            return SCHEMI_ERR_INIT;
-        }
     for (int a = 0; a < 3; ++a)
         P.zero_entry[a] = (dim >= 2) ? 1 : 0;
 
@@ -400,6 +398,18 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         c->mode = force_slopes ? MODE_SLOPES : MODE_FIRST_ORDER;
     else
         c->mode = MODE_SLOPES;
+    // periodic axes need whole tiles so the wrapped neighbours are staged
+    {
+        const bool fo = c->mode == MODE_FIRST_ORDER;
+        const int tile[2] = {dim == 1 ? TileShape<1>::TX
+                                      : (fo ? TileShape<2, true>::TX : TileShape<2>::TX),
+                             fo ? TileShape<2, true>::TY : TileShape<2>::TY};
+        for (int a = 0; a < 2 && a < dim; ++a)
+            if (c->mode != MODE_GENERAL && periodic[a] && (P.N[a] % tile[a]) != 0) {
+                delete c;
+                return SCHEMI_ERR_INIT;
+            }
+    }
     // flatten the per-axis tables
     std::vector<double> geo;
     auto put = [&](const std::vector<double> &v) {
@@ -424,7 +434,7 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         for (int b = 0; b < 3; ++b)
             for (int cls = 0; cls < 2; ++cls)
                 P.off_td[a][b][cls] = (a == b) ? 0 : put(c->tables.td[a][b][cls]);
-    c->kchunk = 16;
+    c->kchunk = SG_KCHUNK;
 
     if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
         delete c;

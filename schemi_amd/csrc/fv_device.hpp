@@ -77,16 +77,15 @@ template <> struct Real<float> {
 };
 
 // libstdc++ 11 std::hypot(x, y, z), used by vector::mag (vector.cpp:37-43).
-__device__ inline double hypot3(double x, double y, double z)
+template <class T> __device__ inline T hypot3(T x, T y, T z)
 {
     x = __builtin_fabs(x);
     y = __builtin_fabs(y);
     z = __builtin_fabs(z);
-    const double a = x < y ? (y < z ? z : y) : (x < z ? z : x);
-    if (a != 0.0)
-        return a * __builtin_sqrt((x / a) * (x / a) + (y / a) * (y / a) +
-                                  (z / a) * (z / a));
-    return 0.0;
+    const T a = x < y ? (y < z ? z : y) : (x < z ? z : x);
+    if (a != T(0))
+        return a * Real<T>::sqrt_((x / a) * (x / a) + (y / a) * (y / a) + (z / a) * (z / a));
+    return T(0);
 }
 
 template <class T> __device__ __forceinline__ T mixM(const DevParams &P, int k);

@@ -160,7 +160,11 @@ gen_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
         const bool ylo = (DIM >= 2) && (idx[1] == 0) && !P.periodic[1];
         const bool zlo = (DIM == 3) && (idx[2] == 0) && !P.periodic[2];
         auto F = [&](int s, int q) -> double { return double(g[s][q]); };
-        update_cell<NS, DIM, STAGE2, MIXED>(P, cg, n, id, Uin, Uout, Un, dt, xlo, ylo, zlo, F,
+        double u[NF], un[NF];
+        load_cons<NS>(Uin, n, id, u);
+        if constexpr (STAGE2)
+            load_cons<NS>(Un, n, id, un);
+        update_cell<NS, DIM, STAGE2, MIXED>(P, cg, n, id, u, Uout, un, dt, xlo, ylo, zlo, F,
                                             sig);
     } else {
         err = 0;

@@ -7,6 +7,25 @@
 
 namespace sg {
 
+// Tile of columns per workgroup in 2-D/3-D (x fastest) and in 1-D, for the
+// fused slope kernel (SG_TILE_*) and the first-order kernel (SG_TILE0_*).
+#ifndef SG_TILE_X
+#define SG_TILE_X 32
+#endif
+#ifndef SG_TILE_Y
+#define SG_TILE_Y 8
+#endif
+#ifndef SG_TILE0_X
+#define SG_TILE0_X 64
+#endif
+#ifndef SG_TILE0_Y
+#define SG_TILE0_Y 8
+#endif
+template <int DIM, bool FIRST_ORDER = false> struct TileShape {
+    static constexpr int TX = (DIM == 1) ? 256 : (FIRST_ORDER ? SG_TILE0_X : SG_TILE_X);
+    static constexpr int TY = (DIM == 1) ? 1 : (FIRST_ORDER ? SG_TILE0_Y : SG_TILE_Y);
+};
+
 // Kernel family of a context.
 enum : int {
     MODE_FIRST_ORDER = 0,  // slopes vanish identically: stage0_kernel

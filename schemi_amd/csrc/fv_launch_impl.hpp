@@ -8,14 +8,9 @@
 
 namespace sg {
 
-template <int DIM> struct TileShape {
-    static constexpr int TX = (DIM == 1) ? 256 : 32;
-    static constexpr int TY = (DIM == 1) ? 1 : 8;
-};
-
-template <int DIM> static dim3 tile_grid(const StageLaunch &L)
+template <int DIM, bool FO = false> static dim3 tile_grid(const StageLaunch &L)
 {
-    constexpr int TX = TileShape<DIM>::TX, TY = TileShape<DIM>::TY;
+    constexpr int TX = TileShape<DIM, FO>::TX, TY = TileShape<DIM, FO>::TY;
     return dim3((L.nx + TX - 1) / TX, (DIM >= 2) ? (L.ny + TY - 1) / TY : 1,
                 (DIM == 3) ? (L.nz + L.kchunk - 1) / L.kchunk : 1);
 }
@@ -24,11 +19,12 @@ template <class T, int NS, int DIM, bool S2, bool MIXED>
 static int launch_mode(const StageLaunch &L)
 {
     constexpr int TX = TileShape<DIM>::TX, TY = TileShape<DIM>::TY;
+    constexpr int TX0 = TileShape<DIM, true>::TX, TY0 = TileShape<DIM, true>::TY;
     switch (L.mode) {
     case MODE_FIRST_ORDER:
-        hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, S2, MIXED, TX, TY>), tile_grid<DIM>(L),
-                           dim3(TX * TY), 0, L.stream, L.P, L.geo, L.Uin, L.Uout, L.Un, L.dt,
-                           L.sigmax, L.err, L.kchunk);
+        hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, S2, MIXED, TX0, TY0>),
+                           (tile_grid<DIM, true>(L)), dim3(TX0 * TY0), 0, L.stream, L.P, L.geo,
+                           L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
         break;
     case MODE_SLOPES:
         if (L.lin)

@@ -264,7 +264,7 @@ __device__ __forceinline__ bool face_flux(const DevParams &P, const AxisGeo &G, 
 
 // Geometry of one cell: volume (ex ey) ez (mesh.cpp:1291-1293) and face areas.
 struct CellGeo {
-    double V, S[3], minlen;
+    double V, S[3], e[3], minlen;
 };
 
 __device__ __forceinline__ CellGeo cell_geo(const DevParams &P, const double *geo, int i, int j, int k)
@@ -277,6 +277,9 @@ __device__ __forceinline__ CellGeo cell_geo(const DevParams &P, const double *ge
     c.S[0] = ey * ez;  // |rX0 - r00| |r0Y - r00| of an x-face
     c.S[1] = ex * ez;
     c.S[2] = ex * ey;
+    c.e[0] = ex;
+    c.e[1] = ey;
+    c.e[2] = ez;
     c.minlen = 1. / tmin(tmin(ex, ey), ez);  // Schemi.cpp:649-662
     return c;
 }
@@ -307,22 +310,23 @@ __device__ __forceinline__ void cell_prims(const DevParams &P, const double *U, 
     }
 }
 
-// Courant signal speed (|v| + c) / min edge of one cell (Schemi.cpp:810-828).
-template <int NS>
-__device__ __forceinline__ double signal_speed(const DevParams &P, const double *u, double minlen)
+// Courant signal speed (|v| + c) / min edge of one cell (Schemi.cpp:810-828),
+// in precision T (FP64 on the parity path).
+template <class T, int NS>
+__device__ __forceinline__ T signal_speed(const DevParams &P, const T *u, T minlen)
 {
-    double w[NS + 4];
-    prims_from_cons<double, NS>(P, u, w);
-    double c0 = 0;
+    T w[NS + 4];
+    prims_from_cons<T, NS>(P, u, w);
+    T c0 = 0;
 #pragma unroll
     for (int k = 0; k < NS; ++k)
         c0 += w[k];
-    const double vx = w[NS], vy = w[NS + 1], vz = w[NS + 2], p = w[NS + 3];
-    const double v2 = ((0. + vx * vx) + vy * vy) + vz * vz;
-    const double U = (u[NS + 4] - u[NS] * v2 * 0.5) - 0.;
-    const double c2 = 0. + gamma1<double, NS>(P, w, c0) * (U + p) / u[NS];
-    const double cs = __builtin_sqrt(c2);
-    return (hypot3(vx, vy, vz) + cs) * minlen;
+    const T vx = w[NS], vy = w[NS + 1], vz = w[NS + 2], p = w[NS + 3];
+    const T v2 = ((T(0) + vx * vx) + vy * vy) + vz * vz;
+    const T U = (u[NS + 4] - u[NS] * v2 * T(0.5)) - T(0);
+    const T c2 = T(0) + gamma1<T, NS>(P, w, c0) * (U + p) / u[NS];
+    const T cs = Real<T>::sqrt_(c2);
+    return (hypot3<T>(vx, vy, vz) + cs) * minlen;
 }
 
 // Residual and update of one cell (Advection2dOrder.cpp:541-602 with the
@@ -332,17 +336,18 @@ __device__ __forceinline__ double signal_speed(const DevParams &P, const double 
 // boundary (xlo/ylo/zlo), and is the neighbour of its other low faces.  The
 // contributions (s g) S are summed in that order in FP64.  Stage 2 also forms
 // the SSP-RK2 average with U^n (bunchOfFields.hpp:166-203) and the Courant
-// signal speed of the new state.
+// signal speed of the new state.  u: the cell's conserved input state, un:
+// its U^n (stage 2), both supplied by the caller.
 template <int NS, int DIM, bool STAGE2, bool MIXED, class FluxAt>
 __device__ __forceinline__ void update_cell(const DevParams &P, const CellGeo &cg, long long n,
-                                            long long id, const double *Uin, double *Uout,
-                                            const double *Un, double dt, bool xlo, bool ylo,
+                                            long long id, const double *u, double *Uout,
+                                            const double *un, double dt, bool xlo, bool ylo,
                                             bool zlo, const FluxAt &F, double &sig)
 {
     constexpr int NF = NS + 5;
-    double u[NF];
-    load_cons<NS>(Uin, n, id, u);
     double unew[NF];
+    // mixed path: one division per cell instead of one per field
+    const double dtV = MIXED ? dt / cg.V : 0.0;
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
         double acc = 0;
@@ -356,7 +361,10 @@ __device__ __forceinline__ void update_cell(const DevParams &P, const CellGeo &c
             acc += (zlo ? F(4, q) : -F(4, q)) * cg.S[2];
             acc += F(5, q) * cg.S[2];
         }
-        unew[q] = u[q] - (acc / cg.V) * dt;
+        if constexpr (MIXED)
+            unew[q] = u[q] - acc * dtV;
+        else
+            unew[q] = u[q] - (acc / cg.V) * dt;
     }
     if (P.gravity) {
         // pre-update rho and v (Advection2dOrder.cpp:547-554)
@@ -380,8 +388,6 @@ __device__ __forceinline__ void update_cell(const DevParams &P, const CellGeo &c
             unew[q] *= f;
     }
     if constexpr (STAGE2) {
-        double un[NF];
-        load_cons<NS>(Un, n, id, un);
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
             if (q >= NS + 1 && q <= NS + 3)
@@ -389,7 +395,17 @@ __device__ __forceinline__ void update_cell(const DevParams &P, const CellGeo &c
             else
                 unew[q] = 0.5 * unew[q] + 0.5 * un[q];
         }
-        sig = tmax(sig, signal_speed<NS>(P, unew, cg.minlen));
+        if constexpr (MIXED) {
+            // Courant speed of the FP64 state evaluated in FP32
+            float uf[NF];
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                uf[q] = float(unew[q]);
+            const float ml = 1.0f / tmin(tmin(float(cg.e[0]), float(cg.e[1])), float(cg.e[2]));
+            sig = tmax(sig, double(signal_speed<float, NS>(P, uf, ml)));
+        } else {
+            sig = tmax(sig, signal_speed<double, NS>(P, unew, cg.minlen));
+        }
     }
 #pragma unroll
     for (int q = 0; q < NF; ++q)
@@ -414,6 +430,30 @@ __device__ __forceinline__ void reduce_signal(double sig, double *sRed,
             m = tmax(m, sRed[w]);
         atomicMax(sigmax, (unsigned long long)__double_as_longlong(m));
     }
+}
+
+// Tile coordinates of this workgroup.  The hardware hands workgroups to the 8
+// XCDs (each with its own L2) round-robin in linear order; with SG_XCD_REMAP
+// the linear id is permuted so that each XCD receives a contiguous range of
+// tiles (x fastest), whose shared halo rows then meet in one L2.
+#ifndef SG_XCD_REMAP
+#define SG_XCD_REMAP 0
+#endif
+__device__ __forceinline__ void tile_coords(int &bx, int &by, int &bz)
+{
+#if SG_XCD_REMAP
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    unsigned lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if (nb % 8u == 0u)
+        lin = (lin % 8u) * (nb / 8u) + lin / 8u;
+    bx = static_cast<int>(lin % gridDim.x);
+    by = static_cast<int>((lin / gridDim.x) % gridDim.y);
+    bz = static_cast<int>(lin / (gridDim.x * gridDim.y));
+#else
+    bx = blockIdx.x;
+    by = blockIdx.y;
+    bz = blockIdx.z;
+#endif
 }
 
 // Index helper for the x-y plane tile in LDS with a 2-cell halo.
@@ -458,9 +498,11 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
     const int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
     const long long n = P.n;
     const long long sy = P.stride[1], sz = P.stride[2];
-    const int x0 = blockIdx.x * TX;
-    const int y0 = (DIM >= 2) ? blockIdx.y * TY : 0;
-    const int kb = (DIM == 3) ? blockIdx.z * kchunk : 0;
+    int bx, by, bz;
+    tile_coords(bx, by, bz);
+    const int x0 = bx * TX;
+    const int y0 = (DIM >= 2) ? by * TY : 0;
+    const int kb = (DIM == 3) ? bz * kchunk : 0;
     const int ke = (DIM == 3) ? min(kb + kchunk, Nz) : 1;
     const int i = x0 + tx, j = y0 + ty;
     const bool act = (i < Nx) && (j < Ny);
@@ -710,7 +752,11 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                 default: return double(fzp[q]);
                 }
             };
-            update_cell<NS, DIM, STAGE2, MIXED>(P, cg, n, id, Uin, Uout, Un, dt, xlo, ylo, zlo,
+            double u[NF], un[NF];
+            load_cons<NS>(Uin, n, id, u);
+            if constexpr (STAGE2)
+                load_cons<NS>(Un, n, id, un);
+            update_cell<NS, DIM, STAGE2, MIXED>(P, cg, n, id, u, Uout, un, dt, xlo, ylo, zlo,
                                                 F, sig);
         }
 
@@ -846,9 +892,11 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
     const int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
     const long long n = P.n;
     const long long sy = P.stride[1], sz = P.stride[2];
-    const int x0 = blockIdx.x * TX;
-    const int y0 = (DIM >= 2) ? blockIdx.y * TY : 0;
-    const int kb = (DIM == 3) ? blockIdx.z * kchunk : 0;
+    int bx, by, bz;
+    tile_coords(bx, by, bz);
+    const int x0 = bx * TX;
+    const int y0 = (DIM >= 2) ? by * TY : 0;
+    const int kb = (DIM == 3) ? bz * kchunk : 0;
     const int ke = (DIM == 3) ? min(kb + kchunk, Nz) : 1;
     const int i = x0 + tx, j = y0 + ty;
     const bool act = (i < Nx) && (j < Ny);
@@ -1004,7 +1052,11 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                 default: return double(fzp[q]);
                 }
             };
-            update_cell<NS, DIM, STAGE2, MIXED>(P, cg, n, id, Uin, Uout, Un, dt, xlo, ylo, zlo,
+            double u[NF], un[NF];
+            load_cons<NS>(Uin, n, id, u);
+            if constexpr (STAGE2)
+                load_cons<NS>(Un, n, id, un);
+            update_cell<NS, DIM, STAGE2, MIXED>(P, cg, n, id, u, Uout, un, dt, xlo, ylo, zlo,
                                                 F, sig);
         }
         if constexpr (DIM == 3) {

@@ -170,11 +170,30 @@ def cpu_baseline(nx=96, ny=48, nz=48, steps=2):
 
 def traffic_from_profiles():
     """HBM bytes per stage launch from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "stage_kernel_rm3d_pmc.md")
+    path = os.path.join(ROOT, "profiles", "stage0_kernel_rm3d_256x128x128_pmc.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
+
+
+def stream_copy_gbs():
+    """Device-to-device copy bandwidth (read + write bytes / time) of a 1 GiB
+    buffer, the best of 5 copies timed with HIP events."""
+    import torch
+
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    best = float("inf")
+    for _ in range(6):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    del src, dst
+    return 2 * (1 << 30) / (best * 1e-3) / 1e9
 
 
 def main():
@@ -208,6 +227,7 @@ def main():
                      "value": n * a.steps / wall / 1e6,
                      "stage_kernel_ms": avg_ms, "launches": launches}
         print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
+    copy_gbs = stream_copy_gbs()
     gate = l1_gate(case, U0, a.l1_steps)
     print(f"# l1 gate: {gate}", file=sys.stderr, flush=True)
     cpu = None if a.no_cpu else cpu_baseline()
@@ -241,6 +261,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},
+        "stream_copy": {"value": copy_gbs, "unit": "GB/s",
+                        "achieved_over_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

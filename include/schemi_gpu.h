@@ -89,13 +89,23 @@ typedef struct {
     double origin[3];  /* zero point */
 } schemi_mesh_desc;
 
-/* Mixture: ideal gas mixture (mixtureIdeal.cpp), molar masses and molar Cv. */
+/* gasModel (Schemi/enumerations/gasModelEnum.hpp:15-18) */
+#define SCHEMI_GAS_IDEAL 0
+#define SCHEMI_GAS_VAN_DER_WAALS 1
+
+/* Mixture (abstractMixtureThermodynamics::createThermodynamics,
+ * Schemi/src/interfaces/abstractMixtureThermodynamics.cpp:32-63): molar
+ * masses, molar Cv, and for the van der Waals mixture
+ * (mixtureVanDerWaals.cpp:19-58) the critical temperature and pressure of
+ * every species (ignored by the ideal mixture). */
 typedef struct {
     int32_t ns;
     double M[SCHEMI_MAX_SPECIES];
     double Cv[SCHEMI_MAX_SPECIES];
     double R;
-    int32_t gas_model; /* gasModel::ideal = 0 */
+    int32_t gas_model; /* SCHEMI_GAS_* */
+    double Tcrit[SCHEMI_MAX_SPECIES];
+    double Pcrit[SCHEMI_MAX_SPECIES];
 } schemi_mix_desc;
 
 /* Scheme: flux, limiter, precision, gravity, time-step control, and one

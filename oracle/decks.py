@@ -30,6 +30,8 @@ _PATCH_LABEL = ("#Tail_boundary_condition", "#Point_boundary_condition",
                 "#Left_boundary_condition", "#Top_boundary_condition")
 _FLUX = {0: "HLL", 2: "HLLC"}
 _LIM = {0: "zero", 1: "minmod", 2: "vanLeer", 3: "linear", 4: "vanAlbada"}
+# gasModel -> #Equation_of_state keyword (abstractMixtureThermodynamics.cpp:38-43)
+_EOS = {0: "ideal", 1: "vanDerWaals", 2: "RedlichKwong"}
 
 
 def ref_available() -> bool:
@@ -87,7 +89,7 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         "#Minimal_timestep", _r(mt),
         "#Number_of_components", str(Ns),
         "#Defines_universal_gas_constant", "1" if case.R == 1.0 else "SI",
-        "#Equation_of_state", "ideal",
+        "#Equation_of_state", _EOS[int(case.gas_model)],
         "#TVD_limiter_type", _LIM[int(case.limiter)],
         "#Number_of_iterations_of_SLE_matrix_solver", "2000",
         "#Diffusion_flag", "off",
@@ -124,7 +126,8 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         files[f"sub_{k + 1}.txt"] = [
             "#Molar_mass_(kg/mol)", _r(case.M[k]),
             "#Molar_isohoric_heat_capacity_(J/T/mol)", _r(case.Cv[k]),
-            "#Critical_temperature_(K)", "1", "#Critical_pressure_(Pa)", "1",
+            "#Critical_temperature_(K)", _r(case.Tc[k]),
+            "#Critical_pressure_(Pa)", _r(case.Pc[k]),
             "#Kinetic_diameter_(A)", "1", "#Lennard_Jones_eps/kB_(K)", "1",
             "#Lennard_Jones_sigma_(A)", "1",
         ] + _bc_lines(case, case.bc_c[k]) + [

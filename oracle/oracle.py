@@ -36,6 +36,11 @@ class _Config(C.Structure):
         ("bc_c", (_BC * 6) * MAXS),
         ("bc_v", _BC * 6),
         ("bc_p", _BC * 6),
+        ("gas_model", C.c_int),
+        ("Tc", C.c_double * MAXS),
+        ("Pc", C.c_double * MAXS),
+        ("a_kl", C.c_double * (MAXS * MAXS)),
+        ("b_kl", C.c_double * (MAXS * MAXS)),
     ]
 
 
@@ -115,6 +120,10 @@ def make_config(case) -> _Config:
         cfg.M[k] = float(case.M[k])
         cfg.Cv[k] = float(case.Cv[k])
     cfg.R = float(case.R)
+    cfg.gas_model = int(getattr(case, "gas_model", 0))
+    for k in range(case.Ns):
+        cfg.Tc[k] = float(case.Tc[k])
+        cfg.Pc[k] = float(case.Pc[k])
     cfg.flux = int(case.flux)
     cfg.limiter = int(case.limiter)
     cfg.gravity = 1 if case.gravity else 0

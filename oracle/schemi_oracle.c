@@ -587,25 +587,118 @@ static double cv_mix(const or_config *cfg, const double *c /* c0..cNs */)
         cv += (c[k + 1] / c[0]) * cfg->Cv[k];
     return cv;
 }
-/* pFromUv = (R / Cv) * U (mixtureIdeal.cpp:383-395, idealFluid.cpp:13-17) */
+/* van der Waals mixture (mixtureVanDerWaals.cpp:19-58): per species
+ * b_kk = Tc R / (8 Pc), a_kk = Pc b_kk b_kk 27; across species
+ * b_kl = (0.5 (cbrt b_kk + cbrt b_ll))^3, a_kl = sqrt(a_kk a_ll). */
+static void vdw_matrices(or_config *cfg)
+{
+    const int n = cfg->Ns;
+    for (int k = 0; k < n; ++k) {
+        const double b = cfg->Tc[k] * cfg->R / (8 * cfg->Pc[k]);
+        cfg->b_kl[k * OR_MAXS + k] = b;
+        cfg->a_kl[k * OR_MAXS + k] = cfg->Pc[k] * b * b * 27;
+    }
+    for (int k = 0; k < n; ++k)
+        for (int l = 0; l < n; ++l)
+            if (k != l) {
+                cfg->b_kl[k * OR_MAXS + l] = pow(
+                    0.5 * (cbrt(cfg->b_kl[k * OR_MAXS + k]) + cbrt(cfg->b_kl[l * OR_MAXS + l])),
+                    3);
+                cfg->a_kl[k * OR_MAXS + l] =
+                    sqrt(cfg->a_kl[k * OR_MAXS + k] * cfg->a_kl[l * OR_MAXS + l]);
+            }
+}
+
+/* a_mix = sum_k sum_l X_k X_l a_kl, b_mix likewise, X_k = c_k / c_0
+ * (mixtureVanDerWaals.cpp:104-109, fractionCalculation.cpp:35-44) */
+static void vdw_mix(const or_config *cfg, const double *c, double *am, double *bm)
+{
+    double X[OR_MAXS];
+    for (int k = 0; k < cfg->Ns; ++k)
+        X[k] = c[k + 1] / c[0];
+    double a = 0, b = 0;
+    for (int k = 0; k < cfg->Ns; ++k)
+        for (int l = 0; l < cfg->Ns; ++l) {
+            a += X[k] * X[l] * cfg->a_kl[k * OR_MAXS + l];
+            b += X[k] * X[l] * cfg->b_kl[k * OR_MAXS + l];
+        }
+    *am = a;
+    *bm = b;
+}
+
+/* pFromUv: ideal (R / Cv) U (mixtureIdeal.cpp:383-395, idealFluid.cpp:13-17);
+ * vdW g1 (U + a c^2) / (1 - c b) - a c^2 (vanDerWaalsFluid.cpp:14-22) */
 static double eos_p(const or_config *cfg, const double *c, double U)
 {
+    if (cfg->gas_model == OR_GAS_VDW) {
+        double a, b;
+        vdw_mix(cfg, c, &a, &b);
+        const double g1 = cfg->R / cv_mix(cfg, c);
+        const double acc = a * (c[0] * c[0]);
+        return g1 * (U + acc) / (1 - c[0] * b) - acc;
+    }
     return (cfg->R / cv_mix(cfg, c)) * U;
 }
-/* UvFromp = p / (R / Cv) (mixtureIdeal.cpp:397-409, idealFluid.cpp:19-23) */
+/* UvFromp: ideal p / (R / Cv) (mixtureIdeal.cpp:397-409, idealFluid.cpp:19-23);
+ * vdW (p + a c^2) (1 - c b) / g1 - a c^2 (vanDerWaalsFluid.cpp:24-31) */
 static double eos_U(const or_config *cfg, const double *c, double p)
 {
+    if (cfg->gas_model == OR_GAS_VDW) {
+        double a, b;
+        vdw_mix(cfg, c, &a, &b);
+        const double g1 = cfg->R / cv_mix(cfg, c);
+        const double acc = a * (c[0] * c[0]);
+        return (p + acc) * (1 - c[0] * b) / g1 - acc;
+    }
     return p / (cfg->R / cv_mix(cfg, c));
 }
-/* TFromUv = U / ((c0 + stab) Cv) (mixtureIdeal.cpp:443-455, idealFluid.cpp:37-41) */
+/* TFromUv: ideal U / ((c0 + stab) Cv) (mixtureIdeal.cpp:443-455,
+ * idealFluid.cpp:37-41); vdW (U + a c^2) / ((c0 + stab) Cv)
+ * (vanDerWaalsFluid.cpp:45-49) */
 static double eos_T(const or_config *cfg, const double *c, double U)
 {
+    if (cfg->gas_model == OR_GAS_VDW) {
+        double a, b;
+        vdw_mix(cfg, c, &a, &b);
+        return (U + a * (c[0] * c[0])) / ((c[0] + STAB) * cv_mix(cfg, c));
+    }
     return U / ((c[0] + STAB) * cv_mix(cfg, c));
 }
-/* sqSonicSpeed = dpdrho + dpdUv (U + p) / rho (mixtureIdeal.cpp:495-501) */
+/* dpdUv: ideal R / Cv; vdW g1 / (1 - c b) (vanDerWaalsFluid.cpp:79-83,
+ * mixtureVanDerWaals.cpp:746-762) */
+static double eos_dpdUv(const or_config *cfg, const double *c)
+{
+    if (cfg->gas_model == OR_GAS_VDW) {
+        double a, b;
+        vdw_mix(cfg, c, &a, &b);
+        return cfg->R / cv_mix(cfg, c) / (1 - c[0] * b);
+    }
+    return cfg->R / cv_mix(cfg, c);
+}
+/* sqSonicSpeed = dpdrho + dpdUv (U + p) / rho (mixtureIdeal.cpp:495-501,
+ * mixtureVanDerWaals.cpp:779-785).  Ideal: dpdrho = 0.  vdW (vanDerWaalsFluid.
+ * cpp:61-77, mixtureVanDerWaals.cpp:719-744): with m = 1 - c b, ac = a c,
+ * dpdrho = (g1 (2 ac / m - b (U + ac c) / m^2) - 2 ac) / M_mix,
+ * M_mix = sum_k X_k M_k. */
 static double eos_c2(const or_config *cfg, const double *c, double rho, double U,
                      double p)
 {
+    if (cfg->gas_model == OR_GAS_VDW) {
+        double a, b;
+        vdw_mix(cfg, c, &a, &b);
+        double cv = 0, mm = 0;
+        for (int k = 0; k < cfg->Ns; ++k) {
+            const double X = c[k + 1] / c[0];
+            cv += X * cfg->Cv[k];
+            mm += X * cfg->M[k];
+        }
+        const double g1 = cfg->R / cv;
+        const double m = 1 - c[0] * b;
+        const double ac = a * c[0];
+        const double dpdrho = (g1 * (2 * ac / m - b * (U + ac * c[0]) / (m * m)) - 2 * ac) / mm;
+        const double dpdUv = eos_dpdUv(cfg, c);
+        return dpdrho + dpdUv * (U + p) / rho;
+    }
     const double dpdrho = 0;
     const double dpdUv = cfg->R / cv_mix(cfg, c);
     return dpdrho + dpdUv * (U + p) / rho;
@@ -702,8 +795,12 @@ or_solver *or_create(const or_config *cfg)
 {
     if (cfg->dim < 1 || cfg->dim > 3 || cfg->Ns < 1 || cfg->Ns > OR_MAXS)
         return NULL;
+    if (cfg->gas_model != OR_GAS_IDEAL && cfg->gas_model != OR_GAS_VDW)
+        return NULL;
     or_solver *s = calloc(1, sizeof(or_solver));
     s->cfg = *cfg;
+    if (cfg->gas_model == OR_GAS_VDW)
+        vdw_matrices(&s->cfg);
     for (int a = 0; a < cfg->dim; ++a)
         if (axis_periodic(s, a) < 0) {
             free(s);
@@ -1239,9 +1336,9 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
 {
     const int nd = cfg->Ns + 1;
     const double cO = sqrt(eos_c2(cfg, O->c, O->rho[0], O->U, O->p) +
-                           (2. / 3.) * 0.0 * (1. + cfg->R / cv_mix(cfg, O->c)));
+                           (2. / 3.) * 0.0 * (1. + eos_dpdUv(cfg, O->c)));
     const double cN = sqrt(eos_c2(cfg, Nb->c, Nb->rho[0], Nb->U, Nb->p) +
-                           (2. / 3.) * 0.0 * (1. + cfg->R / cv_mix(cfg, Nb->c)));
+                           (2. / 3.) * 0.0 * (1. + eos_dpdUv(cfg, Nb->c)));
     const double uO = dot3(O->v, N);
     const double uN = dot3(Nb->v, N);
     const double ptO = (2. / 3.) * 0.0, ptN = (2. / 3.) * 0.0;
@@ -1344,9 +1441,9 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
 {
     const int nd = cfg->Ns + 1;
     const double cO = sqrt(eos_c2(cfg, O->c, O->rho[0], O->U, O->p) +
-                           (2. / 3.) * 0.0 * (1. + cfg->R / cv_mix(cfg, O->c)));
+                           (2. / 3.) * 0.0 * (1. + eos_dpdUv(cfg, O->c)));
     const double cN = sqrt(eos_c2(cfg, Nb->c, Nb->rho[0], Nb->U, Nb->p) +
-                           (2. / 3.) * 0.0 * (1. + cfg->R / cv_mix(cfg, Nb->c)));
+                           (2. / 3.) * 0.0 * (1. + eos_dpdUv(cfg, Nb->c)));
     const double uO = dot3(O->v, N);
     const double uN = dot3(Nb->v, N);
     const double ptO = (2. / 3.) * 0.0, ptN = (2. / 3.) * 0.0;

@@ -4,9 +4,9 @@ A :class:`Case` carries what the reference reads from its ``set/*.txt`` deck
 for the explicit step (Schemi/main/Schemi.cpp:84-163 and
 Schemi/src/thermodynamics/phaseInitialization.cpp:17-660): the parallelepiped
 (``#Radius-vector_of_system's_parallelepiped``, ``#Number_of_cells_in_direction``),
-the species (molar mass and molar Cv per ``sub_k.txt``), the universal gas
-constant, the flux scheme and limiter, gravity, and one boundary condition per
-field and patch.
+the species (molar mass, molar Cv, critical temperature and pressure per
+``sub_k.txt``), the universal gas constant, the equation of state, the flux
+scheme and limiter, gravity, and one boundary condition per field and patch.
 
 Cells are numbered as in the reference: ``id = i + Nx (j + Ny k)``, x fastest
 (Schemi/src/dataStructures/mesh.cpp:1223-1230).
@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 from typing import List, Sequence, Tuple
 
 from .enums import boundaryConditionType as BCT
-from .enums import flowSolver, typeOfTVDLimiter
+from .enums import flowSolver, gasModel, typeOfTVDLimiter
 
 Vec3 = Tuple[float, float, float]
 
@@ -52,10 +52,19 @@ class Case:
     bc_c: List[List[BC]] = field(default_factory=list)
     bc_v: List[BC] = field(default_factory=_free6)
     bc_p: List[BC] = field(default_factory=_free6)
+    # equation of state (gasModel) and the species' critical temperature and
+    # pressure (used by the van der Waals model)
+    gas_model: gasModel = gasModel.ideal
+    Tc: Sequence[float] = ()
+    Pc: Sequence[float] = ()
 
     def __post_init__(self):
         if not self.bc_c:
             self.bc_c = [_free6() for _ in self.M]
+        if not self.Tc:
+            self.Tc = [1.0] * len(self.M)
+        if not self.Pc:
+            self.Pc = [1.0] * len(self.M)
         n = list(self.N)
         if self.dim < 3:
             n[2] = 1

@@ -96,8 +96,16 @@ __global__ void prims_kernel(const DevParams *__restrict__ Pp, const double *__r
         cv += (w[k] / c0) * P.Cv[k];
     W[(NS + 4) * n + id] = p;
     W[(NS + 5) * n + id] = Uint;
-    W[(NS + 6) * n + id] = Uint / ((c0 + STAB) * cv);  // TFromUv (idealFluid.cpp:37-41)
-    const double c2 = 0. + gamma1<double, NS>(P, w, c0) * (Uint + p) / u[NS];
+    // TFromUv: U / ((c0 + stab) Cv) (idealFluid.cpp:37-41), with U + a c0^2
+    // for van der Waals (vanDerWaalsFluid.cpp:45-49)
+    double Ut = Uint;
+    if (P.gas_model == GAS_VDW) {
+        double a, b;
+        vdw_ab<double, NS>(P, w, c0, a, b);
+        Ut = Uint + a * (c0 * c0);
+    }
+    W[(NS + 6) * n + id] = Ut / ((c0 + STAB) * cv);
+    const double c2 = eos_c2<double, NS>(P, w, c0, gamma1<double, NS>(P, w, c0), u[NS], Uint, p);
     W[(NS + 7) * n + id] = __builtin_sqrt(c2);
 }
 
@@ -189,6 +197,7 @@ int launch_stage(schemi_gpu_ctx *c, const double *Uin, double *Uout, const doubl
     L.lin = c->lin;
     L.stage2 = stage2 ? 1 : 0;
     L.mode = c->mode;
+    L.gas_model = c->hp.gas_model;
     L.W = c->W;
     L.Lg = c->Lg;
     L.nx = c->hp.N[0];
@@ -282,8 +291,13 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     for (int a = 0; a < dim; ++a)
         if (mesh->n[a] < 1 || (dim >= 2 && mesh->n[a] < 2) || mesh->n[a] > (1 << 30))
             return SCHEMI_ERR_MESH;
-    if (mix->ns < 1 || mix->ns > 3 || mix->gas_model != 0)
+    if (mix->ns < 1 || mix->ns > 3 ||
+        (mix->gas_model != SCHEMI_GAS_IDEAL && mix->gas_model != SCHEMI_GAS_VAN_DER_WAALS))
         return SCHEMI_ERR_INIT;
+    if (mix->gas_model == SCHEMI_GAS_VAN_DER_WAALS)
+        for (int k = 0; k < mix->ns; ++k)
+            if (!(mix->Tcrit[k] > 0.0) || !(mix->Pcrit[k] > 0.0))
+                return SCHEMI_ERR_INIT;
     if (scheme->flux != SCHEMI_FLUX_HLL && scheme->flux != SCHEMI_FLUX_HLLC)
         return SCHEMI_ERR_INIT;
     const int lim = scheme->limiter;
@@ -322,6 +336,28 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     }
     P.R = mix->R;
     P.Rf = static_cast<float>(mix->R);
+    P.gas_model = mix->gas_model;
+    if (mix->gas_model == SCHEMI_GAS_VAN_DER_WAALS) {
+        // mixtureVanDerWaals.cpp:19-58: b_kk = Tc R / (8 Pc), a_kk = Pc b b 27,
+        // b_kl = (0.5 (cbrt b_kk + cbrt b_ll))^3, a_kl = sqrt(a_kk a_ll)
+        for (int k = 0; k < mix->ns; ++k) {
+            const double b = mix->Tcrit[k] * mix->R / (8 * mix->Pcrit[k]);
+            P.bkl[k][k] = b;
+            P.akl[k][k] = mix->Pcrit[k] * b * b * 27;
+        }
+        for (int k = 0; k < mix->ns; ++k)
+            for (int l = 0; l < mix->ns; ++l)
+                if (k != l) {
+                    P.bkl[k][l] = std::pow(0.5 * (std::cbrt(P.bkl[k][k]) + std::cbrt(P.bkl[l][l])),
+                                           3.0);
+                    P.akl[k][l] = std::sqrt(P.akl[k][k] * P.akl[l][l]);
+                }
+        for (int k = 0; k < mix->ns; ++k)
+            for (int l = 0; l < mix->ns; ++l) {
+                P.aklf[k][l] = static_cast<float>(P.akl[k][l]);
+                P.bklf[k][l] = static_cast<float>(P.bkl[k][l]);
+            }
+    }
     P.gravity = scheme->gravity ? 1 : 0;
     for (int b = 0; b < 3; ++b)
         P.g[b] = scheme->g[b];

@@ -25,6 +25,7 @@ enum : int { BC_FREE = 1, BC_SLIP = 2, BC_FIXED_CELL = 3, BC_FIXED_SURF = 4,
 enum : int { LIM_ZERO = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_LINEAR = 3,
              LIM_VANALBADA = 4 };
 enum : int { FLUX_HLL = 0, FLUX_HLLC = 2 };
+enum : int { GAS_IDEAL = 0, GAS_VDW = 1 };  // gasModel (gasModelEnum.hpp:15-18)
 enum : int { ERRF_RIEMANN = 1, ERRF_BOUNDARY = 2 };
 
 // Parameters shared by all kernels of a context, kept in device memory and
@@ -43,6 +44,10 @@ struct DevParams {
     int limiter, flux;
     double M[MAXS], Cv[MAXS], R;
     float Mf[MAXS], Cvf[MAXS], Rf;
+    // equation of state; van der Waals mixing matrices a_kl, b_kl
+    int gas_model;
+    double akl[MAXS][MAXS], bkl[MAXS][MAXS];
+    float aklf[MAXS][MAXS], bklf[MAXS][MAXS];
     int gravity;
     double g[3];
     // boundary conditions per patch (0 tail .. 5 top) and field
@@ -98,8 +103,15 @@ template <class T> __device__ __forceinline__ T mixR(const DevParams &P);
 template <> __device__ __forceinline__ double mixR<double>(const DevParams &P) { return P.R; }
 template <> __device__ __forceinline__ float mixR<float>(const DevParams &P) { return P.Rf; }
 
+template <class T> __device__ __forceinline__ T mixA(const DevParams &P, int k, int l);
+template <> __device__ __forceinline__ double mixA<double>(const DevParams &P, int k, int l) { return P.akl[k][l]; }
+template <> __device__ __forceinline__ float mixA<float>(const DevParams &P, int k, int l) { return P.aklf[k][l]; }
+template <class T> __device__ __forceinline__ T mixB(const DevParams &P, int k, int l);
+template <> __device__ __forceinline__ double mixB<double>(const DevParams &P, int k, int l) { return P.bkl[k][l]; }
+template <> __device__ __forceinline__ float mixB<float>(const DevParams &P, int k, int l) { return P.bklf[k][l]; }
+
 // R / Cv_mix with Cv_mix = sum_k (c_k / c_0) Cv_k (mixtureIdeal.cpp:383-395,
-// fractionCalculation.cpp:35-44).
+// mixtureVanDerWaals.cpp:570-591, fractionCalculation.cpp:35-44).
 template <class T, int NS>
 __device__ __forceinline__ T gamma1(const DevParams &P, const T *c, T c0)
 {
@@ -110,11 +122,92 @@ __device__ __forceinline__ T gamma1(const DevParams &P, const T *c, T c0)
     return mixR<T>(P) / cv;
 }
 
+// van der Waals mixture parameters a = sum_k sum_l X_k X_l a_kl and b
+// likewise, X_k = c_k / c_0 (mixtureVanDerWaals.cpp:580-587).
+template <class T, int NS>
+__device__ __forceinline__ void vdw_ab(const DevParams &P, const T *c, T c0, T &a, T &b)
+{
+    T X[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        X[k] = c[k] / c0;
+    a = T(0);
+    b = T(0);
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+            a += X[k] * X[l] * mixA<T>(P, k, l);
+            b += X[k] * X[l] * mixB<T>(P, k, l);
+        }
+}
+
+// Gas model of a kernel: GAS >= 0 fixes it at compile time, -1 reads it from
+// the parameter block.
+template <int GAS> __device__ __forceinline__ bool is_vdw(const DevParams &P)
+{
+    if constexpr (GAS >= 0)
+        return GAS == GAS_VDW;
+    else
+        return P.gas_model == GAS_VDW;
+}
+
+// pFromUv: ideal g1 U (idealFluid.cpp:13-17); van der Waals
+// g1 (U + a c0^2) / (1 - c0 b) - a c0^2 (vanDerWaalsFluid.cpp:14-22).
+template <class T, int NS, int GAS = -1>
+__device__ __forceinline__ T eos_p(const DevParams &P, const T *c, T c0, T g1, T U)
+{
+    if (is_vdw<GAS>(P)) {
+        T a, b;
+        vdw_ab<T, NS>(P, c, c0, a, b);
+        const T acc = a * (c0 * c0);
+        return g1 * (U + acc) / (T(1) - c0 * b) - acc;
+    }
+    return g1 * U;
+}
+
+// UvFromp: ideal p / g1 (idealFluid.cpp:19-23); van der Waals
+// (p + a c0^2) (1 - c0 b) / g1 - a c0^2 (vanDerWaalsFluid.cpp:24-31).
+template <class T, int NS, int GAS = -1>
+__device__ __forceinline__ T eos_U(const DevParams &P, const T *c, T c0, T g1, T p)
+{
+    if (is_vdw<GAS>(P)) {
+        T a, b;
+        vdw_ab<T, NS>(P, c, c0, a, b);
+        const T acc = a * (c0 * c0);
+        return (p + acc) * (T(1) - c0 * b) / g1 - acc;
+    }
+    return p / g1;
+}
+
+// sqSonicSpeed = dpdrho + dpdUv (U + p) / rho (mixtureIdeal.cpp:495-501,
+// mixtureVanDerWaals.cpp:779-785).  Ideal: dpdrho = 0, dpdUv = g1.  Van der
+// Waals (vanDerWaalsFluid.cpp:61-83, mixtureVanDerWaals.cpp:719-762): with
+// m = 1 - c0 b, ac = a c0: dpdrho = (g1 (2 ac / m - b (U + ac c0) / m^2)
+// - 2 ac) / M_mix, M_mix = sum_k X_k M_k, dpdUv = g1 / m.
+template <class T, int NS, int GAS = -1>
+__device__ __forceinline__ T eos_c2(const DevParams &P, const T *c, T c0, T g1, T rho, T U,
+                                    T p)
+{
+    if (is_vdw<GAS>(P)) {
+        T a, b, mm = T(0);
+        vdw_ab<T, NS>(P, c, c0, a, b);
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            mm += (c[k] / c0) * mixM<T>(P, k);
+        const T m = T(1) - c0 * b;
+        const T ac = a * c0;
+        const T dpdrho = (g1 * (T(2) * ac / m - b * (U + ac * c0) / (m * m)) - T(2) * ac) / mm;
+        return dpdrho + g1 / m * (U + p) / rho;
+    }
+    return T(0) + g1 * (U + p) / rho;
+}
+
 // Primitive recovery from the conserved vector, as after the stage update
 // (Advection2dOrder.cpp:578-602): c_k = rho_k / M_k, c_0 = sum c_k,
 // v = m / rho, U = E - rho v^2 / 2 - 0, p = (R / Cv) U.
 // u: [rho_1..rho_Ns, rho, m_x, m_y, m_z, E]; w: [c_1..c_Ns, v_x, v_y, v_z, p]
-template <class T, int NS>
+template <class T, int NS, int GAS = -1>
 __device__ __forceinline__ void prims_from_cons(const DevParams &P, const T *u, T *w)
 {
     T c0 = 0;
@@ -130,7 +223,7 @@ __device__ __forceinline__ void prims_from_cons(const DevParams &P, const T *u, 
     w[NS + 2] = vz;
     const T v2 = ((T(0) + vx * vx) + vy * vy) + vz * vz;
     const T U = (u[NS + 4] - rho * v2 * T(0.5)) - T(0);
-    w[NS + 3] = gamma1<T, NS>(P, w, c0) * U;
+    w[NS + 3] = eos_p<T, NS, GAS>(P, w, c0, gamma1<T, NS>(P, w, c0), U);
 }
 
 // Reconstructed face side closed with the EOS (Advection2dOrder.cpp:233-289
@@ -143,7 +236,7 @@ template <class T, int NS> struct Side {
     T p, U, E, cs;
 };
 
-template <class T, int NS>
+template <class T, int NS, int GAS = -1>
 __device__ __forceinline__ void close_side(const DevParams &P, const T *c, const T *v,
                                            T p, Side<T, NS> &s)
 {
@@ -157,7 +250,7 @@ __device__ __forceinline__ void close_side(const DevParams &P, const T *c, const
     s.r0 = r0;
     const T g1 = gamma1<T, NS>(P, c, c0);
     s.p = p;
-    s.U = p / g1;  // UvFromp (idealFluid.cpp:19-23)
+    s.U = eos_U<T, NS, GAS>(P, c, c0, g1, p);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         s.v[a] = v[a];
@@ -165,8 +258,8 @@ __device__ __forceinline__ void close_side(const DevParams &P, const T *c, const
     }
     const T v2 = ((T(0) + v[0] * v[0]) + v[1] * v[1]) + v[2] * v[2];
     s.E = (s.U + r0 * v2 * T(0.5)) + T(0);
-    // sqSonicSpeed (mixtureIdeal.cpp:495-501); the turbulent term is 0
-    const T c2 = T(0) + g1 * (s.U + p) / r0;
+    // sqSonicSpeed; the turbulent term (2/3) k (1 + dpdUv) is 0
+    const T c2 = eos_c2<T, NS, GAS>(P, c, c0, g1, r0, s.U, p);
     s.cs = Real<T>::sqrt_(c2);
 }
 

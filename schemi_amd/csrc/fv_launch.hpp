@@ -42,7 +42,7 @@ struct StageLaunch {
     const double *dt;
     unsigned long long *sigmax;
     int *err;
-    int dim, ns, lin, stage2, mode;
+    int dim, ns, lin, stage2, mode, gas_model;
     int nx, ny, nz;
     int kchunk;
     void *W, *Lg;  // MODE_GENERAL scratch: primitives and gradients

@@ -22,9 +22,14 @@ static int launch_mode(const StageLaunch &L)
     constexpr int TX0 = TileShape<DIM, true>::TX, TY0 = TileShape<DIM, true>::TY;
     switch (L.mode) {
     case MODE_FIRST_ORDER:
-        hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, S2, MIXED, TX0, TY0>),
-                           (tile_grid<DIM, true>(L)), dim3(TX0 * TY0), 0, L.stream, L.P, L.geo,
-                           L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
+        if (L.gas_model == GAS_VDW)
+            hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, S2, MIXED, TX0, TY0, GAS_VDW>),
+                               (tile_grid<DIM, true>(L)), dim3(TX0 * TY0), 0, L.stream, L.P,
+                               L.geo, L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
+        else
+            hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, S2, MIXED, TX0, TY0, GAS_IDEAL>),
+                               (tile_grid<DIM, true>(L)), dim3(TX0 * TY0), 0, L.stream, L.P,
+                               L.geo, L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
         break;
     case MODE_SLOPES:
         if (L.lin)

@@ -312,11 +312,11 @@ __device__ __forceinline__ void cell_prims(const DevParams &P, const double *U, 
 
 // Courant signal speed (|v| + c) / min edge of one cell (Schemi.cpp:810-828),
 // in precision T (FP64 on the parity path).
-template <class T, int NS>
+template <class T, int NS, int GAS = -1>
 __device__ __forceinline__ T signal_speed(const DevParams &P, const T *u, T minlen)
 {
     T w[NS + 4];
-    prims_from_cons<T, NS>(P, u, w);
+    prims_from_cons<T, NS, GAS>(P, u, w);
     T c0 = 0;
 #pragma unroll
     for (int k = 0; k < NS; ++k)
@@ -324,7 +324,7 @@ __device__ __forceinline__ T signal_speed(const DevParams &P, const T *u, T minl
     const T vx = w[NS], vy = w[NS + 1], vz = w[NS + 2], p = w[NS + 3];
     const T v2 = ((T(0) + vx * vx) + vy * vy) + vz * vz;
     const T U = (u[NS + 4] - u[NS] * v2 * T(0.5)) - T(0);
-    const T c2 = T(0) + gamma1<T, NS>(P, w, c0) * (U + p) / u[NS];
+    const T c2 = eos_c2<T, NS, GAS>(P, w, c0, gamma1<T, NS>(P, w, c0), u[NS], U, p);
     const T cs = Real<T>::sqrt_(c2);
     return (hypot3<T>(vx, vy, vz) + cs) * minlen;
 }
@@ -338,7 +338,7 @@ __device__ __forceinline__ T signal_speed(const DevParams &P, const T *u, T minl
 // the SSP-RK2 average with U^n (bunchOfFields.hpp:166-203) and the Courant
 // signal speed of the new state.  u: the cell's conserved input state, un:
 // its U^n (stage 2), both supplied by the caller.
-template <int NS, int DIM, bool STAGE2, bool MIXED, class FluxAt>
+template <int NS, int DIM, bool STAGE2, bool MIXED, int GAS = -1, class FluxAt>
 __device__ __forceinline__ void update_cell(const DevParams &P, const CellGeo &cg, long long n,
                                             long long id, const double *u, double *Uout,
                                             const double *un, double dt, bool xlo, bool ylo,
@@ -402,9 +402,9 @@ __device__ __forceinline__ void update_cell(const DevParams &P, const CellGeo &c
             for (int q = 0; q < NF; ++q)
                 uf[q] = float(unew[q]);
             const float ml = 1.0f / tmin(tmin(float(cg.e[0]), float(cg.e[1])), float(cg.e[2]));
-            sig = tmax(sig, double(signal_speed<float, NS>(P, uf, ml)));
+            sig = tmax(sig, double(signal_speed<float, NS, GAS>(P, uf, ml)));
         } else {
-            sig = tmax(sig, signal_speed<double, NS>(P, unew, cg.minlen));
+            sig = tmax(sig, signal_speed<double, NS, GAS>(P, unew, cg.minlen));
         }
     }
 #pragma unroll
@@ -437,7 +437,7 @@ __device__ __forceinline__ void reduce_signal(double sig, double *sRed,
 // the linear id is permuted so that each XCD receives a contiguous range of
 // tiles (x fastest), whose shared halo rows then meet in one L2.
 #ifndef SG_XCD_REMAP
-#define SG_XCD_REMAP 0
+#define SG_XCD_REMAP 1
 #endif
 __device__ __forceinline__ void tile_coords(int &bx, int &by, int &bz)
 {
@@ -789,11 +789,11 @@ template <class T, int NS> struct Closed {
     static constexpr int NV = NS + 8;  // c[NS], v[3], p, r0, U, E, cs
 };
 
-template <class T, int NS>
+template <class T, int NS, int GAS = -1>
 __device__ __forceinline__ void close_cell(const DevParams &P, const T *w, T *z)
 {
     Side<T, NS> s;
-    close_side<T, NS>(P, w, w + NS, w[NS + 3], s);
+    close_side<T, NS, GAS>(P, w, w + NS, w[NS + 3], s);
 #pragma unroll
     for (int q = 0; q < NS + 4; ++q)
         z[q] = w[q];
@@ -824,7 +824,7 @@ __device__ __forceinline__ void side_of(const DevParams &P, const T *z, Side<T, 
 // Flux of face f along axis a from the closed states of cells f-1 (zo) and f
 // (zn); on a non-periodic boundary only the owner's state is read and the
 // ghost is formed from its primitives (Advection2dOrder.cpp:398-521).
-template <class T, int NS>
+template <class T, int NS, int GAS = -1>
 __device__ __forceinline__ bool face_flux_closed(const DevParams &P, int a, int f, const T *zo,
                                                  const T *zn, T *g)
 {
@@ -840,7 +840,7 @@ __device__ __forceinline__ bool face_flux_closed(const DevParams &P, int a, int 
         T cB[NS], vB[3], pB;
         ghost_side<T, NS>(P, a, side, z, z + NS, z[NS + 3], cB, vB, pB);
         side_of<T, NS>(P, z, O);
-        close_side<T, NS>(P, cB, vB, pB, B);
+        close_side<T, NS, GAS>(P, cB, vB, pB, B);
         const T sg = (side && !P.hi_flip[a]) ? T(1) : T(-1);
         if (P.flux == FLUX_HLL)
             return hll_flux<T, NS>(O, B, a, sg, g);
@@ -863,8 +863,11 @@ __device__ __forceinline__ bool face_flux_closed(const DevParams &P, int a, int 
 // ((phi G) . d + q = q), so each cell is closed with the EOS once, its closed
 // state staged in LDS with a one-cell halo, and no slopes are stored.  Same
 // z-marching tile and update as stage_kernel.
-template <class T, int NS, int DIM, bool STAGE2, bool MIXED, int TX, int TY>
-__global__ void __launch_bounds__(TX * TY)
+// The mixed path asks for 4 waves per SIMD (two 512-thread workgroups per
+// CU); its register budget then allows a few spills, which measured faster
+// than one workgroup per CU.
+template <class T, int NS, int DIM, bool STAGE2, bool MIXED, int TX, int TY, int GAS>
+__global__ void __launch_bounds__(TX * TY, (MIXED && TX * TY == 512) ? 4 : 1)
 stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
               const double *__restrict__ Uin, double *Uout, const double *Un,
               const double *__restrict__ dt_ptr, unsigned long long *sigmax, int *errflag,
@@ -905,25 +908,42 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
     double sig = 0.0;
     const bool zper = (DIM == 3) && P.periodic[2] != 0;
 
-    auto closed_at = [&](long long id, T *z) {
+    auto closed_from = [&](const double *u, T *z) {
         T w[NQ];
-        cell_prims<T, NS, MIXED>(P, Uin, n, id, w);
-        close_cell<T, NS>(P, w, z);
+        if constexpr (MIXED) {
+            T ut[NF];
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                ut[q] = T(u[q]);
+            prims_from_cons<T, NS, GAS>(P, ut, w);
+        } else {
+            prims_from_cons<T, NS, GAS>(P, u, w);
+        }
+        close_cell<T, NS, GAS>(P, w, z);
+    };
+    auto closed_at = [&](long long id, T *z) {
+        double u[NF];
+        load_cons<NS>(Uin, n, id, u);
+        closed_from(u, z);
     };
 
-    // own column: closed states of planes k and k+1, fluxes of its z faces
+    // own column: closed states of planes k and k+1, fluxes of its z faces;
+    // the conserved values of plane k stay in registers for the update (a
+    // reload would miss in L2: the plane was read one iteration earlier)
     T zc[NV], znx[NV];
     T fzm[NF], fzp[NF];
+    double ucur[NF], unx[NF];
     if constexpr (DIM == 3) {
         if (act) {
-            closed_at(i + sy * j + sz * kb, zc);
+            load_cons<NS>(Uin, n, i + sy * j + sz * kb, ucur);
+            closed_from(ucur, zc);
             if (kb > 0 || zper) {
                 T zm[NV];
                 const int km = (kb > 0) ? kb - 1 : Nz - 1;
                 closed_at(i + sy * j + sz * km, zm);
-                if (!face_flux_closed<T, NS>(P, 2, kb, zm, zc, fzm))
+                if (!face_flux_closed<T, NS, GAS>(P, 2, kb, zm, zc, fzm))
                     err |= ERRF_RIEMANN;
-            } else if (!face_flux_closed<T, NS>(P, 2, 0, zc, zc, fzm)) {
+            } else if (!face_flux_closed<T, NS, GAS>(P, 2, 0, zc, zc, fzm)) {
                 err |= ERRF_RIEMANN;
             }
         }
@@ -935,16 +955,18 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
             if constexpr (DIM == 3) {
                 if ((k + 1 < Nz) || zper) {
                     const int kp = (k + 1 < Nz) ? k + 1 : 0;
-                    closed_at(i + sy * j + sz * kp, znx);
+                    load_cons<NS>(Uin, n, i + sy * j + sz * kp, unx);
+                    closed_from(unx, znx);
                 } else {
 #pragma unroll
                     for (int q = 0; q < NV; ++q)
                         znx[q] = zc[q];  // not read: the top face is a boundary face
                 }
-                if (!face_flux_closed<T, NS>(P, 2, k + 1, zc, znx, fzp))
+                if (!face_flux_closed<T, NS, GAS>(P, 2, k + 1, zc, znx, fzp))
                     err |= ERRF_RIEMANN;
             } else {
-                closed_at(i + sy * j, zc);
+                load_cons<NS>(Uin, n, i + sy * j, ucur);
+                closed_from(ucur, zc);
             }
 #pragma unroll
             for (int q = 0; q < NV; ++q)
@@ -1008,7 +1030,7 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                         zo[q] = sZ[q][r + OY][u];
                         zn[q] = sZ[q][r + OY][u + 1];
                     }
-                    if (!face_flux_closed<T, NS>(P, 0, fx, zo, zn, g))
+                    if (!face_flux_closed<T, NS, GAS>(P, 0, fx, zo, zn, g))
                         err |= ERRF_RIEMANN;
 #pragma unroll
                     for (int q = 0; q < NF; ++q)
@@ -1024,7 +1046,7 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                         zo[q] = sZ[q][v][c + 1];
                         zn[q] = sZ[q][v + 1][c + 1];
                     }
-                    if (!face_flux_closed<T, NS>(P, 1, fy, zo, zn, g))
+                    if (!face_flux_closed<T, NS, GAS>(P, 1, fy, zo, zn, g))
                         err |= ERRF_RIEMANN;
 #pragma unroll
                     for (int q = 0; q < NF; ++q)
@@ -1052,14 +1074,20 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                 default: return double(fzp[q]);
                 }
             };
-            double u[NF], un[NF];
-            load_cons<NS>(Uin, n, id, u);
+            double un[NF];
             if constexpr (STAGE2)
                 load_cons<NS>(Un, n, id, un);
-            update_cell<NS, DIM, STAGE2, MIXED>(P, cg, n, id, u, Uout, un, dt, xlo, ylo, zlo,
-                                                F, sig);
+            // mixed path: reload instead of keeping it (its registers are
+            // needed to run two 512-thread workgroups per CU without spills)
+            if constexpr (MIXED)
+                load_cons<NS>(Uin, n, id, ucur);
+            update_cell<NS, DIM, STAGE2, MIXED, GAS>(P, cg, n, id, ucur, Uout, un, dt, xlo, ylo,
+                                                zlo, F, sig);
         }
         if constexpr (DIM == 3) {
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                ucur[q] = unx[q];
 #pragma unroll
             for (int q = 0; q < NV; ++q)
                 zc[q] = znx[q];

@@ -50,7 +50,8 @@ class MeshDesc(C.Structure):
 
 class MixDesc(C.Structure):
     _fields_ = [("ns", C.c_int32), ("M", C.c_double * MAXS), ("Cv", C.c_double * MAXS),
-                ("R", C.c_double), ("gas_model", C.c_int32)]
+                ("R", C.c_double), ("gas_model", C.c_int32), ("Tcrit", C.c_double * MAXS),
+                ("Pcrit", C.c_double * MAXS)]
 
 
 class SchemeDesc(C.Structure):
@@ -141,7 +142,10 @@ def mix_desc(case: Case) -> MixDesc:
         x.M[k] = float(case.M[k])
         x.Cv[k] = float(case.Cv[k])
     x.R = float(case.R)
-    x.gas_model = 0
+    x.gas_model = int(case.gas_model)
+    for k in range(case.Ns):
+        x.Tcrit[k] = float(case.Tc[k])
+        x.Pcrit[k] = float(case.Pc[k])
     return x
 
 
@@ -299,9 +303,38 @@ class GpuSolver:
         return avg.value, cnt.value
 
 
+_LIBM = None
+
+
+def vdw_matrices(case: Case):
+    """van der Waals mixing matrices a_kl, b_kl (mixtureVanDerWaals.cpp:19-58),
+    with glibc's cbrt and pow as the reference's build uses them."""
+    global _LIBM
+    if _LIBM is None:
+        _LIBM = C.CDLL("libm.so.6")
+        for f in (_LIBM.cbrt, _LIBM.pow, _LIBM.sqrt):
+            f.restype = C.c_double
+        _LIBM.cbrt.argtypes = [C.c_double]
+        _LIBM.sqrt.argtypes = [C.c_double]
+        _LIBM.pow.argtypes = [C.c_double, C.c_double]
+    n = case.Ns
+    a = np.zeros((n, n))
+    b = np.zeros((n, n))
+    for k in range(n):
+        b[k, k] = float(case.Tc[k]) * float(case.R) / (8 * float(case.Pc[k]))
+        a[k, k] = float(case.Pc[k]) * b[k, k] * b[k, k] * 27
+    for k in range(n):
+        for l in range(n):
+            if k != l:
+                b[k, l] = _LIBM.pow(0.5 * (_LIBM.cbrt(b[k, k]) + _LIBM.cbrt(b[l, l])), 3.0)
+                a[k, l] = _LIBM.sqrt(a[k, k] * a[l, l])
+    return a, b
+
+
 def conserved_from_primitive(case: Case, c, v, p) -> np.ndarray:
     """Conserved vector from primitives (c_k, v, p), with the reference's
-    initialisation arithmetic (phaseInitialization.cpp:1827-1883).  Pure host
+    initialisation arithmetic (phaseInitialization.cpp:1827-1883) and its
+    UvFromp (idealFluid.cpp:19-23, vanDerWaalsFluid.cpp:24-31).  Pure host
     arithmetic on numpy arrays; used to prepare initial states."""
     c = np.asarray(c, dtype=np.float64)
     v = np.asarray(v, dtype=np.float64)
@@ -319,7 +352,19 @@ def conserved_from_primitive(case: Case, c, v, p) -> np.ndarray:
     cv = np.zeros(n)
     for k in range(Ns):
         cv = cv + (c[k] / c0) * case.Cv[k]
-    U = p / (case.R / cv)
+    if int(case.gas_model) == 1:
+        am, bm = vdw_matrices(case)
+        X = [c[k] / c0 for k in range(Ns)]
+        a = np.zeros(n)
+        b = np.zeros(n)
+        for k in range(Ns):
+            for l in range(Ns):
+                a = a + X[k] * X[l] * am[k, l]
+                b = b + X[k] * X[l] * bm[k, l]
+        acc = a * (c0 * c0)
+        U = (p + acc) * (1 - c0 * b) / (case.R / cv) - acc
+    else:
+        U = p / (case.R / cv)
     v2 = ((0.0 + v[0] * v[0]) + v[1] * v[1]) + v[2] * v[2]
     E = (U + rho * v2 * 0.5) + 0.0
     return np.concatenate([rho_k, rho[None], m, E[None]])

@@ -1,7 +1,7 @@
 """Small parity cases shared by the CPU and GPU tests (seeded, deterministic)."""
 import numpy as np
 
-from schemi_amd import Case, typeOfTVDLimiter as TL, flowSolver as FS
+from schemi_amd import Case, typeOfTVDLimiter as TL, flowSolver as FS, gasModel as GM
 from schemi_amd import boundaryConditionType as BCT
 from schemi_amd.gpu import periodic
 
@@ -89,4 +89,17 @@ def cases():
     out["3d_lz_short_linear_dyadic"] = c
     c = Case(dim=3, N=(10, 6, 6), L=(1.0, 0.6, 0.65), limiter=TL.linear, flux=FS.HLL, **BASE)
     out["3d_lz_long_linear"] = c
+    # van der Waals mixture (c b ~ 0.1, a c^2 ~ p / 4 for these states)
+    vdw = dict(gas_model=GM.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    out["1d_vdw_vanLeer"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer,
+                                 **BASE, **vdw)
+    c = Case(dim=2, N=(64, 32, 1), L=(1.0, 0.5, 0.1), limiter=TL.minmod, flux=FS.HLL, **BASE,
+             **vdw)
+    periodic(c, 0)
+    periodic(c, 1)
+    out["2d_vdw_periodic_hll"] = c
+    c = Case(dim=3, N=(24, 10, 12), L=(1.0, 0.5, 0.5), limiter=TL.vanAlbada, **BASE, **vdw)
+    out["3d_vdw_vanAlbada"] = c
+    c = Case(dim=2, N=(24, 16, 1), L=(0.7, 0.5, 0.1), limiter=TL.linear, **BASE, **vdw)
+    out["2d_vdw_linear_general"] = c
     return out

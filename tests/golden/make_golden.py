@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 
 from oracle import decks  # noqa: E402
 from oracle.oracle import Oracle  # noqa: E402
-from schemi_amd import Case, typeOfTVDLimiter as TL, flowSolver as FS  # noqa: E402
+from schemi_amd import Case, typeOfTVDLimiter as TL, flowSolver as FS, gasModel  # noqa: E402
 from schemi_amd import boundaryConditionType as BCT  # noqa: E402
 from schemi_amd.cases import cv_of  # noqa: E402
 
@@ -34,7 +34,9 @@ KEYS = ("c0", "c", "rho0", "rho", "v", "p", "U", "T", "m", "E")
 
 
 def case_json(case: Case) -> str:
-    return json.dumps({
+    eos = {} if int(case.gas_model) == 0 else {
+        "gas_model": int(case.gas_model), "Tc": list(case.Tc), "Pc": list(case.Pc)}
+    return json.dumps({**eos,
         "dim": case.dim, "N": list(case.N), "L": list(case.L), "M": list(case.M),
         "Cv": list(case.Cv), "R": case.R, "flux": int(case.flux),
         "limiter": int(case.limiter), "gravity": bool(case.gravity), "g": list(case.g),
@@ -171,6 +173,7 @@ def main():
     c = Case(dim=3, N=(8, 6, 6), L=(1.0, 0.5, 0.5), limiter=TL.linear, **base)
     run_const("seeded_3d_linear", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 16))
     lz_cases(base)
+    vdw_cases(base)
 
 
 def lz_cases(base):
@@ -186,8 +189,27 @@ def lz_cases(base):
     run_const("seeded_3d_lz_long_linear", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 19))
 
 
+def vdw_cases(base):
+    """van der Waals mixture (gasModel::vanDerWaals).  With Tc = (24, 3.2),
+    Pc = (1, 1) and R = 1, c b is about 0.1 and a c^2 about a quarter of p for
+    the seeded states: a clearly non-ideal closure."""
+    eos = dict(gas_model=gasModel.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.minmod, **base, **eos)
+    run_const("seeded_1d_vdw_minmod", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 21))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             **base, **eos)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_vdw_hll", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 22))
+    c = Case(dim=3, N=(8, 6, 6), L=(1.0, 0.5, 0.5), limiter=TL.vanAlbada, **base, **eos)
+    run_const("seeded_3d_vdw_vanalbada", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 23))
+
+
 if __name__ == "__main__":
+    b = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
     if len(sys.argv) > 1 and sys.argv[1] == "lz":
-        lz_cases(dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67]))
+        lz_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "vdw":
+        vdw_cases(b)
     else:
         main()

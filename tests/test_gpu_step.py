@@ -217,7 +217,8 @@ def test_multi_d_mesh_needs_two_cells_per_axis(hip_device):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["1d_vanLeer", "2d_linear", "2d_linear_dyadic", "3d_vanAlbada",
-                                  "3d_periodic_yz_linear"])
+                                  "3d_periodic_yz_linear", "2d_vdw_periodic_hll",
+                                  "3d_vdw_vanAlbada"])
 def test_mixed_precision_all_kernels(name, hip_device):
     """The mixed path of every kernel family stays within 1e-5 (relative L1
     of the density) of the FP64 path over 3 steps and keeps sum rho_k = rho."""

@@ -153,3 +153,33 @@ def test_rk2_linear_amplification():
     U2 = b.conserved()
     expect = 0.5 * U2 + 0.5 * U0
     np.testing.assert_allclose(a.conserved(), expect, rtol=1e-15, atol=0)
+
+
+def test_vdw_tends_to_ideal_mixture():
+    """With the critical pressure raised 1e8-fold at fixed Tc, the van der
+    Waals a_kl ~ 1/Pc and b_kl ~ 1/Pc vanish: one step of the van der Waals
+    oracle approaches the ideal-mixture step (relative 1e-6), while at the
+    test's Pc = 1 the two differ by far more (the model is really engaged)."""
+    from schemi_amd import gasModel
+
+    def run(**eos):
+        case = Case(dim=1, N=(48, 1, 1), L=(1.0, 1.0, 1.0), M=[29.0, 4.0], Cv=[2.5, 1 / 0.67],
+                    **eos)
+        o = Oracle(case)
+        r = np.random.default_rng(3)
+        c = np.stack([0.02 + 0.02 * r.random(48), 0.1 + 0.1 * r.random(48)])
+        v = np.zeros((3, 48))
+        v[0] = 0.3 * r.standard_normal(48)
+        p = 0.8 + 0.4 * r.random(48)
+        o.init_primitive(c, v, p)
+        assert o.step(2e-3, 0) == 0
+        return o.state()
+
+    ideal = run()
+    near = run(gas_model=gasModel.vanDerWaals, Tc=[24.0, 3.2], Pc=[1e8, 1e8])
+    far = run(gas_model=gasModel.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    for key in ("p", "E", "rho0"):
+        scale = np.abs(ideal[key]).max()
+        assert np.abs(near[key] - ideal[key]).max() <= 1e-6 * scale, key
+    # at equal initial p the van der Waals internal energy differs by ~a c0^2
+    assert np.abs(far["E"] - ideal["E"]).max() > 1e-2 * np.abs(ideal["E"]).max()

@@ -30,6 +30,8 @@ _PATCH_LABEL = ("#Tail_boundary_condition", "#Point_boundary_condition",
                 "#Left_boundary_condition", "#Top_boundary_condition")
 _FLUX = {0: "HLL", 2: "HLLC"}
 _LIM = {0: "zero", 1: "minmod", 2: "vanLeer", 3: "linear", 4: "vanAlbada"}
+# step solver -> #Approximation_order keyword (abstractStepSolver.cpp:56-146)
+_STEP = {0: "SecondOrder", 1: "SecondOrderRK"}
 # gasModel -> #Equation_of_state keyword (abstractMixtureThermodynamics.cpp:38-43)
 _EOS = {0: "ideal", 1: "vanDerWaals", 2: "RedlichKwong"}
 
@@ -101,7 +103,7 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         "#Constant_timestep", _r(const_dt),
         "#Source_timestep_correction", "Courant=0.25,0.5",
         "#Reconstruction", "star",
-        "#Approximation_order", "SecondOrderRK",
+        "#Approximation_order", _STEP[int(case.step_solver)],
         "#Start_From_Time-Step", "1" if restart is not None else "no",
         "#Calculation_of_mixing_zone_width", "off",
         "#Nonlinearity_iterations", "off",

@@ -84,6 +84,8 @@ def lib():
         L.or_set_conserved.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.or_step.restype = C.c_int
         L.or_step.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.or_step_hancock.restype = C.c_int
+        L.or_step_hancock.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.or_stage.restype = C.c_int
         L.or_stage.argtypes = [C.c_void_p, C.c_double]
         L.or_courant_dt.restype = C.c_double
@@ -203,6 +205,9 @@ class Oracle:
 
     # -- stepping ------------------------------------------------------------
     def step(self, dt: float, mode: int = 0) -> int:
+        """One step of the case's step solver (SSP-RK2 or MUSCL-Hancock)."""
+        if int(getattr(self.case, "step_solver", 1)) == 0:
+            return lib().or_step_hancock(self._s, float(dt), int(mode))
         return lib().or_step(self._s, float(dt), int(mode))
 
     def stage(self, dt: float) -> int:

@@ -1554,9 +1554,107 @@ int or_face_flux(const or_solver *s, const double *N, const double *cL,
 }
 
 /* ------------------------------------------------------------------------ */
-/* one stage: Advection2dOrder (Advection2dOrder.cpp:16-636)                 */
+/* MUSCL-Hancock predictor (HancockStage.cpp:10-448, HancockStage.hpp:23-89)  */
+
+/* Recomputes a face side's primitives from its conserved values after the
+ * predictor (HancockStage.cpp:174-290): c_k = rho_k / M_k, v = m / rho,
+ * U = E - rho v^2 / 2 - rho k, p = pFromUv, T = TFromUv. */
+static void rederive_face(const or_config *cfg, face_state *fs)
+{
+    double c0 = 0;
+    for (int k = 1; k <= cfg->Ns; ++k) {
+        fs->c[k] = fs->rho[k] / cfg->M[k - 1];
+        c0 += fs->c[k];
+    }
+    fs->c[0] = c0;
+    for (int a = 0; a < 3; ++a)
+        fs->v[a] = fs->m[a] / fs->rho[0];
+    const double v2 = dot3(fs->v, fs->v);
+    fs->U = (fs->E - fs->rho[0] * v2 * 0.5) - 0.0;
+    fs->p = eos_p(cfg, fs->c, fs->U);
+    fs->T = eos_T(cfg, fs->c, fs->U);
+}
+
+/* Each cell's divergence of the physical fluxes of its own face sides
+ * (HancockDivergence for rho_k and rho; the momentum and energy loops of
+ * HancockStage.cpp:36-117), divided by V; every face side of the cell then
+ * moves by -div dt / 2 (HancockTimeIntegration) and recovers its primitives.
+ * The turbulent terms (2/3) rho k are zero. */
+static void hancock_predict(or_solver *s, face_state *FO, face_state *FN, double dt)
+{
+    const or_config *cfg = &s->cfg;
+    const int Ns = cfg->Ns;
+    const double half = 0.5 * dt;
+    for (long i = 0; i < s->nc; ++i) {
+        double dr[OR_MAXS + 1] = {0}, dm[3] = {0, 0, 0}, dE = 0;
+        for (int j = 0; j < s->cnf[i]; ++j) {
+            const long f = s->cf[6 * i + j];
+            const face_state *fs = (s->own[f] == i) ? &FO[f] : &FN[f];
+            double n[3];
+            if (s->own[f] == i)
+                memcpy(n, &s->fN[3 * f], sizeof n);
+            else
+                for (int a = 0; a < 3; ++a)
+                    n[a] = s->fN[3 * f + a] * -1;
+            const double S = s->fS[f];
+            for (int k = 0; k <= Ns; ++k) {
+                double rv[3];
+                for (int a = 0; a < 3; ++a)
+                    rv[a] = fs->v[a] * fs->rho[k];
+                dr[k] += dot3(rv, n) * S;
+            }
+            const double pp = fs->p + (2. / 3.) * 0.0;
+            for (int a = 0; a < 3; ++a) {
+                const double T0 = fs->m[a] * fs->v[0], T1 = fs->m[a] * fs->v[1],
+                             T2 = fs->m[a] * fs->v[2];
+                const double row = (T0 * n[0] + T1 * n[1]) + T2 * n[2];
+                dm[a] += (n[a] * pp + row) * S;
+            }
+            double vE[3], vp[3];
+            for (int a = 0; a < 3; ++a) {
+                vE[a] = fs->v[a] * fs->E;
+                vp[a] = fs->v[a] * pp;
+            }
+            dE += (dot3(vE, n) + dot3(vp, n)) * S;
+        }
+        const double V = s->cV[i];
+        for (int k = 0; k <= Ns; ++k)
+            dr[k] /= V;
+        for (int a = 0; a < 3; ++a)
+            dm[a] /= V;
+        dE /= V;
+        for (int j = 0; j < s->cnf[i]; ++j) {
+            const long f = s->cf[6 * i + j];
+            face_state *fs = (s->own[f] == i) ? &FO[f] : &FN[f];
+            for (int k = 0; k <= Ns; ++k)
+                fs->rho[k] -= dr[k] * half;
+            for (int a = 0; a < 3; ++a)
+                fs->m[a] -= dm[a] * half;
+            fs->E -= dE * half;
+        }
+    }
+    /* owner sides of every face, neighbour sides of inner faces */
+    for (long f = 0; f < s->nf; ++f) {
+        if (s->fpatch[f] == -3)
+            continue;
+        rederive_face(cfg, &FO[f]);
+        if (s->fpatch[f] < 0)
+            rederive_face(cfg, &FN[f]);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* one stage: Advection2dOrder (Advection2dOrder.cpp:16-636), or with the    */
+/* Hancock predictor Advection (Advection.cpp:20-514)                         */
+
+static int stage_impl(or_solver *s, double dt, int hancock);
 
 int or_stage(or_solver *s, double dt)
+{
+    return stage_impl(s, dt, 0);
+}
+
+static int stage_impl(or_solver *s, double dt, int hancock)
 {
     const or_config *cfg = &s->cfg;
     or_state *st = &s->st;
@@ -1615,6 +1713,8 @@ int or_stage(or_solver *s, double dt)
         if (s->fpatch[f] < 0)
             close_face(cfg, &FN[f]);
     }
+    if (hancock)
+        hancock_predict(s, FO, FN, dt);
     /* boundary ghosts from the owner face value (Advection2dOrder.cpp:398-521) */
     for (long f = 0; f < nf && !err; ++f) {
         if (s->fpatch[f] < 0)
@@ -1732,6 +1832,18 @@ int or_step(or_solver *s, double dt, int mode)
     if (!err && mode == 1)
         recover_primitives(s, &s->st);
     free(pool);
+    return err;
+}
+
+/* secondOrderStepSolver::calculateStep (secondOrderStepSolver.cpp:48-93):
+ * one Advection with the Hancock predictor; diffusion and chemistry off. */
+int or_step_hancock(or_solver *s, double dt, int mode)
+{
+    if (mode == 1)
+        recover_primitives(s, &s->st);
+    const int err = stage_impl(s, dt, 1);
+    if (!err && mode == 1)
+        recover_primitives(s, &s->st);
     return err;
 }
 

@@ -107,6 +107,9 @@ void or_set_conserved(or_solver *s, const double *rho_k, const double *rho,
 int or_step(or_solver *s, double dt, int mode);
 /* One forward-Euler stage (Advection2dOrder). */
 int or_stage(or_solver *s, double dt);
+/* One MUSCL-Hancock step (secondOrderStepSolver: Advection with the Hancock
+ * half-step predictor of the face states); mode as for or_step. */
+int or_step_hancock(or_solver *s, double dt, int mode);
 /* Courant time step of the reference main loop (Schemi.cpp:807-833). */
 double or_courant_dt(or_solver *s, double courant, double min_time);
 

@@ -57,6 +57,9 @@ class Case:
     gas_model: gasModel = gasModel.ideal
     Tc: Sequence[float] = ()
     Pc: Sequence[float] = ()
+    # step solver (#Approximation_order): 1 SecondOrderRK (SSP-RK2),
+    # 0 SecondOrder (MUSCL-Hancock)
+    step_solver: int = 1
 
     def __post_init__(self):
         if not self.bc_c:

@@ -36,6 +36,8 @@ KEYS = ("c0", "c", "rho0", "rho", "v", "p", "U", "T", "m", "E")
 def case_json(case: Case) -> str:
     eos = {} if int(case.gas_model) == 0 else {
         "gas_model": int(case.gas_model), "Tc": list(case.Tc), "Pc": list(case.Pc)}
+    if int(case.step_solver) != 1:
+        eos["step_solver"] = int(case.step_solver)
     return json.dumps({**eos,
         "dim": case.dim, "N": list(case.N), "L": list(case.L), "M": list(case.M),
         "Cv": list(case.Cv), "R": case.R, "flux": int(case.flux),
@@ -174,6 +176,7 @@ def main():
     run_const("seeded_3d_linear", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 16))
     lz_cases(base)
     vdw_cases(base)
+    hancock_cases(base)
 
 
 def lz_cases(base):
@@ -205,11 +208,41 @@ def vdw_cases(base):
     run_const("seeded_3d_vdw_vanalbada", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 23))
 
 
+def hancock_cases(base):
+    """MUSCL-Hancock (#Approximation_order SecondOrder): one Advection with
+    the half-step face predictor per step."""
+    h = dict(step_solver=0)
+    for lim in (TL.vanLeer, TL.minmod):
+        c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=lim, **base, **h)
+        run_const(f"seeded_1d_hancock_{lim.name}", c, 5, 2e-4, (1, 5),
+                  restart=seeded_primitives(c, 31))
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), flux=FS.HLL, limiter=TL.vanAlbada,
+             **base, **h)
+    c.set_bc(0, BCT.slip)
+    c.set_bc(1, BCT.fixedValueCell, "p", (1.2, 0, 0))
+    run_const("seeded_1d_hancock_hll_slip_fixed", c, 5, 2e-4, (1, 5),
+              restart=seeded_primitives(c, 32))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             **base, **h)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_hancock_minmod_hll", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 33))
+    c = Case(dim=3, N=(8, 6, 6), L=(1.0, 0.5, 0.5), limiter=TL.vanAlbada, **base, **h)
+    for pt in (2, 3, 4, 5):
+        c.set_bc(pt, BCT.slip)
+    run_const("seeded_3d_hancock_vanalbada", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 34))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.vanLeer, **base, **h,
+             gas_model=gasModel.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    run_const("seeded_2d_hancock_vdw", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 35))
+
+
 if __name__ == "__main__":
     b = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
     if len(sys.argv) > 1 and sys.argv[1] == "lz":
         lz_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "vdw":
         vdw_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "hancock":
+        hancock_cases(b)
     else:
         main()

@@ -24,7 +24,8 @@ def load(name):
                 bc_c=[[BC(t, tuple(v)) for t, v in row] for row in d["bc_c"]],
                 bc_v=[BC(t, tuple(v)) for t, v in d["bc_v"]],
                 bc_p=[BC(t, tuple(v)) for t, v in d["bc_p"]],
-                gas_model=d.get("gas_model", 0), Tc=d.get("Tc", ()), Pc=d.get("Pc", ()))
+                gas_model=d.get("gas_model", 0), Tc=d.get("Tc", ()), Pc=d.get("Pc", ()),
+                step_solver=d.get("step_solver", 1))
     snaps = {int(s): {k: z[f"s{s}_{k}"] for k in KEYS} for s in z["steps"]}
     out = {"case": case, "dts": z["dts"], "snaps": snaps}
     if "times" in z:

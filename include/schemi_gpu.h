@@ -92,12 +92,16 @@ typedef struct {
 /* gasModel (Schemi/enumerations/gasModelEnum.hpp:15-18) */
 #define SCHEMI_GAS_IDEAL 0
 #define SCHEMI_GAS_VAN_DER_WAALS 1
+#define SCHEMI_GAS_REDLICH_KWONG 2
 
 /* Mixture (abstractMixtureThermodynamics::createThermodynamics,
  * Schemi/src/interfaces/abstractMixtureThermodynamics.cpp:32-63): molar
- * masses, molar Cv, and for the van der Waals mixture
- * (mixtureVanDerWaals.cpp:19-58) the critical temperature and pressure of
- * every species (ignored by the ideal mixture). */
+ * masses, molar Cv, and for the van der Waals (mixtureVanDerWaals.cpp:19-58)
+ * and Redlich-Kwong (mixtureRedlichKwong.cpp:19-67) mixtures the critical
+ * temperature and pressure of every species (ignored by the ideal mixture).
+ * The Redlich-Kwong temperature is the root of a cubic in sqrt(T), solved
+ * in FP64 by Newton's method on both precision paths (the reference solves
+ * it in long double; results agree to a few ulps, not bit for bit). */
 typedef struct {
     int32_t ns;
     double M[SCHEMI_MAX_SPECIES];
@@ -108,8 +112,17 @@ typedef struct {
     double Pcrit[SCHEMI_MAX_SPECIES];
 } schemi_mix_desc;
 
-/* Scheme: flux, limiter, precision, gravity, time-step control, and one
- * boundary condition per field (c_k, v, p) and patch. */
+/* typeOfSolverEnum (Schemi/enumerations/typeOfSolverEnum.hpp:15):
+ * SecondOrder is the MUSCL-Hancock step (secondOrderStepSolver.cpp:33-48),
+ * SecondOrderRK the SSP-RK2 step (secondOrderStepSolverRK.cpp:48-71).
+ * MUSCL-Hancock runs on 1-D meshes and wherever the limited slopes vanish
+ * (zero limiter; TVD limiters in 2-D and 3-D); the linear limiter in 2-D and
+ * 3-D with it is rejected with SCHEMI_ERR_INIT. */
+#define SCHEMI_STEP_SECOND_ORDER 0
+#define SCHEMI_STEP_SECOND_ORDER_RK 1
+
+/* Scheme: flux, limiter, precision, gravity, time-step control, one
+ * boundary condition per field (c_k, v, p) and patch, and the step solver. */
 typedef struct {
     int32_t flux;
     int32_t limiter;
@@ -122,6 +135,7 @@ typedef struct {
     schemi_bc bc_c[SCHEMI_MAX_SPECIES][6];
     schemi_bc bc_v[6];
     schemi_bc bc_p[6];
+    int32_t step_solver; /* SCHEMI_STEP_* */
 } schemi_scheme_desc;
 
 typedef struct schemi_gpu_ctx schemi_gpu_ctx;
@@ -177,6 +191,35 @@ int schemi_gpu_device_state(schemi_gpu_ctx *ctx, double **dev_ptr,
 
 /* Blocks until the context's stream is idle; returns any pending error. */
 int schemi_gpu_sync(schemi_gpu_ctx *ctx);
+
+/* Device-to-device state copies (SoA, Ns + 5 rows of ncells doubles, cells
+ * in the reference's order): src (nullable) becomes the current state, then
+ * the current state is copied to dst (nullable).  For checkpoints kept in
+ * device memory by the caller; synchronous. */
+int schemi_gpu_state_device(schemi_gpu_ctx *ctx, const double *src, double *dst);
+
+/* Adjoint (no counterpart in the reference: the differentiation of the
+ * north-star harness).  With the current state U^n and device arrays in the
+ * state layout, what = SCHEMI_ADJ_STEP writes
+ *   lambda_out = (dU^{n+1}/dU^n)^T lambda_in
+ * for one SSP-RK2 step of size dt; SCHEMI_ADJ_STAGE the same for one stage
+ * (schemi_gpu_stage).  The step is linearised in FP64 at the forward states
+ * (U^1 is recomputed in the context's precision), with dt held fixed and the
+ * flux's branches (HLLC wave pattern, wave-speed min/max) frozen at the
+ * forward switch state.  The current state is left unchanged; lambda_out may
+ * equal lambda_in.  Supported where the face values are the cell values
+ * (zero limiter; TVD limiters in 2-D and 3-D) with SecondOrderRK; otherwise
+ * SCHEMI_ERR_INIT.  Synchronous. */
+#define SCHEMI_ADJ_STEP 0
+#define SCHEMI_ADJ_STAGE 1
+int schemi_gpu_adjoint_step(schemi_gpu_ctx *ctx, double dt, const double *lambda_in,
+                            double *lambda_out, int32_t what);
+
+/* Mixing width J = sum_i V_i Y_i (1 - Y_i), Y = rho_species / rho, of the
+ * current state (no counterpart in the reference, whose mixedZoneWidth1D,
+ * output.cpp:310-391, is a threshold width); dJ/dU into dJdU (device, state
+ * layout, nullable).  Deterministic; synchronous. */
+int schemi_gpu_mixing_width(schemi_gpu_ctx *ctx, int32_t species, double *J, double *dJdU);
 
 /* Average duration (ms) of the last stage-kernel launches timed with HIP
  * events on the context's stream (0 when none were timed); enables timing

@@ -626,10 +626,185 @@ static void vdw_mix(const or_config *cfg, const double *c, double *am, double *b
     *bm = b;
 }
 
+/* Redlich-Kwong mixture (mixtureRedlichKwong.cpp:19-67): chi = cbrt(2) - 1,
+ * Sigma_a = 1 / (9 chi), Sigma_b = chi / 3; per species
+ * b_kk = Sigma_b R Tc / Pc, a_kk = Sigma_a R R Tc Tc sqrt(Tc) / Pc; across
+ * species b_kl = (0.5 (cbrt b_kk + cbrt b_ll))^3, a_kl = sqrt(a_kk a_ll).
+ * The reference's chi is a constant expression under GCC, folded with the
+ * correctly rounded cube root 0x1.428a2f98d728bp+0 (glibc's cbrt(2.0) is one
+ * ulp above it). */
+static void rk_matrices(or_config *cfg)
+{
+    const double chi = 0x1.428a2f98d728bp+0 - 1.;
+    const double Sa = 1. / (9. * chi), Sb = chi / 3.;
+    const int n = cfg->Ns;
+    for (int k = 0; k < n; ++k) {
+        const double R = cfg->R, Tc = cfg->Tc[k], Pc = cfg->Pc[k];
+        cfg->b_kl[k * OR_MAXS + k] = Sb * R * Tc / Pc;
+        cfg->a_kl[k * OR_MAXS + k] = Sa * R * R * Tc * Tc * sqrt(Tc) / Pc;
+    }
+    for (int k = 0; k < n; ++k)
+        for (int l = 0; l < n; ++l)
+            if (k != l) {
+                cfg->b_kl[k * OR_MAXS + l] = pow(
+                    0.5 * (cbrt(cfg->b_kl[k * OR_MAXS + k]) + cbrt(cfg->b_kl[l * OR_MAXS + l])),
+                    3);
+                cfg->a_kl[k * OR_MAXS + l] =
+                    sqrt(cfg->a_kl[k * OR_MAXS + k] * cfg->a_kl[l * OR_MAXS + l]);
+            }
+}
+
+/* signum with the reference's dead band (sign.cpp:12-24) */
+static int rk_sign(double x)
+{
+    if (x > STAB)
+        return 1;
+    if (x < -STAB)
+        return -1;
+    return 0;
+}
+
+/* cubicEquationSolver (cubicEquationSolver.cpp:17-106): roots of
+ * A x^3 + B x^2 + C x + D in long double (trigonometric / hyperbolic
+ * Cardano), or of the quadratic B x^2 + C x + D when |A| <= 1e-9; missing
+ * roots are NaN.  Constants onethirds, twothirds, Pi are doubles; the
+ * coefficient quotients B / A etc. are double divisions. */
+static void rk_cubic(double A, double B, double C, double D, double x[3])
+{
+    const double third = 1. / 3., two3pi = (2. / 3.) * 3.141592653589793;
+    x[0] = x[1] = x[2] = NAN;
+    if (A > 1e-9 || A < -1e-9) {
+        const long double a = B / A, b = C / A, c = D / A;
+        const long double Q = (a * a - 3 * b) / 9;
+        const long double R = (a * (2 * (a * a) - 9 * b) + 27 * c) / 54;
+        const long double S = Q * Q * Q - R * R;
+        if (S >= STAB) {
+            const long double phi = third * acosl(R / sqrtl(Q * Q * Q));
+            x[0] = -2. * sqrtl(Q) * cosl(phi) - a * third;
+            x[1] = -2. * sqrtl(Q) * cosl(phi - two3pi) - a * third;
+            x[2] = -2. * sqrtl(Q) * cosl(phi + two3pi) - a * third;
+        } else if (S < STAB && S >= -STAB) {
+            x[0] = -2. * rk_sign((double)R) * sqrtl(Q) - a * third;
+            x[1] = rk_sign((double)R) * sqrtl(Q) - a * third;
+            x[2] = x[1];
+        } else if (S < -STAB) {
+            if (Q >= STAB) {
+                const long double phi = third * acoshl(fabsl(R) / (sqrtl(Q * Q * Q) + STAB));
+                x[0] = -2. * rk_sign((double)R) * sqrtl(Q) * coshl(phi) - a * third;
+            } else if (Q < STAB && Q >= -STAB) {
+                x[0] = -cbrtl(c - a * a * a / 27) - a * third;
+            } else if (Q < -STAB) {
+                const long double phi =
+                    third * asinhl(fabsl(R) / (sqrtl(fabsl(Q * Q * Q)) + STAB));
+                x[0] = -2. * rk_sign((double)R) * sqrtl(fabsl(Q)) * sinhl(phi) - a * third;
+            }
+        }
+    } else {
+        const long double a = B, b = C, c = D;
+        const long double Discr = b * b - 4 * a * c;
+        if (Discr >= 0) {
+            const long double a2 = 0.5 / (a + STAB);
+            x[0] = a2 * (-b + sqrtl(Discr));
+            x[1] = a2 * (-b - sqrtl(Discr));
+        }
+    }
+}
+
+/* returnSinglePosValue (cubicEquationSolver.cpp:108-150): the one
+ * non-negative root (NaN when the reference would throw) */
+static double rk_single_pos(double x[3])
+{
+    int np = 0;
+    for (int i = 0; i < 3; ++i) {
+        if (x[i] >= 0)
+            np++;
+        else
+            x[i] = -123.;
+    }
+    if (np == 1) {
+        const double m = (x[0] < x[1]) ? x[1] : x[0];
+        return (m < x[2]) ? x[2] : m;
+    }
+    if (np == 2) {
+        if ((x[0] - x[1]) < STAB && x[2] < 0)
+            return x[0];
+        if ((x[1] - x[2]) < STAB && x[0] < 0)
+            return x[1];
+        if ((x[0] - x[2]) < STAB && x[1] < 0)
+            return x[2];
+        return NAN;
+    }
+    if (np == 3 && (x[0] - x[1]) < STAB && (x[1] - x[2]) < STAB)
+        return x[0];
+    return NAN;
+}
+
+/* sqrt(T) from U: the positive root of c Cv s^3 - U s - 1.5 a c / b log1p(c b)
+ * (RedlichKwongFluid.cpp:46-56, 89-97) */
+static double rk_sqrtT(double Cv, double U, double c, double a, double b)
+{
+    double x[3];
+    rk_cubic(c * Cv, 0, -U, -1.5 * a * c / b * log1p(c * b), x);
+    return rk_single_pos(x);
+}
+
+/* a_mix, b_mix, Cv_mix and M_mix of the Redlich-Kwong mixture
+ * (mixtureRedlichKwong.cpp:604-625, 764-788) */
+static void rk_mix(const or_config *cfg, const double *c, double *am, double *bm, double *cv,
+                   double *mm)
+{
+    double X[OR_MAXS];
+    double cvs = 0, ms = 0;
+    for (int k = 0; k < cfg->Ns; ++k) {
+        X[k] = c[k + 1] / c[0];
+        cvs += X[k] * cfg->Cv[k];
+        ms += X[k] * cfg->M[k];
+    }
+    double a = 0, b = 0;
+    for (int k = 0; k < cfg->Ns; ++k)
+        for (int l = 0; l < cfg->Ns; ++l) {
+            a += X[k] * X[l] * cfg->a_kl[k * OR_MAXS + l];
+            b += X[k] * X[l] * cfg->b_kl[k * OR_MAXS + l];
+        }
+    *am = a;
+    *bm = b;
+    *cv = cvs;
+    *mm = ms;
+}
+
+/* RedlichKwongFluid.cpp:14-44: dP/dC, dP/dT, dUv/dC, dUv/dT at (c, T) */
+static double rk_dPdC(double c, double T, double a, double b, double R)
+{
+    return R * T / (1 - c * b) + c * R * T * b / ((1 - c * b) * (1 - c * b)) -
+           2 * a * c / (sqrt(T) * (1 + b * c)) +
+           a * (c * c) * b / (sqrt(T) * ((1 + b * c) * (1 + b * c)));
+}
+static double rk_dPdT(double c, double T, double a, double b, double R)
+{
+    const double sT = sqrt(T);
+    return c * R / (1 - c * b) + 0.5 * a * (c * c) / (sT * sT * sT * (1 + c * b));
+}
+static double rk_dUvdC(double c, double Cv, double T, double a, double b)
+{
+    return Cv * T - 1.5 * a * log1p(c * b) / (b * sqrt(T)) - 1.5 * a * c / (sqrt(T) * (1 + c * b));
+}
+static double rk_dUvdT(double c, double Cv, double T, double a, double b)
+{
+    const double sT = sqrt(T);
+    return c * Cv + 0.75 * a * c * log1p(c * b) / (b * (sT * sT * sT));
+}
+
 /* pFromUv: ideal (R / Cv) U (mixtureIdeal.cpp:383-395, idealFluid.cpp:13-17);
  * vdW g1 (U + a c^2) / (1 - c b) - a c^2 (vanDerWaalsFluid.cpp:14-22) */
 static double eos_p(const or_config *cfg, const double *c, double U)
 {
+    if (cfg->gas_model == OR_GAS_RK) {
+        /* RedlichKwongFluid.cpp:46-57 */
+        double a, b, cv, mm;
+        rk_mix(cfg, c, &a, &b, &cv, &mm);
+        const double s = rk_sqrtT(cv, U, c[0], a, b);
+        return c[0] * cfg->R * (s * s) / (1 - c[0] * b) - a * (c[0] * c[0]) / (s * (1 + c[0] * b));
+    }
     if (cfg->gas_model == OR_GAS_VDW) {
         double a, b;
         vdw_mix(cfg, c, &a, &b);
@@ -643,6 +818,15 @@ static double eos_p(const or_config *cfg, const double *c, double U)
  * vdW (p + a c^2) (1 - c b) / g1 - a c^2 (vanDerWaalsFluid.cpp:24-31) */
 static double eos_U(const or_config *cfg, const double *c, double p)
 {
+    if (cfg->gas_model == OR_GAS_RK) {
+        /* RedlichKwongFluid.cpp:59-70 */
+        double a, b, cv, mm, x[3];
+        rk_mix(cfg, c, &a, &b, &cv, &mm);
+        const double c0 = c[0];
+        rk_cubic(c0 * cfg->R, 0, -p * (1 - c0 * b), -a * (c0 * c0) * (1 - c0 * b) / (1 + c0 * b), x);
+        const double s = rk_single_pos(x);
+        return c0 * cv * (s * s) - 1.5 * a * c0 / (b * s) * log1p(c0 * b);
+    }
     if (cfg->gas_model == OR_GAS_VDW) {
         double a, b;
         vdw_mix(cfg, c, &a, &b);
@@ -657,6 +841,13 @@ static double eos_U(const or_config *cfg, const double *c, double p)
  * (vanDerWaalsFluid.cpp:45-49) */
 static double eos_T(const or_config *cfg, const double *c, double U)
 {
+    if (cfg->gas_model == OR_GAS_RK) {
+        /* RedlichKwongFluid.cpp:89-97 */
+        double a, b, cv, mm;
+        rk_mix(cfg, c, &a, &b, &cv, &mm);
+        const double s = rk_sqrtT(cv, U, c[0], a, b);
+        return s * s;
+    }
     if (cfg->gas_model == OR_GAS_VDW) {
         double a, b;
         vdw_mix(cfg, c, &a, &b);
@@ -666,6 +857,15 @@ static double eos_T(const or_config *cfg, const double *c, double U)
 }
 /* dpdUv: ideal R / Cv; vdW g1 / (1 - c b) (vanDerWaalsFluid.cpp:79-83,
  * mixtureVanDerWaals.cpp:746-762) */
+static double eos_dpdUv_rk(const or_config *cfg, const double *c, double U)
+{
+    /* RedlichKwongFluid.cpp:117-127 */
+    double a, b, cv, mm;
+    rk_mix(cfg, c, &a, &b, &cv, &mm);
+    const double s = rk_sqrtT(cv, U, c[0], a, b);
+    const double T = s * s;
+    return rk_dPdT(c[0], T, a, b, cfg->R) / rk_dUvdT(c[0], cv, T, a, b);
+}
 static double eos_dpdUv(const or_config *cfg, const double *c)
 {
     if (cfg->gas_model == OR_GAS_VDW) {
@@ -683,6 +883,20 @@ static double eos_dpdUv(const or_config *cfg, const double *c)
 static double eos_c2(const or_config *cfg, const double *c, double rho, double U,
                      double p)
 {
+    if (cfg->gas_model == OR_GAS_RK) {
+        /* RedlichKwongFluid.cpp:99-115, mixtureRedlichKwong.cpp:764-838 */
+        double a, b, cv, mm;
+        rk_mix(cfg, c, &a, &b, &cv, &mm);
+        const double c0 = c[0];
+        const double s = rk_sqrtT(cv, U, c0, a, b);
+        const double T = s * s;
+        const double dpdrho = (rk_dPdC(c0, T, a, b, cfg->R) -
+                               rk_dPdT(c0, T, a, b, cfg->R) * rk_dUvdC(c0, cv, T, a, b) /
+                                   rk_dUvdT(c0, cv, T, a, b)) /
+                              mm;
+        const double dpdUv = eos_dpdUv_rk(cfg, c, U);
+        return dpdrho + dpdUv * (U + p) / rho;
+    }
     if (cfg->gas_model == OR_GAS_VDW) {
         double a, b;
         vdw_mix(cfg, c, &a, &b);
@@ -795,12 +1009,15 @@ or_solver *or_create(const or_config *cfg)
 {
     if (cfg->dim < 1 || cfg->dim > 3 || cfg->Ns < 1 || cfg->Ns > OR_MAXS)
         return NULL;
-    if (cfg->gas_model != OR_GAS_IDEAL && cfg->gas_model != OR_GAS_VDW)
+    if (cfg->gas_model != OR_GAS_IDEAL && cfg->gas_model != OR_GAS_VDW &&
+        cfg->gas_model != OR_GAS_RK)
         return NULL;
     or_solver *s = calloc(1, sizeof(or_solver));
     s->cfg = *cfg;
     if (cfg->gas_model == OR_GAS_VDW)
         vdw_matrices(&s->cfg);
+    else if (cfg->gas_model == OR_GAS_RK)
+        rk_matrices(&s->cfg);
     for (int a = 0; a < cfg->dim; ++a)
         if (axis_periodic(s, a) < 0) {
             free(s);

@@ -6,6 +6,7 @@
 // reduction), so that a run of many steps needs no host round trip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -28,9 +29,9 @@ using namespace sg;
 namespace {
 
 // device scalar block
-enum { SC_DT = 0, SC_TIME, SC_TEND, SC_SIGMAX, SC_DTNEXT, SC_STEPS, SC_DONE, SC_N };
 
-__global__ void finalize_step_kernel(double *sc, const DevParams *P, double const_dt)
+__global__ void finalize_step_kernel(double *sc, const DevParams *P, double const_dt,
+                                     int flip_parity)
 {
     // Schemi.cpp:802-868, 933-935 for one step
     if (sc[SC_DONE] != 0.0)
@@ -52,6 +53,8 @@ __global__ void finalize_step_kernel(double *sc, const DevParams *P, double cons
     sc[SC_TIME] = t;
     sc[SC_DT] = dtn;
     sc[SC_STEPS] = sc[SC_STEPS] + 1.0;
+    if (flip_parity)
+        sc[SC_PARITY] = 1.0 - sc[SC_PARITY];
     reinterpret_cast<unsigned long long *>(sc)[SC_SIGMAX] = 0ull;
     if (!(t < sc[SC_TEND]))
         sc[SC_DONE] = 1.0;
@@ -66,6 +69,19 @@ __global__ void set_scalars_kernel(double *sc, double dt, double t, double tend)
     sc[SC_DTNEXT] = 0.0;
     sc[SC_STEPS] = 0.0;
     sc[SC_DONE] = 0.0;
+    sc[SC_PARITY] = 0.0;
+}
+
+// Moves the current state back into the first buffer when the MUSCL-Hancock
+// steps of a call ended on the second one.
+__global__ void restore_state_kernel(const double *sc, double *U0, const double *U1,
+                                     long long count)
+{
+    if (sc[SC_PARITY] == 0.0)
+        return;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+         i += (long long)gridDim.x * blockDim.x)
+        U0[i] = U1[i];
 }
 
 template <int NS>
@@ -104,7 +120,9 @@ __global__ void prims_kernel(const DevParams *__restrict__ Pp, const double *__r
         vdw_ab<double, NS>(P, w, c0, a, b);
         Ut = Uint + a * (c0 * c0);
     }
-    W[(NS + 6) * n + id] = Ut / ((c0 + STAB) * cv);
+    // Redlich-Kwong: T = s^2 from its cubic (RedlichKwongFluid.cpp:89-97)
+    W[(NS + 6) * n + id] =
+        (P.gas_model == GAS_RK) ? rk_T<NS>(P, w, c0, Uint) : Ut / ((c0 + STAB) * cv);
     const double c2 = eos_c2<double, NS>(P, w, c0, gamma1<double, NS>(P, w, c0), u[NS], Uint, p);
     W[(NS + 7) * n + id] = __builtin_sqrt(c2);
 }
@@ -154,6 +172,7 @@ struct schemi_gpu_ctx {
     int *derr = nullptr;
     int dim = 1, ns = 1, lin = 0, prec = 0;
     int mode = MODE_SLOPES;
+    bool hancock = false;  // SCHEMI_STEP_SECOND_ORDER: one MUSCL-Hancock stage per step
     void *scratch = nullptr;  // MODE_GENERAL: primitives and gradients
     void *W = nullptr, *Lg = nullptr;
     long long n = 0;
@@ -164,6 +183,7 @@ struct schemi_gpu_ctx {
     MeshTables tables;
     bool timing = false;
     std::vector<hipEvent_t> ev;  // start/stop pairs of the timed stage launches
+    double *adj = nullptr;       // adjoint: lambda of U^1 (Ns + 5 rows), partial sums
 };
 
 namespace {
@@ -181,7 +201,7 @@ int hip_fail(hipError_t e)
     } while (0)
 
 int launch_stage(schemi_gpu_ctx *c, const double *Uin, double *Uout, const double *Un,
-                 bool stage2)
+                 int variant)
 {
     StageLaunch L;
     L.P = c->dP;
@@ -195,7 +215,7 @@ int launch_stage(schemi_gpu_ctx *c, const double *Uin, double *Uout, const doubl
     L.dim = c->dim;
     L.ns = c->ns;
     L.lin = c->lin;
-    L.stage2 = stage2 ? 1 : 0;
+    L.variant = variant;
     L.mode = c->mode;
     L.gas_model = c->hp.gas_model;
     L.W = c->W;
@@ -224,16 +244,37 @@ int launch_stage(schemi_gpu_ctx *c, const double *Uin, double *Uout, const doubl
 
 int one_step(schemi_gpu_ctx *c)
 {
-    // secondOrderStepSolverRK::calculateStep (secondOrderStepSolverRK.cpp:48-71):
-    // U1 = L(Un); U2 = L(U1); U^{n+1} = 0.5 U2 + 0.5 Un (fused in stage 2)
-    int r = launch_stage(c, c->U[0], c->U[1], nullptr, false);
-    if (r)
-        return r;
-    r = launch_stage(c, c->U[1], c->U[0], c->U[0], true);
-    if (r)
-        return r;
+    if (c->hancock) {
+        // secondOrderStepSolver::calculateStep (secondOrderStepSolver.cpp:33-48):
+        // U^{n+1} = L_H(U^n), one stage; the kernel reads the buffer the
+        // scalar block's parity names and writes the other one
+        const int r = launch_stage(c, c->U[0], c->U[1], nullptr, SV_HANCOCK);
+        if (r)
+            return r;
+    } else {
+        // secondOrderStepSolverRK::calculateStep (secondOrderStepSolverRK.cpp:48-71):
+        // U1 = L(Un); U2 = L(U1); U^{n+1} = 0.5 U2 + 0.5 Un (fused in stage 2)
+        int r = launch_stage(c, c->U[0], c->U[1], nullptr, SV_RK1);
+        if (r)
+            return r;
+        r = launch_stage(c, c->U[1], c->U[0], c->U[0], SV_RK2);
+        if (r)
+            return r;
+    }
     hipLaunchKernelGGL(finalize_step_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, c->dP,
-                       c->const_dt);
+                       c->const_dt, c->hancock ? 1 : 0);
+    return hip_fail(hipGetLastError());
+}
+
+// After the steps of a call: the current state back in U[0].
+int restore_state(schemi_gpu_ctx *c)
+{
+    if (!c->hancock)
+        return SCHEMI_OK;
+    const long long count = (long long)(c->ns + 5) * c->n;
+    const long long blocks = std::min<long long>((count + 255) / 256, 4096);
+    hipLaunchKernelGGL(restore_state_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream,
+                       c->sc, c->U[0], c->U[1], count);
     return hip_fail(hipGetLastError());
 }
 
@@ -251,6 +292,43 @@ bool bc_ok(int t)
 {
     return t == SCHEMI_BC_FREE || t == SCHEMI_BC_SLIP || t == SCHEMI_BC_FIXED_CELL ||
            t == SCHEMI_BC_FIXED_SURFACE || t == SCHEMI_BC_PERIODIC;
+}
+
+}  // namespace
+
+namespace {
+
+int ensure_adj(schemi_gpu_ctx *c)
+{
+    if (c->adj)
+        return SCHEMI_OK;
+    const size_t count = (size_t)(c->ns + 5) * (size_t)c->n + (size_t)(c->n + 255) / 256 + 1;
+    HIPCHK(hipMalloc(reinterpret_cast<void **>(&c->adj), count * sizeof(double)));
+    return SCHEMI_OK;
+}
+
+int adjoint_stage(schemi_gpu_ctx *c, const double *U, const double *lam, double *out,
+                  double dt, double a, const double *add, double b)
+{
+    AdjointLaunch L;
+    L.P = c->dP;
+    L.geo = c->dgeo;
+    L.U = U;
+    L.lam = lam;
+    L.out = out;
+    L.add = add;
+    L.dt = dt;
+    L.a = a;
+    L.b = b;
+    L.dim = c->dim;
+    L.ns = c->ns;
+    L.gas_model = c->hp.gas_model;
+    L.n = c->n;
+    L.stream = c->stream;
+    const int r = launch_adjoint_stage(L);
+    if (r == -1)
+        return SCHEMI_ERR_INIT;
+    return r == 0 ? SCHEMI_OK : SCHEMI_ERR_SYSTEM;
 }
 
 }  // namespace
@@ -292,9 +370,10 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         if (mesh->n[a] < 1 || (dim >= 2 && mesh->n[a] < 2) || mesh->n[a] > (1 << 30))
             return SCHEMI_ERR_MESH;
     if (mix->ns < 1 || mix->ns > 3 ||
-        (mix->gas_model != SCHEMI_GAS_IDEAL && mix->gas_model != SCHEMI_GAS_VAN_DER_WAALS))
+        (mix->gas_model != SCHEMI_GAS_IDEAL && mix->gas_model != SCHEMI_GAS_VAN_DER_WAALS &&
+         mix->gas_model != SCHEMI_GAS_REDLICH_KWONG))
         return SCHEMI_ERR_INIT;
-    if (mix->gas_model == SCHEMI_GAS_VAN_DER_WAALS)
+    if (mix->gas_model != SCHEMI_GAS_IDEAL)
         for (int k = 0; k < mix->ns; ++k)
             if (!(mix->Tcrit[k] > 0.0) || !(mix->Pcrit[k] > 0.0))
                 return SCHEMI_ERR_INIT;
@@ -306,6 +385,9 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         return SCHEMI_ERR_INIT;
     if (scheme->precision != SCHEMI_PREC_FP64 && scheme->precision != SCHEMI_PREC_MIXED)
         return SCHEMI_ERR_INIT;
+    if (scheme->step_solver != SCHEMI_STEP_SECOND_ORDER &&
+        scheme->step_solver != SCHEMI_STEP_SECOND_ORDER_RK)
+        return SCHEMI_ERR_INIT;
 
     schemi_gpu_ctx *c = new (std::nothrow) schemi_gpu_ctx();
     if (!c)
@@ -315,6 +397,7 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     c->lin = (lim == SCHEMI_LIM_LINEAR) ? 1 : 0;
     c->prec = scheme->precision;
     c->const_dt = scheme->const_dt;
+    c->hancock = scheme->step_solver == SCHEMI_STEP_SECOND_ORDER;
     DevParams &P = c->hp;
     std::memset(&P, 0, sizeof P);
     P.Ns = mix->ns;
@@ -352,12 +435,33 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
                                            3.0);
                     P.akl[k][l] = std::sqrt(P.akl[k][k] * P.akl[l][l]);
                 }
-        for (int k = 0; k < mix->ns; ++k)
-            for (int l = 0; l < mix->ns; ++l) {
-                P.aklf[k][l] = static_cast<float>(P.akl[k][l]);
-                P.bklf[k][l] = static_cast<float>(P.bkl[k][l]);
-            }
     }
+    if (mix->gas_model == SCHEMI_GAS_REDLICH_KWONG) {
+        // mixtureRedlichKwong.cpp:19-67: chi = cbrt(2) - 1 (a constant
+        // expression in the reference, folded to the correctly rounded
+        // 0x1.428a2f98d728bp+0), Sigma_a = 1 / (9 chi), Sigma_b = chi / 3,
+        // b_kk = Sigma_b R Tc / Pc, a_kk = Sigma_a R R Tc Tc sqrt(Tc) / Pc,
+        // b_kl = (0.5 (cbrt b_kk + cbrt b_ll))^3, a_kl = sqrt(a_kk a_ll)
+        const double chi = 0x1.428a2f98d728bp+0 - 1.;
+        const double Sa = 1. / (9. * chi), Sb = chi / 3.;
+        for (int k = 0; k < mix->ns; ++k) {
+            const double R = mix->R, Tc = mix->Tcrit[k], Pc = mix->Pcrit[k];
+            P.bkl[k][k] = Sb * R * Tc / Pc;
+            P.akl[k][k] = Sa * R * R * Tc * Tc * std::sqrt(Tc) / Pc;
+        }
+        for (int k = 0; k < mix->ns; ++k)
+            for (int l = 0; l < mix->ns; ++l)
+                if (k != l) {
+                    P.bkl[k][l] = std::pow(0.5 * (std::cbrt(P.bkl[k][k]) + std::cbrt(P.bkl[l][l])),
+                                           3.0);
+                    P.akl[k][l] = std::sqrt(P.akl[k][k] * P.akl[l][l]);
+                }
+    }
+    for (int k = 0; k < mix->ns; ++k)
+        for (int l = 0; l < mix->ns; ++l) {
+            P.aklf[k][l] = static_cast<float>(P.akl[k][l]);
+            P.bklf[k][l] = static_cast<float>(P.bkl[k][l]);
+        }
     P.gravity = scheme->gravity ? 1 : 0;
     for (int b = 0; b < 3; ++b)
         P.g[b] = scheme->g[b];
@@ -427,13 +531,19 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     // limiter evaluation is requested.  The linear limiter keeps its slopes;
     // on non-separable meshes its face offsets have transverse components.
     const char *full = std::getenv("SCHEMI_GPU_FULL_LIMITER");
-    const bool force_slopes = full && full[0] == '1';
+    const bool force_slopes = full && full[0] == '1' && !c->hancock;
     if (lim == SCHEMI_LIM_LINEAR)
         c->mode = (dim >= 2 && !separable) ? MODE_GENERAL : MODE_SLOPES;
     else if (lim == SCHEMI_LIM_ZERO || dim >= 2)
         c->mode = force_slopes ? MODE_SLOPES : MODE_FIRST_ORDER;
     else
         c->mode = MODE_SLOPES;
+    // MUSCL-Hancock runs where the slopes vanish and on 1-D meshes; the
+    // linear limiter's 2-D/3-D predictor has no kernel here
+    if (c->hancock && c->mode != MODE_FIRST_ORDER && dim >= 2) {
+        delete c;
+        return SCHEMI_ERR_INIT;
+    }
     // periodic axes need whole tiles so the wrapped neighbours are staged
     {
         const bool fo = c->mode == MODE_FIRST_ORDER;
@@ -525,6 +635,7 @@ void schemi_gpu_destroy(schemi_gpu_ctx *c)
     (void)hipFree(c->sc);
     (void)hipFree(c->derr);
     (void)hipFree(c->scratch);
+    (void)hipFree(c->adj);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
@@ -585,6 +696,8 @@ int schemi_gpu_step(schemi_gpu_ctx *c, double dt, double *dt_next)
                        HUGE_VAL);
     HIPCHK(hipGetLastError());
     int r = one_step(c);
+    if (!r)
+        r = restore_state(c);
     if (r)
         return r;
     double sc[SC_N];
@@ -615,6 +728,8 @@ int schemi_gpu_run(schemi_gpu_ctx *c, int64_t nsteps, double dt0, double t_end, 
         if (r)
             return r;
     }
+    if (const int r = restore_state(c))
+        return r;
     double sc[SC_N];
     HIPCHK(hipMemcpyAsync(sc, c->sc, sizeof sc, hipMemcpyDeviceToHost, c->stream));
     const int r = read_error(c);
@@ -637,7 +752,7 @@ int schemi_gpu_stage(schemi_gpu_ctx *c, double dt)
     hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, dt, c->time,
                        HUGE_VAL);
     HIPCHK(hipGetLastError());
-    int r = launch_stage(c, c->U[0], c->U[1], nullptr, false);
+    int r = launch_stage(c, c->U[0], c->U[1], nullptr, c->hancock ? SV_HANCOCK : SV_RK1);
     if (r)
         return r;
     const size_t bytes = sizeof(double) * (size_t)(c->ns + 5) * (size_t)c->n;
@@ -699,6 +814,67 @@ int schemi_gpu_device_state(schemi_gpu_ctx *c, double **dev_ptr, int64_t *ncells
     if (ncells)
         *ncells = c->n;
     return SCHEMI_OK;
+}
+
+int schemi_gpu_state_device(schemi_gpu_ctx *c, const double *src, double *dst)
+{
+    if (!c)
+        return SCHEMI_ERR_INIT;
+    const size_t bytes = sizeof(double) * (size_t)(c->ns + 5) * (size_t)c->n;
+    if (src)
+        HIPCHK(hipMemcpyAsync(c->U[0], src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (dst)
+        HIPCHK(hipMemcpyAsync(dst, c->U[0], bytes, hipMemcpyDeviceToDevice, c->stream));
+    return read_error(c);
+}
+
+int schemi_gpu_adjoint_step(schemi_gpu_ctx *c, double dt, const double *lam_in, double *lam_out,
+                            int32_t what)
+{
+    if (!c || !lam_in || !lam_out || (what != SCHEMI_ADJ_STEP && what != SCHEMI_ADJ_STAGE))
+        return SCHEMI_ERR_INIT;
+    if (c->mode != MODE_FIRST_ORDER || c->hancock || c->hp.gas_model == GAS_RK)
+        return SCHEMI_ERR_INIT;
+    if (c->pending_error)
+        return c->pending_error;
+    if (const int r = ensure_adj(c))
+        return r;
+    if (what == SCHEMI_ADJ_STAGE) {
+        // lam_out = (dL/dU)^T lam_in at U^n
+        if (const int r = adjoint_stage(c, c->U[0], lam_in, c->adj, dt, 1.0, nullptr, 0.0))
+            return r;
+        const size_t bytes = sizeof(double) * (size_t)(c->ns + 5) * (size_t)c->n;
+        HIPCHK(hipMemcpyAsync(lam_out, c->adj, bytes, hipMemcpyDeviceToDevice, c->stream));
+        return read_error(c);
+    }
+    // U^1 = L(U^n) recomputed into U[1]; U^{n+1} = (L(U^1) + U^n) / 2, so
+    // lam^n = (1/2) (dL/dU|_{U^n})^T (dL/dU|_{U^1})^T lam^{n+1} + (1/2) lam^{n+1}
+    hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, dt, c->time,
+                       HUGE_VAL);
+    HIPCHK(hipGetLastError());
+    if (const int r = launch_stage(c, c->U[0], c->U[1], nullptr, SV_RK1))
+        return r;
+    if (const int r = adjoint_stage(c, c->U[1], lam_in, c->adj, dt, 1.0, nullptr, 0.0))
+        return r;
+    if (const int r = adjoint_stage(c, c->U[0], c->adj, lam_out, dt, 0.5, lam_in, 0.5))
+        return r;
+    return read_error(c);
+}
+
+int schemi_gpu_mixing_width(schemi_gpu_ctx *c, int32_t species, double *J, double *dJdU)
+{
+    if (!c || !J || species < 0 || species >= c->ns)
+        return SCHEMI_ERR_INIT;
+    if (const int r = ensure_adj(c))
+        return r;
+    double *partial = c->adj + (size_t)(c->ns + 5) * (size_t)c->n;
+    double *dJ = partial + (c->n + 255) / 256;
+    const int r = launch_mixing_width(c->dP, c->dgeo, c->U[0], c->ns, c->n, species, partial, dJ,
+                                      dJdU, c->stream);
+    if (r)
+        return r == -1 ? SCHEMI_ERR_INIT : SCHEMI_ERR_SYSTEM;
+    HIPCHK(hipMemcpyAsync(J, dJ, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return read_error(c);
 }
 
 int schemi_gpu_sync(schemi_gpu_ctx *c)

@@ -25,7 +25,17 @@ enum : int { BC_FREE = 1, BC_SLIP = 2, BC_FIXED_CELL = 3, BC_FIXED_SURF = 4,
 enum : int { LIM_ZERO = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_LINEAR = 3,
              LIM_VANALBADA = 4 };
 enum : int { FLUX_HLL = 0, FLUX_HLLC = 2 };
-enum : int { GAS_IDEAL = 0, GAS_VDW = 1 };  // gasModel (gasModelEnum.hpp:15-18)
+// gasModel (gasModelEnum.hpp:15-18): ideal, vanDerWaals, RedlichKwong
+enum : int { GAS_IDEAL = 0, GAS_VDW = 1, GAS_RK = 2 };
+// stage variants: the two SSP-RK2 stages and the single MUSCL-Hancock stage
+enum : int { SV_RK1 = 0, SV_RK2 = 1, SV_HANCOCK = 2 };
+// Scalar block of a context (device memory, one double each): time step,
+// time, end time, the signal-speed maximum (bits of a non-negative double,
+// reduced with atomicMax), next time step, step count, end-of-run flag, and
+// the MUSCL-Hancock buffer parity (1: the current state is in the second of
+// the two state buffers).
+enum : int { SC_DT = 0, SC_TIME, SC_TEND, SC_SIGMAX, SC_DTNEXT, SC_STEPS, SC_DONE, SC_PARITY,
+             SC_N };
 enum : int { ERRF_RIEMANN = 1, ERRF_BOUNDARY = 2 };
 
 // Parameters shared by all kernels of a context, kept in device memory and
@@ -81,12 +91,15 @@ template <> struct Real<float> {
     __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
 };
 
+__device__ __forceinline__ double tabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float tabs(float x) { return __builtin_fabsf(x); }
+
 // libstdc++ 11 std::hypot(x, y, z), used by vector::mag (vector.cpp:37-43).
 template <class T> __device__ inline T hypot3(T x, T y, T z)
 {
-    x = __builtin_fabs(x);
-    y = __builtin_fabs(y);
-    z = __builtin_fabs(z);
+    x = tabs(x);
+    y = tabs(y);
+    z = tabs(z);
     const T a = x < y ? (y < z ? z : y) : (x < z ? z : x);
     if (a != T(0))
         return a * Real<T>::sqrt_((x / a) * (x / a) + (y / a) * (y / a) + (z / a) * (z / a));
@@ -151,12 +164,147 @@ template <int GAS> __device__ __forceinline__ bool is_vdw(const DevParams &P)
     else
         return P.gas_model == GAS_VDW;
 }
+// Redlich-Kwong code is compiled only where the model can occur.
+template <int GAS> constexpr bool may_be_rk() { return GAS < 0 || GAS == GAS_RK; }
+template <int GAS> __device__ __forceinline__ bool is_rk(const DevParams &P)
+{
+    if constexpr (GAS >= 0)
+        return GAS == GAS_RK;
+    else
+        return P.gas_model == GAS_RK;
+}
+
+// ---- Redlich-Kwong mixture, evaluated in FP64 on both paths --------------
+// The reference solves its cubics with a long-double trigonometric Cardano
+// formula (cubicEquationSolver.cpp:17-106) and keeps the single non-negative
+// root.  Every cubic here is A s^3 + C s + D = 0 with A > 0 and D < 0, whose
+// positive root is unique; it is found by Newton's method in FP64, starting
+// from the root without D (C < 0) or without C (C >= 0), after which the
+// iterates decrease monotonically (f is convex for s > 0).  The result is
+// within a few ulps of the reference's root.
+__device__ inline double rk_root(double A, double C, double D)
+{
+    double s = (C < 0.0) ? __builtin_sqrt(-C / A) : ::cbrt(-D / A);
+    for (int it = 0; it < 60; ++it) {
+        const double s2 = s * s;
+        const double f = (A * s2 + C) * s + D;
+        const double ds = f / (3.0 * A * s2 + C);
+        s -= ds;
+        if (!(__builtin_fabs(ds) > 1e-13 * s))
+            break;
+    }
+    return s;
+}
+
+// a_mix, b_mix (the van der Waals mixing rule, mixtureRedlichKwong.cpp:
+// 604-625) and Cv_mix = sum_k X_k Cv_k, M_mix = sum_k X_k M_k in FP64.
+template <int NS>
+__device__ __forceinline__ void rk_mix(const DevParams &P, const double *c, double c0, double &a,
+                                       double &b, double &cv, double &mm)
+{
+    double X[NS];
+    cv = 0.0;
+    mm = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        X[k] = c[k] / c0;
+        cv += X[k] * P.Cv[k];
+        mm += X[k] * P.M[k];
+    }
+    a = 0.0;
+    b = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+            a += X[k] * X[l] * P.akl[k][l];
+            b += X[k] * X[l] * P.bkl[k][l];
+        }
+}
+
+// sqrt(T) from U: root of c Cv s^3 - U s - 1.5 a c / b log1p(c b)
+// (RedlichKwongFluid.cpp:46-56, 89-97)
+__device__ __forceinline__ double rk_sqrtT(double cv, double U, double c0, double a, double b)
+{
+    return rk_root(c0 * cv, -U, -1.5 * a * c0 / b * ::log1p(c0 * b));
+}
+
+// pFromUv (RedlichKwongFluid.cpp:46-57)
+template <int NS>
+__device__ inline double rk_p(const DevParams &P, const double *c, double c0, double U)
+{
+    double a, b, cv, mm;
+    rk_mix<NS>(P, c, c0, a, b, cv, mm);
+    const double s = rk_sqrtT(cv, U, c0, a, b);
+    return c0 * P.R * (s * s) / (1 - c0 * b) - a * (c0 * c0) / (s * (1 + c0 * b));
+}
+
+// UvFromp (RedlichKwongFluid.cpp:59-70)
+template <int NS>
+__device__ inline double rk_U(const DevParams &P, const double *c, double c0, double p)
+{
+    double a, b, cv, mm;
+    rk_mix<NS>(P, c, c0, a, b, cv, mm);
+    const double s = rk_root(c0 * P.R, -p * (1 - c0 * b),
+                             -a * (c0 * c0) * (1 - c0 * b) / (1 + c0 * b));
+    return c0 * cv * (s * s) - 1.5 * a * c0 / (b * s) * ::log1p(c0 * b);
+}
+
+// TFromUv (RedlichKwongFluid.cpp:89-97)
+template <int NS>
+__device__ inline double rk_T(const DevParams &P, const double *c, double c0, double U)
+{
+    double a, b, cv, mm;
+    rk_mix<NS>(P, c, c0, a, b, cv, mm);
+    const double s = rk_sqrtT(cv, U, c0, a, b);
+    return s * s;
+}
+
+// sqSonicSpeed = dpdrho + dpdUv (U + p) / rho with dpdrho = (dP/dC - dP/dT
+// dUv/dC / dUv/dT) / M_mix, dpdUv = dP/dT / dUv/dT at T(U)
+// (RedlichKwongFluid.cpp:14-44, 99-127; mixtureRedlichKwong.cpp:764-838)
+template <int NS>
+__device__ inline double rk_c2(const DevParams &P, const double *c, double c0, double rho,
+                               double U, double p)
+{
+    double a, b, cv, mm;
+    rk_mix<NS>(P, c, c0, a, b, cv, mm);
+    const double R = P.R;
+    const double s = rk_sqrtT(cv, U, c0, a, b);
+    const double T = s * s;
+    const double sT = __builtin_sqrt(T);
+    const double l1 = ::log1p(c0 * b);
+    const double dPdC = R * T / (1 - c0 * b) + c0 * R * T * b / ((1 - c0 * b) * (1 - c0 * b)) -
+                        2 * a * c0 / (sT * (1 + b * c0)) +
+                        a * (c0 * c0) * b / (sT * ((1 + b * c0) * (1 + b * c0)));
+    const double dPdT = c0 * R / (1 - c0 * b) + 0.5 * a * (c0 * c0) / (sT * sT * sT * (1 + c0 * b));
+    const double dUdC = cv * T - 1.5 * a * l1 / (b * sT) - 1.5 * a * c0 / (sT * (1 + c0 * b));
+    const double dUdT = c0 * cv + 0.75 * a * c0 * l1 / (b * (sT * sT * sT));
+    const double dpdrho = (dPdC - dPdT * dUdC / dUdT) / mm;
+    return dpdrho + dPdT / dUdT * (U + p) / rho;
+}
+
+template <class T, int NS>
+__device__ __forceinline__ void to_f64(const T *c, double *cd)
+{
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        cd[k] = static_cast<double>(c[k]);
+}
 
 // pFromUv: ideal g1 U (idealFluid.cpp:13-17); van der Waals
-// g1 (U + a c0^2) / (1 - c0 b) - a c0^2 (vanDerWaalsFluid.cpp:14-22).
+// g1 (U + a c0^2) / (1 - c0 b) - a c0^2 (vanDerWaalsFluid.cpp:14-22);
+// Redlich-Kwong rk_p.
 template <class T, int NS, int GAS = -1>
 __device__ __forceinline__ T eos_p(const DevParams &P, const T *c, T c0, T g1, T U)
 {
+    if constexpr (may_be_rk<GAS>()) {
+        if (is_rk<GAS>(P)) {
+            double cd[NS];
+            to_f64<T, NS>(c, cd);
+            return T(rk_p<NS>(P, cd, double(c0), double(U)));
+        }
+    }
     if (is_vdw<GAS>(P)) {
         T a, b;
         vdw_ab<T, NS>(P, c, c0, a, b);
@@ -167,10 +315,18 @@ __device__ __forceinline__ T eos_p(const DevParams &P, const T *c, T c0, T g1, T
 }
 
 // UvFromp: ideal p / g1 (idealFluid.cpp:19-23); van der Waals
-// (p + a c0^2) (1 - c0 b) / g1 - a c0^2 (vanDerWaalsFluid.cpp:24-31).
+// (p + a c0^2) (1 - c0 b) / g1 - a c0^2 (vanDerWaalsFluid.cpp:24-31);
+// Redlich-Kwong rk_U.
 template <class T, int NS, int GAS = -1>
 __device__ __forceinline__ T eos_U(const DevParams &P, const T *c, T c0, T g1, T p)
 {
+    if constexpr (may_be_rk<GAS>()) {
+        if (is_rk<GAS>(P)) {
+            double cd[NS];
+            to_f64<T, NS>(c, cd);
+            return T(rk_U<NS>(P, cd, double(c0), double(p)));
+        }
+    }
     if (is_vdw<GAS>(P)) {
         T a, b;
         vdw_ab<T, NS>(P, c, c0, a, b);
@@ -184,11 +340,19 @@ __device__ __forceinline__ T eos_U(const DevParams &P, const T *c, T c0, T g1, T
 // mixtureVanDerWaals.cpp:779-785).  Ideal: dpdrho = 0, dpdUv = g1.  Van der
 // Waals (vanDerWaalsFluid.cpp:61-83, mixtureVanDerWaals.cpp:719-762): with
 // m = 1 - c0 b, ac = a c0: dpdrho = (g1 (2 ac / m - b (U + ac c0) / m^2)
-// - 2 ac) / M_mix, M_mix = sum_k X_k M_k, dpdUv = g1 / m.
+// - 2 ac) / M_mix, M_mix = sum_k X_k M_k, dpdUv = g1 / m.  Redlich-Kwong
+// rk_c2.
 template <class T, int NS, int GAS = -1>
 __device__ __forceinline__ T eos_c2(const DevParams &P, const T *c, T c0, T g1, T rho, T U,
                                     T p)
 {
+    if constexpr (may_be_rk<GAS>()) {
+        if (is_rk<GAS>(P)) {
+            double cd[NS];
+            to_f64<T, NS>(c, cd);
+            return T(rk_c2<NS>(P, cd, double(c0), double(rho), double(U), double(p)));
+        }
+    }
     if (is_vdw<GAS>(P)) {
         T a, b, mm = T(0);
         vdw_ab<T, NS>(P, c, c0, a, b);

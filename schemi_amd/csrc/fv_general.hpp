@@ -28,7 +28,7 @@ gen_grad_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo
 {
     constexpr int NQ = NS + 4;
     const DevParams &P = *Pp;
-    if (dt_ptr[6] != 0.0)
+    if (dt_ptr[SC_DONE] != 0.0)
         return;
     const long long n = P.n;
     const long long id = blockIdx.x * 256LL + threadIdx.x;
@@ -93,7 +93,7 @@ gen_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
     constexpr int NF = NS + 5;
     __shared__ double sRed[4];
     const DevParams &P = *Pp;
-    if (dt_ptr[6] != 0.0)
+    if (dt_ptr[SC_DONE] != 0.0)
         return;
     const long long n = P.n;
     const long long id = blockIdx.x * 256LL + threadIdx.x;
@@ -164,8 +164,8 @@ gen_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
         load_cons<NS>(Uin, n, id, u);
         if constexpr (STAGE2)
             load_cons<NS>(Un, n, id, un);
-        update_cell<NS, DIM, STAGE2, MIXED>(P, cg, n, id, u, Uout, un, dt, xlo, ylo, zlo, F,
-                                            sig);
+        update_cell<NS, DIM, STAGE2 ? SV_RK2 : SV_RK1, MIXED>(P, cg, n, id, u, Uout, un, dt,
+                                                              xlo, ylo, zlo, F, sig);
     } else {
         err = 0;
     }

@@ -42,7 +42,7 @@ struct StageLaunch {
     const double *dt;
     unsigned long long *sigmax;
     int *err;
-    int dim, ns, lin, stage2, mode, gas_model;
+    int dim, ns, lin, variant, mode, gas_model;  // variant: SV_*
     int nx, ny, nz;
     int kchunk;
     void *W, *Lg;  // MODE_GENERAL scratch: primitives and gradients
@@ -53,5 +53,26 @@ struct StageLaunch {
 // otherwise a hipError_t value.
 int launch_stage_f64(const StageLaunch &L);
 int launch_stage_mixed(const StageLaunch &L);
+
+// Adjoint of one first-order-exact stage linearised at U (FP64):
+// out = a (dL/dU)^T lam + b add (add may be null).
+struct AdjointLaunch {
+    const DevParams *P;
+    const double *geo;
+    const double *U;
+    const double *lam;
+    double *out;
+    const double *add;
+    double dt, a, b;
+    int dim, ns, gas_model;
+    long long n;
+    hipStream_t stream;
+};
+int launch_adjoint_stage(const AdjointLaunch &L);
+
+// Mixing width sum_i V_i Y (1 - Y), Y = rho_sp / rho, into *J (device), and
+// dJ/dU into grad (device, nullable); partial: (n + 255) / 256 doubles.
+int launch_mixing_width(const DevParams *P, const double *geo, const double *U, int ns, long long n,
+                        int sp, double *partial, double *J, double *grad, hipStream_t stream);
 
 }  // namespace sg

@@ -15,7 +15,7 @@ template <int DIM, bool FO = false> static dim3 tile_grid(const StageLaunch &L)
                 (DIM == 3) ? (L.nz + L.kchunk - 1) / L.kchunk : 1);
 }
 
-template <class T, int NS, int DIM, bool S2, bool MIXED>
+template <class T, int NS, int DIM, int SV, bool MIXED>
 static int launch_mode(const StageLaunch &L)
 {
     constexpr int TX = TileShape<DIM>::TX, TY = TileShape<DIM>::TY;
@@ -23,26 +23,33 @@ static int launch_mode(const StageLaunch &L)
     switch (L.mode) {
     case MODE_FIRST_ORDER:
         if (L.gas_model == GAS_VDW)
-            hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, S2, MIXED, TX0, TY0, GAS_VDW>),
+            hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, SV, MIXED, TX0, TY0, GAS_VDW>),
+                               (tile_grid<DIM, true>(L)), dim3(TX0 * TY0), 0, L.stream, L.P,
+                               L.geo, L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
+        else if (L.gas_model == GAS_RK)
+            hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, SV, MIXED, TX0, TY0, GAS_RK>),
                                (tile_grid<DIM, true>(L)), dim3(TX0 * TY0), 0, L.stream, L.P,
                                L.geo, L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
         else
-            hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, S2, MIXED, TX0, TY0, GAS_IDEAL>),
+            hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, SV, MIXED, TX0, TY0, GAS_IDEAL>),
                                (tile_grid<DIM, true>(L)), dim3(TX0 * TY0), 0, L.stream, L.P,
                                L.geo, L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
         break;
     case MODE_SLOPES:
-        if (L.lin)
-            hipLaunchKernelGGL((stage_kernel<T, NS, DIM, true, S2, MIXED, TX, TY>),
+        // the slope kernel runs MUSCL-Hancock on 1-D meshes only
+        if constexpr (SV == SV_HANCOCK && DIM != 1)
+            return -1;
+        else if (L.lin)
+            hipLaunchKernelGGL((stage_kernel<T, NS, DIM, true, SV, MIXED, TX, TY>),
                                tile_grid<DIM>(L), dim3(TX * TY), 0, L.stream, L.P, L.geo, L.Uin,
                                L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
         else
-            hipLaunchKernelGGL((stage_kernel<T, NS, DIM, false, S2, MIXED, TX, TY>),
+            hipLaunchKernelGGL((stage_kernel<T, NS, DIM, false, SV, MIXED, TX, TY>),
                                tile_grid<DIM>(L), dim3(TX * TY), 0, L.stream, L.P, L.geo, L.Uin,
                                L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
         break;
     case MODE_GENERAL: {
-        if constexpr (DIM == 1) {
+        if constexpr (DIM == 1 || SV == SV_HANCOCK) {
             return -1;
         } else {
             const long long n = (long long)L.nx * L.ny * L.nz;
@@ -50,7 +57,7 @@ static int launch_mode(const StageLaunch &L)
             T *W = static_cast<T *>(L.W), *Lg = static_cast<T *>(L.Lg);
             hipLaunchKernelGGL((gen_grad_kernel<T, NS, DIM, MIXED>), g, dim3(256), 0, L.stream,
                                L.P, L.geo, L.Uin, W, Lg, L.dt);
-            hipLaunchKernelGGL((gen_update_kernel<T, NS, DIM, S2, MIXED>), g, dim3(256), 0,
+            hipLaunchKernelGGL((gen_update_kernel<T, NS, DIM, SV == SV_RK2, MIXED>), g, dim3(256), 0,
                                L.stream, L.P, L.geo, L.Uin, L.Uout, L.Un, W, Lg, L.dt, L.sigmax,
                                L.err);
         }
@@ -66,9 +73,18 @@ template <class T, int NS, bool MIXED>
 static int launch_ns(const StageLaunch &L)
 {
 #define SG_CASE(D)                                                               \
-    if (L.dim == D)                                                              \
-        return L.stage2 ? launch_mode<T, NS, D, true, MIXED>(L)                  \
-                        : launch_mode<T, NS, D, false, MIXED>(L);
+    if (L.dim == D) {                                                            \
+        switch (L.variant) {                                                     \
+        case SV_RK1:                                                             \
+            return launch_mode<T, NS, D, SV_RK1, MIXED>(L);                      \
+        case SV_RK2:                                                             \
+            return launch_mode<T, NS, D, SV_RK2, MIXED>(L);                      \
+        case SV_HANCOCK:                                                         \
+            return launch_mode<T, NS, D, SV_HANCOCK, MIXED>(L);                  \
+        default:                                                                 \
+            return -1;                                                           \
+        }                                                                        \
+    }
     SG_CASE(1)
     SG_CASE(2)
     SG_CASE(3)

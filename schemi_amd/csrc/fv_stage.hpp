@@ -334,11 +334,12 @@ __device__ __forceinline__ T signal_speed(const DevParams &P, const T *u, T minl
 // cell's face s in the order x-, x+, y+, y-, z-, z+ (F & N of the face's own
 // normal); the cell owns its high faces and the low faces on a non-periodic
 // boundary (xlo/ylo/zlo), and is the neighbour of its other low faces.  The
-// contributions (s g) S are summed in that order in FP64.  Stage 2 also forms
-// the SSP-RK2 average with U^n (bunchOfFields.hpp:166-203) and the Courant
+// contributions (s g) S are summed in that order in FP64.  RK stage 2 also
+// forms the SSP-RK2 average with U^n (bunchOfFields.hpp:166-203); the last
+// stage of a step (RK stage 2, the MUSCL-Hancock stage) reduces the Courant
 // signal speed of the new state.  u: the cell's conserved input state, un:
-// its U^n (stage 2), both supplied by the caller.
-template <int NS, int DIM, bool STAGE2, bool MIXED, int GAS = -1, class FluxAt>
+// its U^n (RK stage 2), both supplied by the caller.
+template <int NS, int DIM, int SV, bool MIXED, int GAS = -1, class FluxAt>
 __device__ __forceinline__ void update_cell(const DevParams &P, const CellGeo &cg, long long n,
                                             long long id, const double *u, double *Uout,
                                             const double *un, double dt, bool xlo, bool ylo,
@@ -387,7 +388,7 @@ __device__ __forceinline__ void update_cell(const DevParams &P, const CellGeo &c
         for (int q = 0; q < NS; ++q)
             unew[q] *= f;
     }
-    if constexpr (STAGE2) {
+    if constexpr (SV == SV_RK2) {
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
             if (q >= NS + 1 && q <= NS + 3)
@@ -395,6 +396,8 @@ __device__ __forceinline__ void update_cell(const DevParams &P, const CellGeo &c
             else
                 unew[q] = 0.5 * unew[q] + 0.5 * un[q];
         }
+    }
+    if constexpr (SV != SV_RK1) {
         if constexpr (MIXED) {
             // Courant speed of the FP64 state evaluated in FP32
             float uf[NF];
@@ -456,6 +459,285 @@ __device__ __forceinline__ void tile_coords(int &bx, int &by, int &bz)
 #endif
 }
 
+// Cell state closed with the EOS once per cell, for schemes whose face
+// values are the cell values.  Two layouts:
+//   H = false (SSP-RK2 stages): c[NS], v[3], p, r0, U, E, cs -- primitives
+//     plus the closure terms that need a division or a square root
+//     (close_side); the partial densities c_k M_k and the momentum v r0 are
+//     re-formed per face with the same products;
+//   H = true (MUSCL-Hancock): the face side after the half-step predictor,
+//     rho[NS], r0, m[3], E (conserved) and v[3], p, cs re-derived from them
+//     (HancockStage.cpp:174-290); the ghost's concentrations are rho_k / M_k.
+template <class T, int NS, bool H> struct Closed {
+    static constexpr int NV = H ? NS + 10 : NS + 8;
+};
+
+template <class T, int NS, int GAS = -1>
+__device__ __forceinline__ void close_cell(const DevParams &P, const T *w, T *z)
+{
+    Side<T, NS> s;
+    close_side<T, NS, GAS>(P, w, w + NS, w[NS + 3], s);
+#pragma unroll
+    for (int q = 0; q < NS + 4; ++q)
+        z[q] = w[q];
+    z[NS + 4] = s.r0;
+    z[NS + 5] = s.U;
+    z[NS + 6] = s.E;
+    z[NS + 7] = s.cs;
+}
+
+// Predicted face side from its conserved values (rho_k, r0, m, E after the
+// half step): c_k = rho_k / M_k, c_0 = sum c_k, v = m / r0,
+// U = E - r0 v^2 / 2 - rho k, p = pFromUv, cs from sqSonicSpeed
+// (HancockStage.cpp:174-290); stored in the H layout.
+template <class T, int NS, int GAS = -1>
+__device__ __forceinline__ void hancock_rederive(const DevParams &P, const T *rho, T r0,
+                                                 const T *m, T E, T *z)
+{
+    T c[NS], c0 = T(0);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        c[k] = rho[k] / mixM<T>(P, k);
+        c0 += c[k];
+    }
+    T v[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        v[a] = m[a] / r0;
+    const T v2 = ((T(0) + v[0] * v[0]) + v[1] * v[1]) + v[2] * v[2];
+    const T U = (E - r0 * v2 * T(0.5)) - T(0);
+    const T g1 = gamma1<T, NS>(P, c, c0);
+    const T p = eos_p<T, NS, GAS>(P, c, c0, g1, U);
+    const T cs = Real<T>::sqrt_(eos_c2<T, NS, GAS>(P, c, c0, g1, r0, U, p));
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        z[k] = rho[k];
+    z[NS] = r0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        z[NS + 1 + a] = m[a];
+        z[NS + 5 + a] = v[a];
+    }
+    z[NS + 4] = E;
+    z[NS + 8] = p;
+    z[NS + 9] = cs;
+}
+
+// Cell closed for the MUSCL-Hancock stage when its face values are the cell
+// values: the divergence of the physical fluxes of its own face sides
+// cancels pairwise (opposite faces carry the same values, normals and areas,
+// and are summed consecutively), so the predictor adds exactly zero and the
+// face side is the closed cell state with its primitives re-derived.
+template <class T, int NS, int GAS = -1>
+__device__ __forceinline__ void close_cell_hancock(const DevParams &P, const T *w, T *z)
+{
+    Side<T, NS> s;
+    close_side<T, NS, GAS>(P, w, w + NS, w[NS + 3], s);
+    hancock_rederive<T, NS, GAS>(P, s.rho, s.r0, s.m, s.E, z);
+}
+
+template <class T, int NS, bool H>
+__device__ __forceinline__ void side_of(const DevParams &P, const T *z, Side<T, NS> &s)
+{
+    if constexpr (H) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            s.rho[k] = z[k];
+        s.r0 = z[NS];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            s.m[a] = z[NS + 1 + a];
+            s.v[a] = z[NS + 5 + a];
+        }
+        s.E = z[NS + 4];
+        s.p = z[NS + 8];
+        s.cs = z[NS + 9];
+        s.U = T(0);  // not used by the flux
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            s.rho[k] = z[k] * mixM<T>(P, k);
+        s.r0 = z[NS + 4];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            s.v[a] = z[NS + a];
+            s.m[a] = z[NS + a] * s.r0;
+        }
+        s.p = z[NS + 3];
+        s.U = z[NS + 5];
+        s.E = z[NS + 6];
+        s.cs = z[NS + 7];
+    }
+}
+
+// Primitives (c, v, p) of a closed face side, for its boundary ghost.
+template <class T, int NS, bool H>
+__device__ __forceinline__ void prims_of(const DevParams &P, const T *z, T *c, T *v, T &p)
+{
+    if constexpr (H) {
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            c[k] = z[k] / mixM<T>(P, k);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            v[a] = z[NS + 5 + a];
+        p = z[NS + 8];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            c[k] = z[k];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            v[a] = z[NS + a];
+        p = z[NS + 3];
+    }
+}
+
+// Flux of face f along axis a from the closed states of cells f-1 (zo) and f
+// (zn); on a non-periodic boundary only the owner's state is read and the
+// ghost is formed from its primitives (Advection2dOrder.cpp:398-521,
+// HancockStage.cpp:290-448).
+template <class T, int NS, int GAS = -1, bool H = false>
+__device__ __forceinline__ bool face_flux_closed(const DevParams &P, int a, int f, const T *zo,
+                                                 const T *zn, T *g)
+{
+    constexpr int NV = Closed<T, NS, H>::NV;
+    const int Na = P.N[a];
+    const bool per = P.periodic[a] != 0;
+    Side<T, NS> O, B;
+    if (!per && (f == 0 || f == Na)) {
+        const int side = (f == 0) ? 0 : 1;
+        T z[NV];  // element-wise select keeps both in registers
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+            z[q] = side ? zo[q] : zn[q];
+        T c[NS], v[3], p, cB[NS], vB[3], pB;
+        prims_of<T, NS, H>(P, z, c, v, p);
+        ghost_side<T, NS>(P, a, side, c, v, p, cB, vB, pB);
+        side_of<T, NS, H>(P, z, O);
+        close_side<T, NS, GAS>(P, cB, vB, pB, B);
+        const T sg = (side && !P.hi_flip[a]) ? T(1) : T(-1);
+        if (P.flux == FLUX_HLL)
+            return hll_flux<T, NS>(O, B, a, sg, g);
+        return hllc_flux<T, NS>(O, B, a, sg, g);
+    }
+    side_of<T, NS, H>(P, zo, O);
+    side_of<T, NS, H>(P, zn, B);
+    if (P.flux == FLUX_HLL)
+        return hll_flux<T, NS>(O, B, a, T(1), g);
+    return hllc_flux<T, NS>(O, B, a, T(1), g);
+}
+
+// Physical flux of one face side through its face for the MUSCL-Hancock
+// predictor (HancockStage.cpp:10-172): (F(s) . n) S added to the cell's sums,
+// dr[0] for rho, dr[1 + k] for rho_k, dm for the momentum, dE for the energy,
+// in the reference's operation order (dot products from 0, tensor rows
+// (T_a0 n_0 + T_a1 n_1) + T_a2 n_2, the pressure plus 2/3 k with k = 0).
+template <class T, int NS>
+__device__ __forceinline__ void hancock_face_terms(const Side<T, NS> &s, const T *n, T S, T *dr,
+                                                   T *dm, T &dE)
+{
+    auto dot = [&](const T *x) { return ((T(0) + x[0] * n[0]) + x[1] * n[1]) + x[2] * n[2]; };
+#pragma unroll
+    for (int k = 0; k <= NS; ++k) {
+        const T rho = (k == 0) ? s.r0 : s.rho[(k > 0) ? k - 1 : 0];
+        const T rv[3] = {s.v[0] * rho, s.v[1] * rho, s.v[2] * rho};
+        dr[k] += dot(rv) * S;
+    }
+    const T pp = s.p + T(2.0 / 3.0) * T(0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T row = ((s.m[a] * s.v[0]) * n[0] + (s.m[a] * s.v[1]) * n[1]) +
+                      (s.m[a] * s.v[2]) * n[2];
+        dm[a] += (n[a] * pp + row) * S;
+    }
+    const T vE[3] = {s.v[0] * s.E, s.v[1] * s.E, s.v[2] * s.E};
+    const T vp[3] = {s.v[0] * pp, s.v[1] * pp, s.v[2] * pp};
+    dE += (dot(vE) + dot(vp)) * S;
+}
+
+// A face side advanced by half a step with the cell's flux sums (divided by
+// V here) and re-derived into the H layout (HancockStage.cpp:174-290).
+template <class T, int NS, int GAS = -1>
+__device__ __forceinline__ void hancock_advance(const DevParams &P, const Side<T, NS> &s,
+                                                const T *dr, const T *dm, T dE, T half, T *z)
+{
+    T rho[NS], m[3];
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+        rho[k] = s.rho[k] - dr[k + 1] * half;
+    const T r0 = s.r0 - dr[0] * half;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        m[a] = s.m[a] - dm[a] * half;
+    const T E = s.E - dE * half;
+    hancock_rederive<T, NS, GAS>(P, rho, r0, m, E, z);
+}
+
+template <class T, int NS>
+__device__ __forceinline__ void hancock_divide(T *dr, T *dm, T &dE, T V)
+{
+#pragma unroll
+    for (int k = 0; k <= NS; ++k)
+        dr[k] /= V;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        dm[a] /= V;
+    dE /= V;
+}
+
+// MUSCL-Hancock face sides of a 1-D cell with slopes L (HancockStage.cpp:
+// 10-448): both sides reconstructed (x-: d = x_f(i) - x_c(i), x+: d =
+// x_f(i+1) - x_c(i)) and closed, the divergence of their physical fluxes
+// (normals -e_x and +e_x, area S, the cell's faces in the order x-, x+)
+// divided by V, each side advanced by -div dt / 2 and re-derived.  zl: the
+// x- side, zr: the x+ side, both in the H layout.
+template <class T, int NS, bool LIN, int GAS = -1>
+__device__ __forceinline__ void hancock_sides_1d(const DevParams &P, const AxisGeo &G, int i,
+                                                 const T *w, const T *L, T V, T S, T half,
+                                                 T *zl, T *zr)
+{
+    T c[2][NS], v[2][3], p[2];
+    reconstruct<T, NS, LIN>(w, L, 0, to_T<T>(G.dlo[i]), c[0], v[0], p[0]);
+    reconstruct<T, NS, LIN>(w, L, 0, to_T<T>(G.dhi[i]), c[1], v[1], p[1]);
+    Side<T, NS> sd[2];
+    close_side<T, NS, GAS>(P, c[0], v[0], p[0], sd[0]);
+    close_side<T, NS, GAS>(P, c[1], v[1], p[1], sd[1]);
+    T dr[NS + 1], dm[3] = {T(0), T(0), T(0)}, dE = T(0);
+#pragma unroll
+    for (int k = 0; k <= NS; ++k)
+        dr[k] = T(0);
+    const T nl[3] = {T(-1), T(0), T(0)}, nr[3] = {T(1), T(0), T(0)};
+    hancock_face_terms<T, NS>(sd[0], nl, S, dr, dm, dE);
+    hancock_face_terms<T, NS>(sd[1], nr, S, dr, dm, dE);
+    hancock_divide<T, NS>(dr, dm, dE, V);
+    hancock_advance<T, NS, GAS>(P, sd[0], dr, dm, dE, half, zl);
+    hancock_advance<T, NS, GAS>(P, sd[1], dr, dm, dE, half, zr);
+}
+
+// Top-layer cell of a 3-D mesh whose top face normal is reversed (the
+// reference spaces z by Ly / Nz, so with Lz < Ly the top face plane lies
+// below the last cell centres): with zero slopes every side is the cell
+// state, the x and y pairs cancel as elsewhere, but the z- face (normal
+// -e_z seen from this cell) and the top face (normal -e_z) add up, so the
+// predictor is 2 (F(s) . -e_z) S_z / V; all six sides stay equal.
+template <class T, int NS, int GAS = -1>
+__device__ __forceinline__ void close_cell_hancock_zflip(const DevParams &P, const T *w, T S,
+                                                         T V, T half, T *z)
+{
+    Side<T, NS> s;
+    close_side<T, NS, GAS>(P, w, w + NS, w[NS + 3], s);
+    T dr[NS + 1], dm[3] = {T(0), T(0), T(0)}, dE = T(0);
+#pragma unroll
+    for (int k = 0; k <= NS; ++k)
+        dr[k] = T(0);
+    const T nz[3] = {T(0), T(0), T(-1)};
+    hancock_face_terms<T, NS>(s, nz, S, dr, dm, dE);
+    hancock_face_terms<T, NS>(s, nz, S, dr, dm, dE);
+    hancock_divide<T, NS>(dr, dm, dE, V);
+    hancock_advance<T, NS, GAS>(P, s, dr, dm, dE, half, z);
+}
+
 // Index helper for the x-y plane tile in LDS with a 2-cell halo.
 template <int TX, int TY, int DIM> struct Tile {
     static constexpr int BX = TX + 4;
@@ -463,7 +745,7 @@ template <int TX, int TY, int DIM> struct Tile {
     static constexpr int OY = (DIM >= 2) ? 2 : 0;
 };
 
-template <class T, int NS, int DIM, bool LIN, bool STAGE2, bool MIXED, int TX, int TY>
+template <class T, int NS, int DIM, bool LIN, int SV, bool MIXED, int TX, int TY>
 __global__ void __launch_bounds__(TX * TY)
 stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
              const double *__restrict__ Uin, double *Uout, const double *Un,
@@ -485,14 +767,28 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
     __shared__ T sSx[NL][TY][SXW];
     __shared__ T sSy[(DIM >= 2) ? NL : 1][SYH][TX];
     __shared__ T sFx[NF][TY][TX + 1];
+    // MUSCL-Hancock (1-D): predicted x- and x+ face sides of every cell
+    constexpr bool HK = (SV == SV_HANCOCK);
+    constexpr int NVH = Closed<T, NS, true>::NV;
+    static_assert(!HK || DIM == 1, "the slope kernel runs MUSCL-Hancock in 1-D only");
+    __shared__ T sHl[HK ? NVH : 1][HK ? SXW : 1];
+    __shared__ T sHr[HK ? NVH : 1][HK ? SXW : 1];
     __shared__ T sFy[(DIM >= 2) ? NF : 1][FYH][TX];
     __shared__ double sRed[NT / 64];
 
     const DevParams &P = *Pp;
-    // dt_ptr points at the context's scalar block; entry 6 flags that the
-    // run has reached its end time, so queued steps do nothing
-    if (dt_ptr[6] != 0.0)
+    // dt_ptr points at the context's scalar block; once the run has reached
+    // its end time, queued steps do nothing
+    if (dt_ptr[SC_DONE] != 0.0)
         return;
+    // MUSCL-Hancock steps alternate between the two state buffers
+    if constexpr (SV == SV_HANCOCK) {
+        if (dt_ptr[SC_PARITY] != 0.0) {
+            double *t = const_cast<double *>(Uin);
+            Uin = Uout;
+            Uout = t;
+        }
+    }
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
     const int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
@@ -648,9 +944,20 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                     }
                     const CellGeo cg = cell_geo(P, geo, cx, (DIM >= 2) ? cy : 0, (DIM == 3) ? k : 0);
                     slopes_axis<T, NS, LIN>(P, GX, 0, cx, wm, w0, wp, T(cg.S[0]), T(cg.V), L);
+                    if constexpr (HK) {
+                        T zl[NVH], zr[NVH];
+                        hancock_sides_1d<T, NS, LIN>(P, GX, cx, w0, L, T(cg.V), T(cg.S[0]),
+                                                     T(0.5 * dt), zl, zr);
 #pragma unroll
-                    for (int s = 0; s < NL; ++s)
-                        sSx[s][r][u] = L[s];
+                        for (int q = 0; q < NVH; ++q) {
+                            sHl[q][u] = zl[q];
+                            sHr[q][u] = zr[q];
+                        }
+                    } else {
+#pragma unroll
+                        for (int s = 0; s < NL; ++s)
+                            sSx[s][r][u] = L[s];
+                    }
                 } else if constexpr (DIM >= 2) {
                     const int tt = t - NTX;
                     const int c = tt % TX, v = tt / TX;
@@ -691,19 +998,31 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                     const int fx = x0 + u;
                     if (fx > Nx || y0 + r >= Ny)
                         continue;
-                    T wo[NQ], wn[NQ], Lo[NL], Ln[NL];
+                    if constexpr (HK) {
+                        // owner: x+ side of cell fx-1, neighbour: x- side of cell fx
+                        T zo[NVH], zn[NVH];
 #pragma unroll
-                    for (int q = 0; q < NQ; ++q) {
-                        wo[q] = sW[q][r + OY][u + 1];
-                        wn[q] = sW[q][r + OY][u + 2];
-                    }
+                        for (int q = 0; q < NVH; ++q) {
+                            zo[q] = sHr[q][u];
+                            zn[q] = sHl[q][u + 1];
+                        }
+                        if (!face_flux_closed<T, NS, -1, true>(P, 0, fx, zo, zn, gq))
+                            err |= ERRF_RIEMANN;
+                    } else {
+                        T wo[NQ], wn[NQ], Lo[NL], Ln[NL];
 #pragma unroll
-                    for (int s = 0; s < NL; ++s) {
-                        Lo[s] = sSx[s][r][u];
-                        Ln[s] = sSx[s][r][u + 1];
+                        for (int q = 0; q < NQ; ++q) {
+                            wo[q] = sW[q][r + OY][u + 1];
+                            wn[q] = sW[q][r + OY][u + 2];
+                        }
+#pragma unroll
+                        for (int s = 0; s < NL; ++s) {
+                            Lo[s] = sSx[s][r][u];
+                            Ln[s] = sSx[s][r][u + 1];
+                        }
+                        if (!face_flux<T, NS, LIN>(P, GX, 0, fx, wo, Lo, wn, Ln, gq))
+                            err |= ERRF_RIEMANN;
                     }
-                    if (!face_flux<T, NS, LIN>(P, GX, 0, fx, wo, Lo, wn, Ln, gq))
-                        err |= ERRF_RIEMANN;
 #pragma unroll
                     for (int q = 0; q < NF; ++q)
                         sFx[q][r][u] = gq[q];
@@ -754,10 +1073,10 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
             };
             double u[NF], un[NF];
             load_cons<NS>(Uin, n, id, u);
-            if constexpr (STAGE2)
+            if constexpr (SV == SV_RK2)
                 load_cons<NS>(Un, n, id, un);
-            update_cell<NS, DIM, STAGE2, MIXED>(P, cg, n, id, u, Uout, un, dt, xlo, ylo, zlo,
-                                                F, sig);
+            update_cell<NS, DIM, SV, MIXED>(P, cg, n, id, u, Uout, un, dt, xlo, ylo, zlo, F,
+                                            sig);
         }
 
         if constexpr (DIM == 3) {
@@ -777,81 +1096,21 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
 
     if (err)
         atomicOr(errflag, err);
-    if constexpr (STAGE2)
+    if constexpr (SV != SV_RK1)
         reduce_signal<NT>(sig, sRed, sigmax);
 }
 
-// Cell state closed with the EOS once per cell, for schemes whose face
-// values are the cell values: primitives plus the closure terms that need a
-// division or a square root (close_side); the partial densities c_k M_k and
-// the momentum v r0 are re-formed per face with the same products.
-template <class T, int NS> struct Closed {
-    static constexpr int NV = NS + 8;  // c[NS], v[3], p, r0, U, E, cs
-};
-
-template <class T, int NS, int GAS = -1>
-__device__ __forceinline__ void close_cell(const DevParams &P, const T *w, T *z)
-{
-    Side<T, NS> s;
-    close_side<T, NS, GAS>(P, w, w + NS, w[NS + 3], s);
-#pragma unroll
-    for (int q = 0; q < NS + 4; ++q)
-        z[q] = w[q];
-    z[NS + 4] = s.r0;
-    z[NS + 5] = s.U;
-    z[NS + 6] = s.E;
-    z[NS + 7] = s.cs;
-}
-
-template <class T, int NS>
-__device__ __forceinline__ void side_of(const DevParams &P, const T *z, Side<T, NS> &s)
-{
-#pragma unroll
-    for (int k = 0; k < NS; ++k)
-        s.rho[k] = z[k] * mixM<T>(P, k);
-    s.r0 = z[NS + 4];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        s.v[a] = z[NS + a];
-        s.m[a] = z[NS + a] * s.r0;
-    }
-    s.p = z[NS + 3];
-    s.U = z[NS + 5];
-    s.E = z[NS + 6];
-    s.cs = z[NS + 7];
-}
-
-// Flux of face f along axis a from the closed states of cells f-1 (zo) and f
-// (zn); on a non-periodic boundary only the owner's state is read and the
-// ghost is formed from its primitives (Advection2dOrder.cpp:398-521).
-template <class T, int NS, int GAS = -1>
-__device__ __forceinline__ bool face_flux_closed(const DevParams &P, int a, int f, const T *zo,
-                                                 const T *zn, T *g)
-{
-    const int Na = P.N[a];
-    const bool per = P.periodic[a] != 0;
-    Side<T, NS> O, B;
-    if (!per && (f == 0 || f == Na)) {
-        const int side = (f == 0) ? 0 : 1;
-        T z[Closed<T, NS>::NV];  // element-wise select keeps both in registers
-#pragma unroll
-        for (int q = 0; q < Closed<T, NS>::NV; ++q)
-            z[q] = side ? zo[q] : zn[q];
-        T cB[NS], vB[3], pB;
-        ghost_side<T, NS>(P, a, side, z, z + NS, z[NS + 3], cB, vB, pB);
-        side_of<T, NS>(P, z, O);
-        close_side<T, NS, GAS>(P, cB, vB, pB, B);
-        const T sg = (side && !P.hi_flip[a]) ? T(1) : T(-1);
-        if (P.flux == FLUX_HLL)
-            return hll_flux<T, NS>(O, B, a, sg, g);
-        return hllc_flux<T, NS>(O, B, a, sg, g);
-    }
-    side_of<T, NS>(P, zo, O);
-    side_of<T, NS>(P, zn, B);
-    if (P.flux == FLUX_HLL)
-        return hll_flux<T, NS>(O, B, a, T(1), g);
-    return hllc_flux<T, NS>(O, B, a, T(1), g);
-}
+// Mixed path of the first-order kernel: keep the own cell's FP64 state in
+// registers between its load and its update (a reload misses in L2: a plane
+// of every resident workgroup has passed through it since), and park the
+// own column's z+ flux in LDS to make room; measured 3 % faster than the
+// reload on the 3-D headline case, with 12-20 bytes of spills per lane.
+#ifndef SG_MIXED_HOLD_U
+#define SG_MIXED_HOLD_U 1
+#endif
+#ifndef SG_MIXED_FZ_LDS
+#define SG_MIXED_FZ_LDS 1
+#endif
 
 // Stage kernel for schemes whose limited slopes vanish identically (zero
 // limiter; TVD limiters on 2-D and 3-D meshes: the reference's ratio along an
@@ -866,7 +1125,7 @@ __device__ __forceinline__ bool face_flux_closed(const DevParams &P, int a, int 
 // The mixed path asks for 4 waves per SIMD (two 512-thread workgroups per
 // CU); its register budget then allows a few spills, which measured faster
 // than one workgroup per CU.
-template <class T, int NS, int DIM, bool STAGE2, bool MIXED, int TX, int TY, int GAS>
+template <class T, int NS, int DIM, int SV, bool MIXED, int TX, int TY, int GAS>
 __global__ void __launch_bounds__(TX * TY, (MIXED && TX * TY == 512) ? 4 : 1)
 stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
               const double *__restrict__ Uin, double *Uout, const double *Un,
@@ -874,7 +1133,8 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
               int kchunk)
 {
     constexpr int NQ = NS + 4;
-    constexpr int NV = Closed<T, NS>::NV;
+    constexpr bool H = (SV == SV_HANCOCK);
+    constexpr int NV = Closed<T, NS, H>::NV;
     constexpr int NF = NS + 5;
     constexpr int NT = TX * TY;
     constexpr int BX = TX + 2;
@@ -886,10 +1146,23 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
     __shared__ T sFx[NF][TY][TX + 1];
     __shared__ T sFy[(DIM >= 2) ? NF : 1][FYH][TX];
     __shared__ double sRed[NT / 64];
+    // mixed path: the own cell's FP64 state stays in registers (HOLD_U) or is
+    // reloaded for the update; the own column's z+ flux waits in LDS (FZ_LDS)
+    constexpr bool HOLD_U = !MIXED || SG_MIXED_HOLD_U;
+    constexpr bool FZ_LDS = (DIM == 3) && MIXED && SG_MIXED_FZ_LDS;
+    __shared__ T sFz[FZ_LDS ? NF : 1][FZ_LDS ? TY : 1][FZ_LDS ? TX : 1];
 
     const DevParams &P = *Pp;
-    if (dt_ptr[6] != 0.0)
+    if (dt_ptr[SC_DONE] != 0.0)
         return;
+    // MUSCL-Hancock steps alternate between the two state buffers
+    if constexpr (SV == SV_HANCOCK) {
+        if (dt_ptr[SC_PARITY] != 0.0) {
+            double *t = const_cast<double *>(Uin);
+            Uin = Uout;
+            Uout = t;
+        }
+    }
     const int tid = threadIdx.x;
     const int tx = tid % TX, ty = tid / TX;
     const int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
@@ -908,7 +1181,8 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
     double sig = 0.0;
     const bool zper = (DIM == 3) && P.periodic[2] != 0;
 
-    auto closed_from = [&](const double *u, T *z) {
+    // closed state of cell (ci, cj, ck) from its conserved values u
+    auto closed_from = [&](const double *u, T *z, int ci, int cj, int ck) {
         T w[NQ];
         if constexpr (MIXED) {
             T ut[NF];
@@ -919,12 +1193,21 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
         } else {
             prims_from_cons<T, NS, GAS>(P, u, w);
         }
-        close_cell<T, NS, GAS>(P, w, z);
+        if constexpr (H) {
+            if (DIM == 3 && P.hi_flip[2] && ck == Nz - 1) {
+                const CellGeo g = cell_geo(P, geo, ci, cj, ck);
+                close_cell_hancock_zflip<T, NS, GAS>(P, w, T(g.S[2]), T(g.V), T(0.5 * dt), z);
+            } else {
+                close_cell_hancock<T, NS, GAS>(P, w, z);
+            }
+        } else {
+            close_cell<T, NS, GAS>(P, w, z);
+        }
     };
-    auto closed_at = [&](long long id, T *z) {
+    auto closed_at = [&](int ci, int cj, int ck, T *z) {
         double u[NF];
-        load_cons<NS>(Uin, n, id, u);
-        closed_from(u, z);
+        load_cons<NS>(Uin, n, ci + sy * cj + sz * (long long)ck, u);
+        closed_from(u, z, ci, cj, ck);
     };
 
     // own column: closed states of planes k and k+1, fluxes of its z faces;
@@ -936,14 +1219,14 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
     if constexpr (DIM == 3) {
         if (act) {
             load_cons<NS>(Uin, n, i + sy * j + sz * kb, ucur);
-            closed_from(ucur, zc);
+            closed_from(ucur, zc, i, j, kb);
             if (kb > 0 || zper) {
                 T zm[NV];
                 const int km = (kb > 0) ? kb - 1 : Nz - 1;
-                closed_at(i + sy * j + sz * km, zm);
-                if (!face_flux_closed<T, NS, GAS>(P, 2, kb, zm, zc, fzm))
+                closed_at(i, j, km, zm);
+                if (!face_flux_closed<T, NS, GAS, H>(P, 2, kb, zm, zc, fzm))
                     err |= ERRF_RIEMANN;
-            } else if (!face_flux_closed<T, NS, GAS>(P, 2, 0, zc, zc, fzm)) {
+            } else if (!face_flux_closed<T, NS, GAS, H>(P, 2, 0, zc, zc, fzm)) {
                 err |= ERRF_RIEMANN;
             }
         }
@@ -956,17 +1239,22 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                 if ((k + 1 < Nz) || zper) {
                     const int kp = (k + 1 < Nz) ? k + 1 : 0;
                     load_cons<NS>(Uin, n, i + sy * j + sz * kp, unx);
-                    closed_from(unx, znx);
+                    closed_from(unx, znx, i, j, kp);
                 } else {
 #pragma unroll
                     for (int q = 0; q < NV; ++q)
                         znx[q] = zc[q];  // not read: the top face is a boundary face
                 }
-                if (!face_flux_closed<T, NS, GAS>(P, 2, k + 1, zc, znx, fzp))
+                if (!face_flux_closed<T, NS, GAS, H>(P, 2, k + 1, zc, znx, fzp))
                     err |= ERRF_RIEMANN;
+                if constexpr (FZ_LDS) {
+#pragma unroll
+                    for (int q = 0; q < NF; ++q)
+                        sFz[q][ty][tx] = fzp[q];
+                }
             } else {
                 load_cons<NS>(Uin, n, i + sy * j, ucur);
-                closed_from(ucur, zc);
+                closed_from(ucur, zc, i, j, 0);
             }
 #pragma unroll
             for (int q = 0; q < NV; ++q)
@@ -1004,7 +1292,7 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                 }
                 if (okx && oky) {
                     T z[NV];
-                    closed_at(gx + sy * gy + ((DIM == 3) ? sz * (long long)k : 0), z);
+                    closed_at(gx, gy, (DIM == 3) ? k : 0, z);
 #pragma unroll
                     for (int q = 0; q < NV; ++q)
                         sZ[q][byc][bxc] = z[q];
@@ -1030,7 +1318,7 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                         zo[q] = sZ[q][r + OY][u];
                         zn[q] = sZ[q][r + OY][u + 1];
                     }
-                    if (!face_flux_closed<T, NS, GAS>(P, 0, fx, zo, zn, g))
+                    if (!face_flux_closed<T, NS, GAS, H>(P, 0, fx, zo, zn, g))
                         err |= ERRF_RIEMANN;
 #pragma unroll
                     for (int q = 0; q < NF; ++q)
@@ -1046,7 +1334,7 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                         zo[q] = sZ[q][v][c + 1];
                         zn[q] = sZ[q][v + 1][c + 1];
                     }
-                    if (!face_flux_closed<T, NS, GAS>(P, 1, fy, zo, zn, g))
+                    if (!face_flux_closed<T, NS, GAS, H>(P, 1, fy, zo, zn, g))
                         err |= ERRF_RIEMANN;
 #pragma unroll
                     for (int q = 0; q < NF; ++q)
@@ -1071,17 +1359,15 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                 case 2: return double(sFy[q][ty + 1][tx]);
                 case 3: return double(sFy[q][ty][tx]);
                 case 4: return double(fzm[q]);
-                default: return double(fzp[q]);
+                default: return double(FZ_LDS ? sFz[q][ty][tx] : fzp[q]);
                 }
             };
             double un[NF];
-            if constexpr (STAGE2)
+            if constexpr (SV == SV_RK2)
                 load_cons<NS>(Un, n, id, un);
-            // mixed path: reload instead of keeping it (its registers are
-            // needed to run two 512-thread workgroups per CU without spills)
-            if constexpr (MIXED)
+            if constexpr (!HOLD_U)
                 load_cons<NS>(Uin, n, id, ucur);
-            update_cell<NS, DIM, STAGE2, MIXED, GAS>(P, cg, n, id, ucur, Uout, un, dt, xlo, ylo,
+            update_cell<NS, DIM, SV, MIXED, GAS>(P, cg, n, id, ucur, Uout, un, dt, xlo, ylo,
                                                 zlo, F, sig);
         }
         if constexpr (DIM == 3) {
@@ -1093,13 +1379,13 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                 zc[q] = znx[q];
 #pragma unroll
             for (int q = 0; q < NF; ++q)
-                fzm[q] = fzp[q];
+                fzm[q] = FZ_LDS ? sFz[q][ty][tx] : fzp[q];
         }
     }
 
     if (err)
         atomicOr(errflag, err);
-    if constexpr (STAGE2)
+    if constexpr (SV != SV_RK1)
         reduce_signal<NT>(sig, sRed, sigmax);
 }
 

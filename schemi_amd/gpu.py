@@ -58,7 +58,8 @@ class SchemeDesc(C.Structure):
     _fields_ = [("flux", C.c_int32), ("limiter", C.c_int32), ("precision", C.c_int32),
                 ("gravity", C.c_int32), ("g", C.c_double * 3), ("courant", C.c_double),
                 ("min_time", C.c_double), ("const_dt", C.c_double),
-                ("bc_c", (_BC * 6) * MAXS), ("bc_v", _BC * 6), ("bc_p", _BC * 6)]
+                ("bc_c", (_BC * 6) * MAXS), ("bc_v", _BC * 6), ("bc_p", _BC * 6),
+                ("step_solver", C.c_int32)]
 
 
 _LIB = None
@@ -66,7 +67,8 @@ _SYMBOLS = ("schemi_gpu_create", "schemi_gpu_destroy", "schemi_gpu_set_state",
             "schemi_gpu_get_state", "schemi_gpu_step", "schemi_gpu_run",
             "schemi_gpu_stage", "schemi_gpu_face_flux", "schemi_gpu_device_state",
             "schemi_gpu_sync", "schemi_gpu_timing", "schemi_gpu_strerror",
-            "schemi_gpu_face_permutation", "schemi_gpu_axis_table")
+            "schemi_gpu_face_permutation", "schemi_gpu_axis_table", "schemi_gpu_state_device",
+            "schemi_gpu_adjoint_step", "schemi_gpu_mixing_width")
 
 
 def _lib():
@@ -99,6 +101,12 @@ def _lib():
         L.schemi_gpu_device_state.argtypes = [vp, C.POINTER(dp), lp]
         L.schemi_gpu_sync.restype = C.c_int
         L.schemi_gpu_sync.argtypes = [vp]
+        L.schemi_gpu_state_device.restype = C.c_int
+        L.schemi_gpu_state_device.argtypes = [vp, vp, vp]
+        L.schemi_gpu_adjoint_step.restype = C.c_int
+        L.schemi_gpu_adjoint_step.argtypes = [vp, C.c_double, vp, vp, C.c_int32]
+        L.schemi_gpu_mixing_width.restype = C.c_int
+        L.schemi_gpu_mixing_width.argtypes = [vp, C.c_int32, dp, vp]
         L.schemi_gpu_timing.restype = C.c_int
         L.schemi_gpu_timing.argtypes = [vp, C.c_int32, dp, lp]
         L.schemi_gpu_strerror.restype = C.c_char_p
@@ -170,6 +178,7 @@ def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) 
         for a in range(3):
             s.bc_v[p].value[a] = float(case.bc_v[p].value[a])
             s.bc_p[p].value[a] = float(case.bc_p[p].value[a])
+    s.step_solver = int(case.step_solver)
     return s
 
 
@@ -215,7 +224,8 @@ def _nfaces(case: Case) -> int:
 
 
 class GpuSolver:
-    """One device context: the SSP-RK2 step of ``case`` on the current GPU."""
+    """One device context on the current GPU: the step of ``case``, SSP-RK2
+    or MUSCL-Hancock as ``case.step_solver`` selects."""
 
     def __init__(self, case: Case, precision: int = PREC_FP64, const_dt: float = -1.0):
         self.case = case
@@ -296,6 +306,27 @@ class GpuSolver:
     def sync(self):
         self._check(_lib().schemi_gpu_sync(self._ctx), "sync")
 
+    # -- device-resident state, adjoint and mixing width (device pointers) ----
+    def state_device(self, src: int = 0, dst: int = 0):
+        """src (device address or 0) becomes the state; then the state is
+        copied to dst (device address or 0)."""
+        self._check(_lib().schemi_gpu_state_device(self._ctx, src or None, dst or None),
+                    "state_device")
+
+    def adjoint_step(self, dt: float, lam_in: int, lam_out: int, what: int = 0):
+        """lam_out = (dU^{n+1}/dU^n)^T lam_in at the current state (device
+        addresses of (Ns + 5, n) FP64 arrays); what 1: one stage."""
+        self._check(_lib().schemi_gpu_adjoint_step(self._ctx, float(dt), lam_in, lam_out,
+                                                   int(what)), "adjoint_step")
+
+    def mixing_width(self, species: int = 0, grad: int = 0) -> float:
+        """sum_i V_i Y (1 - Y), Y = rho_species / rho; dJ/dU into grad (device
+        address or 0)."""
+        J = C.c_double()
+        self._check(_lib().schemi_gpu_mixing_width(self._ctx, int(species), C.byref(J),
+                                                   grad or None), "mixing_width")
+        return J.value
+
     def timing(self, enable: bool):
         avg, cnt = C.c_double(), C.c_int64()
         self._check(_lib().schemi_gpu_timing(self._ctx, 1 if enable else 0, C.byref(avg),
@@ -304,6 +335,54 @@ class GpuSolver:
 
 
 _LIBM = None
+
+
+def _libm():
+    global _LIBM
+    if _LIBM is None:
+        _LIBM = C.CDLL("libm.so.6")
+        for f in (_LIBM.cbrt, _LIBM.pow, _LIBM.sqrt):
+            f.restype = C.c_double
+        _LIBM.cbrt.argtypes = [C.c_double]
+        _LIBM.sqrt.argtypes = [C.c_double]
+        _LIBM.pow.argtypes = [C.c_double, C.c_double]
+    return _LIBM
+
+
+def rk_matrices(case: Case):
+    """Redlich-Kwong mixing matrices a_kl, b_kl (mixtureRedlichKwong.cpp:19-67):
+    chi = cbrt(2) - 1 with the correctly rounded cube root (the reference's
+    constant expression), Sigma_a = 1 / (9 chi), Sigma_b = chi / 3."""
+    L = _libm()
+    chi = float.fromhex("0x1.428a2f98d728bp+0") - 1.0
+    Sa, Sb = 1.0 / (9.0 * chi), chi / 3.0
+    n = case.Ns
+    a = np.zeros((n, n))
+    b = np.zeros((n, n))
+    R = float(case.R)
+    for k in range(n):
+        Tc, Pc = float(case.Tc[k]), float(case.Pc[k])
+        b[k, k] = Sb * R * Tc / Pc
+        a[k, k] = Sa * R * R * Tc * Tc * L.sqrt(Tc) / Pc
+    for k in range(n):
+        for l in range(n):
+            if k != l:
+                b[k, l] = L.pow(0.5 * (L.cbrt(b[k, k]) + L.cbrt(b[l, l])), 3.0)
+                a[k, l] = L.sqrt(a[k, k] * a[l, l])
+    return a, b
+
+
+def _rk_root(A, C, D):
+    """Positive root of A s^3 + C s + D (A > 0, D < 0) by Newton's method from
+    the root without D, elementwise (the device's rk_root)."""
+    s = np.where(C < 0, np.sqrt(np.maximum(-C / A, 0.0)), np.cbrt(-D / A))
+    for _ in range(60):
+        s2 = s * s
+        ds = ((A * s2 + C) * s + D) / (3.0 * A * s2 + C)
+        s = s - ds
+        if np.all(np.abs(ds) <= 1e-13 * s):
+            break
+    return s
 
 
 def vdw_matrices(case: Case):
@@ -334,7 +413,9 @@ def vdw_matrices(case: Case):
 def conserved_from_primitive(case: Case, c, v, p) -> np.ndarray:
     """Conserved vector from primitives (c_k, v, p), with the reference's
     initialisation arithmetic (phaseInitialization.cpp:1827-1883) and its
-    UvFromp (idealFluid.cpp:19-23, vanDerWaalsFluid.cpp:24-31).  Pure host
+    UvFromp (idealFluid.cpp:19-23, vanDerWaalsFluid.cpp:24-31; Redlich-Kwong
+    RedlichKwongFluid.cpp:59-70 with its cubic solved by Newton's method in
+    FP64, a few ulps from the reference's long-double root).  Pure host
     arithmetic on numpy arrays; used to prepare initial states."""
     c = np.asarray(c, dtype=np.float64)
     v = np.asarray(v, dtype=np.float64)
@@ -363,6 +444,17 @@ def conserved_from_primitive(case: Case, c, v, p) -> np.ndarray:
                 b = b + X[k] * X[l] * bm[k, l]
         acc = a * (c0 * c0)
         U = (p + acc) * (1 - c0 * b) / (case.R / cv) - acc
+    elif int(case.gas_model) == 2:
+        am, bm = rk_matrices(case)
+        X = [c[k] / c0 for k in range(Ns)]
+        a = np.zeros(n)
+        b = np.zeros(n)
+        for k in range(Ns):
+            for l in range(Ns):
+                a = a + X[k] * X[l] * am[k, l]
+                b = b + X[k] * X[l] * bm[k, l]
+        s = _rk_root(c0 * case.R, -p * (1 - c0 * b), -a * (c0 * c0) * (1 - c0 * b) / (1 + c0 * b))
+        U = c0 * cv * (s * s) - 1.5 * a * c0 / (b * s) * np.log1p(c0 * b)
     else:
         U = p / (case.R / cv)
     v2 = ((0.0 + v[0] * v[0]) + v[1] * v[1]) + v[2] * v[2]

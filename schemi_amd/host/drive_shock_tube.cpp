@@ -1,19 +1,22 @@
 // drive_shock_tube.cpp -- the reference's main loop (Schemi/main/Schemi.cpp:
-// 767-940) driving the GPU step through createStepSolver("SecondOrderRK_GPU").
+// 767-940) driving the GPU step through createStepSolver.
 //
 // Runs the 1-D two-component shock tube for a number of steps with the
 // Courant time step and prints the final time, the number of steps and a
 // checksum of every conserved field, one per line, so the test harness can
-// compare them with the oracle.  Usage: drive_shock_tube <nsteps>
+// compare them with the oracle.  Usage: drive_shock_tube <nsteps> [<key>],
+// key "SecondOrderRK_GPU" (default) or "SecondOrder_GPU".
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include "gpuStepSolverRK.hpp"
 
 int main(int argc, char **argv)
 {
     const int nsteps = argc > 1 ? std::atoi(argv[1]) : 10;
+    const std::string key = argc > 2 ? argv[2] : "SecondOrderRK_GPU";
     const std::size_t N = 1000, ns = 2;
     schemi::stepSettings s;
     s.mesh.dim = 1;
@@ -63,7 +66,7 @@ int main(int argc, char **argv)
     }
     double timestep = s.scheme.min_time, time = 0.0;
     try {
-        auto solver = schemi::createStepSolver("SecondOrderRK_GPU", ph, s, timestep);
+        auto solver = schemi::createStepSolver(key, ph, s, timestep);
         auto *gpu = static_cast<schemi::gpuStepSolverRK *>(solver.get());
         for (int step = 0; step < nsteps; ++step) {
             solver->calculateStep();

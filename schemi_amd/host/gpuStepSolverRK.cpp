@@ -96,9 +96,15 @@ std::unique_ptr<abstractStepSolver> createStepSolver(const std::string &key, pha
                                                      const stepSettings &settings,
                                                      const double &timestep)
 {
+    // the key selects the step solver (abstractStepSolver.cpp:83-84, 97-118)
+    stepSettings s = settings;
     if (key == "SecondOrderRK_GPU")
-        return std::make_unique<gpuStepSolverRK>(phase, settings, timestep);
-    throw exception("Unknown or CPU-only step solver: " + key, errors::initialisationError);
+        s.scheme.step_solver = SCHEMI_STEP_SECOND_ORDER_RK;
+    else if (key == "SecondOrder_GPU")
+        s.scheme.step_solver = SCHEMI_STEP_SECOND_ORDER;
+    else
+        throw exception("Unknown or CPU-only step solver: " + key, errors::initialisationError);
+    return std::make_unique<gpuStepSolverRK>(phase, s, timestep);
 }
 
 }  // namespace schemi

@@ -82,8 +82,9 @@ public:
     virtual void calculateStep() = 0;
 };
 
-// SecondOrderRK on the GPU.  timestep is the caller's mesh::timestep()
-// (mesh.hpp:151), read at every call.
+// SecondOrderRK (SSP-RK2) or SecondOrder (MUSCL-Hancock) on the GPU, as
+// settings.scheme.step_solver selects.  timestep is the caller's
+// mesh::timestep() (mesh.hpp:151), read at every call.
 class gpuStepSolverRK final : public abstractStepSolver {
 public:
     gpuStepSolverRK(phaseFields &phase, const stepSettings &settings, const double &timestep);
@@ -113,7 +114,8 @@ private:
 };
 
 // createStepSolver (Schemi/src/interfaces/abstractStepSolver.cpp:56-146) with
-// the GPU key: "SecondOrderRK_GPU".  Other keys throw initialisationError.
+// the GPU keys "SecondOrderRK_GPU" and "SecondOrder_GPU" (they set
+// settings.scheme.step_solver).  Other keys throw initialisationError.
 std::unique_ptr<abstractStepSolver> createStepSolver(const std::string &key, phaseFields &phase,
                                                      const stepSettings &settings,
                                                      const double &timestep);

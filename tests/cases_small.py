@@ -1,4 +1,6 @@
 """Small parity cases shared by the CPU and GPU tests (seeded, deterministic)."""
+import copy
+
 import numpy as np
 
 from schemi_amd import Case, typeOfTVDLimiter as TL, flowSolver as FS, gasModel as GM
@@ -102,4 +104,33 @@ def cases():
     out["3d_vdw_vanAlbada"] = c
     c = Case(dim=2, N=(24, 16, 1), L=(0.7, 0.5, 0.1), limiter=TL.linear, **BASE, **vdw)
     out["2d_vdw_linear_general"] = c
+    # Redlich-Kwong mixture (cubic in sqrt(T)): compared with a tolerance
+    rk = dict(gas_model=GM.RedlichKwong, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    out["1d_rk_vanLeer"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer,
+                                **BASE, **rk)
+    c = Case(dim=2, N=(64, 32, 1), L=(1.0, 0.5, 0.1), limiter=TL.minmod, flux=FS.HLL, **BASE,
+             **rk)
+    periodic(c, 0)
+    c.set_bc(3, BCT.slip)
+    out["2d_rk_periodic_hll"] = c
+    out["3d_rk_vanAlbada"] = Case(dim=3, N=(24, 10, 12), L=(1.0, 0.5, 0.5),
+                                  limiter=TL.vanAlbada, **BASE, **rk)
+    out["2d_rk_linear_general"] = Case(dim=2, N=(24, 16, 1), L=(0.7, 0.5, 0.1),
+                                       limiter=TL.linear, **BASE, **rk)
+    # MUSCL-Hancock (SecondOrder) on every mesh kind it runs on: 1-D with each
+    # limiter and boundary kind, and 2-D/3-D where the slopes vanish
+    # (including the reversed top face normal, 3d_periodic_x_vanLeer)
+    for name in ("1d_minmod", "1d_vanLeer", "1d_vanAlbada", "1d_linear", "1d_zero", "1d_hll",
+                 "1d_slip", "1d_fixed", "1d_periodic", "1d_vdw_vanLeer", "1d_rk_vanLeer",
+                 "2d_vanLeer",
+                 "2d_hll_grav", "2d_periodic_hll", "2d_zero_fixed", "2d_vdw_periodic_hll",
+                 "3d_vanAlbada", "3d_hll_partial_tiles", "3d_periodic_x_vanLeer",
+                 "3d_vdw_vanAlbada"):
+        c = copy.deepcopy(out[name])
+        c.step_solver = 0
+        out["hancock_" + name] = c
+    # reversed top face normal with a reflecting top
+    c = Case(dim=3, N=(16, 8, 8), L=(1.0, 0.5, 0.25), limiter=TL.minmod, step_solver=0, **BASE)
+    c.set_bc(5, BCT.slip)
+    out["hancock_3d_lz_short_slip"] = c
     return out

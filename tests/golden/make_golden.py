@@ -208,6 +208,23 @@ def vdw_cases(base):
     run_const("seeded_3d_vdw_vanalbada", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 23))
 
 
+def rk_cases(base):
+    """Redlich-Kwong mixture (gasModel::RedlichKwong): with Tc = (24, 3.2),
+    Pc = (1, 1), R = 1, c b is about 0.05-0.1 and a c^2 / sqrt(T) about a
+    fifth of p for the seeded states."""
+    eos = dict(gas_model=gasModel.RedlichKwong, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, **base, **eos)
+    run_const("seeded_1d_rk_vanleer", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 41))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             **base, **eos)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_rk_hll", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 42))
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.minmod, step_solver=0,
+             **base, **eos)
+    run_const("seeded_1d_rk_hancock", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 43))
+
+
 def hancock_cases(base):
     """MUSCL-Hancock (#Approximation_order SecondOrder): one Advection with
     the half-step face predictor per step."""
@@ -244,5 +261,7 @@ if __name__ == "__main__":
         vdw_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "hancock":
         hancock_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "rk":
+        rk_cases(b)
     else:
         main()

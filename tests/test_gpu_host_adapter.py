@@ -1,5 +1,5 @@
-"""The C++ drop-in (createStepSolver("SecondOrderRK_GPU"), gpuStepSolverRK)
-driven by the reference's time loop, against the oracle."""
+"""The C++ drop-in (createStepSolver("SecondOrderRK_GPU" / "SecondOrder_GPU"),
+gpuStepSolverRK) driven by the reference's time loop, against the oracle."""
 import os
 import subprocess
 
@@ -11,17 +11,19 @@ DRIVER = os.path.join(ROOT, "schemi_amd", "host", "drive_shock_tube")
 
 
 @pytest.mark.gpu
-def test_cpp_adapter_shock_tube(hip_device):
+@pytest.mark.parametrize("key,step_solver", [("SecondOrderRK_GPU", 1), ("SecondOrder_GPU", 0)])
+def test_cpp_adapter_shock_tube(hip_device, key, step_solver):
     from oracle.oracle import Oracle
     from schemi_amd import cases
 
     if not os.path.exists(DRIVER):
         pytest.fail("schemi_amd/host/drive_shock_tube is not built")
     nsteps = 50
-    r = subprocess.run([DRIVER, str(nsteps)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DRIVER, str(nsteps), key], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     got = dict(line.split() for line in r.stdout.strip().splitlines())
     case, (c, v, p) = cases.shock_tube_1d(1000)
+    case.step_solver = step_solver
     o = Oracle(case)
     o.init_primitive(c, v, p)
     dt, t = case.min_time, 0.0
@@ -36,3 +38,13 @@ def test_cpp_adapter_shock_tube(hip_device):
     assert np.isclose(float(got["density1"]), st["rho"][0].sum(), rtol=1e-13)
     assert np.isclose(float(got["density2"]), st["rho"][1].sum(), rtol=1e-13)
     assert np.isclose(float(got["totalEnergy"]), st["E"].sum(), rtol=1e-13)
+
+
+def test_cpp_adapter_rejects_unknown_key():
+    if not os.path.exists(DRIVER):
+        pytest.skip("schemi_amd/host/drive_shock_tube is not built")
+    # the key is checked before any device call
+    r = subprocess.run([DRIVER, "1", "ThirdOrder_GPU"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode != 0
+    assert "Unknown or CPU-only step solver" in r.stderr

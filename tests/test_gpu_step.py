@@ -218,7 +218,9 @@ def test_multi_d_mesh_needs_two_cells_per_axis(hip_device):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["1d_vanLeer", "2d_linear", "2d_linear_dyadic", "3d_vanAlbada",
                                   "3d_periodic_yz_linear", "2d_vdw_periodic_hll",
-                                  "3d_vdw_vanAlbada"])
+                                  "3d_vdw_vanAlbada", "hancock_1d_vanLeer",
+                                  "hancock_3d_vanAlbada", "hancock_1d_vdw_vanLeer",
+                                  "3d_rk_vanAlbada", "2d_rk_periodic_hll"])
 def test_mixed_precision_all_kernels(name, hip_device):
     """The mixed path of every kernel family stays within 1e-5 (relative L1
     of the density) of the FP64 path over 3 steps and keeps sum rho_k = rho."""
@@ -255,3 +257,78 @@ def test_periodic_axis_with_reversed_face_rejected(hip_device):
     with pytest.raises(SchemiGpuError) as e:
         GpuSolver(case)
     assert e.value.code == errors.meshGenerationError
+
+
+@pytest.mark.gpu
+def test_hancock_linear_multi_d_rejected(hip_device):
+    """MUSCL-Hancock with the linear limiter on a 2-D mesh (slopes that do not
+    vanish) has no kernel: declined with initialisationError."""
+    base = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
+    case = Case(dim=2, N=(16, 8, 1), L=(0.5, 0.25, 0.1), limiter=TL.linear, step_solver=0,
+                **base)
+    with pytest.raises(SchemiGpuError) as e:
+        GpuSolver(case)
+    assert e.value.code == errors.initialisationError
+    case.step_solver = 7
+    case.dim, case.N = 1, (16, 1, 1)
+    with pytest.raises(SchemiGpuError) as e:
+        GpuSolver(case)
+    assert e.value.code == errors.initialisationError
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["hancock_1d_vanLeer", "hancock_2d_periodic_hll"])
+def test_hancock_runs_alternate_buffers(name, hip_device):
+    """MUSCL-Hancock steps alternate between the two state buffers: runs of odd
+    and even length, single steps and a run stopped at t_end all leave the
+    current state where set/get_state and the device pointer expect it, bit
+    for bit equal to the oracle driven with the same Courant steps."""
+    from oracle.oracle import Oracle
+    from tests.cases_small import cases
+
+    case = cases()[name]
+    c, v, p = random_primitives(case, 5)
+    U0 = conserved_from_primitive(case, c, v, p)
+    Ns = case.Ns
+    o = Oracle(case)
+    o.set_conserved(U0[:Ns], U0[Ns], U0[Ns + 1:Ns + 4], U0[Ns + 4])
+    s = GpuSolver(case)
+    s.set_conserved(U0)
+    ptr0, _ = s.device_state()
+
+    def oracle_steps(k, t, dt, t_end=math.inf):
+        # Schemi.cpp:665-670, 802-868: dt clipped to reach t_end exactly
+        if t + dt > t_end:
+            dt = t_end - t
+        n = 0
+        for _ in range(k):
+            assert o.step(dt, 1) == 0
+            t = t + dt
+            n += 1
+            dt = o.courant_dt(case.courant, case.min_time)
+            if t + dt > t_end:
+                dt = t_end - t
+            if not t < t_end:
+                break
+        return t, n, dt
+
+    t, dt = 0.0, case.min_time
+    for k in (3, 4, 1):
+        t_o, _, dt_o = oracle_steps(k, t, dt)
+        t_gpu, steps, dt_gpu = s.run(k, dt)
+        assert steps == k
+        assert t_gpu == t_o and dt_gpu == dt_o
+        assert np.array_equal(s.conserved(), o.conserved()), k
+        assert s.device_state()[0] == ptr0
+        t, dt = t_gpu, dt_gpu
+    nxt = s.step(dt)
+    assert o.step(dt, 1) == 0
+    t = t + dt
+    assert nxt == o.courant_dt(case.courant, case.min_time)
+    assert np.array_equal(s.conserved(), o.conserved())
+    # a run that reaches t_end part-way leaves the later queued steps idle
+    t_end = t + 3.5 * nxt
+    t_o, n_o, _ = oracle_steps(10, t, nxt, t_end)
+    t2, n2, _ = s.run(10, nxt, t_end)
+    assert t_o == t_end and t2 == t_end and n2 == n_o < 10
+    assert np.array_equal(s.conserved(), o.conserved())

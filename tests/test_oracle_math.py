@@ -183,3 +183,25 @@ def test_vdw_tends_to_ideal_mixture():
         assert np.abs(near[key] - ideal[key]).max() <= 1e-6 * scale, key
     # at equal initial p the van der Waals internal energy differs by ~a c0^2
     assert np.abs(far["E"] - ideal["E"]).max() > 1e-2 * np.abs(ideal["E"]).max()
+
+
+def test_rk_initial_state_matches_oracle():
+    """The host's Redlich-Kwong UvFromp (FP64 Newton root of the cubic in
+    sqrt(T)) gives the oracle's conserved state (long-double Cardano, the
+    reference's solver) to ~1e-15, and the mixing matrices agree exactly."""
+    from schemi_amd import Case, gasModel as GM
+    from schemi_amd.gpu import conserved_from_primitive, rk_matrices
+    from oracle.oracle import Oracle
+
+    case = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), M=[29.0, 4.0], Cv=[2.5, 1 / 0.67],
+                gas_model=GM.RedlichKwong, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    c, v, p = random_primitives(case, 9)
+    U = conserved_from_primitive(case, c, v, p)
+    o = Oracle(case)
+    o.init_primitive(c, v, p)
+    ref = o.conserved()
+    for q in range(ref.shape[0]):
+        scale = max(np.abs(ref[q]).max(), 1e-300)
+        assert np.abs(U[q] - ref[q]).max() <= 2e-15 * scale, q
+    a, b = rk_matrices(case)
+    assert a[0, 0] > 1000 and 2.0 < b[0, 0] < 2.2  # Tc 24, Pc 1: strongly non-ideal

@@ -93,6 +93,7 @@ typedef struct {
 #define SCHEMI_GAS_IDEAL 0
 #define SCHEMI_GAS_VAN_DER_WAALS 1
 #define SCHEMI_GAS_REDLICH_KWONG 2
+#define SCHEMI_GAS_STIFFENED 3
 
 /* Mixture (abstractMixtureThermodynamics::createThermodynamics,
  * Schemi/src/interfaces/abstractMixtureThermodynamics.cpp:32-63): molar
@@ -110,6 +111,10 @@ typedef struct {
     int32_t gas_model; /* SCHEMI_GAS_* */
     double Tcrit[SCHEMI_MAX_SPECIES];
     double Pcrit[SCHEMI_MAX_SPECIES];
+    /* stiffened mixture (mixtureStiffened.cpp:19-43, read by the reference
+     * from set/stiffenedFluidData.txt): p0 and gamma of every species */
+    double p0[SCHEMI_MAX_SPECIES];
+    double gamma[SCHEMI_MAX_SPECIES];
 } schemi_mix_desc;
 
 /* typeOfSolverEnum (Schemi/enumerations/typeOfSolverEnum.hpp:15):

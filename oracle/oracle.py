@@ -41,6 +41,8 @@ class _Config(C.Structure):
         ("Pc", C.c_double * MAXS),
         ("a_kl", C.c_double * (MAXS * MAXS)),
         ("b_kl", C.c_double * (MAXS * MAXS)),
+        ("p0", C.c_double * MAXS),
+        ("gam", C.c_double * MAXS),
     ]
 
 
@@ -126,6 +128,8 @@ def make_config(case) -> _Config:
     for k in range(case.Ns):
         cfg.Tc[k] = float(case.Tc[k])
         cfg.Pc[k] = float(case.Pc[k])
+        cfg.p0[k] = float(case.p0[k])
+        cfg.gam[k] = float(case.gammas[k])
     cfg.flux = int(case.flux)
     cfg.limiter = int(case.limiter)
     cfg.gravity = 1 if case.gravity else 0

@@ -654,6 +654,25 @@ static void rk_matrices(or_config *cfg)
             }
 }
 
+/* stiffened mixture (mixtureStiffened.cpp:19-43): p0_kl = sqrt(p0_k p0_l),
+ * gamma_kl = sqrt(gamma_k gamma_l) off the diagonal */
+static void stiff_matrices(or_config *cfg)
+{
+    const int n = cfg->Ns;
+    for (int k = 0; k < n; ++k) {
+        cfg->a_kl[k * OR_MAXS + k] = cfg->p0[k];
+        cfg->b_kl[k * OR_MAXS + k] = cfg->gam[k];
+    }
+    for (int k = 0; k < n; ++k)
+        for (int l = 0; l < n; ++l)
+            if (k != l) {
+                cfg->a_kl[k * OR_MAXS + l] =
+                    sqrt(cfg->a_kl[k * OR_MAXS + k] * cfg->a_kl[l * OR_MAXS + l]);
+                cfg->b_kl[k * OR_MAXS + l] =
+                    sqrt(cfg->b_kl[k * OR_MAXS + k] * cfg->b_kl[l * OR_MAXS + l]);
+            }
+}
+
 /* signum with the reference's dead band (sign.cpp:12-24) */
 static int rk_sign(double x)
 {
@@ -798,6 +817,12 @@ static double rk_dUvdT(double c, double Cv, double T, double a, double b)
  * vdW g1 (U + a c^2) / (1 - c b) - a c^2 (vanDerWaalsFluid.cpp:14-22) */
 static double eos_p(const or_config *cfg, const double *c, double U)
 {
+    if (cfg->gas_model == OR_GAS_STIFFENED) {
+        /* stiffenedFluid.cpp:13-17, mixtureStiffened.cpp:485-503 */
+        double p0, g;
+        vdw_mix(cfg, c, &p0, &g);
+        return (g - 1.0) * U - p0 * g;
+    }
     if (cfg->gas_model == OR_GAS_RK) {
         /* RedlichKwongFluid.cpp:46-57 */
         double a, b, cv, mm;
@@ -818,6 +843,12 @@ static double eos_p(const or_config *cfg, const double *c, double U)
  * vdW (p + a c^2) (1 - c b) / g1 - a c^2 (vanDerWaalsFluid.cpp:24-31) */
 static double eos_U(const or_config *cfg, const double *c, double p)
 {
+    if (cfg->gas_model == OR_GAS_STIFFENED) {
+        /* stiffenedFluid.cpp:19-23, mixtureStiffened.cpp:505-523 */
+        double p0, g;
+        vdw_mix(cfg, c, &p0, &g);
+        return (p + p0 * g) / (g - 1.0);
+    }
     if (cfg->gas_model == OR_GAS_RK) {
         /* RedlichKwongFluid.cpp:59-70 */
         double a, b, cv, mm, x[3];
@@ -841,6 +872,14 @@ static double eos_U(const or_config *cfg, const double *c, double p)
  * (vanDerWaalsFluid.cpp:45-49) */
 static double eos_T(const or_config *cfg, const double *c, double U)
 {
+    if (cfg->gas_model == OR_GAS_STIFFENED) {
+        /* stiffenedFluid.cpp:37-41, mixtureStiffened.cpp:578-596: Cv_mix
+         * first, then p0_mix */
+        const double cv = cv_mix(cfg, c);
+        double p0, g;
+        vdw_mix(cfg, c, &p0, &g);
+        return (U - p0) / ((c[0] + STAB) * cv);
+    }
     if (cfg->gas_model == OR_GAS_RK) {
         /* RedlichKwongFluid.cpp:89-97 */
         double a, b, cv, mm;
@@ -868,6 +907,12 @@ static double eos_dpdUv_rk(const or_config *cfg, const double *c, double U)
 }
 static double eos_dpdUv(const or_config *cfg, const double *c)
 {
+    if (cfg->gas_model == OR_GAS_STIFFENED) {
+        /* stiffenedFluid.cpp:55-58, mixtureStiffened.cpp:627-640 */
+        double p0, g;
+        vdw_mix(cfg, c, &p0, &g);
+        return g - 1.0;
+    }
     if (cfg->gas_model == OR_GAS_VDW) {
         double a, b;
         vdw_mix(cfg, c, &a, &b);
@@ -883,6 +928,12 @@ static double eos_dpdUv(const or_config *cfg, const double *c)
 static double eos_c2(const or_config *cfg, const double *c, double rho, double U,
                      double p)
 {
+    if (cfg->gas_model == OR_GAS_STIFFENED) {
+        /* mixtureStiffened.cpp:657-663: dpdrho = 0, dpdUv = gamma - 1 */
+        const double dpdrho = 0;
+        const double dpdUv = eos_dpdUv(cfg, c);
+        return dpdrho + dpdUv * (U + p) / rho;
+    }
     if (cfg->gas_model == OR_GAS_RK) {
         /* RedlichKwongFluid.cpp:99-115, mixtureRedlichKwong.cpp:764-838 */
         double a, b, cv, mm;
@@ -1010,7 +1061,7 @@ or_solver *or_create(const or_config *cfg)
     if (cfg->dim < 1 || cfg->dim > 3 || cfg->Ns < 1 || cfg->Ns > OR_MAXS)
         return NULL;
     if (cfg->gas_model != OR_GAS_IDEAL && cfg->gas_model != OR_GAS_VDW &&
-        cfg->gas_model != OR_GAS_RK)
+        cfg->gas_model != OR_GAS_RK && cfg->gas_model != OR_GAS_STIFFENED)
         return NULL;
     or_solver *s = calloc(1, sizeof(or_solver));
     s->cfg = *cfg;
@@ -1018,6 +1069,8 @@ or_solver *or_create(const or_config *cfg)
         vdw_matrices(&s->cfg);
     else if (cfg->gas_model == OR_GAS_RK)
         rk_matrices(&s->cfg);
+    else if (cfg->gas_model == OR_GAS_STIFFENED)
+        stiff_matrices(&s->cfg);
     for (int a = 0; a < cfg->dim; ++a)
         if (axis_periodic(s, a) < 0) {
             free(s);

@@ -24,7 +24,7 @@ enum { OR_BC_BLANK = 0, OR_BC_FREE = 1, OR_BC_SLIP = 2, OR_BC_FIXED_CELL = 3,
        OR_BC_PERIODIC = 32 /* no reference counterpart */ };
 enum { OR_FLUX_HLL = 0, OR_FLUX_HLLC = 2 };
 /* gasModel (gasModelEnum.hpp:15-18) */
-enum { OR_GAS_IDEAL = 0, OR_GAS_VDW = 1, OR_GAS_RK = 2 };
+enum { OR_GAS_IDEAL = 0, OR_GAS_VDW = 1, OR_GAS_RK = 2, OR_GAS_STIFFENED = 3 };
 enum { OR_LIM_ZERO = 0, OR_LIM_MINMOD = 1, OR_LIM_VANLEER = 2, OR_LIM_LINEAR = 3,
        OR_LIM_VANALBADA = 4 };
 /* errors enum: noErrors=0, RiemannSolverError=1, boundaryConditionError=5,
@@ -58,10 +58,14 @@ typedef struct {
     int gas_model;
     double Tc[OR_MAXS];
     double Pc[OR_MAXS];
-    /* van der Waals / Redlich-Kwong mixing matrices a_kl, b_kl (row-major);
-     * filled by or_create, inputs ignored */
+    /* mixing matrices (row-major), filled by or_create, inputs ignored:
+     * van der Waals / Redlich-Kwong a_kl, b_kl; stiffened p0_kl in a_kl and
+     * gamma_kl in b_kl */
     double a_kl[OR_MAXS * OR_MAXS];
     double b_kl[OR_MAXS * OR_MAXS];
+    /* stiffened fluid (OR_GAS_STIFFENED): p0 and gamma of every species */
+    double p0[OR_MAXS];
+    double gam[OR_MAXS];
 } or_config;
 
 typedef struct or_solver or_solver;

@@ -57,6 +57,9 @@ class Case:
     gas_model: gasModel = gasModel.ideal
     Tc: Sequence[float] = ()
     Pc: Sequence[float] = ()
+    # stiffened fluid (./set/stiffenedFluidData.txt): p0 and gamma per species
+    p0: Sequence[float] = ()
+    gammas: Sequence[float] = ()
     # step solver (#Approximation_order): 1 SecondOrderRK (SSP-RK2),
     # 0 SecondOrder (MUSCL-Hancock)
     step_solver: int = 1
@@ -68,6 +71,10 @@ class Case:
             self.Tc = [1.0] * len(self.M)
         if not self.Pc:
             self.Pc = [1.0] * len(self.M)
+        if not self.p0:
+            self.p0 = [0.0] * len(self.M)
+        if not self.gammas:
+            self.gammas = [1.4] * len(self.M)
         n = list(self.N)
         if self.dim < 3:
             n[2] = 1

@@ -120,6 +120,12 @@ __global__ void prims_kernel(const DevParams *__restrict__ Pp, const double *__r
         vdw_ab<double, NS>(P, w, c0, a, b);
         Ut = Uint + a * (c0 * c0);
     }
+    // stiffened: U - p0 (stiffenedFluid.cpp:37-41)
+    if (P.gas_model == GAS_STIFF) {
+        double p0, g;
+        vdw_ab<double, NS>(P, w, c0, p0, g);
+        Ut = Uint - p0;
+    }
     // Redlich-Kwong: T = s^2 from its cubic (RedlichKwongFluid.cpp:89-97)
     W[(NS + 6) * n + id] =
         (P.gas_model == GAS_RK) ? rk_T<NS>(P, w, c0, Uint) : Ut / ((c0 + STAB) * cv);
@@ -371,9 +377,13 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
             return SCHEMI_ERR_MESH;
     if (mix->ns < 1 || mix->ns > 3 ||
         (mix->gas_model != SCHEMI_GAS_IDEAL && mix->gas_model != SCHEMI_GAS_VAN_DER_WAALS &&
-         mix->gas_model != SCHEMI_GAS_REDLICH_KWONG))
+         mix->gas_model != SCHEMI_GAS_REDLICH_KWONG && mix->gas_model != SCHEMI_GAS_STIFFENED))
         return SCHEMI_ERR_INIT;
-    if (mix->gas_model != SCHEMI_GAS_IDEAL)
+    if (mix->gas_model == SCHEMI_GAS_STIFFENED)
+        for (int k = 0; k < mix->ns; ++k)
+            if (!(mix->gamma[k] > 1.0) || !(mix->p0[k] >= 0.0))
+                return SCHEMI_ERR_INIT;
+    if (mix->gas_model == SCHEMI_GAS_VAN_DER_WAALS || mix->gas_model == SCHEMI_GAS_REDLICH_KWONG)
         for (int k = 0; k < mix->ns; ++k)
             if (!(mix->Tcrit[k] > 0.0) || !(mix->Pcrit[k] > 0.0))
                 return SCHEMI_ERR_INIT;
@@ -455,6 +465,20 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
                     P.bkl[k][l] = std::pow(0.5 * (std::cbrt(P.bkl[k][k]) + std::cbrt(P.bkl[l][l])),
                                            3.0);
                     P.akl[k][l] = std::sqrt(P.akl[k][k] * P.akl[l][l]);
+                }
+    }
+    if (mix->gas_model == SCHEMI_GAS_STIFFENED) {
+        // mixtureStiffened.cpp:19-43: p0_kl and gamma_kl, geometric means off
+        // the diagonal; kept in the a and b matrices
+        for (int k = 0; k < mix->ns; ++k) {
+            P.akl[k][k] = mix->p0[k];
+            P.bkl[k][k] = mix->gamma[k];
+        }
+        for (int k = 0; k < mix->ns; ++k)
+            for (int l = 0; l < mix->ns; ++l)
+                if (k != l) {
+                    P.akl[k][l] = std::sqrt(P.akl[k][k] * P.akl[l][l]);
+                    P.bkl[k][l] = std::sqrt(P.bkl[k][k] * P.bkl[l][l]);
                 }
     }
     for (int k = 0; k < mix->ns; ++k)
@@ -833,7 +857,8 @@ int schemi_gpu_adjoint_step(schemi_gpu_ctx *c, double dt, const double *lam_in, 
 {
     if (!c || !lam_in || !lam_out || (what != SCHEMI_ADJ_STEP && what != SCHEMI_ADJ_STAGE))
         return SCHEMI_ERR_INIT;
-    if (c->mode != MODE_FIRST_ORDER || c->hancock || c->hp.gas_model == GAS_RK)
+    if (c->mode != MODE_FIRST_ORDER || c->hancock ||
+        (c->hp.gas_model != GAS_IDEAL && c->hp.gas_model != GAS_VDW))
         return SCHEMI_ERR_INIT;
     if (c->pending_error)
         return c->pending_error;

@@ -25,8 +25,9 @@ enum : int { BC_FREE = 1, BC_SLIP = 2, BC_FIXED_CELL = 3, BC_FIXED_SURF = 4,
 enum : int { LIM_ZERO = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_LINEAR = 3,
              LIM_VANALBADA = 4 };
 enum : int { FLUX_HLL = 0, FLUX_HLLC = 2 };
-// gasModel (gasModelEnum.hpp:15-18): ideal, vanDerWaals, RedlichKwong
-enum : int { GAS_IDEAL = 0, GAS_VDW = 1, GAS_RK = 2 };
+// gasModel (gasModelEnum.hpp:15-18): ideal, vanDerWaals, RedlichKwong,
+// stiffened
+enum : int { GAS_IDEAL = 0, GAS_VDW = 1, GAS_RK = 2, GAS_STIFF = 3 };
 // stage variants: the two SSP-RK2 stages and the single MUSCL-Hancock stage
 enum : int { SV_RK1 = 0, SV_RK2 = 1, SV_HANCOCK = 2 };
 // Scalar block of a context (device memory, one double each): time step,
@@ -164,6 +165,13 @@ template <int GAS> __device__ __forceinline__ bool is_vdw(const DevParams &P)
     else
         return P.gas_model == GAS_VDW;
 }
+template <int GAS> __device__ __forceinline__ bool is_stiff(const DevParams &P)
+{
+    if constexpr (GAS >= 0)
+        return GAS == GAS_STIFF;
+    else
+        return P.gas_model == GAS_STIFF;
+}
 // Redlich-Kwong code is compiled only where the model can occur.
 template <int GAS> constexpr bool may_be_rk() { return GAS < 0 || GAS == GAS_RK; }
 template <int GAS> __device__ __forceinline__ bool is_rk(const DevParams &P)
@@ -298,6 +306,13 @@ __device__ __forceinline__ void to_f64(const T *c, double *cd)
 template <class T, int NS, int GAS = -1>
 __device__ __forceinline__ T eos_p(const DevParams &P, const T *c, T c0, T g1, T U)
 {
+    if (is_stiff<GAS>(P)) {
+        // stiffened mixture (stiffenedFluid.cpp:13-17, mixtureStiffened.cpp:
+        // 485-503): p0 and gamma mixed like a and b, held in the same matrices
+        T p0, g;
+        vdw_ab<T, NS>(P, c, c0, p0, g);
+        return (g - T(1)) * U - p0 * g;
+    }
     if constexpr (may_be_rk<GAS>()) {
         if (is_rk<GAS>(P)) {
             double cd[NS];
@@ -320,6 +335,12 @@ __device__ __forceinline__ T eos_p(const DevParams &P, const T *c, T c0, T g1, T
 template <class T, int NS, int GAS = -1>
 __device__ __forceinline__ T eos_U(const DevParams &P, const T *c, T c0, T g1, T p)
 {
+    if (is_stiff<GAS>(P)) {
+        // (p + p0 gamma) / (gamma - 1) (stiffenedFluid.cpp:19-23)
+        T p0, g;
+        vdw_ab<T, NS>(P, c, c0, p0, g);
+        return (p + p0 * g) / (g - T(1));
+    }
     if constexpr (may_be_rk<GAS>()) {
         if (is_rk<GAS>(P)) {
             double cd[NS];
@@ -346,6 +367,12 @@ template <class T, int NS, int GAS = -1>
 __device__ __forceinline__ T eos_c2(const DevParams &P, const T *c, T c0, T g1, T rho, T U,
                                     T p)
 {
+    if (is_stiff<GAS>(P)) {
+        // dpdrho = 0, dpdUv = gamma - 1 (mixtureStiffened.cpp:621-663)
+        T p0, g;
+        vdw_ab<T, NS>(P, c, c0, p0, g);
+        return T(0) + (g - T(1)) * (U + p) / rho;
+    }
     if constexpr (may_be_rk<GAS>()) {
         if (is_rk<GAS>(P)) {
             double cd[NS];

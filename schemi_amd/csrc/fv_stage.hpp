@@ -486,6 +486,40 @@ __device__ __forceinline__ void close_cell(const DevParams &P, const T *w, T *z)
     z[NS + 7] = s.cs;
 }
 
+// Closed state of a cell straight from its conserved values, for the mixed
+// path with the ideal mixture: c_k = rho_k / M_k, v = m / rho,
+// U = E - rho v^2 / 2, g1 = R c0 / sum_k c_k Cv_k, p = g1 U, and the face
+// quantities r0 = rho, U, E (the cell's own), cs^2 = g1 (U + p) / rho.  The
+// reference recomputes U from p and r0 = sum_k rho_k, E from U; on the mixed
+// path (species renormalised so that sum_k rho_k = rho) these agree with the
+// cell values to FP32 rounding, so the round trip is skipped.
+template <int NS>
+__device__ __forceinline__ void close_cell_lean(const DevParams &P, const float *u, float *z)
+{
+    float c0 = 0.0f, ccv = 0.0f;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        z[k] = u[k] / P.Mf[k];
+        c0 += z[k];
+        ccv += z[k] * P.Cvf[k];
+    }
+    const float rho = u[NS];
+    const float ir = 1.0f / rho;
+    const float vx = u[NS + 1] * ir, vy = u[NS + 2] * ir, vz = u[NS + 3] * ir;
+    const float E = u[NS + 4];
+    const float U = E - 0.5f * rho * (vx * vx + vy * vy + vz * vz);
+    const float g1 = P.Rf * c0 / ccv;
+    const float p = g1 * U;
+    z[NS] = vx;
+    z[NS + 1] = vy;
+    z[NS + 2] = vz;
+    z[NS + 3] = p;
+    z[NS + 4] = rho;
+    z[NS + 5] = U;
+    z[NS + 6] = E;
+    z[NS + 7] = __builtin_sqrtf(g1 * (U + p) * ir);
+}
+
 // Predicted face side from its conserved values (rho_k, r0, m, E after the
 // half step): c_k = rho_k / M_k, c_0 = sum c_k, v = m / r0,
 // U = E - r0 v^2 / 2 - rho k, p = pFromUv, cs from sqSonicSpeed
@@ -1111,6 +1145,11 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
 #ifndef SG_MIXED_FZ_LDS
 #define SG_MIXED_FZ_LDS 1
 #endif
+// Mixed path, ideal mixture: closure straight from the conserved values
+// (close_cell_lean), 2 % faster than the reference-order closure.
+#ifndef SG_MIXED_LEAN
+#define SG_MIXED_LEAN 1
+#endif
 
 // Stage kernel for schemes whose limited slopes vanish identically (zero
 // limiter; TVD limiters on 2-D and 3-D meshes: the reference's ratio along an
@@ -1150,6 +1189,9 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
     // reloaded for the update; the own column's z+ flux waits in LDS (FZ_LDS)
     constexpr bool HOLD_U = !MIXED || SG_MIXED_HOLD_U;
     constexpr bool FZ_LDS = (DIM == 3) && MIXED && SG_MIXED_FZ_LDS;
+    // mixed path, ideal mixture, RK stages: closure straight from the
+    // conserved values (close_cell_lean)
+    constexpr bool LEAN = MIXED && SG_MIXED_LEAN && GAS == GAS_IDEAL && !H;
     __shared__ T sFz[FZ_LDS ? NF : 1][FZ_LDS ? TY : 1][FZ_LDS ? TX : 1];
 
     const DevParams &P = *Pp;
@@ -1183,6 +1225,14 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
 
     // closed state of cell (ci, cj, ck) from its conserved values u
     auto closed_from = [&](const double *u, T *z, int ci, int cj, int ck) {
+        if constexpr (LEAN) {
+            float ut[NF];
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                ut[q] = float(u[q]);
+            close_cell_lean<NS>(P, ut, z);
+            return;
+        }
         T w[NQ];
         if constexpr (MIXED) {
             T ut[NF];
@@ -1367,6 +1417,7 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                 load_cons<NS>(Un, n, id, un);
             if constexpr (!HOLD_U)
                 load_cons<NS>(Uin, n, id, ucur);
+
             update_cell<NS, DIM, SV, MIXED, GAS>(P, cg, n, id, ucur, Uout, un, dt, xlo, ylo,
                                                 zlo, F, sig);
         }

@@ -51,7 +51,8 @@ class MeshDesc(C.Structure):
 class MixDesc(C.Structure):
     _fields_ = [("ns", C.c_int32), ("M", C.c_double * MAXS), ("Cv", C.c_double * MAXS),
                 ("R", C.c_double), ("gas_model", C.c_int32), ("Tcrit", C.c_double * MAXS),
-                ("Pcrit", C.c_double * MAXS)]
+                ("Pcrit", C.c_double * MAXS), ("p0", C.c_double * MAXS),
+                ("gamma", C.c_double * MAXS)]
 
 
 class SchemeDesc(C.Structure):
@@ -154,6 +155,8 @@ def mix_desc(case: Case) -> MixDesc:
     for k in range(case.Ns):
         x.Tcrit[k] = float(case.Tc[k])
         x.Pcrit[k] = float(case.Pc[k])
+        x.p0[k] = float(case.p0[k])
+        x.gamma[k] = float(case.gammas[k])
     return x
 
 
@@ -444,6 +447,19 @@ def conserved_from_primitive(case: Case, c, v, p) -> np.ndarray:
                 b = b + X[k] * X[l] * bm[k, l]
         acc = a * (c0 * c0)
         U = (p + acc) * (1 - c0 * b) / (case.R / cv) - acc
+    elif int(case.gas_model) == 3:
+        # stiffened: (p + p0 gamma) / (gamma - 1), p0 and gamma mixed by
+        # geometric means (stiffenedFluid.cpp:19-23, mixtureStiffened.cpp:19-43)
+        X = [c[k] / c0 for k in range(Ns)]
+        p0m = np.zeros(n)
+        gm = np.zeros(n)
+        for k in range(Ns):
+            for l in range(Ns):
+                p0kl = case.p0[k] if k == l else _libm().sqrt(case.p0[k] * case.p0[l])
+                gkl = case.gammas[k] if k == l else _libm().sqrt(case.gammas[k] * case.gammas[l])
+                p0m = p0m + X[k] * X[l] * p0kl
+                gm = gm + X[k] * X[l] * gkl
+        U = (p + p0m * gm) / (gm - 1.0)
     elif int(case.gas_model) == 2:
         am, bm = rk_matrices(case)
         X = [c[k] / c0 for k in range(Ns)]

@@ -117,6 +117,14 @@ def cases():
                                   limiter=TL.vanAlbada, **BASE, **rk)
     out["2d_rk_linear_general"] = Case(dim=2, N=(24, 16, 1), L=(0.7, 0.5, 0.1),
                                        limiter=TL.linear, **BASE, **rk)
+    # stiffened mixture
+    st = dict(gas_model=GM.stiffened, p0=[0.3, 0.05], gammas=[2.8, 1.6])
+    out["1d_stiffened_minmod"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0),
+                                      limiter=TL.minmod, **BASE, **st)
+    c = Case(dim=3, N=(24, 10, 12), L=(1.0, 0.5, 0.5), limiter=TL.vanAlbada, flux=FS.HLL,
+             **BASE, **st)
+    c.set_bc(3, BCT.slip)
+    out["3d_stiffened_hll"] = c
     # MUSCL-Hancock (SecondOrder) on every mesh kind it runs on: 1-D with each
     # limiter and boundary kind, and 2-D/3-D where the slopes vanish
     # (including the reversed top face normal, 3d_periodic_x_vanLeer)

@@ -36,6 +36,8 @@ KEYS = ("c0", "c", "rho0", "rho", "v", "p", "U", "T", "m", "E")
 def case_json(case: Case) -> str:
     eos = {} if int(case.gas_model) == 0 else {
         "gas_model": int(case.gas_model), "Tc": list(case.Tc), "Pc": list(case.Pc)}
+    if int(case.gas_model) == 3:
+        eos.update(p0=list(case.p0), gammas=list(case.gammas))
     if int(case.step_solver) != 1:
         eos["step_solver"] = int(case.step_solver)
     return json.dumps({**eos,
@@ -225,6 +227,21 @@ def rk_cases(base):
     run_const("seeded_1d_rk_hancock", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 43))
 
 
+def stiffened_cases(base):
+    """Stiffened mixture (gasModel::stiffened): p = (gamma - 1) U - gamma p0
+    with p0 = (0.3, 0.05) and gamma = (2.8, 1.6) per species, mixed by
+    geometric means."""
+    eos = dict(gas_model=gasModel.stiffened, p0=[0.3, 0.05], gammas=[2.8, 1.6])
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, **base, **eos)
+    run_const("seeded_1d_stiffened_vanleer", c, 5, 2e-4, (1, 5),
+              restart=seeded_primitives(c, 51))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             **base, **eos)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_stiffened_hll", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 52))
+
+
 def hancock_cases(base):
     """MUSCL-Hancock (#Approximation_order SecondOrder): one Advection with
     the half-step face predictor per step."""
@@ -263,5 +280,7 @@ if __name__ == "__main__":
         hancock_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "rk":
         rk_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "stiffened":
+        stiffened_cases(b)
     else:
         main()

@@ -205,3 +205,22 @@ def test_rk_initial_state_matches_oracle():
         assert np.abs(U[q] - ref[q]).max() <= 2e-15 * scale, q
     a, b = rk_matrices(case)
     assert a[0, 0] > 1000 and 2.0 < b[0, 0] < 2.2  # Tc 24, Pc 1: strongly non-ideal
+
+
+def test_stiffened_initial_state_matches_oracle():
+    """The host's stiffened UvFromp gives the oracle's conserved state bit for
+    bit (closed form, same operation order)."""
+    from schemi_amd import Case, gasModel as GM
+    from schemi_amd.gpu import conserved_from_primitive
+    from oracle.oracle import Oracle
+
+    case = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), M=[29.0, 4.0], Cv=[2.5, 1 / 0.67],
+                gas_model=GM.stiffened, p0=[0.3, 0.05], gammas=[2.8, 1.6])
+    c, v, p = random_primitives(case, 9)
+    U = conserved_from_primitive(case, c, v, p)
+    o = Oracle(case)
+    o.init_primitive(c, v, p)
+    ref = o.conserved()
+    for q in range(ref.shape[0]):
+        scale = max(np.abs(ref[q]).max(), 1e-300)
+        assert np.abs(U[q] - ref[q]).max() <= 1e-15 * scale, q

@@ -120,9 +120,7 @@ typedef struct {
 /* typeOfSolverEnum (Schemi/enumerations/typeOfSolverEnum.hpp:15):
  * SecondOrder is the MUSCL-Hancock step (secondOrderStepSolver.cpp:33-48),
  * SecondOrderRK the SSP-RK2 step (secondOrderStepSolverRK.cpp:48-71).
- * MUSCL-Hancock runs on 1-D meshes and wherever the limited slopes vanish
- * (zero limiter; TVD limiters in 2-D and 3-D); the linear limiter in 2-D and
- * 3-D with it is rejected with SCHEMI_ERR_INIT. */
+ * Both run with every limiter, flux, EOS and mesh here. */
 #define SCHEMI_STEP_SECOND_ORDER 0
 #define SCHEMI_STEP_SECOND_ORDER_RK 1
 

@@ -180,7 +180,7 @@ struct schemi_gpu_ctx {
     int mode = MODE_SLOPES;
     bool hancock = false;  // SCHEMI_STEP_SECOND_ORDER: one MUSCL-Hancock stage per step
     void *scratch = nullptr;  // MODE_GENERAL: primitives and gradients
-    void *W = nullptr, *Lg = nullptr;
+    void *W = nullptr, *Lg = nullptr, *Hs = nullptr;
     long long n = 0;
     int kchunk = 16;
     double const_dt = -1.0;
@@ -226,6 +226,7 @@ int launch_stage(schemi_gpu_ctx *c, const double *Uin, double *Uout, const doubl
     L.gas_model = c->hp.gas_model;
     L.W = c->W;
     L.Lg = c->Lg;
+    L.Hs = c->Hs;
     L.nx = c->hp.N[0];
     L.ny = c->hp.N[1];
     L.nz = c->hp.N[2];
@@ -557,17 +558,14 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     const char *full = std::getenv("SCHEMI_GPU_FULL_LIMITER");
     const bool force_slopes = full && full[0] == '1' && !c->hancock;
     if (lim == SCHEMI_LIM_LINEAR)
-        c->mode = (dim >= 2 && !separable) ? MODE_GENERAL : MODE_SLOPES;
+        c->mode = (dim >= 2 && (!separable || c->hancock)) ? MODE_GENERAL : MODE_SLOPES;
     else if (lim == SCHEMI_LIM_ZERO || dim >= 2)
         c->mode = force_slopes ? MODE_SLOPES : MODE_FIRST_ORDER;
     else
         c->mode = MODE_SLOPES;
-    // MUSCL-Hancock runs where the slopes vanish and on 1-D meshes; the
-    // linear limiter's 2-D/3-D predictor has no kernel here
-    if (c->hancock && c->mode != MODE_FIRST_ORDER && dim >= 2) {
-        delete c;
-        return SCHEMI_ERR_INIT;
-    }
+    // MUSCL-Hancock: the slope kernel in 1-D, the first-order kernel where
+    // the slopes vanish, the general-geometry kernels for the linear limiter
+    // in 2-D/3-D (every slot's predicted side is stored)
     // periodic axes need whole tiles so the wrapped neighbours are staged
     {
         const bool fo = c->mode == MODE_FIRST_ORDER;
@@ -620,10 +618,13 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     if (ok && c->mode == MODE_GENERAL) {
         const size_t tsz = (c->prec == SCHEMI_PREC_MIXED) ? sizeof(float) : sizeof(double);
         const size_t nq = (size_t)(mix->ns + 4) * (size_t)c->n;
-        ok = hipMalloc(&c->scratch, tsz * nq * (size_t)(1 + dim)) == hipSuccess;
+        const size_t nh = c->hancock ? (size_t)(2 * dim) * (size_t)(mix->ns + 10) * (size_t)c->n
+                                     : 0;
+        ok = hipMalloc(&c->scratch, tsz * (nq * (size_t)(1 + dim) + nh)) == hipSuccess;
         if (ok) {
             c->W = c->scratch;
             c->Lg = static_cast<char *>(c->scratch) + tsz * nq;
+            c->Hs = static_cast<char *>(c->scratch) + tsz * nq * (size_t)(1 + dim);
         }
     }
     ok = ok &&

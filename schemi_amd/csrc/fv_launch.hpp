@@ -46,6 +46,7 @@ struct StageLaunch {
     int nx, ny, nz;
     int kchunk;
     void *W, *Lg;  // MODE_GENERAL scratch: primitives and gradients
+    void *Hs;      // MODE_GENERAL with MUSCL-Hancock: predicted face sides
     hipStream_t stream;
 };
 

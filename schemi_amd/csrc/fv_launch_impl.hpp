@@ -53,8 +53,19 @@ static int launch_mode(const StageLaunch &L)
                                L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
         break;
     case MODE_GENERAL: {
-        if constexpr (DIM == 1 || SV == SV_HANCOCK) {
+        if constexpr (DIM == 1) {
             return -1;
+        } else if constexpr (SV == SV_HANCOCK) {
+            const long long n = (long long)L.nx * L.ny * L.nz;
+            const dim3 g((unsigned)((n + 255) / 256));
+            T *W = static_cast<T *>(L.W), *Lg = static_cast<T *>(L.Lg);
+            T *Hs = static_cast<T *>(L.Hs);
+            hipLaunchKernelGGL((gen_grad_kernel<T, NS, DIM, MIXED, true>), g, dim3(256), 0,
+                               L.stream, L.P, L.geo, L.Uin, W, Lg, L.dt, L.Uout);
+            hipLaunchKernelGGL((gen_hancock_sides_kernel<T, NS, DIM, MIXED>), g, dim3(256), 0,
+                               L.stream, L.P, L.geo, W, Lg, Hs, L.dt);
+            hipLaunchKernelGGL((gen_hancock_update_kernel<T, NS, DIM, MIXED>), g, dim3(256), 0,
+                               L.stream, L.P, L.geo, L.Uin, L.Uout, Hs, L.dt, L.sigmax, L.err);
         } else {
             const long long n = (long long)L.nx * L.ny * L.nz;
             const dim3 g((unsigned)((n + 255) / 256));

@@ -1150,6 +1150,9 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
 #ifndef SG_MIXED_LEAN
 #define SG_MIXED_LEAN 1
 #endif
+#ifndef SG_HALO_AHEAD
+#define SG_HALO_AHEAD 0
+#endif
 
 // Stage kernel for schemes whose limited slopes vanish identically (zero
 // limiter; TVD limiters on 2-D and 3-D meshes: the reference's ratio along an
@@ -1192,6 +1195,12 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
     // mixed path, ideal mixture, RK stages: closure straight from the
     // conserved values (close_cell_lean)
     constexpr bool LEAN = MIXED && SG_MIXED_LEAN && GAS == GAS_IDEAL && !H;
+    // halo cells of plane k + 1 are closed during the face tasks of plane k,
+    // by the threads left idle in their last round (HALO_AHEAD)
+    constexpr int NHX = 2 * TY;
+    constexpr int NH = NHX + ((DIM >= 2) ? 2 * TX : 0);
+    constexpr bool HALO_AHEAD = (DIM == 3) && SG_HALO_AHEAD;
+    __shared__ T sHalo[HALO_AHEAD ? NV : 1][HALO_AHEAD ? NH : 1];
     __shared__ T sFz[FZ_LDS ? NF : 1][FZ_LDS ? TY : 1][FZ_LDS ? TX : 1];
 
     const DevParams &P = *Pp;
@@ -1259,6 +1268,36 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
         load_cons<NS>(Uin, n, ci + sy * cj + sz * (long long)ck, u);
         closed_from(u, z, ci, cj, ck);
     };
+    // halo slot h of the tile: its LDS position and global cell; false when
+    // it lies outside a non-periodic mesh.  x: the halo columns wrap on a
+    // periodic axis; y: the halo rows (h >= NHX) wrap, the tile rows of the x
+    // halo never do.
+    auto halo_cell = [&](int h, int &bxc, int &byc, int &gx, int &gy) -> bool {
+        if (h < NHX) {
+            byc = h / 2 + OY;
+            bxc = (h % 2) ? TX + 1 : 0;
+        } else {
+            const int hh = h - NHX;
+            bxc = hh % TX + 1;
+            byc = (hh / TX) ? TY + 1 : 0;
+        }
+        gx = x0 - 1 + bxc;
+        gy = y0 - OY + byc;
+        bool okx = true, oky = true;
+        if (gx < 0 || gx >= Nx) {
+            if (P.periodic[0] && (gx == -1 || gx == Nx))
+                gx = (gx < 0) ? Nx - 1 : 0;
+            else
+                okx = false;
+        }
+        if (gy < 0 || gy >= Ny) {
+            if (DIM >= 2 && h >= NHX && P.periodic[1] && (gy == -1 || gy == Ny))
+                gy = (gy < 0) ? Ny - 1 : 0;
+            else
+                oky = false;
+        }
+        return okx && oky;
+    };
 
     // own column: closed states of planes k and k+1, fluxes of its z faces;
     // the conserved values of plane k stay in registers for the update (a
@@ -1310,42 +1349,25 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
             for (int q = 0; q < NV; ++q)
                 sZ[q][ty + OY][tx + 1] = zc[q];
         }
-        {
-            constexpr int NHX = 2 * TY;
-            constexpr int NH = NHX + ((DIM >= 2) ? 2 * TX : 0);
+        if (!HALO_AHEAD || k == kb) {
             for (int h = tid; h < NH; h += NT) {
-                int bxc, byc;
-                if (h < NHX) {
-                    byc = h / 2 + OY;
-                    bxc = (h % 2) ? TX + 1 : 0;
-                } else {
-                    const int hh = h - NHX;
-                    bxc = hh % TX + 1;
-                    byc = (hh / TX) ? TY + 1 : 0;
-                }
-                int gx = x0 - 1 + bxc;
-                int gy = y0 - OY + byc;
-                // x: the halo columns wrap on a periodic axis; y: the halo rows
-                // (h >= NHX) wrap, the tile rows of the x halo never do
-                bool okx = true, oky = true;
-                if (gx < 0 || gx >= Nx) {
-                    if (P.periodic[0] && (gx == -1 || gx == Nx))
-                        gx = (gx < 0) ? Nx - 1 : 0;
-                    else
-                        okx = false;
-                }
-                if (gy < 0 || gy >= Ny) {
-                    if (DIM >= 2 && h >= NHX && P.periodic[1] && (gy == -1 || gy == Ny))
-                        gy = (gy < 0) ? Ny - 1 : 0;
-                    else
-                        oky = false;
-                }
-                if (okx && oky) {
+                int bxc, byc, gx, gy;
+                if (halo_cell(h, bxc, byc, gx, gy)) {
                     T z[NV];
                     closed_at(gx, gy, (DIM == 3) ? k : 0, z);
 #pragma unroll
                     for (int q = 0; q < NV; ++q)
                         sZ[q][byc][bxc] = z[q];
+                }
+            }
+        } else {
+            // closed during the face tasks of the previous plane
+            for (int h = tid; h < NH; h += NT) {
+                int bxc, byc, gx, gy;
+                if (halo_cell(h, bxc, byc, gx, gy)) {
+#pragma unroll
+                    for (int q = 0; q < NV; ++q)
+                        sZ[q][byc][bxc] = sHalo[q][h];
                 }
             }
         }
@@ -1355,9 +1377,22 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
         {
             constexpr int NTX = (TX + 1) * TY;
             constexpr int NTY = (DIM >= 2) ? TX * FYH : 0;
-            for (int t = tid; t < NTX + NTY; t += NT) {
+            const int ntask = NTX + NTY + ((HALO_AHEAD && k + 1 < ke) ? NH : 0);
+            for (int t = tid; t < ntask; t += NT) {
                 T g[NF];
                 T zo[NV], zn[NV];
+                if (HALO_AHEAD && t >= NTX + NTY) {
+                    const int h = t - NTX - NTY;
+                    int bxc, byc, gx, gy;
+                    if (halo_cell(h, bxc, byc, gx, gy)) {
+                        T z[NV];
+                        closed_at(gx, gy, k + 1, z);
+#pragma unroll
+                        for (int q = 0; q < NV; ++q)
+                            sHalo[q][h] = z[q];
+                    }
+                    continue;
+                }
                 if (t < NTX) {
                     const int u = t % (TX + 1), r = t / (TX + 1);
                     const int fx = x0 + u;

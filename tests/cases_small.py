@@ -137,6 +137,14 @@ def cases():
         c = copy.deepcopy(out[name])
         c.step_solver = 0
         out["hancock_" + name] = c
+    # the linear limiter with MUSCL-Hancock in 2-D/3-D (general-geometry
+    # kernels): non-dyadic, dyadic, periodic, reversed top face
+    for name in ("2d_linear", "2d_linear_dyadic", "2d_periodic_y_linear", "3d_linear",
+                 "3d_lz_short_linear_dyadic", "3d_lz_long_linear", "2d_vdw_linear_general",
+                 "2d_rk_linear_general"):
+        c = copy.deepcopy(out[name])
+        c.step_solver = 0
+        out["hancock_" + name] = c
     # reversed top face normal with a reflecting top
     c = Case(dim=3, N=(16, 8, 8), L=(1.0, 0.5, 0.25), limiter=TL.minmod, step_solver=0, **BASE)
     c.set_bc(5, BCT.slip)

@@ -265,6 +265,12 @@ def hancock_cases(base):
     for pt in (2, 3, 4, 5):
         c.set_bc(pt, BCT.slip)
     run_const("seeded_3d_hancock_vanalbada", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 34))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.linear, **base, **h)
+    c.set_bc(3, BCT.slip)
+    run_const("seeded_2d_hancock_linear", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 35))
+    c = Case(dim=3, N=(8, 6, 6), L=(1.0, 0.5, 0.35), limiter=TL.linear, flux=FS.HLL, **base,
+             **h)
+    run_const("seeded_3d_hancock_linear_lz", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 36))
     c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.vanLeer, **base, **h,
              gas_model=gasModel.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
     run_const("seeded_2d_hancock_vdw", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 35))

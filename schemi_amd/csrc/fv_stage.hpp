@@ -1150,9 +1150,7 @@ stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
 #ifndef SG_MIXED_LEAN
 #define SG_MIXED_LEAN 1
 #endif
-#ifndef SG_HALO_AHEAD
-#define SG_HALO_AHEAD 0
-#endif
+
 
 // Stage kernel for schemes whose limited slopes vanish identically (zero
 // limiter; TVD limiters on 2-D and 3-D meshes: the reference's ratio along an
@@ -1195,12 +1193,9 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
     // mixed path, ideal mixture, RK stages: closure straight from the
     // conserved values (close_cell_lean)
     constexpr bool LEAN = MIXED && SG_MIXED_LEAN && GAS == GAS_IDEAL && !H;
-    // halo cells of plane k + 1 are closed during the face tasks of plane k,
-    // by the threads left idle in their last round (HALO_AHEAD)
+    // halo slots of a plane: two columns of TY cells, two rows of TX cells
     constexpr int NHX = 2 * TY;
     constexpr int NH = NHX + ((DIM >= 2) ? 2 * TX : 0);
-    constexpr bool HALO_AHEAD = (DIM == 3) && SG_HALO_AHEAD;
-    __shared__ T sHalo[HALO_AHEAD ? NV : 1][HALO_AHEAD ? NH : 1];
     __shared__ T sFz[FZ_LDS ? NF : 1][FZ_LDS ? TY : 1][FZ_LDS ? TX : 1];
 
     const DevParams &P = *Pp;
@@ -1349,26 +1344,14 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
             for (int q = 0; q < NV; ++q)
                 sZ[q][ty + OY][tx + 1] = zc[q];
         }
-        if (!HALO_AHEAD || k == kb) {
-            for (int h = tid; h < NH; h += NT) {
-                int bxc, byc, gx, gy;
-                if (halo_cell(h, bxc, byc, gx, gy)) {
-                    T z[NV];
-                    closed_at(gx, gy, (DIM == 3) ? k : 0, z);
+        for (int h = tid; h < NH; h += NT) {
+            int bxc, byc, gx, gy;
+            if (halo_cell(h, bxc, byc, gx, gy)) {
+                T z[NV];
+                closed_at(gx, gy, (DIM == 3) ? k : 0, z);
 #pragma unroll
-                    for (int q = 0; q < NV; ++q)
-                        sZ[q][byc][bxc] = z[q];
-                }
-            }
-        } else {
-            // closed during the face tasks of the previous plane
-            for (int h = tid; h < NH; h += NT) {
-                int bxc, byc, gx, gy;
-                if (halo_cell(h, bxc, byc, gx, gy)) {
-#pragma unroll
-                    for (int q = 0; q < NV; ++q)
-                        sZ[q][byc][bxc] = sHalo[q][h];
-                }
+                for (int q = 0; q < NV; ++q)
+                    sZ[q][byc][bxc] = z[q];
             }
         }
         __syncthreads();
@@ -1377,22 +1360,9 @@ stage0_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
         {
             constexpr int NTX = (TX + 1) * TY;
             constexpr int NTY = (DIM >= 2) ? TX * FYH : 0;
-            const int ntask = NTX + NTY + ((HALO_AHEAD && k + 1 < ke) ? NH : 0);
-            for (int t = tid; t < ntask; t += NT) {
+            for (int t = tid; t < NTX + NTY; t += NT) {
                 T g[NF];
                 T zo[NV], zn[NV];
-                if (HALO_AHEAD && t >= NTX + NTY) {
-                    const int h = t - NTX - NTY;
-                    int bxc, byc, gx, gy;
-                    if (halo_cell(h, bxc, byc, gx, gy)) {
-                        T z[NV];
-                        closed_at(gx, gy, k + 1, z);
-#pragma unroll
-                        for (int q = 0; q < NV; ++q)
-                            sHalo[q][h] = z[q];
-                    }
-                    continue;
-                }
                 if (t < NTX) {
                     const int u = t % (TX + 1), r = t / (TX + 1);
                     const int fx = x0 + u;

@@ -201,18 +201,20 @@ int schemi_gpu_sync(schemi_gpu_ctx *ctx);
  * device memory by the caller; synchronous. */
 int schemi_gpu_state_device(schemi_gpu_ctx *ctx, const double *src, double *dst);
 
-/* Adjoint (no counterpart in the reference: the differentiation of the
- * north-star harness).  With the current state U^n and device arrays in the
- * state layout, what = SCHEMI_ADJ_STEP writes
+/* Adjoint (no counterpart in the reference: gradients of functionals of the
+ * solution with respect to the initial state).  With the current state U^n
+ * and device arrays in the state layout, what = SCHEMI_ADJ_STEP writes
  *   lambda_out = (dU^{n+1}/dU^n)^T lambda_in
  * for one SSP-RK2 step of size dt; SCHEMI_ADJ_STAGE the same for one stage
  * (schemi_gpu_stage).  The step is linearised in FP64 at the forward states
  * (U^1 is recomputed in the context's precision), with dt held fixed and the
  * flux's branches (HLLC wave pattern, wave-speed min/max) frozen at the
- * forward switch state.  The current state is left unchanged; lambda_out may
- * equal lambda_in.  Supported where the face values are the cell values
- * (zero limiter; TVD limiters in 2-D and 3-D) with SecondOrderRK; otherwise
- * SCHEMI_ERR_INIT.  Synchronous. */
+ * forward switch state (and the limiter's: min/max of the ratio entries, the
+ * branches of phi).  The current state is left unchanged; lambda_out may
+ * equal lambda_in.  Supported with SecondOrderRK for every limiter on 1-D
+ * meshes and, on 2-D/3-D meshes, where the face values are the cell values
+ * (zero limiter, TVD limiters); ideal, van der Waals and stiffened mixtures.
+ * Otherwise SCHEMI_ERR_INIT.  Synchronous. */
 #define SCHEMI_ADJ_STEP 0
 #define SCHEMI_ADJ_STAGE 1
 int schemi_gpu_adjoint_step(schemi_gpu_ctx *ctx, double dt, const double *lambda_in,

@@ -330,6 +330,8 @@ int adjoint_stage(schemi_gpu_ctx *c, const double *U, const double *lam, double 
     L.dim = c->dim;
     L.ns = c->ns;
     L.gas_model = c->hp.gas_model;
+    L.slopes = (c->mode == MODE_SLOPES) ? 1 : 0;
+    L.lin = c->lin;
     L.n = c->n;
     L.stream = c->stream;
     const int r = launch_adjoint_stage(L);
@@ -858,8 +860,8 @@ int schemi_gpu_adjoint_step(schemi_gpu_ctx *c, double dt, const double *lam_in, 
 {
     if (!c || !lam_in || !lam_out || (what != SCHEMI_ADJ_STEP && what != SCHEMI_ADJ_STAGE))
         return SCHEMI_ERR_INIT;
-    if (c->mode != MODE_FIRST_ORDER || c->hancock ||
-        (c->hp.gas_model != GAS_IDEAL && c->hp.gas_model != GAS_VDW))
+    const bool kernel = c->mode == MODE_FIRST_ORDER || (c->mode == MODE_SLOPES && c->dim == 1);
+    if (!kernel || c->hancock || c->hp.gas_model == GAS_RK)
         return SCHEMI_ERR_INIT;
     if (c->pending_error)
         return c->pending_error;

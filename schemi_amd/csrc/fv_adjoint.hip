@@ -14,7 +14,9 @@
 // numbers seeded along q (branches frozen at the forward state) and
 // accumulates -mu_f . dg_f/dU_j[q]; the cell's result is written once, so the
 // output is deterministic.  The RK2 step's adjoint composes two of them with
-// the weights of U^{n+1} = (L(L(U^n)) + U^n) / 2 (capi.hip).
+// the weights of U^{n+1} = (L(L(U^n)) + U^n) / 2 (capi.hip).  On 1-D meshes
+// with limited slopes adj_stage_1d_kernel does the same over the four faces
+// whose fluxes a cell reaches through the slopes.
 #include "fv_launch.hpp"
 #include "fv_stage.hpp"
 #include "fv_tangent.hpp"
@@ -141,6 +143,133 @@ adj_stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ ge
     }
 }
 
+// Adjoint of a 1-D stage with limited slopes (TVD limiters, linear limiter):
+// the flux of face f depends on the cells f-2 .. f+1 (the slopes of its two
+// cells use their neighbours), so cell j seeds the fluxes of faces j-1 .. j+2.
+// Each is evaluated with tangent numbers through the forward templates
+// (primitives, slopes_axis with its limiter branches, reconstruction, EOS,
+// boundary ghosts, HLLC/HLL) along every conserved component of cell j.
+template <int NS, bool LIN>
+__global__ void __launch_bounds__(256)
+adj_stage_1d_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
+                    const double *__restrict__ U, const double *__restrict__ lam,
+                    double *__restrict__ out, double dt, double a, const double *__restrict__ add,
+                    double b)
+{
+    constexpr int NF = NS + 5;
+    constexpr int NQ = NS + 4;
+    constexpr int NL = SlopeLayout<NS, LIN>::NL;
+    const DevParams &P = *Pp;
+    const long long n = P.n;
+    const long long id = blockIdx.x * 256LL + threadIdx.x;
+    if (id >= n)
+        return;
+    const int N = P.N[0];
+    const bool per = P.periodic[0] != 0;
+    const int j = int(id);
+    const AxisGeo G = axis_geo(P, geo, 0);
+    double lj[NF], res[NF];
+    load_cons<NS>(lam, n, id, lj);
+#pragma unroll
+    for (int q = 0; q < NF; ++q)
+        res[q] = lj[q];
+    if (P.gravity) {
+        double uj[NF];
+        load_cons<NS>(U, n, id, uj);
+        for (int q = 0; q < NF; ++q) {
+            Tan u[NF];
+#pragma unroll
+            for (int r = 0; r < NF; ++r)
+                u[r] = Tan(uj[r], r == q ? 1.0 : 0.0);
+            const Tan rho = u[NS];
+            const Tan vx = u[NS + 1] / rho, vy = u[NS + 2] / rho, vz = u[NS + 3] / rho;
+            const Tan vg = ((Tan(0.0) + vx * P.g[0]) + vy * P.g[1]) + vz * P.g[2];
+            double acc = lj[NS + 4] * (rho * vg * dt).d;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                acc += lj[NS + 1 + c] * ((P.g[c] * rho) * dt).d;
+            res[q] += acc;
+        }
+    }
+    auto wrap = [&](int i) { return per ? ((i % N) + N) % N : i; };
+    auto exists = [&](int i) { return per || (i >= 0 && i < N); };
+    for (int df = -1; df <= 2; ++df) {
+        int f = j + df;  // face between cells f - 1 and f
+        if (per)
+            f = ((f % N) + N) % N;
+        else if (f < 0 || f > N)
+            continue;
+        const bool lo_bnd = !per && f == 0, hi_bnd = !per && f == N;
+        const int co = lo_bnd ? 0 : wrap(f - 1);  // owner
+        const int cn = hi_bnd ? N - 1 : wrap(f);  // neighbour
+        // mu_f: owner +S dt/V lam, neighbour -S dt/V lam (boundary: owner only)
+        double mu[NF];
+        {
+            const CellGeo go = cell_geo(P, geo, co, 0, 0);
+            double lo[NF];
+            load_cons<NS>(lam, n, co, lo);
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                mu[q] = go.S[0] * (dt / go.V) * lo[q];
+            if (!lo_bnd && !hi_bnd) {
+                const CellGeo gn = cell_geo(P, geo, cn, 0, 0);
+                double ln[NF];
+                load_cons<NS>(lam, n, cn, ln);
+#pragma unroll
+                for (int q = 0; q < NF; ++q)
+                    mu[q] -= gn.S[0] * (dt / gn.V) * ln[q];
+            }
+        }
+        // cells f-2 .. f+1 around the face (owner co, neighbour cn)
+        const int cells[4] = {co - 1, co, cn, cn + 1};
+        for (int q = 0; q < NF; ++q) {
+            Tan w[4][NQ];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int ci = wrap(cells[c]);
+                if (!exists(cells[c])) {
+#pragma unroll
+                    for (int r = 0; r < NQ; ++r)
+                        w[c][r] = Tan(0.0);
+                    continue;
+                }
+                double uc[NF];
+                load_cons<NS>(U, n, ci, uc);
+                Tan u[NF];
+#pragma unroll
+                for (int r = 0; r < NF; ++r)
+                    u[r] = Tan(uc[r], (ci == j && r == q) ? 1.0 : 0.0);
+                prims_from_cons<Tan, NS>(P, u, w[c]);
+            }
+            Tan Lo[NL], Ln[NL], g[NF];
+            {
+                const CellGeo go = cell_geo(P, geo, co, 0, 0);
+                slopes_axis<Tan, NS, LIN>(P, G, 0, co, w[0], w[1], w[2], Tan(go.S[0]),
+                                          Tan(go.V), Lo);
+                const CellGeo gn = cell_geo(P, geo, cn, 0, 0);
+                slopes_axis<Tan, NS, LIN>(P, G, 0, cn, w[1], w[2], w[3], Tan(gn.S[0]),
+                                          Tan(gn.V), Ln);
+            }
+            if (lo_bnd)  // owner is cell 0: its data sit in the neighbour slots
+                face_flux<Tan, NS, LIN>(P, G, 0, f, w[2], Ln, w[2], Ln, g);
+            else
+                face_flux<Tan, NS, LIN>(P, G, 0, f, w[1], Lo, w[2], Ln, g);
+            double acc = 0.0;
+#pragma unroll
+            for (int r = 0; r < NF; ++r)
+                acc += mu[r] * g[r].d;
+            res[q] -= acc;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        double o = a * res[q];
+        if (add)
+            o += b * add[q * n + id];
+        out[q * n + id] = o;
+    }
+}
+
 // J = sum_i V_i Y_i (1 - Y_i), Y = rho_sp / rho: per-block partial sums in a
 // fixed tree order, and dJ/dU when grad is given.
 template <int NS>
@@ -193,7 +322,21 @@ template <int NS, int DIM>
 int adj_dim(const AdjointLaunch &L)
 {
     const dim3 g((unsigned)((L.n + 255) / 256));
-    if (L.gas_model == GAS_VDW)
+    if constexpr (DIM == 1) {
+        if (L.slopes) {
+            if (L.lin)
+                hipLaunchKernelGGL((adj_stage_1d_kernel<NS, true>), g, dim3(256), 0, L.stream,
+                                   L.P, L.geo, L.U, L.lam, L.out, L.dt, L.a, L.add, L.b);
+            else
+                hipLaunchKernelGGL((adj_stage_1d_kernel<NS, false>), g, dim3(256), 0, L.stream,
+                                   L.P, L.geo, L.U, L.lam, L.out, L.dt, L.a, L.add, L.b);
+            return static_cast<int>(hipGetLastError());
+        }
+    }
+    if (L.gas_model == GAS_STIFF)
+        hipLaunchKernelGGL((adj_stage_kernel<NS, DIM, GAS_STIFF>), g, dim3(256), 0, L.stream,
+                           L.P, L.geo, L.U, L.lam, L.out, L.dt, L.a, L.add, L.b);
+    else if (L.gas_model == GAS_VDW)
         hipLaunchKernelGGL((adj_stage_kernel<NS, DIM, GAS_VDW>), g, dim3(256), 0, L.stream, L.P,
                            L.geo, L.U, L.lam, L.out, L.dt, L.a, L.add, L.b);
     else

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace sg {
 
 constexpr int MAXS = 9;
@@ -172,7 +174,8 @@ template <int GAS> __device__ __forceinline__ bool is_stiff(const DevParams &P)
     else
         return P.gas_model == GAS_STIFF;
 }
-// Redlich-Kwong code is compiled only where the model can occur.
+// Redlich-Kwong code is compiled only where the model can occur (and only
+// for plain floating-point T; the adjoint's tangent numbers never meet it).
 template <int GAS> constexpr bool may_be_rk() { return GAS < 0 || GAS == GAS_RK; }
 template <int GAS> __device__ __forceinline__ bool is_rk(const DevParams &P)
 {
@@ -313,7 +316,7 @@ __device__ __forceinline__ T eos_p(const DevParams &P, const T *c, T c0, T g1, T
         vdw_ab<T, NS>(P, c, c0, p0, g);
         return (g - T(1)) * U - p0 * g;
     }
-    if constexpr (may_be_rk<GAS>()) {
+    if constexpr (may_be_rk<GAS>() && std::is_floating_point_v<T>) {
         if (is_rk<GAS>(P)) {
             double cd[NS];
             to_f64<T, NS>(c, cd);
@@ -341,7 +344,7 @@ __device__ __forceinline__ T eos_U(const DevParams &P, const T *c, T c0, T g1, T
         vdw_ab<T, NS>(P, c, c0, p0, g);
         return (p + p0 * g) / (g - T(1));
     }
-    if constexpr (may_be_rk<GAS>()) {
+    if constexpr (may_be_rk<GAS>() && std::is_floating_point_v<T>) {
         if (is_rk<GAS>(P)) {
             double cd[NS];
             to_f64<T, NS>(c, cd);
@@ -373,7 +376,7 @@ __device__ __forceinline__ T eos_c2(const DevParams &P, const T *c, T c0, T g1, 
         vdw_ab<T, NS>(P, c, c0, p0, g);
         return T(0) + (g - T(1)) * (U + p) / rho;
     }
-    if constexpr (may_be_rk<GAS>()) {
+    if constexpr (may_be_rk<GAS>() && std::is_floating_point_v<T>) {
         if (is_rk<GAS>(P)) {
             double cd[NS];
             to_f64<T, NS>(c, cd);

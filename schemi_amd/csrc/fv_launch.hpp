@@ -66,6 +66,7 @@ struct AdjointLaunch {
     const double *add;
     double dt, a, b;
     int dim, ns, gas_model;
+    int slopes, lin;  // 1-D stage with limited slopes (TVD or linear limiter)
     long long n;
     hipStream_t stream;
 };

@@ -14,7 +14,10 @@ from tests.cases_small import cases, random_primitives
 torch = pytest.importorskip("torch")
 
 ADJ_CASES = ["1d_zero", "2d_hll_grav", "2d_periodic_hll", "2d_zero_fixed", "2d_vdw_periodic_hll",
-             "3d_vanAlbada", "3d_periodic_x_vanLeer", "3d_hll_partial_tiles"]
+             "3d_vanAlbada", "3d_periodic_x_vanLeer", "3d_hll_partial_tiles", "3d_stiffened_hll",
+             # 1-D with limited slopes: limiter branches frozen as well
+             "1d_minmod", "1d_vanLeer", "1d_vanAlbada", "1d_linear", "1d_hll", "1d_slip",
+             "1d_fixed", "1d_periodic", "1d_vdw_vanLeer", "1d_stiffened_minmod"]
 
 
 def _dev(x):
@@ -74,8 +77,9 @@ def test_adjoint_dot_product(name, what, hip_device):
 
 @pytest.mark.gpu
 def test_adjoint_unsupported(hip_device):
-    """Only the first-order-exact SSP-RK2 configurations have an adjoint."""
-    for name in ("1d_vanLeer", "2d_linear", "hancock_2d_vanLeer"):
+    """No adjoint for the linear limiter in 2-D/3-D, MUSCL-Hancock or the
+    Redlich-Kwong mixture."""
+    for name in ("2d_linear", "3d_linear_dyadic", "hancock_2d_vanLeer", "1d_rk_vanLeer"):
         case = cases()[name]
         s = GpuSolver(case)
         c, v, p = random_primitives(case, 3)

@@ -327,3 +327,29 @@ def test_hancock_runs_alternate_buffers(name, hip_device):
     t2, n2, _ = s.run(10, nxt, t_end)
     assert t_o == t_end and t2 == t_end and n2 == n_o < 10
     assert np.array_equal(s.conserved(), o.conserved())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_MIXED])
+def test_shock_tube_exact_solution(prec, hip_device):
+    """Two-gas shock tube (400 cells) run on the device with the Courant step
+    to t = 0.2: density, pressure and velocity against the exact Riemann
+    solution (L1 as the oracle's, 3.6e-3 / 3.1e-3 / 8.2e-3; the mixed path
+    within 1 % of it)."""
+    from schemi_amd import cases
+    from tests.exact_riemann import sample
+
+    N = 400
+    case, (c, v, p) = cases.shock_tube_1d(N)
+    s = GpuSolver(case, prec)
+    s.set_conserved(conserved_from_primitive(case, c, v, p))
+    t, steps, _ = s.run(100_000, case.min_time, 0.2)
+    assert t == 0.2
+    W = s.primitives()
+    U = s.conserved()
+    x = (np.arange(N) + 0.5) / N
+    rho, u, pe, _ = sample((1.0, 0.0, 1.0, 1.4), (0.125, 0.0, 0.1, 1.67), x, t, 0.5)
+    e = (np.abs(U[case.Ns] - rho).mean(), np.abs(W["p"] - pe).mean(),
+         np.abs(W["v"][0] - u).mean())
+    print(prec, steps, e)
+    assert e[0] < 3.6e-3 * 1.01 and e[1] < 3.1e-3 * 1.01 and e[2] < 8.2e-3 * 1.01

@@ -224,3 +224,28 @@ def test_stiffened_initial_state_matches_oracle():
     for q in range(ref.shape[0]):
         scale = max(np.abs(ref[q]).max(), 1e-300)
         assert np.abs(U[q] - ref[q]).max() <= 1e-15 * scale, q
+
+
+@pytest.mark.parametrize("model", ["ideal", "vdw", "rk", "stiffened"])
+def test_eos_round_trip(model):
+    """p -> U (UvFromp at initialisation) -> p (pFromUv of the conserved
+    state) returns the pressure to a few ulps for every mixture model, and the
+    temperature of the conserved state equals the initial one."""
+    from schemi_amd import Case, gasModel as GM
+    from oracle.oracle import Oracle
+
+    eos = {"ideal": {}, "vdw": dict(gas_model=GM.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0]),
+           "rk": dict(gas_model=GM.RedlichKwong, Tc=[24.0, 3.2], Pc=[1.0, 1.0]),
+           "stiffened": dict(gas_model=GM.stiffened, p0=[0.3, 0.05], gammas=[2.8, 1.6])}[model]
+    case = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), M=[29.0, 4.0], Cv=[2.5, 1 / 0.67], **eos)
+    c, v, p = random_primitives(case, 13)
+    o = Oracle(case)
+    o.init_primitive(c, v, p)
+    st0 = o.state()
+    U = o.conserved()
+    Ns = case.Ns
+    o2 = Oracle(case)
+    o2.set_conserved(U[:Ns], U[Ns], U[Ns + 1:Ns + 4], U[Ns + 4])
+    st = o2.state()
+    assert np.max(np.abs(st["p"] - p) / np.abs(p)) < 1e-13
+    assert np.max(np.abs(st["T"] - st0["T"]) / np.abs(st0["T"])) < 1e-13

@@ -94,6 +94,7 @@ typedef struct {
 #define SCHEMI_GAS_VAN_DER_WAALS 1
 #define SCHEMI_GAS_REDLICH_KWONG 2
 #define SCHEMI_GAS_STIFFENED 3
+#define SCHEMI_GAS_KATAOKA 4
 
 /* Mixture (abstractMixtureThermodynamics::createThermodynamics,
  * Schemi/src/interfaces/abstractMixtureThermodynamics.cpp:32-63): molar
@@ -115,6 +116,17 @@ typedef struct {
      * from set/stiffenedFluidData.txt): p0 and gamma of every species */
     double p0[SCHEMI_MAX_SPECIES];
     double gamma[SCHEMI_MAX_SPECIES];
+    /* Kataoka-van der Waals mixture (mixtureKataokaVanDerWaals.cpp:19-77,
+     * read by the reference from set/KataokaVanDerWaalsFluidData.txt):
+     * Lennard-Jones epsilon and sigma per species, and b from the critical
+     * point (critical = 1: coeff R Tc / (8 Pc)) or from sigma (0: coeff
+     * sigma^3 N_A).  Its temperature is found by the reference's Newton
+     * iteration with the device's pow and exp (results a few ulps from the
+     * reference's, not bit for bit). */
+    double epsLJ[SCHEMI_MAX_SPECIES];
+    double sigmaLJ[SCHEMI_MAX_SPECIES];
+    int32_t kataoka_b_critical;
+    double kataoka_b_coeff;
 } schemi_mix_desc;
 
 /* typeOfSolverEnum (Schemi/enumerations/typeOfSolverEnum.hpp:15):

@@ -33,7 +33,7 @@ _LIM = {0: "zero", 1: "minmod", 2: "vanLeer", 3: "linear", 4: "vanAlbada"}
 # step solver -> #Approximation_order keyword (abstractStepSolver.cpp:56-146)
 _STEP = {0: "SecondOrder", 1: "SecondOrderRK"}
 # gasModel -> #Equation_of_state keyword (abstractMixtureThermodynamics.cpp:38-43)
-_EOS = {0: "ideal", 1: "vanDerWaals", 2: "RedlichKwong", 3: "stiffened"}
+_EOS = {0: "ideal", 1: "vanDerWaals", 2: "RedlichKwong", 3: "stiffened", 4: "Kataoka"}
 
 
 def ref_available() -> bool:
@@ -139,6 +139,12 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         # (abstractMixtureThermodynamics.cpp:72-99)
         files["stiffenedFluidData.txt"] = [
             f"sub_{k + 1} {_r(case.p0[k])} {_r(case.gammas[k])}" for k in range(Ns)]
+    if int(case.gas_model) == 4:
+        # "<name> eps sigma" per component, then "<name> <b type> <coeff>"
+        # (abstractMixtureThermodynamics.cpp:101-147)
+        files["KataokaVanDerWaalsFluidData.txt"] = [
+            f"sub_{k + 1} {_r(case.epsLJ[k])} {_r(case.sigmaLJ[k])}" for k in range(Ns)] + [
+            f"b {case.kataoka_b[0]} {_r(case.kataoka_b[1])}"]
     files["velocity.txt"] = _bc_lines(case, case.bc_v, vec=True) + [
         "#Values_in_zones", " ".join(_r(x) for x in vL),
         " ".join(_r(x) for x in vR)]

@@ -43,6 +43,11 @@ class _Config(C.Structure):
         ("b_kl", C.c_double * (MAXS * MAXS)),
         ("p0", C.c_double * MAXS),
         ("gam", C.c_double * MAXS),
+        ("epsLJ", C.c_double * MAXS),
+        ("sigLJ", C.c_double * MAXS),
+        ("kb_critical", C.c_int),
+        ("kb_coeff", C.c_double),
+        ("e_kl", C.c_double * (MAXS * MAXS)),
     ]
 
 
@@ -130,6 +135,10 @@ def make_config(case) -> _Config:
         cfg.Pc[k] = float(case.Pc[k])
         cfg.p0[k] = float(case.p0[k])
         cfg.gam[k] = float(case.gammas[k])
+        cfg.epsLJ[k] = float(case.epsLJ[k])
+        cfg.sigLJ[k] = float(case.sigmaLJ[k])
+    cfg.kb_critical = 1 if case.kataoka_b[0] == "critical" else 0
+    cfg.kb_coeff = float(case.kataoka_b[1])
     cfg.flux = int(case.flux)
     cfg.limiter = int(case.limiter)
     cfg.gravity = 1 if case.gravity else 0

@@ -673,6 +673,135 @@ static void stiff_matrices(or_config *cfg)
             }
 }
 
+/* Kataoka-van der Waals mixture (mixtureKataokaVanDerWaals.cpp:19-77):
+ * b_kk = coeff R Tc / (8 Pc) ("critical") or coeff sigma^3 N_A ("Kataoka"),
+ * V0_kk = sigma^3 / sqrt(2) N_A, eps_kk = eps; across species b_kl and V0_kl
+ * = (0.5 (cbrt x_kk + cbrt x_ll))^3, eps_kl = sqrt(eps_kk eps_ll). */
+#define NAVOGADRO 6.02214076E23
+static void kataoka_matrices(or_config *cfg)
+{
+    const int n = cfg->Ns;
+    for (int k = 0; k < n; ++k) {
+        const double s3 = cfg->sigLJ[k] * cfg->sigLJ[k] * cfg->sigLJ[k];
+        cfg->b_kl[k * OR_MAXS + k] =
+            cfg->kb_critical ? cfg->kb_coeff * cfg->R * cfg->Tc[k] / (8 * cfg->Pc[k])
+                             : cfg->kb_coeff * s3 * NAVOGADRO;
+        cfg->a_kl[k * OR_MAXS + k] = s3 / sqrt(2.0) * NAVOGADRO;
+        cfg->e_kl[k * OR_MAXS + k] = cfg->epsLJ[k];
+    }
+    for (int k = 0; k < n; ++k)
+        for (int l = 0; l < n; ++l)
+            if (k != l) {
+                cfg->b_kl[k * OR_MAXS + l] = pow(
+                    0.5 * (cbrt(cfg->b_kl[k * OR_MAXS + k]) + cbrt(cfg->b_kl[l * OR_MAXS + l])),
+                    3);
+                cfg->a_kl[k * OR_MAXS + l] = pow(
+                    0.5 * (cbrt(cfg->a_kl[k * OR_MAXS + k]) + cbrt(cfg->a_kl[l * OR_MAXS + l])),
+                    3);
+                cfg->e_kl[k * OR_MAXS + l] =
+                    sqrt(cfg->e_kl[k * OR_MAXS + k] * cfg->e_kl[l * OR_MAXS + l]);
+            }
+}
+
+/* V0, eps, b, Cv and M of the mixture (mixtureKataokaVanDerWaals.cpp:
+ * 641-663, 810-835): X_k X_l weighted, all three in one loop */
+static void kat_mix(const or_config *cfg, const double *c, double *V0, double *eps,
+                    double *b, double *cv, double *mm)
+{
+    double X[OR_MAXS];
+    double cvs = 0, ms = 0;
+    for (int k = 0; k < cfg->Ns; ++k) {
+        X[k] = c[k + 1] / c[0];
+        cvs += X[k] * cfg->Cv[k];
+        ms += X[k] * cfg->M[k];
+    }
+    double v = 0, e = 0, bb = 0;
+    for (int k = 0; k < cfg->Ns; ++k)
+        for (int l = 0; l < cfg->Ns; ++l) {
+            v += X[k] * X[l] * cfg->a_kl[k * OR_MAXS + l];
+            e += X[k] * X[l] * cfg->e_kl[k * OR_MAXS + l];
+            bb += X[k] * X[l] * cfg->b_kl[k * OR_MAXS + l];
+        }
+    *V0 = v;
+    *eps = e;
+    *b = bb;
+    *cv = cvs;
+    *mm = ms;
+}
+
+/* The association series of KataokaVanDerWaalsFluid.cpp:17-62 (M = 13):
+ * qa = 1 + sum_{J=2..13} (V0 c)^((J-1)/3) exp((J-1) eps / (2 R T)),
+ * UvA = 0.5 c eps (-sum (J-1) (...)) / qa, pA = c R T (-sum (J-1)/3 (...)) / qa */
+static double kat_qa(double c, double T, double V0, double eps, double R)
+{
+    const double V0c = V0 * c, epsRT = 0.5 * eps / (R * T);
+    double series = 1.0;
+    for (int J = 2; J <= 13; ++J)
+        series += pow(V0c, (J - 1.) / 3.) * exp((J - 1.) * epsRT);
+    return series;
+}
+static double kat_UvA(double c, double T, double V0, double eps, double R)
+{
+    const double V0c = V0 * c, epsRT = 0.5 * eps / (R * T);
+    double series = 0;
+    for (int J = 2; J <= 13; ++J)
+        series -= (J - 1.) * pow(V0c, (J - 1.) / 3.) * exp((J - 1.) * epsRT);
+    return 0.5 * c * eps * series / kat_qa(c, T, V0, eps, R);
+}
+static double kat_pA(double c, double T, double V0, double eps, double R)
+{
+    const double V0c = V0 * c, epsRT = 0.5 * eps / (R * T);
+    double series = 0;
+    for (int J = 2; J <= 13; ++J)
+        series -= (J - 1.) / 3. * pow(V0c, (J - 1.) / 3.) * exp((J - 1.) * epsRT);
+    return c * R * T * series / kat_qa(c, T, V0, eps, R);
+}
+
+/* The derivative series of dPdC, dPdT, dUvdC and dUvdT run
+ * "for (J = 2; M < J; ++J)" (KataokaVanDerWaalsFluid.cpp:64-219) and are
+ * empty, so these reduce to their leading terms plus exact zeros:
+ * dPdC = R T / (1 - c b) + c R T b / (1 - c b)^2, dPdT = c R / (1 - c b),
+ * dUvdC = Cv T, dUvdT = c Cv (qa, positive and finite for these states,
+ * only divides the zeros). */
+static double kat_dPdC(double c, double T, double R, double b)
+{
+    return R * T / (1 - c * b) + c * R * T * b / ((1 - c * b) * (1 - c * b));
+}
+static double kat_dPdT(double c, double R, double b) { return c * R / (1 - c * b); }
+
+/* NewtonMethod (NewtonMethod.cpp:13-40): x2 = x1 - f(x1) / f'(x1) until
+ * |x2 - x1| <= 1e-12, at most 100 further iterations.  kind 0: T from U
+ * (f = c Cv T + UvA - U, f' = c Cv); kind 1: T from p
+ * (f = c R T / (1 - c b) + pA - p, f' = c R / (1 - c b)). */
+static double kat_newton(int kind, double x0, double c, double cv, double target, double V0,
+                         double eps, double R, double b)
+{
+    const double tol = 1E-12;
+    int it = 1;
+    double x1 = x0;
+#define KAT_F(T)                                                                           \
+    ((kind == 0) ? c * cv * (T) + kat_UvA(c, (T), V0, eps, R) - target                   \
+                 : c * R * (T) / (1 - c * b) + kat_pA(c, (T), V0, eps, R) - target)
+#define KAT_DF(T) ((kind == 0) ? c * cv : kat_dPdT(c, R, b))
+    double x2 = x1 - KAT_F(x1) / KAT_DF(x1);
+    if (fabs(x2 - x1) <= tol)
+        return x2;
+    while (fabs(x2 - x1) > tol && it <= 100) {
+        x1 = x2;
+        x2 = x1 - KAT_F(x1) / KAT_DF(x1);
+        it++;
+    }
+#undef KAT_F
+#undef KAT_DF
+    return x2;
+}
+
+/* T from U (KataokaVanDerWaalsFluid.cpp:277-294): Newton from U / ((c + stab) Cv) */
+static double kat_T(double c, double cv, double U, double V0, double eps, double R, double b)
+{
+    return kat_newton(0, U / ((c + STAB) * cv), c, cv, U, V0, eps, R, b);
+}
+
 /* signum with the reference's dead band (sign.cpp:12-24) */
 static int rk_sign(double x)
 {
@@ -817,6 +946,13 @@ static double rk_dUvdT(double c, double Cv, double T, double a, double b)
  * vdW g1 (U + a c^2) / (1 - c b) - a c^2 (vanDerWaalsFluid.cpp:14-22) */
 static double eos_p(const or_config *cfg, const double *c, double U)
 {
+    if (cfg->gas_model == OR_GAS_KATAOKA) {
+        /* KataokaVanDerWaalsFluid.cpp:221-240 */
+        double V0, eps, b, cv, mm;
+        kat_mix(cfg, c, &V0, &eps, &b, &cv, &mm);
+        const double T = kat_T(c[0], cv, U, V0, eps, cfg->R, b);
+        return c[0] * cfg->R * T / (1 - c[0] * b) + kat_pA(c[0], T, V0, eps, cfg->R);
+    }
     if (cfg->gas_model == OR_GAS_STIFFENED) {
         /* stiffenedFluid.cpp:13-17, mixtureStiffened.cpp:485-503 */
         double p0, g;
@@ -843,6 +979,14 @@ static double eos_p(const or_config *cfg, const double *c, double U)
  * vdW (p + a c^2) (1 - c b) / g1 - a c^2 (vanDerWaalsFluid.cpp:24-31) */
 static double eos_U(const or_config *cfg, const double *c, double p)
 {
+    if (cfg->gas_model == OR_GAS_KATAOKA) {
+        /* KataokaVanDerWaalsFluid.cpp:242-261: Newton from p / ((c + stab) R) */
+        double V0, eps, b, cv, mm;
+        kat_mix(cfg, c, &V0, &eps, &b, &cv, &mm);
+        const double T =
+            kat_newton(1, p / ((c[0] + STAB) * cfg->R), c[0], cv, p, V0, eps, cfg->R, b);
+        return c[0] * cv * T + kat_UvA(c[0], T, V0, eps, cfg->R);
+    }
     if (cfg->gas_model == OR_GAS_STIFFENED) {
         /* stiffenedFluid.cpp:19-23, mixtureStiffened.cpp:505-523 */
         double p0, g;
@@ -872,6 +1016,11 @@ static double eos_U(const or_config *cfg, const double *c, double p)
  * (vanDerWaalsFluid.cpp:45-49) */
 static double eos_T(const or_config *cfg, const double *c, double U)
 {
+    if (cfg->gas_model == OR_GAS_KATAOKA) {
+        double V0, eps, b, cv, mm;
+        kat_mix(cfg, c, &V0, &eps, &b, &cv, &mm);
+        return kat_T(c[0], cv, U, V0, eps, cfg->R, b);
+    }
     if (cfg->gas_model == OR_GAS_STIFFENED) {
         /* stiffenedFluid.cpp:37-41, mixtureStiffened.cpp:578-596: Cv_mix
          * first, then p0_mix */
@@ -928,6 +1077,19 @@ static double eos_dpdUv(const or_config *cfg, const double *c)
 static double eos_c2(const or_config *cfg, const double *c, double rho, double U,
                      double p)
 {
+    if (cfg->gas_model == OR_GAS_KATAOKA) {
+        /* KataokaVanDerWaalsFluid.cpp:311-353, mixtureKataokaVanDerWaals.cpp:
+         * 810-886: dpdrho = (dPdC - dPdT dUvdC / dUvdT) / M_mix,
+         * dpdUv = dPdT / dUvdT at T(U) */
+        double V0, eps, b, cv, mm;
+        kat_mix(cfg, c, &V0, &eps, &b, &cv, &mm);
+        const double c0 = c[0], R = cfg->R;
+        const double T = kat_T(c0, cv, U, V0, eps, R, b);
+        const double dpdrho =
+            (kat_dPdC(c0, T, R, b) - kat_dPdT(c0, R, b) * (cv * T) / (c0 * cv)) / mm;
+        const double dpdUv = kat_dPdT(c0, R, b) / (c0 * cv);
+        return dpdrho + dpdUv * (U + p) / rho;
+    }
     if (cfg->gas_model == OR_GAS_STIFFENED) {
         /* mixtureStiffened.cpp:657-663: dpdrho = 0, dpdUv = gamma - 1 */
         const double dpdrho = 0;
@@ -1060,8 +1222,7 @@ or_solver *or_create(const or_config *cfg)
 {
     if (cfg->dim < 1 || cfg->dim > 3 || cfg->Ns < 1 || cfg->Ns > OR_MAXS)
         return NULL;
-    if (cfg->gas_model != OR_GAS_IDEAL && cfg->gas_model != OR_GAS_VDW &&
-        cfg->gas_model != OR_GAS_RK && cfg->gas_model != OR_GAS_STIFFENED)
+    if (cfg->gas_model < OR_GAS_IDEAL || cfg->gas_model > OR_GAS_KATAOKA)
         return NULL;
     or_solver *s = calloc(1, sizeof(or_solver));
     s->cfg = *cfg;
@@ -1071,6 +1232,8 @@ or_solver *or_create(const or_config *cfg)
         rk_matrices(&s->cfg);
     else if (cfg->gas_model == OR_GAS_STIFFENED)
         stiff_matrices(&s->cfg);
+    else if (cfg->gas_model == OR_GAS_KATAOKA)
+        kataoka_matrices(&s->cfg);
     for (int a = 0; a < cfg->dim; ++a)
         if (axis_periodic(s, a) < 0) {
             free(s);

@@ -24,7 +24,7 @@ enum { OR_BC_BLANK = 0, OR_BC_FREE = 1, OR_BC_SLIP = 2, OR_BC_FIXED_CELL = 3,
        OR_BC_PERIODIC = 32 /* no reference counterpart */ };
 enum { OR_FLUX_HLL = 0, OR_FLUX_HLLC = 2 };
 /* gasModel (gasModelEnum.hpp:15-18) */
-enum { OR_GAS_IDEAL = 0, OR_GAS_VDW = 1, OR_GAS_RK = 2, OR_GAS_STIFFENED = 3 };
+enum { OR_GAS_IDEAL = 0, OR_GAS_VDW = 1, OR_GAS_RK = 2, OR_GAS_STIFFENED = 3, OR_GAS_KATAOKA = 4 };
 enum { OR_LIM_ZERO = 0, OR_LIM_MINMOD = 1, OR_LIM_VANLEER = 2, OR_LIM_LINEAR = 3,
        OR_LIM_VANALBADA = 4 };
 /* errors enum: noErrors=0, RiemannSolverError=1, boundaryConditionError=5,
@@ -66,6 +66,15 @@ typedef struct {
     /* stiffened fluid (OR_GAS_STIFFENED): p0 and gamma of every species */
     double p0[OR_MAXS];
     double gam[OR_MAXS];
+    /* Kataoka-van der Waals (OR_GAS_KATAOKA): Lennard-Jones epsilon and
+     * sigma of every species, b from the critical point (kb_critical = 1:
+     * kb_coeff R Tc / (8 Pc)) or from sigma (0: kb_coeff sigma^3 N_A);
+     * matrices V0_kl in a_kl, b_kl in b_kl, eps_kl in e_kl */
+    double epsLJ[OR_MAXS];
+    double sigLJ[OR_MAXS];
+    int kb_critical;
+    double kb_coeff;
+    double e_kl[OR_MAXS * OR_MAXS];
 } or_config;
 
 typedef struct or_solver or_solver;

@@ -60,6 +60,12 @@ class Case:
     # stiffened fluid (./set/stiffenedFluidData.txt): p0 and gamma per species
     p0: Sequence[float] = ()
     gammas: Sequence[float] = ()
+    # Kataoka-van der Waals (./set/KataokaVanDerWaalsFluidData.txt):
+    # Lennard-Jones epsilon and sigma per species, and how b is formed
+    # ("critical": coeff R Tc / (8 Pc); "Kataoka": coeff sigma^3 N_A)
+    epsLJ: Sequence[float] = ()
+    sigmaLJ: Sequence[float] = ()
+    kataoka_b: tuple = ("critical", 1.0)
     # step solver (#Approximation_order): 1 SecondOrderRK (SSP-RK2),
     # 0 SecondOrder (MUSCL-Hancock)
     step_solver: int = 1
@@ -75,6 +81,10 @@ class Case:
             self.p0 = [0.0] * len(self.M)
         if not self.gammas:
             self.gammas = [1.4] * len(self.M)
+        if not self.epsLJ:
+            self.epsLJ = [1.0] * len(self.M)
+        if not self.sigmaLJ:
+            self.sigmaLJ = [1e-8] * len(self.M)
         n = list(self.N)
         if self.dim < 3:
             n[2] = 1

@@ -126,9 +126,11 @@ __global__ void prims_kernel(const DevParams *__restrict__ Pp, const double *__r
         vdw_ab<double, NS>(P, w, c0, p0, g);
         Ut = Uint - p0;
     }
-    // Redlich-Kwong: T = s^2 from its cubic (RedlichKwongFluid.cpp:89-97)
-    W[(NS + 6) * n + id] =
-        (P.gas_model == GAS_RK) ? rk_T<NS>(P, w, c0, Uint) : Ut / ((c0 + STAB) * cv);
+    // Redlich-Kwong: T = s^2 from its cubic (RedlichKwongFluid.cpp:89-97);
+    // Kataoka-van der Waals: Newton (KataokaVanDerWaalsFluid.cpp:277-294)
+    W[(NS + 6) * n + id] = (P.gas_model == GAS_RK)        ? rk_T<NS>(P, w, c0, Uint)
+                           : (P.gas_model == GAS_KATAOKA) ? kat_T<NS>(P, w, c0, Uint)
+                                                          : Ut / ((c0 + STAB) * cv);
     const double c2 = eos_c2<double, NS>(P, w, c0, gamma1<double, NS>(P, w, c0), u[NS], Uint, p);
     W[(NS + 7) * n + id] = __builtin_sqrt(c2);
 }
@@ -380,8 +382,13 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
             return SCHEMI_ERR_MESH;
     if (mix->ns < 1 || mix->ns > 3 ||
         (mix->gas_model != SCHEMI_GAS_IDEAL && mix->gas_model != SCHEMI_GAS_VAN_DER_WAALS &&
-         mix->gas_model != SCHEMI_GAS_REDLICH_KWONG && mix->gas_model != SCHEMI_GAS_STIFFENED))
+         mix->gas_model != SCHEMI_GAS_REDLICH_KWONG && mix->gas_model != SCHEMI_GAS_STIFFENED &&
+         mix->gas_model != SCHEMI_GAS_KATAOKA))
         return SCHEMI_ERR_INIT;
+    if (mix->gas_model == SCHEMI_GAS_KATAOKA)
+        for (int k = 0; k < mix->ns; ++k)
+            if (!(mix->sigmaLJ[k] > 0.0) || !(mix->Tcrit[k] > 0.0) || !(mix->Pcrit[k] > 0.0))
+                return SCHEMI_ERR_INIT;
     if (mix->gas_model == SCHEMI_GAS_STIFFENED)
         for (int k = 0; k < mix->ns; ++k)
             if (!(mix->gamma[k] > 1.0) || !(mix->p0[k] >= 0.0))
@@ -468,6 +475,29 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
                     P.bkl[k][l] = std::pow(0.5 * (std::cbrt(P.bkl[k][k]) + std::cbrt(P.bkl[l][l])),
                                            3.0);
                     P.akl[k][l] = std::sqrt(P.akl[k][k] * P.akl[l][l]);
+                }
+    }
+    if (mix->gas_model == SCHEMI_GAS_KATAOKA) {
+        // mixtureKataokaVanDerWaals.cpp:19-77: b_kk from the critical point or
+        // sigma, V0_kk = sigma^3 / sqrt(2) N_A, eps_kk = eps; across species
+        // (0.5 (cbrt x_kk + cbrt x_ll))^3 for b and V0, sqrt(eps_kk eps_ll)
+        const double NA = 6.02214076E23;
+        for (int k = 0; k < mix->ns; ++k) {
+            const double s3 = mix->sigmaLJ[k] * mix->sigmaLJ[k] * mix->sigmaLJ[k];
+            P.bkl[k][k] = mix->kataoka_b_critical
+                              ? mix->kataoka_b_coeff * mix->R * mix->Tcrit[k] / (8 * mix->Pcrit[k])
+                              : mix->kataoka_b_coeff * s3 * NA;
+            P.akl[k][k] = s3 / std::sqrt(2.0) * NA;
+            P.ekl[k][k] = mix->epsLJ[k];
+        }
+        for (int k = 0; k < mix->ns; ++k)
+            for (int l = 0; l < mix->ns; ++l)
+                if (k != l) {
+                    P.bkl[k][l] = std::pow(0.5 * (std::cbrt(P.bkl[k][k]) + std::cbrt(P.bkl[l][l])),
+                                           3.0);
+                    P.akl[k][l] = std::pow(0.5 * (std::cbrt(P.akl[k][k]) + std::cbrt(P.akl[l][l])),
+                                           3.0);
+                    P.ekl[k][l] = std::sqrt(P.ekl[k][k] * P.ekl[l][l]);
                 }
     }
     if (mix->gas_model == SCHEMI_GAS_STIFFENED) {
@@ -861,7 +891,7 @@ int schemi_gpu_adjoint_step(schemi_gpu_ctx *c, double dt, const double *lam_in, 
     if (!c || !lam_in || !lam_out || (what != SCHEMI_ADJ_STEP && what != SCHEMI_ADJ_STAGE))
         return SCHEMI_ERR_INIT;
     const bool kernel = c->mode == MODE_FIRST_ORDER || (c->mode == MODE_SLOPES && c->dim == 1);
-    if (!kernel || c->hancock || c->hp.gas_model == GAS_RK)
+    if (!kernel || c->hancock || c->hp.gas_model == GAS_RK || c->hp.gas_model == GAS_KATAOKA)
         return SCHEMI_ERR_INIT;
     if (c->pending_error)
         return c->pending_error;

@@ -28,8 +28,8 @@ enum : int { LIM_ZERO = 0, LIM_MINMOD = 1, LIM_VANLEER = 2, LIM_LINEAR = 3,
              LIM_VANALBADA = 4 };
 enum : int { FLUX_HLL = 0, FLUX_HLLC = 2 };
 // gasModel (gasModelEnum.hpp:15-18): ideal, vanDerWaals, RedlichKwong,
-// stiffened
-enum : int { GAS_IDEAL = 0, GAS_VDW = 1, GAS_RK = 2, GAS_STIFF = 3 };
+// stiffened, KataokaVanDerWaals
+enum : int { GAS_IDEAL = 0, GAS_VDW = 1, GAS_RK = 2, GAS_STIFF = 3, GAS_KATAOKA = 4 };
 // stage variants: the two SSP-RK2 stages and the single MUSCL-Hancock stage
 enum : int { SV_RK1 = 0, SV_RK2 = 1, SV_HANCOCK = 2 };
 // Scalar block of a context (device memory, one double each): time step,
@@ -60,6 +60,7 @@ struct DevParams {
     // equation of state; van der Waals mixing matrices a_kl, b_kl
     int gas_model;
     double akl[MAXS][MAXS], bkl[MAXS][MAXS];
+    double ekl[MAXS][MAXS];  // Kataoka: eps_kl (V0_kl in akl, b_kl in bkl)
     float aklf[MAXS][MAXS], bklf[MAXS][MAXS];
     int gravity;
     double g[3];
@@ -177,6 +178,14 @@ template <int GAS> __device__ __forceinline__ bool is_stiff(const DevParams &P)
 // Redlich-Kwong code is compiled only where the model can occur (and only
 // for plain floating-point T; the adjoint's tangent numbers never meet it).
 template <int GAS> constexpr bool may_be_rk() { return GAS < 0 || GAS == GAS_RK; }
+template <int GAS> constexpr bool may_be_kataoka() { return GAS < 0 || GAS == GAS_KATAOKA; }
+template <int GAS> __device__ __forceinline__ bool is_kataoka(const DevParams &P)
+{
+    if constexpr (GAS >= 0)
+        return GAS == GAS_KATAOKA;
+    else
+        return P.gas_model == GAS_KATAOKA;
+}
 template <int GAS> __device__ __forceinline__ bool is_rk(const DevParams &P)
 {
     if constexpr (GAS >= 0)
@@ -295,6 +304,121 @@ __device__ inline double rk_c2(const DevParams &P, const double *c, double c0, d
     return dpdrho + dPdT / dUdT * (U + p) / rho;
 }
 
+// ---- Kataoka-van der Waals mixture, FP64 on both paths -------------------
+// Association series (KataokaVanDerWaalsFluid.cpp:17-62, M = 13 terms) and
+// the reference's Newton iteration for T (NewtonMethod.cpp:13-40: stop when
+// a step is <= 1e-12, at most 101 steps); the derivative series of
+// dPdC / dPdT / dUvdC / dUvdT never run in the reference ("J = 2; M < J"),
+// leaving dPdC = R T / (1 - c b) + c R T b / (1 - c b)^2, dPdT = c R / (1 - c b),
+// dUvdC = Cv T, dUvdT = c Cv.  pow and exp are the device's, a few ulps from
+// the host library's.
+template <int NS>
+__device__ __forceinline__ void kat_mix(const DevParams &P, const double *c, double c0, double &V0,
+                                        double &eps, double &b, double &cv, double &mm)
+{
+    double X[NS];
+    cv = 0.0;
+    mm = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        X[k] = c[k] / c0;
+        cv += X[k] * P.Cv[k];
+        mm += X[k] * P.M[k];
+    }
+    V0 = 0.0;
+    eps = 0.0;
+    b = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+            V0 += X[k] * X[l] * P.akl[k][l];
+            eps += X[k] * X[l] * P.ekl[k][l];
+            b += X[k] * X[l] * P.bkl[k][l];
+        }
+}
+
+// qa, -sum (J-1) x_J and -sum (J-1)/3 x_J with x_J = (V0 c)^((J-1)/3) e^((J-1) eps/(2RT))
+__device__ inline void kat_series(double c, double T, double V0, double eps, double R, double &qa,
+                                  double &su, double &sp)
+{
+    const double V0c = V0 * c, epsRT = 0.5 * eps / (R * T);
+    qa = 1.0;
+    su = 0.0;
+    sp = 0.0;
+    for (int J = 2; J <= 13; ++J) {
+        const double x = ::pow(V0c, (J - 1.) / 3.) * ::exp((J - 1.) * epsRT);
+        qa += x;
+        su -= (J - 1.) * x;
+        sp -= (J - 1.) / 3. * x;
+    }
+}
+__device__ __forceinline__ double kat_UvA(double c, double T, double V0, double eps, double R)
+{
+    double qa, su, sp;
+    kat_series(c, T, V0, eps, R, qa, su, sp);
+    return 0.5 * c * eps * su / qa;
+}
+__device__ __forceinline__ double kat_pA(double c, double T, double V0, double eps, double R)
+{
+    double qa, su, sp;
+    kat_series(c, T, V0, eps, R, qa, su, sp);
+    return c * R * T * sp / qa;
+}
+// kind 0: T with c Cv T + UvA(T) = target; kind 1: c R T / (1 - c b) + pA(T) = target
+__device__ inline double kat_newton(int kind, double x0, double c, double cv, double target,
+                                    double V0, double eps, double R, double b)
+{
+    auto f = [&](double T) {
+        return (kind == 0) ? c * cv * T + kat_UvA(c, T, V0, eps, R) - target
+                           : c * R * T / (1 - c * b) + kat_pA(c, T, V0, eps, R) - target;
+    };
+    const double df = (kind == 0) ? c * cv : c * R / (1 - c * b);
+    double x1 = x0;
+    double x2 = x1 - f(x1) / df;
+    for (int it = 1; __builtin_fabs(x2 - x1) > 1e-12 && it <= 100; ++it) {
+        x1 = x2;
+        x2 = x1 - f(x1) / df;
+    }
+    return x2;
+}
+template <int NS>
+__device__ inline double kat_T(const DevParams &P, const double *c, double c0, double U)
+{
+    double V0, eps, b, cv, mm;
+    kat_mix<NS>(P, c, c0, V0, eps, b, cv, mm);
+    return kat_newton(0, U / ((c0 + STAB) * cv), c0, cv, U, V0, eps, P.R, b);
+}
+template <int NS>
+__device__ inline double kat_p(const DevParams &P, const double *c, double c0, double U)
+{
+    double V0, eps, b, cv, mm;
+    kat_mix<NS>(P, c, c0, V0, eps, b, cv, mm);
+    const double T = kat_newton(0, U / ((c0 + STAB) * cv), c0, cv, U, V0, eps, P.R, b);
+    return c0 * P.R * T / (1 - c0 * b) + kat_pA(c0, T, V0, eps, P.R);
+}
+template <int NS>
+__device__ inline double kat_U(const DevParams &P, const double *c, double c0, double p)
+{
+    double V0, eps, b, cv, mm;
+    kat_mix<NS>(P, c, c0, V0, eps, b, cv, mm);
+    const double T = kat_newton(1, p / ((c0 + STAB) * P.R), c0, cv, p, V0, eps, P.R, b);
+    return c0 * cv * T + kat_UvA(c0, T, V0, eps, P.R);
+}
+template <int NS>
+__device__ inline double kat_c2(const DevParams &P, const double *c, double c0, double rho,
+                                double U, double p)
+{
+    double V0, eps, b, cv, mm;
+    kat_mix<NS>(P, c, c0, V0, eps, b, cv, mm);
+    const double R = P.R;
+    const double T = kat_newton(0, U / ((c0 + STAB) * cv), c0, cv, U, V0, eps, R, b);
+    const double dPdC = R * T / (1 - c0 * b) + c0 * R * T * b / ((1 - c0 * b) * (1 - c0 * b));
+    const double dPdT = c0 * R / (1 - c0 * b);
+    const double dpdrho = (dPdC - dPdT * (cv * T) / (c0 * cv)) / mm;
+    return dpdrho + dPdT / (c0 * cv) * (U + p) / rho;
+}
+
 template <class T, int NS>
 __device__ __forceinline__ void to_f64(const T *c, double *cd)
 {
@@ -323,6 +447,13 @@ __device__ __forceinline__ T eos_p(const DevParams &P, const T *c, T c0, T g1, T
             return T(rk_p<NS>(P, cd, double(c0), double(U)));
         }
     }
+    if constexpr (may_be_kataoka<GAS>() && std::is_floating_point_v<T>) {
+        if (is_kataoka<GAS>(P)) {
+            double cd[NS];
+            to_f64<T, NS>(c, cd);
+            return T(kat_p<NS>(P, cd, double(c0), double(U)));
+        }
+    }
     if (is_vdw<GAS>(P)) {
         T a, b;
         vdw_ab<T, NS>(P, c, c0, a, b);
@@ -349,6 +480,13 @@ __device__ __forceinline__ T eos_U(const DevParams &P, const T *c, T c0, T g1, T
             double cd[NS];
             to_f64<T, NS>(c, cd);
             return T(rk_U<NS>(P, cd, double(c0), double(p)));
+        }
+    }
+    if constexpr (may_be_kataoka<GAS>() && std::is_floating_point_v<T>) {
+        if (is_kataoka<GAS>(P)) {
+            double cd[NS];
+            to_f64<T, NS>(c, cd);
+            return T(kat_U<NS>(P, cd, double(c0), double(p)));
         }
     }
     if (is_vdw<GAS>(P)) {
@@ -381,6 +519,13 @@ __device__ __forceinline__ T eos_c2(const DevParams &P, const T *c, T c0, T g1, 
             double cd[NS];
             to_f64<T, NS>(c, cd);
             return T(rk_c2<NS>(P, cd, double(c0), double(rho), double(U), double(p)));
+        }
+    }
+    if constexpr (may_be_kataoka<GAS>() && std::is_floating_point_v<T>) {
+        if (is_kataoka<GAS>(P)) {
+            double cd[NS];
+            to_f64<T, NS>(c, cd);
+            return T(kat_c2<NS>(P, cd, double(c0), double(rho), double(U), double(p)));
         }
     }
     if (is_vdw<GAS>(P)) {

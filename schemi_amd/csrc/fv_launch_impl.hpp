@@ -34,6 +34,10 @@ static int launch_mode(const StageLaunch &L)
             hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, SV, MIXED, TX0, TY0, GAS_STIFF>),
                                (tile_grid<DIM, true>(L)), dim3(TX0 * TY0), 0, L.stream, L.P,
                                L.geo, L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
+        else if (L.gas_model == GAS_KATAOKA)
+            hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, SV, MIXED, TX0, TY0, GAS_KATAOKA>),
+                               (tile_grid<DIM, true>(L)), dim3(TX0 * TY0), 0, L.stream, L.P,
+                               L.geo, L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
         else
             hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, SV, MIXED, TX0, TY0, GAS_IDEAL>),
                                (tile_grid<DIM, true>(L)), dim3(TX0 * TY0), 0, L.stream, L.P,

@@ -52,7 +52,9 @@ class MixDesc(C.Structure):
     _fields_ = [("ns", C.c_int32), ("M", C.c_double * MAXS), ("Cv", C.c_double * MAXS),
                 ("R", C.c_double), ("gas_model", C.c_int32), ("Tcrit", C.c_double * MAXS),
                 ("Pcrit", C.c_double * MAXS), ("p0", C.c_double * MAXS),
-                ("gamma", C.c_double * MAXS)]
+                ("gamma", C.c_double * MAXS), ("epsLJ", C.c_double * MAXS),
+                ("sigmaLJ", C.c_double * MAXS), ("kataoka_b_critical", C.c_int32),
+                ("kataoka_b_coeff", C.c_double)]
 
 
 class SchemeDesc(C.Structure):
@@ -157,6 +159,10 @@ def mix_desc(case: Case) -> MixDesc:
         x.Pcrit[k] = float(case.Pc[k])
         x.p0[k] = float(case.p0[k])
         x.gamma[k] = float(case.gammas[k])
+        x.epsLJ[k] = float(case.epsLJ[k])
+        x.sigmaLJ[k] = float(case.sigmaLJ[k])
+    x.kataoka_b_critical = 1 if case.kataoka_b[0] == "critical" else 0
+    x.kataoka_b_coeff = float(case.kataoka_b[1])
     return x
 
 
@@ -388,6 +394,71 @@ def _rk_root(A, C, D):
     return s
 
 
+def kataoka_matrices(case: Case):
+    """V0_kl, eps_kl, b_kl of the Kataoka-van der Waals mixture
+    (mixtureKataokaVanDerWaals.cpp:19-77)."""
+    L = _libm()
+    NA = 6.02214076E23
+    n = case.Ns
+    V0, eps, b = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
+    crit, coeff = case.kataoka_b[0] == "critical", float(case.kataoka_b[1])
+    for k in range(n):
+        s3 = float(case.sigmaLJ[k]) * float(case.sigmaLJ[k]) * float(case.sigmaLJ[k])
+        b[k, k] = coeff * float(case.R) * float(case.Tc[k]) / (8 * float(case.Pc[k])) if crit \
+            else coeff * s3 * NA
+        V0[k, k] = s3 / L.sqrt(2.0) * NA
+        eps[k, k] = float(case.epsLJ[k])
+    for k in range(n):
+        for l in range(n):
+            if k != l:
+                b[k, l] = L.pow(0.5 * (L.cbrt(b[k, k]) + L.cbrt(b[l, l])), 3.0)
+                V0[k, l] = L.pow(0.5 * (L.cbrt(V0[k, k]) + L.cbrt(V0[l, l])), 3.0)
+                eps[k, l] = L.sqrt(eps[k, k] * eps[l, l])
+    return V0, eps, b
+
+
+def _kataoka_U(case: Case, c, c0, p):
+    """UvFromp of the Kataoka-van der Waals mixture (KataokaVanDerWaalsFluid.cpp:
+    17-62, 242-261): T from c R T / (1 - c b) + pA(T) = p by the reference's
+    Newton iteration, then U = c Cv T + UvA(T)."""
+    V0m, epsm, bm = kataoka_matrices(case)
+    Ns = case.Ns
+    X = [c[k] / c0 for k in range(Ns)]
+    cv = sum(X[k] * case.Cv[k] for k in range(Ns))
+    V0 = np.zeros_like(c0)
+    eps = np.zeros_like(c0)
+    b = np.zeros_like(c0)
+    for k in range(Ns):
+        for l in range(Ns):
+            V0 = V0 + X[k] * X[l] * V0m[k, l]
+            eps = eps + X[k] * X[l] * epsm[k, l]
+            b = b + X[k] * X[l] * bm[k, l]
+    R = float(case.R)
+
+    def series(T):
+        V0c, epsRT = V0 * c0, 0.5 * eps / (R * T)
+        qa, su, sp = np.ones_like(T), np.zeros_like(T), np.zeros_like(T)
+        for J in range(2, 14):
+            x = np.power(V0c, (J - 1.0) / 3.0) * np.exp((J - 1.0) * epsRT)
+            qa, su, sp = qa + x, su - (J - 1.0) * x, sp - (J - 1.0) / 3.0 * x
+        return qa, su, sp
+
+    def f(T):
+        qa, su, sp = series(T)
+        return c0 * R * T / (1 - c0 * b) + c0 * R * T * sp / qa - p
+
+    df = c0 * R / (1 - c0 * b)
+    x1 = p / ((c0 + 5 * np.finfo(float).eps) * R)
+    x2 = x1 - f(x1) / df
+    for _ in range(100):
+        if np.all(np.abs(x2 - x1) <= 1e-12):
+            break
+        x1 = x2
+        x2 = x1 - f(x1) / df
+    qa, su, sp = series(x2)
+    return c0 * cv * x2 + 0.5 * c0 * eps * su / qa
+
+
 def vdw_matrices(case: Case):
     """van der Waals mixing matrices a_kl, b_kl (mixtureVanDerWaals.cpp:19-58),
     with glibc's cbrt and pow as the reference's build uses them."""
@@ -460,6 +531,8 @@ def conserved_from_primitive(case: Case, c, v, p) -> np.ndarray:
                 p0m = p0m + X[k] * X[l] * p0kl
                 gm = gm + X[k] * X[l] * gkl
         U = (p + p0m * gm) / (gm - 1.0)
+    elif int(case.gas_model) == 4:
+        U = _kataoka_U(case, c, c0, p)
     elif int(case.gas_model) == 2:
         am, bm = rk_matrices(case)
         X = [c[k] / c0 for k in range(Ns)]

@@ -125,12 +125,23 @@ def cases():
              **BASE, **st)
     c.set_bc(3, BCT.slip)
     out["3d_stiffened_hll"] = c
+    # Kataoka-van der Waals mixture (Newton for T): compared with a tolerance
+    kat = dict(gas_model=GM.KataokaVanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0],
+               epsLJ=[10.0, 5.0], sigmaLJ=[1.52e-8, 1.2e-8], kataoka_b=("critical", 1.0))
+    out["1d_kataoka_vanLeer"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0),
+                                     limiter=TL.vanLeer, **BASE, **kat)
+    c = Case(dim=2, N=(64, 32, 1), L=(1.0, 0.5, 0.1), limiter=TL.minmod, flux=FS.HLL, **BASE,
+             **dict(kat, kataoka_b=("Kataoka", 2.0)))
+    periodic(c, 0)
+    out["2d_kataoka_periodic_hll"] = c
+    out["3d_kataoka_vanAlbada"] = Case(dim=3, N=(24, 10, 12), L=(1.0, 0.5, 0.5),
+                                       limiter=TL.vanAlbada, **BASE, **kat)
     # MUSCL-Hancock (SecondOrder) on every mesh kind it runs on: 1-D with each
     # limiter and boundary kind, and 2-D/3-D where the slopes vanish
     # (including the reversed top face normal, 3d_periodic_x_vanLeer)
     for name in ("1d_minmod", "1d_vanLeer", "1d_vanAlbada", "1d_linear", "1d_zero", "1d_hll",
                  "1d_slip", "1d_fixed", "1d_periodic", "1d_vdw_vanLeer", "1d_rk_vanLeer",
-                 "2d_vanLeer",
+                 "1d_kataoka_vanLeer", "2d_vanLeer",
                  "2d_hll_grav", "2d_periodic_hll", "2d_zero_fixed", "2d_vdw_periodic_hll",
                  "3d_vanAlbada", "3d_hll_partial_tiles", "3d_periodic_x_vanLeer",
                  "3d_vdw_vanAlbada"):

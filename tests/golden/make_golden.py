@@ -38,6 +38,9 @@ def case_json(case: Case) -> str:
         "gas_model": int(case.gas_model), "Tc": list(case.Tc), "Pc": list(case.Pc)}
     if int(case.gas_model) == 3:
         eos.update(p0=list(case.p0), gammas=list(case.gammas))
+    if int(case.gas_model) == 4:
+        eos.update(epsLJ=list(case.epsLJ), sigmaLJ=list(case.sigmaLJ),
+                   kataoka_b=list(case.kataoka_b))
     if int(case.step_solver) != 1:
         eos["step_solver"] = int(case.step_solver)
     return json.dumps({**eos,
@@ -242,6 +245,23 @@ def stiffened_cases(base):
     run_const("seeded_2d_stiffened_hll", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 52))
 
 
+KATAOKA = dict(gas_model=gasModel.KataokaVanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0],
+               epsLJ=[10.0, 5.0], sigmaLJ=[1.52e-8, 1.2e-8], kataoka_b=("critical", 1.0))
+
+
+def kataoka_cases(base):
+    """Kataoka-van der Waals mixture (gasModel::KataokaVanDerWaals): V0 c
+    about 0.05 and eps / (2 R T) about 0.15 for the seeded states, so the
+    association terms change p by about a fifth."""
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, **base, **KATAOKA)
+    run_const("seeded_1d_kataoka_vanleer", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 61))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             **base, **dict(KATAOKA, kataoka_b=("Kataoka", 2.0)))
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_kataoka_hll", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 62))
+
+
 def hancock_cases(base):
     """MUSCL-Hancock (#Approximation_order SecondOrder): one Advection with
     the half-step face predictor per step."""
@@ -288,5 +308,7 @@ if __name__ == "__main__":
         rk_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "stiffened":
         stiffened_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "kataoka":
+        kataoka_cases(b)
     else:
         main()

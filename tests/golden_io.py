@@ -26,6 +26,8 @@ def load(name):
                 bc_p=[BC(t, tuple(v)) for t, v in d["bc_p"]],
                 gas_model=d.get("gas_model", 0), Tc=d.get("Tc", ()), Pc=d.get("Pc", ()),
                 p0=d.get("p0", ()), gammas=d.get("gammas", ()),
+                epsLJ=d.get("epsLJ", ()), sigmaLJ=d.get("sigmaLJ", ()),
+                kataoka_b=tuple(d.get("kataoka_b", ("critical", 1.0))),
                 step_solver=d.get("step_solver", 1))
     snaps = {int(s): {k: z[f"s{s}_{k}"] for k in KEYS} for s in z["steps"]}
     out = {"case": case, "dts": z["dts"], "snaps": snaps}

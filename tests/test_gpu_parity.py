@@ -41,9 +41,11 @@ def run_parity(name):
         got = g.conserved()
         diff = np.abs(got - ref)
         print(name, s, "max abs diff", diff.max(), "n differing", int((got != ref).sum()))
-        if int(case.gas_model) == 2:
+        if int(case.gas_model) in (2, 4):
             # Redlich-Kwong: the FP64 Newton root of the cubic in sqrt(T) is
-            # a few ulps from the long-double Cardano root of the reference
+            # a few ulps from the long-double Cardano root of the reference;
+            # Kataoka-van der Waals: the device's pow and exp in the series
+            # of its Newton iteration are not the host C library's
             for q in range(ref.shape[0]):
                 scale = max(np.abs(ref[q]).max(), 1e-300)
                 err = diff[q].max() / scale

@@ -226,7 +226,41 @@ def test_stiffened_initial_state_matches_oracle():
         assert np.abs(U[q] - ref[q]).max() <= 1e-15 * scale, q
 
 
-@pytest.mark.parametrize("model", ["ideal", "vdw", "rk", "stiffened"])
+KATAOKA = dict(Tc=[24.0, 3.2], Pc=[1.0, 1.0], epsLJ=[10.0, 5.0], sigmaLJ=[1.52e-8, 1.2e-8])
+
+
+@pytest.mark.parametrize("kb", [("critical", 1.0), ("Kataoka", 2.0)])
+def test_kataoka_initial_state_matches_oracle(kb):
+    """The host's Kataoka-van der Waals UvFromp (numpy Newton on T with the
+    reference's tolerance) gives the oracle's conserved state to the Newton
+    tolerance, and the host's mixing matrices are the oracle's."""
+    from schemi_amd import Case, gasModel as GM
+    from schemi_amd.gpu import conserved_from_primitive, kataoka_matrices
+    from oracle.oracle import Oracle
+
+    case = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), M=[29.0, 4.0], Cv=[2.5, 1 / 0.67],
+                gas_model=GM.KataokaVanDerWaals, kataoka_b=kb, **KATAOKA)
+    c, v, p = random_primitives(case, 9)
+    U = conserved_from_primitive(case, c, v, p)
+    o = Oracle(case)
+    o.init_primitive(c, v, p)
+    ref = o.conserved()
+    for q in range(ref.shape[0]):
+        scale = max(np.abs(ref[q]).max(), 1e-300)
+        assert np.abs(U[q] - ref[q]).max() <= 1e-12 * scale, q
+    V0, eps, b = kataoka_matrices(case)
+    # V0_kk = sigma^3 N_A / sqrt(2); eps geometric mean; b from Tc/Pc or sigma
+    assert np.isclose(V0[0, 0], 1.52e-8 ** 3 * 6.02214076e23 / np.sqrt(2), rtol=1e-15)
+    assert np.isclose(eps[0, 1], np.sqrt(50.0), rtol=1e-15)
+    if kb[0] == "critical":
+        assert np.isclose(b[1, 1], case.R * 3.2 / 8.0, rtol=1e-15)
+    else:
+        assert np.isclose(b[1, 1], 2.0 * 1.2e-8 ** 3 * 6.02214076e23, rtol=1e-15)
+    # mixing rule across species: ((cbrt b_00 + cbrt b_11) / 2)^3
+    assert np.isclose(b[0, 1], (0.5 * (np.cbrt(b[0, 0]) + np.cbrt(b[1, 1]))) ** 3, rtol=1e-14)
+
+
+@pytest.mark.parametrize("model", ["ideal", "vdw", "rk", "stiffened", "kataoka"])
 def test_eos_round_trip(model):
     """p -> U (UvFromp at initialisation) -> p (pFromUv of the conserved
     state) returns the pressure to a few ulps for every mixture model, and the
@@ -236,7 +270,8 @@ def test_eos_round_trip(model):
 
     eos = {"ideal": {}, "vdw": dict(gas_model=GM.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0]),
            "rk": dict(gas_model=GM.RedlichKwong, Tc=[24.0, 3.2], Pc=[1.0, 1.0]),
-           "stiffened": dict(gas_model=GM.stiffened, p0=[0.3, 0.05], gammas=[2.8, 1.6])}[model]
+           "stiffened": dict(gas_model=GM.stiffened, p0=[0.3, 0.05], gammas=[2.8, 1.6]),
+           "kataoka": dict(gas_model=GM.KataokaVanDerWaals, **KATAOKA)}[model]
     case = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), M=[29.0, 4.0], Cv=[2.5, 1 / 0.67], **eos)
     c, v, p = random_primitives(case, 13)
     o = Oracle(case)
@@ -247,5 +282,8 @@ def test_eos_round_trip(model):
     o2 = Oracle(case)
     o2.set_conserved(U[:Ns], U[Ns], U[Ns + 1:Ns + 4], U[Ns + 4])
     st = o2.state()
-    assert np.max(np.abs(st["p"] - p) / np.abs(p)) < 1e-13
-    assert np.max(np.abs(st["T"] - st0["T"]) / np.abs(st0["T"])) < 1e-13
+    # Kataoka-van der Waals: both directions stop their Newton iteration at
+    # |T_{k+1} - T_k| <= 1e-12 (absolute), so the round trip holds to that
+    tol = 1e-11 if model == "kataoka" else 1e-13
+    assert np.max(np.abs(st["p"] - p) / np.abs(p)) < tol
+    assert np.max(np.abs(st["T"] - st0["T"]) / np.abs(st0["T"])) < tol

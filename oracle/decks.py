@@ -94,15 +94,16 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         "#Equation_of_state", _EOS[int(case.gas_model)],
         "#TVD_limiter_type", _LIM[int(case.limiter)],
         "#Number_of_iterations_of_SLE_matrix_solver", "2000",
-        "#Diffusion_flag", "off",
+        "#Diffusion_flag", "on" if getattr(case, "diffusion", False) else "off",
         "#Turbulence_flag", "off",
-        "#Type_of_SLE_solver_flag", "GaussSeidel",
+        "#Type_of_SLE_solver_flag",
+        "explicit" if getattr(case, "diffusion", False) else "GaussSeidel",
         "#Type_of_source_flag", "BHR",
         "#Type_of_flow_solver", _FLUX[int(case.flux)],
         "#Dimensions_of_the_task", dims,
         "#Constant_timestep", _r(const_dt),
         "#Source_timestep_correction", "Courant=0.25,0.5",
-        "#Reconstruction", "star",
+        "#Reconstruction", "linear" if getattr(case, "diffusion", False) else "star",
         "#Approximation_order", _STEP[int(case.step_solver)],
         "#Start_From_Time-Step", "1" if restart is not None else "no",
         "#Calculation_of_mixing_zone_width", "off",
@@ -161,11 +162,19 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
     files["turbPeak.txt"] = ["#Type_of_peak", "no", "#Maximum_k", "1",
                              "#Maximum_epsilon", "1000", "#Center_coordinate", "1.0",
                              "#Left_width", "0.005", "#Right_width", "0.005"]
-    files["transportCoefficients.txt"] = [
-        "#Transport_coefficient_model", "constant", "#Viscosity_coefficient", "2E-5",
-        "#Diffusion_coefficient", "2E-5", "#Heat_conduction_coefficient", "2E-2",
-        "#Flag_for_enthalpy_flow_solution", "explicit",
-        "#Flag_for_molar_mass_diffusion_correction", "on"]
+    if getattr(case, "diffusion", False):
+        mu, Dc, kappa = case.transport
+        files["transportCoefficients.txt"] = [
+            "#Transport_coefficient_model", "constant", "#Viscosity_coefficient", _r(mu),
+            "#Diffusion_coefficient", _r(Dc), "#Heat_conduction_coefficient", _r(kappa),
+            "#Flag_for_enthalpy_flow_solution", "no",
+            "#Flag_for_molar_mass_diffusion_correction", "off"]
+    else:
+        files["transportCoefficients.txt"] = [
+            "#Transport_coefficient_model", "constant", "#Viscosity_coefficient", "2E-5",
+            "#Diffusion_coefficient", "2E-5", "#Heat_conduction_coefficient", "2E-2",
+            "#Flag_for_enthalpy_flow_solution", "explicit",
+            "#Flag_for_molar_mass_diffusion_correction", "on"]
     turb = ["Cmu 0.09", "Turbulent_Schmidt 0.7", "Turbulent_Prandtl_T 0.7",
             "Turbulent_Prandtl_e 1E6", "sigma_k 0.7", "sigma_epsilon 0.6",
             "sigma_a 1.4", "sigma_b 0.7", "CMS_R 1E-1", "CMS_D 1E0",

@@ -48,6 +48,10 @@ class _Config(C.Structure):
         ("kb_critical", C.c_int),
         ("kb_coeff", C.c_double),
         ("e_kl", C.c_double * (MAXS * MAXS)),
+        ("diffusion", C.c_int),
+        ("mu", C.c_double),
+        ("D", C.c_double),
+        ("kappa", C.c_double),
     ]
 
 
@@ -139,6 +143,8 @@ def make_config(case) -> _Config:
         cfg.sigLJ[k] = float(case.sigmaLJ[k])
     cfg.kb_critical = 1 if case.kataoka_b[0] == "critical" else 0
     cfg.kb_coeff = float(case.kataoka_b[1])
+    cfg.diffusion = 1 if getattr(case, "diffusion", False) else 0
+    cfg.mu, cfg.D, cfg.kappa = (float(x) for x in getattr(case, "transport", (0, 0, 0)))
     cfg.flux = int(case.flux)
     cfg.limiter = int(case.limiter)
     cfg.gravity = 1 if case.gravity else 0

@@ -19,7 +19,7 @@
 
 #define STAB (5.0 * DBL_EPSILON) /* globalConstants.hpp:33 stabilizator */
 
-enum { ERR_NONE = 0, ERR_RIEMANN = 1, ERR_BC = 5, ERR_INIT = 11 };
+enum { ERR_NONE = 0, ERR_RIEMANN = 1, ERR_NEG_T = 2, ERR_BC = 5, ERR_INIT = 11 };
 
 struct or_solver {
     or_config cfg;
@@ -1235,10 +1235,14 @@ or_solver *or_create(const or_config *cfg)
     else if (cfg->gas_model == OR_GAS_KATAOKA)
         kataoka_matrices(&s->cfg);
     for (int a = 0; a < cfg->dim; ++a)
-        if (axis_periodic(s, a) < 0) {
+        if (axis_periodic(s, a) < 0 || (cfg->diffusion && axis_periodic(s, a) > 0)) {
             free(s);
             return NULL;
         }
+    if (cfg->diffusion && cfg->gas_model != OR_GAS_IDEAL) {
+        free(s);
+        return NULL;
+    }
     if (build_mesh(s) != ERR_NONE) {
         or_destroy(s);
         return NULL;
@@ -2249,7 +2253,330 @@ static void rk_average(or_solver *s, const or_state *Un)
     }
 }
 
-/* secondOrderStepSolverRK::calculateStep (secondOrderStepSolverRK.cpp:48-71) */
+/* ------------------------------------------------------------------------ */
+/* explicit molecular diffusion stage (Diffusion.cpp:20-1061) with constant  */
+/* transport coefficients, the explicit matrix solver and linear             */
+/* reconstruction; no turbulence, enthalpy flow or molar-mass correction     */
+
+/* surfGrad (gradient.hpp:96-165) of cell values q[ncomp] of cell o across
+ * face f: inner faces ((q_n - q_o) / |d|^2) d with d = r_n - r_o; boundary
+ * faces ((q_g - q_o) / |D|) D / |D| with the ghost value q_g and
+ * D = 2 (r_f - r_o).  out[3 c + b] = d(q_c) along b. */
+static void surf_grad(const or_solver *s, long f, int ncomp, const double *qo,
+                      const double *qn, double *out)
+{
+    const long o = s->own[f];
+    if (s->fpatch[f] < 0) {
+        const long n = s->nei[f];
+        double d[3];
+        for (int b = 0; b < 3; ++b)
+            d[b] = s->crc[3 * n + b] - s->crc[3 * o + b];
+        const double m = or_hypot3(d[0], d[1], d[2]);
+        const double m2 = m * m; /* pow<scalar, 2> (intExpPow.hpp) */
+        for (int c = 0; c < ncomp; ++c)
+            for (int b = 0; b < 3; ++b)
+                out[3 * c + b] = ((qn[c] - qo[c]) / m2) * d[b];
+    } else {
+        double D[3];
+        for (int b = 0; b < 3; ++b)
+            D[b] = (s->frc[3 * f + b] - s->crc[3 * o + b]) * 2.;
+        const double m = or_hypot3(D[0], D[1], D[2]);
+        double nrm[3];
+        for (int b = 0; b < 3; ++b)
+            nrm[b] = D[b] / m;
+        for (int c = 0; c < ncomp; ++c)
+            for (int b = 0; b < 3; ++b)
+                out[3 * c + b] = ((qn[c] - qo[c]) / m) * nrm[b];
+    }
+}
+
+/* outward normal of face f for cell i (owner: N, neighbour: N * -1) */
+static void out_normal(const or_solver *s, long f, long i, double *n)
+{
+    for (int a = 0; a < 3; ++a)
+        n[a] = (s->own[f] == i) ? s->fN[3 * f + a] : s->fN[3 * f + a] * -1;
+}
+
+/* ghost concentrations of a boundary face (boundaryConditionValueCell) */
+static int ghost_conc(const or_solver *s, long f, long o, double *cg /* c0..cNs */)
+{
+    const int p = s->fpatch[f];
+    cg[0] = 0;
+    for (int k = 1; k <= s->cfg.Ns; ++k) {
+        if (bc_cell_scalar(&s->cfg.bc_c[k - 1][p], s->st.c[k][o], &cg[k]))
+            return ERR_BC;
+        cg[0] += cg[k];
+    }
+    return 0;
+}
+
+static int diffusion_explicit(or_solver *s, double dt)
+{
+    const or_config *cfg = &s->cfg;
+    or_state *st = &s->st;
+    const int Ns = cfg->Ns;
+    const long nc = s->nc, nf = s->nf;
+    int err = 0;
+    /* cell work arrays: Y[Ns], CCVold, gradV[9], divV */
+    double *Y = malloc(nc * Ns * sizeof(double));
+    double *CCVold = malloc(nc * sizeof(double));
+    double *gradV = malloc(9 * nc * sizeof(double));
+    double *divV = malloc(nc * sizeof(double));
+    double *vf = malloc(3 * nf * sizeof(double));
+    double *srho = malloc(nf * sizeof(double));
+    double *FY = malloc(3 * Ns * nf * sizeof(double)); /* -rhoD_k surfGrad(Y_k) */
+    double *FT = malloc(3 * nf * sizeof(double));      /* -kappa surfGrad(T) */
+    double *FV = malloc(9 * nf * sizeof(double));      /* -mu surfGrad(v) */
+    double *FS = malloc(9 * nf * sizeof(double));      /* mu strainSurf1 */
+    double cc[OR_MAXS + 1];
+
+    /* mass fractions rho_k / rho, normalised (diffusiveFields.cpp:30-47) */
+    for (long i = 0; i < nc; ++i) {
+        double sum = 0.0;
+        for (int k = 0; k < Ns; ++k) {
+            Y[i * Ns + k] = st->rho[k + 1][i] / st->rho[0][i];
+            sum += Y[i * Ns + k];
+        }
+        for (int k = 0; k < Ns; ++k)
+            Y[i * Ns + k] /= sum;
+        for (int k = 0; k <= Ns; ++k)
+            cc[k] = st->c[k][i];
+        CCVold[i] = st->c[0][i] * cv_mix(cfg, cc); /* Diffusion.cpp:148 */
+    }
+    /* face velocity (linearInterpolate) for grad and divergence of v, and
+     * the face density sum_k c_k,f M_k (Diffusion.cpp:156-166) */
+    for (long f = 0; f < nf; ++f) {
+        const long o = s->own[f];
+        double cs[OR_MAXS + 1];
+        if (s->fpatch[f] < 0) {
+            const long n = s->nei[f];
+            for (int a = 0; a < 3; ++a)
+                vf[3 * f + a] = st->v[a][o] * s->wo[f] + st->v[a][n] * s->wn[f];
+            for (int k = 1; k <= Ns; ++k)
+                cs[k] = st->c[k][o] * s->wo[f] + st->c[k][n] * s->wn[f];
+        } else {
+            const int p = s->fpatch[f];
+            double vo[3] = {st->v[0][o], st->v[1][o], st->v[2][o]};
+            if (bc_surf_vec(&cfg->bc_v[p], vo, &s->fN[3 * f], &vf[3 * f]))
+                err = ERR_BC;
+            for (int k = 1; k <= Ns; ++k)
+                if (bc_surf_scalar(&cfg->bc_c[k - 1][p], st->c[k][o], &cs[k]))
+                    err = ERR_BC;
+        }
+        srho[f] = 0;
+        for (int k = 1; k <= Ns; ++k)
+            srho[f] += cs[k] * cfg->M[k - 1];
+    }
+    if (err)
+        goto done;
+    /* grad v and div v of the cells (gradient.hpp:19-58,
+     * divergence.hpp:19-58) */
+    for (long i = 0; i < nc; ++i) {
+        double acc[9] = {0}, dacc = 0;
+        for (int j = 0; j < s->cnf[i]; ++j) {
+            const long f = s->cf[6 * i + j];
+            double n[3];
+            out_normal(s, f, i, n);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    acc[3 * a + b] += (vf[3 * f + a] * n[b]) * s->fS[f];
+            dacc += dot3(&vf[3 * f], n) * s->fS[f];
+        }
+        for (int q = 0; q < 9; ++q)
+            gradV[9 * i + q] = acc[q] / s->cV[i];
+        divV[i] = dacc / s->cV[i];
+    }
+    /* face fluxes */
+    for (long f = 0; f < nf; ++f) {
+        const long o = s->own[f];
+        const int bnd = s->fpatch[f] >= 0;
+        const long n = bnd ? o : s->nei[f];
+        double qo[OR_MAXS + 3], qn[OR_MAXS + 3], g[3 * (OR_MAXS + 3)];
+        /* mass fractions: ghost rho_k / rho from the ghost concentrations
+         * (calculatedMassFraction, boundaryConditionValue.hpp:356-374) */
+        for (int k = 0; k < Ns; ++k) {
+            qo[k] = Y[o * Ns + k];
+            qn[k] = Y[n * Ns + k];
+        }
+        double cg[OR_MAXS + 1];
+        if (bnd) {
+            if (ghost_conc(s, f, o, cg)) {
+                err = ERR_BC;
+                goto done;
+            }
+            double sumd = 0;
+            for (int k = 1; k <= Ns; ++k)
+                sumd += cg[k] * cfg->M[k - 1];
+            for (int k = 0; k < Ns; ++k)
+                qn[k] = (cg[k + 1] * cfg->M[k]) / sumd;
+        }
+        surf_grad(s, f, Ns, qo, qn, g);
+        for (int k = 0; k < Ns; ++k) {
+            const double rhoD = srho[f] * cfg->D; /* transportCoefficients.hpp:258-261 */
+            for (int b = 0; b < 3; ++b)
+                FY[(k * nf + f) * 3 + b] = (g[3 * k + b] * rhoD) * -1;
+        }
+        /* temperature: ghost T from the ghost concentrations and pressure
+         * (calculatedTemperature, boundaryConditionValue.hpp:329-354) */
+        qo[0] = st->T[o];
+        qn[0] = st->T[n];
+        if (bnd) {
+            double pg;
+            if (bc_cell_scalar(&cfg->bc_p[s->fpatch[f]], st->p[o], &pg)) {
+                err = ERR_BC;
+                goto done;
+            }
+            qn[0] = eos_T(cfg, cg, eos_U(cfg, cg, pg));
+        }
+        surf_grad(s, f, 1, qo, qn, g);
+        for (int b = 0; b < 3; ++b)
+            FT[3 * f + b] = (g[b] * cfg->kappa) * -1;
+        /* velocity: surfGrad (tensor v_a d_b) and surfDivergence
+         * (divergence.hpp:84-150) */
+        for (int a = 0; a < 3; ++a) {
+            qo[a] = st->v[a][o];
+            qn[a] = st->v[a][n];
+        }
+        if (bnd) {
+            double vo[3] = {qo[0], qo[1], qo[2]};
+            if (bc_cell_vec(&cfg->bc_v[s->fpatch[f]], vo, &s->fN[3 * f], qn)) {
+                err = ERR_BC;
+                goto done;
+            }
+        }
+        surf_grad(s, f, 3, qo, qn, g);
+        double divs;
+        {
+            double dq[3], dd[3];
+            if (!bnd) {
+                double d[3];
+                for (int b = 0; b < 3; ++b)
+                    d[b] = s->crc[3 * n + b] - s->crc[3 * o + b];
+                const double m = or_hypot3(d[0], d[1], d[2]);
+                for (int a = 0; a < 3; ++a)
+                    dq[a] = (qn[a] - qo[a]) / (m * m);
+                memcpy(dd, d, sizeof dd);
+            } else {
+                double D[3];
+                for (int b = 0; b < 3; ++b)
+                    D[b] = (s->frc[3 * f + b] - s->crc[3 * o + b]) * 2.;
+                const double m = or_hypot3(D[0], D[1], D[2]);
+                for (int a = 0; a < 3; ++a) {
+                    dq[a] = (qn[a] - qo[a]) / m;
+                    dd[a] = D[a] / m;
+                }
+            }
+            divs = dot3(dq, dd);
+        }
+        for (int q = 0; q < 9; ++q)
+            FV[9 * f + q] = (g[q] * cfg->mu) * -1;
+        /* strainSurf1 = (surfGrad v)^T - 2/3 div_s v I, times mu
+         * (Diffusion.cpp:193-230, 482-483) */
+        const double lam = (2. / 3.) * divs;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                double t = g[3 * b + a];
+                if (a == b)
+                    t -= lam;
+                FS[9 * f + 3 * a + b] = t * cfg->mu;
+            }
+    }
+    /* cell updates (SLEMatrix.cpp:2054-2142 explicit Laplacians, then
+     * Diffusion.cpp:431-458, 462-483, 599-615, 711-760, 795-806, 818-828,
+     * 963-1059) */
+    for (long i = 0; i < nc; ++i) {
+        double dY[OR_MAXS] = {0}, dT = 0, dV[3] = {0}, dS[3] = {0};
+        for (int j = 0; j < s->cnf[i]; ++j) {
+            const long f = s->cf[6 * i + j];
+            double n[3];
+            out_normal(s, f, i, n);
+            const double S = s->fS[f];
+            for (int k = 0; k < Ns; ++k)
+                dY[k] += dot3(&FY[(k * nf + f) * 3], n) * S;
+            dT += dot3(&FT[3 * f], n) * S;
+            for (int a = 0; a < 3; ++a) {
+                /* tensor & vector (tensor.cpp:163-176) */
+                const double *Tv = &FV[9 * f + 3 * a], *Ts = &FS[9 * f + 3 * a];
+                dV[a] += (Tv[0] * n[0] + Tv[1] * n[1] + Tv[2] * n[2]) * S;
+                dS[a] += (Ts[0] * n[0] + Ts[1] * n[1] + Ts[2] * n[2]) * S;
+            }
+        }
+        const double V = s->cV[i], rho = st->rho[0][i];
+        /* mass fractions: (Y rho + (0 - div F) dt) / rho, clipped at 0 and
+         * renormalised */
+        double Yn[OR_MAXS], sum = 0.0;
+        for (int k = 0; k < Ns; ++k) {
+            const double free_term = 0. - dY[k] / V;
+            Yn[k] = (Y[i * Ns + k] * rho + free_term * dt) / rho;
+            if (Yn[k] < 0.)
+                Yn[k] = 0.0;
+            sum += Yn[k];
+        }
+        double cn[OR_MAXS + 1];
+        cn[0] = 0;
+        for (int k = 0; k < Ns; ++k) {
+            Yn[k] /= sum;
+            cn[k + 1] = Yn[k] * rho / cfg->M[k];
+            cn[0] += cn[k + 1];
+        }
+        const double CCVnew = cv_mix(cfg, cn) * cn[0];
+        /* velocity: (rho v + ((0 - div F_v) + div(mu strainSurf1)) dt) / rho */
+        double vn[3];
+        for (int a = 0; a < 3; ++a) {
+            double free_term = 0. - dV[a] / V;
+            free_term += dS[a] / V;
+            vn[a] = (st->v[a][i] * rho + free_term * dt) / rho;
+        }
+        /* temperature: (CCV_old T + ((0 - div F_T) + (strainVol mu) && gradV)
+         * dt) / CCV_new */
+        const double *G = &gradV[9 * i];
+        double sv[9];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                sv[3 * a + b] = G[3 * b + a] + G[3 * a + b];
+        const double lam = (2. / 3.) * divV[i];
+        sv[0] -= lam;
+        sv[4] -= lam;
+        sv[8] -= lam;
+        double heat = 0;
+        for (int q = 0; q < 9; ++q)
+            heat += (sv[q] * cfg->mu) * G[q];
+        double free_t = 0. - dT / V;
+        free_t += heat;
+        const double Tn = (CCVold[i] * st->T[i] + free_t * dt) / CCVnew;
+        /* recalculation of the cell values (Diffusion.cpp:963-1029) */
+        for (int k = 0; k <= Ns; ++k)
+            st->c[k][i] = cn[k];
+        for (int k = 1; k <= Ns; ++k)
+            st->rho[k][i] = cn[k] * cfg->M[k - 1];
+        for (int a = 0; a < 3; ++a) {
+            st->v[a][i] = vn[a];
+            st->m[a][i] = vn[a] * rho;
+        }
+        st->T[i] = Tn;
+        st->U[i] = (cv_mix(cfg, cn) * Tn) * cn[0];
+        st->p[i] = eos_p(cfg, cn, st->U[i]);
+        const double v2 = dot3(vn, vn);
+        st->E[i] = (st->U[i] + rho * v2 * 0.5) + 0.0;
+        if (Tn < 0.)
+            err = ERR_NEG_T;
+    }
+done:
+    free(Y);
+    free(CCVold);
+    free(gradV);
+    free(divV);
+    free(vf);
+    free(srho);
+    free(FY);
+    free(FT);
+    free(FV);
+    free(FS);
+    return err;
+}
+
+/* secondOrderStepSolverRK::calculateStep (secondOrderStepSolverRK.cpp:48-71,
+ * diffusion 102-109) */
 int or_step(or_solver *s, double dt, int mode)
 {
     if (mode == 1)
@@ -2264,19 +2591,31 @@ int or_step(or_solver *s, double dt, int mode)
         rk_average(s, &Un);
     if (!err && mode == 1)
         recover_primitives(s, &s->st);
+    if (!err && s->cfg.diffusion) {
+        err = diffusion_explicit(s, dt);
+        if (!err && mode == 1)
+            recover_primitives(s, &s->st);
+    }
     free(pool);
     return err;
 }
 
 /* secondOrderStepSolver::calculateStep (secondOrderStepSolver.cpp:48-93):
- * one Advection with the Hancock predictor; diffusion and chemistry off. */
+ * one Advection with the Hancock predictor, then the diffusion stage when
+ * enabled; chemistry off. */
 int or_step_hancock(or_solver *s, double dt, int mode)
 {
     if (mode == 1)
         recover_primitives(s, &s->st);
-    const int err = stage_impl(s, dt, 1);
+    int err = stage_impl(s, dt, 1);
     if (!err && mode == 1)
         recover_primitives(s, &s->st);
+    if (!err && s->cfg.diffusion) {
+        /* secondOrderStepSolver.cpp:72-82 */
+        err = diffusion_explicit(s, dt);
+        if (!err && mode == 1)
+            recover_primitives(s, &s->st);
+    }
     return err;
 }
 

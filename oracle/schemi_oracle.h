@@ -75,6 +75,13 @@ typedef struct {
     int kb_critical;
     double kb_coeff;
     double e_kl[OR_MAXS * OR_MAXS];
+    /* explicit molecular diffusion stage after each step (Diffusion.cpp):
+     * constant transport coefficients (abstractTransportModel.cpp:14-34:
+     * viscosity mu, diffusion coefficient D, heat conduction kappa), explicit
+     * matrix solver, linear reconstruction, no turbulence, no enthalpy flow,
+     * no molar-mass correction; ideal mixture, non-periodic axes */
+    int diffusion;
+    double mu, D, kappa;
 } or_config;
 
 typedef struct or_solver or_solver;

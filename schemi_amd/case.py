@@ -69,6 +69,12 @@ class Case:
     # step solver (#Approximation_order): 1 SecondOrderRK (SSP-RK2),
     # 0 SecondOrder (MUSCL-Hancock)
     step_solver: int = 1
+    # molecular diffusion stage after each step (#Diffusion_flag on, explicit
+    # matrix solver, linear reconstruction, constant transport coefficients
+    # of ./set/transportCoefficients.txt: viscosity, diffusion coefficient,
+    # heat conduction coefficient)
+    diffusion: bool = False
+    transport: tuple = (2e-5, 2e-5, 2e-2)
 
     def __post_init__(self):
         if not self.bc_c:

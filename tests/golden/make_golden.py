@@ -43,6 +43,8 @@ def case_json(case: Case) -> str:
                    kataoka_b=list(case.kataoka_b))
     if int(case.step_solver) != 1:
         eos["step_solver"] = int(case.step_solver)
+    if case.diffusion:
+        eos.update(diffusion=True, transport=list(case.transport))
     return json.dumps({**eos,
         "dim": case.dim, "N": list(case.N), "L": list(case.L), "M": list(case.M),
         "Cv": list(case.Cv), "R": case.R, "flux": int(case.flux),
@@ -296,6 +298,41 @@ def hancock_cases(base):
     run_const("seeded_2d_hancock_vdw", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 35))
 
 
+def diffusion_cases(base):
+    """Molecular diffusion stage (#Diffusion_flag on, explicit matrix
+    solver, linear reconstruction, constant transport coefficients) after
+    each step: viscosity mu, diffusion coefficient D and heat conduction
+    kappa chosen so that D dt / dx^2 and the thermal and viscous analogues
+    are 0.01-0.1 for the seeded states."""
+    tr = dict(diffusion=True, transport=(0.004, 0.02, 0.002))
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, **base, **tr)
+    run_const("seeded_1d_diffusion_vanleer", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 71))
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), flux=FS.HLL, limiter=TL.minmod,
+             **base, **tr)
+    c.set_bc(0, BCT.slip)
+    c.set_bc(1, BCT.fixedValueCell, "p", (1.2, 0, 0))
+    c.set_bc(1, BCT.fixedValueCell, "v", (0.05, 0, 0))
+    c.set_bc(1, BCT.fixedValueCell, "c", (0.03, 0, 0))
+    run_const("seeded_1d_diffusion_slip_fixed", c, 5, 2e-4, (1, 5),
+              restart=seeded_primitives(c, 72))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             **base, **tr)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_diffusion_hll", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 73))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.linear, **base, **tr)
+    c.set_bc(3, BCT.slip)
+    run_const("seeded_2d_diffusion_linear", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 74))
+    c = Case(dim=3, N=(8, 6, 6), L=(1.0, 0.5, 0.35), limiter=TL.vanAlbada, **base, **tr)
+    for pt in (2, 5):
+        c.set_bc(pt, BCT.slip)
+    run_const("seeded_3d_diffusion_lz", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 75))
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, step_solver=0,
+             **base, **tr)
+    run_const("seeded_1d_diffusion_hancock", c, 5, 2e-4, (1, 5),
+              restart=seeded_primitives(c, 76))
+
+
 if __name__ == "__main__":
     b = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
     if len(sys.argv) > 1 and sys.argv[1] == "lz":
@@ -310,5 +347,7 @@ if __name__ == "__main__":
         stiffened_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "kataoka":
         kataoka_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "diffusion":
+        diffusion_cases(b)
     else:
         main()

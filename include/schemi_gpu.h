@@ -43,6 +43,7 @@ extern "C" {
 /* errors (Schemi/enumerations/errorsEnum.hpp:15-30) */
 #define SCHEMI_OK 0
 #define SCHEMI_ERR_RIEMANN 1
+#define SCHEMI_ERR_NEGATIVE_TEMPERATURE 2
 #define SCHEMI_ERR_SYSTEM 4
 #define SCHEMI_ERR_BOUNDARY 5
 #define SCHEMI_ERR_MESH 10
@@ -151,6 +152,20 @@ typedef struct {
     schemi_bc bc_v[6];
     schemi_bc bc_p[6];
     int32_t step_solver; /* SCHEMI_STEP_* */
+    /* Molecular diffusion stage after each step (Diffusion.cpp:20-1061,
+     * called at secondOrderStepSolverRK.cpp:102-109 and
+     * secondOrderStepSolver.cpp:72-82; #Diffusion_flag on): the explicit
+     * matrix solver (#Type_of_SLE_solver_flag explicit) with linear
+     * reconstruction (#Reconstruction linear) and the constant transport
+     * model of set/transportCoefficients.txt (viscosity, diffusion and heat
+     * conduction coefficients; enthalpy flow "no", molar-mass correction
+     * "off"); no turbulence.  Ideal mixture, non-periodic axes; otherwise
+     * SCHEMI_ERR_INIT.  A negative temperature after the stage returns
+     * SCHEMI_ERR_NEGATIVE_TEMPERATURE (negativeTemperatureError). */
+    int32_t diffusion;
+    double transport_mu;
+    double transport_D;
+    double transport_kappa;
 } schemi_scheme_desc;
 
 typedef struct schemi_gpu_ctx schemi_gpu_ctx;

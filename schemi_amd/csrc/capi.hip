@@ -192,6 +192,9 @@ struct schemi_gpu_ctx {
     bool timing = false;
     std::vector<hipEvent_t> ev;  // start/stop pairs of the timed stage launches
     double *adj = nullptr;       // adjoint: lambda of U^1 (Ns + 5 rows), partial sums
+    bool diffusion = false;      // explicit molecular diffusion stage after each step
+    double mu = 0, D = 0, kappa = 0;
+    double *Wd = nullptr;        // diffusion: (2 Ns + 5) x n primitives
 };
 
 namespace {
@@ -270,6 +273,28 @@ int one_step(schemi_gpu_ctx *c)
         if (r)
             return r;
     }
+    if (c->diffusion) {
+        // Diffusion (secondOrderStepSolverRK.cpp:102-109,
+        // secondOrderStepSolver.cpp:72-82) on the new state
+        DiffusionLaunch L;
+        L.P = c->dP;
+        L.geo = c->dgeo;
+        L.U = c->U[0];
+        L.Ualt = c->U[1];
+        L.hk = c->hancock ? 1 : 0;
+        L.W = c->Wd;
+        L.sc = c->sc;
+        L.err = c->derr;
+        L.mu = c->mu;
+        L.D = c->D;
+        L.kappa = c->kappa;
+        L.dim = c->dim;
+        L.ns = c->ns;
+        L.n = c->n;
+        L.stream = c->stream;
+        if (launch_diffusion(L))
+            return SCHEMI_ERR_SYSTEM;
+    }
     hipLaunchKernelGGL(finalize_step_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, c->dP,
                        c->const_dt, c->hancock ? 1 : 0);
     return hip_fail(hipGetLastError());
@@ -294,6 +319,10 @@ int read_error(schemi_gpu_ctx *c)
     HIPCHK(hipStreamSynchronize(c->stream));
     if (e & ERRF_RIEMANN)
         return SCHEMI_ERR_RIEMANN;
+    if (e & ERRF_BOUNDARY)
+        return SCHEMI_ERR_BOUNDARY;
+    if (e & ERRF_NEG_T)
+        return SCHEMI_ERR_NEGATIVE_TEMPERATURE;
     return SCHEMI_OK;
 }
 
@@ -353,6 +382,8 @@ const char *schemi_gpu_strerror(int code)
         return "noErrors";
     case SCHEMI_ERR_RIEMANN:
         return "RiemannSolverError: Fatal error in Riemann solver.";
+    case SCHEMI_ERR_NEGATIVE_TEMPERATURE:
+        return "negativeTemperatureError: Negative temperature after diffusion.";
     case SCHEMI_ERR_SYSTEM:
         return "systemError: HIP runtime failure";
     case SCHEMI_ERR_BOUNDARY:
@@ -408,6 +439,10 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     if (scheme->step_solver != SCHEMI_STEP_SECOND_ORDER &&
         scheme->step_solver != SCHEMI_STEP_SECOND_ORDER_RK)
         return SCHEMI_ERR_INIT;
+    if (scheme->diffusion &&
+        (mix->gas_model != SCHEMI_GAS_IDEAL || !(scheme->transport_mu >= 0.0) ||
+         !(scheme->transport_D >= 0.0) || !(scheme->transport_kappa >= 0.0)))
+        return SCHEMI_ERR_INIT;
 
     schemi_gpu_ctx *c = new (std::nothrow) schemi_gpu_ctx();
     if (!c)
@@ -418,6 +453,10 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     c->prec = scheme->precision;
     c->const_dt = scheme->const_dt;
     c->hancock = scheme->step_solver == SCHEMI_STEP_SECOND_ORDER;
+    c->diffusion = scheme->diffusion != 0;
+    c->mu = scheme->transport_mu;
+    c->D = scheme->transport_D;
+    c->kappa = scheme->transport_kappa;
     DevParams &P = c->hp;
     std::memset(&P, 0, sizeof P);
     P.Ns = mix->ns;
@@ -561,6 +600,10 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         }
         periodic[a] = nper != 0;
         P.periodic[a] = periodic[a] ? 1 : 0;
+        if (periodic[a] && c->diffusion) {
+            delete c;
+            return SCHEMI_ERR_INIT;
+        }
     }
     for (int a = 0; a < 3; ++a)
         P.zero_entry[a] = (dim >= 2) ? 1 : 0;
@@ -647,6 +690,9 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
               hipMalloc(&c->U[1], state_bytes) == hipSuccess &&
               hipMalloc(&c->sc, SC_N * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->derr, sizeof(int)) == hipSuccess;
+    if (ok && c->diffusion)
+        ok = hipMalloc(&c->Wd, sizeof(double) * (size_t)(2 * mix->ns + 5) * (size_t)c->n) ==
+             hipSuccess;
     if (ok && c->mode == MODE_GENERAL) {
         const size_t tsz = (c->prec == SCHEMI_PREC_MIXED) ? sizeof(float) : sizeof(double);
         const size_t nq = (size_t)(mix->ns + 4) * (size_t)c->n;
@@ -693,6 +739,7 @@ void schemi_gpu_destroy(schemi_gpu_ctx *c)
     (void)hipFree(c->derr);
     (void)hipFree(c->scratch);
     (void)hipFree(c->adj);
+    (void)hipFree(c->Wd);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
@@ -891,7 +938,8 @@ int schemi_gpu_adjoint_step(schemi_gpu_ctx *c, double dt, const double *lam_in, 
     if (!c || !lam_in || !lam_out || (what != SCHEMI_ADJ_STEP && what != SCHEMI_ADJ_STAGE))
         return SCHEMI_ERR_INIT;
     const bool kernel = c->mode == MODE_FIRST_ORDER || (c->mode == MODE_SLOPES && c->dim == 1);
-    if (!kernel || c->hancock || c->hp.gas_model == GAS_RK || c->hp.gas_model == GAS_KATAOKA)
+    if (!kernel || c->hancock || c->diffusion || c->hp.gas_model == GAS_RK ||
+        c->hp.gas_model == GAS_KATAOKA)
         return SCHEMI_ERR_INIT;
     if (c->pending_error)
         return c->pending_error;

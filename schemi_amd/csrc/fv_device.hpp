@@ -39,7 +39,7 @@ enum : int { SV_RK1 = 0, SV_RK2 = 1, SV_HANCOCK = 2 };
 // the two state buffers).
 enum : int { SC_DT = 0, SC_TIME, SC_TEND, SC_SIGMAX, SC_DTNEXT, SC_STEPS, SC_DONE, SC_PARITY,
              SC_N };
-enum : int { ERRF_RIEMANN = 1, ERRF_BOUNDARY = 2 };
+enum : int { ERRF_RIEMANN = 1, ERRF_BOUNDARY = 2, ERRF_NEG_T = 4 };
 
 // Parameters shared by all kernels of a context, kept in device memory and
 // read with uniform (scalar) loads.

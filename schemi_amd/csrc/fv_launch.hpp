@@ -72,6 +72,25 @@ struct AdjointLaunch {
 };
 int launch_adjoint_stage(const AdjointLaunch &L);
 
+// Explicit molecular diffusion stage on the state after a step (FP64):
+// primitives into W ((2 ns + 5) n doubles), then the update in place of the
+// buffer holding the new state (MUSCL-Hancock: U or Ualt by the scalar
+// block's parity), and the Courant maximum of the result.
+struct DiffusionLaunch {
+    const DevParams *P;
+    const double *geo;
+    double *U, *Ualt;
+    int hk;
+    double *W;
+    double *sc;
+    int *err;
+    double mu, D, kappa;
+    int dim, ns;
+    long long n;
+    hipStream_t stream;
+};
+int launch_diffusion(const DiffusionLaunch &L);
+
 // Mixing width sum_i V_i Y (1 - Y), Y = rho_sp / rho, into *J (device), and
 // dJ/dU into grad (device, nullable); partial: (n + 255) / 256 doubles.
 int launch_mixing_width(const DevParams *P, const double *geo, const double *U, int ns, long long n,

@@ -62,7 +62,9 @@ class SchemeDesc(C.Structure):
                 ("gravity", C.c_int32), ("g", C.c_double * 3), ("courant", C.c_double),
                 ("min_time", C.c_double), ("const_dt", C.c_double),
                 ("bc_c", (_BC * 6) * MAXS), ("bc_v", _BC * 6), ("bc_p", _BC * 6),
-                ("step_solver", C.c_int32)]
+                ("step_solver", C.c_int32), ("diffusion", C.c_int32),
+                ("transport_mu", C.c_double), ("transport_D", C.c_double),
+                ("transport_kappa", C.c_double)]
 
 
 _LIB = None
@@ -188,6 +190,8 @@ def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) 
             s.bc_v[p].value[a] = float(case.bc_v[p].value[a])
             s.bc_p[p].value[a] = float(case.bc_p[p].value[a])
     s.step_solver = int(case.step_solver)
+    s.diffusion = 1 if case.diffusion else 0
+    s.transport_mu, s.transport_D, s.transport_kappa = (float(x) for x in case.transport)
     return s
 
 

@@ -1,0 +1,102 @@
+"""The explicit molecular diffusion stage on the GPU (Diffusion.cpp with the
+explicit matrix solver, linear reconstruction and constant transport
+coefficients) beyond the per-step parity cases: Courant-step runs, the
+negative-temperature error, declined configurations."""
+import numpy as np
+import pytest
+
+from schemi_amd import Case, errors, typeOfTVDLimiter as TL
+from schemi_amd import boundaryConditionType as BCT
+from schemi_amd.gpu import GpuSolver, SchemiGpuError, conserved_from_primitive, periodic
+from tests.cases_small import random_primitives
+
+BASE = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
+DIF = dict(diffusion=True, transport=(0.004, 0.02, 0.002))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("step_solver", [1, 0])
+def test_courant_run_with_diffusion_matches_oracle(step_solver, hip_device):
+    """The Courant step of the next step is taken from the state after the
+    diffusion stage: 60 device-side Courant steps follow the oracle's dt
+    sequence and state bit for bit (RK2 and MUSCL-Hancock)."""
+    from oracle.oracle import Oracle
+
+    case = Case(dim=2, N=(32, 16, 1), L=(1.0, 0.5, 0.1), limiter=TL.vanLeer,
+                step_solver=step_solver, courant=0.3, min_time=1e-6, **BASE, **DIF)
+    case.set_bc(3, BCT.slip)
+    case.set_bc(4, BCT.slip)
+    c, v, p = random_primitives(case, 31)
+    U0 = conserved_from_primitive(case, c, v, p)
+    Ns = case.Ns
+    o = Oracle(case)
+    o.set_conserved(U0[:Ns], U0[Ns], U0[Ns + 1:Ns + 4], U0[Ns + 4])
+    dt, t = case.min_time, 0.0
+    for _ in range(60):
+        assert o.step(dt, 1) == 0
+        t = t + dt
+        dt = o.courant_dt(case.courant, case.min_time)
+    s = GpuSolver(case)
+    s.set_conserved(U0)
+    t_gpu, steps, dt_gpu = s.run(60, case.min_time)
+    assert steps == 60
+    assert t_gpu == t
+    assert dt_gpu == dt
+    assert np.array_equal(s.conserved(), o.conserved())
+
+
+@pytest.mark.gpu
+def test_negative_temperature_reported(hip_device):
+    """A heat conduction coefficient far beyond the explicit stability limit
+    drives the temperature of a hot cell negative in one step: both the
+    oracle and the GPU report negativeTemperatureError."""
+    from oracle.oracle import Oracle
+
+    case = Case(dim=1, N=(16, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.minmod, **BASE,
+                diffusion=True, transport=(0.0, 0.0, 50.0))
+    n = case.ncells
+    c = np.full((2, n), 0.03)
+    v = np.zeros((3, n))
+    p = np.full(n, 1.0)
+    p[8] = 20.0
+    U0 = conserved_from_primitive(case, c, v, p)
+    Ns = case.Ns
+    o = Oracle(case)
+    o.set_conserved(U0[:Ns], U0[Ns], U0[Ns + 1:Ns + 4], U0[Ns + 4])
+    assert o.step(1e-3, 1) == errors.negativeTemperatureError
+    s = GpuSolver(case)
+    s.set_conserved(U0)
+    with pytest.raises(SchemiGpuError) as e:
+        s.step(1e-3)
+    assert e.value.code == errors.negativeTemperatureError
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("what", ["periodic", "vdw", "negative_mu"])
+def test_unsupported_diffusion_rejected(what, hip_device):
+    """Diffusion with a periodic axis, a non-ideal mixture or a negative
+    transport coefficient is declined with initialisationError; the adjoint
+    of a step with diffusion likewise."""
+    kw = dict(BASE)
+    if what == "vdw":
+        kw.update(gas_model=1, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    tr = (-1.0, 0.02, 0.002) if what == "negative_mu" else DIF["transport"]
+    case = Case(dim=2, N=(16, 8, 1), L=(1.0, 0.5, 0.1), limiter=TL.vanLeer, diffusion=True,
+                transport=tr, **kw)
+    if what == "periodic":
+        periodic(case, 0)
+    with pytest.raises(SchemiGpuError) as e:
+        GpuSolver(case)
+    assert e.value.code == errors.initialisationError
+
+
+@pytest.mark.gpu
+def test_adjoint_declined_with_diffusion(hip_device):
+    import torch
+
+    case = Case(dim=1, N=(16, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.zero, **BASE, **DIF)
+    s = GpuSolver(case, const_dt=1e-4)
+    lam = torch.zeros((case.Ns + 5) * case.ncells, dtype=torch.float64, device="cuda")
+    with pytest.raises(SchemiGpuError) as e:
+        s.adjoint_step(1e-4, lam.data_ptr(), lam.data_ptr(), 0)
+    assert e.value.code == errors.initialisationError

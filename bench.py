@@ -13,6 +13,11 @@ per launch over its average HIP-event duration), and the reference solver's
 CPU step timed on a bounded sample of the same workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+With --diffusion the same workload runs with the explicit molecular diffusion
+stage after every step (constant viscosity, diffusion and heat conduction
+coefficients 1e-4), and the line reports the whole step and the two
+diffusion passes, the update pass against the HBM roofline.
 """
 from __future__ import annotations
 
@@ -50,7 +55,18 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 1 GiB device copy (calibrates rocprofv3 byte counters)")
+    ap.add_argument("--diffusion", action="store_true",
+                    help="the workload with the molecular diffusion stage after each step")
     return ap.parse_args()
+
+
+# diffusion stage, algorithmic bytes per cell: the primitives pass reads the
+# conserved vector (Ns + 5 doubles) and writes Y, c, v, T, p (2 Ns + 5); the
+# update pass reads the cell's primitives and conserved vector and writes the
+# conserved vector (neighbours' primitives from cache)
+DIFF_TRANSPORT = (1e-4, 1e-4, 1e-4)
+DIFF_BYTES_PRIMS = 8 * ((NS + 5) + (2 * NS + 5))  # 128
+DIFF_BYTES_UPDATE = 8 * ((2 * NS + 5) + 2 * (NS + 5))  # 184
 
 
 def timed_run(solver, U0, steps, warmup):
@@ -116,7 +132,7 @@ def _ref_stage_ms(root):
     return sum(v for k, v in vals.items() if "Diffusion" not in k), vals
 
 
-def cpu_baseline(nx=96, ny=48, nz=48, steps=2):
+def cpu_baseline(nx=96, ny=48, nz=48, steps=2, diffusion=False):
     """The reference solver (oracle/_ref/schemi, serial) on a 96x48x48 sample
     of the same workload; falls back to the C oracle port when the reference
     binary is absent.  Returns Mcell-updates/s per the same definition, from
@@ -128,8 +144,11 @@ def cpu_baseline(nx=96, ny=48, nz=48, steps=2):
     except Exception as e:  # pragma: no cover
         return {"error": str(e)}
     case, U0 = cases.richtmyer_meshkov_3d(nx=nx, ny=ny, nz=nz, seed=2024)
+    if diffusion:
+        case.diffusion, case.transport = True, DIFF_TRANSPORT
     n = case.ncells
-    sample = f"3-D RM {nx}x{ny}x{nz} ({n} cells), {steps} SSP-RK2 steps"
+    sample = f"3-D RM {nx}x{ny}x{nz} ({n} cells), {steps} SSP-RK2 steps" + (
+        " with the diffusion stage" if diffusion else "")
     if decks.ref_available():
         # primitives for the restart reader
         rho_k, rho, m, E = U0[:NS], U0[NS], U0[NS + 1:NS + 4], U0[NS + 4]
@@ -152,10 +171,17 @@ def cpu_baseline(nx=96, ny=48, nz=48, steps=2):
             if r.returncode != 0:
                 return {"error": "reference run failed"}
             ms, parts = _ref_stage_ms(root)
-        return {"value": n * steps / (ms * 1e-3) / 1e6, "unit": "Mcell-updates/s", "cores": 1,
-                "kind": "reference", "wall_s_incl_io": wall, "stage_ms": ms,
-                "sample": sample + ", serial reference build (-O2, one process, MPI "
-                "decomposition collapsed); time = sum of its stage timers"}
+        out = {"value": n * steps / (ms * 1e-3) / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+               "kind": "reference", "wall_s_incl_io": wall, "stage_ms": ms,
+               "sample": sample + ", serial reference build (-O2, one process, MPI "
+               "decomposition collapsed); time = sum of its stage timers"}
+        if diffusion:
+            dms = sum(v for k, v in parts.items() if "Diffusion" in k)
+            out.update(value=n * steps / ((ms + dms) * 1e-3) / 1e6, stage_ms=ms + dms,
+                       diffusion_ms=dms,
+                       diffusion_value=n * steps / (dms * 1e-3) / 1e6 if dms > 0 else None)
+            out["sample"] += " (advection stages plus its Diffusion timer)"
+        return out
     from oracle.oracle import Oracle
 
     o = Oracle(case)
@@ -168,12 +194,13 @@ def cpu_baseline(nx=96, ny=48, nz=48, steps=2):
             "kind": "port", "sample": sample + ", C restatement (oracle), one core"}
 
 
-def traffic_from_profiles():
-    """HBM bytes per stage launch from the committed rocprofv3 PMC summary."""
-    path = os.path.join(ROOT, "profiles", "stage0_kernel_rm3d_256x128x128_pmc.md")
+def traffic_from_profiles(name="stage0_kernel_rm3d_256x128x128_pmc.md",
+                          key="traffic_bytes_per_launch"):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(key + r":\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -196,6 +223,57 @@ def stream_copy_gbs():
     return 2 * (1 << 30) / (best * 1e-3) / 1e9
 
 
+def main_diffusion(a):
+    """The headline workload with the explicit diffusion stage after each
+    step: whole-step rate (mixed and FP64 paths; the stage itself is FP64 on
+    both) and the two diffusion passes timed with HIP events."""
+    import torch
+
+    from schemi_amd import cases
+    from schemi_amd.gpu import PREC_FP64, PREC_MIXED, GpuSolver
+
+    case, U0 = cases.richtmyer_meshkov_3d(nx=a.nx, ny=a.ny, nz=a.nz, seed=2024)
+    case.diffusion, case.transport = True, DIFF_TRANSPORT
+    n = case.ncells
+    res = {}
+    for prec, name in ((PREC_FP64, "fp64"), (PREC_MIXED, "mixed")):
+        s = GpuSolver(case, prec)
+        wall, avg_ms, launches = timed_run(s, U0, a.steps, a.warmup)
+        prims_ms, update_ms, nd = s.diffusion_timing()
+        s.close()
+        res[name] = {"ms_per_step": 1e3 * wall / a.steps, "value": n * a.steps / wall / 1e6,
+                     "stage_kernel_ms": avg_ms, "diffusion_prims_ms": prims_ms,
+                     "diffusion_update_ms": update_ms, "diffusion_stages": nd}
+        print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    cpu = None if a.no_cpu else cpu_baseline(diffusion=True)
+    mx = res["mixed"]
+    achieved = DIFF_BYTES_UPDATE * n / (mx["diffusion_update_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": "Mcell-updates/s, 3D RM 4.2M cells with the molecular diffusion stage",
+        "value": mx["value"], "unit": "Mcell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": mx["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 flux + f64 state, f64 diffusion",
+        "data": "synthetic (seeded 3-D Richtmyer-Meshkov initial state)",
+        "config": {"workload": f"3-D Richtmyer-Meshkov {a.nx}x{a.ny}x{a.nz} ({n} cells), "
+                               "2 species, HLLC + van Albada, SSP-RK2, explicit diffusion "
+                               f"stage (mu, D, kappa = {DIFF_TRANSPORT}), Courant step on "
+                               "device", "cells": n, "precision": "mixed"},
+        "fp64": res["fp64"], "mixed": mx,
+        "diffusion_stage_ms": mx["diffusion_prims_ms"] + mx["diffusion_update_ms"],
+        "roofline": {"kernel": "diff_update_kernel", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles("diff_update_kernel_rm3d_256x128x128.md",
+                                                      "traffic_bytes_per_launch_update")},
+        "roofline_prims": {"kernel": "diff_prims_kernel", "bound": "hbm",
+                           "achieved": DIFF_BYTES_PRIMS * n / (mx["diffusion_prims_ms"] * 1e-3)
+                           / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"},
+        "cpu_baseline": cpu,
+    }
+    line["roofline_prims"]["frac"] = line["roofline_prims"]["achieved"] / HBM_PEAK_GBS
+    print(json.dumps(line))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -207,15 +285,17 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
-    from schemi_amd import cases
-    from schemi_amd.gpu import PREC_FP64, PREC_MIXED, GpuSolver
-
     if a.copy_calib:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if a.diffusion:
+        main_diffusion(a)
+        return
+    from schemi_amd import cases
+    from schemi_amd.gpu import PREC_FP64, PREC_MIXED, GpuSolver
     case, U0 = cases.richtmyer_meshkov_3d(nx=a.nx, ny=a.ny, nz=a.nz, seed=2024)
     n = case.ncells
     res = {}

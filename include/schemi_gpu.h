@@ -256,6 +256,13 @@ int schemi_gpu_mixing_width(schemi_gpu_ctx *ctx, int32_t species, double *J, dou
 /* Average duration (ms) of the last stage-kernel launches timed with HIP
  * events on the context's stream (0 when none were timed); enables timing
  * when enable != 0. */
+/* Average durations (ms) of the two passes of the diffusion stages run while
+ * timing was enabled (schemi_gpu_timing), measured with HIP events on the
+ * context's stream; clears them.  No counterpart in the reference (its
+ * Diffusion stage keeps a wall-clock total, Diffusion.cpp:1056-1059). */
+int schemi_gpu_diffusion_timing(schemi_gpu_ctx *ctx, double *prims_ms, double *update_ms,
+                                int64_t *launches);
+
 int schemi_gpu_timing(schemi_gpu_ctx *ctx, int32_t enable, double *avg_ms,
                       int64_t *launches);
 

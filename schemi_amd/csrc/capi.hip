@@ -195,6 +195,7 @@ struct schemi_gpu_ctx {
     bool diffusion = false;      // explicit molecular diffusion stage after each step
     double mu = 0, D = 0, kappa = 0;
     double *Wd = nullptr;        // diffusion: (2 Ns + 5) x n primitives
+    std::vector<hipEvent_t> evd; // timed diffusion stages: start, between passes, end
 };
 
 namespace {
@@ -292,8 +293,20 @@ int one_step(schemi_gpu_ctx *c)
         L.ns = c->ns;
         L.n = c->n;
         L.stream = c->stream;
+        L.mid = nullptr;
+        hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+        if (c->timing) {
+            for (hipEvent_t &x : e)
+                HIPCHK(hipEventCreate(&x));
+            HIPCHK(hipEventRecord(e[0], c->stream));
+            L.mid = e[1];
+        }
         if (launch_diffusion(L))
             return SCHEMI_ERR_SYSTEM;
+        if (c->timing) {
+            HIPCHK(hipEventRecord(e[2], c->stream));
+            c->evd.insert(c->evd.end(), e, e + 3);
+        }
     }
     hipLaunchKernelGGL(finalize_step_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, c->dP,
                        c->const_dt, c->hancock ? 1 : 0);
@@ -731,6 +744,8 @@ void schemi_gpu_destroy(schemi_gpu_ctx *c)
         (void)hipStreamSynchronize(c->stream);
     for (hipEvent_t e : c->ev)
         (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->evd)
+        (void)hipEventDestroy(e);
     (void)hipFree(c->dP);
     (void)hipFree(c->dgeo);
     (void)hipFree(c->U[0]);
@@ -1012,6 +1027,35 @@ int schemi_gpu_timing(schemi_gpu_ctx *c, int32_t enable, double *avg_ms, int64_t
         (void)hipEventDestroy(e);
     c->ev.clear();
     c->timing = enable != 0;
+    return SCHEMI_OK;
+}
+
+int schemi_gpu_diffusion_timing(schemi_gpu_ctx *c, double *prims_ms, double *update_ms,
+                                int64_t *launches)
+{
+    if (!c)
+        return SCHEMI_ERR_INIT;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double t0 = 0, t1 = 0;
+    int64_t cnt = 0;
+    for (size_t q = 0; q + 2 < c->evd.size(); q += 3) {
+        float a = 0, b = 0;
+        if (hipEventElapsedTime(&a, c->evd[q], c->evd[q + 1]) == hipSuccess &&
+            hipEventElapsedTime(&b, c->evd[q + 1], c->evd[q + 2]) == hipSuccess) {
+            t0 += a;
+            t1 += b;
+            ++cnt;
+        }
+    }
+    if (prims_ms)
+        *prims_ms = cnt ? t0 / cnt : 0.0;
+    if (update_ms)
+        *update_ms = cnt ? t1 / cnt : 0.0;
+    if (launches)
+        *launches = cnt;
+    for (hipEvent_t e : c->evd)
+        (void)hipEventDestroy(e);
+    c->evd.clear();
     return SCHEMI_OK;
 }
 

@@ -88,6 +88,7 @@ struct DiffusionLaunch {
     int dim, ns;
     long long n;
     hipStream_t stream;
+    hipEvent_t mid;  // recorded between the two passes when not null
 };
 int launch_diffusion(const DiffusionLaunch &L);
 

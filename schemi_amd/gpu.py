@@ -73,7 +73,8 @@ _SYMBOLS = ("schemi_gpu_create", "schemi_gpu_destroy", "schemi_gpu_set_state",
             "schemi_gpu_stage", "schemi_gpu_face_flux", "schemi_gpu_device_state",
             "schemi_gpu_sync", "schemi_gpu_timing", "schemi_gpu_strerror",
             "schemi_gpu_face_permutation", "schemi_gpu_axis_table", "schemi_gpu_state_device",
-            "schemi_gpu_adjoint_step", "schemi_gpu_mixing_width")
+            "schemi_gpu_adjoint_step", "schemi_gpu_mixing_width",
+            "schemi_gpu_diffusion_timing")
 
 
 def _lib():
@@ -114,6 +115,8 @@ def _lib():
         L.schemi_gpu_mixing_width.argtypes = [vp, C.c_int32, dp, vp]
         L.schemi_gpu_timing.restype = C.c_int
         L.schemi_gpu_timing.argtypes = [vp, C.c_int32, dp, lp]
+        L.schemi_gpu_diffusion_timing.restype = C.c_int
+        L.schemi_gpu_diffusion_timing.argtypes = [vp, dp, dp, lp]
         L.schemi_gpu_strerror.restype = C.c_char_p
         L.schemi_gpu_strerror.argtypes = [C.c_int]
         L.schemi_gpu_face_permutation.restype = C.c_int
@@ -345,6 +348,14 @@ class GpuSolver:
         self._check(_lib().schemi_gpu_timing(self._ctx, 1 if enable else 0, C.byref(avg),
                                              C.byref(cnt)), "timing")
         return avg.value, cnt.value
+
+    def diffusion_timing(self):
+        """Average (prims pass ms, update pass ms, stages) of the diffusion
+        stages timed while timing was on."""
+        a, b, cnt = C.c_double(), C.c_double(), C.c_int64()
+        self._check(_lib().schemi_gpu_diffusion_timing(self._ctx, C.byref(a), C.byref(b),
+                                                       C.byref(cnt)), "diffusion_timing")
+        return a.value, b.value, cnt.value
 
 
 _LIBM = None

@@ -93,11 +93,12 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         "#Defines_universal_gas_constant", "1" if case.R == 1.0 else "SI",
         "#Equation_of_state", _EOS[int(case.gas_model)],
         "#TVD_limiter_type", _LIM[int(case.limiter)],
-        "#Number_of_iterations_of_SLE_matrix_solver", "2000",
+        "#Number_of_iterations_of_SLE_matrix_solver", str(int(getattr(case, "sle_iters", 2000))),
         "#Diffusion_flag", "on" if getattr(case, "diffusion", False) else "off",
         "#Turbulence_flag", "off",
         "#Type_of_SLE_solver_flag",
-        "explicit" if getattr(case, "diffusion", False) else "GaussSeidel",
+        getattr(case, "sle_solver", "explicit") if getattr(case, "diffusion", False)
+        else "GaussSeidel",
         "#Type_of_source_flag", "BHR",
         "#Type_of_flow_solver", _FLUX[int(case.flux)],
         "#Dimensions_of_the_task", dims,

@@ -52,6 +52,8 @@ class _Config(C.Structure):
         ("mu", C.c_double),
         ("D", C.c_double),
         ("kappa", C.c_double),
+        ("sle_solver", C.c_int),
+        ("sle_iters", C.c_long),
     ]
 
 
@@ -145,6 +147,8 @@ def make_config(case) -> _Config:
     cfg.kb_coeff = float(case.kataoka_b[1])
     cfg.diffusion = 1 if getattr(case, "diffusion", False) else 0
     cfg.mu, cfg.D, cfg.kappa = (float(x) for x in getattr(case, "transport", (0, 0, 0)))
+    cfg.sle_solver = {"explicit": 0, "Jacobi": 1}[getattr(case, "sle_solver", "explicit")]
+    cfg.sle_iters = int(getattr(case, "sle_iters", 2000))
     cfg.flux = int(case.flux)
     cfg.limiter = int(case.limiter)
     cfg.gravity = 1 if case.gravity else 0

@@ -1239,7 +1239,9 @@ or_solver *or_create(const or_config *cfg)
             free(s);
             return NULL;
         }
-    if (cfg->diffusion && cfg->gas_model != OR_GAS_IDEAL) {
+    if (cfg->diffusion && (cfg->gas_model != OR_GAS_IDEAL ||
+                           (cfg->sle_solver != OR_SLE_EXPLICIT &&
+                            cfg->sle_solver != OR_SLE_JACOBI) || cfg->sle_iters < 1)) {
         free(s);
         return NULL;
     }
@@ -2310,6 +2312,249 @@ static int ghost_conc(const or_solver *s, long f, long o, double *cg /* c0..cNs 
     return 0;
 }
 
+/* generateDTimeLaplacian (SLEMatrix.cpp:410-547 scalar, 706-908 per vector
+ * component): diag = rnew (1 / dt) + sum_f (k_f / |d|) (N S / V & d / |d|),
+ * b = rold (1 / dt) + the same times the ghost value on boundary faces, and
+ * off[6 i + j] = -(k_f / |d|) (...) for the inner face j of cell i
+ * (surfacesOfCells order; 0 on boundary faces).  k: coefficient per face;
+ * ghost: ghost value per face. */
+static void laplacian_matrix(const or_solver *s, const double *k, const double *rnew,
+                             const double *rold, const double *ghost, double dt, double *diag,
+                             double *b, double *off)
+{
+    const double rdt = 1 / dt;
+    for (long i = 0; i < s->nc; ++i) {
+        diag[i] = rnew[i] * rdt;
+        b[i] = rold[i] * rdt;
+        for (int j = 0; j < s->cnf[i]; ++j) {
+            const long f = s->cf[6 * i + j];
+            double on[3], dR[3];
+            for (int a = 0; a < 3; ++a)
+                on[a] = s->fN[3 * f + a] * s->fS[f] / s->cV[i];
+            off[6 * i + j] = 0;
+            if (s->fpatch[f] < 0) {
+                const long o = (s->own[f] == i) ? s->nei[f] : s->own[f];
+                if (s->own[f] != i)
+                    for (int a = 0; a < 3; ++a)
+                        on[a] = on[a] * -1;
+                for (int a = 0; a < 3; ++a)
+                    dR[a] = s->crc[3 * o + a] - s->crc[3 * i + a];
+                const double m = or_hypot3(dR[0], dR[1], dR[2]);
+                double dn[3];
+                for (int a = 0; a < 3; ++a)
+                    dn[a] = dR[a] / m;
+                const double dt_ = dot3(on, dn);
+                diag[i] += k[f] / m * dt_;
+                off[6 * i + j] = -k[f] / m * dt_;
+            } else {
+                for (int a = 0; a < 3; ++a)
+                    dR[a] = (s->frc[3 * f + a] - s->crc[3 * i + a]) * 2;
+                const double m = or_hypot3(dR[0], dR[1], dR[2]);
+                double dn[3];
+                for (int a = 0; a < 3; ++a)
+                    dn[a] = dR[a] / m;
+                const double dt_ = dot3(on, dn);
+                diag[i] += k[f] / m * dt_;
+                b[i] += k[f] / m * dt_ * ghost[f];
+            }
+        }
+    }
+}
+
+/* JacobiSolver::algorithm (JacobiSolver.cpp:12-84) with
+ * relativeIterationDifference and normalize (abstractMatrixSolver.cpp:22-38):
+ * the last sweep of each iteration runs the rows from the last to the first
+ * and sums each row's lower, then upper, triangle from its last entry. */
+static void jacobi(const or_solver *s, const double *diag, const double *b, const double *off,
+                   const double *x0, long maxit, double *x)
+{
+    const long nc = s->nc;
+    double *old = malloc(nc * sizeof(double));
+    memcpy(old, x0, nc * sizeof(double));
+    long it = 0;
+    for (;;) {
+        ++it;
+        for (long i = 0; i < nc; ++i) {
+            const double aii = 1. / diag[i];
+            double row = 0;
+            long lo[6], up[6];
+            double alo[6], aup[6];
+            int nl = 0, nu = 0;
+            for (int j = 0; j < s->cnf[i]; ++j) {
+                const long f = s->cf[6 * i + j];
+                if (s->fpatch[f] >= 0)
+                    continue;
+                const long o = (s->own[f] == i) ? s->nei[f] : s->own[f];
+                if (o < i) {
+                    lo[nl] = o;
+                    alo[nl++] = off[6 * i + j];
+                } else {
+                    up[nu] = o;
+                    aup[nu++] = off[6 * i + j];
+                }
+            }
+            for (int j = nl - 1; j >= 0; --j)
+                row -= alo[j] * old[lo[j]];
+            for (int j = nu - 1; j >= 0; --j)
+                row -= aup[j] * old[up[j]];
+            x[i] = (b[i] + row) * aii;
+        }
+        double mx = 0;
+        for (long i = 0; i < nc; ++i) {
+            const double e = fabs((x[i] - old[i]) / (fabs(x[i]) + STAB));
+            if (i == 0 || mx < e)
+                mx = e;
+        }
+        const double diff = 100. * mx;
+        if (diff < 1E-12 || it >= maxit)
+            break;
+        memcpy(old, x, nc * sizeof(double));
+    }
+    for (long i = 0; i < nc; ++i)
+        if (fabs(x[i]) < DBL_EPSILON)
+            x[i] = 0;
+    free(old);
+}
+
+/* The stage with the Jacobi matrix solver (Diffusion.cpp:318-330, 431-458,
+ * 487-499, 711-760, 795-806, 818-847 with msolver.solve), from the explicit
+ * face terms of diffusion_explicit: mass fractions first, then the new
+ * concentrations, velocity (plus the explicit div(mu strainSurf1)) and
+ * temperature (plus the viscous heating). */
+static int diffusion_implicit(or_solver *s, double dt, const double *Y, const double *CCVold,
+                              const double *gradV, const double *divV, const double *srho,
+                              const double *FS, const double *GH)
+{
+    const or_config *cfg = &s->cfg;
+    or_state *st = &s->st;
+    const int Ns = cfg->Ns;
+    const long nc = s->nc, nf = s->nf;
+    double *kf = malloc(nf * sizeof(double));
+    double *rn = malloc(nc * sizeof(double)), *ro = malloc(nc * sizeof(double));
+    double *x0 = malloc(nc * sizeof(double));
+    double *diag = malloc(nc * sizeof(double)), *b = malloc(nc * sizeof(double));
+    double *off = malloc(6 * nc * sizeof(double));
+    double *Yn = malloc(Ns * nc * sizeof(double));
+    double *cn = malloc((Ns + 1) * nc * sizeof(double));
+    double *CCVnew = malloc(nc * sizeof(double));
+    double *vn = malloc(3 * nc * sizeof(double)), *Tn = malloc(nc * sizeof(double));
+    int err = 0;
+    /* mass fractions: rho Y_k^new = rho Y_k^old + dt div(rho D grad Y_k) */
+    for (int k = 0; k < Ns; ++k) {
+        for (long f = 0; f < nf; ++f)
+            kf[f] = srho[f] * cfg->D;
+        for (long i = 0; i < nc; ++i) {
+            rn[i] = st->rho[0][i];
+            ro[i] = Y[i * Ns + k] * st->rho[0][i];
+            x0[i] = Y[i * Ns + k];
+        }
+        laplacian_matrix(s, kf, rn, ro, GH + k * nf, dt, diag, b, off);
+        jacobi(s, diag, b, off, x0, cfg->sle_iters, Yn + k * nc);
+    }
+    for (long i = 0; i < nc; ++i) {
+        double sum = 0.0;
+        for (int k = 0; k < Ns; ++k) {
+            if (Yn[k * nc + i] < 0.)
+                Yn[k * nc + i] = 0.0;
+            sum += Yn[k * nc + i];
+        }
+        double c[OR_MAXS + 1];
+        c[0] = 0;
+        for (int k = 0; k < Ns; ++k) {
+            Yn[k * nc + i] /= sum;
+            c[k + 1] = Yn[k * nc + i] * st->rho[0][i] / cfg->M[k];
+            c[0] += c[k + 1];
+        }
+        for (int k = 0; k <= Ns; ++k)
+            cn[k * nc + i] = c[k];
+        CCVnew[i] = cv_mix(cfg, c) * c[0];
+    }
+    /* velocity components */
+    for (long f = 0; f < nf; ++f)
+        kf[f] = cfg->mu;
+    for (int a = 0; a < 3; ++a) {
+        for (long i = 0; i < nc; ++i) {
+            rn[i] = st->rho[0][i];
+            ro[i] = st->v[a][i] * st->rho[0][i];
+            x0[i] = st->v[a][i];
+        }
+        laplacian_matrix(s, kf, rn, ro, GH + (Ns + 1 + a) * nf, dt, diag, b, off);
+        for (long i = 0; i < nc; ++i) {
+            double dS = 0;
+            for (int j = 0; j < s->cnf[i]; ++j) {
+                const long f = s->cf[6 * i + j];
+                double n[3];
+                out_normal(s, f, i, n);
+                const double *Ts = &FS[9 * f + 3 * a];
+                dS += (Ts[0] * n[0] + Ts[1] * n[1] + Ts[2] * n[2]) * s->fS[f];
+            }
+            b[i] += dS / s->cV[i];
+        }
+        jacobi(s, diag, b, off, x0, cfg->sle_iters, vn + a * nc);
+    }
+    /* temperature */
+    for (long f = 0; f < nf; ++f)
+        kf[f] = cfg->kappa;
+    for (long i = 0; i < nc; ++i) {
+        rn[i] = CCVnew[i];
+        ro[i] = CCVold[i] * st->T[i];
+        x0[i] = st->T[i];
+    }
+    laplacian_matrix(s, kf, rn, ro, GH + Ns * nf, dt, diag, b, off);
+    for (long i = 0; i < nc; ++i) {
+        const double *G = &gradV[9 * i];
+        double sv[9];
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c)
+                sv[3 * a + c] = G[3 * c + a] + G[3 * a + c];
+        const double lam = (2. / 3.) * divV[i];
+        sv[0] -= lam;
+        sv[4] -= lam;
+        sv[8] -= lam;
+        double heat = 0;
+        for (int q = 0; q < 9; ++q)
+            heat += (sv[q] * cfg->mu) * G[q];
+        b[i] += heat;
+    }
+    jacobi(s, diag, b, off, x0, cfg->sle_iters, Tn);
+    /* recalculation of the cell values (Diffusion.cpp:963-1029) */
+    for (long i = 0; i < nc; ++i) {
+        double c[OR_MAXS + 1];
+        for (int k = 0; k <= Ns; ++k) {
+            c[k] = cn[k * nc + i];
+            st->c[k][i] = c[k];
+        }
+        const double rho = st->rho[0][i];
+        for (int k = 1; k <= Ns; ++k)
+            st->rho[k][i] = c[k] * cfg->M[k - 1];
+        double v[3];
+        for (int a = 0; a < 3; ++a) {
+            v[a] = vn[a * nc + i];
+            st->v[a][i] = v[a];
+            st->m[a][i] = v[a] * rho;
+        }
+        st->T[i] = Tn[i];
+        st->U[i] = (cv_mix(cfg, c) * Tn[i]) * c[0];
+        st->p[i] = eos_p(cfg, c, st->U[i]);
+        st->E[i] = (st->U[i] + rho * dot3(v, v) * 0.5) + 0.0;
+        if (Tn[i] < 0.)
+            err = ERR_NEG_T;
+    }
+    free(kf);
+    free(rn);
+    free(ro);
+    free(x0);
+    free(diag);
+    free(b);
+    free(off);
+    free(Yn);
+    free(cn);
+    free(CCVnew);
+    free(vn);
+    free(Tn);
+    return err;
+}
+
 static int diffusion_explicit(or_solver *s, double dt)
 {
     const or_config *cfg = &s->cfg;
@@ -2328,6 +2573,8 @@ static int diffusion_explicit(or_solver *s, double dt)
     double *FT = malloc(3 * nf * sizeof(double));      /* -kappa surfGrad(T) */
     double *FV = malloc(9 * nf * sizeof(double));      /* -mu surfGrad(v) */
     double *FS = malloc(9 * nf * sizeof(double));      /* mu strainSurf1 */
+    /* boundary ghost values per face: Y_k (Ns), T, v (3) */
+    double *GH = malloc((Ns + 4) * nf * sizeof(double));
     double cc[OR_MAXS + 1];
 
     /* mass fractions rho_k / rho, normalised (diffusiveFields.cpp:30-47) */
@@ -2409,6 +2656,8 @@ static int diffusion_explicit(or_solver *s, double dt)
                 sumd += cg[k] * cfg->M[k - 1];
             for (int k = 0; k < Ns; ++k)
                 qn[k] = (cg[k + 1] * cfg->M[k]) / sumd;
+            for (int k = 0; k < Ns; ++k)
+                GH[k * nf + f] = qn[k];
         }
         surf_grad(s, f, Ns, qo, qn, g);
         for (int k = 0; k < Ns; ++k) {
@@ -2427,6 +2676,7 @@ static int diffusion_explicit(or_solver *s, double dt)
                 goto done;
             }
             qn[0] = eos_T(cfg, cg, eos_U(cfg, cg, pg));
+            GH[Ns * nf + f] = qn[0];
         }
         surf_grad(s, f, 1, qo, qn, g);
         for (int b = 0; b < 3; ++b)
@@ -2443,6 +2693,8 @@ static int diffusion_explicit(or_solver *s, double dt)
                 err = ERR_BC;
                 goto done;
             }
+            for (int a = 0; a < 3; ++a)
+                GH[(Ns + 1 + a) * nf + f] = qn[a];
         }
         surf_grad(s, f, 3, qo, qn, g);
         double divs;
@@ -2480,6 +2732,10 @@ static int diffusion_explicit(or_solver *s, double dt)
                     t -= lam;
                 FS[9 * f + 3 * a + b] = t * cfg->mu;
             }
+    }
+    if (cfg->sle_solver == OR_SLE_JACOBI) {
+        err = diffusion_implicit(s, dt, Y, CCVold, gradV, divV, srho, FS, GH);
+        goto done;
     }
     /* cell updates (SLEMatrix.cpp:2054-2142 explicit Laplacians, then
      * Diffusion.cpp:431-458, 462-483, 599-615, 711-760, 795-806, 818-828,
@@ -2572,6 +2828,7 @@ done:
     free(FT);
     free(FV);
     free(FS);
+    free(GH);
     return err;
 }
 

@@ -75,6 +75,10 @@ class Case:
     # heat conduction coefficient)
     diffusion: bool = False
     transport: tuple = (2e-5, 2e-5, 2e-2)
+    # its matrix solver (#Type_of_SLE_solver_flag): "explicit" or "Jacobi",
+    # and the iteration limit (#Number_of_iterations_of_SLE_matrix_solver)
+    sle_solver: str = "explicit"
+    sle_iters: int = 2000
 
     def __post_init__(self):
         if not self.bc_c:

@@ -45,6 +45,8 @@ def case_json(case: Case) -> str:
         eos["step_solver"] = int(case.step_solver)
     if case.diffusion:
         eos.update(diffusion=True, transport=list(case.transport))
+        if case.sle_solver != "explicit":
+            eos.update(sle_solver=case.sle_solver, sle_iters=int(case.sle_iters))
     return json.dumps({**eos,
         "dim": case.dim, "N": list(case.N), "L": list(case.L), "M": list(case.M),
         "Cv": list(case.Cv), "R": case.R, "flux": int(case.flux),
@@ -333,6 +335,37 @@ def diffusion_cases(base):
               restart=seeded_primitives(c, 76))
 
 
+def jacobi_cases(base):
+    """The diffusion stage with the Jacobi matrix solver
+    (#Type_of_SLE_solver_flag Jacobi): implicit Laplacians solved to the
+    reference's tolerance.  Larger coefficients than the explicit cases, so
+    D dt / dx^2 is about 0.1-1 and the solves take several iterations."""
+    tr = dict(diffusion=True, transport=(0.02, 0.1, 0.01), sle_solver="Jacobi")
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, **base, **tr)
+    run_const("seeded_1d_jacobi_vanleer", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 81))
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), flux=FS.HLL, limiter=TL.minmod,
+             **base, **tr)
+    c.set_bc(0, BCT.slip)
+    c.set_bc(1, BCT.fixedValueCell, "p", (1.2, 0, 0))
+    c.set_bc(1, BCT.fixedValueCell, "v", (0.05, 0, 0))
+    c.set_bc(1, BCT.fixedValueCell, "c", (0.03, 0, 0))
+    run_const("seeded_1d_jacobi_slip_fixed", c, 5, 2e-4, (1, 5),
+              restart=seeded_primitives(c, 82))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             **base, **tr)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_jacobi_hll", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 83))
+    c = Case(dim=3, N=(8, 6, 6), L=(1.0, 0.5, 0.35), limiter=TL.vanAlbada, **base, **tr)
+    for pt in (2, 5):
+        c.set_bc(pt, BCT.slip)
+    run_const("seeded_3d_jacobi_lz", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 84))
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, step_solver=0,
+             sle_iters=3, **base, **tr)
+    run_const("seeded_1d_jacobi_hancock_capped", c, 5, 2e-4, (1, 5),
+              restart=seeded_primitives(c, 85))
+
+
 if __name__ == "__main__":
     b = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
     if len(sys.argv) > 1 and sys.argv[1] == "lz":
@@ -349,5 +382,7 @@ if __name__ == "__main__":
         kataoka_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "diffusion":
         diffusion_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "jacobi":
+        jacobi_cases(b)
     else:
         main()

@@ -30,7 +30,9 @@ def load(name):
                 kataoka_b=tuple(d.get("kataoka_b", ("critical", 1.0))),
                 step_solver=d.get("step_solver", 1),
                 diffusion=bool(d.get("diffusion", False)),
-                transport=tuple(d.get("transport", (2e-5, 2e-5, 2e-2))))
+                transport=tuple(d.get("transport", (2e-5, 2e-5, 2e-2))),
+                sle_solver=d.get("sle_solver", "explicit"),
+                sle_iters=int(d.get("sle_iters", 2000)))
     snaps = {int(s): {k: z[f"s{s}_{k}"] for k in KEYS} for s in z["steps"]}
     out = {"case": case, "dts": z["dts"], "snaps": snaps}
     if "times" in z:

@@ -166,7 +166,17 @@ typedef struct {
     double transport_mu;
     double transport_D;
     double transport_kappa;
+    /* matrix solver of the stage (matrixSolverEnum.hpp:15-25 values):
+     * SCHEMI_SLE_EXPLICIT, or SCHEMI_SLE_JACOBI (JacobiSolver.cpp: implicit
+     * Laplacians solved to the reference's tolerance, at most
+     * diffusion_iterations iterations, #Number_of_iterations_of_SLE_matrix_solver;
+     * the host waits for the solves once per batch of iterations). */
+    int32_t diffusion_solver;
+    int32_t diffusion_iterations;
 } schemi_scheme_desc;
+
+#define SCHEMI_SLE_EXPLICIT 3
+#define SCHEMI_SLE_JACOBI 6
 
 typedef struct schemi_gpu_ctx schemi_gpu_ctx;
 

@@ -196,6 +196,10 @@ struct schemi_gpu_ctx {
     double mu = 0, D = 0, kappa = 0;
     double *Wd = nullptr;        // diffusion: (2 Ns + 5) x n primitives
     std::vector<hipEvent_t> evd; // timed diffusion stages: start, between passes, end
+    int sle_iters = 0;           // Jacobi solver: iteration limit (0: explicit solver)
+    double *Mb = nullptr, *dmax = nullptr;
+    int *done = nullptr;
+    int done_host[16] = {};
 };
 
 namespace {
@@ -294,6 +298,17 @@ int one_step(schemi_gpu_ctx *c)
         L.n = c->n;
         L.stream = c->stream;
         L.mid = nullptr;
+        L.Mb = c->Mb;
+        L.dmax = c->dmax;
+        L.done = c->done;
+        L.done_host = c->done_host;
+        L.maxit = c->sle_iters;
+        if (c->Mb) {
+            HIPCHK(hipMemsetAsync(c->dmax, 0,
+                                  sizeof(double) * (size_t)(c->ns + 4) * (size_t)(c->sle_iters + 2),
+                                  c->stream));
+            HIPCHK(hipMemsetAsync(c->done, 0, sizeof(int) * (size_t)(c->ns + 4), c->stream));
+        }
         hipEvent_t e[3] = {nullptr, nullptr, nullptr};
         if (c->timing) {
             for (hipEvent_t &x : e)
@@ -454,7 +469,10 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         return SCHEMI_ERR_INIT;
     if (scheme->diffusion &&
         (mix->gas_model != SCHEMI_GAS_IDEAL || !(scheme->transport_mu >= 0.0) ||
-         !(scheme->transport_D >= 0.0) || !(scheme->transport_kappa >= 0.0)))
+         !(scheme->transport_D >= 0.0) || !(scheme->transport_kappa >= 0.0) ||
+         (scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT &&
+          scheme->diffusion_solver != SCHEMI_SLE_JACOBI) ||
+         (scheme->diffusion_solver == SCHEMI_SLE_JACOBI && scheme->diffusion_iterations < 1)))
         return SCHEMI_ERR_INIT;
 
     schemi_gpu_ctx *c = new (std::nothrow) schemi_gpu_ctx();
@@ -470,6 +488,9 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     c->mu = scheme->transport_mu;
     c->D = scheme->transport_D;
     c->kappa = scheme->transport_kappa;
+    c->sle_iters = (c->diffusion && scheme->diffusion_solver == SCHEMI_SLE_JACOBI)
+                       ? scheme->diffusion_iterations
+                       : 0;
     DevParams &P = c->hp;
     std::memset(&P, 0, sizeof P);
     P.Ns = mix->ns;
@@ -706,6 +727,12 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     if (ok && c->diffusion)
         ok = hipMalloc(&c->Wd, sizeof(double) * (size_t)(2 * mix->ns + 5) * (size_t)c->n) ==
              hipSuccess;
+    if (ok && c->sle_iters > 0)
+        ok = hipMalloc(&c->Mb, sizeof(double) * diffusion_matrix_doubles(mix->ns) *
+                                   (size_t)c->n) == hipSuccess &&
+             hipMalloc(&c->dmax, sizeof(double) * (size_t)(mix->ns + 4) *
+                                     (size_t)(c->sle_iters + 2)) == hipSuccess &&
+             hipMalloc(&c->done, sizeof(int) * (size_t)(mix->ns + 4)) == hipSuccess;
     if (ok && c->mode == MODE_GENERAL) {
         const size_t tsz = (c->prec == SCHEMI_PREC_MIXED) ? sizeof(float) : sizeof(double);
         const size_t nq = (size_t)(mix->ns + 4) * (size_t)c->n;
@@ -755,6 +782,9 @@ void schemi_gpu_destroy(schemi_gpu_ctx *c)
     (void)hipFree(c->scratch);
     (void)hipFree(c->adj);
     (void)hipFree(c->Wd);
+    (void)hipFree(c->Mb);
+    (void)hipFree(c->dmax);
+    (void)hipFree(c->done);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;

@@ -25,6 +25,8 @@
 //    but the Courant maximum of the new state.
 // Every face normal is +-1 along its axis; a face term of the reference
 // (F & N) S then reduces to F_a N_a S plus signed zeros, which are dropped.
+#include <cfloat>
+
 #include "fv_launch.hpp"
 #include "fv_stage.hpp"
 #include "fv_general.hpp"
@@ -127,6 +129,19 @@ diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const doubl
     W[L::P * n + id] = w[NS + 3];
 }
 
+// Jacobi matrix solver (implicit Laplacians, SLEMatrix.cpp:410-547, 706-908):
+// per cell, field-major (q n + id): the mass-fraction systems' diagonal
+// (shared by the species: one rho D), right-hand sides and face coefficients
+// (k_f / |d|) (N S / V & d / |d|) per face slot; the same for the velocity
+// components (mu) and the temperature (kappa; its diagonal needs the new
+// concentrations), the new concentrations, and two iterate buffers per field
+// (Y_k, v_c, T).
+template <int NS> struct MB {
+    static constexpr int DY = 0, BY = 1, TY = 1 + NS, DV = 7 + NS, BV = 8 + NS, TV = 11 + NS,
+                         BT = 17 + NS, TT = 18 + NS, DT = 24 + NS, CN = 25 + NS,
+                         X = 25 + 2 * NS, NF = NS + 4, N = X + 2 * NF;
+};
+
 template <int NS>
 __device__ __forceinline__ void load_w(const double *W, long long n, long long id, double *w)
 {
@@ -135,12 +150,13 @@ __device__ __forceinline__ void load_w(const double *W, long long n, long long i
         w[q] = W[q * n + id];
 }
 
-template <int NS, int DIM, bool MIXED>
+template <int NS, int DIM, bool MIXED, bool IMPL>
 __global__ void __launch_bounds__(NT)
 diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo, double *U,
                    double *Ualt, int hk, const double *__restrict__ W, double *sc,
-                   int *errflag, double mu, double Dc, double kappa)
+                   int *errflag, double mu, double Dc, double kappa, double *Mb)
 {
+    using M = MB<NS>;
     using L = DW<NS>;
     constexpr int NW = L::NW;
     __shared__ double sRed[NT / 64];
@@ -164,6 +180,30 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         for (int k = 0; k < NS; ++k)
             dY[k] = 0;
         int err = 0;
+        // implicit Laplacians: diagonals and right-hand sides start from the
+        // time derivative, rnew (1 / dt) and rold (1 / dt)
+        double diagY = 0, diagV = 0, bY[NS], bV[3], bT = 0, tY[6], tV[6], tT[6];
+        double rdt = 0, rho_i = 0;
+        if constexpr (IMPL) {
+            rdt = 1 / dt;
+            rho_i = Us[NS * n + id];
+            diagY = rho_i * rdt;
+            diagV = rho_i * rdt;
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+                bY[k] = (w0[L::Y + k] * rho_i) * rdt;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                bV[c] = (w0[L::V + c] * rho_i) * rdt;
+            double c0o = 0;
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+                c0o += w0[L::C + k];
+            bT = ((c0o * cv_mix<NS>(P, w0 + L::C, c0o)) * w0[L::T]) * rdt;
+#pragma unroll
+            for (int s = 0; s < 6; ++s)
+                tY[s] = tV[s] = tT[s] = 0;
+        }
 #if SG_DIFF_UNROLL
 #pragma unroll
 #else
@@ -185,6 +225,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             // owner / neighbour values (ghost for a boundary face)
             double qo[NW], qn[NW];
             double gY[NS][3], gT[3], gv[3][3], divs;
+            double mf = 1, dotf = 0;  // |d| and (N S / V) & d / |d| of the Laplacian
             if (inner) {
                 const long long nb = id + (hi ? P.stride[a] : -P.stride[a]);
                 double wn[NW];
@@ -206,6 +247,8 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 const double da = xc_hi - xc_lo;
                 const double m = tabs(da);
                 const double m2 = m * m;
+                mf = m;
+                dotf = cg.S[a] / cg.V;
                 double dd[3] = {0.0, 0.0, 0.0};
                 dd[a] = da;
 #pragma unroll
@@ -267,6 +310,8 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
 #pragma unroll
                 for (int b = 0; b < 3; ++b)
                     nrm[b] = D[b] / m;
+                mf = m;
+                dotf = ((na * cg.S[a]) / cg.V) * nrm[a];
                 double dq[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
@@ -307,6 +352,27 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                     t -= lam;
                 dFs[c] += ((t * mu) * na) * S;
             }
+            if constexpr (IMPL) {
+                // (k_f / |d|) ((N S / V) & (d / |d|)) into the diagonal, times
+                // the ghost value into the right-hand side of a boundary face
+                const double ty = (srho * Dc) / mf * dotf;
+                const double tv = mu / mf * dotf;
+                const double tt = kappa / mf * dotf;
+                diagY += ty;
+                diagV += tv;
+                tY[s] = ty;
+                tV[s] = tv;
+                tT[s] = tt;
+                if (!inner) {
+#pragma unroll
+                    for (int k = 0; k < NS; ++k)
+                        bY[k] += ty * qn[L::Y + k];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        bV[c] += tv * qn[L::V + c];
+                    bT += tt * qn[L::T];
+                }
+            }
         }
         if (err)
             atomicOr(errflag, ERRF_BOUNDARY);
@@ -316,6 +382,45 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         for (int q = 0; q < 9; ++q)
             G[q] = gV[q] / V;
         const double divV = dV / V;
+        if constexpr (IMPL) {
+            // explicit parts of the free terms: div(mu strainSurf1) for v,
+            // the viscous heating for T (Diffusion.cpp:711-716, 737)
+            double sv[9];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+                    sv[3 * a + b] = G[3 * b + a] + G[3 * a + b];
+            const double lam = (2. / 3.) * divV;
+            sv[0] -= lam;
+            sv[4] -= lam;
+            sv[8] -= lam;
+            double heat = 0;
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                heat += (sv[q] * mu) * G[q];
+            bT += heat;
+            Mb[M::DY * n + id] = diagY;
+            Mb[M::DV * n + id] = diagV;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) {
+                Mb[(M::BY + k) * n + id] = bY[k];
+                Mb[(M::X + 2 * k) * n + id] = w0[L::Y + k];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                Mb[(M::BV + c) * n + id] = bV[c] + dFs[c] / V;
+                Mb[(M::X + 2 * (NS + c)) * n + id] = w0[L::V + c];
+            }
+            Mb[M::BT * n + id] = bT;
+            Mb[(M::X + 2 * (NS + 3)) * n + id] = w0[L::T];
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                Mb[(M::TY + s) * n + id] = tY[s];
+                Mb[(M::TV + s) * n + id] = tV[s];
+                Mb[(M::TT + s) * n + id] = tT[s];
+            }
+        } else {
         double u[NS + 5];
 #pragma unroll
         for (int q = 0; q < NS + 5; ++q)
@@ -391,6 +496,201 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             Us[q * n + id] = un[q];
         (void)MIXED;
         sig = signal_speed<double, NS, GAS_IDEAL>(P, un, cg.minlen);
+        }
+    }
+    if constexpr (!IMPL)
+        reduce_signal<NT>(sig, sRed, reinterpret_cast<unsigned long long *>(sc) + SC_SIGMAX);
+}
+
+// Field f of the Jacobi solves: 0..NS-1 mass fractions, NS..NS+2 velocity
+// components, NS+3 temperature.
+template <int NS> struct JF {
+    __device__ static int diag(int f) { return f < NS ? MB<NS>::DY : (f < NS + 3 ? MB<NS>::DV : MB<NS>::DT); }
+    __device__ static int rhs(int f) { return f < NS ? MB<NS>::BY + f : (f < NS + 3 ? MB<NS>::BV + f - NS : MB<NS>::BT); }
+    __device__ static int coef(int f) { return f < NS ? MB<NS>::TY : (f < NS + 3 ? MB<NS>::TV : MB<NS>::TT); }
+};
+
+// Field f converged with iteration it - 1 (relativeIterationDifference < 1e-12,
+// abstractMatrixSolver.cpp:22-29) or reached the iteration limit.
+__device__ __forceinline__ bool jacobi_stops(const double *dmax, int f, int it, int maxit)
+{
+    if (it <= 1)
+        return false;
+    if (it - 1 >= maxit)
+        return true;
+    return 100. * dmax[(long long)f * (maxit + 2) + it - 1] < 1E-12;
+}
+
+// Iteration it (from 1) of the Jacobi solves of fields f0 .. f1-1
+// (JacobiSolver.cpp:12-84): x_i = (b_i + row_i) / a_ii with row_i summing
+// -a_ij x_j over the lower triangle (neighbours of smaller index: z-, y-, x-
+// from its last entry) and then the upper (z+, y+, x+); iterate it is
+// written to buffer it & 1.  done[f] holds the last iteration of a finished
+// field (0 while it runs).
+template <int NS, int DIM>
+__global__ void __launch_bounds__(NT)
+diff_jacobi_kernel(const DevParams *__restrict__ Pp, double *Mb, double *dmax, int *done,
+                   const double *sc, int f0, int f1, int it, int maxit)
+{
+    using M = MB<NS>;
+    __shared__ double sRed[NT / 64];
+    const DevParams &P = *Pp;
+    if (sc[SC_DONE] != 0.0)
+        return;
+    const long long n = P.n;
+    const long long id = blockIdx.x * (long long)NT + threadIdx.x;
+    const int Nx = P.N[0], Ny = P.N[1], Nz = P.N[2];
+    const long long Nxy = (long long)Nx * Ny;
+    for (int f = f0; f < f1; ++f) {
+        if (done[f] != 0)
+            continue;
+        if (jacobi_stops(dmax, f, it, maxit)) {
+            if (id == 0)
+                done[f] = it - 1;
+            continue;
+        }
+        double e = 0.0;
+        if (id < n) {
+            const int i = int(id % Nx), j = int((id / Nx) % Ny), k = int(id / Nxy);
+            const double *xo = Mb + (M::X + 2 * f + ((it - 1) & 1)) * n;
+            double *xn = Mb + (M::X + 2 * f + (it & 1)) * n;
+            const double *t = Mb + JF<NS>::coef(f) * n;
+            const double aii = 1. / Mb[JF<NS>::diag(f) * n + id];
+            double row = 0;
+            if (DIM == 3 && k > 0)
+                row -= (-t[4 * n + id]) * xo[id - Nxy];
+            if (DIM >= 2 && j > 0)
+                row -= (-t[3 * n + id]) * xo[id - Nx];
+            if (i > 0)
+                row -= (-t[0 * n + id]) * xo[id - 1];
+            if (DIM == 3 && k < Nz - 1)
+                row -= (-t[5 * n + id]) * xo[id + Nxy];
+            if (DIM >= 2 && j < Ny - 1)
+                row -= (-t[2 * n + id]) * xo[id + Nx];
+            if (i < Nx - 1)
+                row -= (-t[1 * n + id]) * xo[id + 1];
+            const double x = (Mb[JF<NS>::rhs(f) * n + id] + row) * aii;
+            xn[id] = x;
+            e = tabs((x - xo[id]) / (tabs(x) + STAB));
+        }
+        reduce_signal<NT>(e, sRed,
+                          reinterpret_cast<unsigned long long *>(dmax) +
+                              (long long)f * (maxit + 2) + it);
+        __syncthreads();
+    }
+}
+
+// After a batch ending before iteration it: marks the fields that stopped
+// with iteration it - 1 (every field, once the run has reached its end).
+__global__ void diff_jacobi_check_kernel(const double *dmax, int *done, const double *sc, int f0,
+                                         int f1, int it, int maxit)
+{
+    for (int f = f0; f < f1; ++f)
+        if (done[f] == 0 && (sc[SC_DONE] != 0.0 || jacobi_stops(dmax, f, it, maxit)))
+            done[f] = (sc[SC_DONE] != 0.0) ? 1 : it - 1;
+}
+
+// normalize (abstractMatrixSolver.cpp:31-38) of the solve's result
+template <int NS>
+__device__ __forceinline__ double jacobi_result(const double *Mb, const int *done, int f,
+                                                long long n, long long id)
+{
+    const double x = Mb[(MB<NS>::X + 2 * f + (done[f] & 1)) * n + id];
+    return tabs(x) < DBL_EPSILON ? 0 : x;
+}
+
+// Mass fractions from their solves: clipped at 0, renormalised, the new
+// concentrations, and the temperature system's diagonal
+// CCV_new / dt + sum of its face coefficients (Diffusion.cpp:431-479).
+template <int NS, int DIM>
+__global__ void __launch_bounds__(NT)
+diff_finish_y_kernel(const DevParams *__restrict__ Pp, const double *U, const double *Ualt,
+                     int hk, double *Mb, const int *done, const double *sc)
+{
+    using M = MB<NS>;
+    const DevParams &P = *Pp;
+    if (sc[SC_DONE] != 0.0)
+        return;
+    const long long n = P.n;
+    const long long id = blockIdx.x * (long long)NT + threadIdx.x;
+    if (id >= n)
+        return;
+    const double *Us = state_of(U, Ualt, sc, hk);
+    const double rho = Us[NS * n + id];
+    double y[NS], sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        y[k] = jacobi_result<NS>(Mb, done, k, n, id);
+        if (y[k] < 0.)
+            y[k] = 0.0;
+        sum += y[k];
+    }
+    double cn[NS], c0 = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        y[k] /= sum;
+        cn[k] = y[k] * rho / P.M[k];
+        c0 += cn[k];
+        Mb[(M::CN + k) * n + id] = cn[k];
+    }
+    const double rdt = 1 / sc[SC_DT];
+    double diag = (cv_mix<NS>(P, cn, c0) * c0) * rdt;
+#pragma unroll
+    for (int s = 0; s < 2 * DIM; ++s)
+        diag += Mb[(M::TT + s) * n + id];
+    Mb[M::DT * n + id] = diag;
+}
+
+// The recalculated cell from the solves (Diffusion.cpp:963-1029) and its
+// Courant signal speed.
+template <int NS>
+__global__ void __launch_bounds__(NT)
+diff_finish_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo, double *U,
+                   double *Ualt, int hk, const double *Mb, const int *done, double *sc,
+                   int *errflag)
+{
+    using M = MB<NS>;
+    __shared__ double sRed[NT / 64];
+    const DevParams &P = *Pp;
+    if (sc[SC_DONE] != 0.0)
+        return;
+    const long long n = P.n;
+    const long long id = blockIdx.x * (long long)NT + threadIdx.x;
+    double sig = 0.0;
+    if (id < n) {
+        double *Us = const_cast<double *>(state_of(U, Ualt, sc, hk));
+        const int i = int(id % P.N[0]), j = int((id / P.N[0]) % P.N[1]),
+                  k = int(id / P.stride[2]);
+        const CellGeo cg = cell_geo(P, geo, i, j, k);
+        const double rho = Us[NS * n + id];
+        double cn[NS], c0 = 0;
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            cn[q] = Mb[(M::CN + q) * n + id];
+            c0 += cn[q];
+        }
+        double vn[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            vn[c] = jacobi_result<NS>(Mb, done, NS + c, n, id);
+        const double Tn = jacobi_result<NS>(Mb, done, NS + 3, n, id);
+        if (Tn < 0.)
+            atomicOr(errflag, ERRF_NEG_T);
+        double un[NS + 5];
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+            un[q] = cn[q] * P.M[q];
+        un[NS] = rho;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            un[NS + 1 + c] = vn[c] * rho;
+        const double Uin = (cv_mix<NS>(P, cn, c0) * Tn) * c0;
+        const double v2 = ((0. + vn[0] * vn[0]) + vn[1] * vn[1]) + vn[2] * vn[2];
+        un[NS + 4] = (Uin + rho * v2 * 0.5) + 0.0;
+#pragma unroll
+        for (int q = 0; q < NS + 5; ++q)
+            Us[q * n + id] = un[q];
+        sig = signal_speed<double, NS, GAS_IDEAL>(P, un, cg.minlen);
     }
     reduce_signal<NT>(sig, sRed, reinterpret_cast<unsigned long long *>(sc) + SC_SIGMAX);
 }
@@ -403,8 +703,46 @@ int launch_dim(const DiffusionLaunch &L)
                        L.Ualt, L.hk, L.W, L.sc);
     if (L.mid && hipEventRecord(L.mid, L.stream) != hipSuccess)
         return 1;
-    hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false>), dim3(blocks), dim3(NT), 0, L.stream,
-                       L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc, L.err, L.mu, L.D, L.kappa);
+    if (!L.Mb) {
+        hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false>), dim3(blocks), dim3(NT), 0,
+                           L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc, L.err, L.mu, L.D,
+                           L.kappa, nullptr);
+        return hipGetLastError() == hipSuccess ? 0 : 1;
+    }
+    // Jacobi: assemble, solve Y_k and v_c, finish Y, solve T, finish; the
+    // host reads the solves' state after each batch of iterations
+    hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, true>), dim3(blocks), dim3(NT), 0,
+                       L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc, L.err, L.mu, L.D,
+                       L.kappa, L.Mb);
+    auto solve = [&](int f0, int f1) -> int {
+        int it = 1, batch = 4;
+        for (;;) {
+            for (int q = 0; q < batch; ++q, ++it)
+                hipLaunchKernelGGL((diff_jacobi_kernel<NS, DIM>), dim3(blocks), dim3(NT), 0,
+                                   L.stream, L.P, L.Mb, L.dmax, L.done, L.sc, f0, f1, it,
+                                   L.maxit);
+            hipLaunchKernelGGL(diff_jacobi_check_kernel, dim3(1), dim3(1), 0, L.stream, L.dmax,
+                               L.done, L.sc, f0, f1, it, L.maxit);
+            if (hipMemcpyAsync(L.done_host, L.done, sizeof(int) * (NS + 4),
+                               hipMemcpyDeviceToHost, L.stream) != hipSuccess ||
+                hipStreamSynchronize(L.stream) != hipSuccess)
+                return 1;
+            bool all = true;
+            for (int f = f0; f < f1; ++f)
+                all = all && L.done_host[f] != 0;
+            if (all)
+                return 0;
+            batch = batch < 64 ? 2 * batch : 64;
+        }
+    };
+    if (solve(0, NS + 3))
+        return 1;
+    hipLaunchKernelGGL((diff_finish_y_kernel<NS, DIM>), dim3(blocks), dim3(NT), 0, L.stream, L.P,
+                       L.U, L.Ualt, L.hk, L.Mb, L.done, L.sc);
+    if (solve(NS + 3, NS + 4))
+        return 1;
+    hipLaunchKernelGGL((diff_finish_kernel<NS>), dim3(blocks), dim3(NT), 0, L.stream, L.P, L.geo,
+                       L.U, L.Ualt, L.hk, L.Mb, L.done, L.sc, L.err);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -421,6 +759,18 @@ template <int NS> int launch_ns(const DiffusionLaunch &L)
 }
 
 }  // namespace
+
+size_t diffusion_matrix_doubles(int ns)
+{
+    switch (ns) {
+    case 1:
+        return MB<1>::N;
+    case 2:
+        return MB<2>::N;
+    default:
+        return MB<3>::N;
+    }
+}
 
 int launch_diffusion(const DiffusionLaunch &L)
 {

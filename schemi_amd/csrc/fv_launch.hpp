@@ -89,8 +89,17 @@ struct DiffusionLaunch {
     long long n;
     hipStream_t stream;
     hipEvent_t mid;  // recorded between the two passes when not null
+    // Jacobi matrix solver (null Mb: explicit): matrices and iterates
+    // (diffusion_matrix_doubles(ns) x n), per-iteration maxima
+    // ((ns + 4) x (maxit + 2), zeroed), finished iterations per field (device
+    // and a host copy, ns + 4, zeroed)
+    double *Mb;
+    double *dmax;
+    int *done, *done_host;
+    int maxit;
 };
 int launch_diffusion(const DiffusionLaunch &L);
+size_t diffusion_matrix_doubles(int ns);
 
 // Mixing width sum_i V_i Y (1 - Y), Y = rho_sp / rho, into *J (device), and
 // dJ/dU into grad (device, nullable); partial: (n + 255) / 256 doubles.

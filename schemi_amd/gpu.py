@@ -64,7 +64,8 @@ class SchemeDesc(C.Structure):
                 ("bc_c", (_BC * 6) * MAXS), ("bc_v", _BC * 6), ("bc_p", _BC * 6),
                 ("step_solver", C.c_int32), ("diffusion", C.c_int32),
                 ("transport_mu", C.c_double), ("transport_D", C.c_double),
-                ("transport_kappa", C.c_double)]
+                ("transport_kappa", C.c_double), ("diffusion_solver", C.c_int32),
+                ("diffusion_iterations", C.c_int32)]
 
 
 _LIB = None
@@ -195,6 +196,9 @@ def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) 
     s.step_solver = int(case.step_solver)
     s.diffusion = 1 if case.diffusion else 0
     s.transport_mu, s.transport_D, s.transport_kappa = (float(x) for x in case.transport)
+    # matrixSolver enum values (matrixSolverEnum.hpp:15-25)
+    s.diffusion_solver = {"explicit": 3, "Jacobi": 6}.get(case.sle_solver, -1)
+    s.diffusion_iterations = int(case.sle_iters)
     return s
 
 

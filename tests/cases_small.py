@@ -162,12 +162,35 @@ def cases():
     out["3d_diffusion_lz_slip"] = c
     out["3d_diffusion_linear_lz"] = Case(dim=3, N=(10, 6, 6), L=(1.0, 0.6, 0.65),
                                          limiter=TL.linear, flux=FS.HLL, **BASE, **dif)
+    # the Jacobi matrix solver (implicit Laplacians), converged and capped
+    jac = dict(diffusion=True, transport=(0.02, 0.1, 0.01), sle_solver="Jacobi")
+    out["1d_jacobi_vanLeer"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0),
+                                    limiter=TL.vanLeer, **BASE, **jac)
+    c = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.minmod, flux=FS.HLL, **BASE,
+             **jac)
+    c.set_bc(0, BCT.slip)
+    c.set_bc(1, BCT.fixedValueCell, "p", (1.2, 0, 0))
+    c.set_bc(1, BCT.fixedValueCell, "v", (0.05, -0.02, 0.01))
+    c.set_bc(1, BCT.fixedValueCell, "c", (0.03, 0, 0))
+    out["1d_jacobi_slip_fixed"] = c
+    c = Case(dim=2, N=(64, 32, 1), L=(1.0, 0.5, 0.1), limiter=TL.minmod, flux=FS.HLL, **BASE,
+             **jac)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    out["2d_jacobi_hll"] = c
+    c = Case(dim=3, N=(24, 10, 12), L=(1.0, 0.5, 0.35), limiter=TL.vanAlbada, **BASE, **jac)
+    c.set_bc(2, BCT.slip)
+    c.set_bc(5, BCT.slip)
+    out["3d_jacobi_lz_slip"] = c
+    out["1d_jacobi_capped"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0),
+                                   limiter=TL.vanLeer, sle_iters=3, **BASE, **jac)
     # MUSCL-Hancock (SecondOrder) on every mesh kind it runs on: 1-D with each
     # limiter and boundary kind, and 2-D/3-D where the slopes vanish
     # (including the reversed top face normal, 3d_periodic_x_vanLeer)
     for name in ("1d_minmod", "1d_vanLeer", "1d_vanAlbada", "1d_linear", "1d_zero", "1d_hll",
                  "1d_slip", "1d_fixed", "1d_periodic", "1d_vdw_vanLeer", "1d_rk_vanLeer",
-                 "1d_kataoka_vanLeer", "1d_diffusion_vanLeer", "2d_diffusion_hll", "2d_vanLeer",
+                 "1d_kataoka_vanLeer", "1d_diffusion_vanLeer", "2d_diffusion_hll",
+                 "1d_jacobi_vanLeer", "2d_vanLeer",
                  "2d_hll_grav", "2d_periodic_hll", "2d_zero_fixed", "2d_vdw_periodic_hll",
                  "3d_vanAlbada", "3d_hll_partial_tiles", "3d_periodic_x_vanLeer",
                  "3d_vdw_vanAlbada"):

@@ -72,15 +72,49 @@ def test_negative_temperature_reported(hip_device):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("what", ["periodic", "vdw", "negative_mu"])
+@pytest.mark.parametrize("step_solver", [1, 0])
+def test_courant_run_with_jacobi_matches_oracle(step_solver, hip_device):
+    """Device-side Courant steps with the Jacobi matrix solver (the host
+    waits for the solves once per batch of iterations) follow the oracle's
+    dt sequence and state bit for bit."""
+    from oracle.oracle import Oracle
+
+    case = Case(dim=2, N=(32, 16, 1), L=(1.0, 0.5, 0.1), limiter=TL.vanLeer,
+                step_solver=step_solver, courant=0.3, min_time=1e-6, **BASE,
+                diffusion=True, transport=(0.02, 0.1, 0.01), sle_solver="Jacobi")
+    case.set_bc(3, BCT.slip)
+    c, v, p = random_primitives(case, 33)
+    U0 = conserved_from_primitive(case, c, v, p)
+    Ns = case.Ns
+    o = Oracle(case)
+    o.set_conserved(U0[:Ns], U0[Ns], U0[Ns + 1:Ns + 4], U0[Ns + 4])
+    dt, t = case.min_time, 0.0
+    for _ in range(20):
+        assert o.step(dt, 1) == 0
+        t = t + dt
+        dt = o.courant_dt(case.courant, case.min_time)
+    s = GpuSolver(case)
+    s.set_conserved(U0)
+    t_gpu, steps, dt_gpu = s.run(20, case.min_time)
+    assert steps == 20 and t_gpu == t and dt_gpu == dt
+    assert np.array_equal(s.conserved(), o.conserved())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("what", ["periodic", "vdw", "negative_mu", "gauss_seidel",
+                                  "zero_iterations"])
 def test_unsupported_diffusion_rejected(what, hip_device):
-    """Diffusion with a periodic axis, a non-ideal mixture or a negative
-    transport coefficient is declined with initialisationError; the adjoint
-    of a step with diffusion likewise."""
+    """Diffusion with a periodic axis, a non-ideal mixture, a negative
+    transport coefficient, a matrix solver other than explicit and Jacobi, or
+    no Jacobi iterations is declined with initialisationError."""
     kw = dict(BASE)
     if what == "vdw":
         kw.update(gas_model=1, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
     tr = (-1.0, 0.02, 0.002) if what == "negative_mu" else DIF["transport"]
+    if what == "gauss_seidel":
+        kw["sle_solver"] = "GaussSeidel"
+    if what == "zero_iterations":
+        kw.update(sle_solver="Jacobi", sle_iters=0)
     case = Case(dim=2, N=(16, 8, 1), L=(1.0, 0.5, 0.1), limiter=TL.vanLeer, diffusion=True,
                 transport=tr, **kw)
     if what == "periodic":

@@ -223,7 +223,7 @@ def test_multi_d_mesh_needs_two_cells_per_axis(hip_device):
                                   "3d_rk_vanAlbada", "2d_rk_periodic_hll",
                                   "3d_kataoka_vanAlbada", "hancock_1d_kataoka_vanLeer",
                                   "2d_diffusion_hll", "3d_diffusion_lz_slip",
-                                  "hancock_1d_diffusion_vanLeer"])
+                                  "hancock_1d_diffusion_vanLeer", "2d_jacobi_hll"])
 def test_mixed_precision_all_kernels(name, hip_device):
     """The mixed path of every kernel family stays within 1e-5 (relative L1
     of the density) of the FP64 path over 3 steps and keeps sum rho_k = rho."""

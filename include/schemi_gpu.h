@@ -167,14 +167,20 @@ typedef struct {
     double transport_D;
     double transport_kappa;
     /* matrix solver of the stage (matrixSolverEnum.hpp:15-25 values):
-     * SCHEMI_SLE_EXPLICIT, or SCHEMI_SLE_JACOBI (JacobiSolver.cpp: implicit
+     * SCHEMI_SLE_EXPLICIT, SCHEMI_SLE_JACOBI (JacobiSolver.cpp: implicit
      * Laplacians solved to the reference's tolerance, at most
      * diffusion_iterations iterations, #Number_of_iterations_of_SLE_matrix_solver;
-     * the host waits for the solves once per batch of iterations). */
+     * the host waits for the solves once per batch of iterations), or
+     * SCHEMI_SLE_GAUSS_SEIDEL: the same systems solved by the parallel Jacobi
+     * sweeps to the same stop test (GaussSeidelSolver.cpp's sequential sweeps
+     * have no parallel form with the same rounding; both converge to the
+     * system's solution, so the results agree to the solver tolerance, not
+     * bit for bit). */
     int32_t diffusion_solver;
     int32_t diffusion_iterations;
 } schemi_scheme_desc;
 
+#define SCHEMI_SLE_GAUSS_SEIDEL 0
 #define SCHEMI_SLE_EXPLICIT 3
 #define SCHEMI_SLE_JACOBI 6
 

@@ -147,7 +147,8 @@ def make_config(case) -> _Config:
     cfg.kb_coeff = float(case.kataoka_b[1])
     cfg.diffusion = 1 if getattr(case, "diffusion", False) else 0
     cfg.mu, cfg.D, cfg.kappa = (float(x) for x in getattr(case, "transport", (0, 0, 0)))
-    cfg.sle_solver = {"explicit": 0, "Jacobi": 1}[getattr(case, "sle_solver", "explicit")]
+    cfg.sle_solver = {"explicit": 0, "Jacobi": 1, "GaussSeidel": 2}[
+        getattr(case, "sle_solver", "explicit")]
     cfg.sle_iters = int(getattr(case, "sle_iters", 2000))
     cfg.flux = int(case.flux)
     cfg.limiter = int(case.limiter)

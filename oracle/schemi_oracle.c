@@ -1240,8 +1240,8 @@ or_solver *or_create(const or_config *cfg)
             return NULL;
         }
     if (cfg->diffusion && (cfg->gas_model != OR_GAS_IDEAL ||
-                           (cfg->sle_solver != OR_SLE_EXPLICIT &&
-                            cfg->sle_solver != OR_SLE_JACOBI) || cfg->sle_iters < 1)) {
+                           cfg->sle_solver < OR_SLE_EXPLICIT ||
+                           cfg->sle_solver > OR_SLE_GAUSS_SEIDEL || cfg->sle_iters < 1)) {
         free(s);
         return NULL;
     }
@@ -2416,6 +2416,80 @@ static void jacobi(const or_solver *s, const double *diag, const double *b, cons
     free(old);
 }
 
+/* GaussSeidelSolver::algorithm (GaussSeidelSolver.cpp:12-91): per iteration
+ * a forward sweep (rows in order, lower triangle from the new iterate, upper
+ * from the old) and a backward sweep (rows in reverse, lower from the old
+ * iterate, upper from the new, each triangle from its last entry), every
+ * term divided as a_ij x_j / a_ii; same stop test as Jacobi. */
+static void gauss_seidel(const or_solver *s, const double *diag, const double *b,
+                         const double *off, const double *x0, long maxit, double *x)
+{
+    const long nc = s->nc;
+    double *old = malloc(nc * sizeof(double));
+    memcpy(old, x0, nc * sizeof(double));
+    memcpy(x, x0, nc * sizeof(double));
+    long it = 0;
+    for (;;) {
+        ++it;
+        for (int pass = 0; pass < 2; ++pass)
+            for (long r = 0; r < nc; ++r) {
+                const long i = pass == 0 ? r : nc - 1 - r;
+                const double aii = 1. / diag[i];
+                long lo[6], up[6];
+                double alo[6], aup[6];
+                int nl = 0, nu = 0;
+                for (int j = 0; j < s->cnf[i]; ++j) {
+                    const long f = s->cf[6 * i + j];
+                    if (s->fpatch[f] >= 0)
+                        continue;
+                    const long o = (s->own[f] == i) ? s->nei[f] : s->own[f];
+                    if (o < i) {
+                        lo[nl] = o;
+                        alo[nl++] = off[6 * i + j];
+                    } else {
+                        up[nu] = o;
+                        aup[nu++] = off[6 * i + j];
+                    }
+                }
+                double xi = b[i] * aii;
+                if (pass == 0) {
+                    for (int j = 0; j < nl; ++j)
+                        xi -= alo[j] * x[lo[j]] * aii;
+                    for (int j = 0; j < nu; ++j)
+                        xi -= aup[j] * old[up[j]] * aii;
+                } else {
+                    for (int j = nl - 1; j >= 0; --j)
+                        xi -= alo[j] * old[lo[j]] * aii;
+                    for (int j = nu - 1; j >= 0; --j)
+                        xi -= aup[j] * x[up[j]] * aii;
+                }
+                x[i] = xi;
+            }
+        double mx = 0;
+        for (long i = 0; i < nc; ++i) {
+            const double e = fabs((x[i] - old[i]) / (fabs(x[i]) + STAB));
+            if (i == 0 || mx < e)
+                mx = e;
+        }
+        if (100. * mx < 1E-12 || it >= maxit)
+            break;
+        memcpy(old, x, nc * sizeof(double));
+    }
+    for (long i = 0; i < nc; ++i)
+        if (fabs(x[i]) < DBL_EPSILON)
+            x[i] = 0;
+    free(old);
+}
+
+static void sle_solve(const or_solver *s, const double *diag, const double *b, const double *off,
+                      const double *x0, double *x)
+{
+    if (s->cfg.sle_solver == OR_SLE_GAUSS_SEIDEL)
+        gauss_seidel(s, diag, b, off, x0, s->cfg.sle_iters, x);
+    else
+        jacobi(s, diag, b, off, x0, s->cfg.sle_iters, x);
+}
+
 /* The stage with the Jacobi matrix solver (Diffusion.cpp:318-330, 431-458,
  * 487-499, 711-760, 795-806, 818-847 with msolver.solve), from the explicit
  * face terms of diffusion_explicit: mass fractions first, then the new
@@ -2449,7 +2523,7 @@ static int diffusion_implicit(or_solver *s, double dt, const double *Y, const do
             x0[i] = Y[i * Ns + k];
         }
         laplacian_matrix(s, kf, rn, ro, GH + k * nf, dt, diag, b, off);
-        jacobi(s, diag, b, off, x0, cfg->sle_iters, Yn + k * nc);
+        sle_solve(s, diag, b, off, x0, Yn + k * nc);
     }
     for (long i = 0; i < nc; ++i) {
         double sum = 0.0;
@@ -2490,7 +2564,7 @@ static int diffusion_implicit(or_solver *s, double dt, const double *Y, const do
             }
             b[i] += dS / s->cV[i];
         }
-        jacobi(s, diag, b, off, x0, cfg->sle_iters, vn + a * nc);
+        sle_solve(s, diag, b, off, x0, vn + a * nc);
     }
     /* temperature */
     for (long f = 0; f < nf; ++f)
@@ -2516,7 +2590,7 @@ static int diffusion_implicit(or_solver *s, double dt, const double *Y, const do
             heat += (sv[q] * cfg->mu) * G[q];
         b[i] += heat;
     }
-    jacobi(s, diag, b, off, x0, cfg->sle_iters, Tn);
+    sle_solve(s, diag, b, off, x0, Tn);
     /* recalculation of the cell values (Diffusion.cpp:963-1029) */
     for (long i = 0; i < nc; ++i) {
         double c[OR_MAXS + 1];
@@ -2733,7 +2807,7 @@ static int diffusion_explicit(or_solver *s, double dt)
                 FS[9 * f + 3 * a + b] = t * cfg->mu;
             }
     }
-    if (cfg->sle_solver == OR_SLE_JACOBI) {
+    if (cfg->sle_solver != OR_SLE_EXPLICIT) {
         err = diffusion_implicit(s, dt, Y, CCVold, gradV, divV, srho, FS, GH);
         goto done;
     }

@@ -82,13 +82,14 @@ typedef struct {
      * no molar-mass correction; ideal mixture, non-periodic axes */
     int diffusion;
     double mu, D, kappa;
-    /* matrix solver of the stage: OR_SLE_EXPLICIT, or OR_SLE_JACOBI
-     * (JacobiSolver.cpp) with at most sle_iters iterations */
+    /* matrix solver of the stage: OR_SLE_EXPLICIT, OR_SLE_JACOBI
+     * (JacobiSolver.cpp) or OR_SLE_GAUSS_SEIDEL (GaussSeidelSolver.cpp) with
+     * at most sle_iters iterations */
     int sle_solver;
     long sle_iters;
 } or_config;
 
-enum { OR_SLE_EXPLICIT = 0, OR_SLE_JACOBI = 1 };
+enum { OR_SLE_EXPLICIT = 0, OR_SLE_JACOBI = 1, OR_SLE_GAUSS_SEIDEL = 2 };
 
 typedef struct or_solver or_solver;
 

@@ -471,8 +471,9 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         (mix->gas_model != SCHEMI_GAS_IDEAL || !(scheme->transport_mu >= 0.0) ||
          !(scheme->transport_D >= 0.0) || !(scheme->transport_kappa >= 0.0) ||
          (scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT &&
-          scheme->diffusion_solver != SCHEMI_SLE_JACOBI) ||
-         (scheme->diffusion_solver == SCHEMI_SLE_JACOBI && scheme->diffusion_iterations < 1)))
+          scheme->diffusion_solver != SCHEMI_SLE_JACOBI &&
+          scheme->diffusion_solver != SCHEMI_SLE_GAUSS_SEIDEL) ||
+         (scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT && scheme->diffusion_iterations < 1)))
         return SCHEMI_ERR_INIT;
 
     schemi_gpu_ctx *c = new (std::nothrow) schemi_gpu_ctx();
@@ -488,7 +489,8 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     c->mu = scheme->transport_mu;
     c->D = scheme->transport_D;
     c->kappa = scheme->transport_kappa;
-    c->sle_iters = (c->diffusion && scheme->diffusion_solver == SCHEMI_SLE_JACOBI)
+    // Jacobi sweeps serve the Jacobi and the Gauss-Seidel solvers
+    c->sle_iters = (c->diffusion && scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT)
                        ? scheme->diffusion_iterations
                        : 0;
     DevParams &P = c->hp;

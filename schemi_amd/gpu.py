@@ -197,7 +197,7 @@ def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) 
     s.diffusion = 1 if case.diffusion else 0
     s.transport_mu, s.transport_D, s.transport_kappa = (float(x) for x in case.transport)
     # matrixSolver enum values (matrixSolverEnum.hpp:15-25)
-    s.diffusion_solver = {"explicit": 3, "Jacobi": 6}.get(case.sle_solver, -1)
+    s.diffusion_solver = {"GaussSeidel": 0, "explicit": 3, "Jacobi": 6}.get(case.sle_solver, -1)
     s.diffusion_iterations = int(case.sle_iters)
     return s
 

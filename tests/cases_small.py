@@ -184,6 +184,15 @@ def cases():
     out["3d_jacobi_lz_slip"] = c
     out["1d_jacobi_capped"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0),
                                    limiter=TL.vanLeer, sle_iters=3, **BASE, **jac)
+    # Gauss-Seidel decks: compared with the oracle's sequential sweeps within
+    # the solver tolerance
+    gs = dict(jac, sle_solver="GaussSeidel")
+    out["1d_gs_vanLeer"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer,
+                                **BASE, **gs)
+    c = Case(dim=3, N=(24, 10, 12), L=(1.0, 0.5, 0.35), limiter=TL.vanAlbada, **BASE, **gs)
+    c.set_bc(2, BCT.slip)
+    c.set_bc(5, BCT.slip)
+    out["3d_gs_lz_slip"] = c
     # MUSCL-Hancock (SecondOrder) on every mesh kind it runs on: 1-D with each
     # limiter and boundary kind, and 2-D/3-D where the slopes vanish
     # (including the reversed top face normal, 3d_periodic_x_vanLeer)

@@ -366,6 +366,24 @@ def jacobi_cases(base):
               restart=seeded_primitives(c, 85))
 
 
+def gauss_seidel_cases(base):
+    """The diffusion stage with the Gauss-Seidel matrix solver (the reference
+    decks' default): the implicit Laplacians of the Jacobi cases solved by
+    forward and backward sweeps."""
+    tr = dict(diffusion=True, transport=(0.02, 0.1, 0.01), sle_solver="GaussSeidel")
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, **base, **tr)
+    run_const("seeded_1d_gs_vanleer", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 91))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             **base, **tr)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_gs_hll", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 92))
+    c = Case(dim=3, N=(8, 6, 6), L=(1.0, 0.5, 0.35), limiter=TL.vanAlbada, **base, **tr)
+    for pt in (2, 5):
+        c.set_bc(pt, BCT.slip)
+    run_const("seeded_3d_gs_lz", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 93))
+
+
 if __name__ == "__main__":
     b = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
     if len(sys.argv) > 1 and sys.argv[1] == "lz":
@@ -384,5 +402,7 @@ if __name__ == "__main__":
         diffusion_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "jacobi":
         jacobi_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "gs":
+        gauss_seidel_cases(b)
     else:
         main()

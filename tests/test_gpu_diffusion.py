@@ -101,18 +101,19 @@ def test_courant_run_with_jacobi_matches_oracle(step_solver, hip_device):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("what", ["periodic", "vdw", "negative_mu", "gauss_seidel",
+@pytest.mark.parametrize("what", ["periodic", "vdw", "negative_mu", "conjugate_gradient",
                                   "zero_iterations"])
 def test_unsupported_diffusion_rejected(what, hip_device):
     """Diffusion with a periodic axis, a non-ideal mixture, a negative
-    transport coefficient, a matrix solver other than explicit and Jacobi, or
+    transport coefficient, a matrix solver other than explicit, Jacobi and
+    Gauss-Seidel, or
     no Jacobi iterations is declined with initialisationError."""
     kw = dict(BASE)
     if what == "vdw":
         kw.update(gas_model=1, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
     tr = (-1.0, 0.02, 0.002) if what == "negative_mu" else DIF["transport"]
-    if what == "gauss_seidel":
-        kw["sle_solver"] = "GaussSeidel"
+    if what == "conjugate_gradient":
+        kw["sle_solver"] = "conjugateGradient"
     if what == "zero_iterations":
         kw.update(sle_solver="Jacobi", sle_iters=0)
     case = Case(dim=2, N=(16, 8, 1), L=(1.0, 0.5, 0.1), limiter=TL.vanLeer, diffusion=True,

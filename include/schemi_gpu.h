@@ -159,7 +159,9 @@ typedef struct {
      * reconstruction (#Reconstruction linear) and the constant transport
      * model of set/transportCoefficients.txt (viscosity, diffusion and heat
      * conduction coefficients; enthalpy flow "no", molar-mass correction
-     * "off"); no turbulence.  Ideal mixture, non-periodic axes; otherwise
+     * "off"); no turbulence.  Ideal mixture (on a periodic axis the wrapped
+     * face is an inner face spanning the centre distance through the
+     * boundary; no reference counterpart); otherwise
      * SCHEMI_ERR_INIT.  A negative temperature after the stage returns
      * SCHEMI_ERR_NEGATIVE_TEMPERATURE (negativeTemperatureError). */
     int32_t diffusion;

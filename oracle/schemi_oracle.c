@@ -1235,7 +1235,7 @@ or_solver *or_create(const or_config *cfg)
     else if (cfg->gas_model == OR_GAS_KATAOKA)
         kataoka_matrices(&s->cfg);
     for (int a = 0; a < cfg->dim; ++a)
-        if (axis_periodic(s, a) < 0 || (cfg->diffusion && axis_periodic(s, a) > 0)) {
+        if (axis_periodic(s, a) < 0) {
             free(s);
             return NULL;
         }
@@ -2260,6 +2260,22 @@ static void rk_average(or_solver *s, const or_state *Un)
 /* transport coefficients, the explicit matrix solver and linear             */
 /* reconstruction; no turbulence, enthalpy flow or molar-mass correction     */
 
+/* r_other - r_i across the inner face f seen from cell i; a periodic face
+ * (no reference counterpart) spans its centre distance through the
+ * boundary along its normal. */
+static void face_delta(const or_solver *s, long f, long i, double *d)
+{
+    const long o = (s->own[f] == i) ? s->nei[f] : s->own[f];
+    if (s->fper[f] > 0) {
+        const double sgn = (s->own[f] == i) ? 1. : -1.;
+        for (int b = 0; b < 3; ++b)
+            d[b] = s->fN[3 * f + b] * s->fper[f] * sgn;
+        return;
+    }
+    for (int b = 0; b < 3; ++b)
+        d[b] = s->crc[3 * o + b] - s->crc[3 * i + b];
+}
+
 /* surfGrad (gradient.hpp:96-165) of cell values q[ncomp] of cell o across
  * face f: inner faces ((q_n - q_o) / |d|^2) d with d = r_n - r_o; boundary
  * faces ((q_g - q_o) / |D|) D / |D| with the ghost value q_g and
@@ -2269,10 +2285,8 @@ static void surf_grad(const or_solver *s, long f, int ncomp, const double *qo,
 {
     const long o = s->own[f];
     if (s->fpatch[f] < 0) {
-        const long n = s->nei[f];
         double d[3];
-        for (int b = 0; b < 3; ++b)
-            d[b] = s->crc[3 * n + b] - s->crc[3 * o + b];
+        face_delta(s, f, o, d);
         const double m = or_hypot3(d[0], d[1], d[2]);
         const double m2 = m * m; /* pow<scalar, 2> (intExpPow.hpp) */
         for (int c = 0; c < ncomp; ++c)
@@ -2337,8 +2351,8 @@ static void laplacian_matrix(const or_solver *s, const double *k, const double *
                 if (s->own[f] != i)
                     for (int a = 0; a < 3; ++a)
                         on[a] = on[a] * -1;
-                for (int a = 0; a < 3; ++a)
-                    dR[a] = s->crc[3 * o + a] - s->crc[3 * i + a];
+                (void)o;
+                face_delta(s, f, i, dR);
                 const double m = or_hypot3(dR[0], dR[1], dR[2]);
                 double dn[3];
                 for (int a = 0; a < 3; ++a)
@@ -2776,8 +2790,7 @@ static int diffusion_explicit(or_solver *s, double dt)
             double dq[3], dd[3];
             if (!bnd) {
                 double d[3];
-                for (int b = 0; b < 3; ++b)
-                    d[b] = s->crc[3 * n + b] - s->crc[3 * o + b];
+                face_delta(s, f, o, d);
                 const double m = or_hypot3(d[0], d[1], d[2]);
                 for (int a = 0; a < 3; ++a)
                     dq[a] = (qn[a] - qo[a]) / (m * m);

@@ -79,7 +79,9 @@ typedef struct {
      * constant transport coefficients (abstractTransportModel.cpp:14-34:
      * viscosity mu, diffusion coefficient D, heat conduction kappa), explicit
      * matrix solver, linear reconstruction, no turbulence, no enthalpy flow,
-     * no molar-mass correction; ideal mixture, non-periodic axes */
+     * no molar-mass correction; ideal mixture.  On a periodic axis (no
+     * reference counterpart) the wrapped face is an inner face whose centre
+     * distance runs through the boundary. */
     int diffusion;
     double mu, D, kappa;
     /* matrix solver of the stage: OR_SLE_EXPLICIT, OR_SLE_JACOBI

@@ -636,10 +636,6 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         }
         periodic[a] = nper != 0;
         P.periodic[a] = periodic[a] ? 1 : 0;
-        if (periodic[a] && c->diffusion) {
-            delete c;
-            return SCHEMI_ERR_INIT;
-        }
     }
     for (int a = 0; a < 3; ++a)
         P.zero_entry[a] = (dim >= 2) ? 1 : 0;

@@ -215,7 +215,10 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             double d[3];
             slot_offset(P, geo, s, idx, d, a, hi, f);
             const int Na = P.N[a];
-            const bool inner = hi ? (idx[a] < Na - 1) : (idx[a] > 0);
+            const bool per = P.periodic[a] != 0;
+            const bool inner = per || (hi ? (idx[a] < Na - 1) : (idx[a] > 0));
+            // the wrapped face of a periodic axis (owner N-1, neighbour 0)
+            const bool wrap = per && (hi ? (idx[a] == Na - 1) : (idx[a] == 0));
             // outward normal along a: the reference points a face normal away
             // from its owner's centre (mesh.cpp:34-53)
             const bool flip = hi && !inner && P.hi_flip[a] != 0;
@@ -227,7 +230,9 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             double gY[NS][3], gT[3], gv[3][3], divs;
             double mf = 1, dotf = 0;  // |d| and (N S / V) & d / |d| of the Laplacian
             if (inner) {
-                const long long nb = id + (hi ? P.stride[a] : -P.stride[a]);
+                const long long nb =
+                    id + (wrap ? (hi ? -(long long)(Na - 1) : (long long)(Na - 1)) * P.stride[a]
+                               : (hi ? P.stride[a] : -P.stride[a]));
                 double wn[NW];
                 load_w<NS>(W, n, nb, wn);
 #pragma unroll
@@ -242,9 +247,10 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
 #pragma unroll
                 for (int k = 0; k < NS; ++k)
                     srho += (qo[L::C + k] * wo + qn[L::C + k] * wnn) * P.M[k];
-                // d = r_n - r_o along a; |d|^2 = pow<2>(hypot3(d))
-                const double xc_lo = geo[P.off_xc[a] + f - 1], xc_hi = geo[P.off_xc[a] + f];
-                const double da = xc_hi - xc_lo;
+                // d = r_n - r_o along a; |d|^2 = pow<2>(hypot3(d)); a
+                // periodic face spans its centre distance through the boundary
+                const double da = wrap ? geo[P.off_dn[a] + f]
+                                       : geo[P.off_xc[a] + f] - geo[P.off_xc[a] + f - 1];
                 const double m = tabs(da);
                 const double m2 = m * m;
                 mf = m;
@@ -556,19 +562,35 @@ diff_jacobi_kernel(const DevParams *__restrict__ Pp, double *Mb, double *dmax, i
             double *xn = Mb + (M::X + 2 * f + (it & 1)) * n;
             const double *t = Mb + JF<NS>::coef(f) * n;
             const double aii = 1. / Mb[JF<NS>::diag(f) * n + id];
+            // neighbour of each face slot (x-, x+, y+, y-, z-, z+), -1 on a
+            // boundary; periodic axes wrap
+            long long nb[6];
+            const int ii[3] = {i, j, k};
+            const int NN[3] = {Nx, Ny, Nz};
+            const long long st[3] = {1, Nx, Nxy};
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                const int a = s >> 1;
+                const bool hi = (s == 1 || s == 2 || s == 5);
+                nb[s] = -1;
+                if (a >= DIM)
+                    continue;
+                if (hi ? ii[a] < NN[a] - 1 : ii[a] > 0)
+                    nb[s] = id + (hi ? st[a] : -st[a]);
+                else if (P.periodic[a])
+                    nb[s] = id + (hi ? -(long long)(NN[a] - 1) : (long long)(NN[a] - 1)) * st[a];
+            }
+            // lower triangle (smaller index), then upper, each from its last
+            // entry (slot order)
             double row = 0;
-            if (DIM == 3 && k > 0)
-                row -= (-t[4 * n + id]) * xo[id - Nxy];
-            if (DIM >= 2 && j > 0)
-                row -= (-t[3 * n + id]) * xo[id - Nx];
-            if (i > 0)
-                row -= (-t[0 * n + id]) * xo[id - 1];
-            if (DIM == 3 && k < Nz - 1)
-                row -= (-t[5 * n + id]) * xo[id + Nxy];
-            if (DIM >= 2 && j < Ny - 1)
-                row -= (-t[2 * n + id]) * xo[id + Nx];
-            if (i < Nx - 1)
-                row -= (-t[1 * n + id]) * xo[id + 1];
+#pragma unroll
+            for (int s = 5; s >= 0; --s)
+                if (nb[s] >= 0 && nb[s] < id)
+                    row -= (-t[s * n + id]) * xo[nb[s]];
+#pragma unroll
+            for (int s = 5; s >= 0; --s)
+                if (nb[s] > id)
+                    row -= (-t[s * n + id]) * xo[nb[s]];
             const double x = (Mb[JF<NS>::rhs(f) * n + id] + row) * aii;
             xn[id] = x;
             e = tabs((x - xo[id]) / (tabs(x) + STAB));

@@ -184,6 +184,17 @@ def cases():
     out["3d_jacobi_lz_slip"] = c
     out["1d_jacobi_capped"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0),
                                    limiter=TL.vanLeer, sle_iters=3, **BASE, **jac)
+    # periodic axes (the wrapped face is an inner face)
+    for nm, extra in (("diffusion", dif), ("jacobi", jac)):
+        c = Case(dim=2, N=(64, 32, 1), L=(1.0, 0.5, 0.1), limiter=TL.vanLeer, **BASE, **extra)
+        periodic(c, 0)
+        periodic(c, 1)
+        out[f"2d_{nm}_periodic_box"] = c
+    c = Case(dim=3, N=(64, 10, 12), L=(1.0, 0.5, 0.5), limiter=TL.minmod, flux=FS.HLL, **BASE,
+             **jac)
+    periodic(c, 0)
+    c.set_bc(3, BCT.slip)
+    out["3d_jacobi_periodic_x"] = c
     # Gauss-Seidel decks: compared with the oracle's sequential sweeps within
     # the solver tolerance
     gs = dict(jac, sle_solver="GaussSeidel")

@@ -101,10 +101,10 @@ def test_courant_run_with_jacobi_matches_oracle(step_solver, hip_device):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("what", ["periodic", "vdw", "negative_mu", "conjugate_gradient",
+@pytest.mark.parametrize("what", ["vdw", "negative_mu", "conjugate_gradient",
                                   "zero_iterations"])
 def test_unsupported_diffusion_rejected(what, hip_device):
-    """Diffusion with a periodic axis, a non-ideal mixture, a negative
+    """Diffusion with a non-ideal mixture, a negative
     transport coefficient, a matrix solver other than explicit, Jacobi and
     Gauss-Seidel, or
     no Jacobi iterations is declined with initialisationError."""
@@ -118,8 +118,6 @@ def test_unsupported_diffusion_rejected(what, hip_device):
         kw.update(sle_solver="Jacobi", sle_iters=0)
     case = Case(dim=2, N=(16, 8, 1), L=(1.0, 0.5, 0.1), limiter=TL.vanLeer, diffusion=True,
                 transport=tr, **kw)
-    if what == "periodic":
-        periodic(case, 0)
     with pytest.raises(SchemiGpuError) as e:
         GpuSolver(case)
     assert e.value.code == errors.initialisationError

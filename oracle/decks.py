@@ -168,8 +168,10 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         files["transportCoefficients.txt"] = [
             "#Transport_coefficient_model", "constant", "#Viscosity_coefficient", _r(mu),
             "#Diffusion_coefficient", _r(Dc), "#Heat_conduction_coefficient", _r(kappa),
-            "#Flag_for_enthalpy_flow_solution", "no",
-            "#Flag_for_molar_mass_diffusion_correction", "off"]
+            "#Flag_for_enthalpy_flow_solution",
+            "explicit" if getattr(case, "enthalpy_flow", False) else "no",
+            "#Flag_for_molar_mass_diffusion_correction",
+            "on" if getattr(case, "molar_mass_correction", False) else "off"]
     else:
         files["transportCoefficients.txt"] = [
             "#Transport_coefficient_model", "constant", "#Viscosity_coefficient", "2E-5",

@@ -54,6 +54,8 @@ class _Config(C.Structure):
         ("kappa", C.c_double),
         ("sle_solver", C.c_int),
         ("sle_iters", C.c_long),
+        ("enth_flow", C.c_int),
+        ("molmass", C.c_int),
     ]
 
 
@@ -150,6 +152,8 @@ def make_config(case) -> _Config:
     cfg.sle_solver = {"explicit": 0, "Jacobi": 1, "GaussSeidel": 2}[
         getattr(case, "sle_solver", "explicit")]
     cfg.sle_iters = int(getattr(case, "sle_iters", 2000))
+    cfg.enth_flow = 1 if getattr(case, "enthalpy_flow", False) else 0
+    cfg.molmass = 1 if getattr(case, "molar_mass_correction", False) else 0
     cfg.flux = int(case.flux)
     cfg.limiter = int(case.limiter)
     cfg.gravity = 1 if case.gravity else 0

@@ -1240,6 +1240,7 @@ or_solver *or_create(const or_config *cfg)
             return NULL;
         }
     if (cfg->diffusion && (cfg->gas_model != OR_GAS_IDEAL ||
+                           (cfg->enth_flow && cfg->sle_solver != OR_SLE_EXPLICIT) ||
                            cfg->sle_solver < OR_SLE_EXPLICIT ||
                            cfg->sle_solver > OR_SLE_GAUSS_SEIDEL || cfg->sle_iters < 1)) {
         free(s);
@@ -2511,7 +2512,7 @@ static void sle_solve(const or_solver *s, const double *diag, const double *b, c
  * temperature (plus the viscous heating). */
 static int diffusion_implicit(or_solver *s, double dt, const double *Y, const double *CCVold,
                               const double *gradV, const double *divV, const double *srho,
-                              const double *FS, const double *GH)
+                              const double *FS, const double *GH, const double *MC)
 {
     const or_config *cfg = &s->cfg;
     or_state *st = &s->st;
@@ -2537,6 +2538,22 @@ static int diffusion_implicit(or_solver *s, double dt, const double *Y, const do
             x0[i] = Y[i * Ns + k];
         }
         laplacian_matrix(s, kf, rn, ro, GH + k * nf, dt, diag, b, off);
+        if (cfg->molmass)
+            for (long i = 0; i < nc; ++i) {
+                /* distributeSourceTerm of -div(corr_k) (Diffusion.cpp:356-364) */
+                double dm = 0;
+                for (int j = 0; j < s->cnf[i]; ++j) {
+                    const long f = s->cf[6 * i + j];
+                    double n[3];
+                    out_normal(s, f, i, n);
+                    dm += dot3(&MC[(k * nf + f) * 3], n) * s->fS[f];
+                }
+                const double src = (dm / s->cV[i]) * (-1.0);
+                if (src >= 0.)
+                    b[i] += src;
+                else
+                    diag[i] -= src / (x0[i] + STAB);
+            }
         sle_solve(s, diag, b, off, x0, Yn + k * nc);
     }
     for (long i = 0; i < nc; ++i) {
@@ -2643,6 +2660,146 @@ static int diffusion_implicit(or_solver *s, double dt, const double *Y, const do
     return err;
 }
 
+/* GaussEliminationSolver of effectiveTransportCoefficients
+ * (transportCoefficients.hpp:397-455): rows swapped once per column by the
+ * largest following entry, elimination and back substitution with
+ * stabilised pivots, normalize. */
+static void gauss_elimination(int N, double A[OR_MAXS][OR_MAXS], double *b, double *phi)
+{
+    for (int k = 0; k < N - 1; ++k) {
+        double mv = A[k][k];
+        int c = k;
+        for (int r = k + 1; r < N; ++r)
+            if (A[r][k] > mv) {
+                mv = A[r][k];
+                c = r;
+            }
+        if (c != k) {
+            for (int j = 0; j < N; ++j) {
+                const double t = A[c][j];
+                A[c][j] = A[k][j];
+                A[k][j] = t;
+            }
+            const double t = b[c];
+            b[c] = b[k];
+            b[k] = t;
+        }
+    }
+    for (int k = 0; k < N - 1; ++k)
+        for (int i = k + 1; i < N; ++i) {
+            const double ratio = A[i][k] / (A[k][k] + STAB);
+            for (int j = k + 1; j < N; ++j)
+                A[i][j] = A[i][j] - ratio * A[k][j];
+            b[i] = b[i] - ratio * b[k];
+        }
+    phi[N - 1] = b[N - 1] / (A[N - 1][N - 1] + STAB);
+    for (int i = N - 2; i >= 0; --i) {
+        double term = 0.0;
+        for (int j = i + 1; j < N; ++j)
+            term += A[i][j] * phi[j];
+        phi[i] = (b[i] - term) / (A[i][i] + STAB);
+    }
+    for (int k = 0; k < N; ++k)
+        if (fabs(phi[k]) < DBL_EPSILON)
+            phi[k] = 0;
+}
+
+/* Molar-mass corrections of the species fluxes (caclulateDFluxes,
+ * transportCoefficients.hpp:285-395, constant D_kl = D; Diffusion.cpp:
+ * 326-377) and the explicit enthalpy flow sum_k (corr_k + W_k) h_k / M_k
+ * times the surface temperature (Diffusion.cpp:379-421, 605-611), with
+ * W_k = -rho D_k surfGrad(Y_k) the face fluxes FY and h_k = (Cv_k T + R T) / T
+ * (mixtureIdeal.cpp:291-298). */
+static int species_corrections(or_solver *s, const double *FY, const double *CSf,
+                               const double *TSf, double *MC, double *EF)
+{
+    const or_config *cfg = &s->cfg;
+    const or_state *st = &s->st;
+    const int Ns = cfg->Ns;
+    const long nf = s->nf;
+    for (long f = 0; f < nf; ++f) {
+        const long o = s->own[f];
+        double DF[OR_MAXS][3];
+        for (int k = 0; k < Ns; ++k)
+            for (int b = 0; b < 3; ++b)
+                DF[k][b] = 0;
+        if (cfg->molmass && Ns > 1) {
+            /* mole fractions c_k / c_0, ghosts from the ghost concentrations
+             * (calculatedMolarFraction) */
+            double qo[OR_MAXS], qn[OR_MAXS], g[3 * OR_MAXS];
+            for (int k = 0; k < Ns; ++k)
+                qo[k] = st->c[k + 1][o] / st->c[0][o];
+            if (s->fpatch[f] < 0) {
+                const long n = s->nei[f];
+                for (int k = 0; k < Ns; ++k)
+                    qn[k] = st->c[k + 1][n] / st->c[0][n];
+            } else {
+                double cg[OR_MAXS + 1];
+                if (ghost_conc(s, f, o, cg))
+                    return ERR_BC;
+                for (int k = 0; k < Ns; ++k)
+                    qn[k] = cg[k + 1] / cg[0];
+            }
+            surf_grad(s, f, Ns, qo, qn, g);
+            const double *N = &CSf[0];
+            double c[OR_MAXS];
+            for (int k = 0; k < Ns; ++k)
+                c[k] = N[(k + 1) * nf + f];
+            double cmin = c[0];
+            for (int k = 1; k < Ns; ++k)
+                if (c[k] < cmin)
+                    cmin = c[k];
+            int rep = 0;
+            for (int k = 0; k < Ns; ++k)
+                if (cmin == c[k]) {
+                    rep = k + 1;
+                    break;
+                }
+            const double N0 = N[f];
+            for (int b = 0; b < 3; ++b) {
+                double A[OR_MAXS][OR_MAXS] = {{0}}, rhs[OR_MAXS], phi[OR_MAXS];
+                for (int k = 0; k < Ns; ++k)
+                    rhs[k] = -g[3 * k + b];
+                for (int k1 = 1; k1 <= Ns; ++k1)
+                    if (k1 != rep) {
+                        for (int k2 = 1; k2 <= Ns; ++k2) {
+                            if (k1 == k2)
+                                continue;
+                            const double N1 = N[k1 * nf + f], N2 = N[k2 * nf + f];
+                            const double Aij = N1 * N2 / (N0 * N0 * cfg->D);
+                            A[k1 - 1][k2 - 1] = -Aij;
+                            A[k1 - 1][k1 - 1] += Aij;
+                        }
+                    } else {
+                        rhs[k1 - 1] = 0;
+                        for (int k2 = 1; k2 <= Ns; ++k2)
+                            A[k1 - 1][k2 - 1] = N[k2 * nf + f] * cfg->M[k2 - 1];
+                    }
+                gauss_elimination(Ns, A, rhs, phi);
+                for (int k = 0; k < Ns; ++k)
+                    DF[k][b] = phi[k] * cfg->M[k] * c[k];
+            }
+        }
+        if (cfg->molmass)
+            for (int k = 0; k < Ns; ++k)
+                for (int b = 0; b < 3; ++b)
+                    MC[(k * nf + f) * 3 + b] = DF[k][b] - FY[(k * nf + f) * 3 + b];
+        if (cfg->enth_flow) {
+            const double T = TSf[f];
+            double e[3] = {0, 0, 0};
+            for (int k = 0; k < Ns; ++k) {
+                const double h = (cfg->Cv[k] * T + cfg->R * T) / T;
+                for (int b = 0; b < 3; ++b)
+                    e[b] += (FY[(k * nf + f) * 3 + b] + MC[(k * nf + f) * 3 + b]) * h /
+                            cfg->M[k];
+            }
+            for (int b = 0; b < 3; ++b)
+                EF[3 * f + b] = e[b] * T;
+        }
+    }
+    return 0;
+}
+
 static int diffusion_explicit(or_solver *s, double dt)
 {
     const or_config *cfg = &s->cfg;
@@ -2663,6 +2820,12 @@ static int diffusion_explicit(or_solver *s, double dt)
     double *FS = malloc(9 * nf * sizeof(double));      /* mu strainSurf1 */
     /* boundary ghost values per face: Y_k (Ns), T, v (3) */
     double *GH = malloc((Ns + 4) * nf * sizeof(double));
+    /* surface concentrations (c_0 = sum, c_1..c_Ns) and temperature, the
+     * molar-mass corrections of the species fluxes and the enthalpy flow */
+    double *CSf = malloc((Ns + 1) * nf * sizeof(double));
+    double *TSf = malloc(nf * sizeof(double));
+    double *MC = calloc(3 * Ns * nf, sizeof(double));
+    double *EF = calloc(3 * nf, sizeof(double));
     double cc[OR_MAXS + 1];
 
     /* mass fractions rho_k / rho, normalised (diffusiveFields.cpp:30-47) */
@@ -2701,6 +2864,22 @@ static int diffusion_explicit(or_solver *s, double dt)
         srho[f] = 0;
         for (int k = 1; k <= Ns; ++k)
             srho[f] += cs[k] * cfg->M[k - 1];
+        cs[0] = 0;
+        for (int k = 1; k <= Ns; ++k)
+            cs[0] += cs[k];
+        for (int k = 0; k <= Ns; ++k)
+            CSf[k * nf + f] = cs[k];
+        /* surface temperature (linearInterpolate; calculatedTemperature on
+         * boundary faces from the surface concentrations and pressure) */
+        if (s->fpatch[f] < 0) {
+            const long n = s->nei[f];
+            TSf[f] = st->T[o] * s->wo[f] + st->T[n] * s->wn[f];
+        } else {
+            double ps = 0;
+            if (bc_surf_scalar(&cfg->bc_p[s->fpatch[f]], st->p[o], &ps))
+                err = ERR_BC;
+            TSf[f] = eos_T(cfg, cs, eos_U(cfg, cs, ps));
+        }
     }
     if (err)
         goto done;
@@ -2820,8 +2999,13 @@ static int diffusion_explicit(or_solver *s, double dt)
                 FS[9 * f + 3 * a + b] = t * cfg->mu;
             }
     }
+    if (cfg->molmass || cfg->enth_flow) {
+        err = species_corrections(s, FY, CSf, TSf, MC, EF);
+        if (err)
+            goto done;
+    }
     if (cfg->sle_solver != OR_SLE_EXPLICIT) {
-        err = diffusion_implicit(s, dt, Y, CCVold, gradV, divV, srho, FS, GH);
+        err = diffusion_implicit(s, dt, Y, CCVold, gradV, divV, srho, FS, GH, MC);
         goto done;
     }
     /* cell updates (SLEMatrix.cpp:2054-2142 explicit Laplacians, then
@@ -2845,12 +3029,32 @@ static int diffusion_explicit(or_solver *s, double dt)
             }
         }
         const double V = s->cV[i], rho = st->rho[0][i];
+        /* divergences of the molar-mass corrections and of the enthalpy
+         * flow, distributed into the free term (>= 0) or the diagonal
+         * (SLEMatrix.cpp:2166-2175) */
+        double dMC[OR_MAXS] = {0}, dEF = 0;
+        for (int j = 0; j < s->cnf[i]; ++j) {
+            const long f = s->cf[6 * i + j];
+            double n[3];
+            out_normal(s, f, i, n);
+            for (int k = 0; k < Ns; ++k)
+                dMC[k] += dot3(&MC[(k * nf + f) * 3], n) * s->fS[f];
+            dEF += dot3(&EF[3 * f], n) * s->fS[f];
+        }
         /* mass fractions: (Y rho + (0 - div F) dt) / rho, clipped at 0 and
          * renormalised */
         double Yn[OR_MAXS], sum = 0.0;
         for (int k = 0; k < Ns; ++k) {
-            const double free_term = 0. - dY[k] / V;
-            Yn[k] = (Y[i * Ns + k] * rho + free_term * dt) / rho;
+            double free_term = 0. - dY[k] / V;
+            double diag = rho;
+            if (cfg->molmass) {
+                const double src = (dMC[k] / V) * (-1.0);
+                if (src >= 0.)
+                    free_term += src;
+                else
+                    diag -= src / (Y[i * Ns + k] + STAB);
+            }
+            Yn[k] = (Y[i * Ns + k] * rho + free_term * dt) / diag;
             if (Yn[k] < 0.)
                 Yn[k] = 0.0;
             sum += Yn[k];
@@ -2885,8 +3089,16 @@ static int diffusion_explicit(or_solver *s, double dt)
         for (int q = 0; q < 9; ++q)
             heat += (sv[q] * cfg->mu) * G[q];
         double free_t = 0. - dT / V;
+        double diag_t = CCVnew;
+        if (cfg->enth_flow) {
+            const double src = (dEF / V) * -1;
+            if (src >= 0.)
+                free_t += src;
+            else
+                diag_t -= src / (st->T[i] + STAB);
+        }
         free_t += heat;
-        const double Tn = (CCVold[i] * st->T[i] + free_t * dt) / CCVnew;
+        const double Tn = (CCVold[i] * st->T[i] + free_t * dt) / diag_t;
         /* recalculation of the cell values (Diffusion.cpp:963-1029) */
         for (int k = 0; k <= Ns; ++k)
             st->c[k][i] = cn[k];
@@ -2916,6 +3128,10 @@ done:
     free(FV);
     free(FS);
     free(GH);
+    free(CSf);
+    free(TSf);
+    free(MC);
+    free(EF);
     return err;
 }
 

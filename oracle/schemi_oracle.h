@@ -89,6 +89,10 @@ typedef struct {
      * at most sle_iters iterations */
     int sle_solver;
     long sle_iters;
+    /* enthalpy flow solved explicitly (1) or not (0), molar-mass diffusion
+     * correction on (1) or off (0) (set/transportCoefficients.txt) */
+    int enth_flow;
+    int molmass;
 } or_config;
 
 enum { OR_SLE_EXPLICIT = 0, OR_SLE_JACOBI = 1, OR_SLE_GAUSS_SEIDEL = 2 };

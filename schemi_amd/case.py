@@ -79,6 +79,10 @@ class Case:
     # and the iteration limit (#Number_of_iterations_of_SLE_matrix_solver)
     sle_solver: str = "explicit"
     sle_iters: int = 2000
+    # #Flag_for_enthalpy_flow_solution explicit (True) or no, and
+    # #Flag_for_molar_mass_diffusion_correction on (True) or off
+    enthalpy_flow: bool = False
+    molar_mass_correction: bool = False
 
     def __post_init__(self):
         if not self.bc_c:

@@ -47,6 +47,10 @@ def case_json(case: Case) -> str:
         eos.update(diffusion=True, transport=list(case.transport))
         if case.sle_solver != "explicit":
             eos.update(sle_solver=case.sle_solver, sle_iters=int(case.sle_iters))
+        if case.enthalpy_flow:
+            eos["enthalpy_flow"] = True
+        if case.molar_mass_correction:
+            eos["molar_mass_correction"] = True
     return json.dumps({**eos,
         "dim": case.dim, "N": list(case.N), "L": list(case.L), "M": list(case.M),
         "Cv": list(case.Cv), "R": case.R, "flux": int(case.flux),
@@ -384,6 +388,37 @@ def gauss_seidel_cases(base):
     run_const("seeded_3d_gs_lz", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 93))
 
 
+def species_flux_cases(base):
+    """The diffusion stage with the molar-mass correction of the species
+    fluxes (Gauss elimination per face) and the explicit enthalpy flow, as
+    the reference's own diffusion test deck enables them."""
+    tr = dict(diffusion=True, transport=(0.004, 0.02, 0.002), molar_mass_correction=True)
+    # the enthalpy flow of the seeded (cell-to-cell random) mass fractions is
+    # strong: a small D keeps the explicit stage stable (the reference fails
+    # with a Riemann error at step 6 with D = 0.02, as does the oracle)
+    en = dict(tr, transport=(0.004, 2e-4, 0.002), enthalpy_flow=True)
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, **base, **en)
+    run_const("seeded_1d_molmass_enthalpy", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 101))
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), flux=FS.HLL, limiter=TL.minmod, **base,
+             **tr)
+    c.set_bc(0, BCT.slip)
+    c.set_bc(1, BCT.fixedValueCell, "p", (1.2, 0, 0))
+    c.set_bc(1, BCT.fixedValueCell, "c", (0.03, 0, 0))
+    run_const("seeded_1d_molmass_slip_fixed", c, 5, 2e-4, (1, 5),
+              restart=seeded_primitives(c, 102))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             **base, **en)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_molmass_enthalpy", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 103))
+    c = Case(dim=3, N=(8, 6, 6), L=(1.0, 0.5, 0.35), limiter=TL.vanAlbada,
+             **base, **dict(tr, transport=(0.02, 0.1, 0.01)), sle_solver="Jacobi")
+    run_const("seeded_3d_molmass_jacobi", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 104))
+    b3 = dict(M=[29.0, 4.0, 44.0], Cv=[2.5, 1 / 0.67, 3.4])
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, **b3, **en)
+    run_const("seeded_1d_molmass_3species", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 105))
+
+
 if __name__ == "__main__":
     b = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
     if len(sys.argv) > 1 and sys.argv[1] == "lz":
@@ -404,5 +439,7 @@ if __name__ == "__main__":
         jacobi_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "gs":
         gauss_seidel_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "molmass":
+        species_flux_cases(b)
     else:
         main()

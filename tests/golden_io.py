@@ -32,7 +32,9 @@ def load(name):
                 diffusion=bool(d.get("diffusion", False)),
                 transport=tuple(d.get("transport", (2e-5, 2e-5, 2e-2))),
                 sle_solver=d.get("sle_solver", "explicit"),
-                sle_iters=int(d.get("sle_iters", 2000)))
+                sle_iters=int(d.get("sle_iters", 2000)),
+                enthalpy_flow=bool(d.get("enthalpy_flow", False)),
+                molar_mass_correction=bool(d.get("molar_mass_correction", False)))
     snaps = {int(s): {k: z[f"s{s}_{k}"] for k in KEYS} for s in z["steps"]}
     out = {"case": case, "dts": z["dts"], "snaps": snaps}
     if "times" in z:

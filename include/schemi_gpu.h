@@ -180,6 +180,14 @@ typedef struct {
      * bit for bit). */
     int32_t diffusion_solver;
     int32_t diffusion_iterations;
+    /* #Flag_for_molar_mass_diffusion_correction on (1): the species fluxes
+     * corrected to the mole-fraction gradients (caclulateDFluxes,
+     * transportCoefficients.hpp:285-395, Diffusion.cpp:326-377);
+     * #Flag_for_enthalpy_flow_solution explicit (1): the enthalpy carried by
+     * the species fluxes (Diffusion.cpp:379-421, 605-635; explicit matrix
+     * solver only, else SCHEMI_ERR_INIT). */
+    int32_t diffusion_molar_mass;
+    int32_t diffusion_enthalpy_flow;
 } schemi_scheme_desc;
 
 #define SCHEMI_SLE_GAUSS_SEIDEL 0

@@ -197,6 +197,7 @@ struct schemi_gpu_ctx {
     double *Wd = nullptr;        // diffusion: (2 Ns + 5) x n primitives
     std::vector<hipEvent_t> evd; // timed diffusion stages: start, between passes, end
     int sle_iters = 0;           // Jacobi solver: iteration limit (0: explicit solver)
+    int molmass = 0, enthalpy = 0;
     double *Mb = nullptr, *dmax = nullptr;
     int *done = nullptr;
     int done_host[16] = {};
@@ -303,6 +304,8 @@ int one_step(schemi_gpu_ctx *c)
         L.done = c->done;
         L.done_host = c->done_host;
         L.maxit = c->sle_iters;
+        L.mm = c->molmass;
+        L.en = c->enthalpy;
         if (c->Mb) {
             HIPCHK(hipMemsetAsync(c->dmax, 0,
                                   sizeof(double) * (size_t)(c->ns + 4) * (size_t)(c->sle_iters + 2),
@@ -473,7 +476,8 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
          (scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT &&
           scheme->diffusion_solver != SCHEMI_SLE_JACOBI &&
           scheme->diffusion_solver != SCHEMI_SLE_GAUSS_SEIDEL) ||
-         (scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT && scheme->diffusion_iterations < 1)))
+         (scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT && scheme->diffusion_iterations < 1) ||
+         (scheme->diffusion_enthalpy_flow && scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT)))
         return SCHEMI_ERR_INIT;
 
     schemi_gpu_ctx *c = new (std::nothrow) schemi_gpu_ctx();
@@ -489,6 +493,8 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     c->mu = scheme->transport_mu;
     c->D = scheme->transport_D;
     c->kappa = scheme->transport_kappa;
+    c->molmass = (c->diffusion && scheme->diffusion_molar_mass) ? 1 : 0;
+    c->enthalpy = (c->diffusion && scheme->diffusion_enthalpy_flow) ? 1 : 0;
     // Jacobi sweeps serve the Jacobi and the Gauss-Seidel solvers
     c->sle_iters = (c->diffusion && scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT)
                        ? scheme->diffusion_iterations

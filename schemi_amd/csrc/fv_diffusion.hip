@@ -137,9 +137,9 @@ diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const doubl
 // concentrations), the new concentrations, and two iterate buffers per field
 // (Y_k, v_c, T).
 template <int NS> struct MB {
-    static constexpr int DY = 0, BY = 1, TY = 1 + NS, DV = 7 + NS, BV = 8 + NS, TV = 11 + NS,
-                         BT = 17 + NS, TT = 18 + NS, DT = 24 + NS, CN = 25 + NS,
-                         X = 25 + 2 * NS, NF = NS + 4, N = X + 2 * NF;
+    static constexpr int DY = 0, BY = NS, TY = 2 * NS, DV = 2 * NS + 6, BV = DV + 1, TV = DV + 4,
+                         BT = TV + 6, TT = BT + 1, DT = TT + 6, CN = DT + 1, X = CN + NS,
+                         NF = NS + 4, N = X + 2 * NF;
 };
 
 template <int NS>
@@ -150,11 +150,119 @@ __device__ __forceinline__ void load_w(const double *W, long long n, long long i
         w[q] = W[q * n + id];
 }
 
-template <int NS, int DIM, bool MIXED, bool IMPL>
+// Molar-mass corrected species flux of one face component (caclulateDFluxes,
+// transportCoefficients.hpp:285-395 with D_kl = D): the rows of all species
+// but the least concentrated one balance the mole-fraction gradient, that
+// row fixes sum_k c_k M_k flux_k = 0; GaussEliminationSolver
+// (transportCoefficients.hpp:397-455: one row swap per column by the largest
+// following entry, stabilised pivots, normalize); flux_k = phi_k M_k c_k.
+template <int NS>
+__device__ __forceinline__ void species_flux(const DevParams &P, const double *cs,
+                                             const double *gXa, double Dc, double *DF)
+{
+    if constexpr (NS == 1) {
+        DF[0] = 0;
+        return;
+    } else {
+        double N0 = 0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            N0 += cs[k];
+        double cmin = cs[0];
+#pragma unroll
+        for (int k = 1; k < NS; ++k)
+            if (cs[k] < cmin)
+                cmin = cs[k];
+        int rep = 0;
+#pragma unroll
+        for (int k = NS - 1; k >= 0; --k)
+            if (cmin == cs[k])
+                rep = k + 1;
+        double A[NS][NS], b[NS];
+#pragma unroll
+        for (int k1 = 0; k1 < NS; ++k1) {
+            b[k1] = -gXa[k1];
+#pragma unroll
+            for (int k2 = 0; k2 < NS; ++k2)
+                A[k1][k2] = 0;
+        }
+#pragma unroll
+        for (int k1 = 1; k1 <= NS; ++k1) {
+            if (k1 != rep) {
+#pragma unroll
+                for (int k2 = 1; k2 <= NS; ++k2) {
+                    if (k1 == k2)
+                        continue;
+                    const double Aij = cs[k1 - 1] * cs[k2 - 1] / (N0 * N0 * Dc);
+                    A[k1 - 1][k2 - 1] = -Aij;
+                    A[k1 - 1][k1 - 1] += Aij;
+                }
+            } else {
+                b[k1 - 1] = 0;
+#pragma unroll
+                for (int k2 = 1; k2 <= NS; ++k2)
+                    A[k1 - 1][k2 - 1] = cs[k2 - 1] * P.M[k2 - 1];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NS - 1; ++k) {
+            double mv = A[k][k];
+            int c = k;
+#pragma unroll
+            for (int r = k + 1; r < NS; ++r)
+                if (A[r][k] > mv) {
+                    mv = A[r][k];
+                    c = r;
+                }
+            if (c != k) {
+#pragma unroll
+                for (int r = k + 1; r < NS; ++r)
+                    if (r == c) {
+#pragma unroll
+                        for (int j = 0; j < NS; ++j) {
+                            const double t = A[r][j];
+                            A[r][j] = A[k][j];
+                            A[k][j] = t;
+                        }
+                        const double t = b[r];
+                        b[r] = b[k];
+                        b[k] = t;
+                    }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NS - 1; ++k)
+#pragma unroll
+            for (int i = k + 1; i < NS; ++i) {
+                const double ratio = A[i][k] / (A[k][k] + STAB);
+#pragma unroll
+                for (int j = k + 1; j < NS; ++j)
+                    A[i][j] = A[i][j] - ratio * A[k][j];
+                b[i] = b[i] - ratio * b[k];
+            }
+        double phi[NS];
+        phi[NS - 1] = b[NS - 1] / (A[NS - 1][NS - 1] + STAB);
+#pragma unroll
+        for (int i = NS - 2; i >= 0; --i) {
+            double term = 0.0;
+#pragma unroll
+            for (int j = i + 1; j < NS; ++j)
+                term += A[i][j] * phi[j];
+            phi[i] = (b[i] - term) / (A[i][i] + STAB);
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+            DF[k] = (tabs(phi[k]) < DBL_EPSILON ? 0.0 : phi[k]) * P.M[k] * cs[k];
+    }
+}
+
+// SPEC: the molar-mass correction of the species fluxes (mm) and the
+// explicit enthalpy flow (en) are evaluated.
+template <int NS, int DIM, bool MIXED, bool IMPL, bool SPEC>
 __global__ void __launch_bounds__(NT)
 diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo, double *U,
                    double *Ualt, int hk, const double *__restrict__ W, double *sc,
-                   int *errflag, double mu, double Dc, double kappa, double *Mb)
+                   int *errflag, double mu, double Dc, double kappa, double *Mb, int mm, int en)
 {
     using M = MB<NS>;
     using L = DW<NS>;
@@ -180,6 +288,12 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         for (int k = 0; k < NS; ++k)
             dY[k] = 0;
         int err = 0;
+        double dMC[NS], dEF = 0, c0o = 0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            dMC[k] = 0;
+            c0o += w0[L::C + k];
+        }
         // implicit Laplacians: diagonals and right-hand sides start from the
         // time derivative, rnew (1 / dt) and rold (1 / dt)
         double diagY = 0, diagV = 0, bY[NS], bV[3], bT = 0, tY[6], tV[6], tT[6];
@@ -229,6 +343,9 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             double qo[NW], qn[NW];
             double gY[NS][3], gT[3], gv[3][3], divs;
             double mf = 1, dotf = 0;  // |d| and (N S / V) & d / |d| of the Laplacian
+            // species corrections: face concentrations, temperature, and the
+            // normal component of surfGrad of the mole fractions
+            double cs[NS], Ts = 0, gXa[NS];
             if (inner) {
                 const long long nb =
                     id + (wrap ? (hi ? -(long long)(Na - 1) : (long long)(Na - 1)) * P.stride[a]
@@ -245,8 +362,10 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 for (int b = 0; b < 3; ++b)
                     vs[b] = qo[L::V + b] * wo + qn[L::V + b] * wnn;
 #pragma unroll
-                for (int k = 0; k < NS; ++k)
-                    srho += (qo[L::C + k] * wo + qn[L::C + k] * wnn) * P.M[k];
+                for (int k = 0; k < NS; ++k) {
+                    cs[k] = qo[L::C + k] * wo + qn[L::C + k] * wnn;
+                    srho += cs[k] * P.M[k];
+                }
                 // d = r_n - r_o along a; |d|^2 = pow<2>(hypot3(d)); a
                 // periodic face spans its centre distance through the boundary
                 const double da = wrap ? geo[P.off_dn[a] + f]
@@ -268,6 +387,18 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                         gv[c][b] = ((qn[L::V + c] - qo[L::V + c]) / m2) * dd[b];
                 }
                 divs = ((qn[L::V + a] - qo[L::V + a]) / m2) * da;
+                if constexpr (SPEC) {
+                    Ts = qo[L::T] * wo + qn[L::T] * wnn;
+                    double c0a = 0, c0b = 0;
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) {
+                        c0a += qo[L::C + k];
+                        c0b += qn[L::C + k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < NS; ++k)
+                        gXa[k] = ((qn[L::C + k] / c0b - qo[L::C + k] / c0a) / m2) * da;
+                }
             } else {
                 const int p = patch_of(a, hi ? 1 : 0);
 #pragma unroll
@@ -280,8 +411,8 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 double cgh[NS], cg0 = 0, sumd = 0;
 #pragma unroll
                 for (int k = 0; k < NS; ++k) {
-                    srho += bc_surface<double>(P.bct[p][k], P.bcv[p][k][0], w0[L::C + k], -1, a) *
-                            P.M[k];
+                    cs[k] = bc_surface<double>(P.bct[p][k], P.bcv[p][k][0], w0[L::C + k], -1, a);
+                    srho += cs[k] * P.M[k];
                     cgh[k] = bc_cell<double>(P.bct[p][k], P.bcv[p][k][0], w0[L::C + k], -1, a);
                     cg0 += cgh[k];
                     sumd += cgh[k] * P.M[k];
@@ -333,6 +464,50 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                         gv[c][b] = dq[c] * nrm[b];
                 }
                 divs = ((0. + dq[0] * nrm[0]) + dq[1] * nrm[1]) + dq[2] * nrm[2];
+                if constexpr (SPEC) {
+                    // calculatedTemperature on the surface; calculatedMolarFraction ghost
+                    double cs0 = 0;
+#pragma unroll
+                    for (int k = 0; k < NS; ++k)
+                        cs0 += cs[k];
+                    const double ps =
+                        bc_surface<double>(P.bct[p][NS + 1], P.bcv[p][NS + 1][0], w0[L::P], -1, a);
+                    const double cvs = cv_mix<NS>(P, cs, cs0);
+                    Ts = (ps / (P.R / cvs)) / ((cs0 + STAB) * cvs);
+#pragma unroll
+                    for (int k = 0; k < NS; ++k)
+                        gXa[k] = ((cgh[k] / cg0 - w0[L::C + k] / c0o) / m) * nrm[a];
+                }
+            }
+            if constexpr (SPEC) {
+                // molar-mass corrections (caclulateDFluxes with D_kl = D,
+                // transportCoefficients.hpp:285-395) of the normal component,
+                // and the enthalpy flow (Diffusion.cpp:379-421, 605-611)
+                const double rhoD = srho * Dc;
+                double Wa[NS], MCa[NS];
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    Wa[k] = (gY[k][a] * rhoD) * -1;
+                    MCa[k] = 0;
+                }
+                if (mm) {
+                    double DF[NS];
+                    species_flux<NS>(P, cs, gXa, Dc, DF);
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) {
+                        MCa[k] = DF[k] - Wa[k];
+                        dMC[k] += (MCa[k] * na) * S;
+                    }
+                }
+                if (en) {
+                    double e = 0;
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) {
+                        const double h = (P.Cv[k] * Ts + P.R * Ts) / Ts;
+                        e += (Wa[k] + MCa[k]) * h / P.M[k];
+                    }
+                    dEF += ((e * Ts) * na) * S;
+                }
             }
             // Green-Gauss sums of grad v (v_c N_b) S and div v (v & N) S
 #pragma unroll
@@ -406,11 +581,19 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             for (int q = 0; q < 9; ++q)
                 heat += (sv[q] * mu) * G[q];
             bT += heat;
-            Mb[M::DY * n + id] = diagY;
             Mb[M::DV * n + id] = diagV;
 #pragma unroll
             for (int k = 0; k < NS; ++k) {
-                Mb[(M::BY + k) * n + id] = bY[k];
+                double dk = diagY, bk = bY[k];
+                if (SPEC && mm) {
+                    const double src = (dMC[k] / V) * (-1.0);
+                    if (src >= 0.)
+                        bk += src;
+                    else
+                        dk -= src / (w0[L::Y + k] + STAB);
+                }
+                Mb[(M::DY + k) * n + id] = dk;
+                Mb[(M::BY + k) * n + id] = bk;
                 Mb[(M::X + 2 * k) * n + id] = w0[L::Y + k];
             }
 #pragma unroll
@@ -436,8 +619,17 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         double Yn[NS], sum = 0.0;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            const double ft = 0. - dY[k] / V;
-            Yn[k] = (w0[L::Y + k] * rho + ft * dt) / rho;
+            double ft = 0. - dY[k] / V;
+            double diag = rho;
+            if (SPEC && mm) {
+                // distributeSourceTerm (SLEMatrix.cpp:2166-2175)
+                const double src = (dMC[k] / V) * (-1.0);
+                if (src >= 0.)
+                    ft += src;
+                else
+                    diag -= src / (w0[L::Y + k] + STAB);
+            }
+            Yn[k] = (w0[L::Y + k] * rho + ft * dt) / diag;
             if (Yn[k] < 0.)
                 Yn[k] = 0.0;
             sum += Yn[k];
@@ -480,8 +672,16 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         for (int q = 0; q < 9; ++q)
             heat += (sv[q] * mu) * G[q];
         double ftT = 0. - dT / V;
+        double diagT = CCVnew;
+        if (SPEC && en) {
+            const double src = (dEF / V) * -1;
+            if (src >= 0.)
+                ftT += src;
+            else
+                diagT -= src / (w0[L::T] + STAB);
+        }
         ftT += heat;
-        const double Tn = (CCVold * w0[L::T] + ftT * dt) / CCVnew;
+        const double Tn = (CCVold * w0[L::T] + ftT * dt) / diagT;
         if (Tn < 0.)
             atomicOr(errflag, ERRF_NEG_T);
         // recalculated cell: rho_k = c_k M_k, m = v rho, U = Cv T c0,
@@ -511,7 +711,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
 // Field f of the Jacobi solves: 0..NS-1 mass fractions, NS..NS+2 velocity
 // components, NS+3 temperature.
 template <int NS> struct JF {
-    __device__ static int diag(int f) { return f < NS ? MB<NS>::DY : (f < NS + 3 ? MB<NS>::DV : MB<NS>::DT); }
+    __device__ static int diag(int f) { return f < NS ? MB<NS>::DY + f : (f < NS + 3 ? MB<NS>::DV : MB<NS>::DT); }
     __device__ static int rhs(int f) { return f < NS ? MB<NS>::BY + f : (f < NS + 3 ? MB<NS>::BV + f - NS : MB<NS>::BT); }
     __device__ static int coef(int f) { return f < NS ? MB<NS>::TY : (f < NS + 3 ? MB<NS>::TV : MB<NS>::TT); }
 };
@@ -725,17 +925,28 @@ int launch_dim(const DiffusionLaunch &L)
                        L.Ualt, L.hk, L.W, L.sc);
     if (L.mid && hipEventRecord(L.mid, L.stream) != hipSuccess)
         return 1;
+    const bool spec = L.mm || L.en;
     if (!L.Mb) {
-        hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false>), dim3(blocks), dim3(NT), 0,
-                           L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc, L.err, L.mu, L.D,
-                           L.kappa, nullptr);
+        if (spec)
+            hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false, true>), dim3(blocks),
+                               dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc,
+                               L.err, L.mu, L.D, L.kappa, nullptr, L.mm, L.en);
+        else
+            hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false, false>), dim3(blocks),
+                               dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc,
+                               L.err, L.mu, L.D, L.kappa, nullptr, 0, 0);
         return hipGetLastError() == hipSuccess ? 0 : 1;
     }
     // Jacobi: assemble, solve Y_k and v_c, finish Y, solve T, finish; the
     // host reads the solves' state after each batch of iterations
-    hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, true>), dim3(blocks), dim3(NT), 0,
-                       L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc, L.err, L.mu, L.D,
-                       L.kappa, L.Mb);
+    if (spec)
+        hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, true, true>), dim3(blocks),
+                           dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc, L.err,
+                           L.mu, L.D, L.kappa, L.Mb, L.mm, 0);
+    else
+        hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, true, false>), dim3(blocks),
+                           dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc, L.err,
+                           L.mu, L.D, L.kappa, L.Mb, 0, 0);
     auto solve = [&](int f0, int f1) -> int {
         int it = 1, batch = 4;
         for (;;) {

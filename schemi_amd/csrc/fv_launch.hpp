@@ -97,6 +97,7 @@ struct DiffusionLaunch {
     double *dmax;
     int *done, *done_host;
     int maxit;
+    int mm, en;  // molar-mass correction, explicit enthalpy flow
 };
 int launch_diffusion(const DiffusionLaunch &L);
 size_t diffusion_matrix_doubles(int ns);

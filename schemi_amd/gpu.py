@@ -65,7 +65,8 @@ class SchemeDesc(C.Structure):
                 ("step_solver", C.c_int32), ("diffusion", C.c_int32),
                 ("transport_mu", C.c_double), ("transport_D", C.c_double),
                 ("transport_kappa", C.c_double), ("diffusion_solver", C.c_int32),
-                ("diffusion_iterations", C.c_int32)]
+                ("diffusion_iterations", C.c_int32), ("diffusion_molar_mass", C.c_int32),
+                ("diffusion_enthalpy_flow", C.c_int32)]
 
 
 _LIB = None
@@ -199,6 +200,8 @@ def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) 
     # matrixSolver enum values (matrixSolverEnum.hpp:15-25)
     s.diffusion_solver = {"GaussSeidel": 0, "explicit": 3, "Jacobi": 6}.get(case.sle_solver, -1)
     s.diffusion_iterations = int(case.sle_iters)
+    s.diffusion_molar_mass = 1 if case.molar_mass_correction else 0
+    s.diffusion_enthalpy_flow = 1 if case.enthalpy_flow else 0
     return s
 
 

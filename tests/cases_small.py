@@ -195,6 +195,24 @@ def cases():
     periodic(c, 0)
     c.set_bc(3, BCT.slip)
     out["3d_jacobi_periodic_x"] = c
+    # molar-mass correction of the species fluxes and the explicit enthalpy
+    # flow (small D: the seeded mass fractions are rough)
+    sp = dict(diffusion=True, transport=(0.004, 2e-4, 0.002), molar_mass_correction=True,
+              enthalpy_flow=True)
+    out["1d_molmass_enthalpy"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0),
+                                      limiter=TL.vanLeer, **BASE, **sp)
+    c = Case(dim=3, N=(24, 10, 12), L=(1.0, 0.5, 0.35), limiter=TL.vanAlbada, **BASE, **sp)
+    c.set_bc(2, BCT.slip)
+    c.set_bc(5, BCT.slip)
+    out["3d_molmass_enthalpy_lz"] = c
+    c = Case(dim=2, N=(64, 32, 1), L=(1.0, 0.5, 0.1), limiter=TL.minmod, flux=FS.HLL, **BASE,
+             **dict(jac, molar_mass_correction=True))
+    periodic(c, 0)
+    c.set_bc(3, BCT.slip)
+    out["2d_molmass_jacobi_periodic_x"] = c
+    out["1d_molmass_enthalpy_3sp"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0),
+                                          limiter=TL.minmod, M=[29.0, 4.0, 44.0],
+                                          Cv=[2.5, 1 / 0.67, 3.4], **sp)
     # Gauss-Seidel decks: compared with the oracle's sequential sweeps within
     # the solver tolerance
     gs = dict(jac, sle_solver="GaussSeidel")

@@ -101,12 +101,12 @@ def test_courant_run_with_jacobi_matches_oracle(step_solver, hip_device):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("what", ["vdw", "negative_mu", "conjugate_gradient",
+@pytest.mark.parametrize("what", ["vdw", "negative_mu", "conjugate_gradient", "enthalpy_jacobi",
                                   "zero_iterations"])
 def test_unsupported_diffusion_rejected(what, hip_device):
     """Diffusion with a non-ideal mixture, a negative
     transport coefficient, a matrix solver other than explicit, Jacobi and
-    Gauss-Seidel, or
+    Gauss-Seidel, the enthalpy flow with an implicit solver, or
     no Jacobi iterations is declined with initialisationError."""
     kw = dict(BASE)
     if what == "vdw":
@@ -114,6 +114,8 @@ def test_unsupported_diffusion_rejected(what, hip_device):
     tr = (-1.0, 0.02, 0.002) if what == "negative_mu" else DIF["transport"]
     if what == "conjugate_gradient":
         kw["sle_solver"] = "conjugateGradient"
+    if what == "enthalpy_jacobi":
+        kw.update(sle_solver="Jacobi", enthalpy_flow=True)
     if what == "zero_iterations":
         kw.update(sle_solver="Jacobi", sle_iters=0)
     case = Case(dim=2, N=(16, 8, 1), L=(1.0, 0.5, 0.1), limiter=TL.vanLeer, diffusion=True,

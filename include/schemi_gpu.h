@@ -154,12 +154,11 @@ typedef struct {
     int32_t step_solver; /* SCHEMI_STEP_* */
     /* Molecular diffusion stage after each step (Diffusion.cpp:20-1061,
      * called at secondOrderStepSolverRK.cpp:102-109 and
-     * secondOrderStepSolver.cpp:72-82; #Diffusion_flag on): the explicit
-     * matrix solver (#Type_of_SLE_solver_flag explicit) with linear
+     * secondOrderStepSolver.cpp:72-82; #Diffusion_flag on) with linear
      * reconstruction (#Reconstruction linear) and the constant transport
      * model of set/transportCoefficients.txt (viscosity, diffusion and heat
-     * conduction coefficients; enthalpy flow "no", molar-mass correction
-     * "off"); no turbulence.  Ideal mixture (on a periodic axis the wrapped
+     * conduction coefficients); the matrix solver, molar-mass correction and
+     * enthalpy flow below; no turbulence.  Ideal mixture (on a periodic axis the wrapped
      * face is an inner face spanning the centre distance through the
      * boundary; no reference counterpart); otherwise
      * SCHEMI_ERR_INIT.  A negative temperature after the stage returns

@@ -1,7 +1,8 @@
-// fv_diffusion.hip -- explicit molecular diffusion stage after each step
-// (Diffusion.cpp:20-1061 with the explicit matrix solver, linear
-// reconstruction, constant transport coefficients, no turbulence, no
-// enthalpy flow, no molar-mass correction; ideal mixture).
+// fv_diffusion.hip -- molecular diffusion stage after each step
+// (Diffusion.cpp:20-1061 with linear reconstruction, constant transport
+// coefficients, no turbulence, ideal mixture): the explicit matrix solver or
+// the implicit Laplacians with Jacobi sweeps, optionally with the molar-mass
+// correction of the species fluxes and (explicit solver) the enthalpy flow.
 //
 // Two passes over the cells, FP64 on both precision paths:
 //  * diff_prims_kernel: mass fractions rho_k / rho renormalised by their sum

@@ -1239,7 +1239,8 @@ or_solver *or_create(const or_config *cfg)
             free(s);
             return NULL;
         }
-    if (cfg->diffusion && (cfg->gas_model != OR_GAS_IDEAL ||
+    if (cfg->diffusion && ((cfg->gas_model != OR_GAS_IDEAL && cfg->gas_model != OR_GAS_VDW) ||
+                           (cfg->enth_flow && cfg->gas_model != OR_GAS_IDEAL) ||
                            (cfg->enth_flow && cfg->sle_solver != OR_SLE_EXPLICIT) ||
                            cfg->sle_solver < OR_SLE_EXPLICIT ||
                            cfg->sle_solver > OR_SLE_GAUSS_SEIDEL || cfg->sle_iters < 1)) {
@@ -2307,6 +2308,27 @@ static void surf_grad(const or_solver *s, long f, int ncomp, const double *qo,
     }
 }
 
+/* nonIdeality (mixtureIdeal.cpp:232-241: 0; mixtureVanDerWaals.cpp:369-387,
+ * vanDerWaalsFluid.cpp:78-82: -a c^2) and UvcFromT (Cv T; vdW Cv T - a c,
+ * vanDerWaalsFluid.cpp:38-42) of a cell with concentrations c (c0..cNs). */
+static double non_ideality(const or_config *cfg, const double *c)
+{
+    if (cfg->gas_model != OR_GAS_VDW)
+        return 0;
+    double a, b;
+    vdw_mix(cfg, c, &a, &b);
+    return -a * (c[0] * c[0]);
+}
+
+static double uvc_from_T(const or_config *cfg, const double *c, double T)
+{
+    if (cfg->gas_model != OR_GAS_VDW)
+        return cv_mix(cfg, c) * T;
+    double a, b;
+    vdw_mix(cfg, c, &a, &b);
+    return cv_mix(cfg, c) * T - a * c[0];
+}
+
 /* outward normal of face f for cell i (owner: N, neighbour: N * -1) */
 static void out_normal(const or_solver *s, long f, long i, double *n)
 {
@@ -2601,8 +2623,13 @@ static int diffusion_implicit(or_solver *s, double dt, const double *Y, const do
     for (long f = 0; f < nf; ++f)
         kf[f] = cfg->kappa;
     for (long i = 0; i < nc; ++i) {
+        double co[OR_MAXS + 1], cw[OR_MAXS + 1];
+        for (int k = 0; k <= Ns; ++k) {
+            co[k] = st->c[k][i];
+            cw[k] = cn[k * nc + i];
+        }
         rn[i] = CCVnew[i];
-        ro[i] = CCVold[i] * st->T[i];
+        ro[i] = CCVold[i] * st->T[i] + non_ideality(cfg, co) - non_ideality(cfg, cw);
         x0[i] = st->T[i];
     }
     laplacian_matrix(s, kf, rn, ro, GH + Ns * nf, dt, diag, b, off);
@@ -2639,7 +2666,7 @@ static int diffusion_implicit(or_solver *s, double dt, const double *Y, const do
             st->m[a][i] = v[a] * rho;
         }
         st->T[i] = Tn[i];
-        st->U[i] = (cv_mix(cfg, c) * Tn[i]) * c[0];
+        st->U[i] = uvc_from_T(cfg, c, Tn[i]) * c[0];
         st->p[i] = eos_p(cfg, c, st->U[i]);
         st->E[i] = (st->U[i] + rho * dot3(v, v) * 0.5) + 0.0;
         if (Tn[i] < 0.)
@@ -3098,7 +3125,12 @@ static int diffusion_explicit(or_solver *s, double dt)
                 diag_t -= src / (st->T[i] + STAB);
         }
         free_t += heat;
-        const double Tn = (CCVold[i] * st->T[i] + free_t * dt) / diag_t;
+        double co[OR_MAXS + 1];
+        for (int k = 0; k <= Ns; ++k)
+            co[k] = st->c[k][i];
+        /* oldEnergyField (Diffusion.cpp:484-485) */
+        const double oldE = CCVold[i] * st->T[i] + non_ideality(cfg, co) - non_ideality(cfg, cn);
+        const double Tn = (oldE + free_t * dt) / diag_t;
         /* recalculation of the cell values (Diffusion.cpp:963-1029) */
         for (int k = 0; k <= Ns; ++k)
             st->c[k][i] = cn[k];
@@ -3109,7 +3141,7 @@ static int diffusion_explicit(or_solver *s, double dt)
             st->m[a][i] = vn[a] * rho;
         }
         st->T[i] = Tn;
-        st->U[i] = (cv_mix(cfg, cn) * Tn) * cn[0];
+        st->U[i] = uvc_from_T(cfg, cn, Tn) * cn[0];
         st->p[i] = eos_p(cfg, cn, st->U[i]);
         const double v2 = dot3(vn, vn);
         st->E[i] = (st->U[i] + rho * v2 * 0.5) + 0.0;

@@ -419,6 +419,27 @@ def species_flux_cases(base):
     run_const("seeded_1d_molmass_3species", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 105))
 
 
+def vdw_diffusion_cases(base):
+    """The diffusion stage with the van der Waals mixture: the non-ideality
+    -a c^2 in the energy balance and U = (Cv T - a c) c in the recalculated
+    cells (Diffusion.cpp:484-485, 991-993)."""
+    eos = dict(gas_model=gasModel.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    tr = dict(diffusion=True, transport=(0.004, 0.02, 0.002))
+    # with the molar-mass correction D = 0.02 ends in a Riemann error at
+    # step 5 in the reference and the oracle alike: D = 0.002
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, **base, **eos,
+             **dict(tr, transport=(0.004, 0.002, 0.002)), molar_mass_correction=True)
+    run_const("seeded_1d_vdw_diffusion", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 111))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             **base, **eos, **dict(tr, transport=(0.02, 0.1, 0.01)), sle_solver="Jacobi")
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_vdw_jacobi", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 112))
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.minmod, step_solver=0,
+             **base, **eos, **dict(tr, transport=(0.02, 0.1, 0.01)), sle_solver="GaussSeidel")
+    run_const("seeded_1d_vdw_gs_hancock", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 113))
+
+
 if __name__ == "__main__":
     b = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
     if len(sys.argv) > 1 and sys.argv[1] == "lz":
@@ -441,5 +462,7 @@ if __name__ == "__main__":
         gauss_seidel_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "molmass":
         species_flux_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "vdwdiff":
+        vdw_diffusion_cases(b)
     else:
         main()

@@ -158,7 +158,8 @@ typedef struct {
      * reconstruction (#Reconstruction linear) and the constant transport
      * model of set/transportCoefficients.txt (viscosity, diffusion and heat
      * conduction coefficients); the matrix solver, molar-mass correction and
-     * enthalpy flow below; no turbulence.  Ideal mixture (on a periodic axis the wrapped
+     * enthalpy flow below; no turbulence.  Ideal or van der Waals mixture
+     * (on a periodic axis the wrapped
      * face is an inner face spanning the centre distance through the
      * boundary; no reference counterpart); otherwise
      * SCHEMI_ERR_INIT.  A negative temperature after the stage returns
@@ -184,7 +185,7 @@ typedef struct {
      * transportCoefficients.hpp:285-395, Diffusion.cpp:326-377);
      * #Flag_for_enthalpy_flow_solution explicit (1): the enthalpy carried by
      * the species fluxes (Diffusion.cpp:379-421, 605-635; explicit matrix
-     * solver only, else SCHEMI_ERR_INIT). */
+     * solver and ideal mixture only, else SCHEMI_ERR_INIT). */
     int32_t diffusion_molar_mass;
     int32_t diffusion_enthalpy_flow;
 } schemi_scheme_desc;

@@ -471,7 +471,9 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         scheme->step_solver != SCHEMI_STEP_SECOND_ORDER_RK)
         return SCHEMI_ERR_INIT;
     if (scheme->diffusion &&
-        (mix->gas_model != SCHEMI_GAS_IDEAL || !(scheme->transport_mu >= 0.0) ||
+        ((mix->gas_model != SCHEMI_GAS_IDEAL && mix->gas_model != SCHEMI_GAS_VAN_DER_WAALS) ||
+         (mix->gas_model != SCHEMI_GAS_IDEAL && scheme->diffusion_enthalpy_flow) ||
+         !(scheme->transport_mu >= 0.0) ||
          !(scheme->transport_D >= 0.0) || !(scheme->transport_kappa >= 0.0) ||
          (scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT &&
           scheme->diffusion_solver != SCHEMI_SLE_JACOBI &&

@@ -77,6 +77,61 @@ template <int NS> __device__ __forceinline__ double cv_mix(const DevParams &P, c
     return cv;
 }
 
+// Ideal or van der Waals closures of the stage: UvFromp, TFromUv
+// (idealFluid.cpp:25-41; vanDerWaalsFluid.cpp:23-49), nonIdeality (0;
+// -a c^2, vanDerWaalsFluid.cpp:78-82) and UvcFromT (Cv T; Cv T - a c,
+// vanDerWaalsFluid.cpp:38-42) of a state with concentrations c, sum c0.
+template <int NS>
+__device__ __forceinline__ double u_from_p(const DevParams &P, const double *c, double c0,
+                                           double p)
+{
+    const double cv = cv_mix<NS>(P, c, c0);
+    if (P.gas_model == GAS_VDW) {
+        double a, b;
+        vdw_ab<double, NS>(P, c, c0, a, b);
+        const double g1 = P.R / cv;
+        const double acc = a * (c0 * c0);
+        return (p + acc) * (1 - c0 * b) / g1 - acc;
+    }
+    return p / (P.R / cv);
+}
+
+template <int NS>
+__device__ __forceinline__ double t_from_u(const DevParams &P, const double *c, double c0,
+                                           double U)
+{
+    const double cv = cv_mix<NS>(P, c, c0);
+    if (P.gas_model == GAS_VDW) {
+        double a, b;
+        vdw_ab<double, NS>(P, c, c0, a, b);
+        return (U + a * (c0 * c0)) / ((c0 + STAB) * cv);
+    }
+    return U / ((c0 + STAB) * cv);
+}
+
+template <int NS>
+__device__ __forceinline__ double non_ideality(const DevParams &P, const double *c, double c0)
+{
+    if (P.gas_model != GAS_VDW)
+        return 0;
+    double a, b;
+    vdw_ab<double, NS>(P, c, c0, a, b);
+    return -a * (c0 * c0);
+}
+
+template <int NS>
+__device__ __forceinline__ double uvc_from_t(const DevParams &P, const double *c, double c0,
+                                             double T)
+{
+    const double cv = cv_mix<NS>(P, c, c0);
+    if (P.gas_model == GAS_VDW) {
+        double a, b;
+        vdw_ab<double, NS>(P, c, c0, a, b);
+        return cv * T - a * c0;
+    }
+    return cv * T;
+}
+
 // Uses the buffer holding the new state: MUSCL-Hancock steps alternate
 // between U and Ualt by the scalar block's parity.
 __device__ __forceinline__ const double *state_of(const double *U, const double *Ualt,
@@ -105,7 +160,7 @@ diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const doubl
 #pragma unroll
     for (int q = 0; q < NS + 5; ++q)
         u[q] = Us[q * n + id];
-    prims_from_cons<double, NS, GAS_IDEAL>(P, u, w);
+    prims_from_cons<double, NS>(P, u, w);
     using L = DW<NS>;
     double y[NS], sum = 0.0, c0 = 0;
 #pragma unroll
@@ -125,8 +180,7 @@ diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const doubl
     const double vx = w[NS], vy = w[NS + 1], vz = w[NS + 2];
     const double v2 = ((0. + vx * vx) + vy * vy) + vz * vz;
     const double Uint = (u[NS + 4] - u[NS] * v2 * 0.5) - 0.;
-    // TFromUv (idealFluid.cpp:37-41)
-    W[L::T * n + id] = Uint / ((c0 + STAB) * cv_mix<NS>(P, w, c0));
+    W[L::T * n + id] = t_from_u<NS>(P, w, c0, Uint);
     W[L::P * n + id] = w[NS + 3];
 }
 
@@ -139,8 +193,8 @@ diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const doubl
 // (Y_k, v_c, T).
 template <int NS> struct MB {
     static constexpr int DY = 0, BY = NS, TY = 2 * NS, DV = 2 * NS + 6, BV = DV + 1, TV = DV + 4,
-                         BT = TV + 6, TT = BT + 1, DT = TT + 6, CN = DT + 1, X = CN + NS,
-                         NF = NS + 4, N = X + 2 * NF;
+                         BT = TV + 6, TT = BT + 1, DT = TT + 6, CN = DT + 1, TB = CN + NS,
+                         HT = TB + 6, X = HT + 1, NF = NS + 4, N = X + 2 * NF;
 };
 
 template <int NS>
@@ -297,7 +351,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         }
         // implicit Laplacians: diagonals and right-hand sides start from the
         // time derivative, rnew (1 / dt) and rold (1 / dt)
-        double diagY = 0, diagV = 0, bY[NS], bV[3], bT = 0, tY[6], tV[6], tT[6];
+        double diagY = 0, diagV = 0, bY[NS], bV[3], bT = 0, tY[6], tV[6], tT[6], tB[6];
         double rdt = 0, rho_i = 0;
         if constexpr (IMPL) {
             rdt = 1 / dt;
@@ -314,10 +368,13 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
 #pragma unroll
             for (int k = 0; k < NS; ++k)
                 c0o += w0[L::C + k];
-            bT = ((c0o * cv_mix<NS>(P, w0 + L::C, c0o)) * w0[L::T]) * rdt;
+            // r_old = CCV_old T + nonIdeality_old - nonIdeality_new needs the new
+            // concentrations: the boundary terms and the heating are kept
+            // and the right-hand side is formed after the species solves
+            bT = 0;
 #pragma unroll
             for (int s = 0; s < 6; ++s)
-                tY[s] = tV[s] = tT[s] = 0;
+                tY[s] = tV[s] = tT[s] = tB[s] = 0;
         }
 #if SG_DIFF_UNROLL
 #pragma unroll
@@ -425,9 +482,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 // calculatedTemperature: TFromUv(c_g, UvFromp(c_g, p_g))
                 const double pg =
                     bc_cell<double>(P.bct[p][NS + 1], P.bcv[p][NS + 1][0], w0[L::P], -1, a);
-                const double cvg = cv_mix<NS>(P, cgh, cg0);
-                const double Ug = pg / (P.R / cvg);
-                qn[L::T] = Ug / ((cg0 + STAB) * cvg);
+                qn[L::T] = t_from_u<NS>(P, cgh, cg0, u_from_p<NS>(P, cgh, cg0, pg));
 #pragma unroll
                 for (int b = 0; b < 3; ++b)
                     qn[L::V + b] =
@@ -545,6 +600,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 tY[s] = ty;
                 tV[s] = tv;
                 tT[s] = tt;
+                tB[s] = 0;
                 if (!inner) {
 #pragma unroll
                     for (int k = 0; k < NS; ++k)
@@ -552,7 +608,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
 #pragma unroll
                     for (int c = 0; c < 3; ++c)
                         bV[c] += tv * qn[L::V + c];
-                    bT += tt * qn[L::T];
+                    tB[s] = tt * qn[L::T];
                 }
             }
         }
@@ -581,7 +637,8 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
 #pragma unroll
             for (int q = 0; q < 9; ++q)
                 heat += (sv[q] * mu) * G[q];
-            bT += heat;
+            (void)bT;
+            Mb[M::HT * n + id] = heat;
             Mb[M::DV * n + id] = diagV;
 #pragma unroll
             for (int k = 0; k < NS; ++k) {
@@ -602,13 +659,14 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 Mb[(M::BV + c) * n + id] = bV[c] + dFs[c] / V;
                 Mb[(M::X + 2 * (NS + c)) * n + id] = w0[L::V + c];
             }
-            Mb[M::BT * n + id] = bT;
+            Mb[M::BT * n + id] = 0;
             Mb[(M::X + 2 * (NS + 3)) * n + id] = w0[L::T];
 #pragma unroll
             for (int s = 0; s < 6; ++s) {
                 Mb[(M::TY + s) * n + id] = tY[s];
                 Mb[(M::TV + s) * n + id] = tV[s];
                 Mb[(M::TT + s) * n + id] = tT[s];
+                Mb[(M::TB + s) * n + id] = tB[s];
             }
         } else {
         double u[NS + 5];
@@ -682,7 +740,10 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 diagT -= src / (w0[L::T] + STAB);
         }
         ftT += heat;
-        const double Tn = (CCVold * w0[L::T] + ftT * dt) / diagT;
+        // oldEnergyField (Diffusion.cpp:484-485)
+        const double oldE = CCVold * w0[L::T] + non_ideality<NS>(P, w0 + L::C, c0o) -
+                            non_ideality<NS>(P, cn, c0n);
+        const double Tn = (oldE + ftT * dt) / diagT;
         if (Tn < 0.)
             atomicOr(errflag, ERRF_NEG_T);
         // recalculated cell: rho_k = c_k M_k, m = v rho, U = Cv T c0,
@@ -695,14 +756,14 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             un[NS + 1 + c] = vn[c] * rho;
-        const double Uin = (cvn * Tn) * c0n;
+        const double Uin = uvc_from_t<NS>(P, cn, c0n, Tn) * c0n;
         const double v2 = ((0. + vn[0] * vn[0]) + vn[1] * vn[1]) + vn[2] * vn[2];
         un[NS + 4] = (Uin + rho * v2 * 0.5) + 0.0;
 #pragma unroll
         for (int q = 0; q < NS + 5; ++q)
             Us[q * n + id] = un[q];
         (void)MIXED;
-        sig = signal_speed<double, NS, GAS_IDEAL>(P, un, cg.minlen);
+        sig = signal_speed<double, NS>(P, un, cg.minlen);
         }
     }
     if constexpr (!IMPL)
@@ -828,7 +889,8 @@ __device__ __forceinline__ double jacobi_result(const double *Mb, const int *don
 template <int NS, int DIM>
 __global__ void __launch_bounds__(NT)
 diff_finish_y_kernel(const DevParams *__restrict__ Pp, const double *U, const double *Ualt,
-                     int hk, double *Mb, const int *done, const double *sc)
+                     int hk, double *Mb, const int *done, const double *sc,
+                     const double *__restrict__ W)
 {
     using M = MB<NS>;
     const DevParams &P = *Pp;
@@ -862,6 +924,24 @@ diff_finish_y_kernel(const DevParams *__restrict__ Pp, const double *U, const do
     for (int s = 0; s < 2 * DIM; ++s)
         diag += Mb[(M::TT + s) * n + id];
     Mb[M::DT * n + id] = diag;
+    // temperature right-hand side: r_old / dt with r_old = CCV_old T +
+    // nonIdeality_old - nonIdeality_new, the boundary terms, the heating
+    const double *Wc = W;
+    double co[NS], c0o = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        co[k] = Wc[(DW<NS>::C + k) * n + id];
+        c0o += co[k];
+    }
+    const double T0 = Wc[DW<NS>::T * n + id];
+    const double rold = (c0o * cv_mix<NS>(P, co, c0o)) * T0 + non_ideality<NS>(P, co, c0o) -
+                        non_ideality<NS>(P, cn, c0);
+    double b = rold * rdt;
+#pragma unroll
+    for (int s = 0; s < 2 * DIM; ++s)
+        b += Mb[(M::TB + s) * n + id];
+    b += Mb[M::HT * n + id];
+    Mb[M::BT * n + id] = b;
 }
 
 // The recalculated cell from the solves (Diffusion.cpp:963-1029) and its
@@ -907,13 +987,13 @@ diff_finish_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             un[NS + 1 + c] = vn[c] * rho;
-        const double Uin = (cv_mix<NS>(P, cn, c0) * Tn) * c0;
+        const double Uin = uvc_from_t<NS>(P, cn, c0, Tn) * c0;
         const double v2 = ((0. + vn[0] * vn[0]) + vn[1] * vn[1]) + vn[2] * vn[2];
         un[NS + 4] = (Uin + rho * v2 * 0.5) + 0.0;
 #pragma unroll
         for (int q = 0; q < NS + 5; ++q)
             Us[q * n + id] = un[q];
-        sig = signal_speed<double, NS, GAS_IDEAL>(P, un, cg.minlen);
+        sig = signal_speed<double, NS>(P, un, cg.minlen);
     }
     reduce_signal<NT>(sig, sRed, reinterpret_cast<unsigned long long *>(sc) + SC_SIGMAX);
 }
@@ -972,7 +1052,7 @@ int launch_dim(const DiffusionLaunch &L)
     if (solve(0, NS + 3))
         return 1;
     hipLaunchKernelGGL((diff_finish_y_kernel<NS, DIM>), dim3(blocks), dim3(NT), 0, L.stream, L.P,
-                       L.U, L.Ualt, L.hk, L.Mb, L.done, L.sc);
+                       L.U, L.Ualt, L.hk, L.Mb, L.done, L.sc, L.W);
     if (solve(NS + 3, NS + 4))
         return 1;
     hipLaunchKernelGGL((diff_finish_kernel<NS>), dim3(blocks), dim3(NT), 0, L.stream, L.P, L.geo,

@@ -213,6 +213,21 @@ def cases():
     out["1d_molmass_enthalpy_3sp"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0),
                                           limiter=TL.minmod, M=[29.0, 4.0, 44.0],
                                           Cv=[2.5, 1 / 0.67, 3.4], **sp)
+    # van der Waals mixture in the stage (non-ideality in the energy balance)
+    vd = dict(gas_model=GM.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    out["1d_vdw_diffusion_molmass"] = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0),
+                                           limiter=TL.vanLeer, **BASE, **vd,
+                                           **dict(dif, transport=(0.004, 0.002, 0.002)),
+                                           molar_mass_correction=True)
+    c = Case(dim=2, N=(64, 32, 1), L=(1.0, 0.5, 0.1), limiter=TL.minmod, flux=FS.HLL, **BASE,
+             **vd, **jac)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    out["2d_vdw_jacobi"] = c
+    c = Case(dim=3, N=(24, 10, 12), L=(1.0, 0.5, 0.35), limiter=TL.vanAlbada, **BASE, **vd,
+             **dif)
+    c.set_bc(2, BCT.slip)
+    out["3d_vdw_diffusion_lz"] = c
     # Gauss-Seidel decks: compared with the oracle's sequential sweeps within
     # the solver tolerance
     gs = dict(jac, sle_solver="GaussSeidel")

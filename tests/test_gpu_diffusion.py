@@ -101,16 +101,20 @@ def test_courant_run_with_jacobi_matches_oracle(step_solver, hip_device):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("what", ["vdw", "negative_mu", "conjugate_gradient", "enthalpy_jacobi",
+@pytest.mark.parametrize("what", ["rk", "vdw_enthalpy", "negative_mu", "conjugate_gradient",
+                                  "enthalpy_jacobi",
                                   "zero_iterations"])
 def test_unsupported_diffusion_rejected(what, hip_device):
-    """Diffusion with a non-ideal mixture, a negative
+    """Diffusion with a mixture other than ideal and van der Waals, the
+    enthalpy flow with van der Waals, a negative
     transport coefficient, a matrix solver other than explicit, Jacobi and
     Gauss-Seidel, the enthalpy flow with an implicit solver, or
     no Jacobi iterations is declined with initialisationError."""
     kw = dict(BASE)
-    if what == "vdw":
-        kw.update(gas_model=1, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    if what == "rk":
+        kw.update(gas_model=2, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    if what == "vdw_enthalpy":
+        kw.update(gas_model=1, Tc=[24.0, 3.2], Pc=[1.0, 1.0], enthalpy_flow=True)
     tr = (-1.0, 0.02, 0.002) if what == "negative_mu" else DIF["transport"]
     if what == "conjugate_gradient":
         kw["sle_solver"] = "conjugateGradient"

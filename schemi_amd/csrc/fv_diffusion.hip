@@ -1,10 +1,11 @@
 // fv_diffusion.hip -- molecular diffusion stage after each step
 // (Diffusion.cpp:20-1061 with linear reconstruction, constant transport
-// coefficients, no turbulence, ideal mixture): the explicit matrix solver or
-// the implicit Laplacians with Jacobi sweeps, optionally with the molar-mass
-// correction of the species fluxes and (explicit solver) the enthalpy flow.
+// coefficients, no turbulence, ideal or van der Waals mixture): the explicit
+// matrix solver or the implicit Laplacians with Jacobi sweeps, optionally
+// with the molar-mass correction of the species fluxes and (explicit solver,
+// ideal mixture) the enthalpy flow.
 //
-// Two passes over the cells, FP64 on both precision paths:
+// Explicit solver: two passes over the cells, FP64 on both precision paths:
 //  * diff_prims_kernel: mass fractions rho_k / rho renormalised by their sum
 //    (diffusiveFields.cpp:30-47), concentrations, velocity, temperature and
 //    pressure of every cell from the conserved state, into W;
@@ -24,6 +25,10 @@
 //    own faces with the face's owner/neighbour roles, so a face gives both
 //    cells the same bits, and writes only itself: deterministic, no atomics
 //    but the Courant maximum of the new state.
+// Jacobi solver: the same primitives pass, diff_update_kernel<IMPL> assembling
+// the implicit systems from the same face loop, batches of diff_jacobi_kernel
+// sweeps, diff_finish_y_kernel (new concentrations, temperature system) and
+// diff_finish_kernel (recalculated cells).
 // Every face normal is +-1 along its axis; a face term of the reference
 // (F & N) S then reduces to F_a N_a S plus signed zeros, which are dropped.
 #include <cfloat>

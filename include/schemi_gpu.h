@@ -188,6 +188,15 @@ typedef struct {
      * solver and ideal mixture only, else SCHEMI_ERR_INIT). */
     int32_t diffusion_molar_mass;
     int32_t diffusion_enthalpy_flow;
+    /* #Source_timestep_correction of main.txt (Schemi.cpp:190-245):
+     * timestep_source 0 "Courant", 1 "Courant-Source" (the Courant step is
+     * capped by the source step of the previous diffusion stage,
+     * Schemi.cpp:828-833, which the explicit enthalpy flow sets to
+     * source_coeff * min(C Cv T / |div enthalpy flow + stab|),
+     * Diffusion.cpp:696-700); "Courant-Source-Diffusion" is not built
+     * (SCHEMI_ERR_INIT for values other than 0 and 1). */
+    int32_t timestep_source;
+    double source_coeff;
 } schemi_scheme_desc;
 
 #define SCHEMI_SLE_GAUSS_SEIDEL 0

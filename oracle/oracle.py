@@ -56,6 +56,8 @@ class _Config(C.Structure):
         ("sle_iters", C.c_long),
         ("enth_flow", C.c_int),
         ("molmass", C.c_int),
+        ("ts_source", C.c_int),
+        ("ts_coeff", C.c_double),
     ]
 
 
@@ -154,6 +156,8 @@ def make_config(case) -> _Config:
     cfg.sle_iters = int(getattr(case, "sle_iters", 2000))
     cfg.enth_flow = 1 if getattr(case, "enthalpy_flow", False) else 0
     cfg.molmass = 1 if getattr(case, "molar_mass_correction", False) else 0
+    cfg.ts_source = 1 if getattr(case, "timestep_source", False) else 0
+    cfg.ts_coeff = float(getattr(case, "source_coeffs", (0.25, 0.5))[0])
     cfg.flux = int(case.flux)
     cfg.limiter = int(case.limiter)
     cfg.gravity = 1 if case.gravity else 0

@@ -18,6 +18,7 @@
 #include <string.h>
 
 #define STAB (5.0 * DBL_EPSILON) /* globalConstants.hpp:33 stabilizator */
+#define VERYBIG (1E-3 * DBL_MAX)   /* globalConstants.hpp:35 veryBig */
 
 enum { ERR_NONE = 0, ERR_RIEMANN = 1, ERR_NEG_T = 2, ERR_BC = 5, ERR_INIT = 11 };
 
@@ -36,6 +37,7 @@ struct or_solver {
     double *cfd;                     /* 6 x 3 per cell: face centre - cell centre */
     double *cnd;                     /* 6 x 3 per cell: neighbour centre - cell centre */
     double *fper;                    /* periodic faces: centre distance across the face */
+    double dt_src;                   /* source time step (mesh::timestepSource) */
     or_state st;
     double *pool;
 };
@@ -1226,6 +1228,7 @@ or_solver *or_create(const or_config *cfg)
         return NULL;
     or_solver *s = calloc(1, sizeof(or_solver));
     s->cfg = *cfg;
+    s->dt_src = VERYBIG; /* mesh.cpp:121 */
     if (cfg->gas_model == OR_GAS_VDW)
         vdw_matrices(&s->cfg);
     else if (cfg->gas_model == OR_GAS_RK)
@@ -3038,6 +3041,7 @@ static int diffusion_explicit(or_solver *s, double dt)
     /* cell updates (SLEMatrix.cpp:2054-2142 explicit Laplacians, then
      * Diffusion.cpp:431-458, 462-483, 599-615, 711-760, 795-806, 818-828,
      * 963-1059) */
+    double dt_src = 0;
     for (long i = 0; i < nc; ++i) {
         double dY[OR_MAXS] = {0}, dT = 0, dV[3] = {0}, dS[3] = {0};
         for (int j = 0; j < s->cnf[i]; ++j) {
@@ -3118,6 +3122,10 @@ static int diffusion_explicit(or_solver *s, double dt)
         double free_t = 0. - dT / V;
         double diag_t = CCVnew;
         if (cfg->enth_flow) {
+            /* source time step coeff * min(CCV_old T_old / |div(flow) +
+             * stab|) (Diffusion.cpp:696-700), reduced over the cells here */
+            const double r = (CCVold[i] * st->T[i]) / fabs(dEF / V + STAB);
+            dt_src = (i == 0 || r < dt_src) ? r : dt_src;
             const double src = (dEF / V) * -1;
             if (src >= 0.)
                 free_t += src;
@@ -3148,6 +3156,8 @@ static int diffusion_explicit(or_solver *s, double dt)
         if (Tn < 0.)
             err = ERR_NEG_T;
     }
+    if (cfg->enth_flow)
+        s->dt_src = cfg->ts_coeff * dt_src;
 done:
     free(Y);
     free(CCVold);
@@ -3227,7 +3237,10 @@ double or_courant_dt(or_solver *s, double courant, double min_time)
             mx = sig;
     }
     double dt = courant / mx;
+    if (s->cfg.ts_source)
+        dt = (s->dt_src < dt) ? s->dt_src : dt; /* std::min(dt, dt_source) */
     dt = smax(dt, min_time);
+    s->dt_src = VERYBIG;
     return dt;
 }
 

@@ -93,6 +93,12 @@ typedef struct {
      * correction on (1) or off (0) (set/transportCoefficients.txt) */
     int enth_flow;
     int molmass;
+    /* time-step flag of main.txt: 0 "Courant", 1 "Courant-Source" (the
+     * Courant step is capped by the source step of the previous diffusion
+     * stage, Schemi.cpp:828-833); ts_coeff is the first source coefficient
+     * of the same entry (Schemi.cpp:205-225) */
+    int ts_source;
+    double ts_coeff;
 } or_config;
 
 enum { OR_SLE_EXPLICIT = 0, OR_SLE_JACOBI = 1, OR_SLE_GAUSS_SEIDEL = 2 };

@@ -83,6 +83,10 @@ class Case:
     # #Flag_for_molar_mass_diffusion_correction on (True) or off
     enthalpy_flow: bool = False
     molar_mass_correction: bool = False
+    # #Source_timestep_correction: "Courant-Source" (True) or "Courant", and
+    # its two coefficients (source, diffusion)
+    timestep_source: bool = False
+    source_coeffs: tuple = (0.25, 0.5)
 
     def __post_init__(self):
         if not self.bc_c:

@@ -40,13 +40,23 @@ __global__ void finalize_step_kernel(double *sc, const DevParams *P, double cons
     const double t = sc[SC_TIME] + dt;
     unsigned long long bits = reinterpret_cast<unsigned long long *>(sc)[SC_SIGMAX];
     double dtn;
+    unsigned long long *srcmin = reinterpret_cast<unsigned long long *>(sc) + SC_SRCMIN;
     if (const_dt > 0.0) {
         dtn = const_dt;
     } else {
         const double smax = __longlong_as_double(static_cast<long long>(bits));
         dtn = P->courant / smax;
+        if (P->ts_source) {
+            // std::min(dt, timestepSource): the source step of this step's
+            // diffusion stage, veryBig when none set it
+            const double dts = (*srcmin == SRCMIN_NONE)
+                                   ? 1e-3 * __DBL_MAX__
+                                   : P->ts_coeff * __longlong_as_double(static_cast<long long>(*srcmin));
+            dtn = (dts < dtn) ? dts : dtn;
+        }
         dtn = (dtn < P->min_time) ? P->min_time : dtn;  // std::max(dt, minTime)
     }
+    *srcmin = SRCMIN_NONE;
     sc[SC_DTNEXT] = dtn;
     if (t + dtn > sc[SC_TEND])
         dtn = sc[SC_TEND] - t;
@@ -66,6 +76,7 @@ __global__ void set_scalars_kernel(double *sc, double dt, double t, double tend)
     sc[SC_TIME] = t;
     sc[SC_TEND] = tend;
     reinterpret_cast<unsigned long long *>(sc)[SC_SIGMAX] = 0ull;
+    reinterpret_cast<unsigned long long *>(sc)[SC_SRCMIN] = SRCMIN_NONE;
     sc[SC_DTNEXT] = 0.0;
     sc[SC_STEPS] = 0.0;
     sc[SC_DONE] = 0.0;
@@ -481,6 +492,9 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
          (scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT && scheme->diffusion_iterations < 1) ||
          (scheme->diffusion_enthalpy_flow && scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT)))
         return SCHEMI_ERR_INIT;
+    if ((scheme->timestep_source != 0 && scheme->timestep_source != 1) ||
+        !(scheme->source_coeff >= 0.0))
+        return SCHEMI_ERR_INIT;
 
     schemi_gpu_ctx *c = new (std::nothrow) schemi_gpu_ctx();
     if (!c)
@@ -607,6 +621,8 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         P.g[b] = scheme->g[b];
     P.courant = scheme->courant;
     P.min_time = scheme->min_time;
+    P.ts_source = scheme->timestep_source;
+    P.ts_coeff = scheme->source_coeff;
 
     // boundary conditions; periodic must pair both patches of an axis for
     // every field

@@ -36,9 +36,12 @@ enum : int { SV_RK1 = 0, SV_RK2 = 1, SV_HANCOCK = 2 };
 // time, end time, the signal-speed maximum (bits of a non-negative double,
 // reduced with atomicMax), next time step, step count, end-of-run flag, and
 // the MUSCL-Hancock buffer parity (1: the current state is in the second of
-// the two state buffers).
+// the two state buffers), and the minimum over the cells of the source-step
+// ratio of the last diffusion stage (bits of a non-negative double, reduced
+// with atomicMin; +inf when no stage set it).
 enum : int { SC_DT = 0, SC_TIME, SC_TEND, SC_SIGMAX, SC_DTNEXT, SC_STEPS, SC_DONE, SC_PARITY,
-             SC_N };
+             SC_SRCMIN, SC_N };
+constexpr unsigned long long SRCMIN_NONE = 0x7FF0000000000000ull;  // +inf
 enum : int { ERRF_RIEMANN = 1, ERRF_BOUNDARY = 2, ERRF_NEG_T = 4 };
 
 // Parameters shared by all kernels of a context, kept in device memory and
@@ -76,6 +79,8 @@ struct DevParams {
     // (0 inner, 1 boundary plane), indexed by the cell index along b
     int off_td[3][3][2];
     double courant, min_time;
+    int ts_source;     // 1: the Courant step is capped by the source step
+    double ts_coeff;   // source step coefficient
 };
 
 // patch of axis a, side (0 = low, 1 = high): tail/point, right/left, bottom/top

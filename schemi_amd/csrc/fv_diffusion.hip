@@ -316,6 +316,26 @@ __device__ __forceinline__ void species_flux(const DevParams &P, const double *c
     }
 }
 
+// Block minimum of non-negative doubles (or +inf), merged into *slot as the
+// bits of a double with atomicMin (integer order equals value order there).
+template <int NB>
+__device__ __forceinline__ void reduce_min_bits(double r, unsigned long long *slot)
+{
+    __shared__ double sMin[NB / 64];
+    const int tid = threadIdx.x;
+    for (int off = 32; off > 0; off >>= 1)
+        r = tmin(r, __shfl_xor(r, off, 64));
+    if ((tid & 63) == 0)
+        sMin[tid >> 6] = r;
+    __syncthreads();
+    if (tid == 0) {
+        double m = sMin[0];
+        for (int w = 1; w < NB / 64; ++w)
+            m = tmin(m, sMin[w]);
+        atomicMin(slot, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
 // SPEC: the molar-mass correction of the species fluxes (mm) and the
 // explicit enthalpy flow (en) are evaluated.
 template <int NS, int DIM, bool MIXED, bool IMPL, bool SPEC>
@@ -334,6 +354,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
     const long long n = P.n;
     const long long id = block_cell0() + threadIdx.x;
     double sig = 0.0;
+    double rsrc = __builtin_inf();  // source-step ratio of the enthalpy flow
     double *Us = const_cast<double *>(state_of(U, Ualt, sc, hk));
     const double dt = sc[SC_DT];
     if (id < n) {
@@ -738,6 +759,10 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         double ftT = 0. - dT / V;
         double diagT = CCVnew;
         if (SPEC && en) {
+            // source time step ratio C Cv_old T_old / |div + stab|
+            // (Diffusion.cpp:696-700); the minimum over the cells times the
+            // coefficient caps the next Courant step
+            rsrc = (CCVold * w0[L::T]) / __builtin_fabs(dEF / V + STAB);
             const double src = (dEF / V) * -1;
             if (src >= 0.)
                 ftT += src;
@@ -771,8 +796,12 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         sig = signal_speed<double, NS>(P, un, cg.minlen);
         }
     }
-    if constexpr (!IMPL)
+    if constexpr (!IMPL) {
         reduce_signal<NT>(sig, sRed, reinterpret_cast<unsigned long long *>(sc) + SC_SIGMAX);
+        if constexpr (SPEC)
+            if (en)
+                reduce_min_bits<NT>(rsrc, reinterpret_cast<unsigned long long *>(sc) + SC_SRCMIN);
+    }
 }
 
 // Field f of the Jacobi solves: 0..NS-1 mass fractions, NS..NS+2 velocity

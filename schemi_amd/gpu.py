@@ -66,7 +66,8 @@ class SchemeDesc(C.Structure):
                 ("transport_mu", C.c_double), ("transport_D", C.c_double),
                 ("transport_kappa", C.c_double), ("diffusion_solver", C.c_int32),
                 ("diffusion_iterations", C.c_int32), ("diffusion_molar_mass", C.c_int32),
-                ("diffusion_enthalpy_flow", C.c_int32)]
+                ("diffusion_enthalpy_flow", C.c_int32), ("timestep_source", C.c_int32),
+                ("source_coeff", C.c_double)]
 
 
 _LIB = None
@@ -202,6 +203,8 @@ def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) 
     s.diffusion_iterations = int(case.sle_iters)
     s.diffusion_molar_mass = 1 if case.molar_mass_correction else 0
     s.diffusion_enthalpy_flow = 1 if case.enthalpy_flow else 0
+    s.timestep_source = 1 if case.timestep_source else 0
+    s.source_coeff = float(case.source_coeffs[0])
     return s
 
 

@@ -51,6 +51,8 @@ def case_json(case: Case) -> str:
             eos["enthalpy_flow"] = True
         if case.molar_mass_correction:
             eos["molar_mass_correction"] = True
+    if case.timestep_source:
+        eos.update(timestep_source=True, source_coeffs=list(case.source_coeffs))
     return json.dumps({**eos,
         "dim": case.dim, "N": list(case.N), "L": list(case.L), "M": list(case.M),
         "Cv": list(case.Cv), "R": case.R, "flux": int(case.flux),
@@ -440,6 +442,41 @@ def vdw_diffusion_cases(base):
     run_const("seeded_1d_vdw_gs_hancock", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 113))
 
 
+def courant_source_cases(base, nsteps=5, keep=(1, 3, 5)):
+    """The "Courant-Source" time step: the explicit enthalpy flow of the
+    diffusion stage sets the source step (Diffusion.cpp:696-700), which caps
+    the next Courant step (Schemi.cpp:828-833).  With D = 0.02 on the seeded
+    mass fractions the source step binds from step 2 on and falls below
+    #Minimal_timestep at step 5.  The oracle finds the step sequence; the
+    reference is run to the time after nsteps and must take the same steps."""
+    tr = dict(diffusion=True, transport=(0.004, 0.02, 0.002), molar_mass_correction=True,
+              enthalpy_flow=True, timestep_source=True)
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, courant=0.3,
+             min_time=1e-6, **base, **tr)
+    init = seeded_primitives(c, 101)
+    o = Oracle(c)
+    o.init_primitive(*init)
+    t, dt, dts, times = 0.0, c.min_time, [], []
+    for _ in range(nsteps):
+        assert o.step(dt, 0) == 0
+        dts.append(dt)
+        t = t + dt
+        times.append(t)
+        dt = o.courant_dt(c.courant, c.min_time)
+    with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as root:
+        decks.write_deck(root, c, t_end=t, restart=init)
+        r = decks.run(root)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference failed\n{r.stdout[-2000:]}\n{r.stderr}")
+        tt = decks.read_times(root)
+        if tt.shape[0] != nsteps + 2:
+            raise RuntimeError(f"reference took {tt.shape[0] - 2} steps, expected {nsteps}")
+        snaps = {s: decks.read_output(root, s + 1, c.Ns) for s in keep}
+        ref_times = tt[2:, 1]
+    print("dts", dts, "reference times", ref_times)
+    save("seeded_1d_courant_source", c, dts, snaps, times=ref_times, restart=init)
+
+
 if __name__ == "__main__":
     b = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
     if len(sys.argv) > 1 and sys.argv[1] == "lz":
@@ -464,5 +501,7 @@ if __name__ == "__main__":
         species_flux_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "vdwdiff":
         vdw_diffusion_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "source":
+        courant_source_cases(b)
     else:
         main()

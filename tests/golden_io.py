@@ -34,7 +34,9 @@ def load(name):
                 sle_solver=d.get("sle_solver", "explicit"),
                 sle_iters=int(d.get("sle_iters", 2000)),
                 enthalpy_flow=bool(d.get("enthalpy_flow", False)),
-                molar_mass_correction=bool(d.get("molar_mass_correction", False)))
+                molar_mass_correction=bool(d.get("molar_mass_correction", False)),
+                timestep_source=bool(d.get("timestep_source", False)),
+                source_coeffs=tuple(d.get("source_coeffs", (0.25, 0.5))))
     snaps = {int(s): {k: z[f"s{s}_{k}"] for k in KEYS} for s in z["steps"]}
     out = {"case": case, "dts": z["dts"], "snaps": snaps}
     if "times" in z:

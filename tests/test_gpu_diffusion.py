@@ -1,7 +1,8 @@
 """The explicit molecular diffusion stage on the GPU (Diffusion.cpp with the
 explicit matrix solver, linear reconstruction and constant transport
-coefficients) beyond the per-step parity cases: Courant-step runs, the
-negative-temperature error, declined configurations."""
+coefficients) beyond the per-step parity cases: Courant-step runs (also with
+the "Courant-Source" cap), the negative-temperature error, declined
+configurations."""
 import numpy as np
 import pytest
 
@@ -139,3 +140,67 @@ def test_adjoint_declined_with_diffusion(hip_device):
     with pytest.raises(SchemiGpuError) as e:
         s.adjoint_step(1e-4, lam.data_ptr(), lam.data_ptr(), 0)
     assert e.value.code == errors.initialisationError
+
+
+def _courant_source_oracle(case, U0, steps):
+    from oracle.oracle import Oracle
+
+    Ns = case.Ns
+    o = Oracle(case)
+    o.set_conserved(U0[:Ns], U0[Ns], U0[Ns + 1:Ns + 4], U0[Ns + 4])
+    dt, t, dts = case.min_time, 0.0, []
+    for _ in range(steps):
+        assert o.step(dt, 1) == 0
+        dts.append(dt)
+        t = t + dt
+        dt = o.courant_dt(case.courant, case.min_time)
+    return o, t, dt, dts
+
+
+@pytest.mark.gpu
+def test_courant_source_run_matches_reference(hip_device):
+    """"Courant-Source" steps on the device: the explicit enthalpy flow's
+    source step caps the next Courant step (and falls below the minimal step
+    at step 5).  Five device-side steps follow the oracle bit for bit and the
+    reference's times and state to 1e-12."""
+    from oracle.oracle import Oracle
+    from tests.golden_io import conserved, load
+
+    g = load("seeded_1d_courant_source")
+    case = g["case"]
+    o0 = Oracle(case)
+    o0.init_primitive(*g["restart"])
+    U0 = o0.conserved()
+    o, t, dt, dts = _courant_source_oracle(case, U0, 5)
+    s = GpuSolver(case)
+    s.set_conserved(U0)
+    t_gpu, steps, dt_gpu = s.run(5, case.min_time)
+    assert steps == 5
+    assert t_gpu == t and dt_gpu == dt
+    assert np.array_equal(s.conserved(), o.conserved())
+    assert abs(t_gpu - g["times"][-1]) <= 1e-12 * g["times"][-1]
+    ref = conserved(g["snaps"][5])
+    got = s.conserved()
+    for q in range(ref.shape[0]):
+        assert np.abs(got[q] - ref[q]).max() <= 1e-12 * max(np.abs(ref[q]).max(), 1e-300)
+
+
+@pytest.mark.gpu
+def test_courant_source_run_multiblock(hip_device):
+    """The source step is a minimum over all cells, reduced across the
+    workgroups: a 2-D run of 8192 cells with the source cap binding follows
+    the oracle's step sequence and state bit for bit."""
+    case = Case(dim=2, N=(128, 64, 1), L=(1.0, 0.5, 0.1), limiter=TL.vanLeer, courant=0.3,
+                min_time=1e-6, **BASE, diffusion=True, transport=(0.004, 0.02, 0.002),
+                molar_mass_correction=True, enthalpy_flow=True, timestep_source=True)
+    case.set_bc(3, BCT.slip)
+    case.set_bc(4, BCT.slip)
+    c, v, p = random_primitives(case, 37)
+    U0 = conserved_from_primitive(case, c, v, p)
+    o, t, dt, dts = _courant_source_oracle(case, U0, 4)
+    s = GpuSolver(case)
+    s.set_conserved(U0)
+    t_gpu, steps, dt_gpu = s.run(4, case.min_time)
+    assert steps == 4
+    assert t_gpu == t and dt_gpu == dt
+    assert np.array_equal(s.conserved(), o.conserved())

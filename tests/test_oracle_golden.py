@@ -5,6 +5,8 @@ the 1-D two-component shock tube with constant and Courant steps, and seeded
 1-D/2-D/3-D states for every limiter, both fluxes, gravity, slip and
 fixed-value boundaries.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -36,13 +38,36 @@ def test_oracle_matches_reference(name):
             break
 
 
-def test_courant_sequence_matches_reference():
-    g = load("shock_tube_1d_cfl")
+@pytest.mark.parametrize("name", ["shock_tube_1d_cfl", "seeded_1d_courant_source"])
+def test_courant_sequence_matches_reference(name):
+    """The oracle's own step sequence reproduces the times the reference
+    stepped through (Courant; Courant-Source with the enthalpy-flow source
+    step capping it)."""
+    g = load(name)
     case = g["case"]
     o = Oracle(case)
-    o.set_full(g["snaps"][0])
-    dt = case.min_time
+    if "restart" in g:
+        o.init_primitive(*g["restart"])
+    else:
+        o.set_full(g["snaps"][0])
+    dt, t = case.min_time, 0.0
     for s in range(len(g["dts"])):
         assert dt == g["dts"][s]
         assert o.step(dt, 0) == 0
+        t = t + dt
+        assert t == g["times"][s]
         dt = o.courant_dt(case.courant, case.min_time)
+
+
+def test_courant_source_fixture_discriminates():
+    """The same run with the plain Courant step leaves the reference's times
+    after the first step: the fixture pins the source cap and its order with
+    #Minimal_timestep (the last step's source step is below it)."""
+    g = load("seeded_1d_courant_source")
+    case = dataclasses.replace(g["case"], timestep_source=False)
+    o = Oracle(case)
+    o.init_primitive(*g["restart"])
+    assert o.step(case.min_time, 0) == 0
+    dt = o.courant_dt(case.courant, case.min_time)
+    assert dt > 1.5 * g["dts"][1]
+    assert g["dts"][-1] == case.min_time

@@ -224,7 +224,8 @@ int schemi_gpu_get_state(schemi_gpu_ctx *ctx, double *cons, double *prim,
 
 /* One SSP-RK2 step with time step dt (secondOrderStepSolverRK::calculateStep).
  * On return *dt_next (if not NULL) holds the Courant step computed from the
- * new state (Schemi.cpp:807-833). */
+ * new state (Schemi.cpp:807-833), capped by the step's source step when
+ * timestep_source is 1. */
 int schemi_gpu_step(schemi_gpu_ctx *ctx, double dt, double *dt_next);
 
 /* nsteps SSP-RK2 steps with the time step kept on the device: the first

@@ -533,6 +533,8 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         P.Cv[k] = mix->Cv[k];
         P.Mf[k] = static_cast<float>(mix->M[k]);
         P.Cvf[k] = static_cast<float>(mix->Cv[k]);
+        P.iMf[k] = static_cast<float>(1.0 / mix->M[k]);
+        P.CvMf[k] = static_cast<float>(mix->Cv[k] / mix->M[k]);
     }
     P.R = mix->R;
     P.Rf = static_cast<float>(mix->R);
@@ -694,6 +696,23 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         c->mode = force_slopes ? MODE_SLOPES : MODE_FIRST_ORDER;
     else
         c->mode = MODE_SLOPES;
+    // Mixed path of the ideal mixture with SSP-RK2 where the slopes vanish:
+    // the lean kernel, when the mesh splits into whole 64 x SG_LEAN_TY tiles
+    // and every wall is free or slip (SCHEMI_GPU_LEAN=0 keeps stage0_kernel)
+    {
+        const char *lean = std::getenv("SCHEMI_GPU_LEAN");
+        bool walls = true;
+        for (int a = 0; a < dim; ++a)
+            for (int side = 0; side < 2 && !periodic[a]; ++side)
+                for (int q = 0; q < mix->ns + 2; ++q) {
+                    const int t = P.bct[patch_of(a, side)][q];
+                    walls = walls && (t == BC_FREE || t == BC_SLIP);
+                }
+        if (c->mode == MODE_FIRST_ORDER && c->prec == SCHEMI_PREC_MIXED &&
+            mix->gas_model == SCHEMI_GAS_IDEAL && !c->hancock && dim >= 2 && walls &&
+            P.N[0] % 64 == 0 && P.N[1] % SG_LEAN_TY == 0 && !(lean && lean[0] == '0'))
+            c->mode = MODE_LEAN;
+    }
     // MUSCL-Hancock: the slope kernel in 1-D, the first-order kernel where
     // the slopes vanish, the general-geometry kernels for the linear limiter
     // in 2-D/3-D (every slot's predicted side is stored)
@@ -1004,7 +1023,8 @@ int schemi_gpu_adjoint_step(schemi_gpu_ctx *c, double dt, const double *lam_in, 
 {
     if (!c || !lam_in || !lam_out || (what != SCHEMI_ADJ_STEP && what != SCHEMI_ADJ_STAGE))
         return SCHEMI_ERR_INIT;
-    const bool kernel = c->mode == MODE_FIRST_ORDER || (c->mode == MODE_SLOPES && c->dim == 1);
+    const bool kernel = c->mode == MODE_FIRST_ORDER || c->mode == MODE_LEAN ||
+                        (c->mode == MODE_SLOPES && c->dim == 1);
     if (!kernel || c->hancock || c->diffusion || c->hp.gas_model == GAS_RK ||
         c->hp.gas_model == GAS_KATAOKA)
         return SCHEMI_ERR_INIT;

@@ -60,6 +60,8 @@ struct DevParams {
     int limiter, flux;
     double M[MAXS], Cv[MAXS], R;
     float Mf[MAXS], Cvf[MAXS], Rf;
+    // mixed lean kernel: 1 / M_k and Cv_k / M_k (molar Cv per unit mass)
+    float iMf[MAXS], CvMf[MAXS];
     // equation of state; van der Waals mixing matrices a_kl, b_kl
     int gas_model;
     double akl[MAXS][MAXS], bkl[MAXS][MAXS];

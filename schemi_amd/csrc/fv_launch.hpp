@@ -21,6 +21,10 @@ namespace sg {
 #ifndef SG_TILE0_Y
 #define SG_TILE0_Y 8
 #endif
+// Rows (wavefronts) per workgroup of the lean mixed kernel (64 columns).
+#ifndef SG_LEAN_TY
+#define SG_LEAN_TY 8
+#endif
 template <int DIM, bool FIRST_ORDER = false> struct TileShape {
     static constexpr int TX = (DIM == 1) ? 256 : (FIRST_ORDER ? SG_TILE0_X : SG_TILE_X);
     static constexpr int TY = (DIM == 1) ? 1 : (FIRST_ORDER ? SG_TILE0_Y : SG_TILE_Y);
@@ -31,6 +35,8 @@ enum : int {
     MODE_FIRST_ORDER = 0,  // slopes vanish identically: stage0_kernel
     MODE_SLOPES = 1,       // fused slopes (TVD in 1-D, linear on separable meshes)
     MODE_GENERAL = 2,      // linear limiter with transverse face offsets
+    MODE_LEAN = 3,         // slopes vanish, mixed path, ideal mixture, free/slip/periodic
+                           // walls, whole 64 x SG_LEAN_TY tiles: stage_lean_kernel
 };
 
 struct StageLaunch {

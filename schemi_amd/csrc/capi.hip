@@ -260,7 +260,9 @@ int launch_stage(schemi_gpu_ctx *c, const double *Uin, double *Uout, const doubl
         HIPCHK(hipEventCreate(&e1));
         HIPCHK(hipEventRecord(e0, c->stream));
     }
-    const int r = c->prec == SCHEMI_PREC_MIXED ? launch_stage_mixed(L) : launch_stage_f64(L);
+    const int r = c->mode == MODE_LEAN               ? launch_stage_lean(L)
+                  : c->prec == SCHEMI_PREC_MIXED ? launch_stage_mixed(L)
+                                                 : launch_stage_f64(L);
     if (c->timing) {
         HIPCHK(hipEventRecord(e1, c->stream));
         c->ev.push_back(e0);
@@ -753,6 +755,12 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
             for (int cls = 0; cls < 2; ++cls)
                 P.off_td[a][b][cls] = (a == b) ? 0 : put(c->tables.td[a][b][cls]);
     c->kchunk = SG_KCHUNK;
+    if (const char *kc = std::getenv("SCHEMI_GPU_KCHUNK")) {
+        // z planes per workgroup of the 3-D kernels (launch shape only)
+        const int v = std::atoi(kc);
+        if (v >= 1 && v <= 4096)
+            c->kchunk = v;
+    }
 
     if (hipGetDevice(&c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
         delete c;

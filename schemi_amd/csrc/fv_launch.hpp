@@ -60,6 +60,7 @@ struct StageLaunch {
 // otherwise a hipError_t value.
 int launch_stage_f64(const StageLaunch &L);
 int launch_stage_mixed(const StageLaunch &L);
+int launch_stage_lean(const StageLaunch &L);  // MODE_LEAN (fv_lean.hip)
 
 // Adjoint of one first-order-exact stage linearised at U (FP64):
 // out = a (dL/dU)^T lam + b add (add may be null).

@@ -5,7 +5,6 @@
 #include "fv_general.hpp"
 #include "fv_launch.hpp"
 #include "fv_stage.hpp"
-#include "fv_stage_lean.hpp"
 
 namespace sg {
 
@@ -43,18 +42,6 @@ static int launch_mode(const StageLaunch &L)
             hipLaunchKernelGGL((stage0_kernel<T, NS, DIM, SV, MIXED, TX0, TY0, GAS_IDEAL>),
                                (tile_grid<DIM, true>(L)), dim3(TX0 * TY0), 0, L.stream, L.P,
                                L.geo, L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err, L.kchunk);
-        break;
-    case MODE_LEAN:
-        if constexpr (MIXED && DIM >= 2 && SV != SV_HANCOCK) {
-            constexpr int TYL = SG_LEAN_TY;
-            const dim3 g((unsigned)(L.nx / 64), (unsigned)(L.ny / TYL),
-                         (DIM == 3) ? (unsigned)((L.nz + L.kchunk - 1) / L.kchunk) : 1u);
-            hipLaunchKernelGGL((stage_lean_kernel<NS, DIM, SV, TYL>), g, dim3(64 * TYL), 0,
-                               L.stream, L.P, L.geo, L.Uin, L.Uout, L.Un, L.dt, L.sigmax, L.err,
-                               L.kchunk);
-        } else {
-            return -1;
-        }
         break;
     case MODE_SLOPES:
         // the slope kernel runs MUSCL-Hancock on 1-D meshes only

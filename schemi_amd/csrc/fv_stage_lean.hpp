@@ -182,6 +182,16 @@ __device__ __forceinline__ double rcp_f64(double x)
     return __builtin_fma(r, e, r);
 }
 
+// Streaming hints: the new state and U^n are touched once per stage, so
+// their lines are marked non-temporal and leave L2 room for the rows that
+// neighbouring tiles read again as their halo.
+#ifndef SG_LEAN_NT_STORE
+#define SG_LEAN_NT_STORE 1
+#endif
+#ifndef SG_LEAN_NT_UN
+#define SG_LEAN_NT_UN 1
+#endif
+
 template <int NS, int DIM, int SV, int TY>
 __global__ void __launch_bounds__(64 * TY, (TY == 8) ? 4 : 2)
 stage_lean_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
@@ -194,7 +204,8 @@ stage_lean_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
     using L = LeanZ<NS>;
     constexpr int NV = L::NV;
     constexpr int NF = NS + 5;
-    // closed states: rows 0 and TY + 1 and columns 0 and TX + 1 are the halo
+    // closed states of the plane: rows 0 and TY + 1 and columns 0 and TX + 1
+    // are the halo
     __shared__ float sZ[NV][TY + 2][TX + 2];
     // face fluxes: x faces x0 + c (c = 0..TX) of each row, y faces y0 + r
     // (r = 0..TY) of each column
@@ -202,7 +213,7 @@ stage_lean_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
     __shared__ float sFy[NF][TY + 1][TX];
     __shared__ double sRed[NT / 64];
     // the own column's z+ flux waits here between its evaluation and the
-    // update (fewer live registers through the face phase)
+    // update (fewer live registers)
     __shared__ float sFz[(DIM == 3) ? NF : 1][TY][TX];
 
     const DevParams &P = *Pp;
@@ -239,75 +250,10 @@ stage_lean_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
     const double ex = geo[P.off_e[0] + i];
     const double ey = geo[P.off_e[1] + j];
 
-    auto closed_at = [&](int ci, int cj, int ck, float *z) {
-        double u[NF];
-        load_cons<NS>(Uin, n, ci + sy * cj + sz * (long long)ck, u);
-        lean_state<NS>(P, u, z);
-    };
-
-    float Zc[NV], Zn[NV];
-    float fzm[NF];
-    double uc[NF], unx[NF];
-    if constexpr (DIM == 3) {
-        load_cons<NS>(Uin, n, i + sy * j + sz * kb, uc);
-        lean_state<NS>(P, uc, Zc);
-        if (kb > 0 || zper) {
-            float zm[NV];
-            closed_at(i, j, (kb > 0) ? kb - 1 : Nz - 1, zm);
-            if (!lean_flux<NS>(flux, zm, Zc, 2, 1.0f, fzm))
-                err |= ERRF_RIEMANN;
-        } else {
-            float gz[NV];
-#pragma unroll
-            for (int q = 0; q < NV; ++q)
-                gz[q] = Zc[q];
-            lean_reflect<NS>(gz, 2, slip_z0);
-            if (!lean_flux<NS>(flux, Zc, gz, 2, -1.0f, fzm))
-                err |= ERRF_RIEMANN;
-        }
-    }
-
-    for (int k = kb; k < ke; ++k) {
-        // ---- closed states of plane k and the own column's z+ face -------
-        if constexpr (DIM == 3) {
-            float fzp[NF];
-            if (k > kb) {
-                // plane k was closed one iteration earlier and parked in the
-                // own slot of the LDS plane buffer
-#pragma unroll
-                for (int q = 0; q < NV; ++q)
-                    Zc[q] = sZ[q][ty + 1][tx + 1];
-            }
-            if (k + 1 < Nz || zper) {
-                const int kp = (k + 1 < Nz) ? k + 1 : 0;
-                load_cons<NS>(Uin, n, i + sy * j + sz * kp, unx);
-                lean_state<NS>(P, unx, Zn);
-                if (!lean_flux<NS>(flux, Zc, Zn, 2, 1.0f, fzp))
-                    err |= ERRF_RIEMANN;
-            } else {
-#pragma unroll
-                for (int q = 0; q < NV; ++q)
-                    Zn[q] = Zc[q];
-                lean_reflect<NS>(Zn, 2, slip_z1);
-                if (!lean_flux<NS>(flux, Zc, Zn, 2, zsg_hi, fzp))
-                    err |= ERRF_RIEMANN;
-            }
-#pragma unroll
-            for (int q = 0; q < NF; ++q)
-                sFz[q][ty][tx] = fzp[q];
-        } else {
-            load_cons<NS>(Uin, n, i + sy * j, uc);
-            lean_state<NS>(P, uc, Zc);
-        }
-        const int kk = (DIM == 3) ? k : 0;
-        const long long id = i + sy * j + sz * kk;
-        if (DIM != 3 || k == kb) {
-#pragma unroll
-            for (int q = 0; q < NV; ++q)
-                sZ[q][ty + 1][tx + 1] = Zc[q];
-        }
-        // halo: rows y0 - 1 and y0 + TY (2 TX cells), then columns x0 - 1 and
-        // x0 + TX (2 TY cells); wrapped on periodic axes, absent elsewhere
+    // halo of plane kk into LDS: rows y0 - 1 and y0 + TY (2 TX cells), then
+    // columns x0 - 1 and x0 + TX (2 TY cells); wrapped on periodic axes,
+    // absent at walls
+    auto stage_halo = [&](int kk) {
         for (int h = tid; h < 2 * TX + 2 * TY; h += NT) {
             int row, col, gx, gy;
             bool ok = true;
@@ -338,19 +284,65 @@ stage_lean_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
                 }
             }
             if (ok) {
+                double u[NF];
                 float z[NV];
-                closed_at(gx, gy, kk, z);
+                load_cons<NS>(Uin, n, gx + sy * gy + sz * (long long)kk, u);
+                lean_state<NS>(P, u, z);
 #pragma unroll
                 for (int q = 0; q < NV; ++q)
                     sZ[q][row][col] = z[q];
             }
         }
+    };
+
+    // prologue: the own cell of the first plane, its z- face, the first halo
+    float Zc[NV];
+    float fzm[NF];
+    double uc[NF], unx[NF];
+    load_cons<NS>(Uin, n, i + sy * j + sz * kb, uc);
+    lean_state<NS>(P, uc, Zc);
+    if constexpr (DIM == 3) {
+        float zm[NV];
+        if (kb > 0 || zper) {
+            double u[NF];
+            load_cons<NS>(Uin, n, i + sy * j + sz * ((kb > 0) ? kb - 1 : Nz - 1), u);
+            lean_state<NS>(P, u, zm);
+            if (!lean_flux<NS>(flux, zm, Zc, 2, 1.0f, fzm))
+                err |= ERRF_RIEMANN;
+        } else {
+#pragma unroll
+            for (int q = 0; q < NV; ++q)
+                zm[q] = Zc[q];
+            lean_reflect<NS>(zm, 2, slip_z0);
+            if (!lean_flux<NS>(flux, Zc, zm, 2, -1.0f, fzm))
+                err |= ERRF_RIEMANN;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+        sZ[q][ty + 1][tx + 1] = Zc[q];
+    stage_halo(kb);
+
+    for (int k = kb; k < ke; ++k) {
+        const bool znext = (DIM == 3) && (k + 1 < Nz || zper);
+        if constexpr (DIM == 3) {
+            if (k > kb) {
+                // plane k was closed during the previous plane and parked in
+                // the own slot of the LDS plane buffer
+#pragma unroll
+                for (int q = 0; q < NV; ++q)
+                    Zc[q] = sZ[q][ty + 1][tx + 1];
+            }
+            // conserved values of the next plane: in flight through the
+            // face phase
+            if (znext)
+                load_cons<NS>(Uin, n, i + sy * j + sz * ((k + 1 < Nz) ? k + 1 : 0), unx);
+        }
         __syncthreads();
 
         // ---- x- and y- faces of the own cell, the tile's seams -----------
         {
-            float fx[NF];
-            float O[NV], B[NV];
+            float O[NV], B[NV], fx[NF];
 #pragma unroll
             for (int q = 0; q < NV; ++q) {
                 O[q] = sZ[q][ty + 1][tx];
@@ -427,17 +419,41 @@ stage_lean_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
             for (int q = 0; q < NF; ++q)
                 sFx[q][r][TX] = g[q];
         }
-        __syncthreads();
+        // ---- the next plane's closed state and the own z+ face -----------
+        float Zn[NV];
         if constexpr (DIM == 3) {
-            // the next plane's closed state into the own slot (no wavefront
-            // reads the plane buffer again before the next first barrier)
+            float fzp[NF];
+            if (znext) {
+                lean_state<NS>(P, unx, Zn);
+                if (!lean_flux<NS>(flux, Zc, Zn, 2, 1.0f, fzp))
+                    err |= ERRF_RIEMANN;
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q)
+                    Zn[q] = Zc[q];
+                lean_reflect<NS>(Zn, 2, slip_z1);
+                if (!lean_flux<NS>(flux, Zc, Zn, 2, zsg_hi, fzp))
+                    err |= ERRF_RIEMANN;
+            }
+#pragma unroll
+            for (int q = 0; q < NF; ++q)
+                sFz[q][ty][tx] = fzp[q];
+        }
+        __syncthreads();
+        // no wavefront reads the plane buffer again before the next first
+        // barrier: park the next plane's own cells and stage its halo
+        if constexpr (DIM == 3) {
 #pragma unroll
             for (int q = 0; q < NV; ++q)
                 sZ[q][ty + 1][tx + 1] = Zn[q];
+            if (k + 1 < ke)
+                stage_halo(k + 1);
         }
 
         // ---- residual and update of the own cell (FP64) -------------------
         {
+            const int kk = (DIM == 3) ? k : 0;
+            const long long id = i + sy * j + sz * kk;
             // volume (ex ey) ez and face areas (mesh.cpp:1291-1293); in 2-D
             // ez is the extent of the single cell layer
             const double ez = geo[P.off_e[2] + kk];
@@ -447,8 +463,12 @@ stage_lean_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
             const bool ylo = (DIM >= 2) && (j == 0) && !yper;
             const bool zlo = (DIM == 3) && (k == 0) && !zper;
             double un[NF];
-            if constexpr (SV == SV_RK2)
-                load_cons<NS>(Un, n, id, un);
+            if constexpr (SV == SV_RK2) {
+#pragma unroll
+                for (int q = 0; q < NF; ++q)
+                    un[q] = SG_LEAN_NT_UN ? __builtin_nontemporal_load(Un + q * n + id)
+                                          : Un[q * n + id];
+            }
             const double dtV = dt * rcp_f64(V);
             double unew[NF];
 #pragma unroll
@@ -501,8 +521,12 @@ stage_lean_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
                 sig = tmax(sig, double((vm + z[L::CS]) * ml));
             }
 #pragma unroll
-            for (int q = 0; q < NF; ++q)
-                Uout[q * n + id] = unew[q];
+            for (int q = 0; q < NF; ++q) {
+                if (SG_LEAN_NT_STORE)
+                    __builtin_nontemporal_store(unew[q], Uout + q * n + id);
+                else
+                    Uout[q * n + id] = unew[q];
+            }
         }
         if constexpr (DIM == 3) {
 #pragma unroll

@@ -1,16 +1,20 @@
 """Benchmark of the explicit SSP-RK2 step on the 3-D Richtmyer-Meshkov case.
 
 Workload: 256 x 128 x 128 = 4,194,304 hexahedral cells on [0,2]x[0,1]x[0,1],
-two species, HLLC flux with the van Albada TVD limiter, Courant time step
-kept on the device.  One step = one SSP-RK2 step (two fused stage kernels +
-the time-step reduction) over all cells; inputs are resident in HBM.
+two species, HLLC flux with the van Albada TVD limiter, transmissive x and
+periodic y/z boundaries, Courant time step kept on the device.  One step =
+one SSP-RK2 step (two fused stage kernels + the time-step reduction) over all
+cells; inputs are resident in HBM.
 
 Reported: Mcell-updates/s of the mixed-precision path (FP32 reconstruction,
 limiter and Riemann solve; FP64 state, accumulation and update) next to the
-FP64 path, the density/pressure L1 error of the mixed path against FP64 after
-1000 steps, the HBM roofline fraction of the stage kernel (algorithmic bytes
-per launch over its average HIP-event duration), and the reference solver's
-CPU step timed on a bounded sample of the same workload.
+FP64 path, the density/pressure/mixing-width error of the mixed path against
+FP64 after 1000 steps (and after 2000 steps of the 2-D Rayleigh-Taylor case,
+256 x 1024, periodic x), the HBM roofline fraction of the stage kernel
+(algorithmic bytes per launch over its average HIP-event duration), the
+stage kernel with the TVD limiter evaluated in full as a secondary figure,
+and the reference solver's CPU step timed on a bounded sample of the same
+workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -52,6 +56,8 @@ def parse():
     ap.add_argument("--ny", type=int, default=128)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--l1-steps", type=int, default=1000)
+    ap.add_argument("--rt-steps", type=int, default=2000,
+                    help="steps of the 2-D Rayleigh-Taylor FP64-vs-mixed gate")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 1 GiB device copy (calibrates rocprofv3 byte counters)")
@@ -96,7 +102,8 @@ def mixing_width(U, V):
 
 def l1_gate(case, U0, nsteps):
     """Density and pressure L1 of the mixed path against FP64 after nsteps
-    steps with the same constant time step (0.8 x the initial Courant step)."""
+    steps with the same constant time step (0.8 x the initial Courant step),
+    and the relative difference of the mixing width."""
     from schemi_amd.gpu import PREC_FP64, PREC_MIXED, GpuSolver
 
     probe = GpuSolver(case, PREC_FP64)
@@ -112,7 +119,9 @@ def l1_gate(case, U0, nsteps):
         out[prec] = (s.conserved(), s.primitives()["p"])
         s.close()
     (U64, p64), (Umx, pmx) = out[PREC_FP64], out[PREC_MIXED]
-    V = (2.0 / case.N[0]) * (1.0 / case.N[1]) * (1.0 / case.N[2])
+    V = 1.0
+    for a in range(case.dim):
+        V *= case.L[a] / case.N[a]
     w64, wmx = mixing_width(U64, V), mixing_width(Umx, V)
     return {
         "steps": nsteps, "dt": dt,
@@ -134,22 +143,24 @@ def _ref_stage_ms(root):
     return sum(v for k, v in vals.items() if "Diffusion" not in k), vals
 
 
-def cpu_baseline(nx=96, ny=48, nz=48, steps=2, diffusion=False):
-    """The reference solver (oracle/_ref/schemi, serial) on a 96x48x48 sample
-    of the same workload; falls back to the C oracle port when the reference
-    binary is absent.  Returns Mcell-updates/s per the same definition, from
-    the reference's own stage timers (no I/O)."""
+def cpu_baseline(nx=128, ny=64, nz=64, steps=3, diffusion=False):
+    """The reference solver (oracle/_ref/schemi, serial) on a 128x64x64 sample
+    of the same workload for 3 steps, with slip y/z walls in place of the
+    periodic ones (the reference has no periodic boundary); falls back to the
+    C oracle port when the reference binary is absent.  Returns
+    Mcell-updates/s per the same definition, from the reference's own stage
+    timers (no I/O), with the host's core count beside it."""
     try:
         from oracle import decks
         from schemi_amd import cases
         from schemi_amd.gpu import conserved_from_primitive  # noqa: F401
     except Exception as e:  # pragma: no cover
         return {"error": str(e)}
-    case, U0 = cases.richtmyer_meshkov_3d(nx=nx, ny=ny, nz=nz, seed=2024)
+    case, U0 = cases.richtmyer_meshkov_3d(nx=nx, ny=ny, nz=nz, seed=2024, periodic_yz=False)
     if diffusion:
         case.diffusion, case.transport = True, DIFF_TRANSPORT
     n = case.ncells
-    sample = f"3-D RM {nx}x{ny}x{nz} ({n} cells), {steps} SSP-RK2 steps" + (
+    sample = f"3-D RM {nx}x{ny}x{nz} ({n} cells, slip y/z walls), {steps} SSP-RK2 steps" + (
         " with the diffusion stage" if diffusion else "")
     if decks.ref_available():
         # primitives for the restart reader
@@ -174,6 +185,7 @@ def cpu_baseline(nx=96, ny=48, nz=48, steps=2, diffusion=False):
                 return {"error": "reference run failed"}
             ms, parts = _ref_stage_ms(root)
         out = {"value": n * steps / (ms * 1e-3) / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+               "host_cores": os.cpu_count(),
                "kind": "reference", "wall_s_incl_io": wall, "stage_ms": ms,
                "sample": sample + ", serial reference build (-O2, one process, MPI "
                "decomposition collapsed); time = sum of its stage timers"}
@@ -193,6 +205,7 @@ def cpu_baseline(nx=96, ny=48, nz=48, steps=2, diffusion=False):
         o.step(1e-4, 1)
     wall = time.perf_counter() - t0
     return {"value": n * steps / wall / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+            "host_cores": os.cpu_count(),
             "kind": "port", "sample": sample + ", C restatement (oracle), one core"}
 
 
@@ -316,9 +329,28 @@ def main():
                      "value": n * a.steps / wall / 1e6,
                      "stage_kernel_ms": avg_ms, "launches": launches}
         print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
+    # the TVD-stencil kernel (limiter evaluated in full; same bits as the
+    # first-order kernel in 2-D/3-D), mixed path, as a secondary figure
+    os.environ["SCHEMI_GPU_FULL_LIMITER"] = "1"
+    try:
+        s = GpuSolver(case, PREC_MIXED)
+        wall, avg_ms, launches = timed_run(s, U0, min(a.steps, 50), a.warmup)
+        s.close()
+    finally:
+        del os.environ["SCHEMI_GPU_FULL_LIMITER"]
+    full_limiter = {"ms_per_step": 1e3 * wall / min(a.steps, 50),
+                    "value": n * min(a.steps, 50) / wall / 1e6, "stage_kernel_ms": avg_ms,
+                    "hbm_frac": BYTES_PER_CELL_LAUNCH * n / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    print(f"# full limiter (stage_kernel, mixed): {full_limiter}", file=sys.stderr, flush=True)
     copy_gbs = stream_copy_gbs()
     gate = l1_gate(case, U0, a.l1_steps)
     print(f"# l1 gate: {gate}", file=sys.stderr, flush=True)
+    # 2-D Rayleigh-Taylor 256 x 1024, periodic x, 2000 steps: FP64 vs mixed
+    rt_case, rt_U0 = cases.rayleigh_taylor_2d()
+    gate_rt = l1_gate(rt_case, rt_U0, a.rt_steps)
+    gate_rt["workload"] = ("2-D Rayleigh-Taylor 256x1024, HLLC + van Leer, gravity -1, "
+                           "periodic x, slip y")
+    print(f"# l1 gate RT: {gate_rt}", file=sys.stderr, flush=True)
     cpu = None if a.no_cpu else cpu_baseline()
     mx = res["mixed"]
     achieved = BYTES_PER_CELL_LAUNCH * n / (mx["stage_kernel_ms"] * 1e-3) / 1e9
@@ -338,8 +370,8 @@ def main():
         "data": "synthetic (seeded 3-D Richtmyer-Meshkov initial state)",
         "config": {"workload": f"3-D Richtmyer-Meshkov {a.nx}x{a.ny}x{a.nz} "
                                f"({n} cells), 2 species, HLLC + van Albada, SSP-RK2, "
-                               "Courant step on device",
-                   "cells": n, "precision": "mixed"},
+                               "transmissive x, periodic y/z, Courant step on device",
+                   "cells": n, "precision": "mixed", "boundaries": "free x, periodic y/z"},
         "fp64": {"value": res["fp64"]["value"], "ms_per_step": res["fp64"]["ms_per_step"],
                  "stage_kernel_ms": res["fp64"]["stage_kernel_ms"],
                  "hbm_frac": BYTES_PER_CELL_LAUNCH * n / (res["fp64"]["stage_kernel_ms"] *
@@ -347,6 +379,9 @@ def main():
         "mixed_over_fp64": mx["value"] / res["fp64"]["value"],
         "l1_gate": gate,
         "l1_ok": gate["rho_l1_rel"] <= 1e-5,
+        "l1_gate_rt2d": gate_rt,
+        "l1_rt2d_ok": gate_rt["rho_l1_rel"] <= 1e-5,
+        "full_limiter": full_limiter,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},

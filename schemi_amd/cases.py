@@ -87,18 +87,20 @@ def rankine_hugoniot(mach: float, gamma: float, rho1: float, p1: float):
 
 
 def richtmyer_meshkov_3d(nx: int = 256, ny: int = 128, nz: int = 128, seed: int = 2024,
-                         x_shock: float = 0.4, x_iface: float = 0.6, modes: int = 32):
+                         x_shock: float = 0.4, x_iface: float = 0.6, modes: int = 32,
+                         periodic_yz: bool = True):
     """3-D Richtmyer-Meshkov: Mach 1.2 shock in light gas (gamma 1.4, M 29,
     rho 1, p 1) hitting a heavy layer (gamma 1.09, M 146, rho 5.03, p 1) at
     x = 0.6 + eta(y, z), eta a sum of 32 seeded Fourier modes with integer
     wavevectors 2 <= |k| <= 16, amplitudes U[0, 0.005], phases U[0, 2 pi).
-    Domain [0,2]x[0,1]x[0,1]; HLLC + van Albada; free x, slip y/z (the
-    reference has no periodic boundary)."""
+    Domain [0,2]x[0,1]x[0,1]; HLLC + van Albada; transmissive (free) x,
+    periodic y/z (periodic_yz=False: slip y/z walls, the surrogate the
+    reference itself can run, as it has no periodic boundary)."""
     case = Case(dim=3, N=(nx, ny, nz), L=(2.0, 1.0, 1.0), M=[29.0, 146.0],
                 Cv=[cv_of(1.4), cv_of(1.09)], flux=flowSolver.HLLC,
                 limiter=typeOfTVDLimiter.vanAlbada, courant=0.4, min_time=1e-15)
     for pt in (2, 3, 4, 5):
-        case.set_bc(pt, BCT.slip)
+        case.set_bc(pt, BCT.periodic if periodic_yz else BCT.slip)
     rng = SplitMix64(seed)
     ks = []
     while len(ks) < modes:
@@ -126,17 +128,21 @@ def richtmyer_meshkov_3d(nx: int = 256, ny: int = 128, nz: int = 128, seed: int 
 
 
 def rayleigh_taylor_2d(nx: int = 256, ny: int = 1024, seed: int = 7, a0: float = 0.01,
-                       modes=None):
+                       modes=None, periodic_x: bool = True):
     """2-D Rayleigh-Taylor: heavy (rho 3, M 87, gamma 1.4) over light (rho 1,
     M 29, gamma 1.4) on [0,0.25]x[0,1], gravity -1 in y, hydrostatic pressure
     with p = 2.5 at y = 0.5, interface 0.5 + a0 cos(8 pi x) + per-column noise
     of +-1 % of a0 (or a sum of single modes a_k cos(2 pi k x / 0.25)), tanh
-    mass-fraction profile of width 2 cells; HLLC + van Leer; slip walls."""
+    mass-fraction profile of width 2 cells; HLLC + van Leer; reflecting (slip)
+    walls in y, periodic in x (periodic_x=False: slip x walls, the reference's
+    surrogate)."""
     case = Case(dim=2, N=(nx, ny, 1), L=(0.25, 1.0, 1.0 / ny), M=[87.0, 29.0],
                 Cv=[cv_of(1.4), cv_of(1.4)], flux=flowSolver.HLLC,
                 limiter=typeOfTVDLimiter.vanLeer, gravity=True, g=(0.0, -1.0, 0.0),
                 courant=0.4, min_time=1e-15)
-    for pt in (0, 1, 3, 4):
+    for pt in (0, 1):
+        case.set_bc(pt, BCT.periodic if periodic_x else BCT.slip)
+    for pt in (3, 4):
         case.set_bc(pt, BCT.slip)
     rng = SplitMix64(seed)
     xs = (np.arange(nx) + 0.5) * (0.25 / nx)

@@ -44,6 +44,11 @@ template <int NS> struct LeanZ {
 // mixtureIdeal.cpp:383-409, 495-501): c_0 = sum_k rho_k / M_k,
 // g1 = R c_0 / sum_k rho_k Cv_k / M_k, U = E - rho v^2 / 2, p = g1 U,
 // c^2 = g1 (U + p) / rho.
+// The translation unit is compiled without contraction and the closure and
+// the fluxes below place their multiply-adds explicitly: a cell closed by its
+// own tile and as another tile's halo, and a seam face evaluated by both tiles
+// that share it, then give the same bits, so every face flux leaves one cell
+// exactly as it enters the other.
 template <int NS>
 __device__ __forceinline__ void lean_state(const DevParams &P, const double *u, float *z)
 {
@@ -53,15 +58,16 @@ __device__ __forceinline__ void lean_state(const DevParams &P, const double *u, 
     for (int k = 0; k < NS; ++k) {
         const float r = float(u[k]);
         z[k] = r;
-        c0 += r * P.iMf[k];
-        ccv += r * P.CvMf[k];
+        c0 = __builtin_fmaf(r, P.iMf[k], c0);
+        ccv = __builtin_fmaf(r, P.CvMf[k], ccv);
     }
     const float r0 = float(u[NS]);
     const float ir = 1.0f / r0;
     const float mx = float(u[NS + 1]), my = float(u[NS + 2]), mz = float(u[NS + 3]);
     const float E = float(u[NS + 4]);
     const float vx = mx * ir, vy = my * ir, vz = mz * ir;
-    const float U = E - 0.5f * r0 * (vx * vx + vy * vy + vz * vz);
+    const float v2 = __builtin_fmaf(vx, vx, __builtin_fmaf(vy, vy, vz * vz));
+    const float U = __builtin_fmaf(-0.5f * r0, v2, E);
     const float g1 = P.Rf * c0 / ccv;
     const float p = g1 * U;
     z[L::R0] = r0;
@@ -102,28 +108,30 @@ __device__ __forceinline__ bool lean_hllc(const float *O, const float *B, int ax
     const float SO = tmin(uO - O[L::CS], uB - B[L::CS]);
     const float SB = tmax(uO + O[L::CS], uB + B[L::CS]);
     const float dO = O[L::R0] * (SO - uO), dB = B[L::R0] * (SB - uB);
-    const float Sc = (B[L::PR] - O[L::PR] + dO * uO - dB * uB) / (dO - dB);
-    const float PLR = 0.5f * (O[L::PR] + B[L::PR] + dO * (Sc - uO) + dB * (Sc - uB));
+    const float num = __builtin_fmaf(-dB, uB, __builtin_fmaf(dO, uO, B[L::PR] - O[L::PR]));
+    const float Sc = num / (dO - dB);
+    const float PLR = 0.5f * __builtin_fmaf(dB, Sc - uB,
+                                            __builtin_fmaf(dO, Sc - uO, O[L::PR] + B[L::PR]));
     const bool own = (SO >= 0.0f) || (Sc >= 0.0f);
     const bool star = own ? !(SO >= 0.0f) : (SB > 0.0f);
     const bool ok = own || (SB > 0.0f) || (SB <= 0.0f);
     const float u = own ? uO : uB;
     const float S = own ? SO : SB;
     const float b = star ? S / (S - Sc) : 0.0f;
-    const float w = u + b * (Sc - u);
+    const float w = __builtin_fmaf(b, Sc - u, u);
     const float p = own ? O[L::PR] : B[L::PR];
-    const float pn = p + b * (PLR - p);
+    const float pn = __builtin_fmaf(b, PLR - p, p);
 #pragma unroll
     for (int k = 0; k <= NS; ++k)
         g[k] = (own ? O[k] : B[k]) * w;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const float m = own ? O[L::M + a] : B[L::M + a];
-        g[NS + 1 + a] = (a == ax) ? m * w + sg * pn : m * w;
+        g[NS + 1 + a] = (a == ax) ? __builtin_fmaf(m, w, sg * pn) : m * w;
     }
     const float E = own ? O[L::E] : B[L::E];
     const float pu = p * u;
-    g[NS + 4] = E * w + pu + b * (PLR * Sc - pu);
+    g[NS + 4] = __builtin_fmaf(E, w, __builtin_fmaf(b, __builtin_fmaf(PLR, Sc, -pu), pu));
     return ok;
 }
 
@@ -149,7 +157,7 @@ __device__ __forceinline__ bool lean_hll(const float *O, const float *B, int ax,
             return X[q] * u;
         if (q < NS + 4) {
             const int a = q - NS - 1;
-            return (a == ax) ? X[L::M + a] * u + sg * X[L::PR] : X[L::M + a] * u;
+            return (a == ax) ? __builtin_fmaf(X[L::M + a], u, sg * X[L::PR]) : X[L::M + a] * u;
         }
         return (X[L::E] + X[L::PR]) * u;
     };
@@ -158,7 +166,8 @@ __device__ __forceinline__ bool lean_hll(const float *O, const float *B, int ax,
     };
 #pragma unroll
     for (int q = 0; q < NS + 5; ++q)
-        g[q] = wO * fl(O, uO, q) + wB * fl(B, uB, q) + wd * (cons(B, q) - cons(O, q));
+        g[q] = __builtin_fmaf(wd, cons(B, q) - cons(O, q),
+                              __builtin_fmaf(wO, fl(O, uO, q), wB * fl(B, uB, q)));
     return ok;
 }
 
@@ -479,14 +488,14 @@ stage_lean_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
                 if constexpr (DIM >= 2) {
                     const double f2 = double(sFy[q][ty + 1][tx]);
                     const double f3 = double(sFy[q][ty][tx]);
-                    acc += (f2 - (ylo ? -f3 : f3)) * S1;
+                    acc = __builtin_fma(f2 - (ylo ? -f3 : f3), S1, acc);
                 }
                 if constexpr (DIM == 3) {
                     const double f4 = double(fzm[q]);
                     const double f5 = double(sFz[q][ty][tx]);
-                    acc += (f5 - (zlo ? -f4 : f4)) * S2;
+                    acc = __builtin_fma(f5 - (zlo ? -f4 : f4), S2, acc);
                 }
-                unew[q] = uc[q] - acc * dtV;
+                unew[q] = __builtin_fma(-acc, dtV, uc[q]);
             }
             if (P.gravity) {
                 // pre-update rho and v (Advection2dOrder.cpp:547-554)
@@ -511,12 +520,13 @@ stage_lean_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ g
             if constexpr (SV == SV_RK2) {
 #pragma unroll
                 for (int q = 0; q < NF; ++q)
-                    unew[q] = 0.5 * unew[q] + 0.5 * un[q];
+                    unew[q] = __builtin_fma(0.5, unew[q], 0.5 * un[q]);
                 // Courant signal speed (|v| + c) / min edge of the new state
                 float z[NV];
                 lean_state<NS>(P, unew, z);
-                const float vm = __builtin_sqrtf(z[L::V] * z[L::V] + z[L::V + 1] * z[L::V + 1] +
-                                                 z[L::V + 2] * z[L::V + 2]);
+                const float vm = __builtin_sqrtf(__builtin_fmaf(
+                    z[L::V], z[L::V],
+                    __builtin_fmaf(z[L::V + 1], z[L::V + 1], z[L::V + 2] * z[L::V + 2])));
                 const float ml = 1.0f / tmin(tmin(float(ex), float(ey)), float(ez));
                 sig = tmax(sig, double((vm + z[L::CS]) * ml));
             }

@@ -48,3 +48,37 @@ def load(name):
 
 def conserved(st):
     return np.concatenate([st["rho"], st["rho0"][None], st["m"], st["E"][None]])
+
+
+# Parity bars of the FP64 path, per conserved field (rows rho_1..rho_Ns, rho,
+# m_x, m_y, m_z, E):
+#  * one step from a common state: max over cells of |got - ref| / |ref| <= 1e-12,
+#    no floor (per_cell_rel with the default floors);
+#  * after several steps against the reference's outputs, whose separately
+#    averaged concentrations differ from the conserved-state model in the last
+#    bits and drift apart step by step: per cell relative to max(|ref|, floor)
+#    with floors LATER_FLOORS (fractions of the field's largest magnitude:
+#    partial densities can be exactly zero where a species is absent, momentum
+#    components pass through zero) <= 1e-10, and the field-normalised maximum
+#    <= 1e-12.
+LATER_FLOORS = {"rho_k": 1e-15, "m": 1e-4}
+
+
+def per_cell_rel(got, ref, ns, rho_k_floor=0.0, m_floor=0.0):
+    """Per-field maxima over cells of |got - ref| / max(|ref|, floor), the
+    floor a fraction of the field's largest magnitude (rho_k_floor for the
+    partial densities, m_floor for the momentum components, none for the
+    total density and the energy)."""
+    out = []
+    for q in range(ref.shape[0]):
+        frac = rho_k_floor if q < ns else (m_floor if ns + 1 <= q <= ns + 3 else 0.0)
+        floor = frac * float(np.abs(ref[q]).max())
+        den = np.maximum(np.abs(ref[q]), max(floor, 1e-300))
+        out.append(float((np.abs(got[q] - ref[q]) / den).max()))
+    return out
+
+
+def field_rel(got, ref):
+    """Per-field maxima of |got - ref| normalised by the field's largest magnitude."""
+    return [float(np.abs(got[q] - ref[q]).max() / max(np.abs(ref[q]).max(), 1e-300))
+            for q in range(ref.shape[0])]

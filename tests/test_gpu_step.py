@@ -11,7 +11,7 @@ from schemi_amd import typeOfTVDLimiter as TL, errors
 from schemi_amd.gpu import (PREC_FP64, PREC_MIXED, GpuSolver, SchemiGpuError,
                             conserved_from_primitive)
 from tests.cases_small import random_primitives
-from tests.golden_io import conserved, load, names
+from tests.golden_io import LATER_FLOORS, conserved, field_rel, load, names, per_cell_rel
 
 NS = 2
 
@@ -23,10 +23,13 @@ def rel_err(a, b, floor=1e-300):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", names())
 def test_golden_fixtures(name, hip_device):
-    """GPU FP64 against the reference's own outputs.  The GPU state is the
-    conserved vector; the reference additionally carries independently
-    averaged concentrations, which differ in the last bit, so the bar is
-    1e-12 relative per field (max over cells, floor 1e-12 of the field scale)."""
+    """GPU FP64 against the reference's own outputs.  The first step starts
+    from the same state: every cell of every field within 1e-12 relative (no
+    floor).  The GPU state is the conserved vector; the reference additionally
+    carries independently averaged concentrations, which differ in the last
+    bits and drift step by step, so later snapshots are held to 1e-10 per cell
+    with the floors of golden_io.LATER_FLOORS and to 1e-12 of each field's
+    largest magnitude."""
     from oracle.oracle import Oracle
 
     g = load(name)
@@ -44,10 +47,13 @@ def test_golden_fixtures(name, hip_device):
         if step in g["snaps"]:
             got = s.conserved()
             ref = conserved(g["snaps"][step])
-            for q in range(ref.shape[0]):
-                scale = max(np.abs(ref[q]).max(), 1e-300)
-                err = np.abs(got[q] - ref[q]).max() / scale
-                assert err <= 1e-12, f"{name} step {step} field {q}: {err:.3e}"
+            if step == 1:
+                for q, err in enumerate(per_cell_rel(got, ref, case.Ns)):
+                    assert err <= 1e-12, f"{name} step 1 field {q}: {err:.3e}"
+            else:
+                cell = per_cell_rel(got, ref, case.Ns, LATER_FLOORS["rho_k"], LATER_FLOORS["m"])
+                for q, (e1, e2) in enumerate(zip(cell, field_rel(got, ref))):
+                    assert e1 <= 1e-10 and e2 <= 1e-12, f"{name} step {step} field {q}: {e1:.3e} {e2:.3e}"
         if step == max(g["snaps"]):
             break
 
@@ -178,7 +184,7 @@ def test_deterministic_repeat(prec, hip_device):
 
 @pytest.mark.gpu
 def test_full_size_invariants(hip_device):
-    """The headline 256x128x128 workload: reflecting y/z walls and a quiescent
+    """The headline 256x128x128 workload: periodic y/z and a quiescent
     far field conserve the species masses and the total density to round-off
     over 20 Courant steps; nothing becomes non-finite."""
     from schemi_amd import cases
@@ -356,3 +362,39 @@ def test_shock_tube_exact_solution(prec, hip_device):
          np.abs(W["v"][0] - u).mean())
     print(prec, steps, e)
     assert e[0] < 3.6e-3 * 1.01 and e[1] < 3.1e-3 * 1.01 and e[2] < 8.2e-3 * 1.01
+
+
+@pytest.mark.gpu
+def test_rayleigh_taylor_mixed_vs_fp64_2000_steps(hip_device):
+    """The 2-D Rayleigh-Taylor workload (256 x 1024, periodic x, slip y,
+    gravity): after 2000 steps at the same constant time step (0.8 x the
+    initial Courant step) the mixed path's density and pressure are within
+    1e-5 relative L1 of the FP64 path's, and so is the mixing width
+    sum_i V Y (1 - Y)."""
+    from schemi_amd import cases
+
+    case, U0 = cases.rayleigh_taylor_2d()
+    probe = GpuSolver(case)
+    probe.set_conserved(U0)
+    dt = 0.8 * probe.step(1e-15)
+    probe.close()
+    out = {}
+    for prec in (PREC_FP64, PREC_MIXED):
+        s = GpuSolver(case, prec, const_dt=dt)
+        s.set_conserved(U0)
+        s.run(2000, dt)
+        out[prec] = (s.conserved(), s.primitives()["p"])
+        s.close()
+    (U64, p64), (Umx, pmx) = out[PREC_FP64], out[PREC_MIXED]
+    rho_l1 = np.abs(Umx[NS] - U64[NS]).sum() / np.abs(U64[NS]).sum()
+    p_l1 = np.abs(pmx - p64).sum() / np.abs(p64).sum()
+
+    def width(U):
+        y = U[0] / U[NS]
+        return float(np.sum(y * (1.0 - y)))
+
+    w_rel = abs(width(Umx) - width(U64)) / width(U64)
+    print("RT 2000 steps: rho L1", rho_l1, "p L1", p_l1, "mixing width", w_rel)
+    assert rho_l1 <= 1e-5 and p_l1 <= 1e-5 and w_rel <= 1e-5
+    # the interface has moved: the state is not the initial one
+    assert np.abs(U64[NS] - U0[NS]).sum() > 1e-3 * np.abs(U0[NS]).sum()

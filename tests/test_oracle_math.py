@@ -15,6 +15,7 @@ from schemi_amd import Case, boundaryConditionType as BCT, flowSolver as FS
 from schemi_amd import typeOfTVDLimiter as TL
 from schemi_amd.gpu import conserved_from_primitive
 from tests.cases_small import random_primitives
+from tests.golden_io import per_cell_rel
 
 BASE = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
 
@@ -200,9 +201,8 @@ def test_rk_initial_state_matches_oracle():
     o = Oracle(case)
     o.init_primitive(c, v, p)
     ref = o.conserved()
-    for q in range(ref.shape[0]):
-        scale = max(np.abs(ref[q]).max(), 1e-300)
-        assert np.abs(U[q] - ref[q]).max() <= 2e-15 * scale, q
+    for q, err in enumerate(per_cell_rel(U, ref, case.Ns)):
+        assert err <= 2e-15, (q, err)
     a, b = rk_matrices(case)
     assert a[0, 0] > 1000 and 2.0 < b[0, 0] < 2.2  # Tc 24, Pc 1: strongly non-ideal
 
@@ -221,9 +221,8 @@ def test_stiffened_initial_state_matches_oracle():
     o = Oracle(case)
     o.init_primitive(c, v, p)
     ref = o.conserved()
-    for q in range(ref.shape[0]):
-        scale = max(np.abs(ref[q]).max(), 1e-300)
-        assert np.abs(U[q] - ref[q]).max() <= 1e-15 * scale, q
+    for q, err in enumerate(per_cell_rel(U, ref, case.Ns)):
+        assert err <= 1e-15, (q, err)
 
 
 KATAOKA = dict(Tc=[24.0, 3.2], Pc=[1.0, 1.0], epsLJ=[10.0, 5.0], sigmaLJ=[1.52e-8, 1.2e-8])
@@ -245,9 +244,8 @@ def test_kataoka_initial_state_matches_oracle(kb):
     o = Oracle(case)
     o.init_primitive(c, v, p)
     ref = o.conserved()
-    for q in range(ref.shape[0]):
-        scale = max(np.abs(ref[q]).max(), 1e-300)
-        assert np.abs(U[q] - ref[q]).max() <= 1e-12 * scale, q
+    for q, err in enumerate(per_cell_rel(U, ref, case.Ns)):
+        assert err <= 1e-12, (q, err)
     V0, eps, b = kataoka_matrices(case)
     # V0_kk = sigma^3 N_A / sqrt(2); eps geometric mean; b from Tc/Pc or sigma
     assert np.isclose(V0[0, 0], 1.52e-8 ** 3 * 6.02214076e23 / np.sqrt(2), rtol=1e-15)

@@ -48,6 +48,9 @@ extern "C" {
 #define SCHEMI_ERR_BOUNDARY 5
 #define SCHEMI_ERR_MESH 10
 #define SCHEMI_ERR_INIT 11
+/* no counterpart in the reference: the context has more cells than the
+ * no-allocation scratch of schemi_gpu_mixing_width_noalloc holds */
+#define SCHEMI_ERR_SCRATCH 100
 
 /* boundaryConditionType (boundaryConditionTypesEnum.hpp:15-34); PERIODIC has
  * no counterpart in the reference */
@@ -288,6 +291,21 @@ int schemi_gpu_adjoint_step(schemi_gpu_ctx *ctx, double dt, const double *lambda
  * output.cpp:310-391, is a threshold width); dJ/dU into dJdU (device, state
  * layout, nullable).  Deterministic; synchronous. */
 int schemi_gpu_mixing_width(schemi_gpu_ctx *ctx, int32_t species, double *J, double *dJdU);
+
+/* No-allocation variant of schemi_gpu_mixing_width.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels and the same result, bit for bit, but the reduction's partial
+ * sums live in a fixed library-owned device array of
+ * SCHEMI_NOALLOC_MAX_CELLS / 256 + 1 doubles (512 KiB; every device the
+ * library runs on holds its own copy until the process ends), so nothing is
+ * allocated or freed.  Contexts with more than SCHEMI_NOALLOC_MAX_CELLS cells
+ * get SCHEMI_ERR_SCRATCH.  Every call on one device shares that array: calls
+ * that overlap on a device (from several host threads or contexts at once)
+ * race and must be serialised by the caller.  Synchronous. */
+#define SCHEMI_NOALLOC_MAX_CELLS 16777216
+int schemi_gpu_mixing_width_noalloc(schemi_gpu_ctx *ctx, int32_t species, double *J,
+                                    double *dJdU);
 
 /* Average duration (ms) of the last stage-kernel launches timed with HIP
  * events on the context's stream (0 when none were timed); enables timing

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -380,6 +381,32 @@ bool bc_ok(int t)
 
 namespace {
 
+// Scratch of schemi_gpu_mixing_width_noalloc: the per-workgroup partial sums
+// of the mixing-width reduction (one per 256 cells) and its result, in a
+// fixed device array that every device holds for the life of the process.
+// Host code never hands the array's name to a kernel or a copy: its address
+// on each device is looked up once, with that device current, when the first
+// context is created there (noalloc_setup), and kept in noalloc_ptr.
+constexpr long long NOALLOC_DOUBLES = SCHEMI_NOALLOC_MAX_CELLS / 256 + 1;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex noalloc_mu;
+double *noalloc_ptr[NOALLOC_DEVICES] = {};
+
+int noalloc_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_DEVICES)
+        return SCHEMI_ERR_SYSTEM;
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (noalloc_ptr[device])
+        return SCHEMI_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+        return SCHEMI_ERR_SYSTEM;
+    noalloc_ptr[device] = static_cast<double *>(p);
+    return SCHEMI_OK;
+}
+
 int ensure_adj(schemi_gpu_ctx *c)
 {
     if (c->adj)
@@ -436,6 +463,9 @@ const char *schemi_gpu_strerror(int code)
         return "meshGenerationError";
     case SCHEMI_ERR_INIT:
         return "initialisationError: configuration not supported by the GPU step";
+    case SCHEMI_ERR_SCRATCH:
+        return "scratchCapacityError: more cells than the no-allocation scratch holds "
+               "(SCHEMI_NOALLOC_MAX_CELLS)";
     default:
         return "unknown error";
     }
@@ -766,6 +796,11 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         delete c;
         return SCHEMI_ERR_SYSTEM;
     }
+    if (noalloc_setup(c->device) != SCHEMI_OK) {
+        hipStreamDestroy(c->stream);
+        delete c;
+        return SCHEMI_ERR_SYSTEM;
+    }
     const size_t state_bytes = sizeof(double) * (size_t)(mix->ns + 5) * (size_t)c->n;
     bool ok = hipMalloc(&c->dP, sizeof(DevParams)) == hipSuccess &&
               hipMalloc(&c->dgeo, geo.size() * sizeof(double)) == hipSuccess &&
@@ -1070,6 +1105,29 @@ int schemi_gpu_mixing_width(schemi_gpu_ctx *c, int32_t species, double *J, doubl
         return r;
     double *partial = c->adj + (size_t)(c->ns + 5) * (size_t)c->n;
     double *dJ = partial + (c->n + 255) / 256;
+    const int r = launch_mixing_width(c->dP, c->dgeo, c->U[0], c->ns, c->n, species, partial, dJ,
+                                      dJdU, c->stream);
+    if (r)
+        return r == -1 ? SCHEMI_ERR_INIT : SCHEMI_ERR_SYSTEM;
+    HIPCHK(hipMemcpyAsync(J, dJ, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return read_error(c);
+}
+
+int schemi_gpu_mixing_width_noalloc(schemi_gpu_ctx *c, int32_t species, double *J,
+                                    double *dJdU)
+{
+    if (!c || !J || species < 0 || species >= c->ns)
+        return SCHEMI_ERR_INIT;
+    if (c->n > SCHEMI_NOALLOC_MAX_CELLS)
+        return SCHEMI_ERR_SCRATCH;
+    double *base = (c->device >= 0 && c->device < NOALLOC_DEVICES) ? noalloc_ptr[c->device]
+                                                                   : nullptr;
+    if (!base)
+        return SCHEMI_ERR_SYSTEM;
+    // every partial slot and the result are written before they are read:
+    // nothing carried over from an earlier call is used
+    double *partial = base;
+    double *dJ = base + (c->n + 255) / 256;
     const int r = launch_mixing_width(c->dP, c->dgeo, c->U[0], c->ns, c->n, species, partial, dJ,
                                       dJdU, c->stream);
     if (r)

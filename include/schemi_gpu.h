@@ -231,6 +231,14 @@ int schemi_gpu_get_state(schemi_gpu_ctx *ctx, double *cons, double *prim,
  * timestep_source is 1. */
 int schemi_gpu_step(schemi_gpu_ctx *ctx, double dt, double *dt_next);
 
+/* Source time step of the last finished step (mesh::timestepSource(),
+ * Schemi/headers/mesh.hpp:155-157, set by the diffusion stage at
+ * Diffusion.cpp:696-700 and read by the main loop's "Courant-Source" step at
+ * Schemi.cpp:828-833): source_coeff times the stage's minimum ratio, or the
+ * reference's veryBig (1e-3 DBL_MAX) when no stage set one.  A drop-in caller
+ * whose own loop forms the time step writes it to mesh::timestepSourceRef(). */
+int schemi_gpu_source_timestep(schemi_gpu_ctx *ctx, double *dt_source);
+
 /* nsteps SSP-RK2 steps with the time step kept on the device: the first
  * step uses dt0, every later one the Courant step of the previous state,
  * clamped so that the time does not pass t_end (Schemi.cpp:933-935).  Stops

@@ -42,21 +42,20 @@ __global__ void finalize_step_kernel(double *sc, const DevParams *P, double cons
     unsigned long long bits = reinterpret_cast<unsigned long long *>(sc)[SC_SIGMAX];
     double dtn;
     unsigned long long *srcmin = reinterpret_cast<unsigned long long *>(sc) + SC_SRCMIN;
+    // the source step of this step's diffusion stage, veryBig when none set it
+    const double dts = (*srcmin == SRCMIN_NONE)
+                           ? 1e-3 * __DBL_MAX__
+                           : P->ts_coeff * __longlong_as_double(static_cast<long long>(*srcmin));
     if (const_dt > 0.0) {
         dtn = const_dt;
     } else {
         const double smax = __longlong_as_double(static_cast<long long>(bits));
         dtn = P->courant / smax;
-        if (P->ts_source) {
-            // std::min(dt, timestepSource): the source step of this step's
-            // diffusion stage, veryBig when none set it
-            const double dts = (*srcmin == SRCMIN_NONE)
-                                   ? 1e-3 * __DBL_MAX__
-                                   : P->ts_coeff * __longlong_as_double(static_cast<long long>(*srcmin));
-            dtn = (dts < dtn) ? dts : dtn;
-        }
+        if (P->ts_source)
+            dtn = (dts < dtn) ? dts : dtn;  // std::min(dt, timestepSource)
         dtn = (dtn < P->min_time) ? P->min_time : dtn;  // std::max(dt, minTime)
     }
+    sc[SC_SRCDT] = dts;
     *srcmin = SRCMIN_NONE;
     sc[SC_DTNEXT] = dtn;
     if (t + dtn > sc[SC_TEND])
@@ -78,6 +77,7 @@ __global__ void set_scalars_kernel(double *sc, double dt, double t, double tend)
     sc[SC_TEND] = tend;
     reinterpret_cast<unsigned long long *>(sc)[SC_SIGMAX] = 0ull;
     reinterpret_cast<unsigned long long *>(sc)[SC_SRCMIN] = SRCMIN_NONE;
+    sc[SC_SRCDT] = 1e-3 * __DBL_MAX__;
     sc[SC_DTNEXT] = 0.0;
     sc[SC_STEPS] = 0.0;
     sc[SC_DONE] = 0.0;
@@ -1134,6 +1134,16 @@ int schemi_gpu_mixing_width_noalloc(schemi_gpu_ctx *c, int32_t species, double *
         return r == -1 ? SCHEMI_ERR_INIT : SCHEMI_ERR_SYSTEM;
     HIPCHK(hipMemcpyAsync(J, dJ, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     return read_error(c);
+}
+
+int schemi_gpu_source_timestep(schemi_gpu_ctx *c, double *dt_source)
+{
+    if (!c || !dt_source)
+        return SCHEMI_ERR_INIT;
+    HIPCHK(hipMemcpyAsync(dt_source, c->sc + SC_SRCDT, sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SCHEMI_OK;
 }
 
 int schemi_gpu_sync(schemi_gpu_ctx *c)

@@ -38,9 +38,11 @@ enum : int { SV_RK1 = 0, SV_RK2 = 1, SV_HANCOCK = 2 };
 // the MUSCL-Hancock buffer parity (1: the current state is in the second of
 // the two state buffers), and the minimum over the cells of the source-step
 // ratio of the last diffusion stage (bits of a non-negative double, reduced
-// with atomicMin; +inf when no stage set it).
+// with atomicMin; +inf when no stage set it), and the source step of the
+// last finished step (the reference's mesh::timestepSource(), veryBig when
+// no stage set one).
 enum : int { SC_DT = 0, SC_TIME, SC_TEND, SC_SIGMAX, SC_DTNEXT, SC_STEPS, SC_DONE, SC_PARITY,
-             SC_SRCMIN, SC_N };
+             SC_SRCMIN, SC_SRCDT, SC_N };
 constexpr unsigned long long SRCMIN_NONE = 0x7FF0000000000000ull;  // +inf
 enum : int { ERRF_RIEMANN = 1, ERRF_BOUNDARY = 2, ERRF_NEG_T = 4 };
 

@@ -77,7 +77,8 @@ _SYMBOLS = ("schemi_gpu_create", "schemi_gpu_destroy", "schemi_gpu_set_state",
             "schemi_gpu_sync", "schemi_gpu_timing", "schemi_gpu_strerror",
             "schemi_gpu_face_permutation", "schemi_gpu_axis_table", "schemi_gpu_state_device",
             "schemi_gpu_adjoint_step", "schemi_gpu_mixing_width",
-            "schemi_gpu_mixing_width_noalloc", "schemi_gpu_diffusion_timing")
+            "schemi_gpu_mixing_width_noalloc", "schemi_gpu_diffusion_timing",
+            "schemi_gpu_source_timestep")
 
 
 def _lib():
@@ -116,6 +117,8 @@ def _lib():
         L.schemi_gpu_adjoint_step.argtypes = [vp, C.c_double, vp, vp, C.c_int32]
         L.schemi_gpu_mixing_width.restype = C.c_int
         L.schemi_gpu_mixing_width.argtypes = [vp, C.c_int32, dp, vp]
+        L.schemi_gpu_source_timestep.restype = C.c_int
+        L.schemi_gpu_source_timestep.argtypes = [vp, dp]
         L.schemi_gpu_mixing_width_noalloc.restype = C.c_int
         L.schemi_gpu_mixing_width_noalloc.argtypes = [vp, C.c_int32, dp, vp]
         L.schemi_gpu_timing.restype = C.c_int
@@ -354,6 +357,12 @@ class GpuSolver:
         self._check(_lib().schemi_gpu_mixing_width(self._ctx, int(species), C.byref(J),
                                                    grad or None), "mixing_width")
         return J.value
+
+    def source_timestep(self) -> float:
+        """Source time step of the last finished step (veryBig when none)."""
+        v = C.c_double()
+        self._check(_lib().schemi_gpu_source_timestep(self._ctx, C.byref(v)), "source_timestep")
+        return v.value
 
     def mixing_width_noalloc(self, species: int = 0, grad: int = 0) -> float:
         """mixing_width through the no-allocation entry point (partial sums in

@@ -1,110 +1,381 @@
 // gpuStepSolverRK.cpp -- see gpuStepSolverRK.hpp.
 #include "gpuStepSolverRK.hpp"
 
+#include <cstring>
+#include <typeinfo>
+
+#include "HLLCSolver.hpp"
+#include "HLLSolver.hpp"
+#include "exception.hpp"
+#include "linearLimiter.hpp"
+#include "mesh.hpp"
+#include "minmodLimiter.hpp"
+#include "mixtureIdeal.hpp"
+#include "vanAlbadaLimiter.hpp"
+#include "vanLeerLimiter.hpp"
+#include "zeroLimiter.hpp"
+
 namespace schemi {
 
 namespace {
 
-void check(int code, const char *where)
+[[noreturn]] void unsupported(const std::string &what)
 {
-    if (code != SCHEMI_OK)
-        throw exception(std::string(where) + ": " + schemi_gpu_strerror(code),
-                        static_cast<errors>(code));
+    throw exception("GPU step solver: " + what + " is not supported on the GPU.",
+                    errors::initialisationError);
+}
+
+// The iteration limit of a matrix solver (abstractMatrixSolver.hpp:27, a
+// protected member), read through a pointer to member formed in a derived
+// class.
+struct solverIterations : abstractMatrixSolver {
+    static std::size_t of(const abstractMatrixSolver &m)
+    {
+        return m.*(&solverIterations::maxIterationNumber);
+    }
+};
+
+// The constant transport model's coefficients (abstractTransportModel.hpp:
+// 24-30, protected accessors), read the same way.
+struct transportConstants : abstractTransportModel {
+    static scalar mu(const abstractTransportModel &t) { return (t.*(&transportConstants::muConst))(); }
+    static scalar D(const abstractTransportModel &t) { return (t.*(&transportConstants::DConst))(); }
+    static scalar kappa(const abstractTransportModel &t)
+    {
+        return (t.*(&transportConstants::kappaConst))();
+    }
+};
+
+// First surface of each patch in the reference's surface numbering
+// (mesh.cpp:1301-1790: tail, inner, point, bottom, right, left, top), in the
+// C ABI's patch order 0 tail, 1 point, 2 bottom, 3 right, 4 left, 5 top, and
+// the number of surfaces of each.
+struct patchRanges {
+    std::size_t begin[6], count[6];
+    explicit patchRanges(const mesh &m)
+    {
+        std::size_t at = 0;
+        begin[0] = at;
+        count[0] = m.tailNumber();
+        at += m.tailNumber() + m.innerNumber();
+        const std::size_t rest[5] = {m.pointNumber(), m.bottomNumber(), m.rightNumber(),
+                                     m.leftNumber(), m.topNumber()};
+        for (int p = 1; p < 6; ++p) {
+            begin[p] = at;
+            count[p] = rest[p - 1];
+            at += rest[p - 1];
+        }
+    }
+};
+
+// One boundary condition per patch: the reference allows sub-patches with
+// their own conditions (field.hpp:159-300); the GPU takes one per patch.
+template <class T>
+std::pair<boundaryConditionType, T> patchCondition(
+    const std::vector<std::pair<boundaryConditionType, T>> &info, const patchRanges &pr, int p,
+    const char *name)
+{
+    const auto &first = info[pr.begin[p]];
+    for (std::size_t j = pr.begin[p]; j < pr.begin[p] + pr.count[p]; ++j) {
+        const auto &c = info[j];
+        bool same = c.first == first.first;
+        if constexpr (std::is_same_v<T, vector>)
+            same = same && c.second() == first.second();
+        else
+            same = same && c.second == first.second;
+        if (!same)
+            unsupported(std::string("a patch with several boundary conditions (") + name + ")");
+    }
+    return first;
+}
+
+int bcCode(boundaryConditionType t, const char *name)
+{
+    switch (t) {
+    case boundaryConditionType::freeBoundary:
+        return SCHEMI_BC_FREE;
+    case boundaryConditionType::slip:
+        return SCHEMI_BC_SLIP;
+    case boundaryConditionType::fixedValueCell:
+        return SCHEMI_BC_FIXED_CELL;
+    case boundaryConditionType::fixedValueSurface:
+        return SCHEMI_BC_FIXED_SURFACE;
+    default:
+        unsupported(std::string("this boundary condition type (") + name + ")");
+    }
+}
+
+struct descriptors {
+    schemi_mesh_desc mesh{};
+    schemi_mix_desc mix{};
+    schemi_scheme_desc scheme{};
+};
+
+descriptors describe(int stepSolver, int precision, const homogeneousPhase<cubicCell> &gasPhase,
+                     const abstractLimiter &limiter, const abstractFlowSolver &fsolver,
+                     bool gravitation, const vector &g, bool diffusion,
+                     const abstractMatrixSolver &msolver,
+                     const std::pair<scalar, scalar> &timestepCoeffs, enthalpyFlow enthalpyFlag,
+                     bool linear, timestep sourceTimeFlag, bool molMassDiffusion,
+                     const chemicalKinetics::abstractChemicalKinetics &chemKin)
+{
+    descriptors d;
+    const mesh &m = gasPhase.pressure.meshRef();
+    // mesh (mesh.hpp:143-149)
+    d.mesh.dim = m.taskDimension() == dimensions::task1D   ? 1
+                 : m.taskDimension() == dimensions::task2D ? 2
+                                                           : 3;
+    for (int a = 0; a < 3; ++a) {
+        d.mesh.n[a] = static_cast<int64_t>(m.nCells()[a]);
+        d.mesh.extent[a] = m.delta()()[a];
+        d.mesh.origin[a] = m.zero()()[a];
+    }
+    // mixture (abstractMixtureThermodynamics.hpp:40-46)
+    const abstractMixtureThermodynamics &mix = *gasPhase.phaseThermodynamics;
+    if (typeid(mix) != typeid(mixtureIdeal))
+        unsupported("this mixture model (the drop-in maps the ideal mixture)");
+    const std::size_t ns = mix.Mv().size();
+    if (ns < 1 || ns > 3)
+        unsupported("a mixture of more than 3 species");
+    d.mix.ns = static_cast<int32_t>(ns);
+    for (std::size_t k = 0; k < ns; ++k) {
+        d.mix.M[k] = mix.Mv()[k];
+        d.mix.Cv[k] = mix.Cvv()[k];
+    }
+    d.mix.R = mix.Rv();
+    d.mix.gas_model = SCHEMI_GAS_IDEAL;
+    // flux and limiter: their dynamic types (abstractFlowSolver.cpp,
+    // abstractLimiter.cpp factories)
+    if (typeid(fsolver) == typeid(HLLCSolver))
+        d.scheme.flux = SCHEMI_FLUX_HLLC;
+    else if (typeid(fsolver) == typeid(HLLSolver))
+        d.scheme.flux = SCHEMI_FLUX_HLL;
+    else
+        unsupported("this flux scheme");
+    if (typeid(limiter) == typeid(minmodLimiter))
+        d.scheme.limiter = SCHEMI_LIM_MINMOD;
+    else if (typeid(limiter) == typeid(vanLeerLimiter))
+        d.scheme.limiter = SCHEMI_LIM_VANLEER;
+    else if (typeid(limiter) == typeid(vanAlbadaLimiter))
+        d.scheme.limiter = SCHEMI_LIM_VANALBADA;
+    else if (typeid(limiter) == typeid(linearLimiter))
+        d.scheme.limiter = SCHEMI_LIM_LINEAR;
+    else if (typeid(limiter) == typeid(zeroLimiter))
+        d.scheme.limiter = SCHEMI_LIM_ZERO;
+    else
+        unsupported("this limiter");
+    d.scheme.precision = precision;
+    d.scheme.step_solver = stepSolver;
+    d.scheme.gravity = gravitation ? 1 : 0;
+    for (int a = 0; a < 3; ++a)
+        d.scheme.g[a] = g()[a];
+    // the main loop forms the time step itself (Schemi.cpp:807-868); the
+    // device's own Courant step is reported by nextTimestep()
+    d.scheme.courant = 0.5;
+    d.scheme.min_time = 0.0;
+    d.scheme.const_dt = -1.0;
+    // boundary conditions per field and patch
+    const patchRanges pr(m);
+    for (int p = 0; p < 6; ++p) {
+        if (pr.count[p] == 0) {
+            for (std::size_t k = 0; k < ns; ++k)
+                d.scheme.bc_c[k][p].type = SCHEMI_BC_FREE;
+            d.scheme.bc_v[p].type = SCHEMI_BC_FREE;
+            d.scheme.bc_p[p].type = SCHEMI_BC_FREE;
+            continue;
+        }
+        for (std::size_t k = 0; k < ns; ++k) {
+            const auto c = patchCondition(gasPhase.concentration.v[k + 1].boundCond(), pr, p,
+                                          "concentration");
+            d.scheme.bc_c[k][p].type = bcCode(c.first, "concentration");
+            d.scheme.bc_c[k][p].value[0] = c.second;
+        }
+        const auto v = patchCondition(gasPhase.velocity.boundCond(), pr, p, "velocity");
+        d.scheme.bc_v[p].type = bcCode(v.first, "velocity");
+        for (int a = 0; a < 3; ++a)
+            d.scheme.bc_v[p].value[a] = v.second()[a];
+        const auto pc = patchCondition(gasPhase.pressure.boundCond(), pr, p, "pressure");
+        d.scheme.bc_p[p].type = bcCode(pc.first, "pressure");
+        d.scheme.bc_p[p].value[0] = pc.second;
+    }
+    if (gasPhase.turbulence->turbulence())
+        unsupported("turbulence");
+    if (chemKin.chemicalReaction)
+        unsupported("chemical kinetics");
+    // diffusion stage (Diffusion.cpp:20-1061)
+    d.scheme.diffusion = diffusion ? 1 : 0;
+    if (diffusion) {
+        const abstractTransportModel &tm = *gasPhase.transportModel;
+        if (typeid(tm) != typeid(abstractTransportModel))
+            unsupported("this transport model (the constant one is)");
+        d.scheme.transport_mu = transportConstants::mu(tm);
+        d.scheme.transport_D = transportConstants::D(tm);
+        d.scheme.transport_kappa = transportConstants::kappa(tm);
+        if (!linear)
+            unsupported("#Reconstruction star");
+        switch (msolver.solverType) {
+        case matrixSolver::explicitSolver:
+            d.scheme.diffusion_solver = SCHEMI_SLE_EXPLICIT;
+            break;
+        case matrixSolver::Jacobi:
+            d.scheme.diffusion_solver = SCHEMI_SLE_JACOBI;
+            break;
+        case matrixSolver::GaussSeidel:
+            d.scheme.diffusion_solver = SCHEMI_SLE_GAUSS_SEIDEL;
+            break;
+        default:
+            unsupported("this matrix solver");
+        }
+        d.scheme.diffusion_iterations = static_cast<int32_t>(solverIterations::of(msolver));
+        d.scheme.diffusion_molar_mass = molMassDiffusion ? 1 : 0;
+        if (enthalpyFlag == enthalpyFlow::explicitSolve)
+            d.scheme.diffusion_enthalpy_flow = 1;
+        else if (enthalpyFlag != enthalpyFlow::noSolve)
+            unsupported("the implicit enthalpy flow");
+        if (sourceTimeFlag == timestep::CourantAndSourceTimeStep)
+            d.scheme.timestep_source = 1;
+        else if (sourceTimeFlag != timestep::CourantTimeStep)
+            unsupported("the Courant-Source-Diffusion time step");
+        d.scheme.source_coeff = timestepCoeffs.first;
+    }
+    return d;
 }
 
 }  // namespace
 
-phaseFields::phaseFields(std::size_t ncells, std::size_t ns)
-    : concentration(ns + 1, std::valarray<double>(0.0, ncells)),
-      density(ns + 1, std::valarray<double>(0.0, ncells)),
-      pressure(0.0, ncells), internalEnergy(0.0, ncells), totalEnergy(0.0, ncells),
-      temperature(0.0, ncells)
+gpuStepSolverRK::gpuStepSolverRK(
+    int stepSolver, int precision, homogeneousPhase<cubicCell> &gasPhase_in,
+    const abstractLimiter &limiter_in, const abstractFlowSolver &fsolver_in,
+    const bool &gravitationFlag_in, const vector &g_in,
+    const boundaryConditionValue &boundaryConditionValueCalc_in, scalar &timeForTVD_in,
+    scalar &timeForHancock_in, scalar &timeForFlowCalculation_in,
+    scalar &timeForTimeIntegration_in, const MPIHandler &parallelism_in,
+    const bool &diffusionFlag_in, const abstractMatrixSolver &msolver_in,
+    const abstractMatrixSolver &msolverEnthFl_in,
+    const std::pair<scalar, scalar> &timestepCoeffs_in, scalar &timeForDiffusion_in,
+    const std::vector<boundaryConditionType> &commonConditions_in,
+    const enthalpyFlow &enthalpyFlowFlag_in, const bool &linearFlag_in,
+    const volumeField<scalar> &minimalLengthScale_in, const timestep &sourceTimeFlag_in,
+    const bool &molMassDiffusionFlag_in, chemicalKinetics::abstractChemicalKinetics &chemKin_in,
+    const bool &nonLinearityIteratonsFlag_in)
+    : abstractStepSolver(gasPhase_in, limiter_in, fsolver_in, gravitationFlag_in, g_in,
+                         boundaryConditionValueCalc_in, timeForTVD_in, timeForHancock_in,
+                         timeForFlowCalculation_in, timeForTimeIntegration_in, parallelism_in,
+                         diffusionFlag_in, msolver_in, msolverEnthFl_in, timestepCoeffs_in,
+                         timeForDiffusion_in, commonConditions_in, enthalpyFlowFlag_in,
+                         linearFlag_in, minimalLengthScale_in, sourceTimeFlag_in,
+                         molMassDiffusionFlag_in, chemKin_in, nonLinearityIteratonsFlag_in)
 {
-    for (auto &v : velocity)
-        v.resize(ncells, 0.0);
-    for (auto &m : momentum)
-        m.resize(ncells, 0.0);
-}
-
-gpuStepSolverRK::gpuStepSolverRK(phaseFields &phase, const stepSettings &s, const double &timestep)
-    : gasPhase(phase), dt(timestep)
-{
-    check(schemi_gpu_create(&s.mesh, &s.mixture, &s.scheme, &ctx), "schemi_gpu_create");
-    ns = static_cast<std::size_t>(s.mixture.ns);
-    n = gasPhase.pressure.size();
-    cons.resize((ns + 5) * n);
-    prim.resize((ns + 8) * n);
+    const descriptors d =
+        describe(stepSolver, precision, gasPhase_in, limiter_in, fsolver_in, gravitationFlag_in,
+                 g_in, diffusionFlag_in, msolver_in, timestepCoeffs_in, enthalpyFlowFlag_in,
+                 linearFlag_in, sourceTimeFlag_in, molMassDiffusionFlag_in, chemKin_in);
+    try {
+        core = std::make_unique<schemi_gpu_host::stepCore>(d.mesh, d.mix, d.scheme);
+    } catch (const schemi_gpu_host::gpu_error &e) {
+        throw exception(e.what(), static_cast<errors>(e.code()));
+    }
 }
 
 gpuStepSolverRK::~gpuStepSolverRK() noexcept
 {
-    schemi_gpu_destroy(ctx);
 }
 
-void gpuStepSolverRK::upload()
+double gpuStepSolverRK::nextTimestep() const noexcept
 {
-    // conserved vector rho_1..rho_Ns, rho, m, E
-    for (std::size_t k = 0; k < ns; ++k)
-        for (std::size_t i = 0; i < n; ++i)
-            cons[k * n + i] = gasPhase.density[k + 1][i];
-    for (std::size_t i = 0; i < n; ++i)
-        cons[ns * n + i] = gasPhase.density[0][i];
-    for (std::size_t a = 0; a < 3; ++a)
-        for (std::size_t i = 0; i < n; ++i)
-            cons[(ns + 1 + a) * n + i] = gasPhase.momentum[a][i];
-    for (std::size_t i = 0; i < n; ++i)
-        cons[(ns + 4) * n + i] = gasPhase.totalEnergy[i];
-    check(schemi_gpu_set_state(ctx, cons.data(), static_cast<int64_t>(n),
-                               SCHEMI_NUMBERING_REFERENCE),
-          "schemi_gpu_set_state");
-    uploaded = true;
-}
-
-void gpuStepSolverRK::download()
-{
-    check(schemi_gpu_get_state(ctx, cons.data(), prim.data(), SCHEMI_NUMBERING_REFERENCE),
-          "schemi_gpu_get_state");
-    for (std::size_t k = 0; k < ns; ++k)
-        for (std::size_t i = 0; i < n; ++i) {
-            gasPhase.density[k + 1][i] = cons[k * n + i];
-            gasPhase.concentration[k + 1][i] = prim[k * n + i];
-        }
-    for (std::size_t i = 0; i < n; ++i) {
-        gasPhase.density[0][i] = cons[ns * n + i];
-        gasPhase.concentration[0][i] = prim[ns * n + i];
-        gasPhase.totalEnergy[i] = cons[(ns + 4) * n + i];
-        gasPhase.pressure[i] = prim[(ns + 4) * n + i];
-        gasPhase.internalEnergy[i] = prim[(ns + 5) * n + i];
-        gasPhase.temperature[i] = prim[(ns + 6) * n + i];
-    }
-    for (std::size_t a = 0; a < 3; ++a)
-        for (std::size_t i = 0; i < n; ++i) {
-            gasPhase.momentum[a][i] = cons[(ns + 1 + a) * n + i];
-            gasPhase.velocity[a][i] = prim[(ns + 1 + a) * n + i];
-        }
+    return core->nextTimestep();
 }
 
 void gpuStepSolverRK::calculateStep()
 {
-    if (!uploaded)
-        upload();
-    check(schemi_gpu_step(ctx, dt, &dtNext), "calculateStep");
-    download();
+    const std::size_t n = core->cells(), ns = core->species();
+    double *cons = core->conserved();
+    // conserved vector rho_1..rho_Ns, rho, m, E (bunchOfFields.hpp:28-45)
+    for (std::size_t k = 0; k < ns; ++k)
+        std::memcpy(cons + k * n, &gasPhase.density[k + 1].cval()[0], n * sizeof(double));
+    std::memcpy(cons + ns * n, &gasPhase.density[0].cval()[0], n * sizeof(double));
+    for (std::size_t i = 0; i < n; ++i) {
+        const auto &m = gasPhase.momentum.cval()[i]();
+        for (std::size_t a = 0; a < 3; ++a)
+            cons[(ns + 1 + a) * n + i] = m[a];
+    }
+    std::memcpy(cons + (ns + 4) * n, &gasPhase.totalEnergy.cval()[0], n * sizeof(double));
+    try {
+        core->upload();
+        core->step(gasPhase.pressure.meshRef().timestep());
+        core->download();
+    } catch (const schemi_gpu_host::gpu_error &e) {
+        throw exception(e.what(), static_cast<errors>(e.code()));
+    }
+    // the fields the reference's stage leaves consistent
+    // (Advection2dOrder.cpp:578-608)
+    const double *U = core->conserved();
+    const double *W = core->primitives();
+    for (std::size_t k = 0; k < ns; ++k) {
+        std::memcpy(&gasPhase.density[k + 1].val()[0], U + k * n, n * sizeof(double));
+        std::memcpy(&gasPhase.concentration.v[k + 1].val()[0], W + k * n, n * sizeof(double));
+    }
+    std::memcpy(&gasPhase.density[0].val()[0], U + ns * n, n * sizeof(double));
+    std::memcpy(&gasPhase.concentration.v[0].val()[0], W + ns * n, n * sizeof(double));
+    for (std::size_t i = 0; i < n; ++i)
+        for (std::size_t a = 0; a < 3; ++a) {
+            gasPhase.momentum.val()[i].wr()[a] = U[(ns + 1 + a) * n + i];
+            gasPhase.velocity.val()[i].wr()[a] = W[(ns + 1 + a) * n + i];
+        }
+    std::memcpy(&gasPhase.totalEnergy.val()[0], U + (ns + 4) * n, n * sizeof(double));
+    std::memcpy(&gasPhase.pressure.val()[0], W + (ns + 4) * n, n * sizeof(double));
+    std::memcpy(&gasPhase.internalEnergy.val()[0], W + (ns + 5) * n, n * sizeof(double));
+    std::memcpy(&gasPhase.temperature.val()[0], W + (ns + 6) * n, n * sizeof(double));
+    gasPhase.HelmholtzEnergy.val() =
+        gasPhase.phaseThermodynamics->Fv(gasPhase.concentration.p, gasPhase.temperature.cval());
+    gasPhase.entropy.val() =
+        gasPhase.phaseThermodynamics->Sv(gasPhase.concentration.p, gasPhase.temperature.cval());
+    // the diffusion stage's source step for the main loop's Courant-Source
+    // time step (Diffusion.cpp:696-700, Schemi.cpp:828-833)
+    // (the fields' mesh is the reference's one mesh object, created by main)
+    if (diffusionFlag)
+        const_cast<mesh &>(gasPhase.pressure.meshRef()).timestepSourceRef() =
+            core->sourceTimestep();
 }
 
-std::unique_ptr<abstractStepSolver> createStepSolver(const std::string &key, phaseFields &phase,
-                                                     const stepSettings &settings,
-                                                     const double &timestep)
+std::unique_ptr<abstractStepSolver> createGpuStepSolver(
+    const std::string &key, homogeneousPhase<cubicCell> &gasPhase_in,
+    const abstractLimiter &limiter_in, const abstractFlowSolver &fsolver_in,
+    const bool &gravitationFlag_in, const vector &g_in,
+    const boundaryConditionValue &boundaryConditionValueCalc_in, scalar &timeForTVD_in,
+    scalar &timeForHancock_in, scalar &timeForFlowCalculation_in,
+    scalar &timeForTimeIntegration_in, const MPIHandler &parallelism_in,
+    const bool &diffusionFlag_in, const abstractMatrixSolver &msolver_in,
+    const abstractMatrixSolver &msolverEnthFl_in,
+    const std::pair<scalar, scalar> &timestepCoeffs_in, scalar &timeForDiffusion_in,
+    const std::vector<boundaryConditionType> &commonConditions_in,
+    const enthalpyFlow &enthalpyFlowFlag_in, const bool &linearFlag_in,
+    const volumeField<scalar> &minimalLengthScale_in, const timestep &sourceTimeFlag_in,
+    const bool &molMassDiffusionFlag_in, chemicalKinetics::abstractChemicalKinetics &chemKin_in,
+    const bool &nonLinearityIteratonsFlag_in)
 {
-    // the key selects the step solver (abstractStepSolver.cpp:83-84, 97-118)
-    stepSettings s = settings;
-    if (key == "SecondOrderRK_GPU")
-        s.scheme.step_solver = SCHEMI_STEP_SECOND_ORDER_RK;
-    else if (key == "SecondOrder_GPU")
-        s.scheme.step_solver = SCHEMI_STEP_SECOND_ORDER;
+    // the key selects the step solver as in abstractStepSolver.cpp:77-118
+    int stepSolver = -1, precision = SCHEMI_PREC_FP64;
+    std::string base = key;
+    const std::string mixed = "_MIXED";
+    if (base.size() > mixed.size() &&
+        base.compare(base.size() - mixed.size(), mixed.size(), mixed) == 0) {
+        precision = SCHEMI_PREC_MIXED;
+        base.resize(base.size() - mixed.size());
+    }
+    if (base == "SecondOrderRK_GPU")
+        stepSolver = SCHEMI_STEP_SECOND_ORDER_RK;
+    else if (base == "SecondOrder_GPU")
+        stepSolver = SCHEMI_STEP_SECOND_ORDER;
     else
         throw exception("Unknown or CPU-only step solver: " + key, errors::initialisationError);
-    return std::make_unique<gpuStepSolverRK>(phase, s, timestep);
+    return std::make_unique<gpuStepSolverRK>(
+        stepSolver, precision, gasPhase_in, limiter_in, fsolver_in, gravitationFlag_in, g_in,
+        boundaryConditionValueCalc_in, timeForTVD_in, timeForHancock_in,
+        timeForFlowCalculation_in, timeForTimeIntegration_in, parallelism_in, diffusionFlag_in,
+        msolver_in, msolverEnthFl_in, timestepCoeffs_in, timeForDiffusion_in,
+        commonConditions_in, enthalpyFlowFlag_in, linearFlag_in, minimalLengthScale_in,
+        sourceTimeFlag_in, molMassDiffusionFlag_in, chemKin_in, nonLinearityIteratonsFlag_in);
 }
 
 }  // namespace schemi

@@ -1,123 +1,99 @@
-// gpuStepSolverRK.hpp -- C++ drop-in for the reference's SecondOrderRK step.
+// gpuStepSolverRK.hpp -- the GPU step as a plugin of the reference's own
+// step-solver interface.
 //
-// Mirrors the reference's step-solver plugin interface
-// (abstractStepSolver::calculateStep / createStepSolver,
-//  Schemi/headers/abstractStepSolver.hpp:76-156,
-//  Schemi/src/interfaces/abstractStepSolver.cpp:56-146) over the C ABI of
-// include/schemi_gpu.h.  The solver holds a reference to the caller's cell
-// fields (structure of arrays, reference cell order), like the reference's
-// solver holds homogeneousPhase<cubicCell>&; calculateStep() advances them by
-// one SSP-RK2 step on the GPU and leaves conserved and primitive fields
-// consistent, so the caller's time loop (Courant step, output) is unchanged.
-// Errors are thrown as schemi::exception with the reference's errors codes.
+// schemi::gpuStepSolverRK derives from the reference's
+// schemi::abstractStepSolver (Schemi/headers/abstractStepSolver.hpp:26-106)
+// and takes exactly the arguments abstractStepSolver::createStepSolver passes
+// to its solvers (abstractStepSolver.hpp:78-101, abstractStepSolver.cpp:56-146),
+// so a maintainer registers it in that factory under the keys
+// "SecondOrderRK_GPU" and "SecondOrder_GPU" (createGpuStepSolver below) and
+// the reference's main loop runs unchanged.  It reads the reference's objects
+// directly: the mesh of the phase's fields (nCells, delta, zero,
+// taskDimension), the mixture (mixtureIdeal: Mv, Cvv, Rv), the flux and
+// limiter objects (their dynamic types), gravity, every field's boundary
+// condition on each patch, and the diffusion settings (constant transport
+// model, explicit / Jacobi / Gauss-Seidel solver, linear reconstruction).
+//
+// calculateStep() uploads the conserved fields of homogeneousPhase<cubicCell>
+// (density[0..Ns], momentum, totalEnergy), advances them by one step of
+// mesh::timestep() on the GPU and writes back every field the reference's
+// step leaves consistent (Advection2dOrder.cpp:578-608): concentrations,
+// densities, velocity, momentum, pressure, internal and total energy,
+// temperature, and the Helmholtz energy and entropy through the phase's own
+// mixture (Fv, Sv), plus mesh::timestepSourceRef() from the diffusion
+// stage's source step for the main loop's "Courant-Source" time step
+// (Schemi.cpp:828-833).  Errors are thrown as the reference's
+// schemi::exception with its errors codes.
+//
+// Compiled against the reference's headers (-I Schemi/headers
+// -I Schemi/enumerations -I Schemi/templates) and linked into the reference
+// with gpuStepCore.cpp and libschemi_gpu.so.
 #pragma once
 
-#include <array>
-#include <exception>
 #include <memory>
 #include <string>
-#include <valarray>
+#include <utility>
 #include <vector>
 
-#include "../../include/schemi_gpu.h"
+#include "abstractStepSolver.hpp"
+#include "gpuStepCore.hpp"
 
 namespace schemi {
 
-// Schemi/enumerations/errorsEnum.hpp:15-30
-enum class errors {
-    noErrors, RiemannSolverError, negativeTemperatureError, tooBigOutputNumberError,
-    systemError, boundaryConditionError, cubicEquationError, positivnessError, MPIError,
-    fieldInitialisationError, meshGenerationError, initialisationError, NaNError
-};
-// Schemi/enumerations/flowSolverEnum.hpp:15-18
-enum class flowSolver { HLL, HLLCF, HLLC, HLLCLM, KT, HLLC2p, Richtmyer };
-// Schemi/enumerations/typeOfTVDLimiterEnum.hpp:15-32
-enum class typeOfTVDLimiter {
-    zero, minmod, vanLeer, linear, vanAlbada, HQUICK, vanLeer2, superbee, vanAlbada2,
-    minmod2, Sweby, Koren, Koren2, UTCDFS, Schmidtmann
-};
-// Schemi/enumerations/boundaryConditionTypesEnum.hpp:15-34 (+ periodic)
-enum class boundaryConditionType {
-    blank, freeBoundary, slip, fixedValueCell, fixedValueSurface, innerSurface, calculated,
-    calculatedTurbulentViscosity, calculatedTemperature, calculatedMassFraction,
-    calculatedAverageMolarMass, calculatedNonidealityCorrectionPerDensity, calculatedCv,
-    calculatedCvM, calculatedDensity, calculatedParallelBoundary, calculatedMolarFraction,
-    periodic = 32
-};
-
-// Schemi/headers/exception.hpp:20-30
-class exception : public std::exception {
-    std::string msg;
-    errors code;
-
-public:
-    exception(std::string m, errors c) : msg(std::move(m)), code(c) {}
-    const char *what() const noexcept override { return msg.c_str(); }
-    errors errCode() const noexcept { return code; }
-};
-
-// Cell fields of homogeneousPhase<cubicCell> on the hot path
-// (Schemi/templates/bunchOfFields.hpp:28-45), one valarray per component.
-struct phaseFields {
-    std::vector<std::valarray<double>> concentration;  // c_0 (sum), c_1 .. c_Ns
-    std::vector<std::valarray<double>> density;        // rho_0 (total), rho_1 .. rho_Ns
-    std::array<std::valarray<double>, 3> velocity;
-    std::array<std::valarray<double>, 3> momentum;
-    std::valarray<double> pressure, internalEnergy, totalEnergy, temperature;
-
-    phaseFields(std::size_t ncells, std::size_t ns);
-};
-
-// Deck settings the step reads (Schemi/main/Schemi.cpp:84-163).
-struct stepSettings {
-    schemi_mesh_desc mesh{};
-    schemi_mix_desc mixture{};
-    schemi_scheme_desc scheme{};
-};
-
-// abstractStepSolver (Schemi/headers/abstractStepSolver.hpp:76-156)
-class abstractStepSolver {
-public:
-    virtual ~abstractStepSolver() noexcept = default;
-    virtual void calculateStep() = 0;
-};
-
-// SecondOrderRK (SSP-RK2) or SecondOrder (MUSCL-Hancock) on the GPU, as
-// settings.scheme.step_solver selects.  timestep is the caller's
-// mesh::timestep() (mesh.hpp:151), read at every call.
 class gpuStepSolverRK final : public abstractStepSolver {
 public:
-    gpuStepSolverRK(phaseFields &phase, const stepSettings &settings, const double &timestep);
-    ~gpuStepSolverRK() noexcept override;
+    // stepSolver: SCHEMI_STEP_SECOND_ORDER_RK or SCHEMI_STEP_SECOND_ORDER;
+    // precision: SCHEMI_PREC_FP64 or SCHEMI_PREC_MIXED; the rest as
+    // secondOrderStepSolverRK (secondOrderStepSolverRK.hpp:17-38).
+    gpuStepSolverRK(int stepSolver, int precision, homogeneousPhase<cubicCell> &gasPhase_in,
+                    const abstractLimiter &limiter_in, const abstractFlowSolver &fsolver_in,
+                    const bool &gravitationFlag_in, const vector &g_in,
+                    const boundaryConditionValue &boundaryConditionValueCalc_in,
+                    scalar &timeForTVD_in, scalar &timeForHancock_in,
+                    scalar &timeForFlowCalculation_in, scalar &timeForTimeIntegration_in,
+                    const MPIHandler &parallelism_in, const bool &diffusionFlag_in,
+                    const abstractMatrixSolver &msolver_in,
+                    const abstractMatrixSolver &msolverEnthFl_in,
+                    const std::pair<scalar, scalar> &timestepCoeffs_in,
+                    scalar &timeForDiffusion_in,
+                    const std::vector<boundaryConditionType> &commonConditions_in,
+                    const enthalpyFlow &enthalpyFlowFlag_in, const bool &linearFlag_in,
+                    const volumeField<scalar> &minimalLengthScale_in,
+                    const timestep &sourceTimeFlag_in, const bool &molMassDiffusionFlag_in,
+                    chemicalKinetics::abstractChemicalKinetics &chemKin_in,
+                    const bool &nonLinearityIteratonsFlag_in);
+
     gpuStepSolverRK(const gpuStepSolverRK &) = delete;
-    gpuStepSolverRK &operator=(const gpuStepSolverRK &) = delete;
+    auto &operator=(const gpuStepSolverRK &) = delete;
+    ~gpuStepSolverRK() noexcept override;
 
     void calculateStep() override;
-    // Courant step computed on the device from the state after the last step
-    // (Schemi.cpp:807-833).
-    double nextTimestep() const noexcept { return dtNext; }
-    // Marks the host fields as changed by the caller (uploaded before the next
-    // step).
-    void invalidate() noexcept { uploaded = false; }
+
+    // Courant step of the state after the last step, computed on the device
+    // (Schemi.cpp:807-833 without the MPI reduction).
+    double nextTimestep() const noexcept;
 
 private:
-    phaseFields &gasPhase;
-    const double &dt;
-    schemi_gpu_ctx *ctx = nullptr;
-    std::size_t n, ns;
-    bool uploaded = false;
-    double dtNext = 0;
-    std::vector<double> cons, prim;
-
-    void upload();
-    void download();
+    std::unique_ptr<schemi_gpu_host::stepCore> core;
 };
 
-// createStepSolver (Schemi/src/interfaces/abstractStepSolver.cpp:56-146) with
-// the GPU keys "SecondOrderRK_GPU" and "SecondOrder_GPU" (they set
-// settings.scheme.step_solver).  Other keys throw initialisationError.
-std::unique_ptr<abstractStepSolver> createStepSolver(const std::string &key, phaseFields &phase,
-                                                     const stepSettings &settings,
-                                                     const double &timestep);
+// createStepSolver for the GPU keys: "SecondOrderRK_GPU" (SSP-RK2),
+// "SecondOrder_GPU" (MUSCL-Hancock), each with the suffix "_MIXED" for the
+// FP32-flux / FP64-accumulate path.  Other keys throw initialisationError.
+std::unique_ptr<abstractStepSolver> createGpuStepSolver(
+    const std::string &key, homogeneousPhase<cubicCell> &gasPhase_in,
+    const abstractLimiter &limiter_in, const abstractFlowSolver &fsolver_in,
+    const bool &gravitationFlag_in, const vector &g_in,
+    const boundaryConditionValue &boundaryConditionValueCalc_in, scalar &timeForTVD_in,
+    scalar &timeForHancock_in, scalar &timeForFlowCalculation_in,
+    scalar &timeForTimeIntegration_in, const MPIHandler &parallelism_in,
+    const bool &diffusionFlag_in, const abstractMatrixSolver &msolver_in,
+    const abstractMatrixSolver &msolverEnthFl_in,
+    const std::pair<scalar, scalar> &timestepCoeffs_in, scalar &timeForDiffusion_in,
+    const std::vector<boundaryConditionType> &commonConditions_in,
+    const enthalpyFlow &enthalpyFlowFlag_in, const bool &linearFlag_in,
+    const volumeField<scalar> &minimalLengthScale_in, const timestep &sourceTimeFlag_in,
+    const bool &molMassDiffusionFlag_in, chemicalKinetics::abstractChemicalKinetics &chemKin_in,
+    const bool &nonLinearityIteratonsFlag_in);
 
 }  // namespace schemi

@@ -109,7 +109,9 @@ typedef struct {
  * in FP64 by Newton's method on both precision paths (the reference solves
  * it in long double; results agree to a few ulps, not bit for bit). */
 typedef struct {
-    int32_t ns;
+    int32_t ns; /* 1 <= ns <= 3: the kernels are instantiated for up to three
+                   species (SCHEMI_ERR_INIT otherwise); the arrays below are
+                   sized for SCHEMI_MAX_SPECIES */
     double M[SCHEMI_MAX_SPECIES];
     double Cv[SCHEMI_MAX_SPECIES];
     double R;

@@ -357,6 +357,31 @@ int restore_state(schemi_gpu_ctx *c)
     return hip_fail(hipGetLastError());
 }
 
+// Device buffers of one API call, released on every exit path (error
+// returns included) by hipFreeAsync on the context's stream.
+struct callBuffers {
+    hipStream_t stream;
+    std::vector<void *> held;
+    explicit callBuffers(hipStream_t s) : stream(s) {}
+    callBuffers(const callBuffers &) = delete;
+    callBuffers &operator=(const callBuffers &) = delete;
+    template <class T> hipError_t get(T **out, size_t bytes)
+    {
+        void *q = nullptr;
+        const hipError_t e = hipMallocAsync(&q, bytes, stream);
+        if (e == hipSuccess) {
+            held.push_back(q);
+            *out = static_cast<T *>(q);
+        }
+        return e;
+    }
+    ~callBuffers()
+    {
+        for (void *q : held)
+            (void)hipFreeAsync(q, stream);
+    }
+};
+
 int read_error(schemi_gpu_ctx *c)
 {
     int e = 0;
@@ -894,10 +919,11 @@ int schemi_gpu_get_state(schemi_gpu_ctx *c, double *cons, double *prim, int32_t 
     const size_t bytes = sizeof(double) * (size_t)(c->ns + 5) * (size_t)c->n;
     if (cons)
         HIPCHK(hipMemcpyAsync(cons, c->U[0], bytes, hipMemcpyDeviceToHost, c->stream));
+    callBuffers buf(c->stream);
     if (prim) {
         double *dW = nullptr;
         const size_t pbytes = sizeof(double) * (size_t)(c->ns + 8) * (size_t)c->n;
-        HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dW), pbytes, c->stream));
+        HIPCHK(buf.get(&dW, pbytes));
         const int bs = 256;
         const dim3 g((unsigned)((c->n + bs - 1) / bs));
         switch (c->ns) {
@@ -913,7 +939,6 @@ int schemi_gpu_get_state(schemi_gpu_ctx *c, double *cons, double *prim, int32_t 
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(prim, dW, pbytes, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipFreeAsync(dW, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return SCHEMI_OK;
@@ -1007,11 +1032,12 @@ int schemi_gpu_face_flux(schemi_gpu_ctx *c, int64_t nfaces, const int32_t *axis,
     const int NQ = c->ns + 4, NF = c->ns + 5;
     int *dax = nullptr;
     double *dsg = nullptr, *dL = nullptr, *dR = nullptr, *dF = nullptr;
-    HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dax), nfaces * sizeof(int), c->stream));
-    HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dsg), nfaces * sizeof(double), c->stream));
-    HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dL), nfaces * NQ * sizeof(double), c->stream));
-    HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dR), nfaces * NQ * sizeof(double), c->stream));
-    HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&dF), nfaces * NF * sizeof(double), c->stream));
+    callBuffers buf(c->stream);
+    HIPCHK(buf.get(&dax, nfaces * sizeof(int)));
+    HIPCHK(buf.get(&dsg, nfaces * sizeof(double)));
+    HIPCHK(buf.get(&dL, nfaces * NQ * sizeof(double)));
+    HIPCHK(buf.get(&dR, nfaces * NQ * sizeof(double)));
+    HIPCHK(buf.get(&dF, nfaces * NF * sizeof(double)));
     HIPCHK(hipMemcpyAsync(dax, axis, nfaces * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(dsg, sgn, nfaces * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(dL, left, nfaces * NQ * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1030,11 +1056,6 @@ int schemi_gpu_face_flux(schemi_gpu_ctx *c, int64_t nfaces, const int32_t *axis,
 #undef SG_FF
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(flux, dF, nfaces * NF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipFreeAsync(dax, c->stream));
-    HIPCHK(hipFreeAsync(dsg, c->stream));
-    HIPCHK(hipFreeAsync(dL, c->stream));
-    HIPCHK(hipFreeAsync(dR, c->stream));
-    HIPCHK(hipFreeAsync(dF, c->stream));
     return read_error(c);
 }
 

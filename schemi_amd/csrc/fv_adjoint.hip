@@ -14,7 +14,9 @@
 // numbers seeded along q (branches frozen at the forward state) and
 // accumulates -mu_f . dg_f/dU_j[q]; the cell's result is written once, so the
 // output is deterministic.  The RK2 step's adjoint composes two of them with
-// the weights of U^{n+1} = (L(L(U^n)) + U^n) / 2 (capi.hip).  On 1-D meshes
+// the weights of U^{n+1} = (L(L(U^n)) + U^n) / 2 (capi.hip); out may then be
+// the same array as add (an in-place step): each thread reads add and writes
+// out for its own cell only.  On 1-D meshes
 // with limited slopes adj_stage_1d_kernel does the same over the four faces
 // whose fluxes a cell reaches through the slopes.
 #include "fv_launch.hpp"
@@ -38,7 +40,7 @@ template <int NS, int DIM, int GAS>
 __global__ void __launch_bounds__(256)
 adj_stage_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                  const double *__restrict__ U, const double *__restrict__ lam,
-                 double *__restrict__ out, double dt, double a, const double *__restrict__ add,
+                 double *out, double dt, double a, const double *add,
                  double b)
 {
     constexpr int NF = NS + 5;
@@ -153,7 +155,7 @@ template <int NS, bool LIN>
 __global__ void __launch_bounds__(256)
 adj_stage_1d_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo,
                     const double *__restrict__ U, const double *__restrict__ lam,
-                    double *__restrict__ out, double dt, double a, const double *__restrict__ add,
+                    double *out, double dt, double a, const double *add,
                     double b)
 {
     constexpr int NF = NS + 5;

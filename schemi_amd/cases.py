@@ -19,7 +19,7 @@ import numpy as np
 
 from .case import Case
 from .enums import boundaryConditionType as BCT
-from .enums import flowSolver, typeOfTVDLimiter
+from .enums import flowSolver, gasModel, typeOfTVDLimiter
 from .gpu import conserved_from_primitive
 
 MASK64 = (1 << 64) - 1
@@ -171,13 +171,44 @@ def rayleigh_taylor_2d(nx: int = 256, ny: int = 1024, seed: int = 7, a0: float =
     return case, conserved_from_primitive(case, c, v, p.reshape(-1))
 
 
-def kelvin_helmholtz_2d(n: int = 512, seed: int = 11):
+def _rk_concentration(case: Case, X, p, T):
+    """Total molar concentration c of the Redlich-Kwong mixture with mole
+    fractions X at pressure p and temperature T: the root of
+    c R T / (1 - c b) - a c^2 / (sqrt(T) (1 + c b)) = p (RedlichKwongFluid.cpp
+    pressure) with a, b mixed by the mole fractions (mixtureRedlichKwong.cpp),
+    by Newton's method from the ideal-gas value."""
+    from .gpu import rk_matrices
+
+    am, bm = rk_matrices(case)
+    a = sum(X[k] * X[l] * am[k, l] for k in range(case.Ns) for l in range(case.Ns))
+    b = sum(X[k] * X[l] * bm[k, l] for k in range(case.Ns) for l in range(case.Ns))
+    R, sT = float(case.R), np.sqrt(T)
+    c = p / (R * T)
+    for _ in range(100):
+        f = c * R * T / (1 - c * b) - a * c * c / (sT * (1 + c * b)) - p
+        df = R * T / (1 - c * b) ** 2 - a * c * (2 + c * b) / (sT * (1 + c * b) ** 2)
+        dc = f / df
+        c = c - dc
+        if np.all(np.abs(dc) <= 1e-15 * np.abs(c)):
+            break
+    return c
+
+
+def kelvin_helmholtz_2d(n: int = 512, seed: int = 11, T0: float = 300.0, p0: float = 5.0e6):
     """2-D Kelvin-Helmholtz shear layer u = +-0.5 tanh((y - 0.5) / 0.01) in a
-    periodic box [0,1]^2 with two species (M 44 and M 28), v perturbation of
-    amplitude 1e-3 over modes 1-4; HLL + minmod.  Ideal-gas closure."""
-    case = Case(dim=2, N=(n, n, 1), L=(1.0, 1.0, 1.0 / n), M=[44.0, 28.0],
-                Cv=[cv_of(1.3), cv_of(1.4)], flux=flowSolver.HLL,
-                limiter=typeOfTVDLimiter.minmod, courant=0.4, min_time=1e-15)
+    periodic box [0,1]^2 (SI units, R = 8.314462618 J/(mol K)) with two
+    species, CO2-like (M 0.044 kg/mol, Cv 28.5 J/(mol K), Tc 304.13 K,
+    Pc 7.3773 MPa) below the layer and N2-like (M 0.028, Cv 20.8, Tc 126.19 K,
+    Pc 3.3958 MPa) above it (mass-fraction tanh profile of the same width), at
+    T = 300 K and p = 5 MPa through the Redlich-Kwong cubic mixture (its
+    temperature and pressure found per cell by the root of a cubic in sqrt(T),
+    solved in FP64 on the device), a v perturbation of amplitude up to 1e-3
+    over modes 1-4 (seeded amplitudes and phases, Gaussian envelope of width
+    0.05 around the layer); HLL + minmod."""
+    case = Case(dim=2, N=(n, n, 1), L=(1.0, 1.0, 1.0 / n), M=[0.044, 0.028],
+                Cv=[28.5, 20.8], R=8.314462618, flux=flowSolver.HLL,
+                limiter=typeOfTVDLimiter.minmod, courant=0.4, min_time=1e-15,
+                gas_model=gasModel.RedlichKwong, Tc=[304.13, 126.19], Pc=[7.3773e6, 3.3958e6])
     for pt in (0, 1, 3, 4):
         case.set_bc(pt, BCT.periodic)
     rng = SplitMix64(seed)
@@ -187,9 +218,11 @@ def kelvin_helmholtz_2d(n: int = 512, seed: int = 11):
     for k in range(1, 5):
         amp, ph = 1e-3 * rng.uniform(), 2.0 * math.pi * rng.uniform()
         vv += amp * np.sin(2.0 * math.pi * k * x + ph) * np.exp(-((y - 0.5) / 0.05) ** 2)
-    y1 = 0.5 * (1.0 + np.tanh((y - 0.5) / 0.01))
-    rho = np.ones_like(x)
-    c = np.stack([y1 * rho / 44.0, (1.0 - y1) * rho / 28.0])
+    Y1 = 0.5 * (1.0 - np.tanh((y - 0.5) / 0.01))  # CO2 below the layer
+    n1, n2 = Y1 / case.M[0], (1.0 - Y1) / case.M[1]
+    X = [n1 / (n1 + n2), n2 / (n1 + n2)]
+    p = np.full_like(x, p0)
+    c0 = _rk_concentration(case, X, p, np.full_like(x, T0))
+    c = np.stack([X[0] * c0, X[1] * c0])
     v = np.stack([u, vv, np.zeros_like(u)])
-    p = np.full_like(x, 2.5)
     return case, conserved_from_primitive(case, c, v, p)

@@ -175,13 +175,23 @@ def test_rt_mixing_width_gradient_full(hip_device):
     a and h = 1e-6 leaves a truncation error of 1e-4 to 1e-3 (measured: the
     differences converge to the adjoint as h decreases, 7e-5 at 1e-7 and
     7e-7 at 1e-8 for mode 1), while FP64 rounding in J stays near 1e-7
-    relative at h = 1e-8."""
+    relative at h = 1e-8.  The same four modes against the committed central
+    differences of the oracle (tests/golden/rt_fd_oracle_256x1024.json,
+    written by tests/golden/make_rt_fd.py, h = 1e-8), <= 1e-4 relative."""
+    import json
+    import os
+
     from schemi_amd import adjoint as adj
+
+    with open(os.path.join(os.path.dirname(__file__), "golden",
+                           "rt_fd_oracle_256x1024.json")) as f:
+        ofd = json.load(f)
 
     case = adj.rt_case(256, 1024)
     rng = np.random.default_rng(2024)
     a = rng.uniform(0.0, 0.005, 16)
     nsteps, dt = 400, 2.5e-4
+    assert (ofd["seed"], ofd["nsteps"], ofd["dt"]) == (2024, nsteps, dt)
     import time
 
     t0 = time.perf_counter()
@@ -202,3 +212,9 @@ def test_rt_mixing_width_gradient_full(hip_device):
               "adjoint(mixed fwd)", gmx[k], "rel", rmx)
         assert r64 <= 1e-4
         assert rmx <= 1e-4
+        fdo = ofd["fd"][f"{k}:1e-08"]
+        o64 = abs(g64[k] - fdo) / abs(fdo)
+        omx = abs(gmx[k] - fdo) / abs(fdo)
+        print("mode", k + 1, "fd(oracle)", fdo, "rel fp64", o64, "rel mixed", omx)
+        assert o64 <= 1e-4
+        assert omx <= 1e-4

@@ -60,7 +60,7 @@ def parse():
                     help="steps of the 2-D Rayleigh-Taylor FP64-vs-mixed gate")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
-                    help="also run one 1 GiB device copy (calibrates rocprofv3 byte counters)")
+                    help="also run one 1 GiB streaming copy (calibrates rocprofv3 byte counters)")
     ap.add_argument("--diffusion", action="store_true",
                     help="the workload with the molecular diffusion stage after each step")
     ap.add_argument("--diffusion-solver", default="explicit", choices=("explicit", "Jacobi"),
@@ -209,7 +209,7 @@ def cpu_baseline(nx=128, ny=64, nz=64, steps=3, diffusion=False):
             "kind": "port", "sample": sample + ", C restatement (oracle), one core"}
 
 
-def traffic_from_profiles(name="stage0_kernel_rm3d_256x128x128_pmc.md",
+def traffic_from_profiles(name="stage_lean_kernel_rm3d_256x128x128_pmc.md",
                           key="traffic_bytes_per_launch"):
     """HBM bytes per launch from a committed rocprofv3 PMC summary."""
     path = os.path.join(ROOT, "profiles", name)
@@ -219,23 +219,47 @@ def traffic_from_profiles(name="stage0_kernel_rm3d_256x128x128_pmc.md",
     return float(m.group(1)) if m else None
 
 
-def stream_copy_gbs():
-    """Device-to-device copy bandwidth (read + write bytes / time) of a 1 GiB
-    buffer, the best of 5 copies timed with HIP events."""
+def _copy_lib():
+    """The streaming-copy calibration kernel (schemi_amd/libschemi_copy.so,
+    built from schemi_amd/csrc/stream_copy.hip)."""
+    import ctypes
+
+    path = os.path.join(ROOT, "schemi_amd", "libschemi_copy.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(path)
+    lib.schemi_stream_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                       ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    lib.schemi_stream_copy.restype = ctypes.c_int
+    return lib
+
+
+def stream_copy(reps):
+    """Copies a 1 GiB device buffer `reps` times with the hand-written
+    streaming kernel; returns the fastest copy in ms (HIP events on the
+    kernel's own stream)."""
+    import ctypes
+
     import torch
 
+    lib = _copy_lib()
     src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
-    best = float("inf")
-    for _ in range(6):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        dst.copy_(src)
-        e1.record()
-        e1.synchronize()
-        best = min(best, e0.elapsed_time(e1))
+    torch.cuda.synchronize()
+    ms = ctypes.c_float(0)
+    rc = lib.schemi_stream_copy(src.data_ptr(), dst.data_ptr(), 1 << 30, reps, ctypes.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"schemi_stream_copy failed: {rc}")
+    if not torch.equal(src, dst):
+        raise RuntimeError("schemi_stream_copy: destination differs from source")
     del src, dst
-    return 2 * (1 << 30) / (best * 1e-3) / 1e9
+    return ms.value
+
+
+def stream_copy_gbs():
+    """Device-to-device copy bandwidth (read + write bytes / time) of a 1 GiB
+    buffer: the best of 6 copies by the hand-written streaming kernel."""
+    return 2 * (1 << 30) / (stream_copy(6) * 1e-3) / 1e9
 
 
 def main_diffusion(a):
@@ -308,11 +332,7 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
     if a.copy_calib:
-        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        stream_copy(1)
     if a.diffusion:
         main_diffusion(a)
         return

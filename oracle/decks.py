@@ -34,6 +34,8 @@ _LIM = {0: "zero", 1: "minmod", 2: "vanLeer", 3: "linear", 4: "vanAlbada"}
 _STEP = {0: "SecondOrder", 1: "SecondOrderRK"}
 # gasModel -> #Equation_of_state keyword (abstractMixtureThermodynamics.cpp:38-43)
 _EOS = {0: "ideal", 1: "vanDerWaals", 2: "RedlichKwong", 3: "stiffened", 4: "Kataoka"}
+# turbulenceModel -> #Type_of_source_flag keyword (abstractTurbulenceModel.cpp:262-280)
+_TURB = {0: "zero", 1: "decay", 2: "shear", 6: "kEpsA", 7: "BHR", 8: "BHRKL"}
 
 
 def ref_available() -> bool:
@@ -66,18 +68,23 @@ def _free_lines(case):
 
 
 def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
-               const_dt=-1.0, min_time=None, courant=None):
+               const_dt=-1.0, min_time=None, courant=None, restart_turb=None,
+               zones_turb=None):
     """Writes ``root/set`` for ``case``.
 
     zones:   ((c_left[Ns], v_left[3], p_left), (c_right, v_right, p_right), x0)
              two half-spaces split by the plane x = x0 (zonesDescription.txt).
     restart: (c[Ns, n], v[3, n], p[n]) initial primitive fields in reference
              cell order, injected through the restart reader.
+    restart_turb: (k[n], eps[n]) initial turbulent fields for the restart
+             reader (zeros when None).
+    zones_turb: ((k_left, eps_left), (k_right, eps_right)) zone values.
     """
     set_dir = os.path.join(root, "set")
     os.makedirs(set_dir, exist_ok=True)
     Ns = case.Ns
     dims = {1: "1D", 2: "2D", 3: "3D"}[case.dim]
+    turb = getattr(case, "turbulence", None)
     mt = case.min_time if min_time is None else min_time
     cfl = case.courant if courant is None else courant
     main = [
@@ -95,11 +102,11 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         "#TVD_limiter_type", _LIM[int(case.limiter)],
         "#Number_of_iterations_of_SLE_matrix_solver", str(int(getattr(case, "sle_iters", 2000))),
         "#Diffusion_flag", "on" if getattr(case, "diffusion", False) else "off",
-        "#Turbulence_flag", "off",
+        "#Turbulence_flag", "on" if turb is not None else "off",
         "#Type_of_SLE_solver_flag",
         getattr(case, "sle_solver", "explicit") if getattr(case, "diffusion", False)
         else "GaussSeidel",
-        "#Type_of_source_flag", "BHR",
+        "#Type_of_source_flag", _TURB[int(turb.model)] if turb is not None else "BHR",
         "#Type_of_flow_solver", _FLUX[int(case.flux)],
         "#Dimensions_of_the_task", dims,
         "#Constant_timestep", _r(const_dt),
@@ -156,6 +163,12 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         "#Values_in_zones", _r(pL), _r(pR)]
     for name, val in (("k.txt", "0.0"), ("epsilon.txt", "0.0"), ("b.txt", "0")):
         files[name] = _free_lines(case) + ["#Values_in_zones", val, val]
+    if turb is not None:
+        kz = zones_turb if zones_turb is not None else ((1e-3, 1e-3), (1e-3, 1e-3))
+        files["k.txt"] = _bc_lines(case, turb.bc_k) + [
+            "#Values_in_zones", _r(kz[0][0]), _r(kz[1][0])]
+        files["epsilon.txt"] = _bc_lines(case, turb.bc_eps) + [
+            "#Values_in_zones", _r(kz[0][1]), _r(kz[1][1])]
     files["a.txt"] = _free_lines(case) + ["#Values_in_zones", "0 0 0", "0 0 0"]
     files["chemicalKinetics.txt"] = ["#Type_of_chemical_reaction", "no"]
     files["MPIRanks.txt"] = ["#MPI_nodes_in_directions", "1 1 1"]
@@ -180,15 +193,29 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
             "#Diffusion_coefficient", "2E-5", "#Heat_conduction_coefficient", "2E-2",
             "#Flag_for_enthalpy_flow_solution", "explicit",
             "#Flag_for_molar_mass_diffusion_correction", "on"]
-    turb = ["Cmu 0.09", "Turbulent_Schmidt 0.7", "Turbulent_Prandtl_T 0.7",
+    turb_default = ["Cmu 0.09", "Turbulent_Schmidt 0.7", "Turbulent_Prandtl_T 0.7",
             "Turbulent_Prandtl_e 1E6", "sigma_k 0.7", "sigma_epsilon 0.6",
             "sigma_a 1.4", "sigma_b 0.7", "CMS_R 1E-1", "CMS_D 1E0",
             "Minimal_k 1E-15", "Minimal_epsilon 1E-15", "Minimal_b 1E-15",
             "C0 0.92", "C1 1.44", "C2 1.92", "C3 2.0", "C4 1.1", "Ca 2.5",
             "Cb1 2.0", "Cb2 5.0", "alpha_2 0.15", "alpha_3 0.2", "alpha_4 0.1",
             "CMS_M 1.0"]
-    files["turbulentParameters.txt"] = [y for x in turb for y in
-                                        ("#" + x.split()[0], x.split()[1])]
+    if turb is not None:
+        # the order in which the k-epsilon models read the file:
+        # shearModel.cpp:35-59, zeroModel.cpp:30-53
+        tp = [("Cmu", turb.Cmu), ("Turbulent_Schmidt", turb.sigmaSc),
+              ("Turbulent_Prandtl_T", turb.sigmaT), ("Turbulent_Prandtl_e", turb.sigmaE),
+              ("sigma_k", turb.sigmaK), ("sigma_epsilon", turb.sigmaEps)]
+        cms = [("CMS_R", turb.CMS_R), ("CMS_D", turb.CMS_D)]
+        mins = [("Minimal_k", turb.mink), ("Minimal_epsilon", turb.mineps)]
+        if int(turb.model) == 2:
+            tp += cms + mins + [(f"C{i + 1}", c) for i, c in enumerate(turb.C[:3])]
+        else:
+            tp += mins + cms
+        files["turbulentParameters.txt"] = [y for name, val in tp for y in ("#" + name, _r(val))]
+    else:
+        files["turbulentParameters.txt"] = [y for x in turb_default for y in
+                                            ("#" + x.split()[0], x.split()[1])]
     for name, lines in files.items():
         with open(os.path.join(set_dir, name), "w") as f:
             f.write("\n".join(lines) + "\n")
@@ -206,8 +233,9 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         np.savetxt(os.path.join(d, "velocity.dat"), np.asarray(v).T, fmt="%.20g",
                    delimiter="\t")
         z = np.zeros(n)
-        for name in ("k.dat", "epsilon.dat", "b.dat"):
-            np.savetxt(os.path.join(d, name), z, fmt="%.20g")
+        kd, ed = (restart_turb if restart_turb is not None else (z, z))
+        for name, arr in (("k.dat", kd), ("epsilon.dat", ed), ("b.dat", z)):
+            np.savetxt(os.path.join(d, name), np.asarray(arr), fmt="%.20g")
         np.savetxt(os.path.join(d, "a.dat"), np.zeros((n, 3)), fmt="%.20g",
                    delimiter="\t")
         with open(os.path.join(root, "result", "Time.tsv"), "w") as f:
@@ -247,6 +275,8 @@ def read_output(root, number, Ns):
         "T": col["temperature"],
         "m": np.stack([col["momentum_x"], col["momentum_y"], col["momentum_z"]]),
         "E": col["total_energy"],
+        "k": col["k"],
+        "eps": col["epsilon"],
         "xyz": np.stack([col["coordinate_x"], col["coordinate_y"],
                          col["coordinate_z"]]),
     }

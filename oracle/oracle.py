@@ -58,6 +58,21 @@ class _Config(C.Structure):
         ("molmass", C.c_int),
         ("ts_source", C.c_int),
         ("ts_coeff", C.c_double),
+        ("turb", C.c_int),
+        ("turb_model", C.c_int),
+        ("Cmu", C.c_double),
+        ("sigmaSc", C.c_double),
+        ("sigmaT", C.c_double),
+        ("sigmaE", C.c_double),
+        ("sigmaK", C.c_double),
+        ("sigmaEps", C.c_double),
+        ("CMS_R", C.c_double),
+        ("CMS_D", C.c_double),
+        ("mink", C.c_double),
+        ("mineps", C.c_double),
+        ("tC", C.c_double * 4),
+        ("bc_k", _BC * 6),
+        ("bc_eps", _BC * 6),
     ]
 
 
@@ -73,6 +88,10 @@ class _State(C.Structure):
         ("T", C.POINTER(C.c_double)),
         ("m", C.POINTER(C.c_double) * 3),
         ("E", C.POINTER(C.c_double)),
+        ("k", C.POINTER(C.c_double)),
+        ("eps", C.POINTER(C.c_double)),
+        ("rk", C.POINTER(C.c_double)),
+        ("reps", C.POINTER(C.c_double)),
     ]
 
 
@@ -99,6 +118,8 @@ def lib():
         L.or_init_primitive.argtypes = [C.c_void_p, dp, dp, dp]
         L.or_set_full.argtypes = [C.c_void_p] + [dp] * 10
         L.or_set_conserved.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_init_turbulence.argtypes = [C.c_void_p, dp, dp]
+        L.or_set_turbulence_conserved.argtypes = [C.c_void_p, dp, dp]
         L.or_step.restype = C.c_int
         L.or_step.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.or_step_hancock.restype = C.c_int
@@ -161,6 +182,19 @@ def make_config(case) -> _Config:
     cfg.flux = int(case.flux)
     cfg.limiter = int(case.limiter)
     cfg.gravity = 1 if case.gravity else 0
+    tb = getattr(case, "turbulence", None)
+    if tb is not None:
+        cfg.turb = 1
+        cfg.turb_model = int(tb.model)
+        (cfg.Cmu, cfg.sigmaSc, cfg.sigmaT, cfg.sigmaE, cfg.sigmaK, cfg.sigmaEps, cfg.CMS_R,
+         cfg.CMS_D, cfg.mink, cfg.mineps) = (float(x) for x in tb.params())
+        for i, x in enumerate(tb.C):
+            cfg.tC[i] = float(x)
+        for p in range(6):
+            for dst, src in ((cfg.bc_k, tb.bc_k), (cfg.bc_eps, tb.bc_eps)):
+                dst[p].type = int(src[p].type)
+                for a in range(3):
+                    dst[p].val[a] = float(src[p].value[a])
     for p in range(6):
         for k in range(case.Ns):
             cfg.bc_c[k][p].type = int(case.bc_c[k][p].type)
@@ -192,10 +226,16 @@ class Oracle:
             self._s = None
 
     # -- state ---------------------------------------------------------------
-    def init_primitive(self, c, v, p):
+    def init_primitive(self, c, v, p, k=None, eps=None):
+        """Initial state from primitives; with turbulence also k and epsilon
+        (clipped to the minimal values as the reference's initialisation)."""
         c = np.ascontiguousarray(c, dtype=np.float64).reshape(-1)
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
         p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        if self._cfg.turb:
+            k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1)
+            eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(-1)
+            lib().or_init_turbulence(self._s, _dp(k), _dp(eps))
         lib().or_init_primitive(self._s, _dp(c), _dp(v), _dp(p))
 
     def set_full(self, st: dict):
@@ -205,10 +245,24 @@ class Oracle:
                           _dp(a["rho"]), _dp(a["v"]), _dp(a["p"]), _dp(a["U"]),
                           _dp(a["T"]), _dp(a["m"]), _dp(a["E"]))
 
-    def set_conserved(self, rho_k, rho, m, E):
+    def set_conserved(self, rho_k, rho, m, E, rk=None, reps=None):
+        """State from the conserved vector; with turbulence also rho k and
+        rho epsilon."""
+        if self._cfg.turb:
+            t = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (rk, reps)]
+            lib().or_set_turbulence_conserved(self._s, _dp(t[0]), _dp(t[1]))
         a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
              for x in (rho_k, rho, m, E)]
         lib().or_set_conserved(self._s, *[_dp(x) for x in a])
+
+    def set_conserved_all(self, U):
+        """State from a (nfields, n) conserved array in the library's layout:
+        rho_1..rho_Ns, rho, m_x, m_y, m_z, E[, rho k, rho epsilon]."""
+        Ns = self.case.Ns
+        if self._cfg.turb:
+            self.set_conserved(U[:Ns], U[Ns], U[Ns + 1:Ns + 4], U[Ns + 4], U[Ns + 5], U[Ns + 6])
+        else:
+            self.set_conserved(U[:Ns], U[Ns], U[Ns + 1:Ns + 4], U[Ns + 4])
 
     def state(self) -> dict:
         st = lib().or_get_state(self._s).contents
@@ -228,12 +282,20 @@ class Oracle:
             "T": arr(st.T),
             "m": np.stack([arr(st.m[a]) for a in range(3)]),
             "E": arr(st.E),
+            "k": arr(st.k),
+            "eps": arr(st.eps),
+            "rk": arr(st.rk),
+            "reps": arr(st.reps),
         }
 
     def conserved(self) -> np.ndarray:
-        """(Ns + 5, n) array: rho_1..rho_Ns, rho, m_x, m_y, m_z, E."""
+        """(Ns + 5, n) array: rho_1..rho_Ns, rho, m_x, m_y, m_z, E; with
+        turbulence (Ns + 7, n), rho k and rho epsilon appended."""
         s = self.state()
-        return np.concatenate([s["rho"], s["rho0"][None], s["m"], s["E"][None]])
+        rows = [s["rho"], s["rho0"][None], s["m"], s["E"][None]]
+        if self._cfg.turb:
+            rows += [s["rk"][None], s["reps"][None]]
+        return np.concatenate(rows)
 
     # -- stepping ------------------------------------------------------------
     def step(self, dt: float, mode: int = 0) -> int:

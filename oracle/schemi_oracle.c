@@ -1140,7 +1140,7 @@ static void alloc_state(or_solver *s)
 {
     const long n = s->nc;
     const int Ns = s->cfg.Ns;
-    const int nfield = 2 * (Ns + 1) + 3 + 3 + 3 + 1;
+    const int nfield = 2 * (Ns + 1) + 3 + 3 + 3 + 1 + 4;
     s->pool = calloc((size_t)nfield * n, sizeof(double));
     double *q = s->pool;
     or_state *st = &s->st;
@@ -1169,6 +1169,11 @@ static void alloc_state(or_solver *s)
         q += n;
     }
     st->E = q;
+    q += n;
+    st->k = q;
+    st->eps = q + n;
+    st->rk = q + 2 * n;
+    st->reps = q + 3 * n;
 }
 
 static void copy_state(const or_state *src, or_state *dst, long n, int Ns)
@@ -1185,11 +1190,15 @@ static void copy_state(const or_state *src, or_state *dst, long n, int Ns)
     memcpy(dst->U, src->U, n * sizeof(double));
     memcpy(dst->T, src->T, n * sizeof(double));
     memcpy(dst->E, src->E, n * sizeof(double));
+    memcpy(dst->k, src->k, n * sizeof(double));
+    memcpy(dst->eps, src->eps, n * sizeof(double));
+    memcpy(dst->rk, src->rk, n * sizeof(double));
+    memcpy(dst->reps, src->reps, n * sizeof(double));
 }
 
 static double *state_pool_new(long n, int Ns, or_state *st)
 {
-    const int nfield = 2 * (Ns + 1) + 3 + 3 + 3 + 1;
+    const int nfield = 2 * (Ns + 1) + 3 + 3 + 3 + 1 + 4;
     double *pool = calloc((size_t)nfield * n, sizeof(double));
     double *q = pool;
     st->n = n;
@@ -1217,6 +1226,11 @@ static double *state_pool_new(long n, int Ns, or_state *st)
         q += n;
     }
     st->E = q;
+    q += n;
+    st->k = q;
+    st->eps = q + n;
+    st->rk = q + 2 * n;
+    st->reps = q + 3 * n;
     return pool;
 }
 
@@ -1247,6 +1261,16 @@ or_solver *or_create(const or_config *cfg)
                            (cfg->enth_flow && cfg->sle_solver != OR_SLE_EXPLICIT) ||
                            cfg->sle_solver < OR_SLE_EXPLICIT ||
                            cfg->sle_solver > OR_SLE_GAUSS_SEIDEL || cfg->sle_iters < 1)) {
+        free(s);
+        return NULL;
+    }
+    /* turbulence: the shear and zero models, ideal or van der Waals mixture,
+     * the diffusion stage (if on) with the explicit solver and neither the
+     * molar-mass correction nor the enthalpy flow */
+    if (cfg->turb && ((cfg->turb_model != OR_TURB_SHEAR && cfg->turb_model != OR_TURB_ZERO) ||
+                      (cfg->gas_model != OR_GAS_IDEAL && cfg->gas_model != OR_GAS_VDW) ||
+                      (cfg->diffusion && (cfg->sle_solver != OR_SLE_EXPLICIT || cfg->molmass ||
+                                          cfg->enth_flow)))) {
         free(s);
         return NULL;
     }
@@ -1319,10 +1343,37 @@ void or_init_primitive(or_solver *s, const double *c, const double *v,
             cc[k] = st->c[k][i];
         st->U[i] = eos_U(&s->cfg, cc, st->p[i]);
         st->T[i] = eos_T(&s->cfg, cc, st->U[i]);
+        if (s->cfg.turb) {
+            st->rk[i] = st->k[i] * r0;
+            st->reps[i] = r0 * st->eps[i];
+        }
         double vel[3] = {st->v[0][i], st->v[1][i], st->v[2][i]};
         const double v2 = dot3(vel, vel);
-        st->E[i] = (st->U[i] + r0 * v2 * 0.5) + 0.0;
+        st->E[i] = (st->U[i] + r0 * v2 * 0.5) + st->rk[i];
     }
+}
+
+/* minimal-value clip of the initial k and epsilon
+ * (phaseInitialization.cpp:1814-1825); or_init_primitive then forms rho k and
+ * rho epsilon (1846-1848) and adds rho k to the total energy (1873-1877) */
+void or_init_turbulence(or_solver *s, const double *k, const double *eps)
+{
+    or_state *st = &s->st;
+    for (long i = 0; i < s->nc; ++i) {
+        if (!s->cfg.turb) {
+            st->k[i] = st->eps[i] = st->rk[i] = st->reps[i] = 0.0;
+            continue;
+        }
+        st->k[i] = (k[i] < s->cfg.mink) ? s->cfg.mink : k[i];
+        st->eps[i] = (eps[i] < s->cfg.mineps) ? s->cfg.mineps : eps[i];
+    }
+}
+
+void or_set_turbulence_conserved(or_solver *s, const double *rk, const double *reps)
+{
+    or_state *st = &s->st;
+    memcpy(st->rk, rk, s->nc * sizeof(double));
+    memcpy(st->reps, reps, s->nc * sizeof(double));
 }
 
 void or_set_full(or_solver *s, const double *c0, const double *c,
@@ -1365,11 +1416,15 @@ static void recover_primitives(or_solver *s, or_state *st)
             st->v[a][i] = st->m[a][i] / st->rho[0][i];
         double vel[3] = {st->v[0][i], st->v[1][i], st->v[2][i]};
         const double v2 = dot3(vel, vel);
-        st->U[i] = (st->E[i] - st->rho[0][i] * v2 * 0.5) - 0.0;
+        st->U[i] = (st->E[i] - st->rho[0][i] * v2 * 0.5) - st->rk[i];
         for (int k = 0; k <= Ns; ++k)
             cc[k] = st->c[k][i];
         st->p[i] = eos_p(&s->cfg, cc, st->U[i]);
         st->T[i] = eos_T(&s->cfg, cc, st->U[i]);
+        if (s->cfg.turb) { /* Advection2dOrder.cpp:610-627 */
+            st->k[i] = st->rk[i] / st->rho[0][i];
+            st->eps[i] = st->reps[i] / st->rho[0][i];
+        }
     }
 }
 
@@ -1744,6 +1799,7 @@ typedef struct {
     double v[3];
     double m[3];
     double p, U, T, E;
+    double k, eps, rk, reps; /* turbulence (zero without it) */
 } face_state;
 
 /* Face-side closure: Advection2dOrder.cpp:233-289 (owner side) and 301-372
@@ -1758,12 +1814,16 @@ static void close_face(const or_config *cfg, face_state *fs)
     }
     fs->c[0] = c0;
     fs->rho[0] = r0;
+    if (cfg->turb) { /* Advection2dOrder.cpp:249-266, 326-347, 440-458 */
+        fs->rk = fs->k * r0;
+        fs->reps = fs->eps * r0;
+    }
     fs->U = eos_U(cfg, fs->c, fs->p);
     fs->T = eos_T(cfg, fs->c, fs->U);
     for (int a = 0; a < 3; ++a)
         fs->m[a] = fs->v[a] * r0;
     const double v2 = dot3(fs->v, fs->v);
-    fs->E = fs->U + r0 * v2 * 0.5 + 0.0;
+    fs->E = fs->U + r0 * v2 * 0.5 + fs->rk;
 }
 
 /* Flux vectors stored like conservativeFlows (conservativeFlows.hpp:22-34):
@@ -1772,6 +1832,7 @@ typedef struct {
     double rho[OR_MAXS + 1][3];
     double m[9];
     double E[3];
+    double rk[3], reps[3]; /* turbulence */
 } face_flux;
 
 /* HLLCSolver::calculateFlows per-face body (HLLCSolver.cpp:52-388) */
@@ -1779,13 +1840,15 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
                      const face_state *Nb, face_flux *F)
 {
     const int nd = cfg->Ns + 1;
+    /* sqSonicSpeed + (2/3) k (1 + dpdUv) (HLLCSolver.cpp:26-50,
+     * HLLSolver.cpp:26-50); turbulent pressure (2/3) rho k (:61-64) */
     const double cO = sqrt(eos_c2(cfg, O->c, O->rho[0], O->U, O->p) +
-                           (2. / 3.) * 0.0 * (1. + eos_dpdUv(cfg, O->c)));
+                           (2. / 3.) * O->k * (1. + eos_dpdUv(cfg, O->c)));
     const double cN = sqrt(eos_c2(cfg, Nb->c, Nb->rho[0], Nb->U, Nb->p) +
-                           (2. / 3.) * 0.0 * (1. + eos_dpdUv(cfg, Nb->c)));
+                           (2. / 3.) * Nb->k * (1. + eos_dpdUv(cfg, Nb->c)));
     const double uO = dot3(O->v, N);
     const double uN = dot3(Nb->v, N);
-    const double ptO = (2. / 3.) * 0.0, ptN = (2. / 3.) * 0.0;
+    const double ptO = (2. / 3.) * O->rk, ptN = (2. / 3.) * Nb->rk;
     const double SO = smin(uO - cO, uN - cN);
     const double SN = smax(uO + cO, uN + cN);
     const double rO = O->rho[0], rN = Nb->rho[0];
@@ -1812,6 +1875,10 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
         const double e = uO * (O->E + O->p + ptO);
         for (int a = 0; a < 3; ++a)
             F->E[a] = N[a] * e;
+        for (int a = 0; a < 3; ++a) { /* HLLCSolver.cpp:126-133 */
+            F->rk[a] = (uO * O->rk) * N[a];
+            F->reps[a] = (uO * N[a]) * O->reps;
+        }
     } else if (Sc >= 0) {
         const double factor = SO - uO;
         const double den = 1 / (SO - Sc);
@@ -1836,6 +1903,14 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
         const double e = (O->E + O->p + ptO) * uO + SO * (Es - O->E);
         for (int a = 0; a < 3; ++a)
             F->E[a] = N[a] * e;
+        /* HLLCSolver.cpp:165-168, 215-228 */
+        const double rks = O->rk * factor * den, reps_s = O->reps * factor * den;
+        const double fk = O->rk * uO + SO * (rks - O->rk);
+        const double fe = O->reps * uO + SO * (reps_s - O->reps);
+        for (int a = 0; a < 3; ++a) {
+            F->rk[a] = fk * N[a];
+            F->reps[a] = fe * N[a];
+        }
     } else if (SN > 0) {
         const double factor = SN - uN;
         const double den = 1 / (SN - Sc);
@@ -1860,6 +1935,14 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
         const double e = (Nb->E + Nb->p + ptN) * uN + SN * (Es - Nb->E);
         for (int a = 0; a < 3; ++a)
             F->E[a] = N[a] * e;
+        /* HLLCSolver.cpp:262-265, 312-325 */
+        const double rks = Nb->rk * factor * den, reps_s = Nb->reps * factor * den;
+        const double fk = Nb->rk * uN + SN * (rks - Nb->rk);
+        const double fe = Nb->reps * uN + SN * (reps_s - Nb->reps);
+        for (int a = 0; a < 3; ++a) {
+            F->rk[a] = fk * N[a];
+            F->reps[a] = fe * N[a];
+        }
     } else if (SN <= 0) {
         for (int k = 0; k < nd; ++k)
             for (int a = 0; a < 3; ++a)
@@ -1873,6 +1956,10 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
         const double e = uN * (Nb->E + Nb->p + ptN);
         for (int a = 0; a < 3; ++a)
             F->E[a] = N[a] * e;
+        for (int a = 0; a < 3; ++a) { /* HLLCSolver.cpp:368-375 */
+            F->rk[a] = (uN * Nb->rk) * N[a];
+            F->reps[a] = (uN * N[a]) * Nb->reps;
+        }
     } else {
         return ERR_RIEMANN;
     }
@@ -1884,13 +1971,15 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
                     const face_state *Nb, face_flux *F)
 {
     const int nd = cfg->Ns + 1;
+    /* sqSonicSpeed + (2/3) k (1 + dpdUv) (HLLCSolver.cpp:26-50,
+     * HLLSolver.cpp:26-50); turbulent pressure (2/3) rho k (:61-64) */
     const double cO = sqrt(eos_c2(cfg, O->c, O->rho[0], O->U, O->p) +
-                           (2. / 3.) * 0.0 * (1. + eos_dpdUv(cfg, O->c)));
+                           (2. / 3.) * O->k * (1. + eos_dpdUv(cfg, O->c)));
     const double cN = sqrt(eos_c2(cfg, Nb->c, Nb->rho[0], Nb->U, Nb->p) +
-                           (2. / 3.) * 0.0 * (1. + eos_dpdUv(cfg, Nb->c)));
+                           (2. / 3.) * Nb->k * (1. + eos_dpdUv(cfg, Nb->c)));
     const double uO = dot3(O->v, N);
     const double uN = dot3(Nb->v, N);
-    const double ptO = (2. / 3.) * 0.0, ptN = (2. / 3.) * 0.0;
+    const double ptO = (2. / 3.) * O->rk, ptN = (2. / 3.) * Nb->rk;
     const double SO = smin(uO - cO, uN - cN);
     const double SN = smax(uO + cO, uN + cN);
     double IN[3];
@@ -1911,6 +2000,10 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
         const double e = uO * (O->E + O->p + ptO);
         for (int a = 0; a < 3; ++a)
             F->E[a] = N[a] * e;
+        for (int a = 0; a < 3; ++a) { /* HLLSolver.cpp:107-114 */
+            F->rk[a] = (uO * O->rk) * N[a];
+            F->reps[a] = (uO * O->reps) * N[a];
+        }
     } else if (SN > 0) {
         const double den = 1 / (SN - SO);
         for (int k = 0; k < nd; ++k) {
@@ -1935,6 +2028,14 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
                          den;
         for (int a = 0; a < 3; ++a)
             F->E[a] = N[a] * e;
+        /* HLLSolver.cpp:209-224 */
+        const double fk = (SN * uO * O->rk - SO * uN * Nb->rk + SN * SO * (Nb->rk - O->rk)) * den;
+        const double fe =
+            (SN * uO * O->reps - SO * uN * Nb->reps + SN * SO * (Nb->reps - O->reps)) * den;
+        for (int a = 0; a < 3; ++a) {
+            F->rk[a] = fk * N[a];
+            F->reps[a] = fe * N[a];
+        }
     } else if (SN <= 0) {
         for (int k = 0; k < nd; ++k)
             for (int a = 0; a < 3; ++a)
@@ -1948,6 +2049,10 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
         const double e = uN * (Nb->E + Nb->p + ptN);
         for (int a = 0; a < 3; ++a)
             F->E[a] = N[a] * e;
+        for (int a = 0; a < 3; ++a) { /* HLLSolver.cpp:277-284 */
+            F->rk[a] = (uN * Nb->rk) * N[a];
+            F->reps[a] = (uN * Nb->reps) * N[a];
+        }
     } else {
         return ERR_RIEMANN;
     }
@@ -2112,6 +2217,9 @@ static int stage_impl(or_solver *s, double dt, int hancock)
     double *Lv = malloc(9 * nc * sizeof(double));
     double *Gp = malloc(3 * nc * sizeof(double));
     double *Lp = malloc(3 * nc * sizeof(double));
+    /* turbulence: limited gradients of k and epsilon */
+    double *Gt = calloc(3 * nc * 2, sizeof(double));
+    double *Lt = calloc(3 * nc * 2, sizeof(double));
     face_state *FO = calloc(nf, sizeof(face_state));
     face_state *FN = calloc(nf, sizeof(face_state));
     face_flux *FF = calloc(nf, sizeof(face_flux));
@@ -2131,6 +2239,15 @@ static int stage_impl(or_solver *s, double dt, int hancock)
         err = grad_scalar(s, st->p, cfg->bc_p, Gp);
     if (!err)
         err = tvd_scalar(s, Gp, st->p, cfg->bc_p, Lp);
+    if (!err && cfg->turb) { /* Advection2dOrder.cpp:77-90 */
+        err = grad_scalar(s, st->k, cfg->bc_k, Gt);
+        if (!err)
+            err = tvd_scalar(s, Gt, st->k, cfg->bc_k, Lt);
+        if (!err)
+            err = grad_scalar(s, st->eps, cfg->bc_eps, &Gt[3 * nc]);
+        if (!err)
+            err = tvd_scalar(s, &Gt[3 * nc], st->eps, cfg->bc_eps, &Lt[3 * nc]);
+    }
     if (err)
         goto done;
 
@@ -2147,6 +2264,10 @@ static int stage_impl(or_solver *s, double dt, int hancock)
                 fs->v[a] = (T[0] * d[0] + T[1] * d[1] + T[2] * d[2]) + st->v[a][i];
             }
             fs->p = dot3(&Lp[3 * i], d) + st->p[i];
+            if (cfg->turb) { /* Advection2dOrder.cpp:143-150 */
+                fs->k = dot3(&Lt[3 * i], d) + st->k[i];
+                fs->eps = dot3(&Lt[3 * nc + 3 * i], d) + st->eps[i];
+            }
         }
     }
     /* owner side everywhere, neighbour side on inner faces */
@@ -2171,6 +2292,11 @@ static int stage_impl(or_solver *s, double dt, int hancock)
             err = bc_cell_vec(&cfg->bc_v[pt], FO[f].v, &s->fN[3 * f], g->v);
         if (!err)
             err = bc_cell_scalar(&cfg->bc_p[pt], FO[f].p, &g->p);
+        if (!err && cfg->turb) { /* Advection2dOrder.cpp:440-458 */
+            err = bc_cell_scalar(&cfg->bc_k[pt], FO[f].k, &g->k);
+            if (!err)
+                err = bc_cell_scalar(&cfg->bc_eps[pt], FO[f].eps, &g->eps);
+        }
         if (!err)
             close_face(cfg, g);
     }
@@ -2187,7 +2313,7 @@ static int stage_impl(or_solver *s, double dt, int hancock)
     /* time integration (Advection2dOrder.cpp:541-602) with divergence
      * (divergence.hpp:315-350) */
     for (long i = 0; i < nc; ++i) {
-        double accE = 0, accm[3] = {0, 0, 0}, accr[OR_MAXS + 1] = {0};
+        double accE = 0, accm[3] = {0, 0, 0}, accr[OR_MAXS + 1] = {0}, acck = 0, acce = 0;
         for (int j = 0; j < s->cnf[i]; ++j) {
             const long f = s->cf[6 * i + j];
             double n[3];
@@ -2204,6 +2330,8 @@ static int stage_impl(or_solver *s, double dt, int hancock)
             }
             for (int k = 0; k <= Ns; ++k)
                 accr[k] += dot3(FF[f].rho[k], n) * S;
+            acck += dot3(FF[f].rk, n) * S;
+            acce += dot3(FF[f].reps, n) * S;
         }
         const double V = s->cV[i];
         const double r0 = st->rho[0][i];
@@ -2218,6 +2346,10 @@ static int stage_impl(or_solver *s, double dt, int hancock)
         }
         for (int k = 0; k <= Ns; ++k)
             st->rho[k][i] = st->rho[k][i] - (accr[k] / V) * dt;
+        if (cfg->turb) { /* Advection2dOrder.cpp:560-564 */
+            st->rk[i] = st->rk[i] - (acck / V) * dt;
+            st->reps[i] = st->reps[i] - (acce / V) * dt;
+        }
     }
     recover_primitives(s, st);
 
@@ -2228,6 +2360,8 @@ done:
     free(Lv);
     free(Gp);
     free(Lp);
+    free(Gt);
+    free(Lt);
     free(FO);
     free(FN);
     free(FF);
@@ -2249,10 +2383,16 @@ static void rk_average(or_solver *s, const or_state *Un)
         for (int a = 0; a < 3; ++a)
             st->m[a][i] = st->m[a][i] * w + Un->m[a][i] * iw;
         st->E[i] = w * st->E[i] + iw * Un->E[i];
+        st->rk[i] = w * st->rk[i] + iw * Un->rk[i];
+        st->reps[i] = w * st->reps[i] + iw * Un->reps[i];
         for (int a = 0; a < 3; ++a)
             st->v[a][i] = st->m[a][i] / st->rho[0][i];
+        if (s->cfg.turb) {
+            st->k[i] = st->rk[i] / st->rho[0][i];
+            st->eps[i] = st->reps[i] / st->rho[0][i];
+        }
         double vel[3] = {st->v[0][i], st->v[1][i], st->v[2][i]};
-        st->U[i] = st->E[i] - 0.0 - 0.5 * st->rho[0][i] * dot3(vel, vel);
+        st->U[i] = st->E[i] - st->rk[i] - 0.5 * st->rho[0][i] * dot3(vel, vel);
         for (int k = 0; k <= Ns; ++k)
             cc[k] = st->c[k][i];
         st->p[i] = eos_p(&s->cfg, cc, st->U[i]);
@@ -2830,6 +2970,18 @@ static int species_corrections(or_solver *s, const double *FY, const double *CSf
     return 0;
 }
 
+/* kEpsModels::thetaS (kEpsModels.cpp:13-26): the realisability factor of
+ * the turbulent viscosity for the velocity divergence divV */
+static double theta_s(const or_config *cfg, double divV, double k, double eps, double CMS)
+{
+    const double Cmu2 = cfg->Cmu * cfg->Cmu;
+    const double k2 = k * k;
+    const double eps2 = eps * eps;
+    const double CMS2 = CMS * CMS;
+    const double divV2 = divV * divV;
+    return 1. / sqrt(1 + 16. / 9. * Cmu2 * k2 / (CMS2 * eps2) * divV2);
+}
+
 static int diffusion_explicit(or_solver *s, double dt)
 {
     const or_config *cfg = &s->cfg;
@@ -2857,6 +3009,21 @@ static int diffusion_explicit(or_solver *s, double dt)
     double *MC = calloc(3 * Ns * nf, sizeof(double));
     double *EF = calloc(3 * nf, sizeof(double));
     double cc[OR_MAXS + 1];
+    /* turbulence (Diffusion.cpp:129-135, 137-183, 242-260, 414-417, 580-598,
+     * 711-933, 999-1028): face k and epsilon, the cells' Cv, CvM = Cv / (rho
+     * / c0) and nonIdeality / rho, the surface Cv, and the face fluxes of k,
+     * epsilon and of the non-ideality correction Laplacian */
+    const int turb = cfg->turb;
+    const long tn = turb ? nc : 1, tf = turb ? nf : 1;
+    double *Cvc = malloc(tn * sizeof(double));
+    double *CvM = malloc(tn * sizeof(double));
+    double *NIR = malloc(tn * sizeof(double));
+    double *ksf = malloc(tf * sizeof(double));
+    double *esf = malloc(tf * sizeof(double));
+    double *Cvs = malloc(tf * sizeof(double));
+    double *FKt = calloc(3 * tf, sizeof(double));
+    double *FEt = calloc(3 * tf, sizeof(double));
+    double *FNI = calloc(3 * tf, sizeof(double));
 
     /* mass fractions rho_k / rho, normalised (diffusiveFields.cpp:30-47) */
     for (long i = 0; i < nc; ++i) {
@@ -2870,6 +3037,12 @@ static int diffusion_explicit(or_solver *s, double dt)
         for (int k = 0; k <= Ns; ++k)
             cc[k] = st->c[k][i];
         CCVold[i] = st->c[0][i] * cv_mix(cfg, cc); /* Diffusion.cpp:148 */
+        if (turb) { /* Diffusion.cpp:137-148, 168 */
+            Cvc[i] = cv_mix(cfg, cc);
+            const double avM = st->rho[0][i] / st->c[0][i];
+            CvM[i] = Cvc[i] / avM;
+            NIR[i] = non_ideality(cfg, cc) / st->rho[0][i];
+        }
     }
     /* face velocity (linearInterpolate) for grad and divergence of v, and
      * the face density sum_k c_k,f M_k (Diffusion.cpp:156-166) */
@@ -2901,6 +3074,23 @@ static int diffusion_explicit(or_solver *s, double dt)
             CSf[k * nf + f] = cs[k];
         /* surface temperature (linearInterpolate; calculatedTemperature on
          * boundary faces from the surface concentrations and pressure) */
+        if (turb) {
+            /* linearInterpolate of k, epsilon and Cv (boundary: the surface
+             * values; calculatedCv from the surface concentrations,
+             * boundaryConditionValue.hpp:210-227) */
+            if (s->fpatch[f] < 0) {
+                const long n = s->nei[f];
+                ksf[f] = st->k[o] * s->wo[f] + st->k[n] * s->wn[f];
+                esf[f] = st->eps[o] * s->wo[f] + st->eps[n] * s->wn[f];
+                Cvs[f] = Cvc[o] * s->wo[f] + Cvc[n] * s->wn[f];
+            } else {
+                const int p = s->fpatch[f];
+                if (bc_surf_scalar(&cfg->bc_k[p], st->k[o], &ksf[f]) ||
+                    bc_surf_scalar(&cfg->bc_eps[p], st->eps[o], &esf[f]))
+                    err = ERR_BC;
+                Cvs[f] = cv_mix(cfg, cs);
+            }
+        }
         if (s->fpatch[f] < 0) {
             const long n = s->nei[f];
             TSf[f] = st->T[o] * s->wo[f] + st->T[n] * s->wn[f];
@@ -2956,9 +3146,21 @@ static int diffusion_explicit(or_solver *s, double dt)
             for (int k = 0; k < Ns; ++k)
                 GH[k * nf + f] = qn[k];
         }
+        /* effective coefficients (transportCoefficients.hpp:65-83, 221-263):
+         * turbulent nu = Cmu k^2 / eps at the face (kEpsModels.cpp:64-74) */
+        double mu_f = cfg->mu, kappa_f = cfg->kappa, tD = 0, tnu = 0, tLam = 0;
+        if (turb) {
+            tnu = cfg->Cmu * ksf[f] * ksf[f] / esf[f];
+            tD = tnu / cfg->sigmaSc;
+            const double tKap = tnu / cfg->sigmaT;
+            tLam = tnu / cfg->sigmaE;
+            mu_f = cfg->mu + srho[f] * tnu;
+            const double cC = CSf[f] * Cvs[f];
+            kappa_f = cfg->kappa + cC * tLam + cC * tKap;
+        }
         surf_grad(s, f, Ns, qo, qn, g);
         for (int k = 0; k < Ns; ++k) {
-            const double rhoD = srho[f] * cfg->D; /* transportCoefficients.hpp:258-261 */
+            const double rhoD = srho[f] * (cfg->D + tD); /* transportCoefficients.hpp:246-261 */
             for (int b = 0; b < 3; ++b)
                 FY[(k * nf + f) * 3 + b] = (g[3 * k + b] * rhoD) * -1;
         }
@@ -2977,7 +3179,7 @@ static int diffusion_explicit(or_solver *s, double dt)
         }
         surf_grad(s, f, 1, qo, qn, g);
         for (int b = 0; b < 3; ++b)
-            FT[3 * f + b] = (g[b] * cfg->kappa) * -1;
+            FT[3 * f + b] = (g[b] * kappa_f) * -1;
         /* velocity: surfGrad (tensor v_a d_b) and surfDivergence
          * (divergence.hpp:84-150) */
         for (int a = 0; a < 3; ++a) {
@@ -3017,7 +3219,7 @@ static int diffusion_explicit(or_solver *s, double dt)
             divs = dot3(dq, dd);
         }
         for (int q = 0; q < 9; ++q)
-            FV[9 * f + q] = (g[q] * cfg->mu) * -1;
+            FV[9 * f + q] = (g[q] * mu_f) * -1;
         /* strainSurf1 = (surfGrad v)^T - 2/3 div_s v I, times mu
          * (Diffusion.cpp:193-230, 482-483) */
         const double lam = (2. / 3.) * divs;
@@ -3026,8 +3228,47 @@ static int diffusion_explicit(or_solver *s, double dt)
                 double t = g[3 * b + a];
                 if (a == b)
                     t -= lam;
-                FS[9 * f + 3 * a + b] = t * cfg->mu;
+                FS[9 * f + 3 * a + b] = t * mu_f;
             }
+        if (turb) {
+            /* k and epsilon Laplacians with thetaS_D of the face divergence
+             * (Diffusion.cpp:247-255, 270-286; SLEMatrix.cpp:2054-2078) */
+            const double th = (divs < 0.) ? theta_s(cfg, divs, ksf[f], esf[f], cfg->CMS_D) : 1.;
+            const double kD = tnu / cfg->sigmaK * th, eD = tnu / cfg->sigmaEps * th;
+            const double rhoDk = cfg->mu + srho[f] * kD, rhoDe = cfg->mu + srho[f] * eD;
+            double to[2] = {st->k[o], st->eps[o]}, tg[2] = {st->k[n], st->eps[n]}, gt[6];
+            if (bnd) {
+                const int p = s->fpatch[f];
+                if (bc_cell_scalar(&cfg->bc_k[p], to[0], &tg[0]) ||
+                    bc_cell_scalar(&cfg->bc_eps[p], to[1], &tg[1])) {
+                    err = ERR_BC;
+                    goto done;
+                }
+            }
+            surf_grad(s, f, 2, to, tg, gt);
+            for (int b = 0; b < 3; ++b) {
+                FKt[3 * f + b] = (gt[b] * rhoDk) * -1;
+                FEt[3 * f + b] = (gt[3 + b] * rhoDe) * -1;
+            }
+            /* non-ideality correction Laplacian -(surfGrad(CvM) T_f +
+             * surfGrad(nonIdeality / rho)) rho_f tLambda (Diffusion.cpp:
+             * 170-172, 414-417); boundary ghosts calculatedCvM and
+             * calculatedNonidealityCorrectionPerDensity from the ghost
+             * concentrations (boundaryConditionValue.hpp:415-484) */
+            double mo[2] = {CvM[o], NIR[o]}, mg[2] = {CvM[n], NIR[n]}, gm[6];
+            if (bnd) {
+                double dg = 0;
+                for (int k = 1; k <= Ns; ++k)
+                    dg += cg[k] * cfg->M[k - 1];
+                mg[0] = cv_mix(cfg, cg) * cg[0] / dg;
+                mg[1] = non_ideality(cfg, cg) / dg;
+            }
+            surf_grad(s, f, 2, mo, mg, gm);
+            for (int b = 0; b < 3; ++b) {
+                const double flow = -1 * (gm[b] * TSf[f] + gm[3 + b]);
+                FNI[3 * f + b] = flow * srho[f] * tLam;
+            }
+        }
     }
     if (cfg->molmass || cfg->enth_flow) {
         err = species_corrections(s, FY, CSf, TSf, MC, EF);
@@ -3042,6 +3283,12 @@ static int diffusion_explicit(or_solver *s, double dt)
      * Diffusion.cpp:431-458, 462-483, 599-615, 711-760, 795-806, 818-828,
      * 963-1059) */
     double dt_src = 0;
+    /* the turbulence model's source step: epsilon below 1e-3 of its maximum
+     * is replaced by veryBig (shearModel.cpp:107-114) */
+    double epsmax = 0, tsrc = VERYBIG;
+    if (turb)
+        for (long i = 0; i < nc; ++i)
+            epsmax = (i == 0 || epsmax < st->eps[i]) ? st->eps[i] : epsmax;
     for (long i = 0; i < nc; ++i) {
         double dY[OR_MAXS] = {0}, dT = 0, dV[3] = {0}, dS[3] = {0};
         for (int j = 0; j < s->cnf[i]; ++j) {
@@ -3121,6 +3368,24 @@ static int diffusion_explicit(or_solver *s, double dt)
             heat += (sv[q] * cfg->mu) * G[q];
         double free_t = 0. - dT / V;
         double diag_t = CCVnew;
+        double dK = 0, dE = 0;
+        if (turb) {
+            double dNI = 0;
+            for (int j = 0; j < s->cnf[i]; ++j) {
+                const long f = s->cf[6 * i + j];
+                double n[3];
+                out_normal(s, f, i, n);
+                dNI += dot3(&FNI[3 * f], n) * s->fS[f];
+                dK += dot3(&FKt[3 * f], n) * s->fS[f];
+                dE += dot3(&FEt[3 * f], n) * s->fS[f];
+            }
+            /* Diffusion.cpp:604-611 (distributeSourceTerm) */
+            const double src = (dNI / V) * (-1.0);
+            if (src >= 0.)
+                free_t += src;
+            else
+                diag_t -= src / (st->T[i] + STAB);
+        }
         if (cfg->enth_flow) {
             /* source time step coeff * min(CCV_old T_old / |div(flow) +
              * stab|) (Diffusion.cpp:696-700), reduced over the cells here */
@@ -3133,6 +3398,51 @@ static int diffusion_explicit(or_solver *s, double dt)
                 diag_t -= src / (st->T[i] + STAB);
         }
         free_t += heat;
+        double kn = 0, en = 0;
+        if (turb) {
+            const double k = st->k[i], eps = st->eps[i];
+            /* rho epsilon (kEpsModels.cpp:76-82) and the filtered part of the
+             * turbulent stress (Diffusion.cpp:586-598, 719-733) */
+            if (cfg->turb_model != OR_TURB_ZERO)
+                free_t += rho * eps;
+            const double tnu = cfg->Cmu * k * k / eps;
+            const double thR = theta_s(cfg, divV[i], k, eps, cfg->CMS_R);
+            const double tnu1 = (1. - thR) * tnu;
+            double philt = 0, gsph = 0, gdev = 0;
+            for (int q = 0; q < 9; ++q)
+                philt += (sv[q] * rho * tnu1) * G[q];
+            free_t += philt;
+            /* spherical and deviatoric Reynolds stresses and their
+             * generation terms (Diffusion.cpp:592-597; shearModel.cpp:116-147) */
+            for (int q = 0; q < 9; ++q) {
+                const double Iq = (q == 0 || q == 4 || q == 8) ? 1. : 0.;
+                const double sph = Iq * -(2. / 3.) * st->rk[i];
+                const double dev = sv[q] * rho * tnu * thR;
+                gsph += sph * G[q];
+                gdev += dev * G[q];
+            }
+            double sk = 0, se = 0;
+            if (cfg->turb_model == OR_TURB_SHEAR) {
+                const double ek = eps / k;
+                const double dissip = -st->reps[i];
+                const double Sk1 = gsph + gdev, Sk2 = dissip / st->k[i];
+                const double Se1 = (cfg->tC[0] * gdev + cfg->tC[2] * gsph) * ek;
+                const double Se2 = cfg->tC[1] * dissip / st->k[i];
+                sk = Sk1 + Sk2 * k;
+                se = Se1 + Se2 * eps;
+                const double me = (eps < 1E-3 * epsmax) ? VERYBIG : eps;
+                const double r = fabs(cfg->ts_coeff * me / ((Se1 + Se2 * st->eps[i]) / rho + STAB));
+                tsrc = (r < tsrc) ? r : tsrc;
+            }
+            /* explicit Laplacians plus the sources (Diffusion.cpp:839-862),
+             * minimal values (905-917) */
+            kn = (k * rho + (0. - dK / V) * dt) / rho + sk / rho * dt;
+            en = (eps * rho + (0. - dE / V) * dt) / rho + se / rho * dt;
+            if (kn < cfg->mink)
+                kn = cfg->mink;
+            if (en < cfg->mineps)
+                en = cfg->mineps;
+        }
         double co[OR_MAXS + 1];
         for (int k = 0; k <= Ns; ++k)
             co[k] = st->c[k][i];
@@ -3151,13 +3461,21 @@ static int diffusion_explicit(or_solver *s, double dt)
         st->T[i] = Tn;
         st->U[i] = uvc_from_T(cfg, cn, Tn) * cn[0];
         st->p[i] = eos_p(cfg, cn, st->U[i]);
+        if (turb) { /* Diffusion.cpp:999-1011 */
+            st->k[i] = kn;
+            st->eps[i] = en;
+            st->rk[i] = kn * rho;
+            st->reps[i] = en * rho;
+        }
         const double v2 = dot3(vn, vn);
-        st->E[i] = (st->U[i] + rho * v2 * 0.5) + 0.0;
+        st->E[i] = (st->U[i] + rho * v2 * 0.5) + st->rk[i];
         if (Tn < 0.)
             err = ERR_NEG_T;
     }
     if (cfg->enth_flow)
         s->dt_src = cfg->ts_coeff * dt_src;
+    if (turb) /* shearModel.cpp:150, zeroModel.cpp:90 */
+        s->dt_src = (tsrc < s->dt_src) ? tsrc : s->dt_src;
 done:
     free(Y);
     free(CCVold);
@@ -3174,6 +3492,15 @@ done:
     free(TSf);
     free(MC);
     free(EF);
+    free(Cvc);
+    free(CvM);
+    free(NIR);
+    free(ksf);
+    free(esf);
+    free(Cvs);
+    free(FKt);
+    free(FEt);
+    free(FNI);
     return err;
 }
 
@@ -3207,6 +3534,8 @@ int or_step(or_solver *s, double dt, int mode)
  * enabled; chemistry off. */
 int or_step_hancock(or_solver *s, double dt, int mode)
 {
+    if (s->cfg.turb)
+        return ERR_INIT; /* turbulence is built for the SSP-RK2 step only */
     if (mode == 1)
         recover_primitives(s, &s->st);
     int err = stage_impl(s, dt, 1);

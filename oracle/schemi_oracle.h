@@ -99,7 +99,25 @@ typedef struct {
      * of the same entry (Schemi.cpp:205-225) */
     int ts_source;
     double ts_coeff;
+    /* k-epsilon turbulence (#Turbulence_flag on, #Type_of_source_flag):
+     * turb_model OR_TURB_SHEAR (shearModel.cpp) or OR_TURB_ZERO
+     * (zeroModel.cpp: transport and diffusion of k and epsilon, no
+     * sources); the parameters of set/turbulentParameters.txt
+     * (kEpsModels.cpp, abstractTurbulenceModel.hpp:26-57); tC the model
+     * constants (shear: C1, C2, C3); bc_k and bc_eps the boundary
+     * conditions of set/k.txt and set/epsilon.txt.  Turbulence needs the
+     * SSP-RK2 step; with the diffusion stage, the explicit matrix solver and
+     * no molar-mass correction or enthalpy flow. */
+    int turb;
+    int turb_model;
+    double Cmu, sigmaSc, sigmaT, sigmaE, sigmaK, sigmaEps, CMS_R, CMS_D, mink, mineps;
+    double tC[4];
+    or_bc bc_k[6];
+    or_bc bc_eps[6];
 } or_config;
+
+/* turbulenceModel (turbulenceModelEnum.hpp) values of the built models */
+enum { OR_TURB_ZERO = 0, OR_TURB_SHEAR = 2 };
 
 enum { OR_SLE_EXPLICIT = 0, OR_SLE_JACOBI = 1, OR_SLE_GAUSS_SEIDEL = 2 };
 
@@ -117,6 +135,9 @@ typedef struct {
     double *T;
     double *m[3];
     double *E;
+    /* turbulence: k, epsilon and the conserved rho k, rho epsilon (zero
+     * without turbulence) */
+    double *k, *eps, *rk, *reps;
 } or_state;
 
 or_solver *or_create(const or_config *cfg);
@@ -129,6 +150,13 @@ or_state *or_get_state(or_solver *s);
  * initialisation does (phaseInitialization.cpp:1827-1883). */
 void or_init_primitive(or_solver *s, const double *c, const double *v,
                        const double *p);
+/* Turbulent fields for or_init_primitive (call it first): k and epsilon,
+ * clipped to the minimal values and multiplied by the density as the
+ * reference's initialisation does (phaseInitialization.cpp:1814-1848, 1873-1877). */
+void or_init_turbulence(or_solver *s, const double *k, const double *eps);
+/* Conserved turbulent fields rho k, rho epsilon for or_set_conserved (call
+ * it first). */
+void or_set_turbulence_conserved(or_solver *s, const double *rk, const double *reps);
 /* Sets the full reference state (as written by the reference's output). */
 void or_set_full(or_solver *s, const double *c0, const double *c,
                  const double *rho0, const double *rho, const double *v,

@@ -1,4 +1,4 @@
 """MI355X-native explicit finite-volume step of the Schemi solver."""
 from .enums import (boundaryConditionType, flowSolver, typeOfTVDLimiter, errors,
-                    gasModel, typeOfSolverEnum, dimensions, timestep)
-from .case import Case, BC
+                    gasModel, typeOfSolverEnum, dimensions, timestep, turbulenceModel)
+from .case import Case, BC, Turbulence

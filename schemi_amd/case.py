@@ -14,10 +14,10 @@ Cells are numbered as in the reference: ``id = i + Nx (j + Ny k)``, x fastest
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List, Sequence, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 from .enums import boundaryConditionType as BCT
-from .enums import flowSolver, gasModel, typeOfTVDLimiter
+from .enums import flowSolver, gasModel, turbulenceModel, typeOfTVDLimiter
 
 Vec3 = Tuple[float, float, float]
 
@@ -32,6 +32,34 @@ class BC:
 
 def _free6() -> List[BC]:
     return [BC() for _ in range(6)]
+
+
+@dataclass
+class Turbulence:
+    """k-epsilon turbulence (``#Turbulence_flag on``): the model of
+    ``#Type_of_source_flag`` (``shear``: shearModel.cpp; ``zero``:
+    zeroModel.cpp), the parameters of ``set/turbulentParameters.txt`` in the
+    order the k-epsilon models read them (kEpsModels.cpp, shearModel.cpp:35-59),
+    and the boundary conditions of ``set/k.txt`` and ``set/epsilon.txt``."""
+
+    model: turbulenceModel = turbulenceModel.shearModel
+    Cmu: float = 0.09
+    sigmaSc: float = 0.7      # Turbulent_Schmidt
+    sigmaT: float = 0.7       # Turbulent_Prandtl_T
+    sigmaE: float = 1e6       # Turbulent_Prandtl_e
+    sigmaK: float = 0.7       # sigma_k
+    sigmaEps: float = 0.6     # sigma_epsilon
+    CMS_R: float = 0.1
+    CMS_D: float = 1.0
+    mink: float = 1e-15       # Minimal_k
+    mineps: float = 1e-15     # Minimal_epsilon
+    C: Tuple[float, ...] = (1.44, 1.92, 2.0)  # shear: C1, C2, C3
+    bc_k: List[BC] = field(default_factory=_free6)
+    bc_eps: List[BC] = field(default_factory=_free6)
+
+    def params(self):
+        return (self.Cmu, self.sigmaSc, self.sigmaT, self.sigmaE, self.sigmaK,
+                self.sigmaEps, self.CMS_R, self.CMS_D, self.mink, self.mineps)
 
 
 @dataclass
@@ -87,6 +115,8 @@ class Case:
     # its two coefficients (source, diffusion)
     timestep_source: bool = False
     source_coeffs: tuple = (0.25, 0.5)
+    # k-epsilon turbulence (None: #Turbulence_flag off)
+    turbulence: Optional[Turbulence] = None
 
     def __post_init__(self):
         if not self.bc_c:
@@ -113,6 +143,12 @@ class Case:
     @property
     def Ns(self) -> int:
         return len(self.M)
+
+    @property
+    def nfields(self) -> int:
+        """Conserved fields per cell: rho_1..rho_Ns, rho, m (3), E, and with
+        turbulence rho k, rho epsilon."""
+        return self.Ns + 5 + (2 if self.turbulence is not None else 0)
 
     @property
     def ncells(self) -> int:

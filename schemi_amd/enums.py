@@ -104,6 +104,19 @@ class dimensions(IntEnum):
     task3D = 2
 
 
+class turbulenceModel(IntEnum):
+    zeroModel = 0
+    decayModel = 1
+    shearModel = 2
+    arithmeticA1Model = 3
+    arithmeticA2Model = 4
+    arithmeticA3Model = 5
+    kEpsAModel = 6
+    BHRModel = 7
+    BHRKLModel = 8
+    unknownModel = 9
+
+
 class timestep(IntEnum):
     CourantTimeStep = 0
     CourantAndSourceTimeStep = 1

@@ -28,6 +28,7 @@ from oracle import decks  # noqa: E402
 from oracle.oracle import Oracle  # noqa: E402
 from schemi_amd import Case, typeOfTVDLimiter as TL, flowSolver as FS, gasModel  # noqa: E402
 from schemi_amd import boundaryConditionType as BCT  # noqa: E402
+from schemi_amd import BC  # noqa: E402
 from schemi_amd.cases import cv_of  # noqa: E402
 
 KEYS = ("c0", "c", "rho0", "rho", "v", "p", "U", "T", "m", "E")
@@ -53,6 +54,12 @@ def case_json(case: Case) -> str:
             eos["molar_mass_correction"] = True
     if case.timestep_source:
         eos.update(timestep_source=True, source_coeffs=list(case.source_coeffs))
+    t = case.turbulence
+    if t is not None:
+        eos["turbulence"] = {
+            "model": int(t.model), "params": list(t.params()), "C": list(t.C),
+            "bc_k": [[int(b.type), list(b.value)] for b in t.bc_k],
+            "bc_eps": [[int(b.type), list(b.value)] for b in t.bc_eps]}
     return json.dumps({**eos,
         "dim": case.dim, "N": list(case.N), "L": list(case.L), "M": list(case.M),
         "Cv": list(case.Cv), "R": case.R, "flux": int(case.flux),
@@ -77,7 +84,15 @@ def seeded_primitives(case: Case, seed: int):
     return c, v, p
 
 
-def save(name, case, dts, snaps, times=None, restart=None):
+def seeded_turbulence(case: Case, seed: int):
+    """Seeded k and epsilon: eps / k between 2 and 60, turbulent viscosity
+    Cmu k^2 / eps up to ~1e-4."""
+    r = np.random.default_rng(seed)
+    n = case.ncells
+    return 1e-3 + 5e-3 * r.random(n), 1e-2 + 5e-2 * r.random(n)
+
+
+def save(name, case, dts, snaps, times=None, restart=None, restart_turb=None):
     out = {"case": np.array(case_json(case)), "dts": np.asarray(dts, dtype=np.float64),
            "steps": np.array(sorted(snaps), dtype=np.int64)}
     if times is not None:
@@ -85,20 +100,24 @@ def save(name, case, dts, snaps, times=None, restart=None):
     if restart is not None:
         # primitive fields given to the reference's restart reader
         out["init_c"], out["init_v"], out["init_p"] = (np.asarray(x) for x in restart)
+    if restart_turb is not None:
+        out["init_k"], out["init_eps"] = (np.asarray(x) for x in restart_turb)
+    keys = KEYS + (("k", "eps") if case.turbulence is not None else ())
     for s, st in snaps.items():
-        for k in KEYS:
+        for k in keys:
             out[f"s{s}_{k}"] = np.asarray(st[k], dtype=np.float64)
     path = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(path, **out)
     print(f"wrote {path} ({os.path.getsize(path) // 1024} KiB)")
 
 
-def run_const(name, case, nsteps, dt, keep, zones=None, restart=None):
+def run_const(name, case, nsteps, dt, keep, zones=None, restart=None, restart_turb=None):
     t = 0.0
     for _ in range(nsteps):
         t = t + dt
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as root:
-        decks.write_deck(root, case, t_end=t, zones=zones, restart=restart, const_dt=dt)
+        decks.write_deck(root, case, t_end=t, zones=zones, restart=restart, const_dt=dt,
+                         restart_turb=restart_turb)
         r = decks.run(root)
         if r.returncode != 0:
             raise RuntimeError(f"{name}: reference failed\n{r.stdout[-2000:]}\n{r.stderr}")
@@ -108,7 +127,7 @@ def run_const(name, case, nsteps, dt, keep, zones=None, restart=None):
             snaps[0] = decks.read_output(root, 0, case.Ns)
         for s in keep:
             snaps[s] = decks.read_output(root, s + off, case.Ns)
-    save(name, case, [dt] * nsteps, snaps, restart=restart)
+    save(name, case, [dt] * nsteps, snaps, restart=restart, restart_turb=restart_turb)
 
 
 def run_cfl_shock_tube(nsteps=200, keep=(1, 2, 10, 200)):
@@ -477,6 +496,92 @@ def courant_source_cases(base, nsteps=5, keep=(1, 3, 5)):
     save("seeded_1d_courant_source", c, dts, snaps, times=ref_times, restart=init)
 
 
+def turbulence_cases(base, nsteps=4, keep=(1, 4)):
+    """k-epsilon turbulence (#Turbulence_flag on): rho k and rho epsilon
+    through the HLLC/HLL flux with the turbulent pressure (2/3) rho k and the
+    sound-speed term (HLLCSolver.cpp:26-64), and, with the diffusion stage,
+    the turbulent transport coefficients, the k and epsilon Laplacians and the
+    shear model's sources (Diffusion.cpp:129-933, shearModel.cpp:77-153)."""
+    from schemi_amd.case import Turbulence
+    tr = dict(diffusion=True, transport=(2e-3, 2e-3, 2e-3))
+    vdw = dict(gas_model=gasModel.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+
+    def slip_y(c):
+        for p in (3, 4):
+            c.set_bc(p, BCT.slip)
+        return c
+
+    def go(name, c, seed, n=nsteps, kp=keep):
+        run_const(name, c, n, 2e-4 if c.dim > 1 else 1e-3, kp,
+                  restart=seeded_primitives(c, seed), restart_turb=seeded_turbulence(c, seed))
+
+    # advection only (turbulence on, diffusion off)
+    go("seeded_2d_turb_advection_hllc",
+       slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), limiter=TL.minmod,
+                   turbulence=Turbulence(), **base)), 201)
+    # shear model with the diffusion stage
+    go("seeded_2d_turb_shear_hllc",
+       slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), limiter=TL.minmod,
+                   turbulence=Turbulence(), **base, **tr)), 202)
+    # the configuration of the Kelvin-Helmholtz workload: HLL, minmod, van der
+    # Waals mixture, shear model
+    go("seeded_2d_turb_shear_vdw_hll",
+       slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), flux=FS.HLL, limiter=TL.minmod,
+                   turbulence=Turbulence(sigmaE=0.9), **base, **tr, **vdw)), 203)
+    # 1-D van Leer (limited slopes of k and epsilon), fixed-value and slip ends
+    c = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer,
+             turbulence=Turbulence(), **base, **tr)
+    c.set_bc(0, BCT.fixedValueCell, "all", (0.03, 0.0, 0.0))
+    c.set_bc(1, BCT.slip)
+    c.turbulence.bc_k[0] = BC(BCT.fixedValueCell, (2e-3, 0.0, 0.0))
+    c.turbulence.bc_eps[0] = BC(BCT.fixedValueCell, (3e-2, 0.0, 0.0))
+    c.turbulence.bc_eps[1] = BC(BCT.slip)
+    go("seeded_1d_turb_shear_vanleer", c, 204)
+    # 3-D van Albada with Lz != Ly (reversed top faces)
+    go("seeded_3d_turb_shear_vanalbada",
+       Case(dim=3, N=(8, 6, 5), L=(1.0, 0.8, 0.6), limiter=TL.vanAlbada,
+            turbulence=Turbulence(), **base, **tr), 205, n=3, kp=(1, 3))
+    # zero model: transport and diffusion of k and epsilon, no sources
+    from schemi_amd.enums import turbulenceModel
+    go("seeded_2d_turb_zero_hllc",
+       slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), limiter=TL.minmod,
+                   turbulence=Turbulence(model=turbulenceModel.zeroModel), **base, **tr)), 206)
+
+
+def turbulence_source_case(base, nsteps=6, keep=(1, 3, 6)):
+    """"Courant-Source" with the shear model's source step (shearModel.cpp:
+    107-150): epsilon over its growth rate caps the next Courant step.  The
+    oracle finds the step sequence; the reference must take the same steps."""
+    from schemi_amd.case import Turbulence
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer, courant=0.3,
+             min_time=1e-6, diffusion=True, transport=(2e-3, 2e-3, 2e-3),
+             timestep_source=True, source_coeffs=(0.02, 0.5),
+             turbulence=Turbulence(), **base)
+    init, tinit = seeded_primitives(c, 207), seeded_turbulence(c, 207)
+    o = Oracle(c)
+    o.init_primitive(*init, k=tinit[0], eps=tinit[1])
+    t, dt, dts, times = 0.0, c.min_time, [], []
+    for _ in range(nsteps):
+        assert o.step(dt, 0) == 0
+        dts.append(dt)
+        t = t + dt
+        times.append(t)
+        dt = o.courant_dt(c.courant, c.min_time)
+    with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as root:
+        decks.write_deck(root, c, t_end=t, restart=init, restart_turb=tinit)
+        r = decks.run(root)
+        if r.returncode != 0:
+            raise RuntimeError(f"reference failed\n{r.stdout[-2000:]}\n{r.stderr}")
+        tt = decks.read_times(root)
+        if tt.shape[0] != nsteps + 2:
+            raise RuntimeError(f"reference took {tt.shape[0] - 2} steps, expected {nsteps}")
+        snaps = {s: decks.read_output(root, s + 1, c.Ns) for s in keep}
+        ref_times = tt[2:, 1]
+    print("dts", dts, "reference times", ref_times)
+    save("seeded_1d_turb_courant_source", c, dts, snaps, times=ref_times, restart=init,
+         restart_turb=tinit)
+
+
 if __name__ == "__main__":
     b = dict(M=[29.0, 4.0], Cv=[2.5, 1 / 0.67])
     if len(sys.argv) > 1 and sys.argv[1] == "lz":
@@ -503,5 +608,8 @@ if __name__ == "__main__":
         vdw_diffusion_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "source":
         courant_source_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "turbulence":
+        turbulence_cases(b)
+        turbulence_source_case(b)
     else:
         main()

@@ -36,14 +36,30 @@ def load(name):
                 enthalpy_flow=bool(d.get("enthalpy_flow", False)),
                 molar_mass_correction=bool(d.get("molar_mass_correction", False)),
                 timestep_source=bool(d.get("timestep_source", False)),
-                source_coeffs=tuple(d.get("source_coeffs", (0.25, 0.5))))
-    snaps = {int(s): {k: z[f"s{s}_{k}"] for k in KEYS} for s in z["steps"]}
+                source_coeffs=tuple(d.get("source_coeffs", (0.25, 0.5))),
+                turbulence=_turbulence(d.get("turbulence")))
+    keys = KEYS + (("k", "eps") if case.turbulence is not None else ())
+    snaps = {int(s): {k: z[f"s{s}_{k}"] for k in keys} for s in z["steps"]}
     out = {"case": case, "dts": z["dts"], "snaps": snaps}
     if "times" in z:
         out["times"] = z["times"]
     if "init_c" in z:
         out["restart"] = (z["init_c"], z["init_v"], z["init_p"])
+    if "init_k" in z:
+        out["restart_turb"] = (z["init_k"], z["init_eps"])
     return out
+
+
+def _turbulence(t):
+    if t is None:
+        return None
+    from schemi_amd.case import Turbulence
+    p = t["params"]
+    return Turbulence(model=t["model"], Cmu=p[0], sigmaSc=p[1], sigmaT=p[2], sigmaE=p[3],
+                      sigmaK=p[4], sigmaEps=p[5], CMS_R=p[6], CMS_D=p[7], mink=p[8],
+                      mineps=p[9], C=tuple(t["C"]),
+                      bc_k=[BC(ty, tuple(v)) for ty, v in t["bc_k"]],
+                      bc_eps=[BC(ty, tuple(v)) for ty, v in t["bc_eps"]])
 
 
 def conserved(st):

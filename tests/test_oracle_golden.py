@@ -14,15 +14,25 @@ from oracle.oracle import Oracle
 from tests.golden_io import KEYS, load, names
 
 
+def start(g):
+    """Oracle at the fixture's initial state (restart fields, with k and
+    epsilon when the case is turbulent, or the reference's output 0)."""
+    o = Oracle(g["case"])
+    if "restart" in g:
+        kw = {}
+        if "restart_turb" in g:
+            kw = dict(k=g["restart_turb"][0], eps=g["restart_turb"][1])
+        o.init_primitive(*g["restart"], **kw)
+    else:
+        o.set_full(g["snaps"][0])
+    return o
+
+
 @pytest.mark.parametrize("name", names())
 def test_oracle_matches_reference(name):
     g = load(name)
     case = g["case"]
-    o = Oracle(case)
-    if "restart" in g:
-        o.init_primitive(*g["restart"])
-    else:
-        o.set_full(g["snaps"][0])
+    o = start(g)
     last = max(g["snaps"])
     t = 0.0
     for s, dt in enumerate(g["dts"], start=1):
@@ -32,24 +42,21 @@ def test_oracle_matches_reference(name):
             assert t == g["times"][s - 1]
         if s in g["snaps"]:
             st = o.state()
-            for k in KEYS:
+            for k in g["snaps"][s]:
                 assert np.array_equal(st[k], g["snaps"][s][k]), f"{name} step {s}: {k}"
         if s == last:
             break
 
 
-@pytest.mark.parametrize("name", ["shock_tube_1d_cfl", "seeded_1d_courant_source"])
+@pytest.mark.parametrize("name", ["shock_tube_1d_cfl", "seeded_1d_courant_source",
+                                  "seeded_1d_turb_courant_source"])
 def test_courant_sequence_matches_reference(name):
     """The oracle's own step sequence reproduces the times the reference
-    stepped through (Courant; Courant-Source with the enthalpy-flow source
-    step capping it)."""
+    stepped through (Courant; Courant-Source with the enthalpy-flow or the
+    turbulence model's source step capping it)."""
     g = load(name)
     case = g["case"]
-    o = Oracle(case)
-    if "restart" in g:
-        o.init_primitive(*g["restart"])
-    else:
-        o.set_full(g["snaps"][0])
+    o = start(g)
     dt, t = case.min_time, 0.0
     for s in range(len(g["dts"])):
         assert dt == g["dts"][s]

@@ -1,0 +1,144 @@
+"""The oracle's k-epsilon turbulence beyond the reference fixtures.
+
+The fixtures (tests/golden/seeded_*_turb_*) pin the oracle to the reference
+bit for bit; these tests show that each turbulent term is exercised by them
+(a run without it leaves the fixture) and check what the mathematics fixes:
+on a uniform state at rest the shear model's diffusion stage is one explicit
+Euler step of dk/dt = -eps, deps/dt = -C2 eps^2 / k with the dissipated
+energy moved into the internal energy, and with negligible k the turbulent
+run is the laminar one.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from schemi_amd import BC, Case, Turbulence
+from schemi_amd import boundaryConditionType as BCT
+from schemi_amd import turbulenceModel as TM
+from schemi_amd import typeOfTVDLimiter as TL
+from tests.golden_io import load
+
+
+def run(g, case=None, steps=None):
+    case = case or g["case"]
+    o = Oracle(case)
+    kw = dict(k=g["restart_turb"][0], eps=g["restart_turb"][1]) if case.turbulence else {}
+    o.init_primitive(*g["restart"], **kw)
+    last = max(g["snaps"]) if steps is None else steps
+    for dt in g["dts"][:last]:
+        assert o.step(dt, 0) == 0
+    return o.state(), g["snaps"][last]
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def replace_turb(case, **kw):
+    return dataclasses.replace(case, turbulence=dataclasses.replace(case.turbulence, **kw))
+
+
+@pytest.mark.parametrize("name,change,field,least", [
+    # the turbulent pressure and the k term of the sound speed in the flux
+    ("seeded_2d_turb_advection_hllc", dict(turbulence=None), "E", 1e-4),
+    # shear production and dissipation sources of k and epsilon
+    ("seeded_2d_turb_shear_hllc", dict(model=TM.zeroModel), "k", 1e-3),
+    ("seeded_2d_turb_shear_hllc", dict(C=(1.0, 1.92, 2.0)), "eps", 1e-5),
+    ("seeded_2d_turb_shear_hllc", dict(C=(1.44, 1.5, 2.0)), "eps", 1e-4),
+    # thetaS of the deviatoric stress (cells) and of the k, eps diffusion
+    # coefficients (faces)
+    ("seeded_2d_turb_shear_hllc", dict(CMS_R=10.0), "k", 1e-6),
+    ("seeded_2d_turb_shear_hllc", dict(CMS_D=10.0), "k", 1e-9),
+    # turbulent diffusion of k and epsilon, of the species, of heat
+    ("seeded_2d_turb_zero_hllc", dict(sigmaK=1e30), "k", 1e-6),
+    ("seeded_2d_turb_zero_hllc", dict(sigmaSc=1e30), "rho", 1e-9),
+    ("seeded_2d_turb_zero_hllc", dict(sigmaT=1e30), "T", 1e-9),
+    # the non-ideality correction Laplacian (turbulent Prandtl number of the
+    # energy, van der Waals mixture)
+    ("seeded_2d_turb_shear_vdw_hll", dict(sigmaE=1e30), "T", 1e-9),
+    # 1-D limited slopes of k and epsilon, fixed-value and slip ends
+    ("seeded_1d_turb_shear_vanleer", dict(Cmu=0.05), "k", 1e-6),
+    ("seeded_3d_turb_shear_vanalbada", dict(model=TM.zeroModel), "eps", 1e-3),
+])
+def test_fixture_exercises_term(name, change, field, least):
+    g = load(name)
+    if "turbulence" in change:
+        case = dataclasses.replace(g["case"], **change)
+    else:
+        case = replace_turb(g["case"], **change)
+    got, ref = run(g, case)
+    if field in ("k", "eps") and case.turbulence is None:
+        field = "E"
+    assert rel(got[field], ref[field]) > least
+
+
+def test_fixed_value_k_boundary_exercised():
+    g = load("seeded_1d_turb_shear_vanleer")
+    case = g["case"]
+    bk = list(case.turbulence.bc_k)
+    bk[0] = BC(BCT.fixedValueCell, (4e-3, 0.0, 0.0))
+    got, ref = run(g, replace_turb(case, bc_k=bk))
+    assert rel(got["k"], ref["k"]) > 1e-6
+
+
+def uniform_case(model=TM.shearModel, dim=2):
+    case = Case(dim=dim, N=(6, 5, 4), L=(1.0, 1.0, 1.0), M=[29.0, 4.0], Cv=[2.5, 1 / 0.67],
+                limiter=TL.minmod, diffusion=True, transport=(1e-3, 1e-3, 1e-3),
+                turbulence=Turbulence(model=model))
+    for p in range(6):
+        case.set_bc(p, BCT.periodic)
+        case.turbulence.bc_k[p] = BC(BCT.periodic)
+        case.turbulence.bc_eps[p] = BC(BCT.periodic)
+    return case
+
+
+def test_uniform_decay_is_one_euler_step():
+    case = uniform_case()
+    n = case.ncells
+    o = Oracle(case)
+    k0, e0 = 2e-3, 5e-2
+    o.init_primitive(np.full((2, n), 0.02), np.zeros((3, n)), np.full(n, 1.0),
+                     k=np.full(n, k0), eps=np.full(n, e0))
+    st0 = o.state()
+    dt = 1e-3
+    assert o.step(dt, 0) == 0
+    st = o.state()
+    C2 = case.turbulence.C[1]
+    np.testing.assert_allclose(st["k"], k0 - e0 * dt, rtol=1e-14)
+    np.testing.assert_allclose(st["eps"], e0 - C2 * e0 * e0 / k0 * dt, rtol=1e-14)
+    # the dissipated turbulent energy heats the gas; total energy is kept
+    rho = st0["rho0"]
+    # (a difference of two O(1) numbers: 1e-6 relative to a 3e-5 change)
+    np.testing.assert_allclose(st["U"] - st0["U"], rho * e0 * dt, rtol=1e-6)
+    np.testing.assert_allclose(st["E"], st0["E"], rtol=1e-13)
+
+
+def test_negligible_k_is_laminar():
+    g = load("seeded_2d_turb_shear_hllc")
+    lam = dataclasses.replace(g["case"], turbulence=None)
+    tur = replace_turb(g["case"], mink=1e-30, mineps=1e-30)
+    n = lam.ncells
+    a, b = Oracle(lam), Oracle(tur)
+    a.init_primitive(*g["restart"])
+    b.init_primitive(*g["restart"], k=np.full(n, 1e-30), eps=np.full(n, 1e-30))
+    for dt in g["dts"]:
+        assert a.step(dt, 0) == 0 and b.step(dt, 0) == 0
+    sa, sb = a.state(), b.state()
+    for key in ("rho", "m", "E", "T", "p"):
+        np.testing.assert_allclose(sb[key], sa[key], rtol=1e-13, atol=1e-15 * np.abs(sa[key]).max())
+
+
+def test_unsupported_turbulence_configurations():
+    base = uniform_case()
+    for bad in (dict(turbulence=Turbulence(model=TM.kEpsAModel)),
+                dict(sle_solver="Jacobi"), dict(molar_mass_correction=True),
+                dict(gas_model=2)):
+        with pytest.raises(ValueError):
+            Oracle(dataclasses.replace(base, **bad))
+    o = Oracle(dataclasses.replace(base, step_solver=0))
+    n = base.ncells
+    o.init_primitive(np.full((2, n), 0.02), np.zeros((3, n)), np.full(n, 1.0),
+                     k=np.full(n, 1e-3), eps=np.full(n, 1e-2))
+    assert o.step(1e-3, 0) == 11

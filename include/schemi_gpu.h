@@ -202,7 +202,46 @@ typedef struct {
      * (SCHEMI_ERR_INIT for values other than 0 and 1). */
     int32_t timestep_source;
     double source_coeff;
+    /* k-epsilon turbulence (#Turbulence_flag on, #Type_of_source_flag; the
+     * models of abstractTurbulenceModel::createTurbulenceModel,
+     * Schemi/src/interfaces/abstractTurbulenceModel.cpp:235-320): the state
+     * carries rho k and rho epsilon after E (Ns + 7 fields, see
+     * schemi_gpu_set_state); the flux carries the turbulent pressure
+     * (2/3) rho k (HLLCSolver.cpp, HLLSolver.cpp) and, with the diffusion
+     * stage, the turbulent transport coefficients, the k and epsilon
+     * Laplacians and the model's sources apply (Diffusion.cpp:129-933).
+     * Models: SCHEMI_TURB_SHEAR (shearModel.cpp, its source step capping a
+     * "Courant-Source" step) and SCHEMI_TURB_ZERO (zeroModel.cpp).  The
+     * parameters are those of set/turbulentParameters.txt in the models'
+     * meaning (kEpsModels.cpp, shearModel.cpp:35-59): Cmu, turbulent Schmidt
+     * and Prandtl numbers (T, e), sigma_k, sigma_epsilon, CMS_R, CMS_D, the
+     * minimal k and epsilon, and the shear model's C1, C2, C3; bc_k, bc_eps
+     * the boundary conditions of set/k.txt and set/epsilon.txt.  Supported
+     * with SecondOrderRK, the ideal or van der Waals mixture, and limiters
+     * whose slopes vanish (2-D/3-D TVD limiters, the zero limiter) or 1-D
+     * meshes with a TVD limiter; the diffusion stage (if on) with the
+     * explicit solver and neither the molar-mass correction nor the enthalpy
+     * flow.  Otherwise SCHEMI_ERR_INIT. */
+    int32_t turbulence;
+    int32_t turbulence_model;
+    double turb_Cmu;
+    double turb_sigma_sc;
+    double turb_sigma_t;
+    double turb_sigma_e;
+    double turb_sigma_k;
+    double turb_sigma_eps;
+    double turb_CMS_R;
+    double turb_CMS_D;
+    double turb_min_k;
+    double turb_min_eps;
+    double turb_C[3];
+    schemi_bc bc_k[6];
+    schemi_bc bc_eps[6];
 } schemi_scheme_desc;
+
+/* turbulenceModel (Schemi/enumerations/turbulenceModelEnum.hpp) values */
+#define SCHEMI_TURB_ZERO 0
+#define SCHEMI_TURB_SHEAR 2
 
 #define SCHEMI_SLE_GAUSS_SEIDEL 0
 #define SCHEMI_SLE_EXPLICIT 3
@@ -218,12 +257,15 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
 
 void schemi_gpu_destroy(schemi_gpu_ctx *ctx);
 
-/* Uploads the conserved state (Ns + 5 fields x ncells doubles). */
+/* Uploads the conserved state (Ns + 5 fields x ncells doubles; with
+ * turbulence Ns + 7: rho k and rho epsilon after E). */
 int schemi_gpu_set_state(schemi_gpu_ctx *ctx, const double *cons,
                          int64_t ncells, int32_t numbering);
 
-/* Downloads the conserved state and, when prim is not NULL, the primitives
- * recovered as in Advection2dOrder.cpp:578-602 (Ns + 8 fields). */
+/* Downloads the conserved state (the fields of schemi_gpu_set_state) and,
+ * when prim is not NULL, the primitives recovered as in
+ * Advection2dOrder.cpp:578-602 (Ns + 8 fields; with turbulence the internal
+ * energy is E - rho v^2 / 2 - rho k). */
 int schemi_gpu_get_state(schemi_gpu_ctx *ctx, double *cons, double *prim,
                          int32_t numbering);
 
@@ -299,7 +341,8 @@ int schemi_gpu_adjoint_step(schemi_gpu_ctx *ctx, double dt, const double *lambda
 /* Mixing width J = sum_i V_i Y_i (1 - Y_i), Y = rho_species / rho, of the
  * current state (no counterpart in the reference, whose mixedZoneWidth1D,
  * output.cpp:310-391, is a threshold width); dJ/dU into dJdU (device, state
- * layout, nullable).  Deterministic; synchronous. */
+ * layout, nullable; the Ns + 5 rows up to E are written, the turbulent rows
+ * of a turbulent state are not).  Deterministic; synchronous. */
 int schemi_gpu_mixing_width(schemi_gpu_ctx *ctx, int32_t species, double *J, double *dJdU);
 
 /* No-allocation variant of schemi_gpu_mixing_width.  No counterpart in the

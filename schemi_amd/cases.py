@@ -17,7 +17,7 @@ import math
 
 import numpy as np
 
-from .case import Case
+from .case import BC, Case
 from .enums import boundaryConditionType as BCT
 from .enums import flowSolver, gasModel, typeOfTVDLimiter
 from .gpu import conserved_from_primitive
@@ -226,3 +226,68 @@ def kelvin_helmholtz_2d(n: int = 512, seed: int = 11, T0: float = 300.0, p0: flo
     c = np.stack([X[0] * c0, X[1] * c0])
     v = np.stack([u, vv, np.zeros_like(u)])
     return case, conserved_from_primitive(case, c, v, p)
+
+
+def _vdw_concentration(case: Case, X, p, T):
+    """Total molar concentration c of the van der Waals mixture with mole
+    fractions X at pressure p and temperature T: the gas root of
+    c R T / (1 - c b) - a c^2 = p (vanDerWaalsFluid.cpp:14-22 with U from T)
+    with a, b mixed by the mole fractions (mixtureVanDerWaals.cpp:104-109),
+    by Newton's method from the ideal-gas value."""
+    from .gpu import vdw_matrices
+
+    am, bm = vdw_matrices(case)
+    a = sum(X[k] * X[l] * am[k, l] for k in range(case.Ns) for l in range(case.Ns))
+    b = sum(X[k] * X[l] * bm[k, l] for k in range(case.Ns) for l in range(case.Ns))
+    R = float(case.R)
+    c = p / (R * T)
+    for _ in range(100):
+        f = c * R * T / (1 - c * b) - a * c * c - p
+        df = R * T / (1 - c * b) ** 2 - 2 * a * c
+        dc = f / df
+        c = c - dc
+        if np.all(np.abs(dc) <= 1e-15 * np.abs(c)):
+            break
+    return c
+
+
+def kelvin_helmholtz_turbulent_2d(n: int = 512, seed: int = 11, T0: float = 300.0,
+                                  p0: float = 5.0e6):
+    """The 2-D Kelvin-Helmholtz shear layer with k-epsilon turbulence: the
+    layer, box, species and perturbation of kelvin_helmholtz_2d through the
+    van der Waals cubic mixture (a, b from Tc, Pc), the diffusion stage with
+    the molecular coefficients of CO2/N2 near 5 MPa (viscosity 1.8e-5 Pa s,
+    diffusion 3e-7 m^2/s, heat conduction 0.03 W/(m K); explicit solver) and
+    the shear k-epsilon model with the reference decks' parameters
+    (Cmu 0.09, turbulent Schmidt / Prandtl 0.7, sigma_k 0.7, sigma_eps 0.6,
+    C1 1.44, C2 1.92, C3 2.0); k = 1.5e-4 m^2/s^2 (2 % intensity of the
+    0.5 m/s stream) in a Gaussian of width 0.02 around the layer over a 1e-8
+    background, eps = Cmu^(3/4) k^(3/2) / 0.01 m; HLL + minmod, periodic box."""
+    from .case import Turbulence
+
+    case = Case(dim=2, N=(n, n, 1), L=(1.0, 1.0, 1.0 / n), M=[0.044, 0.028],
+                Cv=[28.5, 20.8], R=8.314462618, flux=flowSolver.HLL,
+                limiter=typeOfTVDLimiter.minmod, courant=0.4, min_time=1e-15,
+                gas_model=gasModel.vanDerWaals, Tc=[304.13, 126.19], Pc=[7.3773e6, 3.3958e6],
+                diffusion=True, transport=(1.8e-5, 3e-7, 0.03), turbulence=Turbulence())
+    for pt in (0, 1, 3, 4):
+        case.set_bc(pt, BCT.periodic)
+        case.turbulence.bc_k[pt] = BC(BCT.periodic)
+        case.turbulence.bc_eps[pt] = BC(BCT.periodic)
+    rng = SplitMix64(seed)
+    x, y, _ = _cell_centres(case)
+    u = 0.5 * np.tanh((y - 0.5) / 0.01)
+    vv = np.zeros_like(x)
+    for k in range(1, 5):
+        amp, ph = 1e-3 * rng.uniform(), 2.0 * math.pi * rng.uniform()
+        vv += amp * np.sin(2.0 * math.pi * k * x + ph) * np.exp(-((y - 0.5) / 0.05) ** 2)
+    Y1 = 0.5 * (1.0 - np.tanh((y - 0.5) / 0.01))
+    n1, n2 = Y1 / case.M[0], (1.0 - Y1) / case.M[1]
+    X = [n1 / (n1 + n2), n2 / (n1 + n2)]
+    p = np.full_like(x, p0)
+    c0 = _vdw_concentration(case, X, p, np.full_like(x, T0))
+    c = np.stack([X[0] * c0, X[1] * c0])
+    v = np.stack([u, vv, np.zeros_like(u)])
+    kt = 1.5e-4 * np.exp(-((y - 0.5) / 0.02) ** 2) + 1e-8
+    et = 0.09 ** 0.75 * kt ** 1.5 / 0.01
+    return case, conserved_from_primitive(case, c, v, p, kt, et)

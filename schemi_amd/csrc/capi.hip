@@ -18,6 +18,8 @@
 
 #include "../../include/schemi_gpu.h"
 #include "fv_launch.hpp"
+#include "fv_turb.hpp"
+#include "fv_turb_launch.hpp"
 #include "geometry.hpp"
 
 using namespace sg;
@@ -32,7 +34,8 @@ namespace {
 // device scalar block
 
 __global__ void finalize_step_kernel(double *sc, const DevParams *P, double const_dt,
-                                     int flip_parity)
+                                     int flip_parity, int src_scaled,
+                                     unsigned long long *epsmax)
 {
     // Schemi.cpp:802-868, 933-935 for one step
     if (sc[SC_DONE] != 0.0)
@@ -43,9 +46,11 @@ __global__ void finalize_step_kernel(double *sc, const DevParams *P, double cons
     double dtn;
     unsigned long long *srcmin = reinterpret_cast<unsigned long long *>(sc) + SC_SRCMIN;
     // the source step of this step's diffusion stage, veryBig when none set it
-    const double dts = (*srcmin == SRCMIN_NONE)
-                           ? 1e-3 * __DBL_MAX__
-                           : P->ts_coeff * __longlong_as_double(static_cast<long long>(*srcmin));
+    // (the turbulence model's ratios carry the coefficient already,
+    // shearModel.cpp:131-138)
+    const double srcv = __longlong_as_double(static_cast<long long>(*srcmin));
+    const double dts = (*srcmin == SRCMIN_NONE) ? 1e-3 * __DBL_MAX__
+                                                : (src_scaled ? srcv : P->ts_coeff * srcv);
     if (const_dt > 0.0) {
         dtn = const_dt;
     } else {
@@ -66,6 +71,8 @@ __global__ void finalize_step_kernel(double *sc, const DevParams *P, double cons
     if (flip_parity)
         sc[SC_PARITY] = 1.0 - sc[SC_PARITY];
     reinterpret_cast<unsigned long long *>(sc)[SC_SIGMAX] = 0ull;
+    if (epsmax)
+        *epsmax = 0ull;
     if (!(t < sc[SC_TEND]))
         sc[SC_DONE] = 1.0;
 }
@@ -96,7 +103,7 @@ __global__ void restore_state_kernel(const double *sc, double *U0, const double 
         U0[i] = U1[i];
 }
 
-template <int NS>
+template <int NS, bool TURB = false>
 __global__ void prims_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ U,
                              double *__restrict__ W, long long n)
 {
@@ -104,10 +111,14 @@ __global__ void prims_kernel(const DevParams *__restrict__ Pp, const double *__r
     const long long id = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (id >= n)
         return;
-    double u[NS + 5], w[NS + 4];
-    for (int q = 0; q < NS + 5; ++q)
+    constexpr int NF = TURB ? NS + 7 : NS + 5;
+    double u[NF], w[NS + 6];
+    for (int q = 0; q < NF; ++q)
         u[q] = U[q * n + id];
-    prims_from_cons<double, NS>(P, u, w);
+    if constexpr (TURB)
+        prims_turb<double, NS, -1>(P, u, w);
+    else
+        prims_from_cons<double, NS>(P, u, w);
     double c0 = 0;
     for (int k = 0; k < NS; ++k) {
         W[k * n + id] = w[k];
@@ -118,7 +129,7 @@ __global__ void prims_kernel(const DevParams *__restrict__ Pp, const double *__r
         W[(NS + 1 + b) * n + id] = w[NS + b];
     const double vx = w[NS], vy = w[NS + 1], vz = w[NS + 2], p = w[NS + 3];
     const double v2 = ((0. + vx * vx) + vy * vy) + vz * vz;
-    const double Uint = (u[NS + 4] - u[NS] * v2 * 0.5) - 0.;
+    const double Uint = (u[NS + 4] - u[NS] * v2 * 0.5) - (TURB ? u[NS + 5] : 0.);
     double cv = 0;
     for (int k = 0; k < NS; ++k)
         cv += (w[k] / c0) * P.Cv[k];
@@ -210,6 +221,11 @@ struct schemi_gpu_ctx {
     std::vector<hipEvent_t> evd; // timed diffusion stages: start, between passes, end
     int sle_iters = 0;           // Jacobi solver: iteration limit (0: explicit solver)
     int molmass = 0, enthalpy = 0;
+    int nf = 0;                  // conserved fields: ns + 5, with turbulence ns + 7
+    bool turb = false;           // k-epsilon turbulence
+    TurbParams tp{};
+    TurbParams *dTP = nullptr;
+    unsigned long long *epsmax = nullptr;  // largest epsilon of the state (bits)
     double *Mb = nullptr, *dmax = nullptr;
     int *done = nullptr;
     int done_host[16] = {};
@@ -229,9 +245,49 @@ int hip_fail(hipError_t e)
             return SCHEMI_ERR_SYSTEM;            \
     } while (0)
 
+int launch_stage_turb(schemi_gpu_ctx *c, const double *Uin, double *Uout, const double *Un,
+                      int variant)
+{
+    TurbStageLaunch L;
+    L.P = c->dP;
+    L.TP = c->dTP;
+    L.geo = c->dgeo;
+    L.Uin = Uin;
+    L.Uout = Uout;
+    L.Un = Un;
+    L.dt = c->sc;
+    L.sigmax = reinterpret_cast<unsigned long long *>(c->sc + SC_SIGMAX);
+    L.err = c->derr;
+    L.dim = c->dim;
+    L.ns = c->ns;
+    L.variant = variant;
+    L.gas_model = c->hp.gas_model;
+    L.slopes = c->mode == MODE_SLOPES ? 1 : 0;
+    L.n = c->n;
+    L.stream = c->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, c->stream));
+    }
+    const int r = c->prec == SCHEMI_PREC_MIXED ? launch_turb_stage_mixed(L)
+                                                : launch_turb_stage_f64(L);
+    if (c->timing) {
+        HIPCHK(hipEventRecord(e1, c->stream));
+        c->ev.push_back(e0);
+        c->ev.push_back(e1);
+    }
+    if (r == -1)
+        return SCHEMI_ERR_INIT;
+    return r == 0 ? SCHEMI_OK : SCHEMI_ERR_SYSTEM;
+}
+
 int launch_stage(schemi_gpu_ctx *c, const double *Uin, double *Uout, const double *Un,
                  int variant)
 {
+    if (c->turb)
+        return launch_stage_turb(c, Uin, Uout, Un, variant);
     StageLaunch L;
     L.P = c->dP;
     L.geo = c->dgeo;
@@ -333,15 +389,22 @@ int one_step(schemi_gpu_ctx *c)
             HIPCHK(hipEventRecord(e[0], c->stream));
             L.mid = e[1];
         }
-        if (launch_diffusion(L))
+        if (c->turb) {
+            TurbDiffusion T;
+            T.TP = c->dTP;
+            T.epsmax = c->epsmax;
+            if (launch_diffusion_turb(L, T))
+                return SCHEMI_ERR_SYSTEM;
+        } else if (launch_diffusion(L)) {
             return SCHEMI_ERR_SYSTEM;
+        }
         if (c->timing) {
             HIPCHK(hipEventRecord(e[2], c->stream));
             c->evd.insert(c->evd.end(), e, e + 3);
         }
     }
     hipLaunchKernelGGL(finalize_step_kernel, dim3(1), dim3(1), 0, c->stream, c->sc, c->dP,
-                       c->const_dt, c->hancock ? 1 : 0);
+                       c->const_dt, c->hancock ? 1 : 0, c->turb ? 1 : 0, c->epsmax);
     return hip_fail(hipGetLastError());
 }
 
@@ -552,6 +615,22 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     if ((scheme->timestep_source != 0 && scheme->timestep_source != 1) ||
         !(scheme->source_coeff >= 0.0))
         return SCHEMI_ERR_INIT;
+    // turbulence: the shear and zero k-epsilon models with SSP-RK2, ideal or
+    // van der Waals mixture, slopes that vanish or 1-D TVD slopes, the
+    // explicit diffusion stage without the species corrections
+    if (scheme->turbulence != 0 && scheme->turbulence != 1)
+        return SCHEMI_ERR_INIT;
+    if (scheme->turbulence &&
+        ((scheme->turbulence_model != SCHEMI_TURB_SHEAR &&
+          scheme->turbulence_model != SCHEMI_TURB_ZERO) ||
+         scheme->step_solver != SCHEMI_STEP_SECOND_ORDER_RK ||
+         (mix->gas_model != SCHEMI_GAS_IDEAL && mix->gas_model != SCHEMI_GAS_VAN_DER_WAALS) ||
+         lim == SCHEMI_LIM_LINEAR ||
+         (scheme->diffusion && (scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT ||
+                                scheme->diffusion_molar_mass || scheme->diffusion_enthalpy_flow)) ||
+         !(scheme->turb_Cmu > 0.0) || !(scheme->turb_min_k > 0.0) ||
+         !(scheme->turb_min_eps > 0.0)))
+        return SCHEMI_ERR_INIT;
 
     schemi_gpu_ctx *c = new (std::nothrow) schemi_gpu_ctx();
     if (!c)
@@ -567,6 +646,32 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     c->D = scheme->transport_D;
     c->kappa = scheme->transport_kappa;
     c->molmass = (c->diffusion && scheme->diffusion_molar_mass) ? 1 : 0;
+    c->turb = scheme->turbulence != 0;
+    c->nf = mix->ns + 5 + (c->turb ? 2 : 0);
+    if (c->turb) {
+        TurbParams &T = c->tp;
+        T.model = scheme->turbulence_model;
+        T.Cmu = scheme->turb_Cmu;
+        T.sigmaSc = scheme->turb_sigma_sc;
+        T.sigmaT = scheme->turb_sigma_t;
+        T.sigmaE = scheme->turb_sigma_e;
+        T.sigmaK = scheme->turb_sigma_k;
+        T.sigmaEps = scheme->turb_sigma_eps;
+        T.CMS_R = scheme->turb_CMS_R;
+        T.CMS_D = scheme->turb_CMS_D;
+        T.mink = scheme->turb_min_k;
+        T.mineps = scheme->turb_min_eps;
+        T.C1 = scheme->turb_C[0];
+        T.C2 = scheme->turb_C[1];
+        T.C3 = scheme->turb_C[2];
+        T.ts_coeff = scheme->source_coeff;
+        for (int p = 0; p < 6; ++p) {
+            T.bct[p][0] = scheme->bc_k[p].type;
+            T.bcv[p][0] = scheme->bc_k[p].value[0];
+            T.bct[p][1] = scheme->bc_eps[p].type;
+            T.bcv[p][1] = scheme->bc_eps[p].value[0];
+        }
+    }
     c->enthalpy = (c->diffusion && scheme->diffusion_enthalpy_flow) ? 1 : 0;
     // Jacobi sweeps serve the Jacobi and the Gauss-Seidel solvers
     c->sle_iters = (c->diffusion && scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT)
@@ -713,6 +818,19 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
                 ++nfield;
             }
         }
+        if (c->turb)
+            for (int f = 0; f < 2; ++f)
+                for (int p : {plo, phi}) {
+                    const int t = c->tp.bct[p][f];
+                    if (!bc_ok(t)) {
+                        delete c;
+                        return SCHEMI_ERR_BOUNDARY;
+                    }
+                    if (t == SCHEMI_BC_FIXED_SURFACE)
+                        c->pending_error = SCHEMI_ERR_BOUNDARY;
+                    nper += (t == SCHEMI_BC_PERIODIC);
+                    ++nfield;
+                }
         if (nper != 0 && nper != nfield) {
             delete c;
             return SCHEMI_ERR_BOUNDARY;
@@ -765,7 +883,7 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
                     const int t = P.bct[patch_of(a, side)][q];
                     walls = walls && (t == BC_FREE || t == BC_SLIP);
                 }
-        if (c->mode == MODE_FIRST_ORDER && c->prec == SCHEMI_PREC_MIXED &&
+        if (c->mode == MODE_FIRST_ORDER && c->prec == SCHEMI_PREC_MIXED && !c->turb &&
             mix->gas_model == SCHEMI_GAS_IDEAL && !c->hancock && dim >= 2 && walls &&
             P.N[0] % 64 == 0 && P.N[1] % SG_LEAN_TY == 0 && !(lean && lean[0] == '0'))
             c->mode = MODE_LEAN;
@@ -780,7 +898,7 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
                                       : (fo ? TileShape<2, true>::TX : TileShape<2>::TX),
                              fo ? TileShape<2, true>::TY : TileShape<2>::TY};
         for (int a = 0; a < 2 && a < dim; ++a)
-            if (c->mode != MODE_GENERAL && periodic[a] && (P.N[a] % tile[a]) != 0) {
+            if (c->mode != MODE_GENERAL && !c->turb && periodic[a] && (P.N[a] % tile[a]) != 0) {
                 delete c;
                 return SCHEMI_ERR_INIT;
             }
@@ -826,7 +944,7 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         delete c;
         return SCHEMI_ERR_SYSTEM;
     }
-    const size_t state_bytes = sizeof(double) * (size_t)(mix->ns + 5) * (size_t)c->n;
+    const size_t state_bytes = sizeof(double) * (size_t)c->nf * (size_t)c->n;
     bool ok = hipMalloc(&c->dP, sizeof(DevParams)) == hipSuccess &&
               hipMalloc(&c->dgeo, geo.size() * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->U[0], state_bytes) == hipSuccess &&
@@ -834,8 +952,14 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
               hipMalloc(&c->sc, SC_N * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->derr, sizeof(int)) == hipSuccess;
     if (ok && c->diffusion)
-        ok = hipMalloc(&c->Wd, sizeof(double) * (size_t)(2 * mix->ns + 5) * (size_t)c->n) ==
-             hipSuccess;
+        ok = hipMalloc(&c->Wd, sizeof(double) * (size_t)(2 * mix->ns + (c->turb ? 9 : 5)) *
+                                   (size_t)c->n) == hipSuccess;
+    if (ok && c->turb)
+        ok = hipMalloc(&c->dTP, sizeof(TurbParams)) == hipSuccess &&
+             hipMalloc(&c->epsmax, sizeof(unsigned long long)) == hipSuccess &&
+             hipMemcpyAsync(c->dTP, &c->tp, sizeof(TurbParams), hipMemcpyHostToDevice,
+                            c->stream) == hipSuccess &&
+             hipMemsetAsync(c->epsmax, 0, sizeof(unsigned long long), c->stream) == hipSuccess;
     if (ok && c->sle_iters > 0)
         ok = hipMalloc(&c->Mb, sizeof(double) * diffusion_matrix_doubles(mix->ns) *
                                    (size_t)c->n) == hipSuccess &&
@@ -894,6 +1018,8 @@ void schemi_gpu_destroy(schemi_gpu_ctx *c)
     (void)hipFree(c->Mb);
     (void)hipFree(c->dmax);
     (void)hipFree(c->done);
+    (void)hipFree(c->dTP);
+    (void)hipFree(c->epsmax);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;
@@ -904,7 +1030,7 @@ int schemi_gpu_set_state(schemi_gpu_ctx *c, const double *cons, int64_t ncells,
 {
     if (!c || !cons || ncells != c->n || numbering != SCHEMI_NUMBERING_REFERENCE)
         return SCHEMI_ERR_INIT;
-    const size_t bytes = sizeof(double) * (size_t)(c->ns + 5) * (size_t)c->n;
+    const size_t bytes = sizeof(double) * (size_t)c->nf * (size_t)c->n;
     HIPCHK(hipMemcpyAsync(c->U[0], cons, bytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemsetAsync(c->derr, 0, sizeof(int), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -916,7 +1042,7 @@ int schemi_gpu_get_state(schemi_gpu_ctx *c, double *cons, double *prim, int32_t 
 {
     if (!c || numbering != SCHEMI_NUMBERING_REFERENCE)
         return SCHEMI_ERR_INIT;
-    const size_t bytes = sizeof(double) * (size_t)(c->ns + 5) * (size_t)c->n;
+    const size_t bytes = sizeof(double) * (size_t)c->nf * (size_t)c->n;
     if (cons)
         HIPCHK(hipMemcpyAsync(cons, c->U[0], bytes, hipMemcpyDeviceToHost, c->stream));
     callBuffers buf(c->stream);
@@ -926,17 +1052,27 @@ int schemi_gpu_get_state(schemi_gpu_ctx *c, double *cons, double *prim, int32_t 
         HIPCHK(buf.get(&dW, pbytes));
         const int bs = 256;
         const dim3 g((unsigned)((c->n + bs - 1) / bs));
+#define SG_PRIMS(NS)                                                                      \
+    do {                                                                                  \
+        if (c->turb)                                                                      \
+            hipLaunchKernelGGL((prims_kernel<NS, true>), g, dim3(bs), 0, c->stream, c->dP, \
+                               c->U[0], dW, c->n);                                        \
+        else                                                                              \
+            hipLaunchKernelGGL((prims_kernel<NS, false>), g, dim3(bs), 0, c->stream, c->dP, \
+                               c->U[0], dW, c->n);                                        \
+    } while (0)
         switch (c->ns) {
         case 1:
-            hipLaunchKernelGGL(prims_kernel<1>, g, dim3(bs), 0, c->stream, c->dP, c->U[0], dW, c->n);
+            SG_PRIMS(1);
             break;
         case 2:
-            hipLaunchKernelGGL(prims_kernel<2>, g, dim3(bs), 0, c->stream, c->dP, c->U[0], dW, c->n);
+            SG_PRIMS(2);
             break;
         default:
-            hipLaunchKernelGGL(prims_kernel<3>, g, dim3(bs), 0, c->stream, c->dP, c->U[0], dW, c->n);
+            SG_PRIMS(3);
             break;
         }
+#undef SG_PRIMS
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(prim, dW, pbytes, hipMemcpyDeviceToHost, c->stream));
     }
@@ -1013,7 +1149,7 @@ int schemi_gpu_stage(schemi_gpu_ctx *c, double dt)
     int r = launch_stage(c, c->U[0], c->U[1], nullptr, c->hancock ? SV_HANCOCK : SV_RK1);
     if (r)
         return r;
-    const size_t bytes = sizeof(double) * (size_t)(c->ns + 5) * (size_t)c->n;
+    const size_t bytes = sizeof(double) * (size_t)c->nf * (size_t)c->n;
     HIPCHK(hipMemcpyAsync(c->U[0], c->U[1], bytes, hipMemcpyDeviceToDevice, c->stream));
     return read_error(c);
 }
@@ -1074,7 +1210,7 @@ int schemi_gpu_state_device(schemi_gpu_ctx *c, const double *src, double *dst)
 {
     if (!c)
         return SCHEMI_ERR_INIT;
-    const size_t bytes = sizeof(double) * (size_t)(c->ns + 5) * (size_t)c->n;
+    const size_t bytes = sizeof(double) * (size_t)c->nf * (size_t)c->n;
     if (src)
         HIPCHK(hipMemcpyAsync(c->U[0], src, bytes, hipMemcpyDeviceToDevice, c->stream));
     if (dst)
@@ -1089,7 +1225,7 @@ int schemi_gpu_adjoint_step(schemi_gpu_ctx *c, double dt, const double *lam_in, 
         return SCHEMI_ERR_INIT;
     const bool kernel = c->mode == MODE_FIRST_ORDER || c->mode == MODE_LEAN ||
                         (c->mode == MODE_SLOPES && c->dim == 1);
-    if (!kernel || c->hancock || c->diffusion || c->hp.gas_model == GAS_RK ||
+    if (!kernel || c->hancock || c->diffusion || c->turb || c->hp.gas_model == GAS_RK ||
         c->hp.gas_model == GAS_KATAOKA)
         return SCHEMI_ERR_INIT;
     if (c->pending_error)

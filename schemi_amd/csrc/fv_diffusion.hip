@@ -36,6 +36,8 @@
 #include "fv_launch.hpp"
 #include "fv_stage.hpp"
 #include "fv_general.hpp"
+#include "fv_turb.hpp"
+#include "fv_turb_launch.hpp"
 
 namespace sg {
 
@@ -65,10 +67,12 @@ __device__ __forceinline__ long long block_cell0()
     return (long long)b * NT;
 }
 
-// W layout per cell: Y[NS], c[NS], v[3], T, p
+// W layout per cell: Y[NS], c[NS], v[3], T, p; with turbulence then k,
+// epsilon, CvM = Cv / (rho / c0) and nonIdeality / rho (Diffusion.cpp:137-172)
 template <int NS> struct DW {
     static constexpr int Y = 0, C = NS, V = 2 * NS, T = 2 * NS + 3, P = 2 * NS + 4,
-                         NW = 2 * NS + 5;
+                         NW = 2 * NS + 5, K = NW, EPS = NW + 1, CVM = NW + 2, NIR = NW + 3,
+                         NWT = NW + 4;
 };
 
 // Cv_mix = sum_k (c_k / c0) Cv_k (mixtureIdeal.cpp:40-51)
@@ -145,10 +149,32 @@ __device__ __forceinline__ const double *state_of(const double *U, const double 
     return (hk && sc[SC_PARITY] == 0.0) ? Ualt : U;
 }
 
-template <int NS>
+// Positive doubles order like their bit patterns: atomicMax on the bits.
+template <int NB>
+__device__ __forceinline__ void reduce_max_bits(double r, unsigned long long *slot)
+{
+    __shared__ double sMax[NB / 64];
+    const int tid = threadIdx.x;
+    for (int off = 32; off > 0; off >>= 1)
+        r = tmax(r, __shfl_xor(r, off, 64));
+    if ((tid & 63) == 0)
+        sMax[tid >> 6] = r;
+    __syncthreads();
+    if (tid == 0) {
+        double m = sMax[0];
+        for (int w = 1; w < NB / 64; ++w)
+            m = tmax(m, sMax[w]);
+        atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// TURB: the state carries rho k and rho epsilon (U = E - rho v^2 / 2 - rho k);
+// k, epsilon, CvM and nonIdeality / rho go to W, and the maximum of epsilon
+// to *epsmax (shearModel.cpp:107-114; cleared after each step).
+template <int NS, bool TURB = false>
 __global__ void __launch_bounds__(NT)
 diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const double *Ualt, int hk,
-                  double *__restrict__ W, double *sc)
+                  double *__restrict__ W, double *sc, unsigned long long *epsmax = nullptr)
 {
     const DevParams &P = *Pp;
     if (sc[SC_DONE] != 0.0)
@@ -158,14 +184,25 @@ diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const doubl
     // the Courant maximum is taken again over the state after diffusion
     if (id == 0)
         reinterpret_cast<unsigned long long *>(sc)[SC_SIGMAX] = 0ull;
+    if constexpr (TURB) {
+        double e = 0.0;
+        if (id < n)
+            e = state_of(U, Ualt, sc, hk)[(NS + 6) * n + id] /
+                state_of(U, Ualt, sc, hk)[NS * n + id];
+        reduce_max_bits<NT>(e, epsmax);
+    }
     if (id >= n)
         return;
     const double *Us = state_of(U, Ualt, sc, hk);
-    double u[NS + 5], w[NS + 4];
+    constexpr int NF = TURB ? NS + 7 : NS + 5;
+    double u[NF], w[NS + 6];
 #pragma unroll
-    for (int q = 0; q < NS + 5; ++q)
+    for (int q = 0; q < NF; ++q)
         u[q] = Us[q * n + id];
-    prims_from_cons<double, NS>(P, u, w);
+    if constexpr (TURB)
+        prims_turb<double, NS, -1>(P, u, w);
+    else
+        prims_from_cons<double, NS>(P, u, w);
     using L = DW<NS>;
     double y[NS], sum = 0.0, c0 = 0;
 #pragma unroll
@@ -184,9 +221,16 @@ diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const doubl
         W[(L::V + b) * n + id] = w[NS + b];
     const double vx = w[NS], vy = w[NS + 1], vz = w[NS + 2];
     const double v2 = ((0. + vx * vx) + vy * vy) + vz * vz;
-    const double Uint = (u[NS + 4] - u[NS] * v2 * 0.5) - 0.;
+    const double Uint = (u[NS + 4] - u[NS] * v2 * 0.5) - (TURB ? u[NS + 5] : 0.);
     W[L::T * n + id] = t_from_u<NS>(P, w, c0, Uint);
     W[L::P * n + id] = w[NS + 3];
+    if constexpr (TURB) {
+        W[L::K * n + id] = w[NS + 4];
+        W[L::EPS * n + id] = w[NS + 5];
+        const double avM = u[NS] / c0;
+        W[L::CVM * n + id] = cv_mix<NS>(P, w, c0) / avM;
+        W[L::NIR * n + id] = non_ideality<NS>(P, w, c0) / u[NS];
+    }
 }
 
 // Jacobi matrix solver (implicit Laplacians, SLEMatrix.cpp:410-547, 706-908):
@@ -202,11 +246,11 @@ template <int NS> struct MB {
                          HT = TB + 6, X = HT + 1, NF = NS + 4, N = X + 2 * NF;
 };
 
-template <int NS>
+template <int NS, int NWX = DW<NS>::NW>
 __device__ __forceinline__ void load_w(const double *W, long long n, long long id, double *w)
 {
 #pragma unroll
-    for (int q = 0; q < DW<NS>::NW; ++q)
+    for (int q = 0; q < NWX; ++q)
         w[q] = W[q * n + id];
 }
 
@@ -337,16 +381,27 @@ __device__ __forceinline__ void reduce_min_bits(double r, unsigned long long *sl
 }
 
 // SPEC: the molar-mass correction of the species fluxes (mm) and the
-// explicit enthalpy flow (en) are evaluated.
-template <int NS, int DIM, bool MIXED, bool IMPL, bool SPEC>
+// explicit enthalpy flow (en) are evaluated.  TURB (explicit solver, no SPEC):
+// k-epsilon turbulence (Diffusion.cpp:129-135, 242-286, 414-417, 580-598,
+// 711-862, 905-917, 999-1028; transportCoefficients.hpp:65-83, 221-263): the
+// face coefficients with the turbulent viscosity Cmu k^2 / eps of the face
+// values (mu + rho nu_t, rho (D + nu_t / Sc_t), kappa + c Cv nu_t (1 / Pr_e +
+// 1 / Pr_T)), the k and epsilon Laplacians (mu + rho nu_t / sigma, times
+// thetaS_D of the face divergence), the non-ideality correction Laplacian,
+// the dissipation rho eps and the filtered turbulent stress in the
+// temperature equation, and the shear model's sources with its source step.
+template <int NS, int DIM, bool MIXED, bool IMPL, bool SPEC, bool TURB = false>
 __global__ void __launch_bounds__(NT)
 diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo, double *U,
                    double *Ualt, int hk, const double *__restrict__ W, double *sc,
-                   int *errflag, double mu, double Dc, double kappa, double *Mb, int mm, int en)
+                   int *errflag, double mu, double Dc, double kappa, double *Mb, int mm, int en,
+                   const TurbParams *__restrict__ Tp = nullptr,
+                   const unsigned long long *__restrict__ epsmax = nullptr)
 {
+    static_assert(!(TURB && (IMPL || SPEC)), "turbulence: explicit solver without species terms");
     using M = MB<NS>;
     using L = DW<NS>;
-    constexpr int NW = L::NW;
+    constexpr int NW = TURB ? L::NWT : L::NW;
     __shared__ double sRed[NT / 64];
     const DevParams &P = *Pp;
     if (sc[SC_DONE] != 0.0)
@@ -361,7 +416,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         const int idx[3] = {int(id % P.N[0]), int((id / P.N[0]) % P.N[1]),
                             int(id / P.stride[2])};
         double w0[NW];
-        load_w<NS>(W, n, id, w0);
+        load_w<NS, NW>(W, n, id, w0);
         const CellGeo cg = cell_geo(P, geo, idx[0], idx[1], idx[2]);
         double gV[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, dV = 0;
         double dY[NS], dT = 0, dFv[3] = {0, 0, 0}, dFs[3] = {0, 0, 0};
@@ -370,6 +425,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             dY[k] = 0;
         int err = 0;
         double dMC[NS], dEF = 0, c0o = 0;
+        double dK = 0, dE = 0, dNI = 0;  // turbulence: k, epsilon, non-ideality Laplacians
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             dMC[k] = 0;
@@ -430,12 +486,15 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             // species corrections: face concentrations, temperature, and the
             // normal component of surfGrad of the mole fractions
             double cs[NS], Ts = 0, gXa[NS];
+            // turbulence: face k, epsilon, Cv and temperature, and the normal
+            // components of surfGrad of k, epsilon, CvM and nonIdeality / rho
+            double ks = 0, es = 0, cvs = 0, Tsf = 0, gk = 0, ge = 0, gcm = 0, gni = 0;
             if (inner) {
                 const long long nb =
                     id + (wrap ? (hi ? -(long long)(Na - 1) : (long long)(Na - 1)) * P.stride[a]
                                : (hi ? P.stride[a] : -P.stride[a]));
                 double wn[NW];
-                load_w<NS>(W, n, nb, wn);
+                load_w<NS, NW>(W, n, nb, wn);
 #pragma unroll
                 for (int q = 0; q < NW; ++q) {
                     qo[q] = hi ? w0[q] : wn[q];
@@ -471,6 +530,22 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                         gv[c][b] = ((qn[L::V + c] - qo[L::V + c]) / m2) * dd[b];
                 }
                 divs = ((qn[L::V + a] - qo[L::V + a]) / m2) * da;
+                if constexpr (TURB) {
+                    ks = qo[L::K] * wo + qn[L::K] * wnn;
+                    es = qo[L::EPS] * wo + qn[L::EPS] * wnn;
+                    double c0a = 0, c0b = 0;
+#pragma unroll
+                    for (int k = 0; k < NS; ++k) {
+                        c0a += qo[L::C + k];
+                        c0b += qn[L::C + k];
+                    }
+                    cvs = cv_mix<NS>(P, qo + L::C, c0a) * wo + cv_mix<NS>(P, qn + L::C, c0b) * wnn;
+                    Tsf = qo[L::T] * wo + qn[L::T] * wnn;
+                    gk = ((qn[L::K] - qo[L::K]) / m2) * da;
+                    ge = ((qn[L::EPS] - qo[L::EPS]) / m2) * da;
+                    gcm = ((qn[L::CVM] - qo[L::CVM]) / m2) * da;
+                    gni = ((qn[L::NIR] - qo[L::NIR]) / m2) * da;
+                }
                 if constexpr (SPEC) {
                     Ts = qo[L::T] * wo + qn[L::T] * wnn;
                     double c0a = 0, c0b = 0;
@@ -546,6 +621,35 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                         gv[c][b] = dq[c] * nrm[b];
                 }
                 divs = ((0. + dq[0] * nrm[0]) + dq[1] * nrm[1]) + dq[2] * nrm[2];
+                if constexpr (TURB) {
+                    // surface values (linearInterpolate boundary values;
+                    // calculatedCv and calculatedTemperature from the surface
+                    // concentrations and pressure) and the ghosts (k, eps;
+                    // calculatedCvM and calculatedNonidealityCorrectionPerDensity
+                    // from the ghost concentrations, boundaryConditionValue.hpp:
+                    // 415-484)
+                    const TurbParams &TP = *Tp;
+                    ks = bc_surface<double>(TP.bct[p][0], TP.bcv[p][0], w0[L::K], -1, a);
+                    es = bc_surface<double>(TP.bct[p][1], TP.bcv[p][1], w0[L::EPS], -1, a);
+                    double cs0 = 0;
+#pragma unroll
+                    for (int k = 0; k < NS; ++k)
+                        cs0 += cs[k];
+                    cvs = cv_mix<NS>(P, cs, cs0);
+                    const double ps =
+                        bc_surface<double>(P.bct[p][NS + 1], P.bcv[p][NS + 1][0], w0[L::P], -1, a);
+                    Tsf = t_from_u<NS>(P, cs, cs0, u_from_p<NS>(P, cs, cs0, ps));
+                    const double kg = bc_cell<double>(TP.bct[p][0], TP.bcv[p][0], w0[L::K], -1, a);
+                    const double eg = bc_cell<double>(TP.bct[p][1], TP.bcv[p][1], w0[L::EPS], -1, a);
+                    const double cmg = cv_mix<NS>(P, cgh, cg0) * cg0 / sumd;
+                    const double nig = non_ideality<NS>(P, cgh, cg0) / sumd;
+                    gk = ((kg - w0[L::K]) / m) * nrm[a];
+                    ge = ((eg - w0[L::EPS]) / m) * nrm[a];
+                    gcm = ((cmg - w0[L::CVM]) / m) * nrm[a];
+                    gni = ((nig - w0[L::NIR]) / m) * nrm[a];
+                    if (TP.bct[p][0] == BC_FIXED_SURF || TP.bct[p][1] == BC_FIXED_SURF)
+                        err = 1;
+                }
                 if constexpr (SPEC) {
                     // calculatedTemperature on the surface; calculatedMolarFraction ghost
                     double cs0 = 0;
@@ -591,6 +695,32 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                     dEF += ((e * Ts) * na) * S;
                 }
             }
+            // effective coefficients: with turbulence mu + rho nu_t,
+            // rho (D + nu_t / Sc_t), kappa + c Cv nu_t / Pr_e + c Cv nu_t / Pr_T
+            double mu_f = mu, kappa_f = kappa, tD = 0;
+            if constexpr (TURB) {
+                const TurbParams &TP = *Tp;
+                double cs0 = 0;
+#pragma unroll
+                for (int k = 0; k < NS; ++k)
+                    cs0 += cs[k];
+                const double tnu = TP.Cmu * ks * ks / es;
+                tD = tnu / TP.sigmaSc;
+                const double tKap = tnu / TP.sigmaT;
+                const double tLam = tnu / TP.sigmaE;
+                mu_f = mu + srho * tnu;
+                const double cC = cs0 * cvs;
+                kappa_f = kappa + cC * tLam + cC * tKap;
+                // k and epsilon Laplacians, thetaS_D of the face divergence
+                const double th = (divs < 0.) ? theta_s(TP, divs, ks, es, TP.CMS_D) : 1.;
+                const double kD = tnu / TP.sigmaK * th, eD = tnu / TP.sigmaEps * th;
+                const double rhoDk = mu + srho * kD, rhoDe = mu + srho * eD;
+                dK += (((gk * rhoDk) * -1) * na) * S;
+                dE += (((ge * rhoDe) * -1) * na) * S;
+                // -(surfGrad CvM T_f + surfGrad (nonIdeality / rho)) rho tLambda
+                const double flow = -1 * (gcm * Tsf + gni);
+                dNI += ((flow * srho * tLam) * na) * S;
+            }
             // Green-Gauss sums of grad v (v_c N_b) S and div v (v & N) S
 #pragma unroll
             for (int c = 0; c < 3; ++c)
@@ -599,13 +729,13 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             // fluxes -coeff surfGrad, their normal component times S
 #pragma unroll
             for (int k = 0; k < NS; ++k) {
-                const double rhoD = srho * Dc;
+                const double rhoD = srho * (Dc + tD);
                 dY[k] += (((gY[k][a] * rhoD) * -1) * na) * S;
             }
-            dT += (((gT[a] * kappa) * -1) * na) * S;
+            dT += (((gT[a] * kappa_f) * -1) * na) * S;
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                dFv[c] += (((gv[c][a] * mu) * -1) * na) * S;
+                dFv[c] += (((gv[c][a] * mu_f) * -1) * na) * S;
             // mu strainSurf1 & N: row c, column a of (grad_s v)^T - lam I
             const double lam = (2. / 3.) * divs;
 #pragma unroll
@@ -613,7 +743,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 double t = gv[a][c];
                 if (c == a)
                     t -= lam;
-                dFs[c] += ((t * mu) * na) * S;
+                dFs[c] += ((t * mu_f) * na) * S;
             }
             if constexpr (IMPL) {
                 // (k_f / |d|) ((N S / V) & (d / |d|)) into the diagonal, times
@@ -695,9 +825,10 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 Mb[(M::TB + s) * n + id] = tB[s];
             }
         } else {
-        double u[NS + 5];
+        constexpr int NF = TURB ? NS + 7 : NS + 5;
+        double u[NF];
 #pragma unroll
-        for (int q = 0; q < NS + 5; ++q)
+        for (int q = 0; q < NF; ++q)
             u[q] = Us[q * n + id];
         const double rho = u[NS];
         // mass fractions: (Y rho + (0 - div F) dt) / rho, clipped, renormalised
@@ -758,6 +889,15 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             heat += (sv[q] * mu) * G[q];
         double ftT = 0. - dT / V;
         double diagT = CCVnew;
+        if constexpr (TURB) {
+            // non-ideality correction Laplacian, distributeSourceTerm
+            // (Diffusion.cpp:604-611, SLEMatrix.cpp:2166-2175)
+            const double src = (dNI / V) * (-1.0);
+            if (src >= 0.)
+                ftT += src;
+            else
+                diagT -= src / (w0[L::T] + STAB);
+        }
         if (SPEC && en) {
             // source time step ratio C Cv_old T_old / |div + stab|
             // (Diffusion.cpp:696-700); the minimum over the cells times the
@@ -770,6 +910,56 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 diagT -= src / (w0[L::T] + STAB);
         }
         ftT += heat;
+        double kn = 0, en_ = 0;
+        if constexpr (TURB) {
+            const TurbParams &TP = *Tp;
+            const double k = w0[L::K], eps = w0[L::EPS];
+            const double rk = u[NS + 5], reps = u[NS + 6];
+            // dissipation rho eps (kEpsModels.cpp:76-82) and the filtered part
+            // of the turbulent stress (Diffusion.cpp:586-598, 719-733)
+            if (TP.model != TURB_ZERO)
+                ftT += rho * eps;
+            const double tnu = TP.Cmu * k * k / eps;
+            const double thR = theta_s(TP, divV, k, eps, TP.CMS_R);
+            const double tnu1 = (1. - thR) * tnu;
+            double philt = 0, gsph = 0, gdev = 0;
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                philt += (sv[q] * rho * tnu1) * G[q];
+            ftT += philt;
+            // generation by the spherical (-(2/3) rho k I) and deviatoric
+            // (rho nu_t thetaS_R strain) Reynolds stresses
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const double Iq = (q == 0 || q == 4 || q == 8) ? 1. : 0.;
+                gsph += (Iq * -(2. / 3.) * rk) * G[q];
+                gdev += (sv[q] * rho * tnu * thR) * G[q];
+            }
+            double sk = 0, se = 0;
+            if (TP.model == TURB_SHEAR) {
+                // shearModel.cpp:116-147
+                const double ek = eps / k;
+                const double dissip = -reps;
+                const double Sk1 = gsph + gdev, Sk2 = dissip / k;
+                const double Se1 = (TP.C1 * gdev + TP.C3 * gsph) * ek;
+                const double Se2 = TP.C2 * dissip / k;
+                sk = Sk1 + Sk2 * k;
+                se = Se1 + Se2 * eps;
+                // source step: eps below 1e-3 of its maximum is veryBig
+                const double emax =
+                    __longlong_as_double(static_cast<long long>(epsmax[0]));
+                const double me = (eps < 1E-3 * emax) ? 1e-3 * __DBL_MAX__ : eps;
+                rsrc = __builtin_fabs(TP.ts_coeff * me / ((Se1 + Se2 * eps) / rho + STAB));
+            }
+            // explicit Laplacians plus the sources (Diffusion.cpp:839-862),
+            // minimal values (905-917)
+            kn = (k * rho + (0. - dK / V) * dt) / rho + sk / rho * dt;
+            en_ = (eps * rho + (0. - dE / V) * dt) / rho + se / rho * dt;
+            if (kn < TP.mink)
+                kn = TP.mink;
+            if (en_ < TP.mineps)
+                en_ = TP.mineps;
+        }
         // oldEnergyField (Diffusion.cpp:484-485)
         const double oldE = CCVold * w0[L::T] + non_ideality<NS>(P, w0 + L::C, c0o) -
                             non_ideality<NS>(P, cn, c0n);
@@ -777,8 +967,8 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         if (Tn < 0.)
             atomicOr(errflag, ERRF_NEG_T);
         // recalculated cell: rho_k = c_k M_k, m = v rho, U = Cv T c0,
-        // E = (U + rho v.v / 2) + 0
-        double un[NS + 5];
+        // E = (U + rho v.v / 2) + rho k
+        double un[NF];
 #pragma unroll
         for (int k = 0; k < NS; ++k)
             un[k] = cn[k] * P.M[k];
@@ -788,18 +978,30 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             un[NS + 1 + c] = vn[c] * rho;
         const double Uin = uvc_from_t<NS>(P, cn, c0n, Tn) * c0n;
         const double v2 = ((0. + vn[0] * vn[0]) + vn[1] * vn[1]) + vn[2] * vn[2];
-        un[NS + 4] = (Uin + rho * v2 * 0.5) + 0.0;
+        double rkn = 0.0;
+        if constexpr (TURB) {
+            rkn = kn * rho;
+            un[NS + 5] = rkn;
+            un[NS + 6] = en_ * rho;
+        }
+        un[NS + 4] = (Uin + rho * v2 * 0.5) + rkn;
 #pragma unroll
-        for (int q = 0; q < NS + 5; ++q)
+        for (int q = 0; q < NF; ++q)
             Us[q * n + id] = un[q];
         (void)MIXED;
-        sig = signal_speed<double, NS>(P, un, cg.minlen);
+        if constexpr (TURB)
+            sig = signal_speed_turb<double, NS, -1>(P, un, cg.minlen);
+        else
+            sig = signal_speed<double, NS>(P, un, cg.minlen);
         }
     }
     if constexpr (!IMPL) {
         reduce_signal<NT>(sig, sRed, reinterpret_cast<unsigned long long *>(sc) + SC_SIGMAX);
         if constexpr (SPEC)
             if (en)
+                reduce_min_bits<NT>(rsrc, reinterpret_cast<unsigned long long *>(sc) + SC_SRCMIN);
+        if constexpr (TURB)
+            if (Tp->model == TURB_SHEAR)
                 reduce_min_bits<NT>(rsrc, reinterpret_cast<unsigned long long *>(sc) + SC_SRCMIN);
     }
 }
@@ -1032,6 +1234,22 @@ diff_finish_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
     reduce_signal<NT>(sig, sRed, reinterpret_cast<unsigned long long *>(sc) + SC_SIGMAX);
 }
 
+// Turbulent explicit stage: the primitives pass with k, epsilon, CvM and
+// nonIdeality / rho and the maximum of epsilon, then the update.
+template <int NS, int DIM>
+int launch_dim_turb(const DiffusionLaunch &L, const TurbDiffusion &T)
+{
+    const unsigned blocks = (unsigned)((L.n + NT - 1) / NT);
+    hipLaunchKernelGGL((diff_prims_kernel<NS, true>), dim3(blocks), dim3(NT), 0, L.stream, L.P,
+                       L.U, L.Ualt, L.hk, L.W, L.sc, T.epsmax);
+    if (L.mid && hipEventRecord(L.mid, L.stream) != hipSuccess)
+        return 1;
+    hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false, false, true>), dim3(blocks),
+                       dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc, L.err,
+                       L.mu, L.D, L.kappa, nullptr, 0, 0, T.TP, T.epsmax);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 template <int NS, int DIM>
 int launch_dim(const DiffusionLaunch &L)
 {
@@ -1117,6 +1335,29 @@ size_t diffusion_matrix_doubles(int ns)
         return MB<2>::N;
     default:
         return MB<3>::N;
+    }
+}
+
+int launch_diffusion_turb(const DiffusionLaunch &L, const TurbDiffusion &T)
+{
+    auto by_dim = [&](auto ns) -> int {
+        constexpr int NS = decltype(ns)::value;
+        switch (L.dim) {
+        case 1:
+            return launch_dim_turb<NS, 1>(L, T);
+        case 2:
+            return launch_dim_turb<NS, 2>(L, T);
+        default:
+            return launch_dim_turb<NS, 3>(L, T);
+        }
+    };
+    switch (L.ns) {
+    case 1:
+        return by_dim(std::integral_constant<int, 1>{});
+    case 2:
+        return by_dim(std::integral_constant<int, 2>{});
+    default:
+        return by_dim(std::integral_constant<int, 3>{});
     }
 }
 

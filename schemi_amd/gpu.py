@@ -67,7 +67,14 @@ class SchemeDesc(C.Structure):
                 ("transport_kappa", C.c_double), ("diffusion_solver", C.c_int32),
                 ("diffusion_iterations", C.c_int32), ("diffusion_molar_mass", C.c_int32),
                 ("diffusion_enthalpy_flow", C.c_int32), ("timestep_source", C.c_int32),
-                ("source_coeff", C.c_double)]
+                ("source_coeff", C.c_double), ("turbulence", C.c_int32),
+                ("turbulence_model", C.c_int32), ("turb_Cmu", C.c_double),
+                ("turb_sigma_sc", C.c_double), ("turb_sigma_t", C.c_double),
+                ("turb_sigma_e", C.c_double), ("turb_sigma_k", C.c_double),
+                ("turb_sigma_eps", C.c_double), ("turb_CMS_R", C.c_double),
+                ("turb_CMS_D", C.c_double), ("turb_min_k", C.c_double),
+                ("turb_min_eps", C.c_double), ("turb_C", C.c_double * 3),
+                ("bc_k", _BC * 6), ("bc_eps", _BC * 6)]
 
 
 _LIB = None
@@ -210,6 +217,20 @@ def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) 
     s.diffusion_enthalpy_flow = 1 if case.enthalpy_flow else 0
     s.timestep_source = 1 if case.timestep_source else 0
     s.source_coeff = float(case.source_coeffs[0])
+    t = case.turbulence
+    if t is not None:
+        s.turbulence = 1
+        s.turbulence_model = int(t.model)
+        (s.turb_Cmu, s.turb_sigma_sc, s.turb_sigma_t, s.turb_sigma_e, s.turb_sigma_k,
+         s.turb_sigma_eps, s.turb_CMS_R, s.turb_CMS_D, s.turb_min_k,
+         s.turb_min_eps) = (float(x) for x in t.params())
+        for i, x in enumerate(t.C[:3]):
+            s.turb_C[i] = float(x)
+        for p in range(6):
+            for dst, src in ((s.bc_k, t.bc_k), (s.bc_eps, t.bc_eps)):
+                dst[p].type = int(src[p].type)
+                for a in range(3):
+                    dst[p].value[a] = float(src[p].value[a])
     return s
 
 
@@ -271,7 +292,7 @@ class GpuSolver:
             self._ctx = None
             raise SchemiGpuError(r, "schemi_gpu_create")
         self.n = case.ncells
-        self.nf = case.Ns + 5
+        self.nf = case.nfields
 
     def close(self):
         if getattr(self, "_ctx", None):
@@ -529,13 +550,16 @@ def vdw_matrices(case: Case):
     return a, b
 
 
-def conserved_from_primitive(case: Case, c, v, p) -> np.ndarray:
+def conserved_from_primitive(case: Case, c, v, p, kt=None, et=None) -> np.ndarray:
     """Conserved vector from primitives (c_k, v, p), with the reference's
     initialisation arithmetic (phaseInitialization.cpp:1827-1883) and its
     UvFromp (idealFluid.cpp:19-23, vanDerWaalsFluid.cpp:24-31; Redlich-Kwong
     RedlichKwongFluid.cpp:59-70 with its cubic solved by Newton's method in
-    FP64, a few ulps from the reference's long-double root).  Pure host
-    arithmetic on numpy arrays; used to prepare initial states."""
+    FP64, a few ulps from the reference's long-double root).  With turbulence
+    k and epsilon are clipped to the minimal values, rho k = k rho and
+    rho eps = rho eps are appended and rho k is added to E
+    (phaseInitialization.cpp:1814-1848, 1873-1877).  Pure host arithmetic on
+    numpy arrays; used to prepare initial states."""
     c = np.asarray(c, dtype=np.float64)
     v = np.asarray(v, dtype=np.float64)
     p = np.asarray(p, dtype=np.float64)
@@ -592,8 +616,15 @@ def conserved_from_primitive(case: Case, c, v, p) -> np.ndarray:
     else:
         U = p / (case.R / cv)
     v2 = ((0.0 + v[0] * v[0]) + v[1] * v[1]) + v[2] * v[2]
-    E = (U + rho * v2 * 0.5) + 0.0
-    return np.concatenate([rho_k, rho[None], m, E[None]])
+    t = case.turbulence
+    if t is None:
+        E = (U + rho * v2 * 0.5) + 0.0
+        return np.concatenate([rho_k, rho[None], m, E[None]])
+    kt = np.maximum(np.asarray(kt, dtype=np.float64), t.mink)
+    et = np.maximum(np.asarray(et, dtype=np.float64), t.mineps)
+    rk = kt * rho
+    E = (U + rho * v2 * 0.5) + rk
+    return np.concatenate([rho_k, rho[None], m, E[None], rk[None], (rho * et)[None]])
 
 
 def periodic(case: Case, axis: int):

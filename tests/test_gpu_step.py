@@ -36,17 +36,32 @@ def test_golden_fixtures(name, hip_device):
     case = g["case"]
     o = Oracle(case)
     if "restart" in g:
-        o.init_primitive(*g["restart"])
+        kw = {}
+        if "restart_turb" in g:
+            kw = dict(k=g["restart_turb"][0], eps=g["restart_turb"][1])
+        o.init_primitive(*g["restart"], **kw)
         U0 = o.conserved()
     else:
         U0 = conserved(g["snaps"][0])
     s = GpuSolver(case)
     s.set_conserved(U0)
+    Ns = case.Ns
     for step, dt in enumerate(g["dts"], start=1):
         s.step(dt)
         if step in g["snaps"]:
             got = s.conserved()
             ref = conserved(g["snaps"][step])
+            if case.turbulence is not None:
+                # the reference writes k and epsilon; the state carries rho k
+                # and rho epsilon: compare k = rho k / rho, eps = rho eps / rho
+                # (positive everywhere) with the same per-cell bars
+                kt = np.stack([got[Ns + 5] / got[Ns], got[Ns + 6] / got[Ns]])
+                kr = np.stack([g["snaps"][step]["k"], g["snaps"][step]["eps"]])
+                bar = 1e-12 if step == 1 else 1e-10
+                for q in range(2):
+                    err = float((np.abs(kt[q] - kr[q]) / np.abs(kr[q])).max())
+                    assert err <= bar, f"{name} step {step} turbulent field {q}: {err:.3e}"
+                got = got[:Ns + 5]
             if step == 1:
                 for q, err in enumerate(per_cell_rel(got, ref, case.Ns)):
                     assert err <= 1e-12, f"{name} step 1 field {q}: {err:.3e}"

@@ -13,8 +13,9 @@ FP64 after 1000 steps (and after 2000 steps of the 2-D Rayleigh-Taylor case,
 256 x 1024, periodic x), the HBM roofline fraction of the stage kernel
 (algorithmic bytes per launch over its average HIP-event duration), the
 stage kernel with the TVD limiter evaluated in full as a secondary figure,
-and the reference solver's CPU step timed on a bounded sample of the same
-workload.
+the 2-D Kelvin-Helmholtz workload with k-epsilon turbulence (both paths, and
+mixed against FP64 after 3000 steps), and the reference solver's CPU step
+timed on a bounded sample of the same workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -58,6 +59,8 @@ def parse():
     ap.add_argument("--l1-steps", type=int, default=1000)
     ap.add_argument("--rt-steps", type=int, default=2000,
                     help="steps of the 2-D Rayleigh-Taylor FP64-vs-mixed gate")
+    ap.add_argument("--kh-steps", type=int, default=3000,
+                    help="steps of the turbulent Kelvin-Helmholtz FP64-vs-mixed gate")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 1 GiB streaming copy (calibrates rocprofv3 byte counters)")
@@ -129,6 +132,50 @@ def l1_gate(case, U0, nsteps):
         "p_l1_rel": float(np.abs(pmx - p64).sum() / np.abs(p64).sum()),
         "mixing_width_rel": float(abs(wmx - w64) / abs(w64)),
     }
+
+
+def kh_turbulent(steps, warmup, gate_steps):
+    """The 2-D Kelvin-Helmholtz workload with k-epsilon turbulence (512 x 512,
+    periodic, van der Waals CO2/N2 at 300 K and 5 MPa, HLL + minmod, the
+    diffusion stage with the shear model): Mcell-updates/s of both precision
+    paths (whole step: two turbulent stage kernels, the two diffusion passes,
+    the device time step), and the mixed path against FP64 after gate_steps
+    steps at a constant time step (relative L1 of rho, p, k and epsilon)."""
+    from schemi_amd import cases
+    from schemi_amd.gpu import PREC_FP64, PREC_MIXED, GpuSolver
+
+    case, U0 = cases.kelvin_helmholtz_turbulent_2d()
+    n, Ns = case.ncells, case.Ns
+    out = {"workload": "2-D Kelvin-Helmholtz 512x512 (262144 cells), periodic, van der Waals "
+                       "CO2/N2 300 K 5 MPa, HLL + minmod, SSP-RK2, diffusion stage with the "
+                       "shear k-epsilon model, Courant step on device"}
+    for prec, name in ((PREC_FP64, "fp64"), (PREC_MIXED, "mixed")):
+        s = GpuSolver(case, prec)
+        wall, avg_ms, _ = timed_run(s, U0, steps, warmup)
+        s.close()
+        out[name] = {"ms_per_step": 1e3 * wall / steps, "value": n * steps / wall / 1e6,
+                     "stage_kernel_ms": avg_ms}
+    probe = GpuSolver(case, PREC_FP64)
+    probe.set_conserved(U0)
+    dt = 0.8 * probe.step(1e-15)
+    probe.close()
+    fin = {}
+    for prec in (PREC_FP64, PREC_MIXED):
+        s = GpuSolver(case, prec, const_dt=dt)
+        s.set_conserved(U0)
+        s.run(gate_steps, dt)
+        fin[prec] = (s.conserved(), s.primitives()["p"])
+        s.close()
+    (a, pa), (b, pb) = fin[PREC_FP64], fin[PREC_MIXED]
+
+    def l1(x, y):
+        return float(np.abs(y - x).sum() / np.abs(x).sum())
+
+    out["l1_gate"] = {"steps": gate_steps, "dt": dt, "rho_l1_rel": l1(a[Ns], b[Ns]),
+                      "p_l1_rel": l1(pa, pb),
+                      "k_l1_rel": l1(a[Ns + 5] / a[Ns], b[Ns + 5] / b[Ns]),
+                      "eps_l1_rel": l1(a[Ns + 6] / a[Ns], b[Ns + 6] / b[Ns])}
+    return out
 
 
 def _ref_stage_ms(root):
@@ -371,6 +418,8 @@ def main():
     gate_rt["workload"] = ("2-D Rayleigh-Taylor 256x1024, HLLC + van Leer, gravity -1, "
                            "periodic x, slip y")
     print(f"# l1 gate RT: {gate_rt}", file=sys.stderr, flush=True)
+    kh = kh_turbulent(a.steps, a.warmup, a.kh_steps)
+    print(f"# turbulent KH: {kh}", file=sys.stderr, flush=True)
     cpu = None if a.no_cpu else cpu_baseline()
     mx = res["mixed"]
     achieved = BYTES_PER_CELL_LAUNCH * n / (mx["stage_kernel_ms"] * 1e-3) / 1e9
@@ -402,6 +451,7 @@ def main():
         "l1_gate_rt2d": gate_rt,
         "l1_rt2d_ok": gate_rt["rho_l1_rel"] <= 1e-5,
         "full_limiter": full_limiter,
+        "kh_turbulent": kh,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic},

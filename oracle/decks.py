@@ -69,7 +69,7 @@ def _free_lines(case):
 
 def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
                const_dt=-1.0, min_time=None, courant=None, restart_turb=None,
-               zones_turb=None):
+               zones_turb=None, restart_a=None):
     """Writes ``root/set`` for ``case``.
 
     zones:   ((c_left[Ns], v_left[3], p_left), (c_right, v_right, p_right), x0)
@@ -78,6 +78,8 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
              cell order, injected through the restart reader.
     restart_turb: (k[n], eps[n]) initial turbulent fields for the restart
              reader (zeros when None).
+    restart_a: a[3, n], the initial turbulent mass flux of the kEpsA model
+             for the restart reader (zeros when None).
     zones_turb: ((k_left, eps_left), (k_right, eps_right)) zone values.
     """
     set_dir = os.path.join(root, "set")
@@ -170,6 +172,9 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         files["epsilon.txt"] = _bc_lines(case, turb.bc_eps) + [
             "#Values_in_zones", _r(kz[0][1]), _r(kz[1][1])]
     files["a.txt"] = _free_lines(case) + ["#Values_in_zones", "0 0 0", "0 0 0"]
+    if turb is not None and int(turb.model) == 6:
+        files["a.txt"] = _bc_lines(case, turb.bc_a, vec=True) + [
+            "#Values_in_zones", "0 0 0", "0 0 0"]
     files["chemicalKinetics.txt"] = ["#Type_of_chemical_reaction", "no"]
     files["MPIRanks.txt"] = ["#MPI_nodes_in_directions", "1 1 1"]
     files["GoncharovModel.txt"] = ["off", "#Number_of_particles", "1",
@@ -210,6 +215,11 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         mins = [("Minimal_k", turb.mink), ("Minimal_epsilon", turb.mineps)]
         if int(turb.model) == 2:
             tp += cms + mins + [(f"C{i + 1}", c) for i, c in enumerate(turb.C[:3])]
+        elif int(turb.model) == 6:
+            # kEpsAModel.cpp:48-80: sigma_a after sigma_epsilon, then the
+            # eight model constants
+            names = ("C0", "C1", "C2", "C3", "Ca", "CaMax", "CMSA", "chi")
+            tp += [("sigma_a", turb.sigmaa)] + cms + mins + list(zip(names, turb.C[:8]))
         else:
             tp += mins + cms
         files["turbulentParameters.txt"] = [y for name, val in tp for y in ("#" + name, _r(val))]
@@ -236,8 +246,8 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         kd, ed = (restart_turb if restart_turb is not None else (z, z))
         for name, arr in (("k.dat", kd), ("epsilon.dat", ed), ("b.dat", z)):
             np.savetxt(os.path.join(d, name), np.asarray(arr), fmt="%.20g")
-        np.savetxt(os.path.join(d, "a.dat"), np.zeros((n, 3)), fmt="%.20g",
-                   delimiter="\t")
+        ad = np.zeros((3, n)) if restart_a is None else np.asarray(restart_a)
+        np.savetxt(os.path.join(d, "a.dat"), ad.T, fmt="%.20g", delimiter="\t")
         with open(os.path.join(root, "result", "Time.tsv"), "w") as f:
             f.write("0\t0\n1\t0\n")
 
@@ -277,6 +287,8 @@ def read_output(root, number, Ns):
         "E": col["total_energy"],
         "k": col["k"],
         "eps": col["epsilon"],
+        "a": np.stack([col["a_x"], col["a_y"], col["a_z"]]),
+        "b": col["b"],
         "xyz": np.stack([col["coordinate_x"], col["coordinate_y"],
                          col["coordinate_z"]]),
     }

@@ -70,9 +70,11 @@ class _Config(C.Structure):
         ("CMS_D", C.c_double),
         ("mink", C.c_double),
         ("mineps", C.c_double),
-        ("tC", C.c_double * 4),
+        ("tC", C.c_double * 8),
         ("bc_k", _BC * 6),
         ("bc_eps", _BC * 6),
+        ("sigmaa", C.c_double),
+        ("bc_a", _BC * 6),
     ]
 
 
@@ -92,6 +94,9 @@ class _State(C.Structure):
         ("eps", C.POINTER(C.c_double)),
         ("rk", C.POINTER(C.c_double)),
         ("reps", C.POINTER(C.c_double)),
+        ("a", C.POINTER(C.c_double) * 3),
+        ("ra", C.POINTER(C.c_double) * 3),
+        ("b", C.POINTER(C.c_double)),
     ]
 
 
@@ -120,6 +125,8 @@ def lib():
         L.or_set_conserved.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.or_init_turbulence.argtypes = [C.c_void_p, dp, dp]
         L.or_set_turbulence_conserved.argtypes = [C.c_void_p, dp, dp]
+        L.or_init_turbulence_a.argtypes = [C.c_void_p, dp]
+        L.or_set_turbulence_a_conserved.argtypes = [C.c_void_p, dp]
         L.or_step.restype = C.c_int
         L.or_step.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.or_step_hancock.restype = C.c_int
@@ -190,8 +197,9 @@ def make_config(case) -> _Config:
          cfg.CMS_D, cfg.mink, cfg.mineps) = (float(x) for x in tb.params())
         for i, x in enumerate(tb.C):
             cfg.tC[i] = float(x)
+        cfg.sigmaa = float(tb.sigmaa)
         for p in range(6):
-            for dst, src in ((cfg.bc_k, tb.bc_k), (cfg.bc_eps, tb.bc_eps)):
+            for dst, src in ((cfg.bc_k, tb.bc_k), (cfg.bc_eps, tb.bc_eps), (cfg.bc_a, tb.bc_a)):
                 dst[p].type = int(src[p].type)
                 for a in range(3):
                     dst[p].val[a] = float(src[p].value[a])
@@ -226,9 +234,14 @@ class Oracle:
             self._s = None
 
     # -- state ---------------------------------------------------------------
-    def init_primitive(self, c, v, p, k=None, eps=None):
+    @property
+    def a_field(self) -> bool:
+        return bool(self._cfg.turb) and self._cfg.turb_model == 6
+
+    def init_primitive(self, c, v, p, k=None, eps=None, a=None):
         """Initial state from primitives; with turbulence also k and epsilon
-        (clipped to the minimal values as the reference's initialisation)."""
+        (clipped to the minimal values as the reference's initialisation) and
+        with the kEpsA model a (3, n)."""
         c = np.ascontiguousarray(c, dtype=np.float64).reshape(-1)
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
         p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
@@ -236,6 +249,9 @@ class Oracle:
             k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1)
             eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(-1)
             lib().or_init_turbulence(self._s, _dp(k), _dp(eps))
+        if self.a_field:
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            lib().or_init_turbulence_a(self._s, _dp(a))
         lib().or_init_primitive(self._s, _dp(c), _dp(v), _dp(p))
 
     def set_full(self, st: dict):
@@ -245,21 +261,27 @@ class Oracle:
                           _dp(a["rho"]), _dp(a["v"]), _dp(a["p"]), _dp(a["U"]),
                           _dp(a["T"]), _dp(a["m"]), _dp(a["E"]))
 
-    def set_conserved(self, rho_k, rho, m, E, rk=None, reps=None):
+    def set_conserved(self, rho_k, rho, m, E, rk=None, reps=None, ra=None):
         """State from the conserved vector; with turbulence also rho k and
-        rho epsilon."""
+        rho epsilon, and with the kEpsA model rho a (3, n)."""
         if self._cfg.turb:
             t = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (rk, reps)]
             lib().or_set_turbulence_conserved(self._s, _dp(t[0]), _dp(t[1]))
+        if self.a_field:
+            ra = np.ascontiguousarray(ra, dtype=np.float64).reshape(-1)
+            lib().or_set_turbulence_a_conserved(self._s, _dp(ra))
         a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
              for x in (rho_k, rho, m, E)]
         lib().or_set_conserved(self._s, *[_dp(x) for x in a])
 
     def set_conserved_all(self, U):
         """State from a (nfields, n) conserved array in the library's layout:
-        rho_1..rho_Ns, rho, m_x, m_y, m_z, E[, rho k, rho epsilon]."""
+        rho_1..rho_Ns, rho, m_x, m_y, m_z, E[, rho k, rho epsilon[, rho a]]."""
         Ns = self.case.Ns
-        if self._cfg.turb:
+        if self.a_field:
+            self.set_conserved(U[:Ns], U[Ns], U[Ns + 1:Ns + 4], U[Ns + 4], U[Ns + 5], U[Ns + 6],
+                               U[Ns + 7:Ns + 10])
+        elif self._cfg.turb:
             self.set_conserved(U[:Ns], U[Ns], U[Ns + 1:Ns + 4], U[Ns + 4], U[Ns + 5], U[Ns + 6])
         else:
             self.set_conserved(U[:Ns], U[Ns], U[Ns + 1:Ns + 4], U[Ns + 4])
@@ -286,15 +308,21 @@ class Oracle:
             "eps": arr(st.eps),
             "rk": arr(st.rk),
             "reps": arr(st.reps),
+            "a": np.stack([arr(st.a[c]) for c in range(3)]),
+            "ra": np.stack([arr(st.ra[c]) for c in range(3)]),
+            "b": arr(st.b),
         }
 
     def conserved(self) -> np.ndarray:
         """(Ns + 5, n) array: rho_1..rho_Ns, rho, m_x, m_y, m_z, E; with
-        turbulence (Ns + 7, n), rho k and rho epsilon appended."""
+        turbulence (Ns + 7, n), rho k and rho epsilon appended; with the kEpsA
+        model (Ns + 10, n), rho a appended."""
         s = self.state()
         rows = [s["rho"], s["rho0"][None], s["m"], s["E"][None]]
         if self._cfg.turb:
             rows += [s["rk"][None], s["reps"][None]]
+        if self.a_field:
+            rows += [s["ra"]]
         return np.concatenate(rows)
 
     # -- stepping ------------------------------------------------------------

@@ -1136,46 +1136,6 @@ static double eos_c2(const or_config *cfg, const double *c, double rho, double U
 /* ------------------------------------------------------------------------ */
 /* state                                                                     */
 
-static void alloc_state(or_solver *s)
-{
-    const long n = s->nc;
-    const int Ns = s->cfg.Ns;
-    const int nfield = 2 * (Ns + 1) + 3 + 3 + 3 + 1 + 4;
-    s->pool = calloc((size_t)nfield * n, sizeof(double));
-    double *q = s->pool;
-    or_state *st = &s->st;
-    st->n = n;
-    st->Ns = Ns;
-    for (int k = 0; k <= Ns; ++k) {
-        st->c[k] = q;
-        q += n;
-    }
-    for (int k = 0; k <= Ns; ++k) {
-        st->rho[k] = q;
-        q += n;
-    }
-    for (int a = 0; a < 3; ++a) {
-        st->v[a] = q;
-        q += n;
-    }
-    st->p = q;
-    q += n;
-    st->U = q;
-    q += n;
-    st->T = q;
-    q += n;
-    for (int a = 0; a < 3; ++a) {
-        st->m[a] = q;
-        q += n;
-    }
-    st->E = q;
-    q += n;
-    st->k = q;
-    st->eps = q + n;
-    st->rk = q + 2 * n;
-    st->reps = q + 3 * n;
-}
-
 static void copy_state(const or_state *src, or_state *dst, long n, int Ns)
 {
     for (int k = 0; k <= Ns; ++k) {
@@ -1194,11 +1154,16 @@ static void copy_state(const or_state *src, or_state *dst, long n, int Ns)
     memcpy(dst->eps, src->eps, n * sizeof(double));
     memcpy(dst->rk, src->rk, n * sizeof(double));
     memcpy(dst->reps, src->reps, n * sizeof(double));
+    for (int a = 0; a < 3; ++a) {
+        memcpy(dst->a[a], src->a[a], n * sizeof(double));
+        memcpy(dst->ra[a], src->ra[a], n * sizeof(double));
+    }
+    memcpy(dst->b, src->b, n * sizeof(double));
 }
 
 static double *state_pool_new(long n, int Ns, or_state *st)
 {
-    const int nfield = 2 * (Ns + 1) + 3 + 3 + 3 + 1 + 4;
+    const int nfield = 2 * (Ns + 1) + 3 + 3 + 3 + 1 + 4 + 7;
     double *pool = calloc((size_t)nfield * n, sizeof(double));
     double *q = pool;
     st->n = n;
@@ -1231,6 +1196,14 @@ static double *state_pool_new(long n, int Ns, or_state *st)
     st->eps = q + n;
     st->rk = q + 2 * n;
     st->reps = q + 3 * n;
+    q += 4 * n;
+    for (int a = 0; a < 3; ++a) {
+        st->a[a] = q;
+        st->ra[a] = q + 3 * n;
+        q += n;
+    }
+    q += 3 * n;
+    st->b = q;
     return pool;
 }
 
@@ -1264,10 +1237,11 @@ or_solver *or_create(const or_config *cfg)
         free(s);
         return NULL;
     }
-    /* turbulence: the shear and zero models, ideal or van der Waals mixture,
+    /* turbulence: the shear, zero and kEpsA models, ideal or van der Waals mixture,
      * the diffusion stage (if on) with the explicit solver and neither the
      * molar-mass correction nor the enthalpy flow */
-    if (cfg->turb && ((cfg->turb_model != OR_TURB_SHEAR && cfg->turb_model != OR_TURB_ZERO) ||
+    if (cfg->turb && ((cfg->turb_model != OR_TURB_SHEAR && cfg->turb_model != OR_TURB_ZERO &&
+                       cfg->turb_model != OR_TURB_KEPSA) ||
                       (cfg->gas_model != OR_GAS_IDEAL && cfg->gas_model != OR_GAS_VDW) ||
                       (cfg->diffusion && (cfg->sle_solver != OR_SLE_EXPLICIT || cfg->molmass ||
                                           cfg->enth_flow)))) {
@@ -1278,7 +1252,7 @@ or_solver *or_create(const or_config *cfg)
         or_destroy(s);
         return NULL;
     }
-    alloc_state(s);
+    s->pool = state_pool_new(s->nc, cfg->Ns, &s->st);
     return s;
 }
 
@@ -1347,6 +1321,9 @@ void or_init_primitive(or_solver *s, const double *c, const double *v,
             st->rk[i] = st->k[i] * r0;
             st->reps[i] = r0 * st->eps[i];
         }
+        if (s->cfg.turb && s->cfg.turb_model == OR_TURB_KEPSA)
+            for (int a = 0; a < 3; ++a) /* phaseInitialization.cpp:1850 */
+                st->ra[a][i] = st->a[a][i] * r0;
         double vel[3] = {st->v[0][i], st->v[1][i], st->v[2][i]};
         const double v2 = dot3(vel, vel);
         st->E[i] = (st->U[i] + r0 * v2 * 0.5) + st->rk[i];
@@ -1367,6 +1344,24 @@ void or_init_turbulence(or_solver *s, const double *k, const double *eps)
         st->k[i] = (k[i] < s->cfg.mink) ? s->cfg.mink : k[i];
         st->eps[i] = (eps[i] < s->cfg.mineps) ? s->cfg.mineps : eps[i];
     }
+}
+
+void or_init_turbulence_a(or_solver *s, const double *a)
+{
+    or_state *st = &s->st;
+    const long n = s->nc;
+    const int on = s->cfg.turb && s->cfg.turb_model == OR_TURB_KEPSA;
+    for (int c = 0; c < 3; ++c)
+        for (long i = 0; i < n; ++i)
+            st->a[c][i] = on ? a[c * n + i] : 0.0;
+}
+
+void or_set_turbulence_a_conserved(or_solver *s, const double *ra)
+{
+    or_state *st = &s->st;
+    const long n = s->nc;
+    for (int c = 0; c < 3; ++c)
+        memcpy(st->ra[c], ra + c * n, n * sizeof(double));
 }
 
 void or_set_turbulence_conserved(or_solver *s, const double *rk, const double *reps)
@@ -1425,6 +1420,9 @@ static void recover_primitives(or_solver *s, or_state *st)
             st->k[i] = st->rk[i] / st->rho[0][i];
             st->eps[i] = st->reps[i] / st->rho[0][i];
         }
+        if (s->cfg.turb && s->cfg.turb_model == OR_TURB_KEPSA)
+            for (int a = 0; a < 3; ++a)
+                st->a[a][i] = st->ra[a][i] / st->rho[0][i];
     }
 }
 
@@ -1542,20 +1540,9 @@ static int bc_cell_vec(const or_bc *bc, const double *v, const double *N,
 /* gradient (gradient.hpp:19-58) with linearInterpolate                      */
 /* (linearInterpolate.hpp:214-251)                                           */
 
-static int grad_scalar(or_solver *s, const double *q, const or_bc *bcs,
-                       double *G /* 3 per cell */)
+/* gradient of a scalar from its face values (gradient.hpp:19-58) */
+static void grad_faces(const or_solver *s, const double *qf, double *G /* 3 per cell */)
 {
-    double *qf = malloc(s->nf * sizeof(double));
-    for (long f = 0; f < s->nf; ++f) {
-        if (s->fpatch[f] == -3)
-            continue;
-        if (s->fpatch[f] < 0)
-            qf[f] = q[s->own[f]] * s->wo[f] + q[s->nei[f]] * s->wn[f];
-        else if (bc_surf_scalar(&bcs[s->fpatch[f]], q[s->own[f]], &qf[f])) {
-            free(qf);
-            return ERR_BC;
-        }
-    }
     for (long i = 0; i < s->nc; ++i) {
         double acc[3] = {0, 0, 0};
         for (int j = 0; j < s->cnf[i]; ++j) {
@@ -1572,6 +1559,23 @@ static int grad_scalar(or_solver *s, const double *q, const or_bc *bcs,
         for (int a = 0; a < 3; ++a)
             G[3 * i + a] = acc[a] / s->cV[i];
     }
+}
+
+static int grad_scalar(or_solver *s, const double *q, const or_bc *bcs,
+                       double *G /* 3 per cell */)
+{
+    double *qf = malloc(s->nf * sizeof(double));
+    for (long f = 0; f < s->nf; ++f) {
+        if (s->fpatch[f] == -3)
+            continue;
+        if (s->fpatch[f] < 0)
+            qf[f] = q[s->own[f]] * s->wo[f] + q[s->nei[f]] * s->wn[f];
+        else if (bc_surf_scalar(&bcs[s->fpatch[f]], q[s->own[f]], &qf[f])) {
+            free(qf);
+            return ERR_BC;
+        }
+    }
+    grad_faces(s, qf, G);
     free(qf);
     return 0;
 }
@@ -1800,6 +1804,7 @@ typedef struct {
     double m[3];
     double p, U, T, E;
     double k, eps, rk, reps; /* turbulence (zero without it) */
+    double a[3], ra[3];      /* kEpsA turbulent mass flux (zero otherwise) */
 } face_state;
 
 /* Face-side closure: Advection2dOrder.cpp:233-289 (owner side) and 301-372
@@ -1817,6 +1822,8 @@ static void close_face(const or_config *cfg, face_state *fs)
     if (cfg->turb) { /* Advection2dOrder.cpp:249-266, 326-347, 440-458 */
         fs->rk = fs->k * r0;
         fs->reps = fs->eps * r0;
+        for (int a = 0; a < 3; ++a) /* Advection2dOrder.cpp:257-260, 336-340 */
+            fs->ra[a] = fs->a[a] * r0;
     }
     fs->U = eos_U(cfg, fs->c, fs->p);
     fs->T = eos_T(cfg, fs->c, fs->U);
@@ -1833,6 +1840,7 @@ typedef struct {
     double m[9];
     double E[3];
     double rk[3], reps[3]; /* turbulence */
+    double ra[9];          /* kEpsA: rho a as a tensor like the momentum */
 } face_flux;
 
 /* HLLCSolver::calculateFlows per-face body (HLLCSolver.cpp:52-388) */
@@ -1879,6 +1887,9 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
             F->rk[a] = (uO * O->rk) * N[a];
             F->reps[a] = (uO * N[a]) * O->reps;
         }
+        for (int c = 0; c < 3; ++c) /* HLLCSolver.cpp:145-146 */
+            for (int b = 0; b < 3; ++b)
+                F->ra[3 * c + b] = (O->ra[c] * uO) * N[b];
     } else if (Sc >= 0) {
         const double factor = SO - uO;
         const double den = 1 / (SO - Sc);
@@ -1910,6 +1921,12 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
         for (int a = 0; a < 3; ++a) {
             F->rk[a] = fk * N[a];
             F->reps[a] = fe * N[a];
+        }
+        for (int c = 0; c < 3; ++c) { /* HLLCSolver.cpp:175-176, 230-236 */
+            const double ras = O->ra[c] * factor * den;
+            const double fa = O->ra[c] * uO + SO * (ras - O->ra[c]);
+            for (int b = 0; b < 3; ++b)
+                F->ra[3 * c + b] = fa * N[b];
         }
     } else if (SN > 0) {
         const double factor = SN - uN;
@@ -1943,6 +1960,12 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
             F->rk[a] = fk * N[a];
             F->reps[a] = fe * N[a];
         }
+        for (int c = 0; c < 3; ++c) { /* HLLCSolver.cpp:270-271, 327-333 */
+            const double ras = Nb->ra[c] * factor * den;
+            const double fa = Nb->ra[c] * uN + SN * (ras - Nb->ra[c]);
+            for (int b = 0; b < 3; ++b)
+                F->ra[3 * c + b] = fa * N[b];
+        }
     } else if (SN <= 0) {
         for (int k = 0; k < nd; ++k)
             for (int a = 0; a < 3; ++a)
@@ -1960,6 +1983,9 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
             F->rk[a] = (uN * Nb->rk) * N[a];
             F->reps[a] = (uN * N[a]) * Nb->reps;
         }
+        for (int c = 0; c < 3; ++c) /* HLLCSolver.cpp:380-381 */
+            for (int b = 0; b < 3; ++b)
+                F->ra[3 * c + b] = (Nb->ra[c] * uN) * N[b];
     } else {
         return ERR_RIEMANN;
     }
@@ -2004,6 +2030,9 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
             F->rk[a] = (uO * O->rk) * N[a];
             F->reps[a] = (uO * O->reps) * N[a];
         }
+        for (int c = 0; c < 3; ++c) /* HLLSolver.cpp:117-118 */
+            for (int b = 0; b < 3; ++b)
+                F->ra[3 * c + b] = (O->ra[c] * uO) * N[b];
     } else if (SN > 0) {
         const double den = 1 / (SN - SO);
         for (int k = 0; k < nd; ++k) {
@@ -2036,6 +2065,13 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
             F->rk[a] = fk * N[a];
             F->reps[a] = fe * N[a];
         }
+        for (int c = 0; c < 3; ++c) { /* HLLSolver.cpp:159-162, 237-243 */
+            const double fa = (SN * uO * O->ra[c] - SO * uN * Nb->ra[c] +
+                               SN * SO * (Nb->ra[c] - O->ra[c])) *
+                              den;
+            for (int b = 0; b < 3; ++b)
+                F->ra[3 * c + b] = fa * N[b];
+        }
     } else if (SN <= 0) {
         for (int k = 0; k < nd; ++k)
             for (int a = 0; a < 3; ++a)
@@ -2053,6 +2089,9 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
             F->rk[a] = (uN * Nb->rk) * N[a];
             F->reps[a] = (uN * Nb->reps) * N[a];
         }
+        for (int c = 0; c < 3; ++c) /* HLLSolver.cpp:286-287 */
+            for (int b = 0; b < 3; ++b)
+                F->ra[3 * c + b] = (Nb->ra[c] * uN) * N[b];
     } else {
         return ERR_RIEMANN;
     }
@@ -2217,9 +2256,12 @@ static int stage_impl(or_solver *s, double dt, int hancock)
     double *Lv = malloc(9 * nc * sizeof(double));
     double *Gp = malloc(3 * nc * sizeof(double));
     double *Lp = malloc(3 * nc * sizeof(double));
-    /* turbulence: limited gradients of k and epsilon */
+    /* turbulence: limited gradients of k and epsilon, and of a (kEpsA) */
+    const int afield = cfg->turb && cfg->turb_model == OR_TURB_KEPSA;
     double *Gt = calloc(3 * nc * 2, sizeof(double));
     double *Lt = calloc(3 * nc * 2, sizeof(double));
+    double *Ga = calloc(afield ? 9 * nc : 1, sizeof(double));
+    double *La = calloc(afield ? 9 * nc : 1, sizeof(double));
     face_state *FO = calloc(nf, sizeof(face_state));
     face_state *FN = calloc(nf, sizeof(face_state));
     face_flux *FF = calloc(nf, sizeof(face_flux));
@@ -2248,6 +2290,11 @@ static int stage_impl(or_solver *s, double dt, int hancock)
         if (!err)
             err = tvd_scalar(s, &Gt[3 * nc], st->eps, cfg->bc_eps, &Lt[3 * nc]);
     }
+    if (!err && afield) { /* Advection2dOrder.cpp:91-97 */
+        err = grad_vector(s, st->a, cfg->bc_a, Ga);
+        if (!err)
+            err = tvd_vector(s, Ga, st->a, cfg->bc_a, La);
+    }
     if (err)
         goto done;
 
@@ -2268,6 +2315,11 @@ static int stage_impl(or_solver *s, double dt, int hancock)
                 fs->k = dot3(&Lt[3 * i], d) + st->k[i];
                 fs->eps = dot3(&Lt[3 * nc + 3 * i], d) + st->eps[i];
             }
+            if (afield) /* Advection2dOrder.cpp:151-155 */
+                for (int a = 0; a < 3; ++a) {
+                    const double *T = &La[9 * i + 3 * a];
+                    fs->a[a] = (T[0] * d[0] + T[1] * d[1] + T[2] * d[2]) + st->a[a][i];
+                }
         }
     }
     /* owner side everywhere, neighbour side on inner faces */
@@ -2296,6 +2348,8 @@ static int stage_impl(or_solver *s, double dt, int hancock)
             err = bc_cell_scalar(&cfg->bc_k[pt], FO[f].k, &g->k);
             if (!err)
                 err = bc_cell_scalar(&cfg->bc_eps[pt], FO[f].eps, &g->eps);
+            if (!err && afield) /* Advection2dOrder.cpp:460-470 */
+                err = bc_cell_vec(&cfg->bc_a[pt], FO[f].a, &s->fN[3 * f], g->a);
         }
         if (!err)
             close_face(cfg, g);
@@ -2314,6 +2368,7 @@ static int stage_impl(or_solver *s, double dt, int hancock)
      * (divergence.hpp:315-350) */
     for (long i = 0; i < nc; ++i) {
         double accE = 0, accm[3] = {0, 0, 0}, accr[OR_MAXS + 1] = {0}, acck = 0, acce = 0;
+        double acca[3] = {0, 0, 0};
         for (int j = 0; j < s->cnf[i]; ++j) {
             const long f = s->cf[6 * i + j];
             double n[3];
@@ -2332,6 +2387,10 @@ static int stage_impl(or_solver *s, double dt, int hancock)
                 accr[k] += dot3(FF[f].rho[k], n) * S;
             acck += dot3(FF[f].rk, n) * S;
             acce += dot3(FF[f].reps, n) * S;
+            for (int a = 0; a < 3; ++a) {
+                const double *T = &FF[f].ra[3 * a];
+                acca[a] += (T[0] * n[0] + T[1] * n[1] + T[2] * n[2]) * S;
+            }
         }
         const double V = s->cV[i];
         const double r0 = st->rho[0][i];
@@ -2350,6 +2409,9 @@ static int stage_impl(or_solver *s, double dt, int hancock)
             st->rk[i] = st->rk[i] - (acck / V) * dt;
             st->reps[i] = st->reps[i] - (acce / V) * dt;
         }
+        if (afield) /* Advection2dOrder.cpp:567-570 */
+            for (int a = 0; a < 3; ++a)
+                st->ra[a][i] = st->ra[a][i] - (acca[a] / V) * dt;
     }
     recover_primitives(s, st);
 
@@ -2362,6 +2424,8 @@ done:
     free(Lp);
     free(Gt);
     free(Lt);
+    free(Ga);
+    free(La);
     free(FO);
     free(FN);
     free(FF);
@@ -2386,11 +2450,16 @@ static void rk_average(or_solver *s, const or_state *Un)
         st->rk[i] = w * st->rk[i] + iw * Un->rk[i];
         st->reps[i] = w * st->reps[i] + iw * Un->reps[i];
         for (int a = 0; a < 3; ++a)
+            st->ra[a][i] = st->ra[a][i] * w + Un->ra[a][i] * iw;
+        for (int a = 0; a < 3; ++a)
             st->v[a][i] = st->m[a][i] / st->rho[0][i];
         if (s->cfg.turb) {
             st->k[i] = st->rk[i] / st->rho[0][i];
             st->eps[i] = st->reps[i] / st->rho[0][i];
         }
+        if (s->cfg.turb && s->cfg.turb_model == OR_TURB_KEPSA)
+            for (int a = 0; a < 3; ++a)
+                st->a[a][i] = st->ra[a][i] / st->rho[0][i];
         double vel[3] = {st->v[0][i], st->v[1][i], st->v[2][i]};
         st->U[i] = st->E[i] - st->rk[i] - 0.5 * st->rho[0][i] * dot3(vel, vel);
         for (int k = 0; k <= Ns; ++k)
@@ -3024,6 +3093,19 @@ static int diffusion_explicit(or_solver *s, double dt)
     double *FKt = calloc(3 * tf, sizeof(double));
     double *FEt = calloc(3 * tf, sizeof(double));
     double *FNI = calloc(3 * tf, sizeof(double));
+    /* kEpsA (Diffusion.cpp:176-191, 288-305, 511-538, 574-575, 736-800,
+     * 869-887; kEpsAModel.cpp:132-257): the cells' b, grad a, grad p and
+     * grad rho, the divergence of the molecular viscous stress at the faces,
+     * and the face fluxes of the a Laplacian */
+    const int afield = turb && cfg->turb_model == OR_TURB_KEPSA;
+    const long an = afield ? nc : 1, af = afield ? nf : 1;
+    double *bC = malloc(an * sizeof(double));
+    double *Ga = malloc(9 * an * sizeof(double));
+    double *GP = malloc(3 * an * sizeof(double));
+    double *GR = malloc(3 * an * sizeof(double));
+    double *rhof = malloc(af * sizeof(double));
+    double *FA = calloc(9 * af, sizeof(double));
+    double *FDP = calloc(9 * af, sizeof(double));
 
     /* mass fractions rho_k / rho, normalised (diffusiveFields.cpp:30-47) */
     for (long i = 0; i < nc; ++i) {
@@ -3103,6 +3185,39 @@ static int diffusion_explicit(or_solver *s, double dt)
     }
     if (err)
         goto done;
+    if (afield) {
+        /* b = rho sum(x_k / (rho_k + chi rho)) / sum(x_k rho_k / (rho_k +
+         * chi rho)) - 1, clipped at 0 (kEpsAModel.cpp:232-257).  The source
+         * adds the stabilizer to rho_k as field + scalar, which returns the
+         * field unchanged (fieldOperations.hpp:33-43), so it does not enter. */
+        const double chi = cfg->tC[7];
+        for (long i = 0; i < nc; ++i) {
+            double num = 0, den = 0;
+            for (int k = 1; k <= Ns; ++k) {
+                const double x = st->c[k][i] / st->c[0][i];
+                const double q = st->rho[k][i] + chi * st->rho[0][i];
+                num += x / q;
+                den += x * st->rho[k][i] / q;
+            }
+            const double b = st->rho[0][i] * num / den - 1.;
+            bC[i] = (b < 0.) ? 0.0 : b;
+        }
+        /* grad a, grad p (Diffusion.cpp:183-184, 531) and grad rho with the
+         * face density sum_k c_k,f M_k on boundary faces (calculatedDensity) */
+        err = grad_vector(s, st->a, cfg->bc_a, Ga);
+        if (!err)
+            err = grad_scalar(s, st->p, cfg->bc_p, GP);
+        if (err)
+            goto done;
+        for (long f = 0; f < nf; ++f) {
+            if (s->fpatch[f] == -3)
+                continue;
+            rhof[f] = (s->fpatch[f] < 0)
+                          ? st->rho[0][s->own[f]] * s->wo[f] + st->rho[0][s->nei[f]] * s->wn[f]
+                          : srho[f];
+        }
+        grad_faces(s, rhof, GR);
+    }
     /* grad v and div v of the cells (gradient.hpp:19-58,
      * divergence.hpp:19-58) */
     for (long i = 0; i < nc; ++i) {
@@ -3269,6 +3384,29 @@ static int diffusion_explicit(or_solver *s, double dt)
                 FNI[3 * f + b] = flow * srho[f] * tLam;
             }
         }
+        if (afield) {
+            /* molecular viscous stress (surfGrad v + (surfGrad v)^T - 2/3
+             * div_s v I) mu (Diffusion.cpp:210-222, 574) */
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    double t = g[3 * a + b] + g[3 * b + a];
+                    if (a == b)
+                        t -= lam;
+                    FDP[9 * f + 3 * a + b] = t * cfg->mu;
+                }
+            /* a Laplacian with mu + rho_f nu_t / sigma_a
+             * (transportCoefficients.hpp:81, 242; SLEMatrix.cpp:2080-2110) */
+            const double rhoDa = cfg->mu + srho[f] * (tnu / cfg->sigmaa);
+            double ao[3] = {st->a[0][o], st->a[1][o], st->a[2][o]};
+            double ag[3] = {st->a[0][n], st->a[1][n], st->a[2][n]}, ga[9];
+            if (bnd && bc_cell_vec(&cfg->bc_a[s->fpatch[f]], ao, &s->fN[3 * f], ag)) {
+                err = ERR_BC;
+                goto done;
+            }
+            surf_grad(s, f, 3, ao, ag, ga);
+            for (int q = 0; q < 9; ++q)
+                FA[9 * f + q] = (ga[q] * rhoDa) * -1;
+        }
     }
     if (cfg->molmass || cfg->enth_flow) {
         err = species_corrections(s, FY, CSf, TSf, MC, EF);
@@ -3290,7 +3428,7 @@ static int diffusion_explicit(or_solver *s, double dt)
         for (long i = 0; i < nc; ++i)
             epsmax = (i == 0 || epsmax < st->eps[i]) ? st->eps[i] : epsmax;
     for (long i = 0; i < nc; ++i) {
-        double dY[OR_MAXS] = {0}, dT = 0, dV[3] = {0}, dS[3] = {0};
+        double dY[OR_MAXS] = {0}, dT = 0, dV[3] = {0}, dS[3] = {0}, dA[3] = {0}, dP[3] = {0};
         for (int j = 0; j < s->cnf[i]; ++j) {
             const long f = s->cf[6 * i + j];
             double n[3];
@@ -3305,6 +3443,12 @@ static int diffusion_explicit(or_solver *s, double dt)
                 dV[a] += (Tv[0] * n[0] + Tv[1] * n[1] + Tv[2] * n[2]) * S;
                 dS[a] += (Ts[0] * n[0] + Ts[1] * n[1] + Ts[2] * n[2]) * S;
             }
+            if (afield)
+                for (int a = 0; a < 3; ++a) {
+                    const double *Ta = &FA[9 * f + 3 * a], *Tp = &FDP[9 * f + 3 * a];
+                    dA[a] += (Ta[0] * n[0] + Ta[1] * n[1] + Ta[2] * n[2]) * S;
+                    dP[a] += (Tp[0] * n[0] + Tp[1] * n[1] + Tp[2] * n[2]) * S;
+                }
         }
         const double V = s->cV[i], rho = st->rho[0][i];
         /* divergences of the molar-mass corrections and of the enthalpy
@@ -3408,7 +3552,7 @@ static int diffusion_explicit(or_solver *s, double dt)
             const double tnu = cfg->Cmu * k * k / eps;
             const double thR = theta_s(cfg, divV[i], k, eps, cfg->CMS_R);
             const double tnu1 = (1. - thR) * tnu;
-            double philt = 0, gsph = 0, gdev = 0;
+            double philt = 0, gsph = 0, gdev = 0, sph[9], dev[9];
             for (int q = 0; q < 9; ++q)
                 philt += (sv[q] * rho * tnu1) * G[q];
             free_t += philt;
@@ -3416,10 +3560,10 @@ static int diffusion_explicit(or_solver *s, double dt)
              * generation terms (Diffusion.cpp:592-597; shearModel.cpp:116-147) */
             for (int q = 0; q < 9; ++q) {
                 const double Iq = (q == 0 || q == 4 || q == 8) ? 1. : 0.;
-                const double sph = Iq * -(2. / 3.) * st->rk[i];
-                const double dev = sv[q] * rho * tnu * thR;
-                gsph += sph * G[q];
-                gdev += dev * G[q];
+                sph[q] = Iq * -(2. / 3.) * st->rk[i];
+                dev[q] = sv[q] * rho * tnu * thR;
+                gsph += sph[q] * G[q];
+                gdev += dev[q] * G[q];
             }
             double sk = 0, se = 0;
             if (cfg->turb_model == OR_TURB_SHEAR) {
@@ -3433,6 +3577,72 @@ static int diffusion_explicit(or_solver *s, double dt)
                 const double me = (eps < 1E-3 * epsmax) ? VERYBIG : eps;
                 const double r = fabs(cfg->ts_coeff * me / ((Se1 + Se2 * st->eps[i]) / rho + STAB));
                 tsrc = (r < tsrc) ? r : tsrc;
+            } else if (afield) {
+                /* kEpsAModel.cpp:170-225 with C0..C3, Ca, CaMax, CMSA in tC */
+                const double *ga = &Ga[9 * i], *gp = &GP[3 * i], *gr = &GR[3 * i];
+                const double ai[3] = {st->a[0][i], st->a[1][i], st->a[2][i]};
+                const double ek = eps / k;
+                /* thetaS_R of trace(grad v) (tensor.cpp:41-44) */
+                const double thS = theta_s(cfg, G[0] + G[4] + G[8], k, eps, cfg->CMS_R);
+                /* thetaA (kEpsAModel.cpp:16-27) */
+                double thA = 1;
+                if (cfg->tC[6] < 1E15) {
+                    double aa = 0;
+                    for (int c = 0; c < 3; ++c)
+                        aa += ai[c] * ai[c];
+                    thA = (aa / (2 * k) < cfg->tC[6] && bC[i] < cfg->tC[6] && aa / (2 * k) < bC[i])
+                              ? 1
+                              : cfg->tC[5] / cfg->tC[4];
+                }
+                double gdevA = 0, x[3], grav = 0;
+                for (int q = 0; q < 9; ++q)
+                    gdevA += (dev[q] * thS) * G[q];
+                /* a & (grad p - div(mu strainSurf)) */
+                for (int c = 0; c < 3; ++c)
+                    x[c] = gp[c] - dP[c] / V;
+                for (int c = 0; c < 3; ++c)
+                    grav += ai[c] * x[c];
+                const double dissip = -st->reps[i];
+                const double Sk1 = gsph + gdevA + grav, Sk2 = dissip / st->k[i];
+                const double Se1 =
+                    (cfg->tC[1] * gdevA + cfg->tC[3] * gsph + cfg->tC[0] * grav) * ek;
+                const double Se2 = cfg->tC[2] * dissip / st->k[i];
+                sk = Sk1 + Sk2 * k;
+                se = Se1 + Se2 * eps;
+                const double me = (eps < 1E-3 * epsmax) ? VERYBIG : eps;
+                const double r = fabs(cfg->ts_coeff * me / ((Se1 + Se2 * st->eps[i]) / rho + STAB));
+                tsrc = (r < tsrc) ? r : tsrc;
+                /* Sa1 = b (grad p - div(...)) + ((thetaS devR + spherR) / rho) &
+                 * grad rho + rho a & (grad a - grad v); Sa2 = -rho (eps/k) Ca
+                 * thetaA on every component (kEpsAModel.cpp:208-222), summed
+                 * as Sa1 + Sa2 a (Diffusion.cpp:749-762) */
+                double tau[9], D[9];
+                for (int q = 0; q < 9; ++q) {
+                    tau[q] = (dev[q] * thS + sph[q]) / rho;
+                    D[q] = ga[q] - G[q];
+                }
+                const double Sa2 = -rho * ek * cfg->tC[4] * thA;
+                double an[3];
+                for (int c = 0; c < 3; ++c) {
+                    const double bgp = x[c] * bC[i];
+                    const double tgr = tau[3 * c] * gr[0] + tau[3 * c + 1] * gr[1] + tau[3 * c + 2] * gr[2];
+                    const double ragv = D[c] * st->ra[0][i] + D[3 + c] * st->ra[1][i] +
+                                        D[6 + c] * st->ra[2][i];
+                    const double sig = bgp + tgr + ragv + Sa2 * ai[c];
+                    /* explicit Laplacian plus the source (Diffusion.cpp:869-887) */
+                    an[c] = (ai[c] * rho + (0. - dA[c] / V) * dt) / rho + sig / rho * dt;
+                }
+                /* gravEnSink into the temperature (Diffusion.cpp:789-799) */
+                const double src = grav * (-1.0);
+                if (src >= 0.)
+                    free_t += src;
+                else
+                    diag_t -= src / (st->T[i] + STAB);
+                for (int c = 0; c < 3; ++c) {
+                    st->a[c][i] = an[c];
+                    st->ra[c][i] = an[c] * rho;
+                }
+                st->b[i] = bC[i];
             }
             /* explicit Laplacians plus the sources (Diffusion.cpp:839-862),
              * minimal values (905-917) */
@@ -3501,6 +3711,13 @@ done:
     free(FKt);
     free(FEt);
     free(FNI);
+    free(bC);
+    free(Ga);
+    free(GP);
+    free(GR);
+    free(rhof);
+    free(FA);
+    free(FDP);
     return err;
 }
 

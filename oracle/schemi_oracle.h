@@ -100,24 +100,29 @@ typedef struct {
     int ts_source;
     double ts_coeff;
     /* k-epsilon turbulence (#Turbulence_flag on, #Type_of_source_flag):
-     * turb_model OR_TURB_SHEAR (shearModel.cpp) or OR_TURB_ZERO
+     * turb_model OR_TURB_SHEAR (shearModel.cpp), OR_TURB_ZERO
      * (zeroModel.cpp: transport and diffusion of k and epsilon, no
-     * sources); the parameters of set/turbulentParameters.txt
-     * (kEpsModels.cpp, abstractTurbulenceModel.hpp:26-57); tC the model
-     * constants (shear: C1, C2, C3); bc_k and bc_eps the boundary
-     * conditions of set/k.txt and set/epsilon.txt.  Turbulence needs the
-     * SSP-RK2 step; with the diffusion stage, the explicit matrix solver and
-     * no molar-mass correction or enthalpy flow. */
+     * sources) or OR_TURB_KEPSA (kEpsAModel.cpp: with the turbulent mass
+     * flux a, transported as rho a, and the algebraic b of calculate_b);
+     * the parameters of set/turbulentParameters.txt (kEpsModels.cpp,
+     * abstractTurbulenceModel.hpp:26-57; sigmaa for a); tC the model
+     * constants (shear: C1, C2, C3; kEpsA: C0, C1, C2, C3, Ca, CaMax, CMSA,
+     * chi); bc_k, bc_eps and bc_a the boundary conditions of set/k.txt,
+     * set/epsilon.txt and set/a.txt.  Turbulence needs the SSP-RK2 step;
+     * with the diffusion stage, the explicit matrix solver and no molar-mass
+     * correction or enthalpy flow. */
     int turb;
     int turb_model;
     double Cmu, sigmaSc, sigmaT, sigmaE, sigmaK, sigmaEps, CMS_R, CMS_D, mink, mineps;
-    double tC[4];
+    double tC[8];
     or_bc bc_k[6];
     or_bc bc_eps[6];
+    double sigmaa;
+    or_bc bc_a[6];
 } or_config;
 
 /* turbulenceModel (turbulenceModelEnum.hpp) values of the built models */
-enum { OR_TURB_ZERO = 0, OR_TURB_SHEAR = 2 };
+enum { OR_TURB_ZERO = 0, OR_TURB_SHEAR = 2, OR_TURB_KEPSA = 6 };
 
 enum { OR_SLE_EXPLICIT = 0, OR_SLE_JACOBI = 1, OR_SLE_GAUSS_SEIDEL = 2 };
 
@@ -136,8 +141,9 @@ typedef struct {
     double *m[3];
     double *E;
     /* turbulence: k, epsilon and the conserved rho k, rho epsilon (zero
-     * without turbulence) */
+     * without turbulence); kEpsA: a, rho a and b (zero otherwise) */
     double *k, *eps, *rk, *reps;
+    double *a[3], *ra[3], *b;
 } or_state;
 
 or_solver *or_create(const or_config *cfg);
@@ -154,6 +160,11 @@ void or_init_primitive(or_solver *s, const double *c, const double *v,
  * clipped to the minimal values and multiplied by the density as the
  * reference's initialisation does (phaseInitialization.cpp:1814-1848, 1873-1877). */
 void or_init_turbulence(or_solver *s, const double *k, const double *eps);
+/* kEpsA: the initial a (3 n, component-major; multiplied by the density by
+ * or_init_primitive, phaseInitialization.cpp:1850) and the conserved rho a
+ * for or_set_conserved (call either before those). */
+void or_init_turbulence_a(or_solver *s, const double *a);
+void or_set_turbulence_a_conserved(or_solver *s, const double *ra);
 /* Conserved turbulent fields rho k, rho epsilon for or_set_conserved (call
  * it first). */
 void or_set_turbulence_conserved(or_solver *s, const double *rk, const double *reps);

@@ -38,9 +38,13 @@ def _free6() -> List[BC]:
 class Turbulence:
     """k-epsilon turbulence (``#Turbulence_flag on``): the model of
     ``#Type_of_source_flag`` (``shear``: shearModel.cpp; ``zero``:
-    zeroModel.cpp), the parameters of ``set/turbulentParameters.txt`` in the
-    order the k-epsilon models read them (kEpsModels.cpp, shearModel.cpp:35-59),
-    and the boundary conditions of ``set/k.txt`` and ``set/epsilon.txt``."""
+    zeroModel.cpp; ``kEpsA``: kEpsAModel.cpp, with the turbulent mass flux a
+    transported as rho a and the algebraic b), the parameters of
+    ``set/turbulentParameters.txt`` in the order the k-epsilon models read
+    them (kEpsModels.cpp, shearModel.cpp:35-59, kEpsAModel.cpp:48-80), and
+    the boundary conditions of ``set/k.txt``, ``set/epsilon.txt`` and (kEpsA)
+    ``set/a.txt``.  ``C`` holds the model constants: C1, C2, C3 for the shear
+    model; C0, C1, C2, C3, Ca, CaMax, CMSA, chi for kEpsA (:func:`kEpsA`)."""
 
     model: turbulenceModel = turbulenceModel.shearModel
     Cmu: float = 0.09
@@ -56,10 +60,25 @@ class Turbulence:
     C: Tuple[float, ...] = (1.44, 1.92, 2.0)  # shear: C1, C2, C3
     bc_k: List[BC] = field(default_factory=_free6)
     bc_eps: List[BC] = field(default_factory=_free6)
+    sigmaa: float = 1.4       # sigma_a (kEpsA)
+    bc_a: List[BC] = field(default_factory=_free6)
 
     def params(self):
         return (self.Cmu, self.sigmaSc, self.sigmaT, self.sigmaE, self.sigmaK,
                 self.sigmaEps, self.CMS_R, self.CMS_D, self.mink, self.mineps)
+
+    @property
+    def a_field(self) -> bool:
+        """The model transports the turbulent mass flux a (kEpsA)."""
+        return self.model == turbulenceModel.kEpsAModel
+
+    @classmethod
+    def kEpsA(cls, C0=0.92, C1=1.44, C2=1.92, C3=2.0, Ca=2.5, CaMax=5.0, CMSA=1.5,
+              chi=1.0, **kw) -> "Turbulence":
+        """The kEpsA model with its eight constants (kEpsAModel.cpp:86-121);
+        CMSA >= 1e15 switches the thetaA limiter off."""
+        return cls(model=turbulenceModel.kEpsAModel,
+                   C=(C0, C1, C2, C3, Ca, CaMax, CMSA, chi), **kw)
 
 
 @dataclass
@@ -147,8 +166,9 @@ class Case:
     @property
     def nfields(self) -> int:
         """Conserved fields per cell: rho_1..rho_Ns, rho, m (3), E, and with
-        turbulence rho k, rho epsilon."""
-        return self.Ns + 5 + (2 if self.turbulence is not None else 0)
+        turbulence rho k, rho epsilon, and with the kEpsA model rho a (3)."""
+        t = self.turbulence
+        return self.Ns + 5 + (0 if t is None else 5 if t.a_field else 2)
 
     @property
     def ncells(self) -> int:

@@ -3,7 +3,7 @@
 Run in the development container only (it needs oracle/_ref/schemi, built by
 ``make -C oracle ref`` from the reference sources):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [group]
 
 Each fixture holds the case description, the time steps, the reference's
 initial state and its full-precision state after selected steps, parsed from
@@ -60,6 +60,9 @@ def case_json(case: Case) -> str:
             "model": int(t.model), "params": list(t.params()), "C": list(t.C),
             "bc_k": [[int(b.type), list(b.value)] for b in t.bc_k],
             "bc_eps": [[int(b.type), list(b.value)] for b in t.bc_eps]}
+        if t.a_field:
+            eos["turbulence"].update(sigmaa=t.sigmaa,
+                                     bc_a=[[int(b.type), list(b.value)] for b in t.bc_a])
     return json.dumps({**eos,
         "dim": case.dim, "N": list(case.N), "L": list(case.L), "M": list(case.M),
         "Cv": list(case.Cv), "R": case.R, "flux": int(case.flux),
@@ -92,7 +95,16 @@ def seeded_turbulence(case: Case, seed: int):
     return 1e-3 + 5e-3 * r.random(n), 1e-2 + 5e-2 * r.random(n)
 
 
-def save(name, case, dts, snaps, times=None, restart=None, restart_turb=None):
+def seeded_a(case: Case, seed: int):
+    """Seeded turbulent mass flux a (3, n): components up to 0.1 in the
+    mesh's directions, so that a.a / 2k straddles b and CMSA."""
+    r = np.random.default_rng(seed + 1000)
+    a = 0.1 * (r.random((3, case.ncells)) - 0.5)
+    a[case.dim:] = 0
+    return a
+
+
+def save(name, case, dts, snaps, times=None, restart=None, restart_turb=None, restart_a=None):
     out = {"case": np.array(case_json(case)), "dts": np.asarray(dts, dtype=np.float64),
            "steps": np.array(sorted(snaps), dtype=np.int64)}
     if times is not None:
@@ -102,7 +114,11 @@ def save(name, case, dts, snaps, times=None, restart=None, restart_turb=None):
         out["init_c"], out["init_v"], out["init_p"] = (np.asarray(x) for x in restart)
     if restart_turb is not None:
         out["init_k"], out["init_eps"] = (np.asarray(x) for x in restart_turb)
+    if restart_a is not None:
+        out["init_a"] = np.asarray(restart_a)
     keys = KEYS + (("k", "eps") if case.turbulence is not None else ())
+    if case.turbulence is not None and case.turbulence.a_field:
+        keys += ("a", "b")
     for s, st in snaps.items():
         for k in keys:
             out[f"s{s}_{k}"] = np.asarray(st[k], dtype=np.float64)
@@ -111,13 +127,14 @@ def save(name, case, dts, snaps, times=None, restart=None, restart_turb=None):
     print(f"wrote {path} ({os.path.getsize(path) // 1024} KiB)")
 
 
-def run_const(name, case, nsteps, dt, keep, zones=None, restart=None, restart_turb=None):
+def run_const(name, case, nsteps, dt, keep, zones=None, restart=None, restart_turb=None,
+              restart_a=None):
     t = 0.0
     for _ in range(nsteps):
         t = t + dt
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as root:
         decks.write_deck(root, case, t_end=t, zones=zones, restart=restart, const_dt=dt,
-                         restart_turb=restart_turb)
+                         restart_turb=restart_turb, restart_a=restart_a)
         r = decks.run(root)
         if r.returncode != 0:
             raise RuntimeError(f"{name}: reference failed\n{r.stdout[-2000:]}\n{r.stderr}")
@@ -127,7 +144,8 @@ def run_const(name, case, nsteps, dt, keep, zones=None, restart=None, restart_tu
             snaps[0] = decks.read_output(root, 0, case.Ns)
         for s in keep:
             snaps[s] = decks.read_output(root, s + off, case.Ns)
-    save(name, case, [dt] * nsteps, snaps, restart=restart, restart_turb=restart_turb)
+    save(name, case, [dt] * nsteps, snaps, restart=restart, restart_turb=restart_turb,
+         restart_a=restart_a)
 
 
 def run_cfl_shock_tube(nsteps=200, keep=(1, 2, 10, 200)):
@@ -548,6 +566,56 @@ def turbulence_cases(base, nsteps=4, keep=(1, 4)):
                    turbulence=Turbulence(model=turbulenceModel.zeroModel), **base, **tr)), 206)
 
 
+def kepsa_cases(base, nsteps=4, keep=(1, 4)):
+    """kEpsA (kEpsAModel.cpp): rho a through the HLLC/HLL flux like rho k,
+    vector TVD slopes of a in 1-D, and in the diffusion stage the algebraic b
+    (calculate_b), grad a, grad p, grad rho, the molecular viscous stress
+    divergence, the a Laplacian with mu + rho nu_t / sigma_a, the sources of
+    k, epsilon and a with the thetaA limiter, and the a . (grad p - div
+    tau) energy sink (Diffusion.cpp:129-1029)."""
+    from schemi_amd.case import Turbulence
+    tr = dict(diffusion=True, transport=(2e-3, 2e-3, 2e-3))
+    vdw = dict(gas_model=gasModel.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+
+    def slip_y(c):
+        for p in (3, 4):
+            c.set_bc(p, BCT.slip)
+            c.turbulence.bc_a[p] = BC(BCT.slip)
+        return c
+
+    def go(name, c, seed, n=nsteps, kp=keep):
+        run_const(name, c, n, 2e-4 if c.dim > 1 else 1e-3, kp,
+                  restart=seeded_primitives(c, seed), restart_turb=seeded_turbulence(c, seed),
+                  restart_a=seeded_a(c, seed))
+
+    # advection only: rho a transported with the flux of rho k
+    go("seeded_2d_kepsa_advection_hllc",
+       slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), limiter=TL.minmod,
+                   turbulence=Turbulence.kEpsA(), **base)), 211)
+    # with the diffusion stage, HLLC, ideal mixture
+    go("seeded_2d_kepsa_hllc",
+       slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), limiter=TL.minmod,
+                   turbulence=Turbulence.kEpsA(), **base, **tr)), 212)
+    # HLL, van der Waals mixture, thetaA off (CMSA >= 1e15), chi = 0
+    go("seeded_2d_kepsa_vdw_hll",
+       slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), flux=FS.HLL, limiter=TL.minmod,
+                   turbulence=Turbulence.kEpsA(CMSA=1e16, chi=0.0, sigmaE=0.9),
+                   **base, **tr, **vdw)), 213)
+    # 1-D van Leer (vector slopes of a), fixed-value and slip ends
+    c = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer,
+             turbulence=Turbulence.kEpsA(), **base, **tr)
+    c.set_bc(0, BCT.fixedValueCell, "all", (0.03, 0.0, 0.0))
+    c.set_bc(1, BCT.slip)
+    c.turbulence.bc_k[0] = BC(BCT.fixedValueCell, (2e-3, 0.0, 0.0))
+    c.turbulence.bc_a[0] = BC(BCT.fixedValueCell, (0.02, 0.0, 0.0))
+    c.turbulence.bc_a[1] = BC(BCT.slip)
+    go("seeded_1d_kepsa_vanleer", c, 214)
+    # 3-D van Albada with Lz != Ly
+    go("seeded_3d_kepsa_vanalbada",
+       Case(dim=3, N=(8, 6, 5), L=(1.0, 0.8, 0.6), limiter=TL.vanAlbada,
+            turbulence=Turbulence.kEpsA(), **base, **tr), 215, n=3, kp=(1, 3))
+
+
 def turbulence_source_case(base, nsteps=6, keep=(1, 3, 6)):
     """"Courant-Source" with the shear model's source step (shearModel.cpp:
     107-150): epsilon over its growth rate caps the next Courant step.  The
@@ -611,5 +679,7 @@ if __name__ == "__main__":
     elif len(sys.argv) > 1 and sys.argv[1] == "turbulence":
         turbulence_cases(b)
         turbulence_source_case(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "kepsa":
+        kepsa_cases(b)
     else:
         main()

@@ -39,6 +39,8 @@ def load(name):
                 source_coeffs=tuple(d.get("source_coeffs", (0.25, 0.5))),
                 turbulence=_turbulence(d.get("turbulence")))
     keys = KEYS + (("k", "eps") if case.turbulence is not None else ())
+    if case.turbulence is not None and case.turbulence.a_field:
+        keys += ("a", "b")
     snaps = {int(s): {k: z[f"s{s}_{k}"] for k in keys} for s in z["steps"]}
     out = {"case": case, "dts": z["dts"], "snaps": snaps}
     if "times" in z:
@@ -47,6 +49,8 @@ def load(name):
         out["restart"] = (z["init_c"], z["init_v"], z["init_p"])
     if "init_k" in z:
         out["restart_turb"] = (z["init_k"], z["init_eps"])
+    if "init_a" in z:
+        out["restart_a"] = z["init_a"]
     return out
 
 
@@ -59,7 +63,9 @@ def _turbulence(t):
                       sigmaK=p[4], sigmaEps=p[5], CMS_R=p[6], CMS_D=p[7], mink=p[8],
                       mineps=p[9], C=tuple(t["C"]),
                       bc_k=[BC(ty, tuple(v)) for ty, v in t["bc_k"]],
-                      bc_eps=[BC(ty, tuple(v)) for ty, v in t["bc_eps"]])
+                      bc_eps=[BC(ty, tuple(v)) for ty, v in t["bc_eps"]],
+                      sigmaa=t.get("sigmaa", 1.4),
+                      bc_a=[BC(ty, tuple(v)) for ty, v in t.get("bc_a", [[1, [0, 0, 0]]] * 6)])
 
 
 def conserved(st):

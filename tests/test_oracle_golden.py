@@ -16,12 +16,15 @@ from tests.golden_io import KEYS, load, names
 
 def start(g):
     """Oracle at the fixture's initial state (restart fields, with k and
-    epsilon when the case is turbulent, or the reference's output 0)."""
+    epsilon when the case is turbulent and a with the kEpsA model, or the
+    reference's output 0)."""
     o = Oracle(g["case"])
     if "restart" in g:
         kw = {}
         if "restart_turb" in g:
             kw = dict(k=g["restart_turb"][0], eps=g["restart_turb"][1])
+        if "restart_a" in g:
+            kw["a"] = g["restart_a"]
         o.init_primitive(*g["restart"], **kw)
     else:
         o.set_full(g["snaps"][0])

@@ -1,12 +1,14 @@
 """The oracle's k-epsilon turbulence beyond the reference fixtures.
 
-The fixtures (tests/golden/seeded_*_turb_*) pin the oracle to the reference
-bit for bit; these tests show that each turbulent term is exercised by them
-(a run without it leaves the fixture) and check what the mathematics fixes:
-on a uniform state at rest the shear model's diffusion stage is one explicit
-Euler step of dk/dt = -eps, deps/dt = -C2 eps^2 / k with the dissipated
-energy moved into the internal energy, and with negligible k the turbulent
-run is the laminar one.
+The fixtures (tests/golden/seeded_*_turb_*, seeded_*_kepsa_*) pin the oracle
+to the reference bit for bit; these tests show that each turbulent term is
+exercised by them (a run without it leaves the fixture) and check what the
+mathematics fixes: on a uniform state at rest the shear model's diffusion
+stage is one explicit Euler step of dk/dt = -eps, deps/dt = -C2 eps^2 / k
+with the dissipated energy moved into the internal energy; the kEpsA model's
+a decays there at the rate (eps/k) Ca thetaA with the algebraic b of the
+mixture in closed form; and with negligible k the turbulent run is the
+laminar one.
 """
 import dataclasses
 
@@ -25,6 +27,8 @@ def run(g, case=None, steps=None):
     case = case or g["case"]
     o = Oracle(case)
     kw = dict(k=g["restart_turb"][0], eps=g["restart_turb"][1]) if case.turbulence else {}
+    if case.turbulence is not None and case.turbulence.a_field:
+        kw["a"] = g["restart_a"]
     o.init_primitive(*g["restart"], **kw)
     last = max(g["snaps"]) if steps is None else steps
     for dt in g["dts"][:last]:
@@ -61,6 +65,23 @@ def replace_turb(case, **kw):
     # 1-D limited slopes of k and epsilon, fixed-value and slip ends
     ("seeded_1d_turb_shear_vanleer", dict(Cmu=0.05), "k", 1e-6),
     ("seeded_3d_turb_shear_vanalbada", dict(model=TM.zeroModel), "eps", 1e-3),
+    # kEpsA: rho a and its fixed-value boundary value (1-D, van Leer), and
+    # the energy sink / k source a . (grad p - div tau) with C0 in the
+    # epsilon source
+    ("seeded_1d_kepsa_vanleer", dict(bc_a=[BC(BCT.fixedValueCell, (0.04, 0.0, 0.0))] +
+                                     [BC(BCT.slip)] + [BC() for _ in range(4)]), "a", 1e-6),
+    ("seeded_2d_kepsa_hllc", dict(C=(0.5, 1.44, 1.92, 2.0, 2.5, 5.0, 1.5, 1.0)), "eps", 1e-6),
+    # the a source: Ca (cells with thetaA = 1), the thetaA limiter (CaMax /
+    # Ca where the limiter is on), chi of the algebraic b
+    ("seeded_2d_kepsa_hllc", dict(C=(0.92, 1.44, 1.92, 2.0, 2.0, 5.0, 1.5, 1.0)), "a", 1e-4),
+    ("seeded_2d_kepsa_hllc", dict(C=(0.92, 1.44, 1.92, 2.0, 2.5, 5.0, 1e16, 1.0)), "a", 1e-4),
+    ("seeded_2d_kepsa_hllc", dict(C=(0.92, 1.44, 1.92, 2.0, 2.5, 5.0, 1.5, 0.5)), "b", 1e-3),
+    ("seeded_2d_kepsa_vdw_hll", dict(C=(0.92, 1.44, 1.92, 2.0, 2.5, 5.0, 1e16, 0.3)), "a", 1e-6),
+    # the a Laplacian (sigma_a) and the thetaS factor of the deviatoric
+    # stress in the a and k sources
+    ("seeded_2d_kepsa_hllc", dict(sigmaa=1e30), "a", 1e-6),
+    ("seeded_3d_kepsa_vanalbada", dict(CMS_R=10.0), "a", 1e-7),
+    ("seeded_1d_kepsa_vanleer", dict(sigmaa=0.5), "a", 1e-6),
 ])
 def test_fixture_exercises_term(name, change, field, least):
     g = load(name)
@@ -130,9 +151,67 @@ def test_negligible_k_is_laminar():
         np.testing.assert_allclose(sb[key], sa[key], rtol=1e-13, atol=1e-15 * np.abs(sa[key]).max())
 
 
+def kepsa_uniform(CMSA, chi=0.0):
+    case = uniform_case()
+    t = Turbulence.kEpsA(CMSA=CMSA, chi=chi)
+    for p in range(6):
+        t.bc_k[p] = t.bc_eps[p] = t.bc_a[p] = BC(BCT.periodic)
+    return dataclasses.replace(case, turbulence=t)
+
+
+@pytest.mark.parametrize("CMSA,limited", [(0.5, True), (10.0, False), (1e16, False)])
+def test_kepsa_uniform_state(CMSA, limited):
+    """kEpsA on a uniform periodic state at rest with a uniform a: every
+    gradient vanishes, so k and epsilon decay as in the shear model, the
+    algebraic b is M_mean (1/M_1 + 1/M_2) - 1 for equal concentrations and
+    chi = 0 (x_k = 1/2, rho_k = c M_k), and a decays by one explicit Euler
+    step of da/dt = -(eps/k) Ca thetaA a, thetaA = 1 where a.a/2k < b < CMSA
+    and CaMax / Ca otherwise (the limiter off for CMSA >= 1e15)."""
+    case = kepsa_uniform(CMSA)
+    n = case.ncells
+    o = Oracle(case)
+    k0, e0, a0 = 2e-3, 5e-2, np.array([0.03, -0.02, 0.0])
+    o.init_primitive(np.full((2, n), 0.02), np.zeros((3, n)), np.full(n, 1.0),
+                     k=np.full(n, k0), eps=np.full(n, e0), a=np.repeat(a0[:, None], n, axis=1))
+    st0 = o.state()
+    dt = 1e-3
+    assert o.step(dt, 0) == 0
+    st = o.state()
+    M = case.M
+    b = (M[0] + M[1]) / 2 * (1 / M[0] + 1 / M[1]) - 1
+    np.testing.assert_allclose(st["b"], b, rtol=1e-14)
+    C0, C1, C2, C3, Ca, CaMax, _, _ = case.turbulence.C
+    assert (a0 @ a0 / (2 * k0) < b < CMSA) != limited
+    theta = CaMax / Ca if limited else 1.0
+    np.testing.assert_allclose(st["k"], k0 - e0 * dt, rtol=1e-14)
+    np.testing.assert_allclose(st["eps"], e0 - C2 * e0 * e0 / k0 * dt, rtol=1e-14)
+    for c in range(3):
+        np.testing.assert_allclose(st["a"][c], a0[c] * (1 - e0 / k0 * Ca * theta * dt),
+                                   rtol=1e-14, atol=1e-300)
+        np.testing.assert_allclose(st["ra"][c], st["a"][c] * st0["rho0"], rtol=1e-15, atol=1e-300)
+    np.testing.assert_allclose(st["E"], st0["E"], rtol=1e-13)
+
+
+def test_kepsa_b_with_chi():
+    """b for chi > 0: rho sum x_k / (rho_k + chi rho) / sum x_k rho_k /
+    (rho_k + chi rho) - 1 on the uniform state, by hand for two species."""
+    case = kepsa_uniform(0.5, chi=1.0)
+    n = case.ncells
+    o = Oracle(case)
+    c1, c2 = 0.03, 0.01
+    o.init_primitive(np.stack([np.full(n, c1), np.full(n, c2)]), np.zeros((3, n)),
+                     np.full(n, 1.0), k=np.full(n, 2e-3), eps=np.full(n, 5e-2),
+                     a=np.zeros((3, n)))
+    assert o.step(1e-4, 0) == 0
+    r1, r2 = c1 * case.M[0], c2 * case.M[1]
+    rho, x1, x2 = r1 + r2, c1 / (c1 + c2), c2 / (c1 + c2)
+    b = rho * (x1 / (r1 + rho) + x2 / (r2 + rho)) / (x1 * r1 / (r1 + rho) + x2 * r2 / (r2 + rho)) - 1
+    np.testing.assert_allclose(o.state()["b"], b, rtol=1e-13)
+
+
 def test_unsupported_turbulence_configurations():
     base = uniform_case()
-    for bad in (dict(turbulence=Turbulence(model=TM.kEpsAModel)),
+    for bad in (dict(turbulence=Turbulence(model=TM.BHRModel)),
                 dict(sle_solver="Jacobi"), dict(molar_mass_correction=True),
                 dict(gas_model=2)):
         with pytest.raises(ValueError):

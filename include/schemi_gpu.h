@@ -211,12 +211,19 @@ typedef struct {
      * stage, the turbulent transport coefficients, the k and epsilon
      * Laplacians and the model's sources apply (Diffusion.cpp:129-933).
      * Models: SCHEMI_TURB_SHEAR (shearModel.cpp, its source step capping a
-     * "Courant-Source" step) and SCHEMI_TURB_ZERO (zeroModel.cpp).  The
+     * "Courant-Source" step), SCHEMI_TURB_ZERO (zeroModel.cpp) and
+     * SCHEMI_TURB_KEPSA (kEpsAModel.cpp: the state also carries rho a after
+     * rho epsilon, Ns + 10 fields; the algebraic b is recomputed from the
+     * mixture in every diffusion stage and is not part of the state; the
+     * source step caps "Courant-Source" as the shear model's).  The
      * parameters are those of set/turbulentParameters.txt in the models'
      * meaning (kEpsModels.cpp, shearModel.cpp:35-59): Cmu, turbulent Schmidt
      * and Prandtl numbers (T, e), sigma_k, sigma_epsilon, CMS_R, CMS_D, the
      * minimal k and epsilon, and the shear model's C1, C2, C3; bc_k, bc_eps
-     * the boundary conditions of set/k.txt and set/epsilon.txt.  Supported
+     * the boundary conditions of set/k.txt and set/epsilon.txt; for kEpsA
+     * also sigma_a, the eight constants of kEpsAModel.cpp:73-80 (C0, C1, C2,
+     * C3, Ca, CaMax, CMSA, chi; turb_C is not read) and the vector boundary
+     * conditions of set/a.txt.  Supported
      * with SecondOrderRK, the ideal or van der Waals mixture, and limiters
      * whose slopes vanish (2-D/3-D TVD limiters, the zero limiter) or 1-D
      * meshes with a TVD limiter; the diffusion stage (if on) with the
@@ -237,11 +244,15 @@ typedef struct {
     double turb_C[3];
     schemi_bc bc_k[6];
     schemi_bc bc_eps[6];
+    double turb_sigma_a;
+    double turb_kepsa_C[8];
+    schemi_bc bc_a[6];
 } schemi_scheme_desc;
 
 /* turbulenceModel (Schemi/enumerations/turbulenceModelEnum.hpp) values */
 #define SCHEMI_TURB_ZERO 0
 #define SCHEMI_TURB_SHEAR 2
+#define SCHEMI_TURB_KEPSA 6
 
 #define SCHEMI_SLE_GAUSS_SEIDEL 0
 #define SCHEMI_SLE_EXPLICIT 3
@@ -258,7 +269,8 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
 void schemi_gpu_destroy(schemi_gpu_ctx *ctx);
 
 /* Uploads the conserved state (Ns + 5 fields x ncells doubles; with
- * turbulence Ns + 7: rho k and rho epsilon after E). */
+ * turbulence Ns + 7: rho k and rho epsilon after E; with the kEpsA model
+ * Ns + 10: rho a_x, rho a_y, rho a_z after rho epsilon). */
 int schemi_gpu_set_state(schemi_gpu_ctx *ctx, const double *cons,
                          int64_t ncells, int32_t numbering);
 

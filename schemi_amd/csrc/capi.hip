@@ -221,8 +221,9 @@ struct schemi_gpu_ctx {
     std::vector<hipEvent_t> evd; // timed diffusion stages: start, between passes, end
     int sle_iters = 0;           // Jacobi solver: iteration limit (0: explicit solver)
     int molmass = 0, enthalpy = 0;
-    int nf = 0;                  // conserved fields: ns + 5, with turbulence ns + 7
+    int nf = 0;                  // conserved fields: ns + 5, turbulence ns + 7, kEpsA ns + 10
     bool turb = false;           // k-epsilon turbulence
+    bool afield = false;         // kEpsA: the state carries rho a
     TurbParams tp{};
     TurbParams *dTP = nullptr;
     unsigned long long *epsmax = nullptr;  // largest epsilon of the state (bits)
@@ -263,6 +264,7 @@ int launch_stage_turb(schemi_gpu_ctx *c, const double *Uin, double *Uout, const 
     L.variant = variant;
     L.gas_model = c->hp.gas_model;
     L.slopes = c->mode == MODE_SLOPES ? 1 : 0;
+    L.na = c->afield ? 3 : 0;
     L.n = c->n;
     L.stream = c->stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -393,6 +395,7 @@ int one_step(schemi_gpu_ctx *c)
             TurbDiffusion T;
             T.TP = c->dTP;
             T.epsmax = c->epsmax;
+            T.afield = c->afield ? 1 : 0;
             if (launch_diffusion_turb(L, T))
                 return SCHEMI_ERR_SYSTEM;
         } else if (launch_diffusion(L)) {
@@ -615,14 +618,15 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     if ((scheme->timestep_source != 0 && scheme->timestep_source != 1) ||
         !(scheme->source_coeff >= 0.0))
         return SCHEMI_ERR_INIT;
-    // turbulence: the shear and zero k-epsilon models with SSP-RK2, ideal or
-    // van der Waals mixture, slopes that vanish or 1-D TVD slopes, the
-    // explicit diffusion stage without the species corrections
+    // turbulence: the shear, zero and kEpsA k-epsilon models with SSP-RK2,
+    // ideal or van der Waals mixture, slopes that vanish or 1-D TVD slopes,
+    // the explicit diffusion stage without the species corrections
     if (scheme->turbulence != 0 && scheme->turbulence != 1)
         return SCHEMI_ERR_INIT;
     if (scheme->turbulence &&
         ((scheme->turbulence_model != SCHEMI_TURB_SHEAR &&
-          scheme->turbulence_model != SCHEMI_TURB_ZERO) ||
+          scheme->turbulence_model != SCHEMI_TURB_ZERO &&
+          scheme->turbulence_model != SCHEMI_TURB_KEPSA) ||
          scheme->step_solver != SCHEMI_STEP_SECOND_ORDER_RK ||
          (mix->gas_model != SCHEMI_GAS_IDEAL && mix->gas_model != SCHEMI_GAS_VAN_DER_WAALS) ||
          lim == SCHEMI_LIM_LINEAR ||
@@ -647,7 +651,8 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     c->kappa = scheme->transport_kappa;
     c->molmass = (c->diffusion && scheme->diffusion_molar_mass) ? 1 : 0;
     c->turb = scheme->turbulence != 0;
-    c->nf = mix->ns + 5 + (c->turb ? 2 : 0);
+    c->afield = c->turb && scheme->turbulence_model == SCHEMI_TURB_KEPSA;
+    c->nf = mix->ns + 5 + (c->afield ? 5 : c->turb ? 2 : 0);
     if (c->turb) {
         TurbParams &T = c->tp;
         T.model = scheme->turbulence_model;
@@ -664,6 +669,24 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         T.C1 = scheme->turb_C[0];
         T.C2 = scheme->turb_C[1];
         T.C3 = scheme->turb_C[2];
+        if (c->afield) {
+            // kEpsAModel.cpp:86-121: C0, C1, C2, C3, Ca, CaMax, CMSA, chi
+            const double *K = scheme->turb_kepsa_C;
+            T.C0 = K[0];
+            T.C1 = K[1];
+            T.C2 = K[2];
+            T.C3 = K[3];
+            T.Ca = K[4];
+            T.CaMax = K[5];
+            T.CMSA = K[6];
+            T.chi = K[7];
+            T.sigmaa = scheme->turb_sigma_a;
+            for (int p = 0; p < 6; ++p) {
+                T.bcta[p] = scheme->bc_a[p].type;
+                for (int b = 0; b < 3; ++b)
+                    T.bcva[p][b] = scheme->bc_a[p].value[b];
+            }
+        }
         T.ts_coeff = scheme->source_coeff;
         for (int p = 0; p < 6; ++p) {
             T.bct[p][0] = scheme->bc_k[p].type;
@@ -819,9 +842,9 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
             }
         }
         if (c->turb)
-            for (int f = 0; f < 2; ++f)
+            for (int f = 0; f < (c->afield ? 3 : 2); ++f)
                 for (int p : {plo, phi}) {
-                    const int t = c->tp.bct[p][f];
+                    const int t = f < 2 ? c->tp.bct[p][f] : c->tp.bcta[p];
                     if (!bc_ok(t)) {
                         delete c;
                         return SCHEMI_ERR_BOUNDARY;
@@ -940,7 +963,7 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         return SCHEMI_ERR_SYSTEM;
     }
     if (noalloc_setup(c->device) != SCHEMI_OK) {
-        hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->stream);
         delete c;
         return SCHEMI_ERR_SYSTEM;
     }
@@ -952,7 +975,8 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
               hipMalloc(&c->sc, SC_N * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->derr, sizeof(int)) == hipSuccess;
     if (ok && c->diffusion)
-        ok = hipMalloc(&c->Wd, sizeof(double) * (size_t)(2 * mix->ns + (c->turb ? 9 : 5)) *
+        ok = hipMalloc(&c->Wd, sizeof(double) *
+                                   (size_t)(2 * mix->ns + (c->afield ? 13 : c->turb ? 9 : 5)) *
                                    (size_t)c->n) == hipSuccess;
     if (ok && c->turb)
         ok = hipMalloc(&c->dTP, sizeof(TurbParams)) == hipSuccess &&

@@ -68,11 +68,12 @@ __device__ __forceinline__ long long block_cell0()
 }
 
 // W layout per cell: Y[NS], c[NS], v[3], T, p; with turbulence then k,
-// epsilon, CvM = Cv / (rho / c0) and nonIdeality / rho (Diffusion.cpp:137-172)
+// epsilon, CvM = Cv / (rho / c0) and nonIdeality / rho (Diffusion.cpp:137-172);
+// with the kEpsA model then a[3] and rho
 template <int NS> struct DW {
     static constexpr int Y = 0, C = NS, V = 2 * NS, T = 2 * NS + 3, P = 2 * NS + 4,
                          NW = 2 * NS + 5, K = NW, EPS = NW + 1, CVM = NW + 2, NIR = NW + 3,
-                         NWT = NW + 4;
+                         NWT = NW + 4, A = NWT, RHO = NWT + 3, NWA = NWT + 4;
 };
 
 // Cv_mix = sum_k (c_k / c0) Cv_k (mixtureIdeal.cpp:40-51)
@@ -170,8 +171,9 @@ __device__ __forceinline__ void reduce_max_bits(double r, unsigned long long *sl
 
 // TURB: the state carries rho k and rho epsilon (U = E - rho v^2 / 2 - rho k);
 // k, epsilon, CvM and nonIdeality / rho go to W, and the maximum of epsilon
-// to *epsmax (shearModel.cpp:107-114; cleared after each step).
-template <int NS, bool TURB = false>
+// to *epsmax (shearModel.cpp:107-114; cleared after each step).  AF (kEpsA):
+// the state also carries rho a; a and rho go to W.
+template <int NS, bool TURB = false, bool AF = false>
 __global__ void __launch_bounds__(NT)
 diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const double *Ualt, int hk,
                   double *__restrict__ W, double *sc, unsigned long long *epsmax = nullptr)
@@ -194,13 +196,14 @@ diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const doubl
     if (id >= n)
         return;
     const double *Us = state_of(U, Ualt, sc, hk);
-    constexpr int NF = TURB ? NS + 7 : NS + 5;
-    double u[NF], w[NS + 6];
+    constexpr int NA = AF ? 3 : 0;
+    constexpr int NF = TURB ? NS + 7 + NA : NS + 5;
+    double u[NF], w[NS + 6 + NA];
 #pragma unroll
     for (int q = 0; q < NF; ++q)
         u[q] = Us[q * n + id];
     if constexpr (TURB)
-        prims_turb<double, NS, -1>(P, u, w);
+        prims_turb<double, NS, -1, NA>(P, u, w);
     else
         prims_from_cons<double, NS>(P, u, w);
     using L = DW<NS>;
@@ -230,6 +233,12 @@ diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const doubl
         const double avM = u[NS] / c0;
         W[L::CVM * n + id] = cv_mix<NS>(P, w, c0) / avM;
         W[L::NIR * n + id] = non_ideality<NS>(P, w, c0) / u[NS];
+    }
+    if constexpr (AF) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            W[(L::A + c) * n + id] = w[NS + 6 + c];
+        W[L::RHO * n + id] = u[NS];
     }
 }
 
@@ -390,7 +399,15 @@ __device__ __forceinline__ void reduce_min_bits(double r, unsigned long long *sl
 // thetaS_D of the face divergence), the non-ideality correction Laplacian,
 // the dissipation rho eps and the filtered turbulent stress in the
 // temperature equation, and the shear model's sources with its source step.
-template <int NS, int DIM, bool MIXED, bool IMPL, bool SPEC, bool TURB = false>
+// AF (TURB, kEpsA; kEpsAModel.cpp:132-257, Diffusion.cpp:176-191, 288-305,
+// 511-538, 574, 736-800, 869-887, 1005-1016): the algebraic b of the cell,
+// grad a, grad p and grad rho (Green-Gauss; rho on a boundary face from the
+// surface concentrations), the divergence of the molecular viscous stress
+// mu (grad_s v + grad_s v^T - 2/3 div_s v I), the a Laplacian (mu +
+// rho nu_t / sigma_a), the k, epsilon and a sources with the thetaA limiter,
+// and the energy sink a . (grad p - div tau) distributed into the
+// temperature equation.
+template <int NS, int DIM, bool MIXED, bool IMPL, bool SPEC, bool TURB = false, bool AF = false>
 __global__ void __launch_bounds__(NT)
 diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo, double *U,
                    double *Ualt, int hk, const double *__restrict__ W, double *sc,
@@ -399,9 +416,10 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                    const unsigned long long *__restrict__ epsmax = nullptr)
 {
     static_assert(!(TURB && (IMPL || SPEC)), "turbulence: explicit solver without species terms");
+    static_assert(TURB || !AF, "the a field needs turbulence");
     using M = MB<NS>;
     using L = DW<NS>;
-    constexpr int NW = TURB ? L::NWT : L::NW;
+    constexpr int NW = TURB ? (AF ? L::NWA : L::NWT) : L::NW;
     __shared__ double sRed[NT / 64];
     const DevParams &P = *Pp;
     if (sc[SC_DONE] != 0.0)
@@ -426,6 +444,10 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         int err = 0;
         double dMC[NS], dEF = 0, c0o = 0;
         double dK = 0, dE = 0, dNI = 0;  // turbulence: k, epsilon, non-ideality Laplacians
+        // kEpsA: Green-Gauss sums of grad a, grad p, grad rho; the a Laplacian
+        // and the divergence of the molecular viscous stress
+        double gA[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, gP[3] = {0, 0, 0}, gR[3] = {0, 0, 0};
+        double dA[3] = {0, 0, 0}, dDP[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             dMC[k] = 0;
@@ -489,6 +511,8 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             // turbulence: face k, epsilon, Cv and temperature, and the normal
             // components of surfGrad of k, epsilon, CvM and nonIdeality / rho
             double ks = 0, es = 0, cvs = 0, Tsf = 0, gk = 0, ge = 0, gcm = 0, gni = 0;
+            // kEpsA: face a, p and rho, and the normal components of surfGrad a
+            double as_[3] = {0, 0, 0}, ps_ = 0, rs_ = 0, ga_[3] = {0, 0, 0};
             if (inner) {
                 const long long nb =
                     id + (wrap ? (hi ? -(long long)(Na - 1) : (long long)(Na - 1)) * P.stride[a]
@@ -545,6 +569,15 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                     ge = ((qn[L::EPS] - qo[L::EPS]) / m2) * da;
                     gcm = ((qn[L::CVM] - qo[L::CVM]) / m2) * da;
                     gni = ((qn[L::NIR] - qo[L::NIR]) / m2) * da;
+                }
+                if constexpr (AF) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        as_[c] = qo[L::A + c] * wo + qn[L::A + c] * wnn;
+                        ga_[c] = ((qn[L::A + c] - qo[L::A + c]) / m2) * da;
+                    }
+                    ps_ = qo[L::P] * wo + qn[L::P] * wnn;
+                    rs_ = qo[L::RHO] * wo + qn[L::RHO] * wnn;
                 }
                 if constexpr (SPEC) {
                     Ts = qo[L::T] * wo + qn[L::T] * wnn;
@@ -649,6 +682,21 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                     gni = ((nig - w0[L::NIR]) / m) * nrm[a];
                     if (TP.bct[p][0] == BC_FIXED_SURF || TP.bct[p][1] == BC_FIXED_SURF)
                         err = 1;
+                    if constexpr (AF) {
+                        // a: surface values and ghosts of set/a.txt; p: its
+                        // surface value; rho: calculatedDensity, sum_k c_k,f M_k
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            as_[c] = bc_surface<double>(TP.bcta[p], TP.bcva[p][c], w0[L::A + c], c, a);
+                            const double ag =
+                                bc_cell<double>(TP.bcta[p], TP.bcva[p][c], w0[L::A + c], c, a);
+                            ga_[c] = ((ag - w0[L::A + c]) / m) * nrm[a];
+                        }
+                        ps_ = ps;
+                        rs_ = srho;
+                        if (TP.bcta[p] == BC_FIXED_SURF)
+                            err = 1;
+                    }
                 }
                 if constexpr (SPEC) {
                     // calculatedTemperature on the surface; calculatedMolarFraction ghost
@@ -720,6 +768,23 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 // -(surfGrad CvM T_f + surfGrad (nonIdeality / rho)) rho tLambda
                 const double flow = -1 * (gcm * Tsf + gni);
                 dNI += ((flow * srho * tLam) * na) * S;
+                if constexpr (AF) {
+                    // grad a (a_c N_b), grad p and grad rho (N_b q) sums; the a
+                    // Laplacian; mu (grad_s v + grad_s v^T - 2/3 div_s v I) & N
+                    const double rhoDa = mu + srho * (tnu / TP.sigmaa);
+                    const double lam_s = (2. / 3.) * divs;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        gA[3 * c + a] += (as_[c] * na) * S;
+                        dA[c] += (((ga_[c] * rhoDa) * -1) * na) * S;
+                        double t = gv[c][a] + gv[a][c];
+                        if (c == a)
+                            t -= lam_s;
+                        dDP[c] += ((t * mu) * na) * S;
+                    }
+                    gP[a] += (na * ps_) * S;
+                    gR[a] += (na * rs_) * S;
+                }
             }
             // Green-Gauss sums of grad v (v_c N_b) S and div v (v & N) S
 #pragma unroll
@@ -825,7 +890,8 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 Mb[(M::TB + s) * n + id] = tB[s];
             }
         } else {
-        constexpr int NF = TURB ? NS + 7 : NS + 5;
+        constexpr int NA = AF ? 3 : 0;
+        constexpr int NF = TURB ? NS + 7 + NA : NS + 5;
         double u[NF];
 #pragma unroll
         for (int q = 0; q < NF; ++q)
@@ -910,7 +976,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 diagT -= src / (w0[L::T] + STAB);
         }
         ftT += heat;
-        double kn = 0, en_ = 0;
+        double kn = 0, en_ = 0, an[3] = {0, 0, 0};
         if constexpr (TURB) {
             const TurbParams &TP = *Tp;
             const double k = w0[L::K], eps = w0[L::EPS];
@@ -929,11 +995,14 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             ftT += philt;
             // generation by the spherical (-(2/3) rho k I) and deviatoric
             // (rho nu_t thetaS_R strain) Reynolds stresses
+            double sph[9], dev[9];
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 const double Iq = (q == 0 || q == 4 || q == 8) ? 1. : 0.;
-                gsph += (Iq * -(2. / 3.) * rk) * G[q];
-                gdev += (sv[q] * rho * tnu * thR) * G[q];
+                sph[q] = Iq * -(2. / 3.) * rk;
+                dev[q] = sv[q] * rho * tnu * thR;
+                gsph += sph[q] * G[q];
+                gdev += dev[q] * G[q];
             }
             double sk = 0, se = 0;
             if (TP.model == TURB_SHEAR) {
@@ -950,6 +1019,84 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                     __longlong_as_double(static_cast<long long>(epsmax[0]));
                 const double me = (eps < 1E-3 * emax) ? 1e-3 * __DBL_MAX__ : eps;
                 rsrc = __builtin_fabs(TP.ts_coeff * me / ((Se1 + Se2 * eps) / rho + STAB));
+            }
+            if constexpr (AF) {
+                // kEpsAModel.cpp:170-225.  b = rho sum x_k / (rho_k + chi rho) /
+                // sum x_k rho_k / (rho_k + chi rho) - 1, clipped at 0
+                // (kEpsAModel.cpp:232-257; the reference's field + scalar drops
+                // the stabilizer there, fieldOperations.hpp:33-43)
+                double num = 0, den = 0;
+#pragma unroll
+                for (int q = 0; q < NS; ++q) {
+                    const double x = w0[L::C + q] / c0o;
+                    const double r = u[q] + TP.chi * rho;
+                    num += x / r;
+                    den += x * u[q] / r;
+                }
+                double b = rho * num / den - 1.;
+                if (b < 0.)
+                    b = 0.0;
+                const double av[3] = {w0[L::A], w0[L::A + 1], w0[L::A + 2]};
+                const double ek = eps / k;
+                // thetaS_R of trace(grad v); thetaA (kEpsAModel.cpp:16-27)
+                const double thS = theta_s(TP, G[0] + G[4] + G[8], k, eps, TP.CMS_R);
+                double thA = 1;
+                if (TP.CMSA < 1E15) {
+                    double aa = 0;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        aa += av[c] * av[c];
+                    thA = (aa / (2 * k) < TP.CMSA && b < TP.CMSA && aa / (2 * k) < b)
+                              ? 1
+                              : TP.CaMax / TP.Ca;
+                }
+                double gdevA = 0, x[3], grav = 0;
+#pragma unroll
+                for (int q = 0; q < 9; ++q)
+                    gdevA += (dev[q] * thS) * G[q];
+                // a . (grad p - div(mu strainSurf))
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    x[c] = gP[c] / V - dDP[c] / V;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    grav += av[c] * x[c];
+                const double dissip = -reps;
+                const double Sk1 = gsph + gdevA + grav, Sk2 = dissip / k;
+                const double Se1 = (TP.C1 * gdevA + TP.C3 * gsph + TP.C0 * grav) * ek;
+                const double Se2 = TP.C2 * dissip / k;
+                sk = Sk1 + Sk2 * k;
+                se = Se1 + Se2 * eps;
+                const double emax = __longlong_as_double(static_cast<long long>(epsmax[0]));
+                const double me = (eps < 1E-3 * emax) ? 1e-3 * __DBL_MAX__ : eps;
+                rsrc = __builtin_fabs(TP.ts_coeff * me / ((Se1 + Se2 * eps) / rho + STAB));
+                // a: explicit Laplacian plus b (grad p - div tau) + ((thetaS
+                // devR + spherR) / rho) & grad rho + rho a & (grad a - grad v)
+                // - rho (eps / k) Ca thetaA a (Diffusion.cpp:749-762, 869-887)
+                double tau[9], D[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    tau[q] = (dev[q] * thS + sph[q]) / rho;
+                    D[q] = gA[q] / V - G[q];
+                }
+                const double gr[3] = {gR[0] / V, gR[1] / V, gR[2] / V};
+                const double Sa2 = -rho * ek * TP.Ca * thA;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double bgp = x[c] * b;
+                    const double tgr = tau[3 * c] * gr[0] + tau[3 * c + 1] * gr[1] + tau[3 * c + 2] * gr[2];
+                    const double ragv =
+                        D[c] * u[NS + 7] + D[3 + c] * u[NS + 8] + D[6 + c] * u[NS + 9];
+                    const double sig = bgp + tgr + ragv + Sa2 * av[c];
+                    an[c] = (av[c] * rho + (0. - dA[c] / V) * dt) / rho + sig / rho * dt;
+                }
+                // the energy sink into the temperature, distributeSourceTerm
+                // (Diffusion.cpp:780-799, SLEMatrix.cpp:2166-2175)
+                const double src = grav * (-1.0);
+                if (src >= 0.)
+                    ftT += src;
+                else
+                    diagT -= src / (w0[L::T] + STAB);
             }
             // explicit Laplacians plus the sources (Diffusion.cpp:839-862),
             // minimal values (905-917)
@@ -984,6 +1131,9 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             un[NS + 5] = rkn;
             un[NS + 6] = en_ * rho;
         }
+#pragma unroll
+        for (int c = 0; c < NA; ++c)
+            un[NS + 7 + c] = an[c] * rho;
         un[NS + 4] = (Uin + rho * v2 * 0.5) + rkn;
 #pragma unroll
         for (int q = 0; q < NF; ++q)
@@ -1001,7 +1151,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             if (en)
                 reduce_min_bits<NT>(rsrc, reinterpret_cast<unsigned long long *>(sc) + SC_SRCMIN);
         if constexpr (TURB)
-            if (Tp->model == TURB_SHEAR)
+            if (Tp->model == TURB_SHEAR || Tp->model == TURB_KEPSA)
                 reduce_min_bits<NT>(rsrc, reinterpret_cast<unsigned long long *>(sc) + SC_SRCMIN);
     }
 }
@@ -1236,17 +1386,17 @@ diff_finish_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
 
 // Turbulent explicit stage: the primitives pass with k, epsilon, CvM and
 // nonIdeality / rho and the maximum of epsilon, then the update.
-template <int NS, int DIM>
+template <int NS, int DIM, bool AF>
 int launch_dim_turb(const DiffusionLaunch &L, const TurbDiffusion &T)
 {
     const unsigned blocks = (unsigned)((L.n + NT - 1) / NT);
-    hipLaunchKernelGGL((diff_prims_kernel<NS, true>), dim3(blocks), dim3(NT), 0, L.stream, L.P,
-                       L.U, L.Ualt, L.hk, L.W, L.sc, T.epsmax);
+    hipLaunchKernelGGL((diff_prims_kernel<NS, true, AF>), dim3(blocks), dim3(NT), 0, L.stream,
+                       L.P, L.U, L.Ualt, L.hk, L.W, L.sc, T.epsmax);
     if (L.mid && hipEventRecord(L.mid, L.stream) != hipSuccess)
         return 1;
-    hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false, false, true>), dim3(blocks),
-                       dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc, L.err,
-                       L.mu, L.D, L.kappa, nullptr, 0, 0, T.TP, T.epsmax);
+    hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false, false, true, AF>),
+                       dim3(blocks), dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W,
+                       L.sc, L.err, L.mu, L.D, L.kappa, nullptr, 0, 0, T.TP, T.epsmax);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -1344,11 +1494,14 @@ int launch_diffusion_turb(const DiffusionLaunch &L, const TurbDiffusion &T)
         constexpr int NS = decltype(ns)::value;
         switch (L.dim) {
         case 1:
-            return launch_dim_turb<NS, 1>(L, T);
+            return T.afield ? launch_dim_turb<NS, 1, true>(L, T)
+                            : launch_dim_turb<NS, 1, false>(L, T);
         case 2:
-            return launch_dim_turb<NS, 2>(L, T);
+            return T.afield ? launch_dim_turb<NS, 2, true>(L, T)
+                            : launch_dim_turb<NS, 2, false>(L, T);
         default:
-            return launch_dim_turb<NS, 3>(L, T);
+            return T.afield ? launch_dim_turb<NS, 3, true>(L, T)
+                            : launch_dim_turb<NS, 3, false>(L, T);
         }
     };
     switch (L.ns) {

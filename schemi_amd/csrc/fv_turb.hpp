@@ -1,11 +1,16 @@
 // fv_turb.hpp -- k-epsilon turbulence on the step: the conserved state
-// carries rho k and rho epsilon after E (Ns + 7 fields), the face closure
-// adds rho k to the total energy and (2/3) k (1 + dpdUv) to the squared sound
-// speed, the HLLC / HLL flux carries the turbulent pressure (2/3) rho k and
-// the fluxes of rho k and rho epsilon (HLLCSolver.cpp:26-375,
-// HLLSolver.cpp:26-284, Advection2dOrder.cpp:77-627), and the internal energy
-// is E - rho v^2 / 2 - rho k.  The diffusion stage's turbulent terms live in
+// carries rho k and rho epsilon after E (Ns + 7 fields; the kEpsA model adds
+// rho a, Ns + 10), the face closure adds rho k to the total energy and
+// (2/3) k (1 + dpdUv) to the squared sound speed, the HLLC / HLL flux carries
+// the turbulent pressure (2/3) rho k and the fluxes of rho k, rho epsilon and
+// rho a (HLLCSolver.cpp:26-388, HLLSolver.cpp:26-300,
+// Advection2dOrder.cpp:77-627), and the internal energy is
+// E - rho v^2 / 2 - rho k.  The diffusion stage's turbulent terms live in
 // fv_diffusion.hip and use TurbParams.
+//
+// NA (0 or 3) is the number of components of a carried by the state: each
+// component of rho a is fluxed with the arithmetic of rho k, so the kEpsA
+// model's transport adds three passive fields.
 //
 // turb_stage_kernel: one thread per cell.  The cell evaluates its 2 DIM faces
 // in the reference's order (x-, x+, y+, y-, z-, z+) with the face's
@@ -14,7 +19,9 @@
 // values are the cell values where the limited slopes vanish identically
 // (TVD limiters in 2-D and 3-D, the zero limiter); on 1-D meshes with a TVD
 // limiter every side is reconstructed from the limited slopes of c_k, v, p,
-// k and epsilon of its cell (stencil i-2 .. i+2).
+// k, epsilon and a of its cell (stencil i-2 .. i+2; of v and a only the
+// component along the axis has a slope, TVDLimiter.cpp:127-273 gives the
+// transverse components an exact zero limiter entry).
 #pragma once
 
 #include "fv_stage.hpp"
@@ -22,11 +29,12 @@
 namespace sg {
 
 // turbulenceModel (turbulenceModelEnum.hpp) values built here
-enum : int { TURB_ZERO = 0, TURB_SHEAR = 2 };
+enum : int { TURB_ZERO = 0, TURB_SHEAR = 2, TURB_KEPSA = 6 };
 
 // Parameters of set/turbulentParameters.txt in the k-epsilon models' meaning
-// (abstractTurbulenceModel.hpp:26-57, kEpsModels.hpp, shearModel.cpp:35-59)
-// and the boundary conditions of k (0) and epsilon (1) per patch.
+// (abstractTurbulenceModel.hpp:26-57, kEpsModels.hpp, shearModel.cpp:35-59,
+// kEpsAModel.cpp:29-121) and the boundary conditions of k (0) and epsilon (1)
+// per patch, and of the vector a (kEpsA).
 struct TurbParams {
     int model;
     double Cmu, sigmaSc, sigmaT, sigmaE, sigmaK, sigmaEps, CMS_R, CMS_D, mink, mineps;
@@ -34,6 +42,10 @@ struct TurbParams {
     double ts_coeff;  // first coefficient of #Source_timestep_correction
     int bct[6][2];
     double bcv[6][2];
+    // kEpsA: sigma_a, C0, Ca, CaMax, CMSA, chi (C1..C3 above)
+    double sigmaa, C0, Ca, CaMax, CMSA, chi;
+    int bcta[6];
+    double bcva[6][3];
 };
 
 // kEpsModels::thetaS (kEpsModels.cpp:13-26): realisability factor of the
@@ -51,9 +63,10 @@ __device__ __forceinline__ double theta_s(const TurbParams &TP, double divV, dou
 
 // Primitives of a turbulent conserved state: c_k = rho_k / M_k, v = m / rho,
 // U = (E - rho v^2 / 2) - rho k, p = pFromUv, k = rho k / rho,
-// epsilon = rho epsilon / rho (Advection2dOrder.cpp:578-627).
-// u: [rho_1..rho_Ns, rho, m (3), E, rho k, rho eps]; w: [c (Ns), v (3), p, k, eps]
-template <class T, int NS, int GAS>
+// epsilon = rho epsilon / rho, a = rho a / rho (Advection2dOrder.cpp:578-627).
+// u: [rho_1..rho_Ns, rho, m (3), E, rho k, rho eps, rho a (NA)];
+// w: [c (Ns), v (3), p, k, eps, a (NA)]
+template <class T, int NS, int GAS, int NA = 0>
 __device__ __forceinline__ void prims_turb(const DevParams &P, const T *u, T *w)
 {
     T c0 = 0;
@@ -72,22 +85,27 @@ __device__ __forceinline__ void prims_turb(const DevParams &P, const T *u, T *w)
     w[NS + 3] = eos_p<T, NS, GAS>(P, w, c0, gamma1<T, NS>(P, w, c0), U);
     w[NS + 4] = u[NS + 5] / rho;
     w[NS + 5] = u[NS + 6] / rho;
+#pragma unroll
+    for (int c = 0; c < NA; ++c)
+        w[NS + 6 + c] = u[NS + 7 + c] / rho;
 }
 
 // Face side with turbulence (Advection2dOrder.cpp:233-289, 301-372, 436-515):
-// rho k = k rho, rho eps = eps rho, E = (U + rho v^2 / 2) + rho k, and the
-// sound speed sqrt(c^2 + (2/3) k (1 + dpdUv)) of the flux (HLLCSolver.cpp:26-50).
-template <class T, int NS> struct TSide {
+// rho k = k rho, rho eps = eps rho, rho a = a rho, E = (U + rho v^2 / 2) +
+// rho k, and the sound speed sqrt(c^2 + (2/3) k (1 + dpdUv)) of the flux
+// (HLLCSolver.cpp:26-50).
+template <class T, int NS, int NA = 0> struct TSide {
     T rho[NS];
     T r0;
     T v[3];
     T m[3];
     T p, U, E, cs, rk, reps;
+    T ra[NA > 0 ? NA : 1];
 };
 
-template <class T, int NS, int GAS>
+template <class T, int NS, int GAS, int NA = 0>
 __device__ __forceinline__ void close_side_turb(const DevParams &P, const T *c, const T *v, T p,
-                                                T k, T eps, TSide<T, NS> &s)
+                                                T k, T eps, const T *a, TSide<T, NS, NA> &s)
 {
     T c0 = 0, r0 = 0;
 #pragma unroll
@@ -99,6 +117,9 @@ __device__ __forceinline__ void close_side_turb(const DevParams &P, const T *c, 
     s.r0 = r0;
     s.rk = k * r0;
     s.reps = eps * r0;
+#pragma unroll
+    for (int q = 0; q < NA; ++q)
+        s.ra[q] = a[q] * r0;
     const T g1 = gamma1<T, NS>(P, c, c0);
     s.p = p;
     s.U = eos_U<T, NS, GAS>(P, c, c0, g1, p);
@@ -122,13 +143,14 @@ __device__ __forceinline__ void close_side_turb(const DevParams &P, const T *c, 
 
 // HLLC flux F & N of the turbulent state (HLLCSolver.cpp:52-388): the
 // pressure P = p + (2/3) rho k in the wave speeds, the star pressure, the
-// momentum and the energy ((E + p) + (2/3) rho k) u; rho k and rho epsilon
-// carried like the partial densities.  g: NS + 7 fluxes.
-template <class T, int NS>
-__device__ __forceinline__ bool hllc_flux_turb(const TSide<T, NS> &O, const TSide<T, NS> &B,
-                                               int ax, T sg, T *g)
+// momentum and the energy ((E + p) + (2/3) rho k) u; rho k, rho epsilon and
+// the components of rho a carried like the partial densities.  g: NS + 7 + NA
+// fluxes.
+template <class T, int NS, int NA = 0>
+__device__ __forceinline__ bool hllc_flux_turb(const TSide<T, NS, NA> &O,
+                                               const TSide<T, NS, NA> &B, int ax, T sg, T *g)
 {
-    constexpr int NF = NS + 7;
+    constexpr int NF = NS + 7 + NA;
     const T uO = sg * O.v[ax], uN = sg * B.v[ax];
     const T ptO = T(2. / 3.) * O.rk, ptN = T(2. / 3.) * B.rk;
     const T PO = O.p + ptO, PN = B.p + ptN;
@@ -146,7 +168,7 @@ __device__ __forceinline__ bool hllc_flux_turb(const TSide<T, NS> &O, const TSid
             g[q] = SO;  // NaN
         return false;
     }
-    const TSide<T, NS> &X = own ? O : B;
+    const TSide<T, NS, NA> &X = own ? O : B;
     const T u = own ? uO : uN;
     const T S = own ? SO : SN;
     const T pt = own ? ptO : ptN;
@@ -161,6 +183,9 @@ __device__ __forceinline__ bool hllc_flux_turb(const TSide<T, NS> &O, const TSid
     g[NS + 4] = ((X.E + X.p) + pt) * u;
     g[NS + 5] = X.rk * u;
     g[NS + 6] = X.reps * u;
+#pragma unroll
+    for (int q = 0; q < NA; ++q)
+        g[NS + 7 + q] = X.ra[q] * u;
     if (star) {
         const T factor = S - u;
         const T den = T(1) / (S - Sc);
@@ -184,16 +209,21 @@ __device__ __forceinline__ bool hllc_flux_turb(const TSide<T, NS> &O, const TSid
         const T ks = X.rk * factor * den, es = X.reps * factor * den;
         g[NS + 5] = g[NS + 5] + S * (ks - X.rk);
         g[NS + 6] = g[NS + 6] + S * (es - X.reps);
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const T as = X.ra[q] * factor * den;
+            g[NS + 7 + q] = g[NS + 7 + q] + S * (as - X.ra[q]);
+        }
     }
     return true;
 }
 
 // HLL flux of the turbulent state (HLLSolver.cpp:52-300).
-template <class T, int NS>
-__device__ __forceinline__ bool hll_flux_turb(const TSide<T, NS> &O, const TSide<T, NS> &B,
-                                              int ax, T sg, T *g)
+template <class T, int NS, int NA = 0>
+__device__ __forceinline__ bool hll_flux_turb(const TSide<T, NS, NA> &O,
+                                              const TSide<T, NS, NA> &B, int ax, T sg, T *g)
 {
-    constexpr int NF = NS + 7;
+    constexpr int NF = NS + 7 + NA;
     const T uO = sg * O.v[ax], uN = sg * B.v[ax];
     const T ptO = T(2. / 3.) * O.rk, ptN = T(2. / 3.) * B.rk;
     const T PO = O.p + ptO, PN = B.p + ptN;
@@ -207,7 +237,7 @@ __device__ __forceinline__ bool hll_flux_turb(const TSide<T, NS> &O, const TSide
             return false;
         }
         const bool own = SO >= T(0);
-        const TSide<T, NS> &X = own ? O : B;
+        const TSide<T, NS, NA> &X = own ? O : B;
         const T u = own ? uO : uN;
         const T PX = own ? PO : PN;
         const T pt = own ? ptO : ptN;
@@ -221,6 +251,9 @@ __device__ __forceinline__ bool hll_flux_turb(const TSide<T, NS> &O, const TSide
         g[NS + 4] = u * ((X.E + X.p) + pt);
         g[NS + 5] = u * X.rk;
         g[NS + 6] = u * X.reps;
+#pragma unroll
+        for (int q = 0; q < NA; ++q)
+            g[NS + 7 + q] = u * X.ra[q];
         return true;
     }
     const T den = T(1) / (SN - SO);
@@ -241,29 +274,31 @@ __device__ __forceinline__ bool hll_flux_turb(const TSide<T, NS> &O, const TSide
                  SN * SO * (B.E - O.E)) * den;
     g[NS + 5] = (SN * uO * O.rk - SO * uN * B.rk + SN * SO * (B.rk - O.rk)) * den;
     g[NS + 6] = (SN * uO * O.reps - SO * uN * B.reps + SN * SO * (B.reps - O.reps)) * den;
+#pragma unroll
+    for (int q = 0; q < NA; ++q)
+        g[NS + 7 + q] =
+            (SN * uO * O.ra[q] - SO * uN * B.ra[q] + SN * SO * (B.ra[q] - O.ra[q])) * den;
     return true;
 }
 
-// Limited slope along axis a of a turbulent scalar (k: f = 0, epsilon: f = 1)
-// of cell i with neighbour values qm, qp (TVDLimiter.cpp:12-125 with the
-// one-axis stencil of slopes_axis; boundary values from TurbParams).
+// Limited slope along axis a of cell i with neighbour values qm, qp
+// (TVDLimiter.cpp:12-273 with the one-axis stencil of slopes_axis): boundary
+// conditions tlo / thi with values vlo / vhi on the low / high patch of the
+// axis, comp the vector component of the value (-1 for scalars).
 template <class T>
-__device__ __forceinline__ T slope_turb(const DevParams &P, const TurbParams &TP,
-                                        const AxisGeo &G, int a, int i, int f, T qm, T q0, T qp,
-                                        T S, T V)
+__device__ __forceinline__ T slope_turb_bc(const DevParams &P, const AxisGeo &G, int a, int i,
+                                           int tlo, T vlo, int thi, T vhi, int comp, T qm, T q0,
+                                           T qp, T S, T V)
 {
     const int Na = P.N[a];
     const bool per = P.periodic[a] != 0;
     const bool has_lo = (i > 0) || per;
     const bool has_hi = (i < Na - 1) || per;
-    const int plo = patch_of(a, 0), phi_ = patch_of(a, 1);
     const bool flip = !has_hi && P.hi_flip[a] != 0;
-    const int tlo = TP.bct[plo][f], thi = TP.bct[phi_][f];
-    const T vlo = T(TP.bcv[plo][f]), vhi = T(TP.bcv[phi_][f]);
     const T elo = has_lo ? -((qm - q0) / to_T<T>(G.dn[i]))
-                         : -((bc_cell<T>(tlo, vlo, q0, -1, a) - q0) / to_T<T>(P.ghost_lo[a]));
+                         : -((bc_cell<T>(tlo, vlo, q0, comp, a) - q0) / to_T<T>(P.ghost_lo[a]));
     T ehi = has_hi ? (qp - q0) / to_T<T>(G.dn[i + 1])
-                   : (bc_cell<T>(thi, vhi, q0, -1, a) - q0) / to_T<T>(P.ghost_hi[a]);
+                   : (bc_cell<T>(thi, vhi, q0, comp, a) - q0) / to_T<T>(P.ghost_hi[a]);
     if (flip)
         ehi = -ehi;
     T mn = tmin(elo, ehi), mx = tmax(elo, ehi);
@@ -274,21 +309,40 @@ __device__ __forceinline__ T slope_turb(const DevParams &P, const TurbParams &TP
     if (mn <= T(0) && mx + T(STAB) > T(0))
         return T(0);
     const T flo = has_lo ? qm * to_T<T>(G.wo[i]) + q0 * to_T<T>(G.wn[i])
-                         : bc_surface<T>(tlo, vlo, q0, -1, a);
+                         : bc_surface<T>(tlo, vlo, q0, comp, a);
     const T fhi = has_hi ? q0 * to_T<T>(G.wo[i + 1]) + qp * to_T<T>(G.wn[i + 1])
-                         : bc_surface<T>(thi, vhi, q0, -1, a);
+                         : bc_surface<T>(thi, vhi, q0, comp, a);
     const T grad = ((flip ? -fhi : fhi) * S - flo * S) / V;
     const T r = mn / (mx + T(STAB));
     return phi<T>(P.limiter, r) * grad;
 }
 
-// Face side values c (NS), v (3), p, k, eps of a cell (q: NS + 6 values).
-template <class T, int NS> struct TFace {
-    T q[NS + 6];
-};
+// Limited slope of a turbulent scalar (k: f = 0, epsilon: f = 1).
+template <class T>
+__device__ __forceinline__ T slope_turb(const DevParams &P, const TurbParams &TP,
+                                        const AxisGeo &G, int a, int i, int f, T qm, T q0, T qp,
+                                        T S, T V)
+{
+    const int plo = patch_of(a, 0), phi_ = patch_of(a, 1);
+    return slope_turb_bc<T>(P, G, a, i, TP.bct[plo][f], T(TP.bcv[plo][f]), TP.bct[phi_][f],
+                            T(TP.bcv[phi_][f]), -1, qm, q0, qp, S, V);
+}
 
-// Ghost side of a boundary face from the owner side (Advection2dOrder.cpp:398-521).
-template <class T, int NS>
+// Limited slope along axis a of the a component along that axis (the
+// transverse components have none).
+template <class T>
+__device__ __forceinline__ T slope_turb_a(const DevParams &P, const TurbParams &TP,
+                                          const AxisGeo &G, int a, int i, T qm, T q0, T qp, T S,
+                                          T V)
+{
+    const int plo = patch_of(a, 0), phi_ = patch_of(a, 1);
+    return slope_turb_bc<T>(P, G, a, i, TP.bcta[plo], T(TP.bcva[plo][a]), TP.bcta[phi_],
+                            T(TP.bcva[phi_][a]), a, qm, q0, qp, S, V);
+}
+
+// Ghost side of a boundary face from the owner side (Advection2dOrder.cpp:398-521);
+// side values c (NS), v (3), p, k, eps, a (NA).
+template <class T, int NS, int NA = 0>
 __device__ __forceinline__ void ghost_turb(const DevParams &P, const TurbParams &TP, int a,
                                            int side, const T *o, T *g)
 {
@@ -296,18 +350,23 @@ __device__ __forceinline__ void ghost_turb(const DevParams &P, const TurbParams 
     const int p = patch_of(a, side);
     g[NS + 4] = bc_cell<T>(TP.bct[p][0], T(TP.bcv[p][0]), o[NS + 4], -1, a);
     g[NS + 5] = bc_cell<T>(TP.bct[p][1], T(TP.bcv[p][1]), o[NS + 5], -1, a);
+#pragma unroll
+    for (int c = 0; c < NA; ++c)
+        g[NS + 6 + c] = bc_cell<T>(TP.bcta[p], T(TP.bcva[p][c]), o[NS + 6 + c], c, a);
 }
 
-template <class T, int NS, int GAS>
+template <class T, int NS, int GAS, int NA = 0>
 __device__ __forceinline__ bool flux_turb(const DevParams &P, int a, T sg, const T *o,
                                           const T *b, T *g)
 {
-    TSide<T, NS> O, B;
-    close_side_turb<T, NS, GAS>(P, o, o + NS, o[NS + 3], o[NS + 4], o[NS + 5], O);
-    close_side_turb<T, NS, GAS>(P, b, b + NS, b[NS + 3], b[NS + 4], b[NS + 5], B);
+    TSide<T, NS, NA> O, B;
+    close_side_turb<T, NS, GAS, NA>(P, o, o + NS, o[NS + 3], o[NS + 4], o[NS + 5], o + NS + 6,
+                                    O);
+    close_side_turb<T, NS, GAS, NA>(P, b, b + NS, b[NS + 3], b[NS + 4], b[NS + 5], b + NS + 6,
+                                    B);
     if (P.flux == FLUX_HLL)
-        return hll_flux_turb<T, NS>(O, B, a, sg, g);
-    return hllc_flux_turb<T, NS>(O, B, a, sg, g);
+        return hll_flux_turb<T, NS, NA>(O, B, a, sg, g);
+    return hllc_flux_turb<T, NS, NA>(O, B, a, sg, g);
 }
 
 // Courant signal speed (|v| + c) / min edge with U = E - rho v^2 / 2 - rho k
@@ -330,14 +389,14 @@ __device__ __forceinline__ T signal_speed_turb(const DevParams &P, const T *u, T
 
 // One forward-Euler stage (SV_RK1) or the second SSP-RK2 stage with the
 // average (SV_RK2) of a turbulent state.  SLOPES: 1-D mesh with a TVD limiter.
-template <class T, int NS, int DIM, int SV, bool MIXED, bool SLOPES, int GAS>
+template <class T, int NS, int DIM, int SV, bool MIXED, bool SLOPES, int GAS, int NA>
 __global__ void __launch_bounds__(256)
 turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
                   const double *__restrict__ geo, const double *Uin, double *Uout,
                   const double *Un, const double *__restrict__ dt_ptr,
                   unsigned long long *sigmax, int *errflag)
 {
-    constexpr int NF = NS + 7, NQ = NS + 6;
+    constexpr int NF = NS + 7 + NA, NQ = NS + 6 + NA;
     __shared__ double sRed[4];
     const DevParams &P = *Pp;
     const TurbParams &TP = *Tp;
@@ -358,7 +417,7 @@ turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict
 #pragma unroll
         for (int q = 0; q < NF; ++q)
             ut[q] = T(Uin[q * n + c]);
-        prims_turb<T, NS, GAS>(P, ut, w);
+        prims_turb<T, NS, GAS, NA>(P, ut, w);
     };
     // neighbour of cell id along a at offset dlt (periodic wrap; the caller
     // checks existence on non-periodic axes)
@@ -391,7 +450,7 @@ turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict
         }
         const AxisGeo G = axis_geo(P, geo, 0);
         using SL = SlopeLayout<NS, false>;
-        T L[3][SL::NL], Lt[3][2];
+        T L[3][SL::NL], Lt[3][2], La[3];
 #pragma unroll
         for (int d = -1; d <= 1; ++d) {
             const bool ok = per || (i + d >= 0 && i + d < Nx);
@@ -404,6 +463,9 @@ turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict
                 Lt[d + 1][f] = slope_turb<T>(P, TP, G, 0, j, f, w[d + 1][NS + 4 + f],
                                              w[d + 2][NS + 4 + f], w[d + 3][NS + 4 + f],
                                              T(cgj.S[0]), T(cgj.V));
+            if constexpr (NA > 0)
+                La[d + 1] = slope_turb_a<T>(P, TP, G, 0, j, w[d + 1][NS + 6], w[d + 2][NS + 6],
+                                            w[d + 3][NS + 6], T(cgj.S[0]), T(cgj.V));
         }
         // face side of cell i + d (d = -1, 0, 1) at its low (hi = false) or
         // high face
@@ -421,30 +483,33 @@ turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict
             s[NS + 3] = p;
             s[NS + 4] = Lt[d + 1][0] * off + w[d + 2][NS + 4];
             s[NS + 5] = Lt[d + 1][1] * off + w[d + 2][NS + 5];
+#pragma unroll
+            for (int c = 0; c < NA; ++c)
+                s[NS + 6 + c] = (c == 0) ? La[d + 1] * off + w[d + 2][NS + 6] : w[d + 2][NS + 6 + c];
         };
         T so[NQ], sb[NQ];
         // x- face
         if (!per && i == 0) {
             side(0, false, so);
-            ghost_turb<T, NS>(P, TP, 0, 0, so, sb);
-            if (!flux_turb<T, NS, GAS>(P, 0, T(-1), so, sb, g[0]))
+            ghost_turb<T, NS, NA>(P, TP, 0, 0, so, sb);
+            if (!flux_turb<T, NS, GAS, NA>(P, 0, T(-1), so, sb, g[0]))
                 err |= ERRF_RIEMANN;
         } else {
             side(-1, true, so);
             side(0, false, sb);
-            if (!flux_turb<T, NS, GAS>(P, 0, T(1), so, sb, g[0]))
+            if (!flux_turb<T, NS, GAS, NA>(P, 0, T(1), so, sb, g[0]))
                 err |= ERRF_RIEMANN;
         }
         // x+ face
         if (!per && i == Nx - 1) {
             side(0, true, so);
-            ghost_turb<T, NS>(P, TP, 0, 1, so, sb);
-            if (!flux_turb<T, NS, GAS>(P, 0, P.hi_flip[0] ? T(-1) : T(1), so, sb, g[1]))
+            ghost_turb<T, NS, NA>(P, TP, 0, 1, so, sb);
+            if (!flux_turb<T, NS, GAS, NA>(P, 0, P.hi_flip[0] ? T(-1) : T(1), so, sb, g[1]))
                 err |= ERRF_RIEMANN;
         } else {
             side(0, true, so);
             side(1, false, sb);
-            if (!flux_turb<T, NS, GAS>(P, 0, T(1), so, sb, g[1]))
+            if (!flux_turb<T, NS, GAS, NA>(P, 0, T(1), so, sb, g[1]))
                 err |= ERRF_RIEMANN;
         }
     } else {
@@ -458,14 +523,14 @@ turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict
             const bool bnd = !per && (hi ? idx[a] == Na - 1 : idx[a] == 0);
             T wb[NQ];
             if (bnd) {
-                ghost_turb<T, NS>(P, TP, a, hi ? 1 : 0, w0, wb);
+                ghost_turb<T, NS, NA>(P, TP, a, hi ? 1 : 0, w0, wb);
                 const T sg = hi ? (P.hi_flip[a] ? T(-1) : T(1)) : T(-1);
-                if (!flux_turb<T, NS, GAS>(P, a, sg, w0, wb, g[s]))
+                if (!flux_turb<T, NS, GAS, NA>(P, a, sg, w0, wb, g[s]))
                     err |= ERRF_RIEMANN;
             } else {
                 prims(nbr(a, hi ? 1 : -1), wb);
-                const bool ok = hi ? flux_turb<T, NS, GAS>(P, a, T(1), w0, wb, g[s])
-                                   : flux_turb<T, NS, GAS>(P, a, T(1), wb, w0, g[s]);
+                const bool ok = hi ? flux_turb<T, NS, GAS, NA>(P, a, T(1), w0, wb, g[s])
+                                   : flux_turb<T, NS, GAS, NA>(P, a, T(1), wb, w0, g[s]);
                 if (!ok)
                     err |= ERRF_RIEMANN;
             }

@@ -7,21 +7,32 @@
 
 namespace sg {
 
-template <class T, int NS, int DIM, int SV, bool MIXED, bool SLOPES>
-static int launch_turb_gas(const TurbStageLaunch &L)
+template <class T, int NS, int DIM, int SV, bool MIXED, bool SLOPES, int NA>
+static int launch_turb_na(const TurbStageLaunch &L)
 {
     const unsigned blocks = (unsigned)((L.n + 255) / 256);
     if (L.gas_model == GAS_VDW)
-        hipLaunchKernelGGL((turb_stage_kernel<T, NS, DIM, SV, MIXED, SLOPES, GAS_VDW>),
+        hipLaunchKernelGGL((turb_stage_kernel<T, NS, DIM, SV, MIXED, SLOPES, GAS_VDW, NA>),
                            dim3(blocks), dim3(256), 0, L.stream, L.P, L.TP, L.geo, L.Uin, L.Uout,
                            L.Un, L.dt, L.sigmax, L.err);
     else if (L.gas_model == GAS_IDEAL)
-        hipLaunchKernelGGL((turb_stage_kernel<T, NS, DIM, SV, MIXED, SLOPES, GAS_IDEAL>),
+        hipLaunchKernelGGL((turb_stage_kernel<T, NS, DIM, SV, MIXED, SLOPES, GAS_IDEAL, NA>),
                            dim3(blocks), dim3(256), 0, L.stream, L.P, L.TP, L.geo, L.Uin, L.Uout,
                            L.Un, L.dt, L.sigmax, L.err);
     else
         return -1;
     return static_cast<int>(hipGetLastError());
+}
+
+// na: components of a in the state (0, or 3 for the kEpsA model)
+template <class T, int NS, int DIM, int SV, bool MIXED, bool SLOPES>
+static int launch_turb_gas(const TurbStageLaunch &L)
+{
+    if (L.na == 3)
+        return launch_turb_na<T, NS, DIM, SV, MIXED, SLOPES, 3>(L);
+    if (L.na == 0)
+        return launch_turb_na<T, NS, DIM, SV, MIXED, SLOPES, 0>(L);
+    return -1;
 }
 
 template <class T, int NS, int DIM, bool MIXED>

@@ -74,7 +74,8 @@ class SchemeDesc(C.Structure):
                 ("turb_sigma_eps", C.c_double), ("turb_CMS_R", C.c_double),
                 ("turb_CMS_D", C.c_double), ("turb_min_k", C.c_double),
                 ("turb_min_eps", C.c_double), ("turb_C", C.c_double * 3),
-                ("bc_k", _BC * 6), ("bc_eps", _BC * 6)]
+                ("bc_k", _BC * 6), ("bc_eps", _BC * 6), ("turb_sigma_a", C.c_double),
+                ("turb_kepsa_C", C.c_double * 8), ("bc_a", _BC * 6)]
 
 
 _LIB = None
@@ -224,10 +225,15 @@ def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) 
         (s.turb_Cmu, s.turb_sigma_sc, s.turb_sigma_t, s.turb_sigma_e, s.turb_sigma_k,
          s.turb_sigma_eps, s.turb_CMS_R, s.turb_CMS_D, s.turb_min_k,
          s.turb_min_eps) = (float(x) for x in t.params())
-        for i, x in enumerate(t.C[:3]):
-            s.turb_C[i] = float(x)
+        if t.a_field:
+            for i, x in enumerate(t.C[:8]):
+                s.turb_kepsa_C[i] = float(x)
+            s.turb_sigma_a = float(t.sigmaa)
+        else:
+            for i, x in enumerate(t.C[:3]):
+                s.turb_C[i] = float(x)
         for p in range(6):
-            for dst, src in ((s.bc_k, t.bc_k), (s.bc_eps, t.bc_eps)):
+            for dst, src in ((s.bc_k, t.bc_k), (s.bc_eps, t.bc_eps), (s.bc_a, t.bc_a)):
                 dst[p].type = int(src[p].type)
                 for a in range(3):
                     dst[p].value[a] = float(src[p].value[a])
@@ -550,7 +556,7 @@ def vdw_matrices(case: Case):
     return a, b
 
 
-def conserved_from_primitive(case: Case, c, v, p, kt=None, et=None) -> np.ndarray:
+def conserved_from_primitive(case: Case, c, v, p, kt=None, et=None, at=None) -> np.ndarray:
     """Conserved vector from primitives (c_k, v, p), with the reference's
     initialisation arithmetic (phaseInitialization.cpp:1827-1883) and its
     UvFromp (idealFluid.cpp:19-23, vanDerWaalsFluid.cpp:24-31; Redlich-Kwong
@@ -558,8 +564,9 @@ def conserved_from_primitive(case: Case, c, v, p, kt=None, et=None) -> np.ndarra
     FP64, a few ulps from the reference's long-double root).  With turbulence
     k and epsilon are clipped to the minimal values, rho k = k rho and
     rho eps = rho eps are appended and rho k is added to E
-    (phaseInitialization.cpp:1814-1848, 1873-1877).  Pure host arithmetic on
-    numpy arrays; used to prepare initial states."""
+    (phaseInitialization.cpp:1814-1848, 1873-1877); with the kEpsA model the
+    turbulent mass flux at (3, n) follows as rho a = a rho (:1850).  Pure host
+    arithmetic on numpy arrays; used to prepare initial states."""
     c = np.asarray(c, dtype=np.float64)
     v = np.asarray(v, dtype=np.float64)
     p = np.asarray(p, dtype=np.float64)
@@ -624,7 +631,10 @@ def conserved_from_primitive(case: Case, c, v, p, kt=None, et=None) -> np.ndarra
     et = np.maximum(np.asarray(et, dtype=np.float64), t.mineps)
     rk = kt * rho
     E = (U + rho * v2 * 0.5) + rk
-    return np.concatenate([rho_k, rho[None], m, E[None], rk[None], (rho * et)[None]])
+    rows = [rho_k, rho[None], m, E[None], rk[None], (rho * et)[None]]
+    if t.a_field:
+        rows.append(np.asarray(at, dtype=np.float64) * rho)
+    return np.concatenate(rows)
 
 
 def periodic(case: Case, axis: int):

@@ -4,10 +4,10 @@ FP64: the GPU step against the oracle in conserved-state mode, bit for bit,
 on the configurations of the reference's turbulent fixtures (advection only;
 the shear model with the diffusion stage, ideal and van der Waals mixtures,
 HLLC and HLL; 1-D van Leer slopes of k and epsilon with fixed-value and slip
-ends; 3-D with reversed top faces; the zero model) and on periodic boxes, and
-the device's "Courant-Source" step sequence capped by the shear model's
-source step.  Mixed precision against FP64.  The turbulent Kelvin-Helmholtz
-workload at full size.
+ends; 3-D with reversed top faces; the zero model; the kEpsA model with rho a
+in the state) and on periodic boxes, and the device's "Courant-Source" step
+sequence capped by the shear model's source step.  Mixed precision against
+FP64.  The turbulent Kelvin-Helmholtz workload at full size.
 """
 import dataclasses
 
@@ -25,14 +25,16 @@ from tests.golden_io import load
 
 FIXTURES = ["seeded_2d_turb_advection_hllc", "seeded_2d_turb_shear_hllc",
             "seeded_2d_turb_shear_vdw_hll", "seeded_1d_turb_shear_vanleer",
-            "seeded_3d_turb_shear_vanalbada", "seeded_2d_turb_zero_hllc"]
+            "seeded_3d_turb_shear_vanalbada", "seeded_2d_turb_zero_hllc",
+            "seeded_2d_kepsa_advection_hllc", "seeded_2d_kepsa_hllc", "seeded_2d_kepsa_vdw_hll",
+            "seeded_1d_kepsa_vanleer", "seeded_3d_kepsa_vanalbada"]
 
 
-def oracle_at(case, prim, turb):
+def oracle_at(case, prim, turb, a=None):
     from oracle.oracle import Oracle
 
     o = Oracle(case)
-    o.init_primitive(*prim, k=turb[0], eps=turb[1])
+    o.init_primitive(*prim, k=turb[0], eps=turb[1], a=a)
     return o
 
 
@@ -54,23 +56,43 @@ def seeded(case, seed):
     return (c, v, p), (1e-3 + 5e-3 * r.random(n), 1e-2 + 5e-2 * r.random(n))
 
 
+def seeded_a(case, seed):
+    a = 0.1 * (np.random.default_rng(seed + 1000).random((3, case.ncells)) - 0.5)
+    a[case.dim:] = 0
+    return a
+
+
+def seeded_kepsa(case, seed):
+    """Seeded state for the kEpsA model with pressure variations of 10 %.
+    The model's energy sink a . (grad p - div tau) enters the temperature
+    equation through distributeSourceTerm, which puts a negative source on
+    the diagonal without the time step (SLEMatrix.cpp:2166-2175 with the
+    explicit update of Diffusion.cpp:837-841), so every step cools a cell by
+    the factor C Cv / (C Cv + |sink| / T); on the rough states of seeded()
+    that drives the reference's temperature down within a few steps."""
+    (c, v, p), turb = seeded(case, seed)
+    return (c, v, 1.0 + 0.1 * (p - 1.0)), turb, seeded_a(case, seed)
+
+
 def periodic_case(dim=2, gas=gasModel.ideal, flux=FS.HLL, model=TM.shearModel):
     n = (16, 16, 8)
+    turb = Turbulence.kEpsA() if model == TM.kEpsAModel else Turbulence(model=model)
     case = Case(dim=dim, N=n, L=(1.0, 1.0, 1.0), M=[29.0, 4.0], Cv=[2.5, 1 / 0.67],
                 flux=flux, limiter=TL.minmod, diffusion=True, transport=(2e-3, 2e-3, 2e-3),
-                turbulence=Turbulence(model=model))
+                turbulence=turb)
     if gas == gasModel.vanDerWaals:
         case.gas_model, case.Tc, case.Pc = gas, [24.0, 3.2], [1.0, 1.0]
     for p in range(6):
         case.set_bc(p, BCT.periodic)
         case.turbulence.bc_k[p] = BC(BCT.periodic)
         case.turbulence.bc_eps[p] = BC(BCT.periodic)
+        case.turbulence.bc_a[p] = BC(BCT.periodic)
     return case
 
 
-def parity(case, prim, turb, dts):
+def parity(case, prim, turb, dts, a=None):
     """GPU FP64 steps against the oracle's conserved-state steps, bit for bit."""
-    o = oracle_at(case, prim, turb)
+    o = oracle_at(case, prim, turb, a)
     U0 = o.conserved()
     o = oracle_conserved(case, U0)
     g = GpuSolver(case)
@@ -89,7 +111,7 @@ def parity(case, prim, turb, dts):
 @pytest.mark.parametrize("name", FIXTURES)
 def test_fixture_configuration_bitexact(name, hip_device):
     g = load(name)
-    parity(g["case"], g["restart"], g["restart_turb"], list(g["dts"]))
+    parity(g["case"], g["restart"], g["restart_turb"], list(g["dts"]), g.get("restart_a"))
 
 
 @pytest.mark.gpu
@@ -99,11 +121,17 @@ def test_fixture_configuration_bitexact(name, hip_device):
     (2, gasModel.vanDerWaals, FS.HLLC, TM.zeroModel),
     (3, gasModel.ideal, FS.HLLC, TM.shearModel),
     (1, gasModel.vanDerWaals, FS.HLLC, TM.shearModel),
+    (2, gasModel.vanDerWaals, FS.HLL, TM.kEpsAModel),
+    (3, gasModel.ideal, FS.HLLC, TM.kEpsAModel),
+    (1, gasModel.ideal, FS.HLL, TM.kEpsAModel),
 ])
 def test_periodic_bitexact(dim, gas, flux, model, hip_device):
     case = periodic_case(dim, gas, flux, model)
-    prim, turb = seeded(case, 300 + dim)
-    parity(case, prim, turb, [2e-4] * 4)
+    if model == TM.kEpsAModel:
+        prim, turb, a = seeded_kepsa(case, 300 + dim)
+    else:
+        (prim, turb), a = seeded(case, 300 + dim), None
+    parity(case, prim, turb, [2e-4] * 4, a)
 
 
 @pytest.mark.gpu
@@ -150,22 +178,28 @@ def test_courant_source_sequence(hip_device):
 
 
 @pytest.mark.gpu
-def test_mixed_precision_turbulent(hip_device):
+@pytest.mark.parametrize("model", [TM.shearModel, TM.kEpsAModel])
+def test_mixed_precision_turbulent(model, hip_device):
     """Mixed FP32-flux / FP64-update path of the turbulent kernels against
-    FP64 after 40 steps of the periodic van der Waals shear case."""
-    case = periodic_case(2, gasModel.vanDerWaals, FS.HLL)
-    prim, turb = seeded(case, 77)
-    U0 = oracle_at(case, prim, turb).conserved()
+    FP64 after 40 steps of the periodic van der Waals case (shear model) or
+    10 steps (kEpsA, whose energy sink cools the state quickly; also rho a)."""
+    case = periodic_case(2, gasModel.vanDerWaals, FS.HLL, model)
+    if model == TM.kEpsAModel:
+        prim, turb, a = seeded_kepsa(case, 77)
+    else:
+        (prim, turb), a = seeded(case, 77), None
+    U0 = oracle_at(case, prim, turb, a).conserved()
     out = {}
     for prec in (PREC_FP64, PREC_MIXED):
         g = GpuSolver(case, prec)
         g.set_conserved(U0)
-        for _ in range(40):
+        for _ in range(10 if model == TM.kEpsAModel else 40):
             g.step(2e-4)
         out[prec] = g.conserved()
     a, b = out[PREC_FP64], out[PREC_MIXED]
     Ns = case.Ns
-    for q in (Ns, Ns + 4, Ns + 5, Ns + 6):
+    rows = (Ns, Ns + 4, Ns + 5, Ns + 6) + ((Ns + 7, Ns + 8) if model == TM.kEpsAModel else ())
+    for q in rows:
         l1 = np.abs(b[q] - a[q]).sum() / np.abs(a[q]).sum()
         assert l1 <= 1e-5, (q, l1)
     assert not np.array_equal(a, b)
@@ -176,7 +210,7 @@ def test_unsupported_turbulence(hip_device):
     case = periodic_case()
     for bad in (dict(limiter=TL.linear), dict(step_solver=0), dict(sle_solver="Jacobi"),
                 dict(molar_mass_correction=True), dict(gas_model=gasModel.RedlichKwong),
-                dict(turbulence=Turbulence(model=TM.kEpsAModel))):
+                dict(turbulence=Turbulence(model=TM.BHRModel))):
         with pytest.raises(SchemiGpuError):
             GpuSolver(dataclasses.replace(case, **bad))
 

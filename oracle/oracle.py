@@ -150,6 +150,12 @@ def lib():
     return _lib
 
 
+def _need(a, size, name):
+    """The C side reads exactly size doubles: refuse any other length."""
+    if a.size != size or a.dtype != np.float64:
+        raise ValueError(f"oracle: {name} needs {size} float64 values, got {a.size} {a.dtype}")
+
+
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
@@ -245,12 +251,19 @@ class Oracle:
         c = np.ascontiguousarray(c, dtype=np.float64).reshape(-1)
         v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
         p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        n = self.n
+        _need(c, self.case.Ns * n, "c")
+        _need(v, 3 * n, "v")
+        _need(p, n, "p")
         if self._cfg.turb:
             k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1)
             eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(-1)
+            _need(k, n, "k")
+            _need(eps, n, "eps")
             lib().or_init_turbulence(self._s, _dp(k), _dp(eps))
         if self.a_field:
             a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            _need(a, 3 * n, "a")
             lib().or_init_turbulence_a(self._s, _dp(a))
         lib().or_init_primitive(self._s, _dp(c), _dp(v), _dp(p))
 
@@ -264,14 +277,20 @@ class Oracle:
     def set_conserved(self, rho_k, rho, m, E, rk=None, reps=None, ra=None):
         """State from the conserved vector; with turbulence also rho k and
         rho epsilon, and with the kEpsA model rho a (3, n)."""
+        n = self.n
         if self._cfg.turb:
             t = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (rk, reps)]
+            _need(t[0], n, "rk")
+            _need(t[1], n, "reps")
             lib().or_set_turbulence_conserved(self._s, _dp(t[0]), _dp(t[1]))
         if self.a_field:
             ra = np.ascontiguousarray(ra, dtype=np.float64).reshape(-1)
+            _need(ra, 3 * n, "ra")
             lib().or_set_turbulence_a_conserved(self._s, _dp(ra))
         a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
              for x in (rho_k, rho, m, E)]
+        for x, size, name in zip(a, (self.case.Ns * n, n, 3 * n, n), ("rho_k", "rho", "m", "E")):
+            _need(x, size, name)
         lib().or_set_conserved(self._s, *[_dp(x) for x in a])
 
     def set_conserved_all(self, U):

@@ -39,6 +39,8 @@ def test_golden_fixtures(name, hip_device):
         kw = {}
         if "restart_turb" in g:
             kw = dict(k=g["restart_turb"][0], eps=g["restart_turb"][1])
+        if "restart_a" in g:
+            kw["a"] = g["restart_a"]
         o.init_primitive(*g["restart"], **kw)
         U0 = o.conserved()
     else:
@@ -61,6 +63,15 @@ def test_golden_fixtures(name, hip_device):
                 for q in range(2):
                     err = float((np.abs(kt[q] - kr[q]) / np.abs(kr[q])).max())
                     assert err <= bar, f"{name} step {step} turbulent field {q}: {err:.3e}"
+                if case.turbulence.a_field:
+                    # a = rho a / rho against the reference's a, per cell with a
+                    # floor of 1e-4 of its largest magnitude (components pass
+                    # through zero)
+                    at, ar = got[Ns + 7:Ns + 10] / got[Ns], g["snaps"][step]["a"]
+                    for c in range(case.dim):
+                        fl = 1e-4 * float(np.abs(ar[c]).max())
+                        err = rel_err(at[c], ar[c], fl)
+                        assert err <= bar, f"{name} step {step} a_{c}: {err:.3e}"
                 got = got[:Ns + 5]
             if step == 1:
                 for q, err in enumerate(per_cell_rel(got, ref, case.Ns)):

@@ -115,7 +115,8 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         "#Source_timestep_correction",
         ("Courant-Source=" if getattr(case, "timestep_source", False) else "Courant=")
         + ",".join(_r(x) for x in getattr(case, "source_coeffs", (0.25, 0.5))),
-        "#Reconstruction", "linear" if getattr(case, "diffusion", False) else "star",
+        "#Reconstruction", getattr(case, "reconstruction", "linear")
+        if getattr(case, "diffusion", False) else "star",
         "#Approximation_order", _STEP[int(case.step_solver)],
         "#Start_From_Time-Step", "1" if restart is not None else "no",
         "#Calculation_of_mixing_zone_width", "off",

@@ -75,6 +75,7 @@ class _Config(C.Structure):
         ("bc_eps", _BC * 6),
         ("sigmaa", C.c_double),
         ("bc_a", _BC * 6),
+        ("star_rec", C.c_int),
     ]
 
 
@@ -192,6 +193,7 @@ def make_config(case) -> _Config:
     cfg.molmass = 1 if getattr(case, "molar_mass_correction", False) else 0
     cfg.ts_source = 1 if getattr(case, "timestep_source", False) else 0
     cfg.ts_coeff = float(getattr(case, "source_coeffs", (0.25, 0.5))[0])
+    cfg.star_rec = 1 if getattr(case, "reconstruction", "linear") == "star" else 0
     cfg.flux = int(case.flux)
     cfg.limiter = int(case.limiter)
     cfg.gravity = 1 if case.gravity else 0

@@ -40,6 +40,10 @@ struct or_solver {
     double dt_src;                   /* source time step (mesh::timestepSource) */
     or_state st;
     double *pool;
+    /* star states (starFields) of the last stage and of the step, 8 per
+     * face: rho, v (3), p, a (3) */
+    double *star_stage, *star;
+    int hancock_step; /* the diffusion stage follows a MUSCL-Hancock step */
 };
 
 /* ------------------------------------------------------------------------ */
@@ -1253,6 +1257,8 @@ or_solver *or_create(const or_config *cfg)
         return NULL;
     }
     s->pool = state_pool_new(s->nc, cfg->Ns, &s->st);
+    s->star_stage = calloc(8 * (size_t)s->nf, sizeof(double));
+    s->star = calloc(8 * (size_t)s->nf, sizeof(double));
     return s;
 }
 
@@ -1279,6 +1285,8 @@ void or_destroy(or_solver *s)
     free(s->cnd);
     free(s->fper);
     free(s->pool);
+    free(s->star_stage);
+    free(s->star);
     free(s);
 }
 
@@ -1541,7 +1549,7 @@ static int bc_cell_vec(const or_bc *bc, const double *v, const double *N,
 /* (linearInterpolate.hpp:214-251)                                           */
 
 /* gradient of a scalar from its face values (gradient.hpp:19-58) */
-static void grad_faces(const or_solver *s, const double *qf, double *G /* 3 per cell */)
+static void grad_faces(const or_solver *s, const double *qf, int st, double *G /* 3 per cell */)
 {
     for (long i = 0; i < s->nc; ++i) {
         double acc[3] = {0, 0, 0};
@@ -1554,7 +1562,7 @@ static void grad_faces(const or_solver *s, const double *qf, double *G /* 3 per 
                 for (int a = 0; a < 3; ++a)
                     n[a] = s->fN[3 * f + a] * -1;
             for (int a = 0; a < 3; ++a)
-                acc[a] += (n[a] * qf[f]) * s->fS[f];
+                acc[a] += (n[a] * qf[st * f]) * s->fS[f];
         }
         for (int a = 0; a < 3; ++a)
             G[3 * i + a] = acc[a] / s->cV[i];
@@ -1575,9 +1583,31 @@ static int grad_scalar(or_solver *s, const double *q, const or_bc *bcs,
             return ERR_BC;
         }
     }
-    grad_faces(s, qf, G);
+    grad_faces(s, qf, 1, G);
     free(qf);
     return 0;
+}
+
+/* gradient of a vector from its face values vf[st * f + a] (gradient.hpp:60-93) */
+static void grad_vector_faces(const or_solver *s, const double *vf, int st, double *G)
+{
+    for (long i = 0; i < s->nc; ++i) {
+        double acc[9] = {0};
+        for (int j = 0; j < s->cnf[i]; ++j) {
+            const long f = s->cf[6 * i + j];
+            double n[3];
+            if (s->own[f] == i)
+                memcpy(n, &s->fN[3 * f], sizeof n);
+            else
+                for (int a = 0; a < 3; ++a)
+                    n[a] = s->fN[3 * f + a] * -1;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    acc[3 * a + b] += (vf[st * f + a] * n[b]) * s->fS[f];
+        }
+        for (int q = 0; q < 9; ++q)
+            G[9 * i + q] = acc[q] / s->cV[i];
+    }
 }
 
 static int grad_vector(or_solver *s, double *const *v, const or_bc *bcs,
@@ -1600,23 +1630,7 @@ static int grad_vector(or_solver *s, double *const *v, const or_bc *bcs,
             }
         }
     }
-    for (long i = 0; i < s->nc; ++i) {
-        double acc[9] = {0};
-        for (int j = 0; j < s->cnf[i]; ++j) {
-            const long f = s->cf[6 * i + j];
-            double n[3];
-            if (s->own[f] == i)
-                memcpy(n, &s->fN[3 * f], sizeof n);
-            else
-                for (int a = 0; a < 3; ++a)
-                    n[a] = s->fN[3 * f + a] * -1;
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b)
-                    acc[3 * a + b] += (vf[3 * f + a] * n[b]) * s->fS[f];
-        }
-        for (int q = 0; q < 9; ++q)
-            G[9 * i + q] = acc[q] / s->cV[i];
-    }
+    grad_vector_faces(s, vf, 3, G);
     free(vf);
     return 0;
 }
@@ -1841,7 +1855,41 @@ typedef struct {
     double E[3];
     double rk[3], reps[3]; /* turbulence */
     double ra[9];          /* kEpsA: rho a as a tensor like the momentum */
+    /* the face's star state (starFields): rho, v (3), p, a (3)
+     * (HLLCSolver.cpp:390-402, HLLSolver.cpp:302-314) */
+    double st[8];
 } face_flux;
+
+/* pressureStarClass::pressureStar (pressureStarClass.cpp:12-37): pFromUv of
+ * the star concentrations rho_k / M_k and the star internal energy
+ * (E - 0.5 (m & m) / rho) - rho k.  ds: Ns + 1 densities, ds[0] the total. */
+static double pressure_star(const or_config *cfg, const double *ds, const double *ms,
+                            double Es, double rks)
+{
+    double mm = 0;
+    for (int a = 0; a < 3; ++a)
+        mm += ms[a] * ms[a];
+    const double kin = 0.5 * mm / ds[0];
+    const double U = Es - kin - rks;
+    double cc[OR_MAXS + 1];
+    cc[0] = 0;
+    for (int k = 1; k <= cfg->Ns; ++k) {
+        cc[k] = ds[k] / cfg->M[k - 1];
+        cc[0] += cc[k];
+    }
+    return eos_p(cfg, cc, U);
+}
+
+/* star state of a face side taken as it is (supersonic branches) */
+static void star_side(const face_state *X, double *st)
+{
+    st[0] = X->rho[0];
+    for (int a = 0; a < 3; ++a)
+        st[1 + a] = X->v[a];
+    st[4] = X->p;
+    for (int c = 0; c < 3; ++c)
+        st[5 + c] = X->ra[c] / X->rho[0];
+}
 
 /* HLLCSolver::calculateFlows per-face body (HLLCSolver.cpp:52-388) */
 static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
@@ -1890,6 +1938,7 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
         for (int c = 0; c < 3; ++c) /* HLLCSolver.cpp:145-146 */
             for (int b = 0; b < 3; ++b)
                 F->ra[3 * c + b] = (O->ra[c] * uO) * N[b];
+        star_side(O, F->st); /* HLLCSolver.cpp:107-121 */
     } else if (Sc >= 0) {
         const double factor = SO - uO;
         const double den = 1 / (SO - Sc);
@@ -1927,7 +1976,13 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
             const double fa = O->ra[c] * uO + SO * (ras - O->ra[c]);
             for (int b = 0; b < 3; ++b)
                 F->ra[3 * c + b] = fa * N[b];
+            F->st[5 + c] = ras / rs[0];
         }
+        /* HLLCSolver.cpp:180-185 */
+        F->st[0] = rs[0];
+        for (int a = 0; a < 3; ++a)
+            F->st[1 + a] = ms[a] / rs[0];
+        F->st[4] = pressure_star(cfg, rs, ms, Es, rks);
     } else if (SN > 0) {
         const double factor = SN - uN;
         const double den = 1 / (SN - Sc);
@@ -1965,7 +2020,13 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
             const double fa = Nb->ra[c] * uN + SN * (ras - Nb->ra[c]);
             for (int b = 0; b < 3; ++b)
                 F->ra[3 * c + b] = fa * N[b];
+            F->st[5 + c] = ras / rs[0];
         }
+        /* HLLCSolver.cpp:275-280 */
+        F->st[0] = rs[0];
+        for (int a = 0; a < 3; ++a)
+            F->st[1 + a] = ms[a] / rs[0];
+        F->st[4] = pressure_star(cfg, rs, ms, Es, rks);
     } else if (SN <= 0) {
         for (int k = 0; k < nd; ++k)
             for (int a = 0; a < 3; ++a)
@@ -1986,6 +2047,7 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
         for (int c = 0; c < 3; ++c) /* HLLCSolver.cpp:380-381 */
             for (int b = 0; b < 3; ++b)
                 F->ra[3 * c + b] = (Nb->ra[c] * uN) * N[b];
+        star_side(Nb, F->st); /* HLLCSolver.cpp:344-358 */
     } else {
         return ERR_RIEMANN;
     }
@@ -2033,6 +2095,7 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
         for (int c = 0; c < 3; ++c) /* HLLSolver.cpp:117-118 */
             for (int b = 0; b < 3; ++b)
                 F->ra[3 * c + b] = (O->ra[c] * uO) * N[b];
+        star_side(O, F->st); /* HLLSolver.cpp:78-94 */
     } else if (SN > 0) {
         const double den = 1 / (SN - SO);
         for (int k = 0; k < nd; ++k) {
@@ -2072,6 +2135,24 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
             for (int b = 0; b < 3; ++b)
                 F->ra[3 * c + b] = fa * N[b];
         }
+        {
+            /* the HLL average state (HLLSolver.cpp:125-173) */
+            const double fN = SN - uN, fO = SO - uO;
+            double ds[OR_MAXS + 1], ms[3];
+            for (int k = 0; k < nd; ++k)
+                ds[k] = (Nb->rho[k] * fN - O->rho[k] * fO) * den;
+            for (int a = 0; a < 3; ++a)
+                ms[a] = (Nb->m[a] * fN - O->m[a] * fO - N[a] * ((Nb->p + ptN) - (O->p + ptO))) * den;
+            const double Es =
+                (Nb->E * fN - O->E * fO - ((Nb->p + ptN) * uN - (O->p + ptO) * uO)) * den;
+            const double rks = (Nb->rk * fN - O->rk * fO) * den;
+            F->st[0] = ds[0];
+            for (int a = 0; a < 3; ++a)
+                F->st[1 + a] = ms[a] / ds[0];
+            F->st[4] = pressure_star(cfg, ds, ms, Es, rks);
+            for (int c = 0; c < 3; ++c)
+                F->st[5 + c] = ((Nb->ra[c] * fN - O->ra[c] * fO) * den) / ds[0];
+        }
     } else if (SN <= 0) {
         for (int k = 0; k < nd; ++k)
             for (int a = 0; a < 3; ++a)
@@ -2092,6 +2173,7 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
         for (int c = 0; c < 3; ++c) /* HLLSolver.cpp:286-287 */
             for (int b = 0; b < 3; ++b)
                 F->ra[3 * c + b] = (Nb->ra[c] * uN) * N[b];
+        star_side(Nb, F->st); /* HLLSolver.cpp:255-270 */
     } else {
         return ERR_RIEMANN;
     }
@@ -2357,10 +2439,12 @@ static int stage_impl(or_solver *s, double dt, int hancock)
     if (err)
         goto done;
 
-    /* numerical fluxes (Advection2dOrder.cpp:528) */
+    /* numerical fluxes and the star states (Advection2dOrder.cpp:528) */
     for (long f = 0; f < nf && !err; ++f)
-        if (s->fpatch[f] != -3)
+        if (s->fpatch[f] != -3) {
             err = face_flux_dispatch(cfg, &s->fN[3 * f], &FO[f], &FN[f], &FF[f]);
+            memcpy(&s->star_stage[8 * f], FF[f].st, sizeof FF[f].st);
+        }
     if (err)
         goto done;
 
@@ -3185,6 +3269,15 @@ static int diffusion_explicit(or_solver *s, double dt)
     }
     if (err)
         goto done;
+    /* #Reconstruction star: grad v and div v from the step's star velocity
+     * (Diffusion.cpp:187-193); after a MUSCL-Hancock step with gravity the
+     * reference replaces it by the interpolated velocity
+     * (secondOrderStepSolver.cpp:74-76), which vf already holds */
+    const int star_v = cfg->star_rec && !(s->hancock_step && cfg->gravity);
+    if (star_v)
+        for (long f = 0; f < nf; ++f)
+            for (int a = 0; a < 3; ++a)
+                vf[3 * f + a] = s->star[8 * f + 1 + a];
     if (afield) {
         /* b = rho sum(x_k / (rho_k + chi rho)) / sum(x_k rho_k / (rho_k +
          * chi rho)) - 1, clipped at 0 (kEpsAModel.cpp:232-257).  The source
@@ -3203,20 +3296,28 @@ static int diffusion_explicit(or_solver *s, double dt)
             bC[i] = (b < 0.) ? 0.0 : b;
         }
         /* grad a, grad p (Diffusion.cpp:183-184, 531) and grad rho with the
-         * face density sum_k c_k,f M_k on boundary faces (calculatedDensity) */
-        err = grad_vector(s, st->a, cfg->bc_a, Ga);
-        if (!err)
-            err = grad_scalar(s, st->p, cfg->bc_p, GP);
-        if (err)
-            goto done;
-        for (long f = 0; f < nf; ++f) {
-            if (s->fpatch[f] == -3)
-                continue;
-            rhof[f] = (s->fpatch[f] < 0)
-                          ? st->rho[0][s->own[f]] * s->wo[f] + st->rho[0][s->nei[f]] * s->wn[f]
-                          : srho[f];
+         * face density sum_k c_k,f M_k on boundary faces (calculatedDensity);
+         * with star reconstruction the star states' (Diffusion.cpp:189-191,
+         * 536) */
+        if (cfg->star_rec) {
+            grad_vector_faces(s, &s->star[5], 8, Ga);
+            grad_faces(s, &s->star[4], 8, GP);
+            grad_faces(s, &s->star[0], 8, GR);
+        } else {
+            err = grad_vector(s, st->a, cfg->bc_a, Ga);
+            if (!err)
+                err = grad_scalar(s, st->p, cfg->bc_p, GP);
+            if (err)
+                goto done;
+            for (long f = 0; f < nf; ++f) {
+                if (s->fpatch[f] == -3)
+                    continue;
+                rhof[f] = (s->fpatch[f] < 0)
+                              ? st->rho[0][s->own[f]] * s->wo[f] + st->rho[0][s->nei[f]] * s->wn[f]
+                              : srho[f];
+            }
+            grad_faces(s, rhof, 1, GR);
         }
-        grad_faces(s, rhof, GR);
     }
     /* grad v and div v of the cells (gradient.hpp:19-58,
      * divergence.hpp:19-58) */
@@ -3727,14 +3828,21 @@ int or_step(or_solver *s, double dt, int mode)
 {
     if (mode == 1)
         recover_primitives(s, &s->st);
+    s->hancock_step = 0;
     or_state Un;
     double *pool = state_pool_new(s->nc, s->cfg.Ns, &Un);
     copy_state(&s->st, &Un, s->nc, s->cfg.Ns);
     int err = or_stage(s, dt);
-    if (!err)
+    if (!err) {
+        memcpy(s->star, s->star_stage, 8 * (size_t)s->nf * sizeof(double));
         err = or_stage(s, dt);
-    if (!err)
+    }
+    if (!err) {
+        /* star = (star + star1) / 2 (secondOrderStepSolverRK.cpp:73-85) */
+        for (long q = 0; q < 8 * s->nf; ++q)
+            s->star[q] = (s->star[q] + s->star_stage[q]) / 2;
         rk_average(s, &Un);
+    }
     if (!err && mode == 1)
         recover_primitives(s, &s->st);
     if (!err && s->cfg.diffusion) {
@@ -3756,6 +3864,9 @@ int or_step_hancock(or_solver *s, double dt, int mode)
     if (mode == 1)
         recover_primitives(s, &s->st);
     int err = stage_impl(s, dt, 1);
+    if (!err)
+        memcpy(s->star, s->star_stage, 8 * (size_t)s->nf * sizeof(double));
+    s->hancock_step = 1;
     if (!err && mode == 1)
         recover_primitives(s, &s->st);
     if (!err && s->cfg.diffusion) {

@@ -119,6 +119,10 @@ typedef struct {
     or_bc bc_eps[6];
     double sigmaa;
     or_bc bc_a[6];
+    /* #Reconstruction of main.txt for the diffusion stage: 0 linear, 1 star
+     * (the cell gradients of v, rho, p and a from the step's star states,
+     * Diffusion.cpp:176-193, 529-537) */
+    int star_rec;
 } or_config;
 
 /* turbulenceModel (turbulenceModelEnum.hpp) values of the built models */

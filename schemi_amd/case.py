@@ -130,6 +130,10 @@ class Case:
     # #Flag_for_molar_mass_diffusion_correction on (True) or off
     enthalpy_flow: bool = False
     molar_mass_correction: bool = False
+    # #Reconstruction of the stage's cell gradients of v, rho, p (and a):
+    # "linear" (interpolated cell values) or "star" (the step's Riemann star
+    # states on the faces, averaged over the two SSP-RK2 stages)
+    reconstruction: str = "linear"
     # #Source_timestep_correction: "Courant-Source" (True) or "Courant", and
     # its two coefficients (source, diffusion)
     timestep_source: bool = False

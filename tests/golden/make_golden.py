@@ -52,6 +52,8 @@ def case_json(case: Case) -> str:
             eos["enthalpy_flow"] = True
         if case.molar_mass_correction:
             eos["molar_mass_correction"] = True
+        if case.reconstruction != "linear":
+            eos["reconstruction"] = case.reconstruction
     if case.timestep_source:
         eos.update(timestep_source=True, source_coeffs=list(case.source_coeffs))
     t = case.turbulence
@@ -616,6 +618,85 @@ def kepsa_cases(base, nsteps=4, keep=(1, 4)):
             turbulence=Turbulence.kEpsA(), **base, **tr), 215, n=3, kp=(1, 3))
 
 
+def star_cases(base):
+    """#Reconstruction star: the diffusion stage takes the cell gradients of
+    v, rho, p (and a) from the Riemann star states of the faces, averaged over
+    the two SSP-RK2 stages (Diffusion.cpp:176-193, 529-537;
+    secondOrderStepSolverRK.cpp:73-85; HLLCSolver.cpp:390-402,
+    HLLSolver.cpp:302-314, pressureStarClass.cpp), or of the one
+    MUSCL-Hancock stage, whose star velocity is replaced by the interpolated
+    velocity under gravity (secondOrderStepSolver.cpp:74-76).  The
+    reconstruction every diffusion-on reference deck uses."""
+    from schemi_amd.case import Turbulence
+    st = dict(diffusion=True, transport=(0.004, 0.02, 0.002), reconstruction="star")
+    vdw = dict(gas_model=gasModel.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, **base, **st)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_star_hllc", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 231))
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+             **base, **st, **vdw)
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_star_vdw_hll", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 232))
+    c = Case(dim=1, N=(64, 1, 1), L=(1.0, 1.0, 1.0), flux=FS.HLL, limiter=TL.vanLeer,
+             **base, **st)
+    c.set_bc(0, BCT.slip)
+    c.set_bc(1, BCT.fixedValueCell, "p", (1.2, 0, 0))
+    c.set_bc(1, BCT.fixedValueCell, "v", (0.05, 0, 0))
+    c.set_bc(1, BCT.fixedValueCell, "c", (0.03, 0, 0))
+    run_const("seeded_1d_star_vanleer_hll", c, 5, 2e-4, (1, 5), restart=seeded_primitives(c, 233))
+    c = Case(dim=3, N=(8, 6, 6), L=(1.0, 0.5, 0.35), limiter=TL.vanAlbada, **base, **st)
+    for pt in (2, 5):
+        c.set_bc(pt, BCT.slip)
+    run_const("seeded_3d_star_vanalbada", c, 3, 2e-4, (1, 3), restart=seeded_primitives(c, 234))
+    # gravity with the SSP-RK2 step (the star velocity stays) and with
+    # MUSCL-Hancock (replaced by the interpolated velocity)
+    for name, extra, seed in (("seeded_2d_star_gravity", {}, 235),
+                              ("seeded_2d_star_hancock_gravity", dict(step_solver=0), 236),
+                              ("seeded_2d_star_hancock", dict(step_solver=0, gravity=False), 237)):
+        kw = dict(gravity=True, g=(0.0, -1.0, 0.0))
+        kw.update(extra)
+        c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod, flux=FS.HLL,
+                 **base, **st, **kw)
+        c.set_bc(3, BCT.slip)
+        c.set_bc(4, BCT.slip)
+        run_const(name, c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, seed))
+    # the Jacobi solver's heating term
+    c = Case(dim=2, N=(16, 12, 1), L=(0.5, 0.375, 0.1), limiter=TL.minmod,
+             **base, **dict(st, transport=(0.02, 0.1, 0.01), sle_solver="Jacobi"))
+    c.set_bc(3, BCT.slip)
+    c.set_bc(4, BCT.slip)
+    run_const("seeded_2d_star_jacobi", c, 4, 2e-4, (1, 4), restart=seeded_primitives(c, 238))
+    # turbulence: the shear model's grad v terms and kEpsA's grad p, grad rho
+    # and grad a
+    tr = dict(diffusion=True, transport=(2e-3, 2e-3, 2e-3), reconstruction="star")
+
+    def slip_y(c):
+        for p in (3, 4):
+            c.set_bc(p, BCT.slip)
+            c.turbulence.bc_a[p] = BC(BCT.slip)
+        return c
+
+    c = slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), limiter=TL.minmod,
+                    turbulence=Turbulence(), **base, **tr))
+    run_const("seeded_2d_star_turb_shear_hllc", c, 4, 2e-4, (1, 4),
+              restart=seeded_primitives(c, 239), restart_turb=seeded_turbulence(c, 239))
+    c = slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), flux=FS.HLL, limiter=TL.minmod,
+                    turbulence=Turbulence.kEpsA(), **base, **tr, **vdw))
+    run_const("seeded_2d_star_kepsa_vdw_hll", c, 4, 2e-4, (1, 4),
+              restart=seeded_primitives(c, 240), restart_turb=seeded_turbulence(c, 240),
+              restart_a=seeded_a(c, 240))
+    c = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer,
+             turbulence=Turbulence.kEpsA(), **base, **tr)
+    c.set_bc(0, BCT.fixedValueCell, "all", (0.03, 0.0, 0.0))
+    c.set_bc(1, BCT.slip)
+    c.turbulence.bc_a[1] = BC(BCT.slip)
+    run_const("seeded_1d_star_kepsa_vanleer", c, 4, 1e-3, (1, 4),
+              restart=seeded_primitives(c, 241), restart_turb=seeded_turbulence(c, 241),
+              restart_a=seeded_a(c, 241))
+
+
 def turbulence_source_case(base, nsteps=6, keep=(1, 3, 6)):
     """"Courant-Source" with the shear model's source step (shearModel.cpp:
     107-150): epsilon over its growth rate caps the next Courant step.  The
@@ -681,5 +762,7 @@ if __name__ == "__main__":
         turbulence_source_case(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "kepsa":
         kepsa_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "star":
+        star_cases(b)
     else:
         main()

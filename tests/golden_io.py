@@ -35,6 +35,7 @@ def load(name):
                 sle_iters=int(d.get("sle_iters", 2000)),
                 enthalpy_flow=bool(d.get("enthalpy_flow", False)),
                 molar_mass_correction=bool(d.get("molar_mass_correction", False)),
+                reconstruction=d.get("reconstruction", "linear"),
                 timestep_source=bool(d.get("timestep_source", False)),
                 source_coeffs=tuple(d.get("source_coeffs", (0.25, 0.5))),
                 turbulence=_turbulence(d.get("turbulence")))

@@ -247,7 +247,23 @@ typedef struct {
     double turb_sigma_a;
     double turb_kepsa_C[8];
     schemi_bc bc_a[6];
+    /* #Reconstruction of main.txt (Schemi.cpp:152-171) for the diffusion
+     * stage: SCHEMI_RECONSTRUCTION_LINEAR (0; grad v, div v and, with
+     * turbulence, grad rho, grad p, grad a from the interpolated cell values)
+     * or SCHEMI_RECONSTRUCTION_STAR (1; from the faces' Riemann star states,
+     * averaged over the two SSP-RK2 stages: Diffusion.cpp:176-193, 529-537,
+     * secondOrderStepSolverRK.cpp:73-85).  Star needs limiters whose slopes
+     * vanish or a 1-D TVD limiter (not the linear limiter); with
+     * MUSCL-Hancock it is built only under gravity, where the reference
+     * replaces the star velocity by the interpolated one
+     * (secondOrderStepSolver.cpp:74-76) and a laminar stage therefore equals
+     * the linear one.  Otherwise SCHEMI_ERR_INIT.  Ignored without the
+     * diffusion stage. */
+    int32_t reconstruction;
 } schemi_scheme_desc;
+
+#define SCHEMI_RECONSTRUCTION_LINEAR 0
+#define SCHEMI_RECONSTRUCTION_STAR 1
 
 /* turbulenceModel (Schemi/enumerations/turbulenceModelEnum.hpp) values */
 #define SCHEMI_TURB_ZERO 0

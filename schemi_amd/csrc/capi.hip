@@ -20,6 +20,7 @@
 #include "fv_launch.hpp"
 #include "fv_turb.hpp"
 #include "fv_turb_launch.hpp"
+#include "fv_star_launch.hpp"
 #include "geometry.hpp"
 
 using namespace sg;
@@ -224,6 +225,8 @@ struct schemi_gpu_ctx {
     int nf = 0;                  // conserved fields: ns + 5, turbulence ns + 7, kEpsA ns + 10
     bool turb = false;           // k-epsilon turbulence
     bool afield = false;         // kEpsA: the state carries rho a
+    bool star = false;           // #Reconstruction star: star states per face
+    double *dS = nullptr;        // star buffer, (5 + na) x hp.star_nft
     TurbParams tp{};
     TurbParams *dTP = nullptr;
     unsigned long long *epsmax = nullptr;  // largest epsilon of the state (bits)
@@ -332,6 +335,33 @@ int launch_stage(schemi_gpu_ctx *c, const double *Uin, double *Uout, const doubl
     return r == 0 ? SCHEMI_OK : SCHEMI_ERR_SYSTEM;
 }
 
+// Star states of the faces of state U (first stage: avg 0, second: 1).
+int launch_star_stage(schemi_gpu_ctx *c, const double *U, int avg)
+{
+    if (!c->star)
+        return SCHEMI_OK;
+    StarLaunch S;
+    S.P = c->dP;
+    S.TP = c->dTP;
+    S.geo = c->dgeo;
+    S.U = U;
+    S.S = c->dS;
+    S.sc = c->sc;
+    S.avg = avg;
+    S.dim = c->dim;
+    S.ns = c->ns;
+    S.gas_model = c->hp.gas_model;
+    S.slopes = (c->dim == 1 && c->mode == MODE_SLOPES) ? 1 : 0;
+    S.nt = c->turb ? 2 : 0;
+    S.na = c->afield ? 3 : 0;
+    S.n = c->n;
+    S.stream = c->stream;
+    const int r = launch_star(S);
+    if (r == -1)
+        return SCHEMI_ERR_INIT;
+    return r == 0 ? SCHEMI_OK : SCHEMI_ERR_SYSTEM;
+}
+
 int one_step(schemi_gpu_ctx *c)
 {
     if (c->hancock) {
@@ -343,8 +373,16 @@ int one_step(schemi_gpu_ctx *c)
             return r;
     } else {
         // secondOrderStepSolverRK::calculateStep (secondOrderStepSolverRK.cpp:48-71):
-        // U1 = L(Un); U2 = L(U1); U^{n+1} = 0.5 U2 + 0.5 Un (fused in stage 2)
-        int r = launch_stage(c, c->U[0], c->U[1], nullptr, SV_RK1);
+        // U1 = L(Un); U2 = L(U1); U^{n+1} = 0.5 U2 + 0.5 Un (fused in stage 2);
+        // with star reconstruction the faces' star states of each stage's
+        // input, averaged (secondOrderStepSolverRK.cpp:73-85)
+        int r = launch_star_stage(c, c->U[0], 0);
+        if (r)
+            return r;
+        r = launch_stage(c, c->U[0], c->U[1], nullptr, SV_RK1);
+        if (r)
+            return r;
+        r = launch_star_stage(c, c->U[1], 1);
         if (r)
             return r;
         r = launch_stage(c, c->U[1], c->U[0], c->U[0], SV_RK2);
@@ -378,6 +416,7 @@ int one_step(schemi_gpu_ctx *c)
         L.maxit = c->sle_iters;
         L.mm = c->molmass;
         L.en = c->enthalpy;
+        L.star = c->star ? c->dS : nullptr;
         if (c->Mb) {
             HIPCHK(hipMemsetAsync(c->dmax, 0,
                                   sizeof(double) * (size_t)(c->ns + 4) * (size_t)(c->sle_iters + 2),
@@ -614,6 +653,9 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
           scheme->diffusion_solver != SCHEMI_SLE_GAUSS_SEIDEL) ||
          (scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT && scheme->diffusion_iterations < 1) ||
          (scheme->diffusion_enthalpy_flow && scheme->diffusion_solver != SCHEMI_SLE_EXPLICIT)))
+        return SCHEMI_ERR_INIT;
+    if (scheme->reconstruction != SCHEMI_RECONSTRUCTION_LINEAR &&
+        scheme->reconstruction != SCHEMI_RECONSTRUCTION_STAR)
         return SCHEMI_ERR_INIT;
     if ((scheme->timestep_source != 0 && scheme->timestep_source != 1) ||
         !(scheme->source_coeff >= 0.0))
@@ -894,6 +936,33 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         c->mode = force_slopes ? MODE_SLOPES : MODE_FIRST_ORDER;
     else
         c->mode = MODE_SLOPES;
+    // #Reconstruction star (diffusion stage): star_kernel serves the meshes
+    // whose face values are the cell values and 1-D TVD slopes.  After a
+    // MUSCL-Hancock step under gravity the reference replaces the star
+    // velocity by the interpolated one, which is the linear stage of a laminar
+    // state; the other Hancock cases are not built.
+    if (c->diffusion && scheme->reconstruction == SCHEMI_RECONSTRUCTION_STAR) {
+        if (c->hancock) {
+            if (!scheme->gravity || c->turb) {
+                delete c;
+                return SCHEMI_ERR_INIT;
+            }
+        } else if (lim == SCHEMI_LIM_LINEAR || c->mode == MODE_GENERAL) {
+            delete c;
+            return SCHEMI_ERR_INIT;
+        } else {
+            c->star = true;
+        }
+    }
+    if (c->star) {
+        long long off = 0;
+        for (int a = 0; a < 3; ++a) {
+            P.star_off[a] = off;
+            if (a < dim)
+                off += (long long)(P.N[a] + 1) * (P.n / P.N[a]);
+        }
+        P.star_nft = off;
+    }
     // Mixed path of the ideal mixture with SSP-RK2 where the slopes vanish:
     // the lean kernel, when the mesh splits into whole 64 x SG_LEAN_TY tiles
     // and every wall is free or slip (SCHEMI_GPU_LEAN=0 keeps stage0_kernel)
@@ -978,7 +1047,10 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         ok = hipMalloc(&c->Wd, sizeof(double) *
                                    (size_t)(2 * mix->ns + (c->afield ? 13 : c->turb ? 9 : 5)) *
                                    (size_t)c->n) == hipSuccess;
-    if (ok && c->turb)
+    if (ok && c->star)
+        ok = hipMalloc(&c->dS, sizeof(double) * (size_t)(5 + (c->afield ? 3 : 0)) *
+                                   (size_t)P.star_nft) == hipSuccess;
+    if (ok && (c->turb || c->star))
         ok = hipMalloc(&c->dTP, sizeof(TurbParams)) == hipSuccess &&
              hipMalloc(&c->epsmax, sizeof(unsigned long long)) == hipSuccess &&
              hipMemcpyAsync(c->dTP, &c->tp, sizeof(TurbParams), hipMemcpyHostToDevice,
@@ -1044,6 +1116,7 @@ void schemi_gpu_destroy(schemi_gpu_ctx *c)
     (void)hipFree(c->done);
     (void)hipFree(c->dTP);
     (void)hipFree(c->epsmax);
+    (void)hipFree(c->dS);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;

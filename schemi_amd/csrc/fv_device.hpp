@@ -85,6 +85,9 @@ struct DevParams {
     double courant, min_time;
     int ts_source;     // 1: the Courant step is capped by the source step
     double ts_coeff;   // source step coefficient
+    // #Reconstruction star: first face of each axis in the star buffer and
+    // the number of faces (fv_star.hpp)
+    long long star_off[3], star_nft;
 };
 
 // patch of axis a, side (0 = low, 1 = high): tail/point, right/left, bottom/top

@@ -38,6 +38,7 @@
 #include "fv_general.hpp"
 #include "fv_turb.hpp"
 #include "fv_turb_launch.hpp"
+#include "fv_star.hpp"
 
 namespace sg {
 
@@ -413,7 +414,8 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                    double *Ualt, int hk, const double *__restrict__ W, double *sc,
                    int *errflag, double mu, double Dc, double kappa, double *Mb, int mm, int en,
                    const TurbParams *__restrict__ Tp = nullptr,
-                   const unsigned long long *__restrict__ epsmax = nullptr)
+                   const unsigned long long *__restrict__ epsmax = nullptr,
+                   const double *__restrict__ star = nullptr)
 {
     static_assert(!(TURB && (IMPL || SPEC)), "turbulence: explicit solver without species terms");
     static_assert(TURB || !AF, "the a field needs turbulence");
@@ -711,6 +713,22 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
 #pragma unroll
                     for (int k = 0; k < NS; ++k)
                         gXa[k] = ((cgh[k] / cg0 - w0[L::C + k] / c0o) / m) * nrm[a];
+                }
+            }
+            if (star) {
+                // #Reconstruction star: the step's star states of the face
+                // for grad v, div v (and grad rho, grad p, grad a)
+                const long long nft = P.star_nft;
+                const long long fs = star_face(P, a, f, idx);
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    vs[c] = star[(1 + c) * nft + fs];
+                if constexpr (AF) {
+                    rs_ = star[fs];
+                    ps_ = star[4 * nft + fs];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        as_[c] = star[(5 + c) * nft + fs];
                 }
             }
             if constexpr (SPEC) {
@@ -1396,7 +1414,7 @@ int launch_dim_turb(const DiffusionLaunch &L, const TurbDiffusion &T)
         return 1;
     hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false, false, true, AF>),
                        dim3(blocks), dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W,
-                       L.sc, L.err, L.mu, L.D, L.kappa, nullptr, 0, 0, T.TP, T.epsmax);
+                       L.sc, L.err, L.mu, L.D, L.kappa, nullptr, 0, 0, T.TP, T.epsmax, L.star);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -1413,11 +1431,13 @@ int launch_dim(const DiffusionLaunch &L)
         if (spec)
             hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false, true>), dim3(blocks),
                                dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc,
-                               L.err, L.mu, L.D, L.kappa, nullptr, L.mm, L.en);
+                               L.err, L.mu, L.D, L.kappa, nullptr, L.mm, L.en, nullptr, nullptr,
+                               L.star);
         else
             hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false, false>), dim3(blocks),
                                dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc,
-                               L.err, L.mu, L.D, L.kappa, nullptr, 0, 0);
+                               L.err, L.mu, L.D, L.kappa, nullptr, 0, 0, nullptr, nullptr,
+                               L.star);
         return hipGetLastError() == hipSuccess ? 0 : 1;
     }
     // Jacobi: assemble, solve Y_k and v_c, finish Y, solve T, finish; the
@@ -1425,11 +1445,11 @@ int launch_dim(const DiffusionLaunch &L)
     if (spec)
         hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, true, true>), dim3(blocks),
                            dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc, L.err,
-                           L.mu, L.D, L.kappa, L.Mb, L.mm, 0);
+                           L.mu, L.D, L.kappa, L.Mb, L.mm, 0, nullptr, nullptr, L.star);
     else
         hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, true, false>), dim3(blocks),
                            dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W, L.sc, L.err,
-                           L.mu, L.D, L.kappa, L.Mb, 0, 0);
+                           L.mu, L.D, L.kappa, L.Mb, 0, 0, nullptr, nullptr, L.star);
     auto solve = [&](int f0, int f1) -> int {
         int it = 1, batch = 4;
         for (;;) {

@@ -105,6 +105,9 @@ struct DiffusionLaunch {
     int *done, *done_host;
     int maxit;
     int mm, en;  // molar-mass correction, explicit enthalpy flow
+    // #Reconstruction star: the step's star states (fv_star.hpp), or null
+    // for linear reconstruction
+    const double *star;
 };
 int launch_diffusion(const DiffusionLaunch &L);
 size_t diffusion_matrix_doubles(int ns);

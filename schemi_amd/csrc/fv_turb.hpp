@@ -483,9 +483,12 @@ turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict
             s[NS + 3] = p;
             s[NS + 4] = Lt[d + 1][0] * off + w[d + 2][NS + 4];
             s[NS + 5] = Lt[d + 1][1] * off + w[d + 2][NS + 5];
+            if constexpr (NA > 0) {
+                s[NS + 6] = La[d + 1] * off + w[d + 2][NS + 6];
 #pragma unroll
-            for (int c = 0; c < NA; ++c)
-                s[NS + 6 + c] = (c == 0) ? La[d + 1] * off + w[d + 2][NS + 6] : w[d + 2][NS + 6 + c];
+                for (int c = 1; c < NA; ++c)
+                    s[NS + 6 + c] = w[d + 2][NS + 6 + c];
+            }
         };
         T so[NQ], sb[NQ];
         // x- face

@@ -75,7 +75,8 @@ class SchemeDesc(C.Structure):
                 ("turb_CMS_D", C.c_double), ("turb_min_k", C.c_double),
                 ("turb_min_eps", C.c_double), ("turb_C", C.c_double * 3),
                 ("bc_k", _BC * 6), ("bc_eps", _BC * 6), ("turb_sigma_a", C.c_double),
-                ("turb_kepsa_C", C.c_double * 8), ("bc_a", _BC * 6)]
+                ("turb_kepsa_C", C.c_double * 8), ("bc_a", _BC * 6),
+                ("reconstruction", C.c_int32)]
 
 
 _LIB = None
@@ -218,6 +219,7 @@ def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) 
     s.diffusion_enthalpy_flow = 1 if case.enthalpy_flow else 0
     s.timestep_source = 1 if case.timestep_source else 0
     s.source_coeff = float(case.source_coeffs[0])
+    s.reconstruction = 1 if case.reconstruction == "star" else 0
     t = case.turbulence
     if t is not None:
         s.turbulence = 1

@@ -34,6 +34,12 @@ def test_golden_fixtures(name, hip_device):
 
     g = load(name)
     case = g["case"]
+    if case.reconstruction == "star" and case.step_solver == 0 and not case.gravity:
+        # star reconstruction after a MUSCL-Hancock step is built only under
+        # gravity (where the reference uses the interpolated velocity)
+        with pytest.raises(SchemiGpuError):
+            GpuSolver(case)
+        return
     o = Oracle(case)
     if "restart" in g:
         kw = {}

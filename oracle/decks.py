@@ -69,7 +69,7 @@ def _free_lines(case):
 
 def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
                const_dt=-1.0, min_time=None, courant=None, restart_turb=None,
-               zones_turb=None, restart_a=None):
+               zones_turb=None, restart_a=None, restart_b=None):
     """Writes ``root/set`` for ``case``.
 
     zones:   ((c_left[Ns], v_left[3], p_left), (c_right, v_right, p_right), x0)
@@ -78,8 +78,9 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
              cell order, injected through the restart reader.
     restart_turb: (k[n], eps[n]) initial turbulent fields for the restart
              reader (zeros when None).
-    restart_a: a[3, n], the initial turbulent mass flux of the kEpsA model
-             for the restart reader (zeros when None).
+    restart_a: a[3, n], the initial turbulent mass flux of the kEpsA and
+             BHR models for the restart reader (zeros when None).
+    restart_b: b[n], the initial b of the BHR model (zeros when None).
     zones_turb: ((k_left, eps_left), (k_right, eps_right)) zone values.
     """
     set_dir = os.path.join(root, "set")
@@ -173,9 +174,11 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
         files["epsilon.txt"] = _bc_lines(case, turb.bc_eps) + [
             "#Values_in_zones", _r(kz[0][1]), _r(kz[1][1])]
     files["a.txt"] = _free_lines(case) + ["#Values_in_zones", "0 0 0", "0 0 0"]
-    if turb is not None and int(turb.model) == 6:
+    if turb is not None and int(turb.model) in (6, 7):
         files["a.txt"] = _bc_lines(case, turb.bc_a, vec=True) + [
             "#Values_in_zones", "0 0 0", "0 0 0"]
+    if turb is not None and int(turb.model) == 7:
+        files["b.txt"] = _bc_lines(case, turb.bc_b) + ["#Values_in_zones", "0", "0"]
     files["chemicalKinetics.txt"] = ["#Type_of_chemical_reaction", "no"]
     files["MPIRanks.txt"] = ["#MPI_nodes_in_directions", "1 1 1"]
     files["GoncharovModel.txt"] = ["off", "#Number_of_particles", "1",
@@ -221,6 +224,13 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
             # eight model constants
             names = ("C0", "C1", "C2", "C3", "Ca", "CaMax", "CMSA", "chi")
             tp += [("sigma_a", turb.sigmaa)] + cms + mins + list(zip(names, turb.C[:8]))
+        elif int(turb.model) == 7:
+            # BHRModel.cpp:42-83: sigma_a, sigma_b, CMS, the minimal k,
+            # epsilon and b, then the twelve model constants
+            names = ("C0", "C1", "C2", "C3", "C4", "Ca", "Cb1", "Cb2", "alpha_2", "alpha_3",
+                     "alpha_4", "CMS_M")
+            tp += ([("sigma_a", turb.sigmaa), ("sigma_b", turb.sigmab)] + cms + mins +
+                   [("Minimal_b", turb.minb)] + list(zip(names, turb.C[:12])))
         else:
             tp += mins + cms
         files["turbulentParameters.txt"] = [y for name, val in tp for y in ("#" + name, _r(val))]
@@ -245,7 +255,8 @@ def write_deck(root, case, *, t_end, zones=None, restart=None, out_freq=1,
                    delimiter="\t")
         z = np.zeros(n)
         kd, ed = (restart_turb if restart_turb is not None else (z, z))
-        for name, arr in (("k.dat", kd), ("epsilon.dat", ed), ("b.dat", z)):
+        bd = z if restart_b is None else np.asarray(restart_b)
+        for name, arr in (("k.dat", kd), ("epsilon.dat", ed), ("b.dat", bd)):
             np.savetxt(os.path.join(d, name), np.asarray(arr), fmt="%.20g")
         ad = np.zeros((3, n)) if restart_a is None else np.asarray(restart_a)
         np.savetxt(os.path.join(d, "a.dat"), ad.T, fmt="%.20g", delimiter="\t")

@@ -70,11 +70,14 @@ class _Config(C.Structure):
         ("CMS_D", C.c_double),
         ("mink", C.c_double),
         ("mineps", C.c_double),
-        ("tC", C.c_double * 8),
+        ("tC", C.c_double * 12),
         ("bc_k", _BC * 6),
         ("bc_eps", _BC * 6),
         ("sigmaa", C.c_double),
         ("bc_a", _BC * 6),
+        ("sigmab", C.c_double),
+        ("minb", C.c_double),
+        ("bc_b", _BC * 6),
         ("star_rec", C.c_int),
     ]
 
@@ -98,6 +101,7 @@ class _State(C.Structure):
         ("a", C.POINTER(C.c_double) * 3),
         ("ra", C.POINTER(C.c_double) * 3),
         ("b", C.POINTER(C.c_double)),
+        ("rb", C.POINTER(C.c_double)),
     ]
 
 
@@ -128,6 +132,8 @@ def lib():
         L.or_set_turbulence_conserved.argtypes = [C.c_void_p, dp, dp]
         L.or_init_turbulence_a.argtypes = [C.c_void_p, dp]
         L.or_set_turbulence_a_conserved.argtypes = [C.c_void_p, dp]
+        L.or_init_turbulence_b.argtypes = [C.c_void_p, dp]
+        L.or_set_turbulence_b_conserved.argtypes = [C.c_void_p, dp]
         L.or_step.restype = C.c_int
         L.or_step.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.or_step_hancock.restype = C.c_int
@@ -206,8 +212,11 @@ def make_config(case) -> _Config:
         for i, x in enumerate(tb.C):
             cfg.tC[i] = float(x)
         cfg.sigmaa = float(tb.sigmaa)
+        cfg.sigmab = float(tb.sigmab)
+        cfg.minb = float(tb.minb)
         for p in range(6):
-            for dst, src in ((cfg.bc_k, tb.bc_k), (cfg.bc_eps, tb.bc_eps), (cfg.bc_a, tb.bc_a)):
+            for dst, src in ((cfg.bc_k, tb.bc_k), (cfg.bc_eps, tb.bc_eps), (cfg.bc_a, tb.bc_a),
+                             (cfg.bc_b, tb.bc_b)):
                 dst[p].type = int(src[p].type)
                 for a in range(3):
                     dst[p].val[a] = float(src[p].value[a])
@@ -244,9 +253,13 @@ class Oracle:
     # -- state ---------------------------------------------------------------
     @property
     def a_field(self) -> bool:
-        return bool(self._cfg.turb) and self._cfg.turb_model == 6
+        return bool(self._cfg.turb) and self._cfg.turb_model in (6, 7)
 
-    def init_primitive(self, c, v, p, k=None, eps=None, a=None):
+    @property
+    def b_field(self) -> bool:
+        return bool(self._cfg.turb) and self._cfg.turb_model == 7
+
+    def init_primitive(self, c, v, p, k=None, eps=None, a=None, b=None):
         """Initial state from primitives; with turbulence also k and epsilon
         (clipped to the minimal values as the reference's initialisation) and
         with the kEpsA model a (3, n)."""
@@ -267,6 +280,10 @@ class Oracle:
             a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
             _need(a, 3 * n, "a")
             lib().or_init_turbulence_a(self._s, _dp(a))
+        if self.b_field:
+            b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+            _need(b, n, "b")
+            lib().or_init_turbulence_b(self._s, _dp(b))
         lib().or_init_primitive(self._s, _dp(c), _dp(v), _dp(p))
 
     def set_full(self, st: dict):
@@ -276,7 +293,7 @@ class Oracle:
                           _dp(a["rho"]), _dp(a["v"]), _dp(a["p"]), _dp(a["U"]),
                           _dp(a["T"]), _dp(a["m"]), _dp(a["E"]))
 
-    def set_conserved(self, rho_k, rho, m, E, rk=None, reps=None, ra=None):
+    def set_conserved(self, rho_k, rho, m, E, rk=None, reps=None, ra=None, rb=None):
         """State from the conserved vector; with turbulence also rho k and
         rho epsilon, and with the kEpsA model rho a (3, n)."""
         n = self.n
@@ -289,6 +306,10 @@ class Oracle:
             ra = np.ascontiguousarray(ra, dtype=np.float64).reshape(-1)
             _need(ra, 3 * n, "ra")
             lib().or_set_turbulence_a_conserved(self._s, _dp(ra))
+        if self.b_field:
+            rb = np.ascontiguousarray(rb, dtype=np.float64).reshape(-1)
+            _need(rb, n, "rb")
+            lib().or_set_turbulence_b_conserved(self._s, _dp(rb))
         a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
              for x in (rho_k, rho, m, E)]
         for x, size, name in zip(a, (self.case.Ns * n, n, 3 * n, n), ("rho_k", "rho", "m", "E")):
@@ -299,7 +320,10 @@ class Oracle:
         """State from a (nfields, n) conserved array in the library's layout:
         rho_1..rho_Ns, rho, m_x, m_y, m_z, E[, rho k, rho epsilon[, rho a]]."""
         Ns = self.case.Ns
-        if self.a_field:
+        if self.b_field:
+            self.set_conserved(U[:Ns], U[Ns], U[Ns + 1:Ns + 4], U[Ns + 4], U[Ns + 5], U[Ns + 6],
+                               U[Ns + 7:Ns + 10], U[Ns + 10])
+        elif self.a_field:
             self.set_conserved(U[:Ns], U[Ns], U[Ns + 1:Ns + 4], U[Ns + 4], U[Ns + 5], U[Ns + 6],
                                U[Ns + 7:Ns + 10])
         elif self._cfg.turb:
@@ -332,6 +356,7 @@ class Oracle:
             "a": np.stack([arr(st.a[c]) for c in range(3)]),
             "ra": np.stack([arr(st.ra[c]) for c in range(3)]),
             "b": arr(st.b),
+            "rb": arr(st.rb),
         }
 
     def conserved(self) -> np.ndarray:
@@ -344,6 +369,8 @@ class Oracle:
             rows += [s["rk"][None], s["reps"][None]]
         if self.a_field:
             rows += [s["ra"]]
+        if self.b_field:
+            rows += [s["rb"][None]]
         return np.concatenate(rows)
 
     # -- stepping ------------------------------------------------------------

@@ -44,7 +44,18 @@ struct or_solver {
      * face: rho, v (3), p, a (3) */
     double *star_stage, *star;
     int hancock_step; /* the diffusion stage follows a MUSCL-Hancock step */
+    int keep_b;       /* recover_primitives leaves b as it is */
 };
+
+/* star values per face: rho, v (3), p, a (3), b */
+#define NSTAR 9
+
+/* the model transports a (kEpsA, BHR) / b (BHR) */
+static int a_field(const or_config *c)
+{
+    return c->turb && (c->turb_model == OR_TURB_KEPSA || c->turb_model == OR_TURB_BHR);
+}
+static int b_field(const or_config *c) { return c->turb && c->turb_model == OR_TURB_BHR; }
 
 /* ------------------------------------------------------------------------ */
 /* primitives                                                                */
@@ -1163,11 +1174,12 @@ static void copy_state(const or_state *src, or_state *dst, long n, int Ns)
         memcpy(dst->ra[a], src->ra[a], n * sizeof(double));
     }
     memcpy(dst->b, src->b, n * sizeof(double));
+    memcpy(dst->rb, src->rb, n * sizeof(double));
 }
 
 static double *state_pool_new(long n, int Ns, or_state *st)
 {
-    const int nfield = 2 * (Ns + 1) + 3 + 3 + 3 + 1 + 4 + 7;
+    const int nfield = 2 * (Ns + 1) + 3 + 3 + 3 + 1 + 4 + 8;
     double *pool = calloc((size_t)nfield * n, sizeof(double));
     double *q = pool;
     st->n = n;
@@ -1208,6 +1220,7 @@ static double *state_pool_new(long n, int Ns, or_state *st)
     }
     q += 3 * n;
     st->b = q;
+    st->rb = q + n;
     return pool;
 }
 
@@ -1245,7 +1258,7 @@ or_solver *or_create(const or_config *cfg)
      * the diffusion stage (if on) with the explicit solver and neither the
      * molar-mass correction nor the enthalpy flow */
     if (cfg->turb && ((cfg->turb_model != OR_TURB_SHEAR && cfg->turb_model != OR_TURB_ZERO &&
-                       cfg->turb_model != OR_TURB_KEPSA) ||
+                       cfg->turb_model != OR_TURB_KEPSA && cfg->turb_model != OR_TURB_BHR) ||
                       (cfg->gas_model != OR_GAS_IDEAL && cfg->gas_model != OR_GAS_VDW) ||
                       (cfg->diffusion && (cfg->sle_solver != OR_SLE_EXPLICIT || cfg->molmass ||
                                           cfg->enth_flow)))) {
@@ -1257,8 +1270,8 @@ or_solver *or_create(const or_config *cfg)
         return NULL;
     }
     s->pool = state_pool_new(s->nc, cfg->Ns, &s->st);
-    s->star_stage = calloc(8 * (size_t)s->nf, sizeof(double));
-    s->star = calloc(8 * (size_t)s->nf, sizeof(double));
+    s->star_stage = calloc(NSTAR * (size_t)s->nf, sizeof(double));
+    s->star = calloc(NSTAR * (size_t)s->nf, sizeof(double));
     return s;
 }
 
@@ -1329,9 +1342,11 @@ void or_init_primitive(or_solver *s, const double *c, const double *v,
             st->rk[i] = st->k[i] * r0;
             st->reps[i] = r0 * st->eps[i];
         }
-        if (s->cfg.turb && s->cfg.turb_model == OR_TURB_KEPSA)
+        if (a_field(&s->cfg))
             for (int a = 0; a < 3; ++a) /* phaseInitialization.cpp:1850 */
                 st->ra[a][i] = st->a[a][i] * r0;
+        if (b_field(&s->cfg)) /* phaseInitialization.cpp:1852 */
+            st->rb[i] = r0 * st->b[i];
         double vel[3] = {st->v[0][i], st->v[1][i], st->v[2][i]};
         const double v2 = dot3(vel, vel);
         st->E[i] = (st->U[i] + r0 * v2 * 0.5) + st->rk[i];
@@ -1358,10 +1373,21 @@ void or_init_turbulence_a(or_solver *s, const double *a)
 {
     or_state *st = &s->st;
     const long n = s->nc;
-    const int on = s->cfg.turb && s->cfg.turb_model == OR_TURB_KEPSA;
+    const int on = a_field(&s->cfg);
     for (int c = 0; c < 3; ++c)
         for (long i = 0; i < n; ++i)
             st->a[c][i] = on ? a[c * n + i] : 0.0;
+}
+
+void or_init_turbulence_b(or_solver *s, const double *b)
+{
+    for (long i = 0; i < s->nc; ++i)
+        s->st.b[i] = b_field(&s->cfg) ? b[i] : 0.0;
+}
+
+void or_set_turbulence_b_conserved(or_solver *s, const double *rb)
+{
+    memcpy(s->st.rb, rb, s->nc * sizeof(double));
 }
 
 void or_set_turbulence_a_conserved(or_solver *s, const double *ra)
@@ -1428,9 +1454,11 @@ static void recover_primitives(or_solver *s, or_state *st)
             st->k[i] = st->rk[i] / st->rho[0][i];
             st->eps[i] = st->reps[i] / st->rho[0][i];
         }
-        if (s->cfg.turb && s->cfg.turb_model == OR_TURB_KEPSA)
+        if (a_field(&s->cfg))
             for (int a = 0; a < 3; ++a)
                 st->a[a][i] = st->ra[a][i] / st->rho[0][i];
+        if (b_field(&s->cfg) && !s->keep_b) /* Advection2dOrder.cpp:623-625 */
+            st->b[i] = st->rb[i] / st->rho[0][i];
     }
 }
 
@@ -1819,6 +1847,7 @@ typedef struct {
     double p, U, T, E;
     double k, eps, rk, reps; /* turbulence (zero without it) */
     double a[3], ra[3];      /* kEpsA turbulent mass flux (zero otherwise) */
+    double b, rb;            /* BHR b (zero otherwise) */
 } face_state;
 
 /* Face-side closure: Advection2dOrder.cpp:233-289 (owner side) and 301-372
@@ -1838,6 +1867,7 @@ static void close_face(const or_config *cfg, face_state *fs)
         fs->reps = fs->eps * r0;
         for (int a = 0; a < 3; ++a) /* Advection2dOrder.cpp:257-260, 336-340 */
             fs->ra[a] = fs->a[a] * r0;
+        fs->rb = fs->b * r0; /* Advection2dOrder.cpp:262-264, 342-345, 480-482 */
     }
     fs->U = eos_U(cfg, fs->c, fs->p);
     fs->T = eos_T(cfg, fs->c, fs->U);
@@ -1857,7 +1887,8 @@ typedef struct {
     double ra[9];          /* kEpsA: rho a as a tensor like the momentum */
     /* the face's star state (starFields): rho, v (3), p, a (3)
      * (HLLCSolver.cpp:390-402, HLLSolver.cpp:302-314) */
-    double st[8];
+    double st[NSTAR];
+    double rb[3]; /* BHR: rho b */
 } face_flux;
 
 /* pressureStarClass::pressureStar (pressureStarClass.cpp:12-37): pFromUv of
@@ -1889,6 +1920,7 @@ static void star_side(const face_state *X, double *st)
     st[4] = X->p;
     for (int c = 0; c < 3; ++c)
         st[5 + c] = X->ra[c] / X->rho[0];
+    st[8] = X->rb / X->rho[0];
 }
 
 /* HLLCSolver::calculateFlows per-face body (HLLCSolver.cpp:52-388) */
@@ -1934,6 +1966,7 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
         for (int a = 0; a < 3; ++a) { /* HLLCSolver.cpp:126-133 */
             F->rk[a] = (uO * O->rk) * N[a];
             F->reps[a] = (uO * N[a]) * O->reps;
+            F->rb[a] = (uO * N[a]) * O->rb; /* HLLCSolver.cpp:147-148 */
         }
         for (int c = 0; c < 3; ++c) /* HLLCSolver.cpp:145-146 */
             for (int b = 0; b < 3; ++b)
@@ -1983,6 +2016,14 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
         for (int a = 0; a < 3; ++a)
             F->st[1 + a] = ms[a] / rs[0];
         F->st[4] = pressure_star(cfg, rs, ms, Es, rks);
+        {
+            /* HLLCSolver.cpp:177-178, 237-243 */
+            const double rbs = O->rb * factor * den;
+            const double fb = O->rb * uO + SO * (rbs - O->rb);
+            for (int a = 0; a < 3; ++a)
+                F->rb[a] = fb * N[a];
+            F->st[8] = rbs / rs[0];
+        }
     } else if (SN > 0) {
         const double factor = SN - uN;
         const double den = 1 / (SN - Sc);
@@ -2027,6 +2068,14 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
         for (int a = 0; a < 3; ++a)
             F->st[1 + a] = ms[a] / rs[0];
         F->st[4] = pressure_star(cfg, rs, ms, Es, rks);
+        {
+            /* HLLCSolver.cpp:272-273, 334-340 */
+            const double rbs = Nb->rb * factor * den;
+            const double fb = Nb->rb * uN + SN * (rbs - Nb->rb);
+            for (int a = 0; a < 3; ++a)
+                F->rb[a] = fb * N[a];
+            F->st[8] = rbs / rs[0];
+        }
     } else if (SN <= 0) {
         for (int k = 0; k < nd; ++k)
             for (int a = 0; a < 3; ++a)
@@ -2043,6 +2092,7 @@ static int hllc_face(const or_config *cfg, const double *N, const face_state *O,
         for (int a = 0; a < 3; ++a) { /* HLLCSolver.cpp:368-375 */
             F->rk[a] = (uN * Nb->rk) * N[a];
             F->reps[a] = (uN * N[a]) * Nb->reps;
+            F->rb[a] = (uN * N[a]) * Nb->rb; /* HLLCSolver.cpp:382-383 */
         }
         for (int c = 0; c < 3; ++c) /* HLLCSolver.cpp:380-381 */
             for (int b = 0; b < 3; ++b)
@@ -2091,6 +2141,7 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
         for (int a = 0; a < 3; ++a) { /* HLLSolver.cpp:107-114 */
             F->rk[a] = (uO * O->rk) * N[a];
             F->reps[a] = (uO * O->reps) * N[a];
+            F->rb[a] = (uO * O->rb) * N[a]; /* HLLSolver.cpp:119-120 */
         }
         for (int c = 0; c < 3; ++c) /* HLLSolver.cpp:117-118 */
             for (int b = 0; b < 3; ++b)
@@ -2152,6 +2203,11 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
             F->st[4] = pressure_star(cfg, ds, ms, Es, rks);
             for (int c = 0; c < 3; ++c)
                 F->st[5 + c] = ((Nb->ra[c] * fN - O->ra[c] * fO) * den) / ds[0];
+            F->st[8] = ((Nb->rb * fN - O->rb * fO) * den) / ds[0];
+            /* HLLSolver.cpp:245-251 */
+            const double fb = (SN * uO * O->rb - SO * uN * Nb->rb + SN * SO * (Nb->rb - O->rb)) * den;
+            for (int a = 0; a < 3; ++a)
+                F->rb[a] = fb * N[a];
         }
     } else if (SN <= 0) {
         for (int k = 0; k < nd; ++k)
@@ -2169,6 +2225,7 @@ static int hll_face(const or_config *cfg, const double *N, const face_state *O,
         for (int a = 0; a < 3; ++a) { /* HLLSolver.cpp:277-284 */
             F->rk[a] = (uN * Nb->rk) * N[a];
             F->reps[a] = (uN * Nb->reps) * N[a];
+            F->rb[a] = (uN * Nb->rb) * N[a]; /* HLLSolver.cpp:289-290 */
         }
         for (int c = 0; c < 3; ++c) /* HLLSolver.cpp:286-287 */
             for (int b = 0; b < 3; ++b)
@@ -2339,9 +2396,9 @@ static int stage_impl(or_solver *s, double dt, int hancock)
     double *Gp = malloc(3 * nc * sizeof(double));
     double *Lp = malloc(3 * nc * sizeof(double));
     /* turbulence: limited gradients of k and epsilon, and of a (kEpsA) */
-    const int afield = cfg->turb && cfg->turb_model == OR_TURB_KEPSA;
-    double *Gt = calloc(3 * nc * 2, sizeof(double));
-    double *Lt = calloc(3 * nc * 2, sizeof(double));
+    const int afield = a_field(cfg), bfield = b_field(cfg);
+    double *Gt = calloc(3 * nc * 3, sizeof(double));
+    double *Lt = calloc(3 * nc * 3, sizeof(double));
     double *Ga = calloc(afield ? 9 * nc : 1, sizeof(double));
     double *La = calloc(afield ? 9 * nc : 1, sizeof(double));
     face_state *FO = calloc(nf, sizeof(face_state));
@@ -2377,6 +2434,11 @@ static int stage_impl(or_solver *s, double dt, int hancock)
         if (!err)
             err = tvd_vector(s, Ga, st->a, cfg->bc_a, La);
     }
+    if (!err && bfield) { /* Advection2dOrder.cpp:99-106 */
+        err = grad_scalar(s, st->b, cfg->bc_b, &Gt[6 * nc]);
+        if (!err)
+            err = tvd_scalar(s, &Gt[6 * nc], st->b, cfg->bc_b, &Lt[6 * nc]);
+    }
     if (err)
         goto done;
 
@@ -2402,6 +2464,8 @@ static int stage_impl(or_solver *s, double dt, int hancock)
                     const double *T = &La[9 * i + 3 * a];
                     fs->a[a] = (T[0] * d[0] + T[1] * d[1] + T[2] * d[2]) + st->a[a][i];
                 }
+            if (bfield) /* Advection2dOrder.cpp:156-158 */
+                fs->b = dot3(&Lt[6 * nc + 3 * i], d) + st->b[i];
         }
     }
     /* owner side everywhere, neighbour side on inner faces */
@@ -2432,6 +2496,8 @@ static int stage_impl(or_solver *s, double dt, int hancock)
                 err = bc_cell_scalar(&cfg->bc_eps[pt], FO[f].eps, &g->eps);
             if (!err && afield) /* Advection2dOrder.cpp:460-470 */
                 err = bc_cell_vec(&cfg->bc_a[pt], FO[f].a, &s->fN[3 * f], g->a);
+            if (!err && bfield) /* Advection2dOrder.cpp:472-482 */
+                err = bc_cell_scalar(&cfg->bc_b[pt], FO[f].b, &g->b);
         }
         if (!err)
             close_face(cfg, g);
@@ -2443,7 +2509,7 @@ static int stage_impl(or_solver *s, double dt, int hancock)
     for (long f = 0; f < nf && !err; ++f)
         if (s->fpatch[f] != -3) {
             err = face_flux_dispatch(cfg, &s->fN[3 * f], &FO[f], &FN[f], &FF[f]);
-            memcpy(&s->star_stage[8 * f], FF[f].st, sizeof FF[f].st);
+            memcpy(&s->star_stage[NSTAR * f], FF[f].st, sizeof FF[f].st);
         }
     if (err)
         goto done;
@@ -2452,7 +2518,7 @@ static int stage_impl(or_solver *s, double dt, int hancock)
      * (divergence.hpp:315-350) */
     for (long i = 0; i < nc; ++i) {
         double accE = 0, accm[3] = {0, 0, 0}, accr[OR_MAXS + 1] = {0}, acck = 0, acce = 0;
-        double acca[3] = {0, 0, 0};
+        double acca[3] = {0, 0, 0}, accb = 0;
         for (int j = 0; j < s->cnf[i]; ++j) {
             const long f = s->cf[6 * i + j];
             double n[3];
@@ -2475,6 +2541,7 @@ static int stage_impl(or_solver *s, double dt, int hancock)
                 const double *T = &FF[f].ra[3 * a];
                 acca[a] += (T[0] * n[0] + T[1] * n[1] + T[2] * n[2]) * S;
             }
+            accb += dot3(FF[f].rb, n) * S;
         }
         const double V = s->cV[i];
         const double r0 = st->rho[0][i];
@@ -2496,6 +2563,8 @@ static int stage_impl(or_solver *s, double dt, int hancock)
         if (afield) /* Advection2dOrder.cpp:567-570 */
             for (int a = 0; a < 3; ++a)
                 st->ra[a][i] = st->ra[a][i] - (acca[a] / V) * dt;
+        if (bfield) /* Advection2dOrder.cpp:572-574 */
+            st->rb[i] = st->rb[i] - (accb / V) * dt;
     }
     recover_primitives(s, st);
 
@@ -2535,13 +2604,16 @@ static void rk_average(or_solver *s, const or_state *Un)
         st->reps[i] = w * st->reps[i] + iw * Un->reps[i];
         for (int a = 0; a < 3; ++a)
             st->ra[a][i] = st->ra[a][i] * w + Un->ra[a][i] * iw;
+        /* rho b is averaged; b is not re-derived (bunchOfFields.hpp:186-192):
+         * it keeps the second stage's rho b / rho */
+        st->rb[i] = w * st->rb[i] + iw * Un->rb[i];
         for (int a = 0; a < 3; ++a)
             st->v[a][i] = st->m[a][i] / st->rho[0][i];
         if (s->cfg.turb) {
             st->k[i] = st->rk[i] / st->rho[0][i];
             st->eps[i] = st->reps[i] / st->rho[0][i];
         }
-        if (s->cfg.turb && s->cfg.turb_model == OR_TURB_KEPSA)
+        if (a_field(&s->cfg))
             for (int a = 0; a < 3; ++a)
                 st->a[a][i] = st->ra[a][i] / st->rho[0][i];
         double vel[3] = {st->v[0][i], st->v[1][i], st->v[2][i]};
@@ -3181,8 +3253,12 @@ static int diffusion_explicit(or_solver *s, double dt)
      * 869-887; kEpsAModel.cpp:132-257): the cells' b, grad a, grad p and
      * grad rho, the divergence of the molecular viscous stress at the faces,
      * and the face fluxes of the a Laplacian */
-    const int afield = turb && cfg->turb_model == OR_TURB_KEPSA;
+    const int afield = a_field(cfg), bfield = b_field(cfg);
     const long an = afield ? nc : 1, af = afield ? nf : 1;
+    /* BHR: the face fluxes of the b Laplacian and a_f a_f of div(a a)
+     * (BHRModel.cpp:185-186) */
+    double *FB = calloc(3 * af, sizeof(double));
+    double *AAf = calloc(9 * af, sizeof(double));
     double *bC = malloc(an * sizeof(double));
     double *Ga = malloc(9 * an * sizeof(double));
     double *GP = malloc(3 * an * sizeof(double));
@@ -3277,14 +3353,17 @@ static int diffusion_explicit(or_solver *s, double dt)
     if (star_v)
         for (long f = 0; f < nf; ++f)
             for (int a = 0; a < 3; ++a)
-                vf[3 * f + a] = s->star[8 * f + 1 + a];
+                vf[3 * f + a] = s->star[NSTAR * f + 1 + a];
     if (afield) {
-        /* b = rho sum(x_k / (rho_k + chi rho)) / sum(x_k rho_k / (rho_k +
-         * chi rho)) - 1, clipped at 0 (kEpsAModel.cpp:232-257).  The source
-         * adds the stabilizer to rho_k as field + scalar, which returns the
-         * field unchanged (fieldOperations.hpp:33-43), so it does not enter. */
+        /* kEpsA: b = rho sum(x_k / (rho_k + chi rho)) / sum(x_k rho_k /
+         * (rho_k + chi rho)) - 1, clipped at 0 (kEpsAModel.cpp:232-257).  The
+         * source adds the stabilizer to rho_k as field + scalar, which
+         * returns the field unchanged (fieldOperations.hpp:33-43), so it does
+         * not enter.  BHR: the transported b. */
         const double chi = cfg->tC[7];
-        for (long i = 0; i < nc; ++i) {
+        for (long i = 0; i < nc && bfield; ++i)
+            bC[i] = st->b[i];
+        for (long i = 0; i < nc && !bfield; ++i) {
             double num = 0, den = 0;
             for (int k = 1; k <= Ns; ++k) {
                 const double x = st->c[k][i] / st->c[0][i];
@@ -3300,9 +3379,9 @@ static int diffusion_explicit(or_solver *s, double dt)
          * with star reconstruction the star states' (Diffusion.cpp:189-191,
          * 536) */
         if (cfg->star_rec) {
-            grad_vector_faces(s, &s->star[5], 8, Ga);
-            grad_faces(s, &s->star[4], 8, GP);
-            grad_faces(s, &s->star[0], 8, GR);
+            grad_vector_faces(s, &s->star[5], NSTAR, Ga);
+            grad_faces(s, &s->star[4], NSTAR, GP);
+            grad_faces(s, &s->star[0], NSTAR, GR);
         } else {
             err = grad_vector(s, st->a, cfg->bc_a, Ga);
             if (!err)
@@ -3507,6 +3586,31 @@ static int diffusion_explicit(or_solver *s, double dt)
             surf_grad(s, f, 3, ao, ag, ga);
             for (int q = 0; q < 9; ++q)
                 FA[9 * f + q] = (ga[q] * rhoDa) * -1;
+            if (bfield) {
+                /* b Laplacian with mu + rho_f nu_t / sigma_b
+                 * (transportCoefficients.hpp:82, 244) */
+                const double rhoDb = cfg->mu + srho[f] * (tnu / cfg->sigmab);
+                double bo = st->b[o], bg = st->b[n], gb[3];
+                if (bnd && bc_cell_scalar(&cfg->bc_b[s->fpatch[f]], bo, &bg)) {
+                    err = ERR_BC;
+                    goto done;
+                }
+                surf_grad(s, f, 1, &bo, &bg, gb);
+                for (int q = 0; q < 3; ++q)
+                    FB[3 * f + q] = (gb[q] * rhoDb) * -1;
+                /* a_f a_f of the interpolated a (BHRModel.cpp:185-186) */
+                double afv[3];
+                if (!bnd) {
+                    for (int c = 0; c < 3; ++c)
+                        afv[c] = st->a[c][o] * s->wo[f] + st->a[c][n] * s->wn[f];
+                } else if (bc_surf_vec(&cfg->bc_a[s->fpatch[f]], ao, &s->fN[3 * f], afv)) {
+                    err = ERR_BC;
+                    goto done;
+                }
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c)
+                        AAf[9 * f + 3 * r + c] = afv[r] * afv[c];
+            }
         }
     }
     if (cfg->molmass || cfg->enth_flow) {
@@ -3530,6 +3634,7 @@ static int diffusion_explicit(or_solver *s, double dt)
             epsmax = (i == 0 || epsmax < st->eps[i]) ? st->eps[i] : epsmax;
     for (long i = 0; i < nc; ++i) {
         double dY[OR_MAXS] = {0}, dT = 0, dV[3] = {0}, dS[3] = {0}, dA[3] = {0}, dP[3] = {0};
+        double dB = 0, dAA[3] = {0, 0, 0};
         for (int j = 0; j < s->cnf[i]; ++j) {
             const long f = s->cf[6 * i + j];
             double n[3];
@@ -3550,6 +3655,13 @@ static int diffusion_explicit(or_solver *s, double dt)
                     dA[a] += (Ta[0] * n[0] + Ta[1] * n[1] + Ta[2] * n[2]) * S;
                     dP[a] += (Tp[0] * n[0] + Tp[1] * n[1] + Tp[2] * n[2]) * S;
                 }
+            if (bfield) {
+                dB += dot3(&FB[3 * f], n) * S;
+                for (int a = 0; a < 3; ++a) {
+                    const double *T = &AAf[9 * f + 3 * a];
+                    dAA[a] += (T[0] * n[0] + T[1] * n[1] + T[2] * n[2]) * S;
+                }
+            }
         }
         const double V = s->cV[i], rho = st->rho[0][i];
         /* divergences of the molar-mass corrections and of the enthalpy
@@ -3648,7 +3760,16 @@ static int diffusion_explicit(or_solver *s, double dt)
             const double k = st->k[i], eps = st->eps[i];
             /* rho epsilon (kEpsModels.cpp:76-82) and the filtered part of the
              * turbulent stress (Diffusion.cpp:586-598, 719-733) */
-            if (cfg->turb_model != OR_TURB_ZERO)
+            /* BHR: rho eps (1 + CMS_M 2 k / c^2) (BHRModel.cpp:262-272) */
+            double cc_i[OR_MAXS + 1], a_s2 = 0;
+            if (bfield) {
+                for (int q = 0; q <= Ns; ++q)
+                    cc_i[q] = st->c[q][i];
+                a_s2 = eos_c2(cfg, cc_i, rho, st->U[i], st->p[i]);
+            }
+            if (bfield)
+                free_t += rho * eps * (1 + cfg->tC[11] * 2 * k / a_s2);
+            else if (cfg->turb_model != OR_TURB_ZERO)
                 free_t += rho * eps;
             const double tnu = cfg->Cmu * k * k / eps;
             const double thR = theta_s(cfg, divV[i], k, eps, cfg->CMS_R);
@@ -3678,6 +3799,76 @@ static int diffusion_explicit(or_solver *s, double dt)
                 const double me = (eps < 1E-3 * epsmax) ? VERYBIG : eps;
                 const double r = fabs(cfg->ts_coeff * me / ((Se1 + Se2 * st->eps[i]) / rho + STAB));
                 tsrc = (r < tsrc) ? r : tsrc;
+            } else if (bfield) {
+                /* BHRModel.cpp:188-252 (generation flag 1: the Goncharov
+                 * initialisation model off); tC: C0, C1, C2, C3, C4, Ca, Cb1,
+                 * Cb2, alpha2, alpha3, alpha4, CMS_M */
+                const double *C = cfg->tC;
+                const double *ga = &Ga[9 * i], *gp = &GP[3 * i], *gr = &GR[3 * i];
+                const double ai[3] = {st->a[0][i], st->a[1][i], st->a[2][i]};
+                const double ek = eps / k;
+                const double Mat2 = 2 * k / a_s2;
+                const double Mat = sqrt(Mat2);
+                const double thS = theta_s(cfg, G[0] + G[4] + G[8], k, eps, cfg->CMS_R);
+                double gdevA = 0, x[3], grav = 0;
+                for (int q = 0; q < 9; ++q)
+                    gdevA += (dev[q] * thS) * G[q];
+                for (int c = 0; c < 3; ++c)
+                    x[c] = gp[c] - dP[c] / V;
+                for (int c = 0; c < 3; ++c)
+                    grav += ai[c] * x[c];
+                grav = grav * 1.0;
+                const double PiK =
+                    (Mat * C[9] * st->reps[i] - (C[8] * gdevA + 8 * Mat * C[10] * gsph)) * Mat;
+                const double dissip = -st->reps[i];
+                const double Sk1 = gsph + gdevA + grav + PiK;
+                const double Sk2 = (1 + C[11] * Mat2) * dissip / st->k[i];
+                const double gpos = (grav < 0.0) ? 0.0 : grav;
+                const double Se1 = (C[1] * gdevA + C[3] * gsph + C[0] * gpos + C[4] * PiK) * ek;
+                const double Se2 = C[2] * dissip / st->k[i];
+                sk = Sk1 + Sk2 * k;
+                se = Se1 + Se2 * eps;
+                const double me = (eps < 1E-3 * epsmax) ? VERYBIG : eps;
+                const double r = fabs(cfg->ts_coeff * me / ((Se1 + Se2 * st->eps[i]) / rho + STAB));
+                const double rU = fabs(cfg->ts_coeff * fabs(st->U[i]) / (grav + STAB));
+                tsrc = (r < tsrc) ? r : tsrc;
+                tsrc = (rU < tsrc) ? rU : tsrc;
+                /* a: (grad p - div tau) b + ((thetaS devR + spherR) / rho) &
+                 * grad rho + rho a & (grad a - grad v) + rho div(a a) -
+                 * rho (eps / k) Ca a */
+                double tau[9], D[9];
+                for (int q = 0; q < 9; ++q) {
+                    tau[q] = 1.0 * (dev[q] * thS + sph[q]) / rho;
+                    D[q] = ga[q] - G[q];
+                }
+                const double Sa2 = -rho * ek * C[5];
+                double an[3];
+                for (int c = 0; c < 3; ++c) {
+                    const double bgp = x[c] * bC[i];
+                    const double tgr = tau[3 * c] * gr[0] + tau[3 * c + 1] * gr[1] + tau[3 * c + 2] * gr[2];
+                    const double ragv = D[c] * st->ra[0][i] + D[3 + c] * st->ra[1][i] +
+                                        D[6 + c] * st->ra[2][i];
+                    const double redis = (dAA[c] / V) * rho;
+                    const double sig = bgp + tgr + ragv + redis + Sa2 * ai[c];
+                    an[c] = (ai[c] * rho + (0. - dA[c] / V) * dt) / rho + sig / rho * dt;
+                }
+                /* b: the explicit Laplacian; the explicit solver leaves b's
+                 * sources out (sigmaSourceb stays zero, Diffusion.cpp:745-763,
+                 * 889-899), minimal value (917-923) */
+                double bn = (bC[i] * rho + (0. - dB / V) * dt) / rho + 0. / rho * dt;
+                if (bn < cfg->minb)
+                    bn = cfg->minb;
+                const double src = grav * (-1.0);
+                if (src >= 0.)
+                    free_t += src;
+                else
+                    diag_t -= src / (st->T[i] + STAB);
+                for (int c = 0; c < 3; ++c) {
+                    st->a[c][i] = an[c];
+                    st->ra[c][i] = an[c] * rho;
+                }
+                st->b[i] = bn;
+                st->rb[i] = bn * rho;
             } else if (afield) {
                 /* kEpsAModel.cpp:170-225 with C0..C3, Ca, CaMax, CMSA in tC */
                 const double *ga = &Ga[9 * i], *gp = &GP[3 * i], *gr = &GR[3 * i];
@@ -3819,6 +4010,8 @@ done:
     free(rhof);
     free(FA);
     free(FDP);
+    free(FB);
+    free(AAf);
     return err;
 }
 
@@ -3834,17 +4027,20 @@ int or_step(or_solver *s, double dt, int mode)
     copy_state(&s->st, &Un, s->nc, s->cfg.Ns);
     int err = or_stage(s, dt);
     if (!err) {
-        memcpy(s->star, s->star_stage, 8 * (size_t)s->nf * sizeof(double));
+        memcpy(s->star, s->star_stage, NSTAR * (size_t)s->nf * sizeof(double));
         err = or_stage(s, dt);
     }
     if (!err) {
         /* star = (star + star1) / 2 (secondOrderStepSolverRK.cpp:73-85) */
-        for (long q = 0; q < 8 * s->nf; ++q)
+        for (long q = 0; q < NSTAR * s->nf; ++q)
             s->star[q] = (s->star[q] + s->star_stage[q]) / 2;
         rk_average(s, &Un);
     }
-    if (!err && mode == 1)
+    if (!err && mode == 1) {
+        s->keep_b = 1; /* b stays the second stage's, as in the reference */
         recover_primitives(s, &s->st);
+        s->keep_b = 0;
+    }
     if (!err && s->cfg.diffusion) {
         err = diffusion_explicit(s, dt);
         if (!err && mode == 1)
@@ -3865,7 +4061,7 @@ int or_step_hancock(or_solver *s, double dt, int mode)
         recover_primitives(s, &s->st);
     int err = stage_impl(s, dt, 1);
     if (!err)
-        memcpy(s->star, s->star_stage, 8 * (size_t)s->nf * sizeof(double));
+        memcpy(s->star, s->star_stage, NSTAR * (size_t)s->nf * sizeof(double));
     s->hancock_step = 1;
     if (!err && mode == 1)
         recover_primitives(s, &s->st);

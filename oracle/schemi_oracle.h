@@ -102,8 +102,9 @@ typedef struct {
     /* k-epsilon turbulence (#Turbulence_flag on, #Type_of_source_flag):
      * turb_model OR_TURB_SHEAR (shearModel.cpp), OR_TURB_ZERO
      * (zeroModel.cpp: transport and diffusion of k and epsilon, no
-     * sources) or OR_TURB_KEPSA (kEpsAModel.cpp: with the turbulent mass
-     * flux a, transported as rho a, and the algebraic b of calculate_b);
+     * sources), OR_TURB_KEPSA (kEpsAModel.cpp: with the turbulent mass
+     * flux a, transported as rho a, and the algebraic b of calculate_b) or
+     * OR_TURB_BHR (BHRModel.cpp: rho a and a transported rho b);
      * the parameters of set/turbulentParameters.txt (kEpsModels.cpp,
      * abstractTurbulenceModel.hpp:26-57; sigmaa for a); tC the model
      * constants (shear: C1, C2, C3; kEpsA: C0, C1, C2, C3, Ca, CaMax, CMSA,
@@ -114,11 +115,17 @@ typedef struct {
     int turb;
     int turb_model;
     double Cmu, sigmaSc, sigmaT, sigmaE, sigmaK, sigmaEps, CMS_R, CMS_D, mink, mineps;
-    double tC[8];
+    double tC[12];
     or_bc bc_k[6];
     or_bc bc_eps[6];
     double sigmaa;
     or_bc bc_a[6];
+    /* BHR (BHRModel.cpp, the Goncharov initialisation model off as in every
+     * reference deck): the transported b as rho b with sigma_b, its minimal
+     * value and set/b.txt; tC holds C0, C1, C2, C3, C4, Ca, Cb1, Cb2,
+     * alpha2, alpha3, alpha4, CMS_M */
+    double sigmab, minb;
+    or_bc bc_b[6];
     /* #Reconstruction of main.txt for the diffusion stage: 0 linear, 1 star
      * (the cell gradients of v, rho, p and a from the step's star states,
      * Diffusion.cpp:176-193, 529-537) */
@@ -126,7 +133,7 @@ typedef struct {
 } or_config;
 
 /* turbulenceModel (turbulenceModelEnum.hpp) values of the built models */
-enum { OR_TURB_ZERO = 0, OR_TURB_SHEAR = 2, OR_TURB_KEPSA = 6 };
+enum { OR_TURB_ZERO = 0, OR_TURB_SHEAR = 2, OR_TURB_KEPSA = 6, OR_TURB_BHR = 7 };
 
 enum { OR_SLE_EXPLICIT = 0, OR_SLE_JACOBI = 1, OR_SLE_GAUSS_SEIDEL = 2 };
 
@@ -148,6 +155,7 @@ typedef struct {
      * without turbulence); kEpsA: a, rho a and b (zero otherwise) */
     double *k, *eps, *rk, *reps;
     double *a[3], *ra[3], *b;
+    double *rb; /* BHR: rho b (zero otherwise) */
 } or_state;
 
 or_solver *or_create(const or_config *cfg);
@@ -169,6 +177,10 @@ void or_init_turbulence(or_solver *s, const double *k, const double *eps);
  * for or_set_conserved (call either before those). */
 void or_init_turbulence_a(or_solver *s, const double *a);
 void or_set_turbulence_a_conserved(or_solver *s, const double *ra);
+/* BHR: the initial b (n; rho b = rho b, phaseInitialization.cpp:1852) and
+ * the conserved rho b for or_set_conserved (call either before those). */
+void or_init_turbulence_b(or_solver *s, const double *b);
+void or_set_turbulence_b_conserved(or_solver *s, const double *rb);
 /* Conserved turbulent fields rho k, rho epsilon for or_set_conserved (call
  * it first). */
 void or_set_turbulence_conserved(or_solver *s, const double *rk, const double *reps);

@@ -60,8 +60,11 @@ class Turbulence:
     C: Tuple[float, ...] = (1.44, 1.92, 2.0)  # shear: C1, C2, C3
     bc_k: List[BC] = field(default_factory=_free6)
     bc_eps: List[BC] = field(default_factory=_free6)
-    sigmaa: float = 1.4       # sigma_a (kEpsA)
+    sigmaa: float = 1.4       # sigma_a (kEpsA, BHR)
     bc_a: List[BC] = field(default_factory=_free6)
+    sigmab: float = 0.7       # sigma_b (BHR)
+    minb: float = 1e-15       # Minimal_b (BHR)
+    bc_b: List[BC] = field(default_factory=_free6)
 
     def params(self):
         return (self.Cmu, self.sigmaSc, self.sigmaT, self.sigmaE, self.sigmaK,
@@ -69,8 +72,22 @@ class Turbulence:
 
     @property
     def a_field(self) -> bool:
-        """The model transports the turbulent mass flux a (kEpsA)."""
-        return self.model == turbulenceModel.kEpsAModel
+        """The model carries the turbulent mass flux a (kEpsA, BHR)."""
+        return self.model in (turbulenceModel.kEpsAModel, turbulenceModel.BHRModel)
+
+    @property
+    def b_field(self) -> bool:
+        """The model transports b (BHR)."""
+        return self.model == turbulenceModel.BHRModel
+
+    @classmethod
+    def BHR(cls, C0=0.92, C1=1.44, C2=1.92, C3=2.0, C4=1.1, Ca=2.5, Cb1=2.0, Cb2=5.0,
+            alpha2=0.15, alpha3=0.2, alpha4=0.1, CMSM=1.0, **kw) -> "Turbulence":
+        """The BHR model with its twelve constants (BHRModel.cpp:86-137; the
+        defaults of the reference's decks) and the Goncharov initialisation
+        model off."""
+        return cls(model=turbulenceModel.BHRModel,
+                   C=(C0, C1, C2, C3, C4, Ca, Cb1, Cb2, alpha2, alpha3, alpha4, CMSM), **kw)
 
     @classmethod
     def kEpsA(cls, C0=0.92, C1=1.44, C2=1.92, C3=2.0, Ca=2.5, CaMax=5.0, CMSA=1.5,
@@ -170,9 +187,12 @@ class Case:
     @property
     def nfields(self) -> int:
         """Conserved fields per cell: rho_1..rho_Ns, rho, m (3), E, and with
-        turbulence rho k, rho epsilon, and with the kEpsA model rho a (3)."""
+        turbulence rho k, rho epsilon, with the kEpsA and BHR models rho a (3),
+        with BHR rho b."""
         t = self.turbulence
-        return self.Ns + 5 + (0 if t is None else 5 if t.a_field else 2)
+        if t is None:
+            return self.Ns + 5
+        return self.Ns + 7 + (3 if t.a_field else 0) + (1 if t.b_field else 0)
 
     @property
     def ncells(self) -> int:

@@ -65,6 +65,9 @@ def case_json(case: Case) -> str:
         if t.a_field:
             eos["turbulence"].update(sigmaa=t.sigmaa,
                                      bc_a=[[int(b.type), list(b.value)] for b in t.bc_a])
+        if t.b_field:
+            eos["turbulence"].update(sigmab=t.sigmab, minb=t.minb,
+                                     bc_b=[[int(b.type), list(b.value)] for b in t.bc_b])
     return json.dumps({**eos,
         "dim": case.dim, "N": list(case.N), "L": list(case.L), "M": list(case.M),
         "Cv": list(case.Cv), "R": case.R, "flux": int(case.flux),
@@ -106,7 +109,13 @@ def seeded_a(case: Case, seed: int):
     return a
 
 
-def save(name, case, dts, snaps, times=None, restart=None, restart_turb=None, restart_a=None):
+def seeded_b(case: Case, seed: int):
+    """Seeded b of the BHR model, 0.05 .. 0.5."""
+    return 0.05 + 0.45 * np.random.default_rng(seed + 2000).random(case.ncells)
+
+
+def save(name, case, dts, snaps, times=None, restart=None, restart_turb=None, restart_a=None,
+         restart_b=None):
     out = {"case": np.array(case_json(case)), "dts": np.asarray(dts, dtype=np.float64),
            "steps": np.array(sorted(snaps), dtype=np.int64)}
     if times is not None:
@@ -118,6 +127,8 @@ def save(name, case, dts, snaps, times=None, restart=None, restart_turb=None, re
         out["init_k"], out["init_eps"] = (np.asarray(x) for x in restart_turb)
     if restart_a is not None:
         out["init_a"] = np.asarray(restart_a)
+    if restart_b is not None:
+        out["init_b"] = np.asarray(restart_b)
     keys = KEYS + (("k", "eps") if case.turbulence is not None else ())
     if case.turbulence is not None and case.turbulence.a_field:
         keys += ("a", "b")
@@ -130,13 +141,13 @@ def save(name, case, dts, snaps, times=None, restart=None, restart_turb=None, re
 
 
 def run_const(name, case, nsteps, dt, keep, zones=None, restart=None, restart_turb=None,
-              restart_a=None):
+              restart_a=None, restart_b=None):
     t = 0.0
     for _ in range(nsteps):
         t = t + dt
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as root:
         decks.write_deck(root, case, t_end=t, zones=zones, restart=restart, const_dt=dt,
-                         restart_turb=restart_turb, restart_a=restart_a)
+                         restart_turb=restart_turb, restart_a=restart_a, restart_b=restart_b)
         r = decks.run(root)
         if r.returncode != 0:
             raise RuntimeError(f"{name}: reference failed\n{r.stdout[-2000:]}\n{r.stderr}")
@@ -147,7 +158,7 @@ def run_const(name, case, nsteps, dt, keep, zones=None, restart=None, restart_tu
         for s in keep:
             snaps[s] = decks.read_output(root, s + off, case.Ns)
     save(name, case, [dt] * nsteps, snaps, restart=restart, restart_turb=restart_turb,
-         restart_a=restart_a)
+         restart_a=restart_a, restart_b=restart_b)
 
 
 def run_cfl_shock_tube(nsteps=200, keep=(1, 2, 10, 200)):
@@ -618,6 +629,54 @@ def kepsa_cases(base, nsteps=4, keep=(1, 4)):
             turbulence=Turbulence.kEpsA(), **base, **tr), 215, n=3, kp=(1, 3))
 
 
+def bhr_cases(base, nsteps=4, keep=(1, 4)):
+    """BHR (BHRModel.cpp, the Goncharov initialisation model off as in every
+    reference deck): rho a and rho b through the HLLC/HLL flux and the TVD
+    slopes, and in the diffusion stage the a and b Laplacians, the sources
+    with the turbulent Mach number, the pressure-dilatation term Pi_K, the
+    redistribution rho div(a a), the energy sink, rho eps (1 + CMS_M M_t^2)
+    in the temperature equation and the source step with |U| / G
+    (Diffusion.cpp:129-1029, BHRModel.cpp:139-272)."""
+    from schemi_amd.case import Turbulence
+    tr = dict(diffusion=True, transport=(2e-3, 2e-3, 2e-3))
+    vdw = dict(gas_model=gasModel.vanDerWaals, Tc=[24.0, 3.2], Pc=[1.0, 1.0])
+
+    def slip_y(c):
+        for p in (3, 4):
+            c.set_bc(p, BCT.slip)
+            c.turbulence.bc_a[p] = BC(BCT.slip)
+        return c
+
+    def go(name, c, seed, n=nsteps, kp=keep):
+        run_const(name, c, n, 2e-4 if c.dim > 1 else 1e-3, kp,
+                  restart=seeded_primitives(c, seed), restart_turb=seeded_turbulence(c, seed),
+                  restart_a=seeded_a(c, seed), restart_b=seeded_b(c, seed))
+
+    go("seeded_2d_bhr_advection_hllc",
+       slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), limiter=TL.minmod,
+                   turbulence=Turbulence.BHR(), **base)), 251)
+    go("seeded_2d_bhr_hllc",
+       slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), limiter=TL.minmod,
+                   turbulence=Turbulence.BHR(), **base, **tr)), 252)
+    go("seeded_2d_bhr_vdw_hll",
+       slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), flux=FS.HLL, limiter=TL.minmod,
+                   turbulence=Turbulence.BHR(sigmaE=0.9), **base, **tr, **vdw)), 253)
+    c = Case(dim=1, N=(40, 1, 1), L=(1.0, 1.0, 1.0), limiter=TL.vanLeer,
+             turbulence=Turbulence.BHR(), **base, **tr)
+    c.set_bc(0, BCT.fixedValueCell, "all", (0.03, 0.0, 0.0))
+    c.set_bc(1, BCT.slip)
+    c.turbulence.bc_a[1] = BC(BCT.slip)
+    c.turbulence.bc_b[0] = BC(BCT.fixedValueCell, (0.3, 0.0, 0.0))
+    go("seeded_1d_bhr_vanleer", c, 254)
+    go("seeded_3d_bhr_vanalbada",
+       Case(dim=3, N=(8, 6, 5), L=(1.0, 0.8, 0.6), limiter=TL.vanAlbada,
+            turbulence=Turbulence.BHR(), **base, **tr), 255, n=3, kp=(1, 3))
+    # the reference decks' setting: star reconstruction
+    go("seeded_2d_bhr_star_hll",
+       slip_y(Case(dim=2, N=(12, 10, 1), L=(1.0, 0.8, 1.0), flux=FS.HLL, limiter=TL.minmod,
+                   turbulence=Turbulence.BHR(), reconstruction="star", **base, **tr)), 256)
+
+
 def star_cases(base):
     """#Reconstruction star: the diffusion stage takes the cell gradients of
     v, rho, p (and a) from the Riemann star states of the faces, averaged over
@@ -764,5 +823,7 @@ if __name__ == "__main__":
         kepsa_cases(b)
     elif len(sys.argv) > 1 and sys.argv[1] == "star":
         star_cases(b)
+    elif len(sys.argv) > 1 and sys.argv[1] == "bhr":
+        bhr_cases(b)
     else:
         main()

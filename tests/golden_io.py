@@ -52,6 +52,8 @@ def load(name):
         out["restart_turb"] = (z["init_k"], z["init_eps"])
     if "init_a" in z:
         out["restart_a"] = z["init_a"]
+    if "init_b" in z:
+        out["restart_b"] = z["init_b"]
     return out
 
 
@@ -65,7 +67,9 @@ def _turbulence(t):
                       mineps=p[9], C=tuple(t["C"]),
                       bc_k=[BC(ty, tuple(v)) for ty, v in t["bc_k"]],
                       bc_eps=[BC(ty, tuple(v)) for ty, v in t["bc_eps"]],
-                      sigmaa=t.get("sigmaa", 1.4),
+                      sigmaa=t.get("sigmaa", 1.4), sigmab=t.get("sigmab", 0.7),
+                      minb=t.get("minb", 1e-15),
+                      bc_b=[BC(ty, tuple(v)) for ty, v in t.get("bc_b", [[1, [0, 0, 0]]] * 6)],
                       bc_a=[BC(ty, tuple(v)) for ty, v in t.get("bc_a", [[1, [0, 0, 0]]] * 6)])
 
 

@@ -47,6 +47,8 @@ def test_golden_fixtures(name, hip_device):
             kw = dict(k=g["restart_turb"][0], eps=g["restart_turb"][1])
         if "restart_a" in g:
             kw["a"] = g["restart_a"]
+        if "restart_b" in g:
+            kw["b"] = g["restart_b"]
         o.init_primitive(*g["restart"], **kw)
         U0 = o.conserved()
     else:

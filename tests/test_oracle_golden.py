@@ -25,6 +25,8 @@ def start(g):
             kw = dict(k=g["restart_turb"][0], eps=g["restart_turb"][1])
         if "restart_a" in g:
             kw["a"] = g["restart_a"]
+        if "restart_b" in g:
+            kw["b"] = g["restart_b"]
         o.init_primitive(*g["restart"], **kw)
     else:
         o.set_full(g["snaps"][0])

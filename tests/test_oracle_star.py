@@ -30,6 +30,8 @@ def run(g, case):
         kw = dict(k=g["restart_turb"][0], eps=g["restart_turb"][1])
     if "restart_a" in g:
         kw["a"] = g["restart_a"]
+    if "restart_b" in g:
+        kw["b"] = g["restart_b"]
     o.init_primitive(*g["restart"], **kw)
     last = max(g["snaps"])
     for dt in g["dts"][:last]:

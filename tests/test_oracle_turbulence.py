@@ -29,6 +29,8 @@ def run(g, case=None, steps=None):
     kw = dict(k=g["restart_turb"][0], eps=g["restart_turb"][1]) if case.turbulence else {}
     if case.turbulence is not None and case.turbulence.a_field:
         kw["a"] = g["restart_a"]
+    if case.turbulence is not None and case.turbulence.b_field:
+        kw["b"] = g["restart_b"]
     o.init_primitive(*g["restart"], **kw)
     last = max(g["snaps"]) if steps is None else steps
     for dt in g["dts"][:last]:
@@ -82,6 +84,22 @@ def replace_turb(case, **kw):
     ("seeded_2d_kepsa_hllc", dict(sigmaa=1e30), "a", 1e-6),
     ("seeded_3d_kepsa_vanalbada", dict(CMS_R=10.0), "a", 1e-7),
     ("seeded_1d_kepsa_vanleer", dict(sigmaa=0.5), "a", 1e-6),
+    # BHR: the pressure-dilatation term Pi_K (alpha3, C4), the turbulent
+    # Mach number in the dissipation and in rho eps (CMS_M), C0, Ca, the b
+    # Laplacian (sigma_b) and its fixed-value boundary, rho b in the flux
+    ("seeded_2d_bhr_hllc", dict(C=(0.92, 1.44, 1.92, 2.0, 1.1, 2.5, 2.0, 5.0, 0.15, 0.0,
+                                   0.1, 1.0)), "k", 1e-8),
+    ("seeded_2d_bhr_hllc", dict(C=(0.92, 1.44, 1.92, 2.0, 0.0, 2.5, 2.0, 5.0, 0.15, 0.2,
+                                   0.1, 1.0)), "eps", 1e-8),
+    ("seeded_2d_bhr_hllc", dict(C=(0.92, 1.44, 1.92, 2.0, 1.1, 2.5, 2.0, 5.0, 0.15, 0.2,
+                                   0.1, 0.0)), "T", 1e-10),
+    ("seeded_2d_bhr_hllc", dict(C=(0.5, 1.44, 1.92, 2.0, 1.1, 2.5, 2.0, 5.0, 0.15, 0.2,
+                                   0.1, 1.0)), "eps", 1e-6),
+    ("seeded_2d_bhr_vdw_hll", dict(C=(0.92, 1.44, 1.92, 2.0, 1.1, 1.5, 2.0, 5.0, 0.15, 0.2,
+                                      0.1, 1.0)), "a", 1e-4),
+    ("seeded_2d_bhr_hllc", dict(sigmab=1e30), "b", 1e-6),
+    ("seeded_1d_bhr_vanleer", dict(bc_b=[BC(BCT.fixedValueCell, (0.1, 0.0, 0.0))] +
+                                   [BC() for _ in range(5)]), "b", 1e-6),
 ])
 def test_fixture_exercises_term(name, change, field, least):
     g = load(name)
@@ -93,6 +111,13 @@ def test_fixture_exercises_term(name, change, field, least):
     if field in ("k", "eps") and case.turbulence is None:
         field = "E"
     assert rel(got[field], ref[field]) > least
+
+
+def test_bhr_b_is_advected():
+    """The advection-only BHR fixture moves b (so its bit-exact b pins the
+    flux of rho b and the stage recovery b = rho b / rho)."""
+    g = load("seeded_2d_bhr_advection_hllc")
+    assert rel(g["snaps"][4]["b"], g["restart_b"]) > 1e-4
 
 
 def test_fixed_value_k_boundary_exercised():
@@ -192,6 +217,52 @@ def test_kepsa_uniform_state(CMSA, limited):
     np.testing.assert_allclose(st["E"], st0["E"], rtol=1e-13)
 
 
+@pytest.mark.parametrize("CMSM", [1.0, 0.0])
+def test_bhr_uniform_state(CMSM):
+    """BHR on a uniform periodic state at rest with uniform a and b: every
+    gradient, G = a . grad p and rho div(a a) vanish, so one step is an Euler
+    step of rho dk/dt = (alpha3 M^2 - 1 - CMS_M M^2) rho eps, rho deps/dt =
+    (C4 alpha3 M^2 rho eps) eps/k - C2 rho eps^2/k, da/dt = -(eps/k) Ca a with
+    the turbulent Mach number M^2 = 2k/c^2 (c^2 = gamma p / rho of the ideal
+    mixture), b unchanged (the explicit solver gives b no sources), the
+    dissipation rho eps (1 + CMS_M M^2) moved into the internal energy, so
+    that E grows by the Pi_K term alpha3 M^2 rho eps alone."""
+    case = uniform_case()
+    t = Turbulence.BHR(CMSM=CMSM)
+    for p in range(6):
+        t.bc_k[p] = t.bc_eps[p] = t.bc_a[p] = t.bc_b[p] = BC(BCT.periodic)
+    case = dataclasses.replace(case, turbulence=t)
+    n = case.ncells
+    o = Oracle(case)
+    k0, e0, b0, a0 = 2e-3, 5e-2, 0.3, np.array([0.03, -0.02, 0.0])
+    o.init_primitive(np.full((2, n), 0.02), np.zeros((3, n)), np.full(n, 1.0),
+                     k=np.full(n, k0), eps=np.full(n, e0), a=np.repeat(a0[:, None], n, axis=1),
+                     b=np.full(n, b0))
+    st0 = o.state()
+    dt = 1e-3
+    assert o.step(dt, 0) == 0
+    st = o.state()
+    C0, C1, C2, C3, C4, Ca, Cb1, Cb2, al2, al3, al4, cmsm = t.C
+    rho = st0["rho0"][0]
+    cv = (case.Cv[0] + case.Cv[1]) / 2
+    c2 = (1 + case.R / cv) * 1.0 / rho
+    M2 = 2 * k0 / c2
+    np.testing.assert_allclose(st["k"], k0 + (al3 * M2 - 1 - cmsm * M2) * e0 * dt, rtol=1e-12)
+    np.testing.assert_allclose(st["eps"],
+                               e0 + (C4 * al3 * M2 - C2) * e0 * e0 / k0 * dt, rtol=1e-12)
+    for c in range(3):
+        np.testing.assert_allclose(st["a"][c], a0[c] * (1 - e0 / k0 * Ca * dt),
+                                   rtol=1e-14, atol=1e-300)
+    assert np.all(st["b"] == b0)
+    np.testing.assert_allclose(st["rb"], b0 * st0["rho0"], rtol=1e-15)
+    np.testing.assert_allclose(st["U"] - st0["U"], rho * e0 * (1 + cmsm * M2) * dt, rtol=1e-6)
+    # Pi_K feeds k and is not taken from the internal energy: E grows by it
+    # (to 1e-13 absolute: the reference's T from U divides by c0 + stab and
+    # its U from T does not, which shifts U by U stab / c0 ~ 5e-14)
+    np.testing.assert_allclose(st["E"] - st0["E"], al3 * M2 * rho * e0 * dt, rtol=0,
+                               atol=1e-13)
+
+
 def test_kepsa_b_with_chi():
     """b for chi > 0: rho sum x_k / (rho_k + chi rho) / sum x_k rho_k /
     (rho_k + chi rho) - 1 on the uniform state, by hand for two species."""
@@ -211,7 +282,7 @@ def test_kepsa_b_with_chi():
 
 def test_unsupported_turbulence_configurations():
     base = uniform_case()
-    for bad in (dict(turbulence=Turbulence(model=TM.BHRModel)),
+    for bad in (dict(turbulence=Turbulence(model=TM.BHRKLModel)),
                 dict(sle_solver="Jacobi"), dict(molar_mass_correction=True),
                 dict(gas_model=2)):
         with pytest.raises(ValueError):

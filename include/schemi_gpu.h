@@ -260,6 +260,17 @@ typedef struct {
      * the linear one.  Otherwise SCHEMI_ERR_INIT.  Ignored without the
      * diffusion stage. */
     int32_t reconstruction;
+    /* SCHEMI_TURB_BHR (BHRModel.cpp with the Goncharov initialisation model
+     * off, as in every reference deck; the state carries rho a and rho b
+     * after rho epsilon, Ns + 11 fields): sigma_a in turb_sigma_a, sigma_b,
+     * the minimal b, the twelve constants of BHRModel.cpp:73-84 (C0, C1, C2,
+     * C3, C4, Ca, Cb1, Cb2, alpha2, alpha3, alpha4, CMS_M), and the boundary
+     * conditions of set/a.txt (bc_a) and set/b.txt.  With the explicit solver
+     * b has no sources (the reference's explicit branch leaves them out). */
+    double turb_sigma_b;
+    double turb_min_b;
+    double turb_bhr_C[12];
+    schemi_bc bc_b[6];
 } schemi_scheme_desc;
 
 #define SCHEMI_RECONSTRUCTION_LINEAR 0
@@ -269,6 +280,7 @@ typedef struct {
 #define SCHEMI_TURB_ZERO 0
 #define SCHEMI_TURB_SHEAR 2
 #define SCHEMI_TURB_KEPSA 6
+#define SCHEMI_TURB_BHR 7
 
 #define SCHEMI_SLE_GAUSS_SEIDEL 0
 #define SCHEMI_SLE_EXPLICIT 3
@@ -286,7 +298,8 @@ void schemi_gpu_destroy(schemi_gpu_ctx *ctx);
 
 /* Uploads the conserved state (Ns + 5 fields x ncells doubles; with
  * turbulence Ns + 7: rho k and rho epsilon after E; with the kEpsA model
- * Ns + 10: rho a_x, rho a_y, rho a_z after rho epsilon). */
+ * Ns + 10: rho a_x, rho a_y, rho a_z after rho epsilon; with BHR Ns + 11:
+ * rho b after rho a). */
 int schemi_gpu_set_state(schemi_gpu_ctx *ctx, const double *cons,
                          int64_t ncells, int32_t numbering);
 

@@ -224,7 +224,9 @@ struct schemi_gpu_ctx {
     int molmass = 0, enthalpy = 0;
     int nf = 0;                  // conserved fields: ns + 5, turbulence ns + 7, kEpsA ns + 10
     bool turb = false;           // k-epsilon turbulence
-    bool afield = false;         // kEpsA: the state carries rho a
+    bool afield = false;         // kEpsA, BHR: the state carries rho a
+    bool bfield = false;         // BHR: the state carries rho b
+    double *db2 = nullptr;       // BHR: the second stage's b (n)
     bool star = false;           // #Reconstruction star: star states per face
     double *dS = nullptr;        // star buffer, (5 + na) x hp.star_nft
     TurbParams tp{};
@@ -267,7 +269,8 @@ int launch_stage_turb(schemi_gpu_ctx *c, const double *Uin, double *Uout, const 
     L.variant = variant;
     L.gas_model = c->hp.gas_model;
     L.slopes = c->mode == MODE_SLOPES ? 1 : 0;
-    L.na = c->afield ? 3 : 0;
+    L.na = c->bfield ? 4 : c->afield ? 3 : 0;
+    L.b2 = c->db2;
     L.n = c->n;
     L.stream = c->stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -353,7 +356,7 @@ int launch_star_stage(schemi_gpu_ctx *c, const double *U, int avg)
     S.gas_model = c->hp.gas_model;
     S.slopes = (c->dim == 1 && c->mode == MODE_SLOPES) ? 1 : 0;
     S.nt = c->turb ? 2 : 0;
-    S.na = c->afield ? 3 : 0;
+    S.na = c->bfield ? 4 : c->afield ? 3 : 0;
     S.n = c->n;
     S.stream = c->stream;
     const int r = launch_star(S);
@@ -434,7 +437,8 @@ int one_step(schemi_gpu_ctx *c)
             TurbDiffusion T;
             T.TP = c->dTP;
             T.epsmax = c->epsmax;
-            T.afield = c->afield ? 1 : 0;
+            T.amodel = c->bfield ? 2 : c->afield ? 1 : 0;
+            T.b2 = c->db2;
             if (launch_diffusion_turb(L, T))
                 return SCHEMI_ERR_SYSTEM;
         } else if (launch_diffusion(L)) {
@@ -668,7 +672,8 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     if (scheme->turbulence &&
         ((scheme->turbulence_model != SCHEMI_TURB_SHEAR &&
           scheme->turbulence_model != SCHEMI_TURB_ZERO &&
-          scheme->turbulence_model != SCHEMI_TURB_KEPSA) ||
+          scheme->turbulence_model != SCHEMI_TURB_KEPSA &&
+          scheme->turbulence_model != SCHEMI_TURB_BHR) ||
          scheme->step_solver != SCHEMI_STEP_SECOND_ORDER_RK ||
          (mix->gas_model != SCHEMI_GAS_IDEAL && mix->gas_model != SCHEMI_GAS_VAN_DER_WAALS) ||
          lim == SCHEMI_LIM_LINEAR ||
@@ -693,8 +698,10 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
     c->kappa = scheme->transport_kappa;
     c->molmass = (c->diffusion && scheme->diffusion_molar_mass) ? 1 : 0;
     c->turb = scheme->turbulence != 0;
-    c->afield = c->turb && scheme->turbulence_model == SCHEMI_TURB_KEPSA;
-    c->nf = mix->ns + 5 + (c->afield ? 5 : c->turb ? 2 : 0);
+    c->afield = c->turb && (scheme->turbulence_model == SCHEMI_TURB_KEPSA ||
+                            scheme->turbulence_model == SCHEMI_TURB_BHR);
+    c->bfield = c->turb && scheme->turbulence_model == SCHEMI_TURB_BHR;
+    c->nf = mix->ns + 5 + (c->turb ? 2 : 0) + (c->afield ? 3 : 0) + (c->bfield ? 1 : 0);
     if (c->turb) {
         TurbParams &T = c->tp;
         T.model = scheme->turbulence_model;
@@ -711,7 +718,33 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
         T.C1 = scheme->turb_C[0];
         T.C2 = scheme->turb_C[1];
         T.C3 = scheme->turb_C[2];
-        if (c->afield) {
+        if (c->bfield) {
+            // BHRModel.cpp:86-137: C0, C1, C2, C3, C4, Ca, Cb1, Cb2, alpha2,
+            // alpha3, alpha4, CMS_M
+            const double *K = scheme->turb_bhr_C;
+            T.C0 = K[0];
+            T.C1 = K[1];
+            T.C2 = K[2];
+            T.C3 = K[3];
+            T.C4 = K[4];
+            T.Ca = K[5];
+            T.Cb1 = K[6];
+            T.Cb2 = K[7];
+            T.alpha2 = K[8];
+            T.alpha3 = K[9];
+            T.alpha4 = K[10];
+            T.CMSM = K[11];
+            T.sigmaa = scheme->turb_sigma_a;
+            T.sigmab = scheme->turb_sigma_b;
+            T.minb = scheme->turb_min_b;
+            for (int p = 0; p < 6; ++p) {
+                T.bcta[p] = scheme->bc_a[p].type;
+                for (int b = 0; b < 3; ++b)
+                    T.bcva[p][b] = scheme->bc_a[p].value[b];
+                T.bctb[p] = scheme->bc_b[p].type;
+                T.bcvb[p] = scheme->bc_b[p].value[0];
+            }
+        } else if (c->afield) {
             // kEpsAModel.cpp:86-121: C0, C1, C2, C3, Ca, CaMax, CMSA, chi
             const double *K = scheme->turb_kepsa_C;
             T.C0 = K[0];
@@ -884,9 +917,9 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
             }
         }
         if (c->turb)
-            for (int f = 0; f < (c->afield ? 3 : 2); ++f)
+            for (int f = 0; f < (c->bfield ? 4 : c->afield ? 3 : 2); ++f)
                 for (int p : {plo, phi}) {
-                    const int t = f < 2 ? c->tp.bct[p][f] : c->tp.bcta[p];
+                    const int t = f < 2 ? c->tp.bct[p][f] : f == 2 ? c->tp.bcta[p] : c->tp.bctb[p];
                     if (!bc_ok(t)) {
                         delete c;
                         return SCHEMI_ERR_BOUNDARY;
@@ -1045,10 +1078,16 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
               hipMalloc(&c->derr, sizeof(int)) == hipSuccess;
     if (ok && c->diffusion)
         ok = hipMalloc(&c->Wd, sizeof(double) *
-                                   (size_t)(2 * mix->ns + (c->afield ? 13 : c->turb ? 9 : 5)) *
+                                   (size_t)(2 * mix->ns + (c->bfield   ? 16
+                                                           : c->afield ? 13
+                                                           : c->turb   ? 9
+                                                                       : 5)) *
                                    (size_t)c->n) == hipSuccess;
+    if (ok && c->bfield)
+        ok = hipMalloc(&c->db2, sizeof(double) * (size_t)c->n) == hipSuccess;
     if (ok && c->star)
-        ok = hipMalloc(&c->dS, sizeof(double) * (size_t)(5 + (c->afield ? 3 : 0)) *
+        ok = hipMalloc(&c->dS, sizeof(double) *
+                                   (size_t)(5 + (c->bfield ? 4 : c->afield ? 3 : 0)) *
                                    (size_t)P.star_nft) == hipSuccess;
     if (ok && (c->turb || c->star))
         ok = hipMalloc(&c->dTP, sizeof(TurbParams)) == hipSuccess &&
@@ -1117,6 +1156,7 @@ void schemi_gpu_destroy(schemi_gpu_ctx *c)
     (void)hipFree(c->dTP);
     (void)hipFree(c->epsmax);
     (void)hipFree(c->dS);
+    (void)hipFree(c->db2);
     if (c->stream)
         (void)hipStreamDestroy(c->stream);
     delete c;

@@ -70,11 +70,13 @@ __device__ __forceinline__ long long block_cell0()
 
 // W layout per cell: Y[NS], c[NS], v[3], T, p; with turbulence then k,
 // epsilon, CvM = Cv / (rho / c0) and nonIdeality / rho (Diffusion.cpp:137-172);
-// with the kEpsA model then a[3] and rho
+// with the kEpsA and BHR models then a[3] and rho; with BHR then the second
+// stage's b, the internal energy and the squared sound speed of the cell
 template <int NS> struct DW {
     static constexpr int Y = 0, C = NS, V = 2 * NS, T = 2 * NS + 3, P = 2 * NS + 4,
                          NW = 2 * NS + 5, K = NW, EPS = NW + 1, CVM = NW + 2, NIR = NW + 3,
-                         NWT = NW + 4, A = NWT, RHO = NWT + 3, NWA = NWT + 4;
+                         NWT = NW + 4, A = NWT, RHO = NWT + 3, NWA = NWT + 4, B = NWA,
+                         UI = NWA + 1, CS2 = NWA + 2, NWB = NWA + 3;
 };
 
 // Cv_mix = sum_k (c_k / c0) Cv_k (mixtureIdeal.cpp:40-51)
@@ -172,12 +174,14 @@ __device__ __forceinline__ void reduce_max_bits(double r, unsigned long long *sl
 
 // TURB: the state carries rho k and rho epsilon (U = E - rho v^2 / 2 - rho k);
 // k, epsilon, CvM and nonIdeality / rho go to W, and the maximum of epsilon
-// to *epsmax (shearModel.cpp:107-114; cleared after each step).  AF (kEpsA):
-// the state also carries rho a; a and rho go to W.
-template <int NS, bool TURB = false, bool AF = false>
+// to *epsmax (shearModel.cpp:107-114; cleared after each step).  AM (a
+// model: 1 kEpsA, 2 BHR): the state also carries rho a (and rho b); a and rho
+// go to W, with BHR also the second stage's b (b2), U and c^2.
+template <int NS, bool TURB = false, int AM = 0>
 __global__ void __launch_bounds__(NT)
 diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const double *Ualt, int hk,
-                  double *__restrict__ W, double *sc, unsigned long long *epsmax = nullptr)
+                  double *__restrict__ W, double *sc, unsigned long long *epsmax = nullptr,
+                  const double *__restrict__ b2 = nullptr)
 {
     const DevParams &P = *Pp;
     if (sc[SC_DONE] != 0.0)
@@ -197,7 +201,7 @@ diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const doubl
     if (id >= n)
         return;
     const double *Us = state_of(U, Ualt, sc, hk);
-    constexpr int NA = AF ? 3 : 0;
+    constexpr int NA = AM == 2 ? 4 : AM == 1 ? 3 : 0;
     constexpr int NF = TURB ? NS + 7 + NA : NS + 5;
     double u[NF], w[NS + 6 + NA];
 #pragma unroll
@@ -235,11 +239,18 @@ diff_prims_kernel(const DevParams *__restrict__ Pp, const double *U, const doubl
         W[L::CVM * n + id] = cv_mix<NS>(P, w, c0) / avM;
         W[L::NIR * n + id] = non_ideality<NS>(P, w, c0) / u[NS];
     }
-    if constexpr (AF) {
+    if constexpr (AM != 0) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             W[(L::A + c) * n + id] = w[NS + 6 + c];
         W[L::RHO * n + id] = u[NS];
+    }
+    if constexpr (AM == 2) {
+        // sqSonicSpeed of the cell (BHRModel.cpp:182-184, 267-268)
+        W[L::B * n + id] = b2[id];
+        W[L::UI * n + id] = Uint;
+        W[L::CS2 * n + id] =
+            eos_c2<double, NS, -1>(P, w, c0, gamma1<double, NS>(P, w, c0), u[NS], Uint, w[NS + 3]);
     }
 }
 
@@ -407,8 +418,13 @@ __device__ __forceinline__ void reduce_min_bits(double r, unsigned long long *sl
 // mu (grad_s v + grad_s v^T - 2/3 div_s v I), the a Laplacian (mu +
 // rho nu_t / sigma_a), the k, epsilon and a sources with the thetaA limiter,
 // and the energy sink a . (grad p - div tau) distributed into the
-// temperature equation.
-template <int NS, int DIM, bool MIXED, bool IMPL, bool SPEC, bool TURB = false, bool AF = false>
+// temperature equation.  AM: 0 no a field, 1 kEpsA, 2 BHR (BHRModel.cpp:
+// 139-272, the Goncharov initialisation model off): also the b Laplacian
+// (mu + rho nu_t / sigma_b) of the second stage's b, rho div(a a), the
+// turbulent Mach number 2k/c^2 in the dissipation, Pi_K, the dissipation
+// rho eps (1 + CMS_M M^2) in the temperature equation and the |U| / G source
+// step; the explicit solver leaves b's own sources out.
+template <int NS, int DIM, bool MIXED, bool IMPL, bool SPEC, bool TURB = false, int AM = 0>
 __global__ void __launch_bounds__(NT)
 diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ geo, double *U,
                    double *Ualt, int hk, const double *__restrict__ W, double *sc,
@@ -418,10 +434,11 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                    const double *__restrict__ star = nullptr)
 {
     static_assert(!(TURB && (IMPL || SPEC)), "turbulence: explicit solver without species terms");
+    constexpr bool AF = AM != 0;
     static_assert(TURB || !AF, "the a field needs turbulence");
     using M = MB<NS>;
     using L = DW<NS>;
-    constexpr int NW = TURB ? (AF ? L::NWA : L::NWT) : L::NW;
+    constexpr int NW = TURB ? (AM == 2 ? L::NWB : AF ? L::NWA : L::NWT) : L::NW;
     __shared__ double sRed[NT / 64];
     const DevParams &P = *Pp;
     if (sc[SC_DONE] != 0.0)
@@ -450,6 +467,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
         // and the divergence of the molecular viscous stress
         double gA[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, gP[3] = {0, 0, 0}, gR[3] = {0, 0, 0};
         double dA[3] = {0, 0, 0}, dDP[3] = {0, 0, 0};
+        double dB = 0, dAA[3] = {0, 0, 0};  // BHR: b Laplacian, div(a a)
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             dMC[k] = 0;
@@ -513,8 +531,10 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             // turbulence: face k, epsilon, Cv and temperature, and the normal
             // components of surfGrad of k, epsilon, CvM and nonIdeality / rho
             double ks = 0, es = 0, cvs = 0, Tsf = 0, gk = 0, ge = 0, gcm = 0, gni = 0;
-            // kEpsA: face a, p and rho, and the normal components of surfGrad a
+            // kEpsA: face a, p and rho, and the normal components of surfGrad a;
+            // BHR: the interpolated face a of div(a a) and surfGrad b
             double as_[3] = {0, 0, 0}, ps_ = 0, rs_ = 0, ga_[3] = {0, 0, 0};
+            double ai_[3] = {0, 0, 0}, gb_ = 0;
             if (inner) {
                 const long long nb =
                     id + (wrap ? (hi ? -(long long)(Na - 1) : (long long)(Na - 1)) * P.stride[a]
@@ -580,7 +600,12 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                     }
                     ps_ = qo[L::P] * wo + qn[L::P] * wnn;
                     rs_ = qo[L::RHO] * wo + qn[L::RHO] * wnn;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        ai_[c] = as_[c];
                 }
+                if constexpr (AM == 2)
+                    gb_ = ((qn[L::B] - qo[L::B]) / m2) * da;
                 if constexpr (SPEC) {
                     Ts = qo[L::T] * wo + qn[L::T] * wnn;
                     double c0a = 0, c0b = 0;
@@ -698,6 +723,15 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                         rs_ = srho;
                         if (TP.bcta[p] == BC_FIXED_SURF)
                             err = 1;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+                            ai_[c] = as_[c];
+                    }
+                    if constexpr (AM == 2) {
+                        const double bg = bc_cell<double>(TP.bctb[p], TP.bcvb[p], w0[L::B], -1, a);
+                        gb_ = ((bg - w0[L::B]) / m) * nrm[a];
+                        if (TP.bctb[p] == BC_FIXED_SURF)
+                            err = 1;
                     }
                 }
                 if constexpr (SPEC) {
@@ -802,6 +836,14 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                     }
                     gP[a] += (na * ps_) * S;
                     gR[a] += (na * rs_) * S;
+                }
+                if constexpr (AM == 2) {
+                    // the b Laplacian with mu + rho nu_t / sigma_b; (a_f a_f) & N
+                    const double rhoDb = mu + srho * (tnu / TP.sigmab);
+                    dB += (((gb_ * rhoDb) * -1) * na) * S;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        dAA[c] += ((ai_[c] * ai_[a]) * na) * S;
                 }
             }
             // Green-Gauss sums of grad v (v_c N_b) S and div v (v & N) S
@@ -908,7 +950,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 Mb[(M::TB + s) * n + id] = tB[s];
             }
         } else {
-        constexpr int NA = AF ? 3 : 0;
+        constexpr int NA = AM == 2 ? 4 : AM == 1 ? 3 : 0;
         constexpr int NF = TURB ? NS + 7 + NA : NS + 5;
         double u[NF];
 #pragma unroll
@@ -994,14 +1036,16 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 diagT -= src / (w0[L::T] + STAB);
         }
         ftT += heat;
-        double kn = 0, en_ = 0, an[3] = {0, 0, 0};
+        double kn = 0, en_ = 0, an[3] = {0, 0, 0}, bn = 0;
         if constexpr (TURB) {
             const TurbParams &TP = *Tp;
             const double k = w0[L::K], eps = w0[L::EPS];
             const double rk = u[NS + 5], reps = u[NS + 6];
             // dissipation rho eps (kEpsModels.cpp:76-82) and the filtered part
             // of the turbulent stress (Diffusion.cpp:586-598, 719-733)
-            if (TP.model != TURB_ZERO)
+            if constexpr (AM == 2)
+                ftT += rho * eps * (1 + TP.CMSM * 2 * k / w0[L::CS2]);  // BHRModel.cpp:262-272
+            else if (TP.model != TURB_ZERO)
                 ftT += rho * eps;
             const double tnu = TP.Cmu * k * k / eps;
             const double thR = theta_s(TP, divV, k, eps, TP.CMS_R);
@@ -1038,7 +1082,72 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 const double me = (eps < 1E-3 * emax) ? 1e-3 * __DBL_MAX__ : eps;
                 rsrc = __builtin_fabs(TP.ts_coeff * me / ((Se1 + Se2 * eps) / rho + STAB));
             }
-            if constexpr (AF) {
+            if constexpr (AM == 2) {
+                // BHRModel.cpp:188-252 with the generation flag 1
+                const double av[3] = {w0[L::A], w0[L::A + 1], w0[L::A + 2]};
+                const double b = w0[L::B];
+                const double ek = eps / k;
+                const double Mat2 = 2 * k / w0[L::CS2];
+                const double Mat = __builtin_sqrt(Mat2);
+                const double thS = theta_s(TP, G[0] + G[4] + G[8], k, eps, TP.CMS_R);
+                double gdevA = 0, x[3], grav = 0;
+#pragma unroll
+                for (int q = 0; q < 9; ++q)
+                    gdevA += (dev[q] * thS) * G[q];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    x[c] = gP[c] / V - dDP[c] / V;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    grav += av[c] * x[c];
+                grav = grav * 1.0;
+                const double PiK =
+                    (Mat * TP.alpha3 * reps - (TP.alpha2 * gdevA + 8 * Mat * TP.alpha4 * gsph)) * Mat;
+                const double dissip = -reps;
+                const double Sk1 = gsph + gdevA + grav + PiK;
+                const double Sk2 = (1 + TP.CMSM * Mat2) * dissip / k;
+                const double gpos = (grav < 0.0) ? 0.0 : grav;
+                const double Se1 =
+                    (TP.C1 * gdevA + TP.C3 * gsph + TP.C0 * gpos + TP.C4 * PiK) * ek;
+                const double Se2 = TP.C2 * dissip / k;
+                sk = Sk1 + Sk2 * k;
+                se = Se1 + Se2 * eps;
+                const double emax = __longlong_as_double(static_cast<long long>(epsmax[0]));
+                const double me = (eps < 1E-3 * emax) ? 1e-3 * __DBL_MAX__ : eps;
+                rsrc = __builtin_fabs(TP.ts_coeff * me / ((Se1 + Se2 * eps) / rho + STAB));
+                const double rU =
+                    __builtin_fabs(TP.ts_coeff * __builtin_fabs(w0[L::UI]) / (grav + STAB));
+                rsrc = (rU < rsrc) ? rU : rsrc;
+                double tau[9], D[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    tau[q] = 1.0 * (dev[q] * thS + sph[q]) / rho;
+                    D[q] = gA[q] / V - G[q];
+                }
+                const double gr[3] = {gR[0] / V, gR[1] / V, gR[2] / V};
+                const double Sa2 = -rho * ek * TP.Ca;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double bgp = x[c] * b;
+                    const double tgr = tau[3 * c] * gr[0] + tau[3 * c + 1] * gr[1] + tau[3 * c + 2] * gr[2];
+                    const double ragv =
+                        D[c] * u[NS + 7] + D[3 + c] * u[NS + 8] + D[6 + c] * u[NS + 9];
+                    const double redis = (dAA[c] / V) * rho;
+                    const double sig = bgp + tgr + ragv + redis + Sa2 * av[c];
+                    an[c] = (av[c] * rho + (0. - dA[c] / V) * dt) / rho + sig / rho * dt;
+                }
+                // b: the explicit Laplacian only (the explicit solver gives b
+                // no sources, Diffusion.cpp:745-763), the minimal value
+                bn = (b * rho + (0. - dB / V) * dt) / rho + 0. / rho * dt;
+                if (bn < TP.minb)
+                    bn = TP.minb;
+                const double src = grav * (-1.0);
+                if (src >= 0.)
+                    ftT += src;
+                else
+                    diagT -= src / (w0[L::T] + STAB);
+            }
+            if constexpr (AM == 1) {
                 // kEpsAModel.cpp:170-225.  b = rho sum x_k / (rho_k + chi rho) /
                 // sum x_k rho_k / (rho_k + chi rho) - 1, clipped at 0
                 // (kEpsAModel.cpp:232-257; the reference's field + scalar drops
@@ -1150,8 +1259,10 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             un[NS + 6] = en_ * rho;
         }
 #pragma unroll
-        for (int c = 0; c < NA; ++c)
+        for (int c = 0; c < NA && c < 3; ++c)
             un[NS + 7 + c] = an[c] * rho;
+        if constexpr (AM == 2)
+            un[NS + 10] = bn * rho;
         un[NS + 4] = (Uin + rho * v2 * 0.5) + rkn;
 #pragma unroll
         for (int q = 0; q < NF; ++q)
@@ -1169,7 +1280,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
             if (en)
                 reduce_min_bits<NT>(rsrc, reinterpret_cast<unsigned long long *>(sc) + SC_SRCMIN);
         if constexpr (TURB)
-            if (Tp->model == TURB_SHEAR || Tp->model == TURB_KEPSA)
+            if (Tp->model == TURB_SHEAR || Tp->model == TURB_KEPSA || Tp->model == TURB_BHR)
                 reduce_min_bits<NT>(rsrc, reinterpret_cast<unsigned long long *>(sc) + SC_SRCMIN);
     }
 }
@@ -1404,15 +1515,15 @@ diff_finish_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
 
 // Turbulent explicit stage: the primitives pass with k, epsilon, CvM and
 // nonIdeality / rho and the maximum of epsilon, then the update.
-template <int NS, int DIM, bool AF>
+template <int NS, int DIM, int AM>
 int launch_dim_turb(const DiffusionLaunch &L, const TurbDiffusion &T)
 {
     const unsigned blocks = (unsigned)((L.n + NT - 1) / NT);
-    hipLaunchKernelGGL((diff_prims_kernel<NS, true, AF>), dim3(blocks), dim3(NT), 0, L.stream,
-                       L.P, L.U, L.Ualt, L.hk, L.W, L.sc, T.epsmax);
+    hipLaunchKernelGGL((diff_prims_kernel<NS, true, AM>), dim3(blocks), dim3(NT), 0, L.stream,
+                       L.P, L.U, L.Ualt, L.hk, L.W, L.sc, T.epsmax, T.b2);
     if (L.mid && hipEventRecord(L.mid, L.stream) != hipSuccess)
         return 1;
-    hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false, false, true, AF>),
+    hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false, false, true, AM>),
                        dim3(blocks), dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W,
                        L.sc, L.err, L.mu, L.D, L.kappa, nullptr, 0, 0, T.TP, T.epsmax, L.star);
     return hipGetLastError() == hipSuccess ? 0 : 1;
@@ -1512,16 +1623,24 @@ int launch_diffusion_turb(const DiffusionLaunch &L, const TurbDiffusion &T)
 {
     auto by_dim = [&](auto ns) -> int {
         constexpr int NS = decltype(ns)::value;
+        auto by_model = [&](auto dim) -> int {
+            constexpr int D = decltype(dim)::value;
+            switch (T.amodel) {
+            case 1:
+                return launch_dim_turb<NS, D, 1>(L, T);
+            case 2:
+                return launch_dim_turb<NS, D, 2>(L, T);
+            default:
+                return launch_dim_turb<NS, D, 0>(L, T);
+            }
+        };
         switch (L.dim) {
         case 1:
-            return T.afield ? launch_dim_turb<NS, 1, true>(L, T)
-                            : launch_dim_turb<NS, 1, false>(L, T);
+            return by_model(std::integral_constant<int, 1>{});
         case 2:
-            return T.afield ? launch_dim_turb<NS, 2, true>(L, T)
-                            : launch_dim_turb<NS, 2, false>(L, T);
+            return by_model(std::integral_constant<int, 2>{});
         default:
-            return T.afield ? launch_dim_turb<NS, 3, true>(L, T)
-                            : launch_dim_turb<NS, 3, false>(L, T);
+            return by_model(std::integral_constant<int, 3>{});
         }
     };
     switch (L.ns) {

@@ -11,7 +11,7 @@
 // writes the average (S + S') / 2 with the first stage's value.  NT: the
 // state carries rho k and rho epsilon (2) or not (0; k = epsilon = 0, which
 // leaves the closure, the wave speeds and the star state of a laminar face
-// unchanged); NA: components of rho a.
+// unchanged); NA: the passive fields after rho epsilon (rho a; rho a, rho b).
 #include "fv_star.hpp"
 #include "fv_star_launch.hpp"
 
@@ -94,7 +94,7 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
         }
         const AxisGeo G = axis_geo(P, geo, 0);
         using SL = SlopeLayout<NS, false>;
-        T L[3][SL::NL], Lt[3][2], La[3];
+        T L[3][SL::NL], Lt[3][2], La[3], Lb[3];
 #pragma unroll
         for (int d = -1; d <= 1; ++d) {
             const bool ok = per || (i + d >= 0 && i + d < Nx);
@@ -110,6 +110,9 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
             if constexpr (NA > 0)
                 La[d + 1] = slope_turb_a<T>(P, TP, G, 0, j, w[d + 1][NS + 6], w[d + 2][NS + 6],
                                             w[d + 3][NS + 6], T(cgj.S[0]), T(cgj.V));
+            if constexpr (NA == 4)
+                Lb[d + 1] = slope_turb_b<T>(P, TP, G, 0, j, w[d + 1][NS + 9], w[d + 2][NS + 9],
+                                            w[d + 3][NS + 9], T(cgj.S[0]), T(cgj.V));
         }
         auto side = [&](int d, bool hi, T *s) {
             const int j = ((i + d) % Nx + Nx) % Nx;
@@ -127,10 +130,11 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
             s[NS + 5] = Lt[d + 1][1] * off + w[d + 2][NS + 5];
             if constexpr (NA > 0) {
                 s[NS + 6] = La[d + 1] * off + w[d + 2][NS + 6];
-#pragma unroll
-                for (int c2 = 1; c2 < NA; ++c2)
-                    s[NS + 6 + c2] = w[d + 2][NS + 6 + c2];
+                s[NS + 7] = w[d + 2][NS + 7];
+                s[NS + 8] = w[d + 2][NS + 8];
             }
+            if constexpr (NA == 4)
+                s[NS + 9] = Lb[d + 1] * off + w[d + 2][NS + 9];
         };
         T so[NQ], sb[NQ];
         if (!per && i == 0) {
@@ -193,6 +197,8 @@ template <int NS, int DIM, bool SLOPES> int launch_star_fields(const StarLaunch 
         return launch_star_gas<NS, DIM, SLOPES, 2, 0>(L);
     if (L.nt == 2 && L.na == 3)
         return launch_star_gas<NS, DIM, SLOPES, 2, 3>(L);
+    if (L.nt == 2 && L.na == 4)
+        return launch_star_gas<NS, DIM, SLOPES, 2, 4>(L);
     return -1;
 }
 

@@ -14,12 +14,12 @@ struct StarLaunch {
     const TurbParams *TP;  // zero-filled for a laminar state
     const double *geo;
     const double *U;       // input state of the stage
-    double *S;             // star buffer, (5 + na) x P.star_nft doubles
+    double *S;             // star buffer, (5 + na) x P.star_nft doubles (a, b)
     const double *sc;      // scalar block (SC_DONE)
     int avg;               // 0: first stage (store), 1: second (average)
     int dim, ns, gas_model;
     int slopes;            // 1-D mesh with a TVD limiter
-    int nt, na;            // rho k, rho eps rows (0 or 2); rho a rows (0 or 3)
+    int nt, na;            // rho k, rho eps rows (0 or 2); rho a, rho b rows (0, 3, 4)
     long long n;
     hipStream_t stream;
 };

@@ -8,9 +8,11 @@
 // E - rho v^2 / 2 - rho k.  The diffusion stage's turbulent terms live in
 // fv_diffusion.hip and use TurbParams.
 //
-// NA (0 or 3) is the number of components of a carried by the state: each
-// component of rho a is fluxed with the arithmetic of rho k, so the kEpsA
-// model's transport adds three passive fields.
+// NA is the number of passive fields after rho epsilon: 0, 3 (rho a of the
+// kEpsA model) or 4 (rho a and rho b of the BHR model).  Each is fluxed
+// with the arithmetic of rho k (HLLCSolver.cpp:145-147, HLLSolver.cpp:
+// 117-119, 237-251), so the a and b transport adds passive fields; they
+// differ only in their boundary conditions and slopes (a: vector, b: scalar).
 //
 // turb_stage_kernel: one thread per cell.  The cell evaluates its 2 DIM faces
 // in the reference's order (x-, x+, y+, y-, z-, z+) with the face's
@@ -29,7 +31,7 @@
 namespace sg {
 
 // turbulenceModel (turbulenceModelEnum.hpp) values built here
-enum : int { TURB_ZERO = 0, TURB_SHEAR = 2, TURB_KEPSA = 6 };
+enum : int { TURB_ZERO = 0, TURB_SHEAR = 2, TURB_KEPSA = 6, TURB_BHR = 7 };
 
 // Parameters of set/turbulentParameters.txt in the k-epsilon models' meaning
 // (abstractTurbulenceModel.hpp:26-57, kEpsModels.hpp, shearModel.cpp:35-59,
@@ -46,6 +48,11 @@ struct TurbParams {
     double sigmaa, C0, Ca, CaMax, CMSA, chi;
     int bcta[6];
     double bcva[6][3];
+    // BHR (BHRModel.cpp:86-137; C0..C3, Ca, sigma_a above): C4, Cb1, Cb2,
+    // alpha2, alpha3, alpha4, CMS_M, sigma_b, the minimal b, set/b.txt
+    double C4, Cb1, Cb2, alpha2, alpha3, alpha4, CMSM, sigmab, minb;
+    int bctb[6];
+    double bcvb[6];
 };
 
 // kEpsModels::thetaS (kEpsModels.cpp:13-26): realisability factor of the
@@ -328,6 +335,17 @@ __device__ __forceinline__ T slope_turb(const DevParams &P, const TurbParams &TP
                             T(TP.bcv[phi_][f]), -1, qm, q0, qp, S, V);
 }
 
+// Limited slope of the BHR model's b.
+template <class T>
+__device__ __forceinline__ T slope_turb_b(const DevParams &P, const TurbParams &TP,
+                                          const AxisGeo &G, int a, int i, T qm, T q0, T qp, T S,
+                                          T V)
+{
+    const int plo = patch_of(a, 0), phi_ = patch_of(a, 1);
+    return slope_turb_bc<T>(P, G, a, i, TP.bctb[plo], T(TP.bcvb[plo]), TP.bctb[phi_],
+                            T(TP.bcvb[phi_]), -1, qm, q0, qp, S, V);
+}
+
 // Limited slope along axis a of the a component along that axis (the
 // transverse components have none).
 template <class T>
@@ -351,8 +369,10 @@ __device__ __forceinline__ void ghost_turb(const DevParams &P, const TurbParams 
     g[NS + 4] = bc_cell<T>(TP.bct[p][0], T(TP.bcv[p][0]), o[NS + 4], -1, a);
     g[NS + 5] = bc_cell<T>(TP.bct[p][1], T(TP.bcv[p][1]), o[NS + 5], -1, a);
 #pragma unroll
-    for (int c = 0; c < NA; ++c)
+    for (int c = 0; c < NA && c < 3; ++c)
         g[NS + 6 + c] = bc_cell<T>(TP.bcta[p], T(TP.bcva[p][c]), o[NS + 6 + c], c, a);
+    if constexpr (NA == 4)
+        g[NS + 9] = bc_cell<T>(TP.bctb[p], T(TP.bcvb[p]), o[NS + 9], -1, a);
 }
 
 template <class T, int NS, int GAS, int NA = 0>
@@ -394,7 +414,7 @@ __global__ void __launch_bounds__(256)
 turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
                   const double *__restrict__ geo, const double *Uin, double *Uout,
                   const double *Un, const double *__restrict__ dt_ptr,
-                  unsigned long long *sigmax, int *errflag)
+                  unsigned long long *sigmax, int *errflag, double *__restrict__ b2)
 {
     constexpr int NF = NS + 7 + NA, NQ = NS + 6 + NA;
     __shared__ double sRed[4];
@@ -450,7 +470,7 @@ turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict
         }
         const AxisGeo G = axis_geo(P, geo, 0);
         using SL = SlopeLayout<NS, false>;
-        T L[3][SL::NL], Lt[3][2], La[3];
+        T L[3][SL::NL], Lt[3][2], La[3], Lb[3];
 #pragma unroll
         for (int d = -1; d <= 1; ++d) {
             const bool ok = per || (i + d >= 0 && i + d < Nx);
@@ -466,6 +486,9 @@ turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict
             if constexpr (NA > 0)
                 La[d + 1] = slope_turb_a<T>(P, TP, G, 0, j, w[d + 1][NS + 6], w[d + 2][NS + 6],
                                             w[d + 3][NS + 6], T(cgj.S[0]), T(cgj.V));
+            if constexpr (NA == 4)
+                Lb[d + 1] = slope_turb_b<T>(P, TP, G, 0, j, w[d + 1][NS + 9], w[d + 2][NS + 9],
+                                            w[d + 3][NS + 9], T(cgj.S[0]), T(cgj.V));
         }
         // face side of cell i + d (d = -1, 0, 1) at its low (hi = false) or
         // high face
@@ -485,10 +508,11 @@ turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict
             s[NS + 5] = Lt[d + 1][1] * off + w[d + 2][NS + 5];
             if constexpr (NA > 0) {
                 s[NS + 6] = La[d + 1] * off + w[d + 2][NS + 6];
-#pragma unroll
-                for (int c = 1; c < NA; ++c)
-                    s[NS + 6 + c] = w[d + 2][NS + 6 + c];
+                s[NS + 7] = w[d + 2][NS + 7];
+                s[NS + 8] = w[d + 2][NS + 8];
             }
+            if constexpr (NA == 4)
+                s[NS + 9] = Lb[d + 1] * off + w[d + 2][NS + 9];
         };
         T so[NQ], sb[NQ];
         // x- face
@@ -588,6 +612,11 @@ turb_stage_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict
                 un[q] *= f;
         }
         if constexpr (SV == SV_RK2) {
+            // BHR: the RK average does not re-derive b (bunchOfFields.hpp:
+            // 166-203); the diffusion stage reads the second stage's
+            // rho b / rho (Advection2dOrder.cpp:623-625)
+            if constexpr (NA == 4)
+                b2[id] = un[NS + 10] / un[NS];
 #pragma unroll
             for (int q = 0; q < NF; ++q) {
                 const double o = Un[q * n + id];

@@ -14,20 +14,22 @@ static int launch_turb_na(const TurbStageLaunch &L)
     if (L.gas_model == GAS_VDW)
         hipLaunchKernelGGL((turb_stage_kernel<T, NS, DIM, SV, MIXED, SLOPES, GAS_VDW, NA>),
                            dim3(blocks), dim3(256), 0, L.stream, L.P, L.TP, L.geo, L.Uin, L.Uout,
-                           L.Un, L.dt, L.sigmax, L.err);
+                           L.Un, L.dt, L.sigmax, L.err, L.b2);
     else if (L.gas_model == GAS_IDEAL)
         hipLaunchKernelGGL((turb_stage_kernel<T, NS, DIM, SV, MIXED, SLOPES, GAS_IDEAL, NA>),
                            dim3(blocks), dim3(256), 0, L.stream, L.P, L.TP, L.geo, L.Uin, L.Uout,
-                           L.Un, L.dt, L.sigmax, L.err);
+                           L.Un, L.dt, L.sigmax, L.err, L.b2);
     else
         return -1;
     return static_cast<int>(hipGetLastError());
 }
 
-// na: components of a in the state (0, or 3 for the kEpsA model)
+// na: passive fields after rho epsilon (0; 3 kEpsA: rho a; 4 BHR: rho a, rho b)
 template <class T, int NS, int DIM, int SV, bool MIXED, bool SLOPES>
 static int launch_turb_gas(const TurbStageLaunch &L)
 {
+    if (L.na == 4)
+        return launch_turb_na<T, NS, DIM, SV, MIXED, SLOPES, 4>(L);
     if (L.na == 3)
         return launch_turb_na<T, NS, DIM, SV, MIXED, SLOPES, 3>(L);
     if (L.na == 0)

@@ -23,7 +23,8 @@ struct TurbStageLaunch {
     int *err;
     int dim, ns, variant, gas_model;  // variant: SV_RK1 or SV_RK2
     int slopes;                       // 1-D mesh with a TVD limiter
-    int na;                           // components of a in the state (0 or 3)
+    int na;                           // passive fields after rho eps (0, 3: a, 4: a and b)
+    double *b2;                       // BHR: the second stage's b (n), else null
     long long n;
     hipStream_t stream;
 };
@@ -34,13 +35,15 @@ int launch_turb_stage_f64(const TurbStageLaunch &L);
 int launch_turb_stage_mixed(const TurbStageLaunch &L);
 
 // Explicit diffusion stage with k-epsilon turbulence (fv_diffusion.hip): W
-// holds (2 ns + 9) n doubles, (2 ns + 13) n with the kEpsA model (afield);
-// epsmax is the bit pattern of the largest epsilon of the state (one word,
-// zero before each stage).
+// holds (2 ns + 9) n doubles, (2 ns + 13) n with the kEpsA model and
+// (2 ns + 16) n with BHR; epsmax is the bit pattern of the largest epsilon of
+// the state (one word, zero before each stage); b2 the BHR model's second
+// stage b.
 struct TurbDiffusion {
     const TurbParams *TP;
     unsigned long long *epsmax;
-    int afield;
+    int amodel;  // 0: no a field, 1: kEpsA, 2: BHR
+    const double *b2;
 };
 int launch_diffusion_turb(const DiffusionLaunch &L, const TurbDiffusion &T);
 

@@ -76,7 +76,9 @@ class SchemeDesc(C.Structure):
                 ("turb_min_eps", C.c_double), ("turb_C", C.c_double * 3),
                 ("bc_k", _BC * 6), ("bc_eps", _BC * 6), ("turb_sigma_a", C.c_double),
                 ("turb_kepsa_C", C.c_double * 8), ("bc_a", _BC * 6),
-                ("reconstruction", C.c_int32)]
+                ("reconstruction", C.c_int32), ("turb_sigma_b", C.c_double),
+                ("turb_min_b", C.c_double), ("turb_bhr_C", C.c_double * 12),
+                ("bc_b", _BC * 6)]
 
 
 _LIB = None
@@ -227,7 +229,13 @@ def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) 
         (s.turb_Cmu, s.turb_sigma_sc, s.turb_sigma_t, s.turb_sigma_e, s.turb_sigma_k,
          s.turb_sigma_eps, s.turb_CMS_R, s.turb_CMS_D, s.turb_min_k,
          s.turb_min_eps) = (float(x) for x in t.params())
-        if t.a_field:
+        if t.b_field:
+            for i, x in enumerate(t.C[:12]):
+                s.turb_bhr_C[i] = float(x)
+            s.turb_sigma_a = float(t.sigmaa)
+            s.turb_sigma_b = float(t.sigmab)
+            s.turb_min_b = float(t.minb)
+        elif t.a_field:
             for i, x in enumerate(t.C[:8]):
                 s.turb_kepsa_C[i] = float(x)
             s.turb_sigma_a = float(t.sigmaa)
@@ -235,7 +243,8 @@ def scheme_desc(case: Case, precision: int = PREC_FP64, const_dt: float = -1.0) 
             for i, x in enumerate(t.C[:3]):
                 s.turb_C[i] = float(x)
         for p in range(6):
-            for dst, src in ((s.bc_k, t.bc_k), (s.bc_eps, t.bc_eps), (s.bc_a, t.bc_a)):
+            for dst, src in ((s.bc_k, t.bc_k), (s.bc_eps, t.bc_eps), (s.bc_a, t.bc_a),
+                             (s.bc_b, t.bc_b)):
                 dst[p].type = int(src[p].type)
                 for a in range(3):
                     dst[p].value[a] = float(src[p].value[a])
@@ -558,7 +567,8 @@ def vdw_matrices(case: Case):
     return a, b
 
 
-def conserved_from_primitive(case: Case, c, v, p, kt=None, et=None, at=None) -> np.ndarray:
+def conserved_from_primitive(case: Case, c, v, p, kt=None, et=None, at=None,
+                             bt=None) -> np.ndarray:
     """Conserved vector from primitives (c_k, v, p), with the reference's
     initialisation arithmetic (phaseInitialization.cpp:1827-1883) and its
     UvFromp (idealFluid.cpp:19-23, vanDerWaalsFluid.cpp:24-31; Redlich-Kwong
@@ -636,6 +646,8 @@ def conserved_from_primitive(case: Case, c, v, p, kt=None, et=None, at=None) -> 
     rows = [rho_k, rho[None], m, E[None], rk[None], (rho * et)[None]]
     if t.a_field:
         rows.append(np.asarray(at, dtype=np.float64) * rho)
+    if t.b_field:
+        rows.append((rho * np.asarray(bt, dtype=np.float64))[None])
     return np.concatenate(rows)
 
 

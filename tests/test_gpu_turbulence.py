@@ -27,14 +27,16 @@ FIXTURES = ["seeded_2d_turb_advection_hllc", "seeded_2d_turb_shear_hllc",
             "seeded_2d_turb_shear_vdw_hll", "seeded_1d_turb_shear_vanleer",
             "seeded_3d_turb_shear_vanalbada", "seeded_2d_turb_zero_hllc",
             "seeded_2d_kepsa_advection_hllc", "seeded_2d_kepsa_hllc", "seeded_2d_kepsa_vdw_hll",
-            "seeded_1d_kepsa_vanleer", "seeded_3d_kepsa_vanalbada"]
+            "seeded_1d_kepsa_vanleer", "seeded_3d_kepsa_vanalbada",
+            "seeded_2d_bhr_advection_hllc", "seeded_2d_bhr_hllc", "seeded_2d_bhr_vdw_hll",
+            "seeded_1d_bhr_vanleer", "seeded_3d_bhr_vanalbada", "seeded_2d_bhr_star_hll"]
 
 
-def oracle_at(case, prim, turb, a=None):
+def oracle_at(case, prim, turb, a=None, b=None):
     from oracle.oracle import Oracle
 
     o = Oracle(case)
-    o.init_primitive(*prim, k=turb[0], eps=turb[1], a=a)
+    o.init_primitive(*prim, k=turb[0], eps=turb[1], a=a, b=b)
     return o
 
 
@@ -76,7 +78,8 @@ def seeded_kepsa(case, seed):
 
 def periodic_case(dim=2, gas=gasModel.ideal, flux=FS.HLL, model=TM.shearModel):
     n = (16, 16, 8)
-    turb = Turbulence.kEpsA() if model == TM.kEpsAModel else Turbulence(model=model)
+    turb = (Turbulence.kEpsA() if model == TM.kEpsAModel else
+            Turbulence.BHR() if model == TM.BHRModel else Turbulence(model=model))
     case = Case(dim=dim, N=n, L=(1.0, 1.0, 1.0), M=[29.0, 4.0], Cv=[2.5, 1 / 0.67],
                 flux=flux, limiter=TL.minmod, diffusion=True, transport=(2e-3, 2e-3, 2e-3),
                 turbulence=turb)
@@ -87,12 +90,13 @@ def periodic_case(dim=2, gas=gasModel.ideal, flux=FS.HLL, model=TM.shearModel):
         case.turbulence.bc_k[p] = BC(BCT.periodic)
         case.turbulence.bc_eps[p] = BC(BCT.periodic)
         case.turbulence.bc_a[p] = BC(BCT.periodic)
+        case.turbulence.bc_b[p] = BC(BCT.periodic)
     return case
 
 
-def parity(case, prim, turb, dts, a=None):
+def parity(case, prim, turb, dts, a=None, b=None):
     """GPU FP64 steps against the oracle's conserved-state steps, bit for bit."""
-    o = oracle_at(case, prim, turb, a)
+    o = oracle_at(case, prim, turb, a, b)
     U0 = o.conserved()
     o = oracle_conserved(case, U0)
     g = GpuSolver(case)
@@ -111,7 +115,8 @@ def parity(case, prim, turb, dts, a=None):
 @pytest.mark.parametrize("name", FIXTURES)
 def test_fixture_configuration_bitexact(name, hip_device):
     g = load(name)
-    parity(g["case"], g["restart"], g["restart_turb"], list(g["dts"]), g.get("restart_a"))
+    parity(g["case"], g["restart"], g["restart_turb"], list(g["dts"]), g.get("restart_a"),
+           g.get("restart_b"))
 
 
 @pytest.mark.gpu
@@ -124,14 +129,20 @@ def test_fixture_configuration_bitexact(name, hip_device):
     (2, gasModel.vanDerWaals, FS.HLL, TM.kEpsAModel),
     (3, gasModel.ideal, FS.HLLC, TM.kEpsAModel),
     (1, gasModel.ideal, FS.HLL, TM.kEpsAModel),
+    (2, gasModel.vanDerWaals, FS.HLLC, TM.BHRModel),
+    (3, gasModel.ideal, FS.HLL, TM.BHRModel),
+    (1, gasModel.ideal, FS.HLLC, TM.BHRModel),
 ])
 def test_periodic_bitexact(dim, gas, flux, model, hip_device):
     case = periodic_case(dim, gas, flux, model)
-    if model == TM.kEpsAModel:
+    b = None
+    if model in (TM.kEpsAModel, TM.BHRModel):
         prim, turb, a = seeded_kepsa(case, 300 + dim)
+        if model == TM.BHRModel:
+            b = 0.05 + 0.45 * np.random.default_rng(400 + dim).random(case.ncells)
     else:
         (prim, turb), a = seeded(case, 300 + dim), None
-    parity(case, prim, turb, [2e-4] * 4, a)
+    parity(case, prim, turb, [2e-4] * 4, a, b)
 
 
 @pytest.mark.gpu
@@ -178,27 +189,32 @@ def test_courant_source_sequence(hip_device):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("model", [TM.shearModel, TM.kEpsAModel])
+@pytest.mark.parametrize("model", [TM.shearModel, TM.kEpsAModel, TM.BHRModel])
 def test_mixed_precision_turbulent(model, hip_device):
     """Mixed FP32-flux / FP64-update path of the turbulent kernels against
     FP64 after 40 steps of the periodic van der Waals case (shear model) or
-    10 steps (kEpsA, whose energy sink cools the state quickly; also rho a)."""
+    10 steps (kEpsA and BHR, whose energy sink cools the state quickly; also
+    rho a, and rho b for BHR)."""
     case = periodic_case(2, gasModel.vanDerWaals, FS.HLL, model)
-    if model == TM.kEpsAModel:
+    b = None
+    if model in (TM.kEpsAModel, TM.BHRModel):
         prim, turb, a = seeded_kepsa(case, 77)
+        if model == TM.BHRModel:
+            b = 0.05 + 0.45 * np.random.default_rng(477).random(case.ncells)
     else:
         (prim, turb), a = seeded(case, 77), None
-    U0 = oracle_at(case, prim, turb, a).conserved()
+    U0 = oracle_at(case, prim, turb, a, b).conserved()
     out = {}
     for prec in (PREC_FP64, PREC_MIXED):
         g = GpuSolver(case, prec)
         g.set_conserved(U0)
-        for _ in range(10 if model == TM.kEpsAModel else 40):
+        for _ in range(40 if model == TM.shearModel else 10):
             g.step(2e-4)
         out[prec] = g.conserved()
     a, b = out[PREC_FP64], out[PREC_MIXED]
     Ns = case.Ns
-    rows = (Ns, Ns + 4, Ns + 5, Ns + 6) + ((Ns + 7, Ns + 8) if model == TM.kEpsAModel else ())
+    rows = (Ns, Ns + 4, Ns + 5, Ns + 6) + ((Ns + 7, Ns + 8) if model != TM.shearModel else ()) + (
+        (Ns + 10,) if model == TM.BHRModel else ())
     for q in rows:
         l1 = np.abs(b[q] - a[q]).sum() / np.abs(a[q]).sum()
         assert l1 <= 1e-5, (q, l1)
@@ -210,7 +226,7 @@ def test_unsupported_turbulence(hip_device):
     case = periodic_case()
     for bad in (dict(limiter=TL.linear), dict(step_solver=0), dict(sle_solver="Jacobi"),
                 dict(molar_mass_correction=True), dict(gas_model=gasModel.RedlichKwong),
-                dict(turbulence=Turbulence(model=TM.BHRModel))):
+                dict(turbulence=Turbulence(model=TM.BHRKLModel))):
         with pytest.raises(SchemiGpuError):
             GpuSolver(dataclasses.replace(case, **bad))
 

@@ -324,6 +324,18 @@ int schemi_gpu_step(schemi_gpu_ctx *ctx, double dt, double *dt_next);
  * whose own loop forms the time step writes it to mesh::timestepSourceRef(). */
 int schemi_gpu_source_timestep(schemi_gpu_ctx *ctx, double *dt_source);
 
+/* The turbulent fields that the last step's diffusion stage leaves in the
+ * reference's phase besides the state: the k and epsilon the stage started
+ * from (rows 0 and 1; the phase's turbulent transport coefficients,
+ * nu_t = Cmu k^2 / epsilon and nu_t / sigma for the others, are formed from
+ * them at Diffusion.cpp:129-135 with transportCoefficients.hpp:65-83) and,
+ * with the kEpsA model, the stage's algebraic b, which the phase keeps as its
+ * b (row 2; kEpsAModel.cpp:232-257, Diffusion.cpp:525-527, 1006).  out:
+ * 2 (kEpsA: 3) x ncells doubles.  SCHEMI_ERR_INIT without turbulence or
+ * before a step with the diffusion stage since the last set_state.
+ * Synchronous. */
+int schemi_gpu_get_turbulence_stage(schemi_gpu_ctx *ctx, double *out, int32_t numbering);
+
 /* nsteps SSP-RK2 steps with the time step kept on the device: the first
  * step uses dt0, every later one the Courant step of the previous state,
  * clamped so that the time does not pass t_end (Schemi.cpp:933-935).  Stops

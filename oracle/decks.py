@@ -301,6 +301,7 @@ def read_output(root, number, Ns):
         "eps": col["epsilon"],
         "a": np.stack([col["a_x"], col["a_y"], col["a_z"]]),
         "b": col["b"],
+        "nut": col["turbulent_viscosity"],
         "xyz": np.stack([col["coordinate_x"], col["coordinate_y"],
                          col["coordinate_z"]]),
     }

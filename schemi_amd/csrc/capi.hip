@@ -226,7 +226,8 @@ struct schemi_gpu_ctx {
     bool turb = false;           // k-epsilon turbulence
     bool afield = false;         // kEpsA, BHR: the state carries rho a
     bool bfield = false;         // BHR: the state carries rho b
-    double *db2 = nullptr;       // BHR: the second stage's b (n)
+    double *db2 = nullptr;       // BHR: the second stage's b; kEpsA: the diffusion stage's b (n)
+    bool stage_turb = false;     // a turbulent diffusion stage has filled Wd's k, epsilon
     bool star = false;           // #Reconstruction star: star states per face
     double *dS = nullptr;        // star buffer, (5 + na) x hp.star_nft
     TurbParams tp{};
@@ -439,8 +440,10 @@ int one_step(schemi_gpu_ctx *c)
             T.epsmax = c->epsmax;
             T.amodel = c->bfield ? 2 : c->afield ? 1 : 0;
             T.b2 = c->db2;
+            T.bk = c->afield && !c->bfield ? c->db2 : nullptr;
             if (launch_diffusion_turb(L, T))
                 return SCHEMI_ERR_SYSTEM;
+            c->stage_turb = true;
         } else if (launch_diffusion(L)) {
             return SCHEMI_ERR_SYSTEM;
         }
@@ -1083,7 +1086,7 @@ int schemi_gpu_create(const schemi_mesh_desc *mesh, const schemi_mix_desc *mix,
                                                            : c->turb   ? 9
                                                                        : 5)) *
                                    (size_t)c->n) == hipSuccess;
-    if (ok && c->bfield)
+    if (ok && c->afield)
         ok = hipMalloc(&c->db2, sizeof(double) * (size_t)c->n) == hipSuccess;
     if (ok && c->star)
         ok = hipMalloc(&c->dS, sizeof(double) *
@@ -1172,6 +1175,7 @@ int schemi_gpu_set_state(schemi_gpu_ctx *c, const double *cons, int64_t ncells,
     HIPCHK(hipMemsetAsync(c->derr, 0, sizeof(int), c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->time = 0.0;
+    c->stage_turb = false;
     return SCHEMI_OK;
 }
 
@@ -1213,6 +1217,20 @@ int schemi_gpu_get_state(schemi_gpu_ctx *c, double *cons, double *prim, int32_t 
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(prim, dW, pbytes, hipMemcpyDeviceToHost, c->stream));
     }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return SCHEMI_OK;
+}
+
+int schemi_gpu_get_turbulence_stage(schemi_gpu_ctx *c, double *out, int32_t numbering)
+{
+    if (!c || !out || numbering != SCHEMI_NUMBERING_REFERENCE || !c->turb || !c->stage_turb)
+        return SCHEMI_ERR_INIT;
+    // Wd rows k and epsilon of the stage's primitives (DW<NS>::K, EPS)
+    const size_t row = sizeof(double) * (size_t)c->n;
+    const size_t k = (size_t)(2 * c->ns + 5) * (size_t)c->n;
+    HIPCHK(hipMemcpyAsync(out, c->Wd + k, 2 * row, hipMemcpyDeviceToHost, c->stream));
+    if (c->afield && !c->bfield)
+        HIPCHK(hipMemcpyAsync(out + 2 * c->n, c->db2, row, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return SCHEMI_OK;
 }

@@ -431,7 +431,7 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                    int *errflag, double mu, double Dc, double kappa, double *Mb, int mm, int en,
                    const TurbParams *__restrict__ Tp = nullptr,
                    const unsigned long long *__restrict__ epsmax = nullptr,
-                   const double *__restrict__ star = nullptr)
+                   const double *__restrict__ star = nullptr, double *__restrict__ bk = nullptr)
 {
     static_assert(!(TURB && (IMPL || SPEC)), "turbulence: explicit solver without species terms");
     constexpr bool AF = AM != 0;
@@ -1163,6 +1163,9 @@ diff_update_kernel(const DevParams *__restrict__ Pp, const double *__restrict__ 
                 double b = rho * num / den - 1.;
                 if (b < 0.)
                     b = 0.0;
+                // the phase keeps it as its b (Diffusion.cpp:525-527, 1006)
+                if (bk)
+                    bk[id] = b;
                 const double av[3] = {w0[L::A], w0[L::A + 1], w0[L::A + 2]};
                 const double ek = eps / k;
                 // thetaS_R of trace(grad v); thetaA (kEpsAModel.cpp:16-27)
@@ -1525,7 +1528,8 @@ int launch_dim_turb(const DiffusionLaunch &L, const TurbDiffusion &T)
         return 1;
     hipLaunchKernelGGL((diff_update_kernel<NS, DIM, false, false, false, true, AM>),
                        dim3(blocks), dim3(NT), 0, L.stream, L.P, L.geo, L.U, L.Ualt, L.hk, L.W,
-                       L.sc, L.err, L.mu, L.D, L.kappa, nullptr, 0, 0, T.TP, T.epsmax, L.star);
+                       L.sc, L.err, L.mu, L.D, L.kappa, nullptr, 0, 0, T.TP, T.epsmax, L.star,
+                       AM == 1 ? T.bk : nullptr);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

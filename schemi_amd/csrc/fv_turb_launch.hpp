@@ -44,6 +44,7 @@ struct TurbDiffusion {
     unsigned long long *epsmax;
     int amodel;  // 0: no a field, 1: kEpsA, 2: BHR
     const double *b2;
+    double *bk;  // kEpsA: the stage's algebraic b (n) is written here when not null
 };
 int launch_diffusion_turb(const DiffusionLaunch &L, const TurbDiffusion &T);
 

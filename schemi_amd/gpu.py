@@ -89,7 +89,7 @@ _SYMBOLS = ("schemi_gpu_create", "schemi_gpu_destroy", "schemi_gpu_set_state",
             "schemi_gpu_face_permutation", "schemi_gpu_axis_table", "schemi_gpu_state_device",
             "schemi_gpu_adjoint_step", "schemi_gpu_mixing_width",
             "schemi_gpu_mixing_width_noalloc", "schemi_gpu_diffusion_timing",
-            "schemi_gpu_source_timestep")
+            "schemi_gpu_source_timestep", "schemi_gpu_get_turbulence_stage")
 
 
 def _lib():
@@ -130,6 +130,8 @@ def _lib():
         L.schemi_gpu_mixing_width.argtypes = [vp, C.c_int32, dp, vp]
         L.schemi_gpu_source_timestep.restype = C.c_int
         L.schemi_gpu_source_timestep.argtypes = [vp, dp]
+        L.schemi_gpu_get_turbulence_stage.restype = C.c_int
+        L.schemi_gpu_get_turbulence_stage.argtypes = [vp, dp, C.c_int32]
         L.schemi_gpu_mixing_width_noalloc.restype = C.c_int
         L.schemi_gpu_mixing_width_noalloc.argtypes = [vp, C.c_int32, dp, vp]
         L.schemi_gpu_timing.restype = C.c_int
@@ -401,6 +403,19 @@ class GpuSolver:
         v = C.c_double()
         self._check(_lib().schemi_gpu_source_timestep(self._ctx, C.byref(v)), "source_timestep")
         return v.value
+
+    def turbulence_stage(self) -> dict:
+        """k and epsilon at the start of the last diffusion stage and, with the
+        kEpsA model, that stage's algebraic b."""
+        t = self.case.turbulence
+        kepsa = t is not None and t.a_field and not t.b_field
+        out = np.zeros((3 if kepsa else 2, self.n))
+        self._check(_lib().schemi_gpu_get_turbulence_stage(self._ctx, _dp(out), 0),
+                    "get_turbulence_stage")
+        d = {"k": out[0], "eps": out[1]}
+        if kepsa:
+            d["b"] = out[2]
+        return d
 
     def mixing_width_noalloc(self, species: int = 0, grad: int = 0) -> float:
         """mixing_width through the no-allocation entry point (partial sums in

@@ -24,7 +24,17 @@ stepCore::stepCore(const schemi_mesh_desc &mesh, const schemi_mix_desc &mix,
     check(schemi_gpu_create(&mesh, &mix, &scheme, &ctx), "schemi_gpu_create");
     ns = static_cast<std::size_t>(mix.ns);
     n = static_cast<std::size_t>(mesh.n[0] * mesh.n[1] * mesh.n[2]);
-    cons.assign((ns + 5) * n, 0.0);
+    nf = ns + 5;
+    if (scheme.turbulence) {
+        nf += 2;
+        if (scheme.turbulence_model == SCHEMI_TURB_KEPSA)
+            nf += 3;
+        else if (scheme.turbulence_model == SCHEMI_TURB_BHR)
+            nf += 4;
+    }
+    cons.assign(nf * n, 0.0);
+    if (scheme.turbulence && scheme.diffusion)
+        nstage = scheme.turbulence_model == SCHEMI_TURB_KEPSA ? 3 : 2;
     prim.assign((ns + 8) * n, 0.0);
 }
 
@@ -50,6 +60,15 @@ void stepCore::download()
 {
     check(schemi_gpu_get_state(ctx, cons.data(), prim.data(), SCHEMI_NUMBERING_REFERENCE),
           "schemi_gpu_get_state");
+}
+
+std::vector<double> stepCore::turbulenceStage() const
+{
+    std::vector<double> out(nstage * n);
+    if (nstage)
+        check(schemi_gpu_get_turbulence_stage(ctx, out.data(), SCHEMI_NUMBERING_REFERENCE),
+              "schemi_gpu_get_turbulence_stage");
+    return out;
 }
 
 }  // namespace schemi_gpu_host

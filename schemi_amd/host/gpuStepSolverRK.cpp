@@ -7,6 +7,7 @@
 #include "HLLCSolver.hpp"
 #include "HLLSolver.hpp"
 #include "exception.hpp"
+#include "kEpsModels.hpp"
 #include "linearLimiter.hpp"
 #include "mesh.hpp"
 #include "minmodLimiter.hpp"
@@ -43,6 +44,16 @@ struct transportConstants : abstractTransportModel {
     static scalar kappa(const abstractTransportModel &t)
     {
         return (t.*(&transportConstants::kappaConst))();
+    }
+};
+
+// A turbulence model's constants as read from set/turbulentParameters.txt
+// (abstractTurbulenceModel.hpp:55, a protected accessor; shearModel.cpp:
+// 56-58, kEpsAModel.cpp:73-80, BHRModel.cpp:73-84), read the same way.
+struct turbulenceConstants : abstractTurbulenceModel {
+    static const std::valarray<scalar> &of(const abstractTurbulenceModel &t)
+    {
+        return (t.*(&turbulenceConstants::modelParameters))();
     }
 };
 
@@ -198,8 +209,85 @@ descriptors describe(int stepSolver, int precision, const homogeneousPhase<cubic
         d.scheme.bc_p[p].type = bcCode(pc.first, "pressure");
         d.scheme.bc_p[p].value[0] = pc.second;
     }
-    if (gasPhase.turbulence->turbulence())
-        unsupported("turbulence");
+    // k-epsilon turbulence (abstractTurbulenceModel.cpp:235-320): the model,
+    // its constants and the boundary conditions of k, epsilon, a and b
+    const abstractTurbulenceModel &tm = *gasPhase.turbulence;
+    if (tm.turbulence()) {
+        const auto *ke = dynamic_cast<const kEpsModels *>(&tm);
+        int model = -1;
+        switch (tm.model()) {
+        case turbulenceModel::zeroModel:
+            model = SCHEMI_TURB_ZERO;
+            break;
+        case turbulenceModel::shearModel:
+            model = SCHEMI_TURB_SHEAR;
+            break;
+        case turbulenceModel::kEpsAModel:
+            model = SCHEMI_TURB_KEPSA;
+            break;
+        case turbulenceModel::BHRModel:
+            model = SCHEMI_TURB_BHR;
+            break;
+        default:
+            break;
+        }
+        if (model < 0 || ke == nullptr)
+            unsupported("this turbulence model");
+        if (tm.isInitialisationModelUsed())
+            unsupported("the Goncharov initialisation model");
+        d.scheme.turbulence = 1;
+        d.scheme.turbulence_model = model;
+        d.scheme.turb_Cmu = tm.Cmu();
+        d.scheme.turb_sigma_sc = tm.sigmaSc();
+        d.scheme.turb_sigma_t = tm.sigmaT();
+        d.scheme.turb_sigma_e = tm.sigmaE();
+        d.scheme.turb_sigma_k = tm.sigmaK();
+        d.scheme.turb_sigma_eps = tm.sigmaEps();
+        d.scheme.turb_CMS_R = ke->CMS_R();
+        d.scheme.turb_CMS_D = ke->CMS_D();
+        d.scheme.turb_min_k = tm.mink();
+        d.scheme.turb_min_eps = tm.minepsilon();
+        const std::valarray<scalar> &C = turbulenceConstants::of(tm);
+        if (model == SCHEMI_TURB_SHEAR)
+            for (std::size_t j = 0; j < 3 && j < C.size(); ++j)
+                d.scheme.turb_C[j] = C[j];
+        if (model == SCHEMI_TURB_KEPSA) {
+            d.scheme.turb_sigma_a = tm.sigmaa();
+            for (std::size_t j = 0; j < 8 && j < C.size(); ++j)
+                d.scheme.turb_kepsa_C[j] = C[j];
+        }
+        if (model == SCHEMI_TURB_BHR) {
+            d.scheme.turb_sigma_a = tm.sigmaa();
+            d.scheme.turb_sigma_b = tm.sigmab();
+            d.scheme.turb_min_b = tm.minb();
+            for (std::size_t j = 0; j < 12 && j < C.size(); ++j)
+                d.scheme.turb_bhr_C[j] = C[j];
+        }
+        for (int p = 0; p < 6; ++p) {
+            if (pr.count[p] == 0) {
+                d.scheme.bc_k[p].type = d.scheme.bc_eps[p].type = SCHEMI_BC_FREE;
+                d.scheme.bc_a[p].type = d.scheme.bc_b[p].type = SCHEMI_BC_FREE;
+                continue;
+            }
+            const auto k = patchCondition(gasPhase.kTurb.boundCond(), pr, p, "k");
+            d.scheme.bc_k[p].type = bcCode(k.first, "k");
+            d.scheme.bc_k[p].value[0] = k.second;
+            const auto e = patchCondition(gasPhase.epsTurb.boundCond(), pr, p, "epsilon");
+            d.scheme.bc_eps[p].type = bcCode(e.first, "epsilon");
+            d.scheme.bc_eps[p].value[0] = e.second;
+            if (tm.aField()) {
+                const auto a = patchCondition(gasPhase.aTurb.boundCond(), pr, p, "a");
+                d.scheme.bc_a[p].type = bcCode(a.first, "a");
+                for (int c = 0; c < 3; ++c)
+                    d.scheme.bc_a[p].value[c] = a.second()[c];
+            }
+            if (tm.bField()) {
+                const auto b = patchCondition(gasPhase.bTurb.boundCond(), pr, p, "b");
+                d.scheme.bc_b[p].type = bcCode(b.first, "b");
+                d.scheme.bc_b[p].value[0] = b.second;
+            }
+        }
+    }
     if (chemKin.chemicalReaction)
         unsupported("chemical kinetics");
     // diffusion stage (Diffusion.cpp:20-1061)
@@ -211,8 +299,9 @@ descriptors describe(int stepSolver, int precision, const homogeneousPhase<cubic
         d.scheme.transport_mu = transportConstants::mu(tm);
         d.scheme.transport_D = transportConstants::D(tm);
         d.scheme.transport_kappa = transportConstants::kappa(tm);
-        if (!linear)
-            unsupported("#Reconstruction star");
+        // #Reconstruction (Schemi.cpp:152-171): linear or star
+        d.scheme.reconstruction =
+            linear ? SCHEMI_RECONSTRUCTION_LINEAR : SCHEMI_RECONSTRUCTION_STAR;
         switch (msolver.solverType) {
         case matrixSolver::explicitSolver:
             d.scheme.diffusion_solver = SCHEMI_SLE_EXPLICIT;
@@ -300,6 +389,18 @@ void gpuStepSolverRK::calculateStep()
             cons[(ns + 1 + a) * n + i] = m[a];
     }
     std::memcpy(cons + (ns + 4) * n, &gasPhase.totalEnergy.cval()[0], n * sizeof(double));
+    // turbulent conserved fields after E (bunchOfFields.hpp:40-43)
+    const abstractTurbulenceModel &tm = *gasPhase.turbulence;
+    if (tm.turbulence()) {
+        std::memcpy(cons + (ns + 5) * n, &gasPhase.rhokTurb.cval()[0], n * sizeof(double));
+        std::memcpy(cons + (ns + 6) * n, &gasPhase.rhoepsTurb.cval()[0], n * sizeof(double));
+        if (tm.aField())
+            for (std::size_t i = 0; i < n; ++i)
+                for (std::size_t a = 0; a < 3; ++a)
+                    cons[(ns + 7 + a) * n + i] = gasPhase.rhoaTurb.cval()[i]()[a];
+        if (tm.bField())
+            std::memcpy(cons + (ns + 10) * n, &gasPhase.rhobTurb.cval()[0], n * sizeof(double));
+    }
     try {
         core->upload();
         core->step(gasPhase.pressure.meshRef().timestep());
@@ -326,6 +427,43 @@ void gpuStepSolverRK::calculateStep()
     std::memcpy(&gasPhase.pressure.val()[0], W + (ns + 4) * n, n * sizeof(double));
     std::memcpy(&gasPhase.internalEnergy.val()[0], W + (ns + 5) * n, n * sizeof(double));
     std::memcpy(&gasPhase.temperature.val()[0], W + (ns + 6) * n, n * sizeof(double));
+    // the turbulent fields and their densities (Advection2dOrder.cpp:610-627,
+    // Diffusion.cpp:999-1020): k = rho k / rho and so on
+    if (tm.turbulence()) {
+        const double *rho = U + ns * n;
+        auto scatter = [&](std::size_t row, auto &rf, auto &f) {
+            std::memcpy(&rf.val()[0], U + row * n, n * sizeof(double));
+            for (std::size_t i = 0; i < n; ++i)
+                f.val()[i] = U[row * n + i] / rho[i];
+        };
+        scatter(ns + 5, gasPhase.rhokTurb, gasPhase.kTurb);
+        scatter(ns + 6, gasPhase.rhoepsTurb, gasPhase.epsTurb);
+        if (tm.aField())
+            for (std::size_t i = 0; i < n; ++i)
+                for (std::size_t a = 0; a < 3; ++a) {
+                    gasPhase.rhoaTurb.val()[i].wr()[a] = U[(ns + 7 + a) * n + i];
+                    gasPhase.aTurb.val()[i].wr()[a] = U[(ns + 7 + a) * n + i] / rho[i];
+                }
+        if (tm.bField())
+            scatter(ns + 10, gasPhase.rhobTurb, gasPhase.bTurb);
+    }
+    // the diffusion stage's turbulent transport coefficients, formed from the
+    // k and epsilon it started from (Diffusion.cpp:129-135), and the kEpsA
+    // model's algebraic b of that stage (Diffusion.cpp:525-527, 1006)
+    if (tm.turbulence() && diffusionFlag) {
+        std::vector<double> st;
+        try {
+            st = core->turbulenceStage();
+        } catch (const schemi_gpu_host::gpu_error &e) {
+            throw exception(e.what(), static_cast<errors>(e.code()));
+        }
+        auto kStage = gasPhase.kTurb, epsStage = gasPhase.epsTurb;
+        std::memcpy(&kStage.val()[0], st.data(), n * sizeof(double));
+        std::memcpy(&epsStage.val()[0], st.data() + n, n * sizeof(double));
+        gasPhase.calculateCoefficients(kStage, epsStage, tm);
+        if (tm.aField() && !tm.bField())
+            std::memcpy(&gasPhase.bTurb.val()[0], st.data() + 2 * n, n * sizeof(double));
+    }
     gasPhase.HelmholtzEnergy.val() =
         gasPhase.phaseThermodynamics->Fv(gasPhase.concentration.p, gasPhase.temperature.cval());
     gasPhase.entropy.val() =

@@ -294,3 +294,38 @@ def test_zero_model_conserves_turbulent_energy(hip_device):
     Ns = case.Ns
     for q in (Ns + 5, Ns + 6):
         assert abs(U[q].sum() - U0[q].sum()) <= 1e-13 * np.abs(U0[q]).sum(), q
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["seeded_2d_kepsa_hllc", "seeded_3d_bhr_vanalbada",
+                                  "seeded_1d_turb_shear_vanleer"])
+def test_turbulence_stage_fields(name, hip_device):
+    """schemi_gpu_get_turbulence_stage: an error before the first step; after
+    a step, the k and epsilon the diffusion stage started from are those of
+    the same step with the diffusion stage off (rho k / rho, rho eps / rho),
+    bit for bit; kEpsA adds its algebraic b (non-negative; held to the
+    reference's output by the drop-in test)."""
+    g = load(name)
+    case = g["case"]
+    U0 = oracle_at(case, g["restart"], g["restart_turb"], g.get("restart_a"),
+                   g.get("restart_b")).conserved()
+    dt = float(g["dts"][0])
+    s = GpuSolver(case)
+    s.set_conserved(U0)
+    with pytest.raises(SchemiGpuError):
+        s.turbulence_stage()
+    s.step(dt)
+    st = s.turbulence_stage()
+    adv = GpuSolver(dataclasses.replace(case, diffusion=False))
+    adv.set_conserved(U0)
+    adv.step(dt)
+    U, Ns = adv.conserved(), case.Ns
+    assert np.array_equal(st["k"], U[Ns + 5] / U[Ns])
+    assert np.array_equal(st["eps"], U[Ns + 6] / U[Ns])
+    assert ("b" in st) == (case.turbulence.a_field and not case.turbulence.b_field)
+    if "b" in st:
+        assert np.all(st["b"] >= 0) and np.any(st["b"] > 0)
+    # a new state clears it
+    s.set_conserved(U0)
+    with pytest.raises(SchemiGpuError):
+        s.turbulence_stage()

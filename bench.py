@@ -22,7 +22,10 @@ timed on a bounded sample of the same workload.
 With --diffusion the same workload runs with the explicit molecular diffusion
 stage after every step (constant viscosity, diffusion and heat conduction
 coefficients 1e-4), and the line reports the whole step and the two
-diffusion passes, the update pass against the HBM roofline.
+diffusion passes, the update pass against the HBM roofline.  With
+--turbulent it runs with the reference's turbulent deck settings (BHR model,
+#Reconstruction star, the same diffusion stage) and reports the turbulent
+stage kernel against the roofline.
 """
 from __future__ import annotations
 
@@ -64,6 +67,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 1 GiB streaming copy (calibrates rocprofv3 byte counters)")
+    ap.add_argument("--turbulent", action="store_true",
+                    help="the workload with the BHR model, star reconstruction and the "
+                         "diffusion stage")
     ap.add_argument("--diffusion", action="store_true",
                     help="the workload with the molecular diffusion stage after each step")
     ap.add_argument("--diffusion-solver", default="explicit", choices=("explicit", "Jacobi"),
@@ -78,6 +84,11 @@ def parse():
 DIFF_TRANSPORT = (1e-4, 1e-4, 1e-4)
 DIFF_BYTES_PRIMS = 8 * ((NS + 5) + (2 * NS + 5))  # 128
 DIFF_BYTES_UPDATE = 8 * ((2 * NS + 5) + 2 * (NS + 5))  # 184
+# turbulent stage kernel with the BHR model (Ns + 11 conserved fields): stage
+# 1 reads U^n and writes U^1, stage 2 reads U^1 and U^n, writes U^n+1 and the
+# second stage's b; average over the two launches
+TURB_NF = NS + 11
+TURB_BYTES_PER_CELL_LAUNCH = 8 * (2 * TURB_NF + 3 * TURB_NF + 1) / 2  # 264
 
 
 def timed_run(solver, U0, steps, warmup):
@@ -367,6 +378,52 @@ def main_diffusion(a):
     print(json.dumps(line))
 
 
+def main_turbulent(a):
+    """The headline mesh with the reference's turbulent deck settings (BHR
+    model, #Reconstruction star, explicit diffusion stage): whole-step rate of
+    both precision paths, the turbulent stage kernel against the HBM roofline
+    and the two diffusion passes (HIP events)."""
+    import torch
+
+    from schemi_amd import cases
+    from schemi_amd.gpu import PREC_FP64, PREC_MIXED, GpuSolver
+
+    case, U0 = cases.richtmyer_meshkov_bhr_3d(nx=a.nx, ny=a.ny, nz=a.nz, seed=2024)
+    n = case.ncells
+    res = {}
+    for prec, name in ((PREC_FP64, "fp64"), (PREC_MIXED, "mixed")):
+        s = GpuSolver(case, prec)
+        wall, avg_ms, launches = timed_run(s, U0, a.steps, a.warmup)
+        prims_ms, update_ms, nd = s.diffusion_timing()
+        s.close()
+        res[name] = {"ms_per_step": 1e3 * wall / a.steps, "value": n * a.steps / wall / 1e6,
+                     "stage_kernel_ms": avg_ms, "diffusion_prims_ms": prims_ms,
+                     "diffusion_update_ms": update_ms, "diffusion_stages": nd}
+        print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    mx = res["mixed"]
+    achieved = TURB_BYTES_PER_CELL_LAUNCH * n / (mx["stage_kernel_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": "Mcell-updates/s, 3D RM 4.2M cells with BHR turbulence, star reconstruction "
+                  "and the diffusion stage",
+        "value": mx["value"], "unit": "Mcell-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": mx["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 flux + f64 state, f64 diffusion",
+        "data": "synthetic (seeded 3-D Richtmyer-Meshkov initial state, Gaussian k band)",
+        "config": {"workload": f"3-D Richtmyer-Meshkov {a.nx}x{a.ny}x{a.nz} ({n} cells), "
+                               "2 species, HLLC + van Albada, free x, periodic y/z, SSP-RK2, "
+                               "BHR k-epsilon-a-b model, #Reconstruction star, explicit "
+                               "diffusion stage (mu, D, kappa = 1e-4), Courant step on device",
+                   "cells": n, "precision": "mixed"},
+        "fp64": res["fp64"], "mixed": mx,
+        "roofline": {"kernel": "turb_stage_kernel", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -382,6 +439,9 @@ def main():
         stream_copy(1)
     if a.diffusion:
         main_diffusion(a)
+        return
+    if a.turbulent:
+        main_turbulent(a)
         return
     from schemi_amd import cases
     from schemi_amd.gpu import PREC_FP64, PREC_MIXED, GpuSolver

@@ -127,6 +127,41 @@ def richtmyer_meshkov_3d(nx: int = 256, ny: int = 128, nz: int = 128, seed: int 
     return case, conserved_from_primitive(case, c, v, p)
 
 
+def richtmyer_meshkov_bhr_3d(nx: int = 256, ny: int = 128, nz: int = 128, seed: int = 2024):
+    """richtmyer_meshkov_3d with the reference's turbulent deck settings: the
+    diffusion stage (constant mu, D, kappa of 1e-4; explicit solver) with the
+    BHR model (its default constants, Goncharov initialisation off) and
+    #Reconstruction star.  Initial turbulence: k = 1e-4 in a Gaussian of width
+    0.02 around the unperturbed interface over a 1e-8 background,
+    eps = Cmu^(3/4) k^(3/2) / 0.01, a = 0, b = 4e-3 Y (1 - Y) + 1e-15 with Y
+    the heavy mass fraction; free x, periodic y/z for every field."""
+    from .case import Turbulence
+
+    base, U = richtmyer_meshkov_3d(nx, ny, nz, seed)
+    case = base
+    case.diffusion, case.transport = True, (1e-4, 1e-4, 1e-4)
+    case.reconstruction = "star"
+    case.turbulence = Turbulence.BHR()
+    t = case.turbulence
+    for pt in (2, 3, 4, 5):
+        t.bc_k[pt] = BC(BCT.periodic)
+        t.bc_eps[pt] = BC(BCT.periodic)
+        t.bc_a[pt] = BC(BCT.periodic)
+        t.bc_b[pt] = BC(BCT.periodic)
+    x, _, _ = _cell_centres(case)
+    Ns, n = case.Ns, case.ncells
+    rho = U[Ns]
+    yh = U[1] / rho
+    kt = 1e-4 * np.exp(-((x - 0.6) / 0.02) ** 2) + 1e-8
+    et = 0.09 ** 0.75 * kt ** 1.5 / 0.01
+    bt = 4e-3 * yh * (1.0 - yh) + 1e-15
+    extra = np.stack([kt * rho, et * rho, np.zeros(n), np.zeros(n), np.zeros(n), bt * rho])
+    # E gains rho k (the turbulent energy is part of the total energy)
+    U = U.copy()
+    U[Ns + 4] = U[Ns + 4] + extra[0]
+    return case, np.concatenate([U, extra])
+
+
 def rayleigh_taylor_2d(nx: int = 256, ny: int = 1024, seed: int = 7, a0: float = 0.01,
                        modes=None, periodic_x: bool = True):
     """2-D Rayleigh-Taylor: heavy (rho 3, M 87, gamma 1.4) over light (rho 1,

@@ -56,9 +56,24 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
         j = (j % Na + Na) % Na;
         return id + (long long)(j - idx[a]) * P.stride[a];
     };
+    // the second stage averages with the first stage's values: the cell's
+    // high faces (every cell owns its high face along each axis) are loaded
+    // before any work so that their latency overlaps the closure and the
+    // Riemann solves
+    T pre[DIM][NV];
+    if (avg) {
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) {
+            const long long f = star_face(P, a, idx[a] + 1, idx);
+#pragma unroll
+            for (int q = 0; q < NV; ++q)
+                pre[a][q] = S[q * nft + f];
+        }
+    }
     // star state of the face with owner-side values o and neighbour-side
-    // values b, written at star index f
-    auto emit = [&](int a, T sg, const T *o, const T *b, long long f) {
+    // values b, written at star index f (old: the face's first-stage values
+    // when already loaded)
+    auto emit = [&](int a, T sg, const T *o, const T *b, long long f, const T *old) {
         TSide<T, NS, NA> O, B;
         close_side_turb<T, NS, GAS, NA>(P, o, o + NS, o[NS + 3], o[NS + 4], o[NS + 5], o + NS + 6,
                                         O);
@@ -69,7 +84,7 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             double *s = S + q * nft + f;
-            *s = avg ? (*s + st[q]) / 2 : st[q];
+            *s = avg ? ((old ? old[q] : *s) + st[q]) / 2 : st[q];
         }
     };
 
@@ -140,15 +155,15 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
         if (!per && i == 0) {
             side(0, false, so);
             ghost_turb<T, NS, NA>(P, TP, 0, 0, so, sb);
-            emit(0, T(-1), so, sb, star_face(P, 0, 0, idx));
+            emit(0, T(-1), so, sb, star_face(P, 0, 0, idx), nullptr);
         }
         side(0, true, so);
         if (!per && i == Nx - 1) {
             ghost_turb<T, NS, NA>(P, TP, 0, 1, so, sb);
-            emit(0, P.hi_flip[0] ? T(-1) : T(1), so, sb, star_face(P, 0, i + 1, idx));
+            emit(0, P.hi_flip[0] ? T(-1) : T(1), so, sb, star_face(P, 0, i + 1, idx), pre[0]);
         } else {
             side(1, false, sb);
-            emit(0, T(1), so, sb, star_face(P, 0, i + 1, idx));
+            emit(0, T(1), so, sb, star_face(P, 0, i + 1, idx), pre[0]);
         }
     } else {
         // face values are the cell values
@@ -159,14 +174,14 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
             T wb[NQ];
             if (!per && idx[a] == 0) {
                 ghost_turb<T, NS, NA>(P, TP, a, 0, w0, wb);
-                emit(a, T(-1), w0, wb, star_face(P, a, 0, idx));
+                emit(a, T(-1), w0, wb, star_face(P, a, 0, idx), nullptr);
             }
             if (!per && idx[a] == Na - 1) {
                 ghost_turb<T, NS, NA>(P, TP, a, 1, w0, wb);
-                emit(a, P.hi_flip[a] ? T(-1) : T(1), w0, wb, star_face(P, a, Na, idx));
+                emit(a, P.hi_flip[a] ? T(-1) : T(1), w0, wb, star_face(P, a, Na, idx), pre[a]);
             } else {
                 prims(nbr(a, 1), wb);
-                emit(a, T(1), w0, wb, star_face(P, a, idx[a] + 1, idx));
+                emit(a, T(1), w0, wb, star_face(P, a, idx[a] + 1, idx), pre[a]);
             }
         }
     }

@@ -7,10 +7,12 @@
 //
 // Star buffer: value q (0 rho, 1..3 v, 4 p, 5..7 a) of face f at
 // S[q * P.star_nft + f].  Faces normal to axis a are numbered
-// P.star_off[a] + i + (N_a + 1) (j + N_b1 k) with i = 0..N_a the position
-// along a and (j, k) the cell indices along the other two axes in increasing
-// order; the wrapped face of a periodic axis (owner N_a - 1, neighbour 0) is
-// i = N_a, and i = 0 is unused there.
+// P.star_off[a] + f_x + F_x (f_y + F_y f_z) on the grid of their positions:
+// f_a = 0..N_a the position along a (F_a = N_a + 1), the cell indices along
+// the other two axes (F = N), so that x is the fastest index for every axis
+// and the cells of a wavefront (consecutive x) write consecutive faces; the
+// wrapped face of a periodic axis (owner N_a - 1, neighbour 0) is f_a = N_a,
+// and f_a = 0 is unused there.
 #pragma once
 
 #include "fv_turb.hpp"
@@ -21,11 +23,11 @@ namespace sg {
 // (f = idx[a] for the low face, idx[a] + 1 for the high face).
 __device__ __forceinline__ long long star_face(const DevParams &P, int a, int f, const int *idx)
 {
-    const int b1 = a == 0 ? 1 : 0, b2 = a == 2 ? 1 : 2;
     if (f == 0 && P.periodic[a] != 0)
         f = P.N[a];
-    const long long r = idx[b1] + (long long)P.N[b1] * idx[b2];
-    return P.star_off[a] + f + (long long)(P.N[a] + 1) * r;
+    const long long c0 = a == 0 ? f : idx[0], c1 = a == 1 ? f : idx[1], c2 = a == 2 ? f : idx[2];
+    const long long F0 = P.N[0] + (a == 0), F1 = P.N[1] + (a == 1);
+    return P.star_off[a] + c0 + F0 * (c1 + F1 * c2);
 }
 
 // pressureStarClass::pressureStar (pressureStarClass.cpp:12-37): pFromUv of

@@ -329,3 +329,24 @@ def test_turbulence_stage_fields(name, hip_device):
     s.set_conserved(U0)
     with pytest.raises(SchemiGpuError):
         s.turbulence_stage()
+
+
+@pytest.mark.gpu
+def test_rm_bhr_star_workload_small(hip_device):
+    """The turbulent Richtmyer-Meshkov workload of the benchmark (BHR model,
+    #Reconstruction star, explicit diffusion stage, free x, periodic y/z) at
+    32 x 16 x 16: three steps of the device's own Courant sequence, bit for
+    bit with the oracle's conserved-state steps."""
+    from schemi_amd import cases
+
+    case, U0 = cases.richtmyer_meshkov_bhr_3d(32, 16, 16)
+    o = oracle_conserved(case, U0)
+    g = GpuSolver(case)
+    g.set_conserved(U0)
+    dt = 1e-4
+    for s in range(1, 4):
+        assert o.step(dt, 1) == 0
+        nxt = g.step(dt)
+        ref, got = o.conserved(), g.conserved()
+        assert np.array_equal(got, ref), (s, np.nonzero((got != ref).any(axis=1))[0])
+        dt = nxt

@@ -17,11 +17,11 @@
 
 namespace sg {
 
-template <int NS, int DIM, bool SLOPES, int GAS, int NT, int NA>
+template <int NS, int DIM, bool SLOPES, int GAS, int NT, int NA, bool AVG>
 __global__ void __launch_bounds__(256)
 star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
             const double *__restrict__ geo, const double *__restrict__ Uin, double *S,
-            const double *__restrict__ sc, int avg)
+            const double *__restrict__ sc)
 {
     using T = double;
     constexpr int NFS = NS + 5 + NT + NA;  // rows of the state
@@ -61,7 +61,7 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
     // before any work so that their latency overlaps the closure and the
     // Riemann solves
     T pre[DIM][NV];
-    if (avg) {
+    if constexpr (AVG) {
 #pragma unroll
         for (int a = 0; a < DIM; ++a) {
             const long long f = star_face(P, a, idx[a] + 1, idx);
@@ -84,7 +84,10 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
 #pragma unroll
         for (int q = 0; q < NV; ++q) {
             double *s = S + q * nft + f;
-            *s = avg ? ((old ? old[q] : *s) + st[q]) / 2 : st[q];
+            if constexpr (AVG)
+                *s = ((old ? old[q] : *s) + st[q]) / 2;
+            else
+                *s = st[q];
         }
     };
 
@@ -160,10 +163,11 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
         side(0, true, so);
         if (!per && i == Nx - 1) {
             ghost_turb<T, NS, NA>(P, TP, 0, 1, so, sb);
-            emit(0, P.hi_flip[0] ? T(-1) : T(1), so, sb, star_face(P, 0, i + 1, idx), pre[0]);
+            emit(0, P.hi_flip[0] ? T(-1) : T(1), so, sb, star_face(P, 0, i + 1, idx),
+                 AVG ? pre[0] : nullptr);
         } else {
             side(1, false, sb);
-            emit(0, T(1), so, sb, star_face(P, 0, i + 1, idx), pre[0]);
+            emit(0, T(1), so, sb, star_face(P, 0, i + 1, idx), AVG ? pre[0] : nullptr);
         }
     } else {
         // face values are the cell values
@@ -178,10 +182,12 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
             }
             if (!per && idx[a] == Na - 1) {
                 ghost_turb<T, NS, NA>(P, TP, a, 1, w0, wb);
-                emit(a, P.hi_flip[a] ? T(-1) : T(1), w0, wb, star_face(P, a, Na, idx), pre[a]);
+                emit(a, P.hi_flip[a] ? T(-1) : T(1), w0, wb, star_face(P, a, Na, idx),
+                     AVG ? pre[a] : nullptr);
             } else {
                 prims(nbr(a, 1), wb);
-                emit(a, T(1), w0, wb, star_face(P, a, idx[a] + 1, idx), pre[a]);
+                emit(a, T(1), w0, wb, star_face(P, a, idx[a] + 1, idx),
+                     AVG ? pre[a] : nullptr);
             }
         }
     }
@@ -193,14 +199,20 @@ template <int NS, int DIM, bool SLOPES, int NT, int NA>
 int launch_star_gas(const StarLaunch &L)
 {
     const unsigned blocks = (unsigned)((L.n + 255) / 256);
-    if (L.gas_model == GAS_VDW)
-        hipLaunchKernelGGL((star_kernel<NS, DIM, SLOPES, GAS_VDW, NT, NA>), dim3(blocks),
-                           dim3(256), 0, L.stream, L.P, L.TP, L.geo, L.U, L.S, L.sc, L.avg);
+#define SG_STAR(GASV, AV)                                                                  \
+    hipLaunchKernelGGL((star_kernel<NS, DIM, SLOPES, GASV, NT, NA, AV>), dim3(blocks), dim3(256), \
+                       0, L.stream, L.P, L.TP, L.geo, L.U, L.S, L.sc)
+    if (L.gas_model == GAS_VDW && L.avg)
+        SG_STAR(GAS_VDW, true);
+    else if (L.gas_model == GAS_VDW)
+        SG_STAR(GAS_VDW, false);
+    else if (L.gas_model == GAS_IDEAL && L.avg)
+        SG_STAR(GAS_IDEAL, true);
     else if (L.gas_model == GAS_IDEAL)
-        hipLaunchKernelGGL((star_kernel<NS, DIM, SLOPES, GAS_IDEAL, NT, NA>), dim3(blocks),
-                           dim3(256), 0, L.stream, L.P, L.TP, L.geo, L.U, L.S, L.sc, L.avg);
+        SG_STAR(GAS_IDEAL, false);
     else
         return -1;
+#undef SG_STAR
     return static_cast<int>(hipGetLastError());
 }
 

@@ -418,7 +418,9 @@ def main_turbulent(a):
         "fp64": res["fp64"], "mixed": mx,
         "roofline": {"kernel": "turb_stage_kernel", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None},
+                     "traffic": traffic_from_profiles(
+                         "turb_star_diffusion_rm3d_bhr_256x128x128_trace.md",
+                         "traffic_bytes_per_launch_turb")},
         "cpu_baseline": None,
     }
     print(json.dumps(line))

@@ -32,7 +32,7 @@ star_kernel(const DevParams *__restrict__ Pp, const TurbParams *__restrict__ Tp,
     if (sc[SC_DONE] != 0.0)
         return;
     const long long n = P.n;
-    const long long id = blockIdx.x * 256LL + threadIdx.x;
+    const long long id = turb_block(blockIdx.x, gridDim.x) * 256LL + threadIdx.x;
     if (id >= n)
         return;
     const int idx[3] = {int(id % P.N[0]), int((id / P.N[0]) % P.N[1]), int(id / P.stride[2])};
